@@ -1,0 +1,148 @@
+/*
+ * vfvm_oracle.h -- CPU Float64 restatement of VoronoiFVM.jl v3.5.2's
+ * assemble-and-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
+ * never by the product library (voronoifvm.jl_amd/csrc) and never as a
+ * fallback for it.
+ *
+ * Every function restates one piece of the reference; the file:line it
+ * follows is cited in vfvm_oracle.c.  Indices in this API are 0-based, node
+ * regions and boundary regions keep the reference's 1-based numbering.
+ *
+ * Pinning (what checks each part):
+ *  - orc_bernoulli / orc_bernoulli_pm: mpmath goldens (tests/golden), the
+ *    reference's own tolerance of test/test010_bernoulli.jl.
+ *  - form factors: closed forms (right-triangle / Kuhn-cube factors),
+ *    sum of node volumes = domain measure, 2D formula vs embedded 3D boundary
+ *    formula (test/test020_formfactors.jl), Example301 known answer.
+ *  - assembly: finite-difference Jacobian on small grids, edge conservation,
+ *    cellwise == edgewise values, test030 exactness, Example105/106/207/210/
+ *    301/440 known answers.
+ *  - Newton / transient drivers: the same example known answers.
+ *  - linear solvers: scipy spsolve on the same CSC matrix (tests only) and
+ *    Krylov vs direct agreement (examples/DevEx003_Solvers.jl rule).
+ */
+#ifndef VFVM_ORACLE_H
+#define VFVM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* physics registry (shared numbering with include/vfvm_hip.h) */
+enum { ORC_FLUX_NONE = 0, ORC_FLUX_LINEAR = 1, ORC_FLUX_POWER = 2, ORC_FLUX_NLDIFF = 3, ORC_FLUX_DD = 4 };
+enum { ORC_REACT_NONE = 0, ORC_REACT_SINH = 1, ORC_REACT_POW = 2, ORC_REACT_EXCHANGE = 3, ORC_REACT_DD = 4 };
+enum { ORC_STOR_NONE = 0, ORC_STOR_IDENTITY = 1 };
+enum { ORC_BC_NONE = 0, ORC_BC_DIRICHLET = 1, ORC_BC_DIRICHLET_LEGACY = 2, ORC_BC_ROBIN = 3 };
+enum { ORC_CELLWISE = 0, ORC_EDGEWISE = 1 };
+enum { ORC_LIN_DIRECT = 0, ORC_LIN_BICGSTAB_JACOBI = 1, ORC_LIN_BICGSTAB_ILU0 = 2,
+       ORC_LIN_GMRES_BLOCKJACOBI = 3 };
+enum { ORC_OK = 0, ORC_ERR_CONVERGENCE = 1, ORC_ERR_ASSEMBLY = 2, ORC_ERR_LINEAR = 3, ORC_ERR_ARG = 4 };
+
+#define ORC_MAXSPEC 3
+#define ORC_NPARAM 16
+
+/* parameter slots of orc_physics.p (same meaning as vfvm_physics_t.p) */
+enum {
+  ORC_P_D0 = 0,      /* LINEAR: D_i = p[D0+i];  POWER: D;  NLDIFF: d0 */
+  ORC_P_D2 = 3,      /* NLDIFF: d2 (D(u) = d0 + d2 u^2) */
+  ORC_P_M = 4,       /* POWER flux exponent (integer valued) */
+  ORC_P_RK = 5,      /* SINH / POW / EXCHANGE reaction factor k */
+  ORC_P_RM = 6,      /* POW reaction exponent (integer valued) */
+  ORC_P_LAMBDA2 = 7, /* DD: lambda^2 (Poisson flux factor) */
+  ORC_P_ZN = 8,      /* DD: electron charge number (-1) */
+  ORC_P_ZP = 9,      /* DD: hole charge number (+1) */
+  ORC_P_NI2 = 10,    /* DD: n_i^2 in (np - ni^2)/(n + p + 2 n_i) */
+  ORC_P_NI = 11
+};
+
+typedef struct {
+  int32_t flux, reaction, storage, pad;
+  double p[ORC_NPARAM];
+} orc_physics;
+
+/* boundary condition for (bregion, species); value(lambda) = value + value_lambda*lambda */
+typedef struct {
+  int32_t kind, pad;
+  double value, value_lambda, factor;
+} orc_bc;
+
+typedef struct {
+  double abstol, reltol, tol_round, tol_mono, damp_initial, damp_growth;
+  int32_t maxiters, max_round, updatecontrol, method_linear;
+  double reltol_linear, abstol_linear;
+  int32_t maxiters_linear, gmres_restart;
+  int32_t is_linear, factorize_every_newtonstep, factorize_every_timestep, pad;
+  /* transient / embedding control (src/vfvm_solvercontrol.jl:138-236) */
+  double dt, dt_min, dt_max, dt_grow, dt_decrease;
+  double dp, dp_min, dp_max, dp_grow, dp_decrease;
+  double du_opt, du_max_factor;
+  int32_t num_final_steps, force_first_step;
+} orc_control;
+
+typedef struct {
+  int32_t niter, nlin_total, nlu, converged;
+  double last_update_norm;
+  double updatenorm[128];
+} orc_history;
+
+typedef struct orc_sys orc_sys;
+
+void orc_default_control(orc_control *c);
+
+orc_sys *orc_create(int dim, int nnodes, const double *coord,
+                    int ncells, const int32_t *cellnodes, const int32_t *cellregions,
+                    int nbfaces, const int32_t *bfacenodes, const int32_t *bfaceregions,
+                    int nspec, const orc_physics *phys, int nbregions, const orc_bc *bc,
+                    int assembly);
+void orc_destroy(orc_sys *s);
+/* per-node source, nspec x nnodes (species fastest); NULL = none */
+void orc_set_source(orc_sys *s, const double *src);
+void orc_set_bc(orc_sys *s, const orc_bc *bc);
+const char *orc_last_error(orc_sys *s);
+
+int orc_num_edges(const orc_sys *s);
+/* edgewise data: edge endpoints (2 x E), aggregated edge factors (E), node volumes (n) */
+void orc_edges(const orc_sys *s, int32_t *edgenodes, double *edgefac);
+void orc_nodevolumes(const orc_sys *s, double *vol);
+
+/* a9: eval_and_assemble; matrix kept inside the system (pattern persists) */
+int orc_assemble(orc_sys *s, const double *U, const double *Uold, double time, double tstep,
+                 double lambda, double *F);
+int64_t orc_matrix_nnz(const orc_sys *s);
+void orc_matrix_csc(const orc_sys *s, int64_t *colptr, int32_t *rowval, double *nzval);
+void orc_matrix_reset(orc_sys *s); /* forget the accumulated pattern */
+
+int orc_initialize_dirichlet(orc_sys *s, double *U, double time, double lambda);
+
+/* a3: solve_step!  U <- Newton solution started from Uold */
+int orc_solve_step(orc_sys *s, double *U, const double *Uold, const orc_control *c,
+                   double time, double tstep, double lambda, orc_history *h);
+
+/* a4: solve_transient!  transient=1: lambdas are times; 0: embedding.
+   Final solution in U_final; number of accepted steps in *nsteps. */
+int orc_solve_transient(orc_sys *s, const double *inival, int nlambdas, const double *lambdas,
+                        int transient, const orc_control *c, double time, double *U_final,
+                        int *nsteps, int *newton_total);
+
+/* linear solve on the currently assembled matrix: x = A^{-1} b */
+int orc_linsolve(orc_sys *s, const orc_control *c, const double *b, double *x, int *iters);
+
+double orc_bernoulli(double x);
+void orc_bernoulli_pm(double x, double *bp, double *bm);
+/* derivative of B via the same dual arithmetic the assembly uses */
+void orc_bernoulli_dual(double x, double *b, double *db);
+
+/* form factors of one cell / boundary face (tests) */
+void orc_cellfactors(int dim, const double *coord, const int32_t *nodes, double *nodefac,
+                     double *edgefac);
+void orc_bfacefactors(int dim, const double *coord, const int32_t *nodes, double *nodefac);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,171 @@
+"""Pins the oracle's factors and assembly by mathematics: closed forms,
+invariants, finite-difference Jacobians and cellwise/edgewise agreement.
+Each check is chosen so a dropped term, wrong sign or transposed index fails."""
+import numpy as np
+import pytest
+
+import oracle as O
+from vfvm.grids import linspace, simplexgrid
+
+
+def test_node_volumes_sum_to_domain_measure():
+    # test/test120_norms.jl:103-108,129
+    for axes in ([linspace(0, 1, 7)], [linspace(0, 1, 5), linspace(0, 2, 4)],
+                 [linspace(0, 1, 4), linspace(0, 1, 3), linspace(0, 3, 5)]):
+        g = simplexgrid(*axes)
+        meas = np.prod([a[-1] - a[0] for a in axes])
+        for asm in (O.CELLWISE, O.EDGEWISE):
+            s = O.OracleSystem(g, 1, O.physics("linear", D0=1.0), {}, asm)
+            assert s.nodevolumes().sum() == pytest.approx(meas, rel=1e-14)
+
+
+def test_triangle_factors_closed_form_and_embedded():
+    # 2D factor (b^2+c^2-a^2)/(8A) vs the 3D boundary-triangle formula (test/test020_formfactors.jl:8-27)
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        P = rng.random((3, 2))
+        coord = P.ravel().copy()
+        npar, epar = np.zeros(3), np.zeros(3)
+        O.lib().orc_cellfactors(2, coord, np.array([0, 1, 2], np.int32), npar, epar)
+        a2 = [np.sum((P[(i + 1) % 3] - P[(i + 2) % 3]) ** 2) for i in range(3)]
+        u, v = P[1] - P[0], P[2] - P[0]
+        area = 0.5 * abs(u[0] * v[1] - u[1] * v[0])
+        for i in range(3):
+            j, k = (i + 1) % 3, (i + 2) % 3
+            assert epar[i] == pytest.approx((a2[j] + a2[k] - a2[i]) / (8 * area), rel=1e-12, abs=1e-12)
+        assert npar.sum() == pytest.approx(area, rel=1e-12)
+        P3 = np.hstack([P, np.zeros((3, 1))])
+        bn = np.zeros(3)
+        O.lib().orc_bfacefactors(3, P3.ravel().copy(), np.array([0, 1, 2], np.int32), bn)
+        assert np.allclose(bn, npar, rtol=1e-7, atol=1e-12)
+
+
+def test_kuhn_cube_factors():
+    # uniform cube grid: interior axis edges sigma/h = h, diagonals ~ 0, interior node volume h^3
+    n, h = 5, 0.25
+    X = linspace(0, 1, n)
+    g = simplexgrid(X, X, X)
+    s = O.OracleSystem(g, 1, O.physics("linear", D0=1.0), {}, O.EDGEWISE)
+    en, ef = s.edges()
+    c = g.coord
+    d = np.abs(c[:, en[:, 0]] - c[:, en[:, 1]])
+    naxis = (d > 1e-12).sum(axis=0)
+    interior = np.all((np.minimum(c[:, en[:, 0]], c[:, en[:, 1]]) > 1e-12) &
+                      (np.maximum(c[:, en[:, 0]], c[:, en[:, 1]]) < 1 - 1e-12), axis=0)
+    assert np.allclose(ef[(naxis == 1) & interior], h, rtol=1e-12)
+    assert np.all(np.abs(ef[naxis > 1]) < 1e-14)
+    vol = s.nodevolumes()
+    inner = np.all((c > 1e-12) & (c < 1 - 1e-12), axis=0)
+    assert np.allclose(vol[inner], h ** 3, rtol=1e-12)
+
+
+def _fd_jacobian(s, U, **kw):
+    n = U.size
+    J = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        hj = 1e-6 * max(1.0, abs(U[j]))
+        e[j] = hj
+        J[:, j] = (s.assemble(U + e, **kw) - s.assemble(U - e, **kw)) / (2 * hj)
+    return J
+
+
+CASES = [
+    ("linear", dict(flux="linear", D0=0.7), 1, {}),
+    ("power+pow", dict(flux="power", reaction="pow", D0=0.3, m=3, k=2.0, rm=2), 1, {}),
+    ("nldiff+storage", dict(flux="nldiff", storage="identity", D0=1.0, d2=1.0), 1, {}),
+    ("sinh", dict(flux="linear", reaction="sinh", D0=1e-2, k=1.0), 1, {}),
+    ("exchange", dict(flux="linear", reaction="exchange", storage="identity", D0=0.5, D1=0.2, k=1.5), 2, {}),
+    ("dd", dict(flux="dd", reaction="dd", lambda2=0.05, zn=-1.0, zp=1.0, ni2=1.0, ni=1.0), 3, {}),
+]
+
+
+@pytest.mark.parametrize("name,kw,ns,_", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("assembly", [O.CELLWISE, O.EDGEWISE])
+def test_jacobian_matches_finite_differences(name, kw, ns, _, assembly):
+    g = simplexgrid(linspace(0, 1, 4), linspace(0, 1, 3))
+    rng = np.random.default_rng(7)
+    s = O.OracleSystem(g, ns, O.physics(**kw), {(1, 0): ("dirichlet", 0.3), (2, 0): ("robin", 0.2, 1.5)},
+                       assembly, rng.random(ns * g.num_nodes))
+    U = 0.5 + rng.random(ns * g.num_nodes)
+    if name == "dd":
+        U.reshape(-1, 3)[:, 0] = rng.uniform(-3, 3, g.num_nodes)  # Bernoulli args across all branches
+    tstep = 0.3 if kw.get("storage") else np.inf
+    Uold = U * 0.9
+    s.assemble(U, Uold, tstep=tstep)
+    J = s.matrix_scipy().toarray()
+    # FD on the penalty-free part: rows with a Dirichlet penalty carry 1e30
+    pen = np.abs(np.diag(J)) > 1e20
+    Jfd = _fd_jacobian(s, U, Uold=Uold, tstep=tstep)
+    ok = ~pen
+    scale = np.abs(J[ok]).max()
+    assert np.abs(J[ok] - Jfd[ok]).max() <= 1e-6 * scale
+
+
+def test_edge_conservation_and_symmetry():
+    # each edge adds +g at K and -g at L: without sources, reactions or BCs sum(F) = 0
+    g = simplexgrid(linspace(0, 1, 6), linspace(0, 1, 5))
+    rng = np.random.default_rng(3)
+    s = O.OracleSystem(g, 1, O.physics("nldiff", D0=1.0, d2=1.0), {}, O.EDGEWISE)
+    F = s.assemble(rng.random(g.num_nodes))
+    assert abs(F.sum()) < 1e-13
+    # the linear diffusion Jacobian is symmetric with zero row sums
+    s2 = O.OracleSystem(g, 1, O.physics("linear", D0=2.0), {}, O.EDGEWISE)
+    s2.assemble(rng.random(g.num_nodes))
+    A = s2.matrix_scipy().toarray()
+    assert np.allclose(A, A.T, atol=1e-14)
+    assert np.abs(A.sum(axis=1)).max() < 1e-13
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_cellwise_equals_edgewise(dim):
+    # the reference's examples assert both assembly loops agree (e.g. Example105:96-101)
+    axes = [linspace(0, 1, 5)] * dim
+    g = simplexgrid(*axes)
+    rng = np.random.default_rng(dim)
+    kw = dict(flux="nldiff", reaction="sinh", storage="identity", D0=1.0, d2=0.5, k=1.0)
+    U = rng.random(g.num_nodes)
+    res = []
+    for asm in (O.CELLWISE, O.EDGEWISE):
+        s = O.OracleSystem(g, 1, O.physics(**kw), {(1, 0): ("dirichlet", 0.1)}, asm, rng.random(g.num_nodes))
+        F = s.assemble(U, U * 0.5, tstep=0.1)
+        res.append((F, s.matrix_scipy().toarray()))
+    (F1, A1), (F2, A2) = res
+    assert np.abs(F1 - F2).max() <= 1e-12 * np.abs(F1).max()
+    assert np.abs(A1 - A2).max() <= 1e-12 * np.abs(A1).max()
+
+
+def test_linear_problem_converges_in_one_step():
+    g = simplexgrid(linspace(0, 1, 9), linspace(0, 1, 9))
+    s = O.OracleSystem(g, 1, O.physics("linear", D0=1.0),
+                       {(4, 0): ("dirichlet", 0.0), (2, 0): ("dirichlet", 1.0)}, O.EDGEWISE)
+    U, h = s.solve_step(np.zeros(g.num_nodes), O.control(is_linear=1))
+    assert h.niter == 1
+    # u = x solves the Laplace problem with these BCs exactly (Neumann on y faces)
+    assert np.abs(U - g.coord[0]).max() < 1e-12
+
+
+def test_krylov_agrees_with_direct():
+    # examples/DevEx003_Solvers.jl:85-177: every solver agrees with the direct one to 1e-7
+    g = simplexgrid(linspace(0, 1, 21), linspace(0, 1, 21))
+    src = np.ones(g.num_nodes)
+    bcs = {(1, 0): ("dirichlet", 0.0), (4, 0): ("dirichlet", 0.0), (2, 0): ("dirichlet", 1.0), (3, 0): ("dirichlet", 1.0)}
+    s = O.OracleSystem(g, 1, O.physics("linear", "sinh", D0=1e-2, k=1.0), bcs, O.EDGEWISE, src)
+    ref, _ = s.solve_step(np.zeros(g.num_nodes))
+    for m in ("bicgstab_jacobi", "bicgstab_ilu0", "gmres_blockjacobi"):
+        U, _ = s.solve_step(np.zeros(g.num_nodes), O.control(method_linear=m, reltol_linear=1e-10,
+                                                              abstol_linear=1e-13, maxiters_linear=2000))
+        assert np.abs(U - ref).max() < 1e-7, m
+
+
+def test_direct_solver_matches_scipy():
+    import scipy.sparse.linalg as sla
+    g = simplexgrid(linspace(0, 1, 9), linspace(0, 1, 7))
+    rng = np.random.default_rng(5)
+    s = O.OracleSystem(g, 3, O.physics("dd", "dd", lambda2=0.1, zn=-1.0, zp=1.0, ni2=1.0, ni=1.0),
+                       {(4, 0): ("dirichlet", 0.2), (4, 1): ("dirichlet", 1.0), (4, 2): ("dirichlet", 1.0)})
+    U = np.abs(rng.random(3 * g.num_nodes)) + 0.5
+    F = s.assemble(U)
+    x, _ = s.linsolve(F)
+    y = sla.spsolve(s.matrix_scipy().tocsc(), F)
+    assert np.abs(x - y).max() <= 1e-10 * np.abs(y).max()
