@@ -1,0 +1,202 @@
+/*
+ * vfvm_hip.h -- C ABI of the MI355X (gfx950) assemble-and-solve library.
+ *
+ * The library replaces VoronoiFVM.jl's hot path: eval_and_assemble's node,
+ * edge and boundary-node loops, the damped Newton loop of solve_step! and the
+ * linear solve of _solve_linear!.  A Julia host keeps the VoronoiFVM.System /
+ * Physics / solve(sys; inival, control) surface and reaches these entry points
+ * with `ccall` (see the Julia shim in voronoifvm.jl_amd/julia).  Everything here
+ * is plain C: pointers, sizes and status codes; no torch types.
+ *
+ * Conventions
+ *  - Arrays coming from Julia keep Julia's layout: coordinates dim x nnodes
+ *    column-major, cellnodes (dim+1) x ncells and bfacenodes dim x nbfaces as
+ *    1-based int64 (grid[Coordinates], grid[CellNodes], grid[BFaceNodes]).
+ *  - Solution-like arrays (U, Uold, F, source) are nspec x nnodes column-major,
+ *    i.e. the reference's dofs(U) order: dof(K, i) = (K-1)*nspec + i
+ *    (src/vfvm_system.jl:808-811, src/vfvm_densesolution.jl:49,68).
+ *  - Matrices cross the boundary as CSC with 1-based int64 indices and rows
+ *    sorted within columns, exactly what SparseMatrixCSC(state.matrix) gives
+ *    (src/vfvm_linsolve.jl:3-4).  Inside, the device keeps CSR + a slot map.
+ *  - Host pointers are copied in and out; no pointer is retained after a call.
+ *    Device buffers belong to the handle and are freed by its destroy call.
+ *  - Every call returns a status: 0 OK, or one of the VFVM_ERR_* codes that
+ *    mirror the reference's exceptions (src/vfvm_logging_exceptions.jl:6-29).
+ *  - Handles are independent and each owns one HIP stream: concurrent calls on
+ *    different handles are allowed, calls on one handle must be serialised
+ *    (the reference serialises through gridaccesslock, src/vfvm_system.jl:118).
+ */
+#ifndef VFVM_HIP_H
+#define VFVM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (src/vfvm_logging_exceptions.jl:6-29) ---- */
+#define VFVM_OK 0
+#define VFVM_ERR_CONVERGENCE 1 /* ConvergenceError, src/vfvm_solver.jl:196-198 */
+#define VFVM_ERR_ASSEMBLY 2    /* AssemblyError / "trying to assemble NaN", src/vfvm_assembly.jl:10-12 */
+#define VFVM_ERR_LINEAR 3      /* LinearSolverError, src/vfvm_linsolve.jl:50-58 */
+#define VFVM_ERR_ARG 4         /* invalid argument (shape, unsupported physics) */
+#define VFVM_ERR_HIP 5         /* HIP runtime failure or no device */
+
+/* ---- physics registry: the fixed set of device callbacks ----
+ * Flux (src/vfvm_physics.jl:67-182 `flux`), reaction, storage.  Parameters in p[]:
+ *   p[0..2] D_i (LINEAR per species; POWER and NLDIFF use p[0])
+ *   p[3] d2 (NLDIFF: D(u) = p[0] + p[3]*u^2 at the edge mean), p[4] m (POWER exponent)
+ *   p[5] k (SINH/POW/EXCHANGE factor), p[6] POW exponent
+ *   p[7] lambda^2, p[8] z_n, p[9] z_p, p[10] n_i^2, p[11] n_i (DD)            */
+#define VFVM_FLUX_NONE 0
+#define VFVM_FLUX_LINEAR 1 /* D_i (u_k - u_l)          Example105_NonlinearPoisson1D.jl:41-44 */
+#define VFVM_FLUX_POWER 2  /* D (u_k^m - u_l^m)        Example106_NonlinearDiffusion1D.jl:57-60 */
+#define VFVM_FLUX_NLDIFF 3 /* (d0 + d2 ubar^2)(u_k-u_l) src/vfvm_physics.jl:478-480 */
+#define VFVM_FLUX_DD 4     /* psi: lambda^2 dpsi; n,p: Scharfetter-Gummel, Example160:43-50 */
+#define VFVM_REACT_NONE 0
+#define VFVM_REACT_SINH 1     /* k (e^u - e^-u)      Example105:56-59 */
+#define VFVM_REACT_POW 2      /* k u^m               Example207:34-36 */
+#define VFVM_REACT_EXCHANGE 3 /* k(u1-u2), k(u2-u1)  Example210:25-30 */
+#define VFVM_REACT_DD 4       /* n - p ; R = (np - ni^2)/(n + p + 2 ni) */
+#define VFVM_STOR_NONE 0
+#define VFVM_STOR_IDENTITY 1 /* storage u_i          Example106:63-66 */
+
+typedef struct {
+  int32_t flux, reaction, storage, pad;
+  double p[16];
+} vfvm_physics_t;
+
+/* ---- boundary conditions per (boundary region, species) ----
+ * DIRICHLET: breaction with boundary_dirichlet!, penalty 1e30 * fac (src/vfvm_physics.jl:487-494)
+ * DIRICHLET_LEGACY: boundary_dirichlet!(sys, ...), penalty 1e30 without fac (src/vfvm_assembly.jl:370-379)
+ * ROBIN: boundary_robin!/boundary_neumann! (factor*u - value), src/vfvm_physics.jl:533,552
+ * value(lambda) = value + value_lambda * lambda (embedding / bias continuation). */
+#define VFVM_BC_NONE 0
+#define VFVM_BC_DIRICHLET 1
+#define VFVM_BC_DIRICHLET_LEGACY 2
+#define VFVM_BC_ROBIN 3
+
+typedef struct {
+  int32_t kind, pad;
+  double value, value_lambda, factor;
+} vfvm_bc_t;
+
+/* ---- linear solver selection (SolverControl.method_linear, src/vfvm_solvercontrol.jl:106) ---- */
+#define VFVM_KRYLOV_BICGSTAB 0
+#define VFVM_KRYLOV_GMRES 1
+#define VFVM_PRECON_JACOBI 0       /* point Jacobi (row scaling) */
+#define VFVM_PRECON_BLOCKJACOBI 1  /* nspec x nspec node blocks */
+#define VFVM_PRECON_ILU0 2         /* ILU(0), multicolour level order */
+#define VFVM_PREC_FP64 0
+#define VFVM_PREC_FP32_REFINE 1    /* FP32 Krylov + preconditioner inside FP64 iterative refinement */
+
+/* SolverControl subset on the path (src/vfvm_solvercontrol.jl:30-131) plus the
+ * device solver choice.  vfvm_default_control() fills the reference defaults. */
+typedef struct {
+  double abstol, reltol, tol_round, tol_mono, damp_initial, damp_growth;
+  int32_t maxiters, max_round, updatecontrol, is_linear;
+  double reltol_linear, abstol_linear;
+  int32_t maxiters_linear, krylov, precon, precision;
+  int32_t gmres_restart, max_refinements, reuse_precon, check_every;
+  double inner_reltol; /* FP32 inner solve tolerance per refinement sweep */
+} vfvm_control_t;
+
+/* NewtonSolverHistory subset (src/vfvm_history.jl:10-37) */
+typedef struct {
+  int32_t niter, nlin, nlu, converged;
+  double time, tasm, tlinsolve;
+  double last_update_norm;
+  double residual_norm; /* ||F||_inf over non-penalty rows at the returned solution */
+  double updatenorm[64];
+} vfvm_history_t;
+
+typedef struct {
+  int32_t iters, refinements, converged, pad;
+  double resnorm0, resnorm;
+} vfvm_linstats_t;
+
+typedef struct vfvm_ctx vfvm_ctx;
+typedef struct vfvm_sys vfvm_sys;
+
+void vfvm_default_control(vfvm_control_t *c);
+
+/* Context: one device, one stream.  No counterpart in the reference (it has no
+ * device); it plays the role of the process-wide state SystemState relies on. */
+int vfvm_ctx_create(int device, vfvm_ctx **out);
+int vfvm_ctx_destroy(vfvm_ctx *ctx);
+/* last error message of this context (or of the last failed create when ctx is NULL) */
+const char *vfvm_last_error(vfvm_ctx *ctx);
+
+/* System creation: grid upload + discretisation setup (dof map, Voronoi
+ * factors, edgewise aggregation, boundary node factors, CSR pattern and slot
+ * map).  Replaces System(grid; physics...) + _complete!/update_grid_edgewise!
+ * (src/vfvm_system.jl:215-261, 530-596, 690-755; src/vfvm_state.jl:99-157).
+ * cellregions/bfaceregions are 1-based region numbers (int64). */
+int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coord, int64_t ncells,
+                       const int64_t *cellnodes, const int64_t *cellregions, int64_t nbfaces,
+                       const int64_t *bfacenodes, const int64_t *bfaceregions, int nspec,
+                       const vfvm_physics_t *physics, int nbregions, const vfvm_bc_t *bc,
+                       vfvm_sys **out);
+int vfvm_system_destroy(vfvm_sys *sys);
+/* per-node source term, nspec x nnodes (Physics(source = array), src/vfvm_system.jl:312-319); NULL clears */
+int vfvm_system_set_source(vfvm_sys *sys, const double *src);
+/* replace the boundary-condition table (nbregions x nspec entries, region-major) */
+int vfvm_system_set_bc(vfvm_sys *sys, const vfvm_bc_t *bc);
+/* sizes: ndof, nnz of the Jacobian pattern, number of stored edges */
+int vfvm_system_sizes(vfvm_sys *sys, int64_t *ndof, int64_t *nnz, int64_t *nedges);
+/* node volumes (sum of node factors), for integrate() and tests (src/vfvm_postprocess.jl:397-412) */
+int vfvm_nodevolumes(vfvm_sys *sys, double *vol);
+
+/* Jacobian pattern as CSC, 1-based, rows sorted: what SparseMatrixCSC(state.matrix)
+ * exposes (src/vfvm_linsolve.jl:3-4, src/vfvm_state.jl:136).  colptr has ndof+1 entries. */
+int vfvm_pattern(vfvm_sys *sys, int64_t *colptr, int64_t *rowval);
+
+/* Dirichlet initialisation of U (in place): _initialize_dirichlet!, src/vfvm_system.jl:947-1003 */
+int vfvm_initialize_dirichlet(vfvm_sys *sys, double *U, double time, double lambda);
+
+/* One assembly: eval_and_assemble(system, U, UOld, F, matrix, ...), src/vfvm_assembly.jl:520-643.
+ * tstep = +Inf means stationary.  F is nspec x nnodes; nzval (may be NULL) receives the
+ * Jacobian values in CSC order of vfvm_pattern. */
+int vfvm_assemble(vfvm_sys *sys, const double *U, const double *Uold, double time, double tstep,
+                  double lambda, double *F, double *nzval);
+
+/* Linear solve A x = b on the currently assembled Jacobian (after vfvm_assemble or inside
+ * vfvm_newton): _solve_linear!, src/vfvm_linsolve.jl:6-61. */
+int vfvm_linsolve(vfvm_sys *sys, const double *b, double *x, const vfvm_control_t *control,
+                  vfvm_linstats_t *stats);
+
+/* Damped Newton: solve_step!, src/vfvm_solver.jl:13-222.  On entry U is ignored and Uold is
+ * the initial value / old time step; on return U holds the solution.  Assembly, linear
+ * solves, damping and the norms stay on the device; hist mirrors NewtonSolverHistory. */
+int vfvm_newton(vfvm_sys *sys, double *U, const double *Uold, double time, double tstep,
+                double lambda, const vfvm_control_t *control, vfvm_history_t *hist);
+
+/* Implicit-Euler time stepping with fixed steps or embedding with fixed parameter steps:
+ * solve_transient!, src/vfvm_solver.jl:267-537 with Δt_min = Δt_max (fixed_timesteps!,
+ * src/vfvm_solvercontrol.jl:320-327).  transient = 1: lambdas are times and each step is
+ * solve_step!(tstep = Δt); transient = 0: lambdas are embedding values (tstep = Inf).
+ * Solutions stay on the device between steps; U_final receives the last one. */
+int vfvm_solve_fixed_steps(vfvm_sys *sys, const double *inival, int64_t nlambdas,
+                           const double *lambdas, int transient, const vfvm_control_t *control,
+                           double *U_final, int64_t *newton_total);
+
+/* ||F||_inf over non-penalty rows for U (residual-only assembly) */
+int vfvm_residual_norm(vfvm_sys *sys, const double *U, const double *Uold, double time,
+                       double tstep, double lambda, double *norm);
+
+/* Autotune the lanes-per-node / lanes-per-row group size of the assembly and SpMV
+ * kernels for this system (project-written: times each legal size on the device). */
+int vfvm_autotune(vfvm_sys *sys, int32_t *asm_group, int32_t *spmv_group);
+
+/* ---- device-resident interface (pointers are device pointers on the handle's device) ----
+ * For hosts that keep U on the GPU (e.g. a Julia AMDGPU.jl array): same contracts as the host
+ * versions, nzval_csr in the device CSR order of vfvm_pattern_csr. */
+int vfvm_pattern_csr(vfvm_sys *sys, int32_t *rowptr, int32_t *colind);
+int vfvm_assemble_device(vfvm_sys *sys, const double *dU, const double *dUold, double time,
+                         double tstep, double lambda, double *dF, double *dnzval_csr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

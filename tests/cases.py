@@ -1,0 +1,137 @@
+"""Builds the same problem twice: as a vfvm.System (HIP path) and as an
+OracleSystem (CPU restatement), from one description."""
+import math
+
+import numpy as np
+
+import oracle as O
+import vfvm
+from vfvm.grids import linspace, simplexgrid
+
+ORACLE_BC = {vfvm.Dirichlet: "dirichlet", vfvm.Robin: "robin"}
+
+
+def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None):
+    """phys_kw: dict(flux=..., reaction=..., storage=...) of vfvm registry objects.
+    bcs: vfvm.Dirichlet / vfvm.Robin entries; legacy: (species, region, value) Dirichlet."""
+    ph = vfvm.Physics(bcondition=list(bcs), source=source, **phys_kw)
+    sys = vfvm.System(grid, ph, species=list(range(1, nspec + 1)))
+    for (sp, reg, val) in legacy:
+        sys.boundary_dirichlet(sp, reg, val)
+    # oracle parameter block from the same registry objects
+    flux, react, stor, p = ph.param_block(nspec)
+    oph = O.Physics()
+    oph.flux, oph.reaction, oph.storage = flux, react, stor
+    for k in range(16):
+        oph.p[k] = p[k]
+    obcs = {}
+    for b in bcs:
+        if isinstance(b, vfvm.Dirichlet):
+            obcs[(b.region, b.species - 1)] = ("dirichlet", b.value, 0.0, b.value_lambda)
+        else:
+            obcs[(b.region, b.species - 1)] = ("robin", b.value, b.factor)
+    for (sp, reg, val) in legacy:
+        obcs[(reg, sp - 1)] = ("dirichlet_legacy", val)
+    osrc = None if source is None else np.asarray(source, dtype=float).reshape(-1, order="F")
+    osys = O.OracleSystem(grid, nspec, oph, obcs, O.EDGEWISE, osrc)
+    return sys, osys
+
+
+def nonlinear_poisson_2d(n=101, eps=1e-2, f=1.0, values=(0.0, 1.0, 1.0, 0.0)):
+    """Nonlinear Poisson: flux eps (u_k - u_l), reaction e^u - e^-u - f; Dirichlet
+    values on south, east, north, west (regions 1..4)."""
+    X = linspace(0.0, 1.0, n)
+    g = simplexgrid(X, X)
+    bcs = [vfvm.Dirichlet(species=1, region=r + 1, value=v) for r, v in enumerate(values)]
+    src = np.full((1, g.num_nodes), f, order="F")
+    return pair(g, 1, dict(flux=vfvm.LinearFlux(eps), reaction=vfvm.SinhReaction()), bcs, source=src)
+
+
+def nonlinear_diffusion_3d(n=17):
+    """3D nonlinear diffusion D(u) = 1 + u^2, storage u, u = 0 on all faces."""
+    X = linspace(0.0, 1.0, n)
+    g = simplexgrid(X, X, X)
+    bcs = [vfvm.Dirichlet(species=1, region=r, value=0.0) for r in range(1, 7)]
+    return pair(g, 1, dict(flux=vfvm.NonlinearDiffusionFlux(1.0, 1.0), storage=vfvm.IdentityStorage()), bcs)
+
+
+def initial_sine_3d(grid):
+    x, y, z = grid.coord
+    return 0.5 * np.sin(math.pi * x) * np.sin(math.pi * y) * np.sin(math.pi * z)
+
+
+def drift_diffusion_2d(n=33, lambda2=1e-4):
+    """Bipolar drift-diffusion (psi, n, p), doping +1 for x < 0.5 else -1, ohmic
+    contacts on west (x = 0) and east (x = 1); east contact biased by lambda."""
+    X = linspace(0.0, 1.0, n)
+    g = simplexgrid(X, X)
+    x = g.coord[0]
+    C = np.where(x < 0.5, 1.0, -1.0)
+    src = np.zeros((3, g.num_nodes), order="F")
+    src[0] = C
+    bcs = []
+    for reg, c, bias in ((4, 1.0, 0.0), (2, -1.0, 1.0)):
+        nn = 0.5 * (c + math.sqrt(c * c + 4.0))
+        pp = 1.0 / nn
+        psi = math.asinh(c / 2.0)
+        bcs += [vfvm.Dirichlet(species=1, region=reg, value=psi, value_lambda=bias),
+                vfvm.Dirichlet(species=2, region=reg, value=nn),
+                vfvm.Dirichlet(species=3, region=reg, value=pp)]
+    phys = dict(flux=vfvm.DriftDiffusionFlux(lambda2, -1.0, 1.0), reaction=vfvm.DriftDiffusionReaction(1.0))
+    return pair(g, 3, phys, bcs, source=src)
+
+
+def dd_initial(grid):
+    x = grid.coord[0]
+    C = np.where(x < 0.5, 1.0, -1.0)
+    U = np.zeros((3, grid.num_nodes), order="F")
+    U[0] = np.arcsinh(C / 2.0)
+    U[1] = 0.5 * (C + np.sqrt(C * C + 4))
+    U[2] = 1.0 / U[1]
+    return U
+
+
+def penalty_rows(osys):
+    """dof mask of rows carrying a Dirichlet penalty (diagonal >= 1e20)."""
+    A = osys.matrix_scipy()
+    return np.abs(A.diagonal()) > 1e20
+
+
+def compare_assembly(sys, osys, U, Uold=None, tstep=math.inf, embed=0.0, rtol=1e-12):
+    """Assemble on both paths; pattern must be identical, values within rtol (normwise,
+    penalty and non-penalty rows separately).  Returns the max relative errors."""
+    Uold = U if Uold is None else Uold
+    Fo = osys.assemble(vfvm.dofs(U), vfvm.dofs(Uold), tstep=tstep, embed=embed)
+    Ao = osys.matrix_scipy().tocsc()
+    Ao.sort_indices()
+    import ctypes as C
+    from vfvm import _capi
+    L = _capi.lib()
+    h = sys.handle()
+    ndof, nnz, _ = sys.sizes()
+    Ud = np.ascontiguousarray(vfvm.dofs(U), dtype=float)
+    Uod = np.ascontiguousarray(vfvm.dofs(Uold), dtype=float)
+    F = np.zeros(ndof)
+    nz = np.zeros(nnz)
+    _capi.check(L.vfvm_assemble(h, _capi.dptr(Ud), _capi.dptr(Uod), 0.0, tstep, embed, _capi.dptr(F), _capi.dptr(nz)))
+    cp, rv = sys.pattern()
+    assert np.array_equal(cp - 1, Ao.indptr), "colptr differs from the oracle pattern"
+    assert np.array_equal(rv - 1, Ao.indices), "rowval differs from the oracle pattern"
+    pen = np.abs(Ao.diagonal()) > 1e20
+    rows = np.repeat(np.arange(ndof), 0)  # placeholder
+    # entrywise row index of each CSC value
+    rowidx = Ao.indices
+    errs = {}
+    for name, mask in (("free", ~pen), ("penalty", pen)):
+        if mask.sum() == 0:
+            continue
+        fo, fg = Fo[mask], F[mask]
+        sc = max(np.abs(fo).max(), 1e-300)
+        errs["F_" + name] = np.abs(fo - fg).max() / sc
+        em = mask[rowidx]
+        jo, jg = Ao.data[em], nz[em]
+        sc = max(np.abs(jo).max(), 1e-300)
+        errs["J_" + name] = np.abs(jo - jg).max() / sc
+    for k, v in errs.items():
+        assert v <= rtol, (k, v)
+    return errs

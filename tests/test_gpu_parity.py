@@ -1,0 +1,197 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: Jacobian pattern identical; residual and Jacobian values within 1e-12
+relative (normwise, penalty and non-penalty rows separately); converged
+solutions within 1e-9 in the max norm; ||F||_inf <= 1e-10 over non-penalty
+rows at the returned solution.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+import oracle as O
+import vfvm
+from vfvm.grids import linspace, simplexgrid
+
+pytestmark = pytest.mark.gpu
+
+ASM_RTOL = 1e-12
+SOL_ATOL = 1e-9
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("n", [5, 33])
+def test_assembly_nonlinear_poisson_2d(n):
+    _print("case assembly poisson2d", n)
+    sys, osys = cases.nonlinear_poisson_2d(n)
+    rng = np.random.default_rng(n)
+    U = rng.uniform(-1, 1, (1, sys.grid.num_nodes))
+    cases.compare_assembly(sys, osys, U)
+
+
+def test_assembly_nonlinear_diffusion_3d_transient():
+    _print("case assembly nldiff3d")
+    sys, osys = cases.nonlinear_diffusion_3d(9)
+    rng = np.random.default_rng(1)
+    U = rng.uniform(0, 1, (1, sys.grid.num_nodes))
+    cases.compare_assembly(sys, osys, U, Uold=0.7 * U, tstep=1e-2)
+
+
+def test_assembly_drift_diffusion_2d():
+    _print("case assembly dd2d")
+    sys, osys = cases.drift_diffusion_2d(17, lambda2=1e-2)
+    rng = np.random.default_rng(2)
+    U = cases.dd_initial(sys.grid)
+    U[0] += rng.uniform(-2, 2, sys.grid.num_nodes)  # Bernoulli arguments across all branches
+    U[1:] *= rng.uniform(0.5, 1.5, (2, sys.grid.num_nodes))
+    cases.compare_assembly(sys, osys, U, embed=0.3)
+
+
+def test_assembly_two_species_exchange_legacy_bc():
+    _print("case assembly exchange")
+    X = linspace(0, 1, 12)
+    g = simplexgrid(X, X)
+    rng = np.random.default_rng(3)
+    src = rng.random((2, g.num_nodes))
+    sys, osys = cases.pair(g, 2, dict(flux=vfvm.LinearFlux([1e-2, 2e-2]), reaction=vfvm.ExchangeReaction(1.0),
+                                      storage=vfvm.IdentityStorage()),
+                           legacy=[(1, 2, 0.1), (2, 4, 0.3)], source=np.asfortranarray(src))
+    U = rng.random((2, g.num_nodes))
+    cases.compare_assembly(sys, osys, np.asfortranarray(U), Uold=np.asfortranarray(0.5 * U), tstep=0.01)
+
+
+def test_assembly_1d_power_robin():
+    _print("case assembly power1d")
+    g = simplexgrid(linspace(0, 1, 41))
+    sys, osys = cases.pair(g, 1, dict(flux=vfvm.PowerFlux(2, 1.0), reaction=vfvm.PowerReaction(2, 1.0)),
+                           [vfvm.Dirichlet(1, 1, 0.1), vfvm.Neumann(1, 2, 3.0)])
+    U = np.linspace(0.1, 2.0, g.num_nodes).reshape(1, -1)
+    cases.compare_assembly(sys, osys, U)
+
+
+def test_newton_nonlinear_poisson_matches_oracle():
+    """cfg0: 101 x 101 nonlinear Poisson, FP64 device Newton vs oracle (direct solve)."""
+    _print("case newton poisson2d 101")
+    sys, osys = cases.nonlinear_poisson_2d(101)
+    Uo, ho = osys.solve_step(np.zeros(sys.grid.num_nodes))
+    c = vfvm.SolverControl(reltol_linear=1e-12, abstol_linear=1e-14, maxiters_linear=2000, precon="ilu0")
+    U = vfvm.solve(sys, inival=0.0, control=c)
+    err = np.abs(vfvm.dofs(U) - Uo).max()
+    _print("newton iters gpu", sys.history.niter, "oracle", ho.niter, "err", err, "res", sys.history.residual_norm)
+    assert err <= SOL_ATOL
+    assert sys.history.residual_norm <= 1e-10
+
+
+@pytest.mark.parametrize("precon,precision", [("jacobi", "fp64"), ("ilu0", "fp64"), ("jacobi", "fp32"),
+                                              ("ilu0", "fp32")])
+def test_newton_solver_variants(precon, precision):
+    _print("case newton variants", precon, precision)
+    sys, osys = cases.nonlinear_poisson_2d(41, eps=1e-2, f=2.0, values=(-0.5, 1.0, 0.2, 0.0))
+    Uo, _ = osys.solve_step(np.zeros(sys.grid.num_nodes))
+    c = vfvm.SolverControl(precon=precon, precision=precision, reltol_linear=1e-10, abstol_linear=1e-14,
+                           maxiters_linear=3000)
+    U = vfvm.solve(sys, inival=0.0, control=c)
+    assert np.abs(vfvm.dofs(U) - Uo).max() <= SOL_ATOL
+    assert sys.history.residual_norm <= 1e-10
+
+
+@pytest.mark.parametrize("krylov,precon,precision", [("bicgstab", "jacobi", "fp64"), ("bicgstab", "ilu0", "fp64"),
+                                                     ("gmres", "jacobi", "fp64"), ("gmres", "ilu0", "fp32"),
+                                                     ("bicgstab", "ilu0", "fp32")])
+def test_linear_solvers_agree_with_direct(krylov, precon, precision):
+    """examples/DevEx003_Solvers.jl:85-177 rule: every solver agrees with the direct one to 1e-7."""
+    _print("case linsolve", krylov, precon, precision)
+    import scipy.sparse.linalg as sla
+    sys, osys = cases.nonlinear_poisson_2d(51)
+    rng = np.random.default_rng(4)
+    U = rng.uniform(-1, 1, (1, sys.grid.num_nodes))
+    F, A = vfvm.evaluate_residual_and_jacobian(sys, U)
+    b = vfvm.dofs(F)
+    xd = sla.spsolve(A.tocsc(), b)
+    from vfvm import _capi
+    import ctypes as C
+    c = vfvm.SolverControl(krylov=krylov, precon=precon, precision=precision, reltol_linear=1e-11,
+                           abstol_linear=0.0, maxiters_linear=5000).to_c()
+    x = np.zeros_like(b)
+    st = _capi.LinStatsT()
+    _capi.check(_capi.lib().vfvm_linsolve(sys.handle(), _capi.dptr(np.ascontiguousarray(b)), _capi.dptr(x),
+                                          C.byref(c), C.byref(st)))
+    _print("iters", st.iters, "refinements", st.refinements)
+    assert np.abs(x - xd).max() <= 1e-7 * max(1.0, np.abs(xd).max())
+
+
+def test_block_jacobi_gmres_drift_diffusion_continuation():
+    """cfg2 at small size: bias continuation with GMRES + 3x3 block-Jacobi vs oracle."""
+    _print("case dd continuation")
+    sys, osys = cases.drift_diffusion_2d(33, lambda2=1e-2)
+    U0 = cases.dd_initial(sys.grid)
+    lam = np.linspace(0.0, 1.0, 6)
+    c = O.control(method_linear="direct", dp=0.2, dp_min=0.2, dp_max=0.2, du_opt=10.0)
+    Uo, _, _ = osys.solve_transient(vfvm.dofs(U0), lam, c, transient=False)
+    for precision in ("fp64", "fp32"):
+        ctrl = vfvm.SolverControl(krylov="gmres", precon="blockjacobi", precision=precision, reltol_linear=1e-10,
+                                  abstol_linear=0.0, maxiters_linear=3000, gmres_restart=30)
+        U = vfvm.solve(sys, inival=U0, control=ctrl, embed=lam)
+        err = np.abs(vfvm.dofs(U) - Uo).max()
+        _print("dd", precision, "err", err)
+        assert err <= SOL_ATOL
+
+
+def test_transient_nonlinear_diffusion_3d():
+    """cfg1 at small size: implicit Euler, 10 steps of 1e-2, BiCGStab + ILU(0) FP32 vs oracle."""
+    _print("case transient nldiff3d")
+    sys, osys = cases.nonlinear_diffusion_3d(9)
+    u0 = cases.initial_sine_3d(sys.grid).reshape(1, -1)
+    times = np.linspace(0.0, 0.1, 11)
+    c = O.control(method_linear="direct", dt=1e-2, dt_min=1e-2, dt_max=1e-2, du_opt=10.0)
+    Uo, nsteps, _ = osys.solve_transient(u0.ravel(), times, c)
+    assert nsteps == 10
+    for precision in ("fp64", "fp32"):
+        ctrl = vfvm.SolverControl(precon="ilu0", precision=precision, reltol_linear=1e-10, abstol_linear=0.0,
+                                  maxiters_linear=2000)
+        U = vfvm.solve(sys, inival=np.asfortranarray(u0), control=ctrl, times=times)
+        assert np.abs(vfvm.dofs(U) - Uo).max() <= SOL_ATOL
+
+
+def test_examples_known_answers_on_gpu():
+    """The GPU path reproduces the reference's own testvals (Example105, Example301)."""
+    _print("case known answers")
+    X = np.arange(11) / 10.0
+    g = simplexgrid(X)
+    sys = vfvm.System(g, vfvm.Physics(flux=vfvm.LinearFlux(1e-3), reaction=vfvm.SinhReaction(),
+                                      source=np.where(X <= 0.5, 1.0, -1.0)), species=[1])
+    sys.boundary_dirichlet(1, 1, 0.0)
+    sys.boundary_dirichlet(1, 2, 1.0)
+    c = vfvm.SolverControl(reltol_linear=1e-13, abstol_linear=0.0, maxiters_linear=200)
+    U = vfvm.solve(sys, inival=0.5, control=c)
+    assert U.sum() == pytest.approx(1.5247901344230088, rel=1.4901161193847656e-08)
+    X = np.arange(6) / 5.0
+    g = simplexgrid(X, X, X)
+    x, y, z = g.coord
+    sys = vfvm.System(g, vfvm.Physics(flux=vfvm.LinearFlux(1.0), source=x * np.sin(5.0 * y) * np.exp(z)),
+                      species=[1])
+    sys.boundary_dirichlet(1, 5, 0.0)
+    sys.boundary_dirichlet(1, 6, 0.0)
+    U = vfvm.solve(sys, control=c)
+    assert vfvm.dofs(U)[42] == pytest.approx(0.012234524449380824, rel=1.4901161193847656e-08)
+
+
+def test_nan_raises_assembly_error():
+    _print("case nan")
+    sys, _ = cases.nonlinear_poisson_2d(9)
+    U = np.zeros((1, sys.grid.num_nodes), order="F")
+    U[0, 40] = np.nan
+    with pytest.raises(vfvm.AssemblyError):
+        vfvm.evaluate_residual_and_jacobian(sys, U)
+
+
+def test_maxiters_raises_convergence_error():
+    _print("case maxiters")
+    sys, _ = cases.nonlinear_poisson_2d(21)
+    with pytest.raises(vfvm.ConvergenceError):
+        vfvm.solve(sys, inival=0.0, control=vfvm.SolverControl(maxiters=1))

@@ -1,0 +1,319 @@
+// Residual + Jacobian assembly for gfx950: node-centric, atomics-free.
+//
+// Replaces eval_and_assemble's loops (src/vfvm_assembly.jl:38-442):
+//   assemble_nodes  -> node terms of row K (reaction, storage, source)
+//   assemble_edges  -> each node evaluates the flux of its incident edges
+//   assemble_bnodes -> boundary items of node K (penalty Dirichlet, Robin)
+// The reference scatters every edge's 4 s^2 partials into a CSC matrix with a
+// search per entry (_addnz -> rawupdateindex!, src/vfvm_assembly.jl:20-28).
+// Here a group of G lanes owns node K and writes its rows of the CSR Jacobian
+// directly: lane l evaluates the fluxes of adjacency entries l, l+G, ...  with
+// K as the first argument (every registry flux is antisymmetric,
+// flux(u_l, u_k) = -flux(u_k, u_l) exactly in IEEE arithmetic, so row K's
+// contribution +fac*flux(u_k, u_l) equals the reference's -fac*flux(u_l, u_k)
+// when K is the edge's second node).  Off-diagonal block values go straight to
+// their slots; the diagonal block and F are reduced across the group.  No
+// atomics, no scatter, each J entry written exactly once.
+//
+// Slot map: rows of node K are contiguous and ordered by species; within row
+// (K, i) the blocks follow the sorted node graph, a block holding the species
+// columns of emask[i] (dmask[i] on the diagonal).  The slot of (K, i, block e,
+// species j) is therefore computed from adj_ptr[K], adj_diag[K] and the masks,
+// without any per-entry index array.
+#include "vfvm_physics.h"
+
+namespace vfvm {
+
+namespace {
+
+constexpr int ASM_BLOCK = 256;
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+template <int NS>
+__device__ __forceinline__ int rank_in(uint8_t mask, int j) {
+  return __builtin_popcount((unsigned)mask & ((1u << j) - 1u));
+}
+
+template <int NS, int G, bool JAC>
+__global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
+  constexpr int NP = 2 * NS;  // edge partials: u_k then u_l
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t K = (int64_t)blockIdx.x * (ASM_BLOCK / G) + threadIdx.x / G;
+  if (K >= a.nnodes) return;  // whole group leaves together (G divides the wave)
+  const int inst = a.node_inst ? a.node_inst[K] : 0;
+  const InstParams *P = a.inst + inst;
+  const int flux_kind = P->flux;
+  double p[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) p[k] = P->p[k];
+
+  const int32_t a0 = a.adj_ptr[K], a1 = a.adj_ptr[K + 1];
+  const int deg1 = a1 - a0;
+  const int diag = a.adj_diag[K];
+  double uK[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) uK[i] = a.U[K * NS + i];
+
+  // first slot of row (K, i): rows of earlier nodes, then earlier species of K
+  int64_t rowbase[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++)
+    rowbase[i] = (int64_t)(a0 - K) * a.m.etot + K * a.m.dtot + (int64_t)(deg1 - 1) * a.m.eoff[i] + a.m.doff[i];
+
+  double Fe[NS];
+  double Jd[NS][NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    Fe[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; j++) Jd[i][j] = 0.0;
+  }
+  bool bad = false;
+
+  for (int e = lane; e < deg1; e += G) {
+    const int32_t M = a.adj_col[a0 + e];
+    if (M == K) continue;
+    const double f = a.adj_fac[a0 + e];
+    if constexpr (JAC) {
+      Dual<NP> uk[NS], ul[NS], g[NS];
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        uk[i] = dvar<NP>(uK[i], i);
+        ul[i] = dvar<NP>(a.U[(int64_t)M * NS + i], NS + i);
+      }
+      eval_flux<NS>(flux_kind, p, uk, ul, g);
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        Fe[i] += f * g[i].v;
+        const int64_t boff =
+            rowbase[i] + (e < diag ? (int64_t)e * a.m.ecnt[i] : (int64_t)(e - 1) * a.m.ecnt[i] + a.m.dcnt[i]);
+        const uint8_t em = a.m.emask[i];
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+          if (em & (1u << j)) {
+            const double dk = g[i].d[j], dl = g[i].d[NS + j];
+            bad |= (dk != dk) | (dl != dl);
+            Jd[i][j] += f * dk;
+            a.J[boff + rank_in<NS>(em, j)] = f * dl;
+          }
+        }
+      }
+    } else {
+      double uk[NS], ul[NS], g[NS];
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        uk[i] = uK[i];
+        ul[i] = a.U[(int64_t)M * NS + i];
+      }
+      eval_flux<NS>(flux_kind, p, uk, ul, g);
+#pragma unroll
+      for (int i = 0; i < NS; i++) Fe[i] += f * g[i];
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    Fe[i] = group_sum<G>(Fe[i]);
+    if constexpr (JAC) {
+#pragma unroll
+      for (int j = 0; j < NS; j++) Jd[i][j] = group_sum<G>(Jd[i][j]);
+    }
+  }
+
+  if (lane == 0) {
+    // node item: src/vfvm_assembly.jl:81-122
+    const double vol = a.nodevol[K];
+    double Fn[NS], Jn[NS][NS];
+    {
+      double uo[NS], so[NS];
+#pragma unroll
+      for (int i = 0; i < NS; i++) uo[i] = a.Uold[K * NS + i];
+      eval_storage<NS>(P->storage, uo, so);
+      if constexpr (JAC) {
+        Dual<NS> u[NS], r[NS], st[NS];
+#pragma unroll
+        for (int i = 0; i < NS; i++) u[i] = dvar<NS>(uK[i], i);
+        eval_reaction<NS>(P->reaction, p, u, r);
+        eval_storage<NS>(P->storage, u, st);
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          const double src = a.src ? a.src[K * NS + i] : 0.0;
+          Fn[i] = vol * (r[i].v - src + (st[i].v - so[i]) * a.tstepinv);
+#pragma unroll
+          for (int j = 0; j < NS; j++) {
+            const double v = r[i].d[j] + st[i].d[j] * a.tstepinv;
+            bad |= (v != v);
+            Jn[i][j] = v * vol;
+          }
+        }
+      } else {
+        double r[NS], st[NS];
+        eval_reaction<NS>(P->reaction, p, uK, r);
+        eval_storage<NS>(P->storage, uK, st);
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          const double src = a.src ? a.src[K * NS + i] : 0.0;
+          Fn[i] = vol * (r[i] - src + (st[i] - so[i]) * a.tstepinv);
+        }
+      }
+    }
+    // boundary items: src/vfvm_assembly.jl:352-407
+    double Fb[NS], Jb[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) Fb[i] = Jb[i] = 0.0;
+    for (int32_t b = a.bn_ptr[K]; b < a.bn_ptr[K + 1]; b++) {
+      const int reg = a.bn_reg[b];
+      const double fac = a.bn_fac[b];
+      if (reg < 1 || reg > MAXBREG) continue;
+      double penalty = DIRICHLET_PENALTY;
+      if (P->has_legacy) {
+        penalty = DIRICHLET_PENALTY / fac;
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          const BCDev &bc = P->bc[reg - 1][i];
+          if (bc.kind == VFVM_BC_DIRICHLET_LEGACY) {
+            Fb[i] += DIRICHLET_PENALTY * (uK[i] - (bc.value + bc.value_lambda * a.lambda));
+            Jb[i] += DIRICHLET_PENALTY;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        const BCDev &bc = P->bc[reg - 1][i];
+        if (bc.kind == VFVM_BC_DIRICHLET) {
+          Fb[i] += fac * (penalty * (uK[i] - (bc.value + bc.value_lambda * a.lambda)) - 0.0);
+          Jb[i] += penalty * fac;
+        } else if (bc.kind == VFVM_BC_ROBIN) {
+          Fb[i] += fac * ((bc.factor * uK[i] - (bc.value + bc.value_lambda * a.lambda)) - 0.0);
+          Jb[i] += bc.factor * fac;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NS; i++) a.F[K * NS + i] = (Fn[i] + Fe[i]) + Fb[i];
+    if constexpr (JAC) {
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        const int64_t boff = rowbase[i] + (int64_t)diag * a.m.ecnt[i];
+        const uint8_t dm = a.m.dmask[i];
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+          if (dm & (1u << j)) {
+            double v = Jn[i][j] + Jd[i][j];
+            if (i == j) v += Jb[i];
+            a.J[boff + rank_in<NS>(dm, j)] = v;
+          }
+        }
+      }
+    }
+  }
+  if (JAC && bad) atomicOr(a.flags, 1);
+}
+
+template <int NS, int G>
+void launch_ns_g(const AsmArgs &a, hipStream_t s) {
+  const int64_t nodes_per_block = ASM_BLOCK / G;
+  const int64_t grid = (a.nnodes + nodes_per_block - 1) / nodes_per_block;
+  if (grid == 0) return;
+  if (a.J)
+    hipLaunchKernelGGL((k_assemble<NS, G, true>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_assemble<NS, G, false>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
+}
+
+template <int NS>
+void launch_ns(const AsmArgs &a, int group, hipStream_t s) {
+  switch (group) {
+    case 4: launch_ns_g<NS, 4>(a, s); break;
+    case 8: launch_ns_g<NS, 8>(a, s); break;
+    case 16: launch_ns_g<NS, 16>(a, s); break;
+    case 32: launch_ns_g<NS, 32>(a, s); break;
+    default: throw Error(VFVM_ERR_ARG, "assembly group size must be 4, 8, 16 or 32");
+  }
+}
+
+// _initialize_dirichlet!: the last boundary item (bface order) with a Dirichlet
+// condition for species i sets U[K, i] (src/vfvm_system.jl:975-1000)
+__global__ void k_init_dirichlet(int64_t nb, const int32_t *bnodes, const int32_t *bn_ptr, const int32_t *bn_reg,
+                                 const int32_t *node_inst, const InstParams *inst, int nspec, double lambda,
+                                 double *U) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  const int32_t K = bnodes[t];
+  const InstParams *P = inst + (node_inst ? node_inst[K] : 0);
+  for (int i = 0; i < nspec; i++) {
+    double v = 0.0;
+    bool set = false;
+    for (int32_t b = bn_ptr[K]; b < bn_ptr[K + 1]; b++) {
+      const int reg = bn_reg[b];
+      if (reg < 1 || reg > MAXBREG) continue;
+      const BCDev &bc = P->bc[reg - 1][i];
+      if (bc.kind == VFVM_BC_DIRICHLET || bc.kind == VFVM_BC_DIRICHLET_LEGACY) {
+        v = bc.value + bc.value_lambda * lambda;
+        set = true;
+      }
+    }
+    if (set) U[(int64_t)K * nspec + i] = v;
+  }
+}
+
+// per-block max of |F| over rows that carry no Dirichlet penalty; partials are
+// non-negative doubles, combined per instance with an integer atomicMax on their bits
+__global__ void k_penalty_free_norm(int64_t nnodes, int nspec, const double *F, const int32_t *bn_ptr,
+                                    const int32_t *bn_reg, const int32_t *node_inst, const InstParams *inst,
+                                    unsigned long long *out) {
+  int64_t K = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (K >= nnodes) return;
+  const int id = node_inst ? node_inst[K] : 0;
+  const InstParams *P = inst + id;
+  double m = 0.0;
+  for (int i = 0; i < nspec; i++) {
+    bool pen = false;
+    for (int32_t b = bn_ptr[K]; b < bn_ptr[K + 1]; b++) {
+      const int reg = bn_reg[b];
+      if (reg < 1 || reg > MAXBREG) continue;
+      const int kind = P->bc[reg - 1][i].kind;
+      pen |= (kind == VFVM_BC_DIRICHLET || kind == VFVM_BC_DIRICHLET_LEGACY);
+    }
+    if (!pen) m = fmax(m, fabs(F[K * nspec + i]));
+  }
+  if (m > 0.0) atomicMax(out + id, (unsigned long long)__double_as_longlong(m));
+}
+
+}  // namespace
+
+void launch_assemble(const AsmArgs &a, int group, hipStream_t s) {
+  switch (a.nspec) {
+    case 1: launch_ns<1>(a, group, s); break;
+    case 2: launch_ns<2>(a, group, s); break;
+    case 3: launch_ns<3>(a, group, s); break;
+    default: throw Error(VFVM_ERR_ARG, "nspec must be 1, 2 or 3");
+  }
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_init_dirichlet(int64_t nb, const int32_t *bnodes, const int32_t *bn_ptr, const int32_t *bn_reg,
+                           const int32_t *node_inst, const InstParams *inst, int nspec, double lambda, double *U,
+                           hipStream_t s) {
+  if (nb == 0) return;
+  hipLaunchKernelGGL(k_init_dirichlet, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, nb, bnodes, bn_ptr, bn_reg,
+                     node_inst, inst, nspec, lambda, U);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_penalty_free_norm(int64_t nnodes, int nspec, const double *F, const int32_t *bn_ptr,
+                              const int32_t *bn_reg, const int32_t *node_inst, const InstParams *inst,
+                              double *out, int64_t, hipStream_t s) {
+  if (nnodes == 0) return;
+  hipLaunchKernelGGL(k_penalty_free_norm, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, s, nnodes, nspec, F,
+                     bn_ptr, bn_reg, node_inst, inst, reinterpret_cast<unsigned long long *>(out));
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace vfvm

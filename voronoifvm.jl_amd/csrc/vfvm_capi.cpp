@@ -1,0 +1,755 @@
+// extern "C" entry points of libvfvm_hip.so and the device-resident Newton driver.
+//
+// The Newton loop restates solve_step! (src/vfvm_solver.jl:13-222): assembly,
+// linear solve, damped update and the update-norm / round-off / monotonicity
+// tests.  Assembly, solve, update and both norms run on the device; per Newton
+// iteration the host reads back two doubles and one flag word.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "vfvm_common.h"
+#include "vfvm_linalg.h"
+
+using namespace vfvm;
+
+struct vfvm_sys {
+  vfvm_ctx *ctx = nullptr;
+  int dim = 0, nspec = 1, nbreg = 0;
+  int64_t nnodes = 0, ndof = 0, nnz = 0;
+  vfvm_physics_t phys{};
+  std::vector<vfvm_bc_t> bc;
+  Masks masks{};
+  HostSetup hs;
+  std::vector<int32_t> rowptr, colind, csc_perm, csc_rowval, dofdiag;
+  std::vector<int64_t> csc_colptr;
+  // device data
+  DevBuf<int32_t> d_adj_ptr, d_adj_col, d_adj_diag, d_bn_ptr, d_bn_reg, d_bnodes, d_rowptr, d_colind, d_csc_perm,
+      d_dofdiag, d_flags;
+  DevBuf<double> d_adj_fac, d_nodevol, d_bn_fac, d_U, d_Uold, d_F, d_J, d_src, d_dU, d_tmp, d_red, d_partials;
+  DevBuf<unsigned> d_ticket;
+  DevBuf<InstParams> d_inst;
+  bool has_src = false;
+  // multicolour ILU(0) structure
+  IluStruct ilu;
+  bool ilu_built = false;
+  DevBuf<int32_t> d_ilu_rowptr, d_ilu_colind, d_ilu_diag, d_ilu_perm, d_ilu_src, d_ilu_srcrow;
+  LinearSolver lin;
+  int asm_group = 8, spmv_group = 8;
+  double *h_pinned = nullptr;  // small pinned readback area
+};
+
+namespace {
+
+std::mutex g_err_mutex;
+std::string g_last_create_error;
+
+int fail(vfvm_ctx *ctx, int code, const std::string &msg) {
+  if (ctx)
+    ctx->err = msg;
+  else {
+    std::lock_guard<std::mutex> lk(g_err_mutex);
+    g_last_create_error = msg;
+  }
+  return code;
+}
+
+template <class F>
+int guarded(vfvm_ctx *ctx, F &&f) {
+  try {
+    return f();
+  } catch (const Error &e) {
+    return fail(ctx, e.code, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, VFVM_ERR_HIP, "host out of memory");
+  } catch (const std::exception &e) {
+    return fail(ctx, VFVM_ERR_HIP, e.what());
+  }
+}
+
+int next_pow2_clamp(int v, int lo, int hi) {
+  int g = lo;
+  while (g < v && g < hi) g *= 2;
+  return g;
+}
+
+InstParams make_inst(const vfvm_sys &s) {
+  InstParams P{};
+  P.flux = s.phys.flux;
+  P.reaction = s.phys.reaction;
+  P.storage = s.phys.storage;
+  for (int k = 0; k < 16; k++) P.p[k] = s.phys.p[k];
+  P.has_legacy = 0;
+  for (int r = 0; r < s.nbreg && r < MAXBREG; r++)
+    for (int i = 0; i < s.nspec; i++) {
+      const vfvm_bc_t &b = s.bc[(size_t)r * s.nspec + i];
+      BCDev &d = P.bc[r][i];
+      d.kind = b.kind;
+      d.value = b.value;
+      d.value_lambda = b.value_lambda;
+      d.factor = b.factor;
+      if (b.kind == VFVM_BC_DIRICHLET_LEGACY) P.has_legacy = 1;
+    }
+  return P;
+}
+
+void upload_inst(vfvm_sys &s) {
+  InstParams P = make_inst(s);
+  s.d_inst.upload(&P, 1, s.ctx->stream);
+  VFVM_HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+}
+
+AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, double *J, double tstep,
+                 double lambda) {
+  AsmArgs a{};
+  a.nnodes = s.nnodes;
+  a.adj_ptr = s.d_adj_ptr.p;
+  a.adj_col = s.d_adj_col.p;
+  a.adj_diag = s.d_adj_diag.p;
+  a.adj_fac = s.d_adj_fac.p;
+  a.nodevol = s.d_nodevol.p;
+  a.bn_ptr = s.d_bn_ptr.p;
+  a.bn_reg = s.d_bn_reg.p;
+  a.bn_fac = s.d_bn_fac.p;
+  a.node_inst = nullptr;
+  a.inst = s.d_inst.p;
+  a.U = U;
+  a.Uold = Uold;
+  a.src = s.has_src ? s.d_src.p : nullptr;
+  a.F = F;
+  a.J = J;
+  a.tstepinv = 1.0 / tstep;  // 1/Inf = 0: stationary (src/vfvm_assembly.jl:554)
+  a.lambda = lambda;
+  a.flags = s.d_flags.p;
+  a.m = s.masks;
+  a.nspec = s.nspec;
+  return a;
+}
+
+void build_ilu(vfvm_sys &s) {
+  if (s.ilu_built) return;
+  if (s.nspec != 1) throw Error(VFVM_ERR_ARG, "ILU(0) is provided for single-species systems; use block-Jacobi");
+  std::vector<int32_t> color;
+  int nc = greedy_colors(s.hs, color);
+  int64_t n = s.nnodes;
+  std::vector<int32_t> perm((size_t)n), iperm((size_t)n);
+  std::vector<int32_t> cptr((size_t)nc + 1, 0);
+  for (int64_t i = 0; i < n; i++) cptr[color[i] + 1]++;
+  for (int c = 0; c < nc; c++) cptr[c + 1] += cptr[c];
+  {
+    std::vector<int32_t> nx(cptr.begin(), cptr.end() - 1);
+    for (int64_t i = 0; i < n; i++) {
+      int32_t p = nx[color[i]]++;
+      perm[p] = (int32_t)i;
+      iperm[i] = p;
+    }
+  }
+  std::vector<int32_t> prow((size_t)n + 1, 0), pcol((size_t)s.nnz), pdiag((size_t)n), psrc((size_t)s.nnz),
+      psrcrow((size_t)s.nnz);
+  for (int64_t i = 0; i < n; i++) prow[i + 1] = prow[i] + (s.rowptr[perm[i] + 1] - s.rowptr[perm[i]]);
+  std::vector<std::pair<int32_t, int32_t>> tmp;
+  for (int64_t i = 0; i < n; i++) {
+    int32_t r = perm[i];
+    tmp.clear();
+    for (int32_t k = s.rowptr[r]; k < s.rowptr[r + 1]; k++) tmp.emplace_back(iperm[s.colind[k]], k);
+    std::sort(tmp.begin(), tmp.end());
+    int32_t p = prow[i];
+    pdiag[i] = -1;
+    for (auto &e : tmp) {
+      pcol[p] = e.first;
+      psrc[p] = e.second;
+      psrcrow[p] = r;
+      if (e.first == i) pdiag[i] = p;
+      p++;
+    }
+    if (pdiag[i] < 0) throw Error(VFVM_ERR_ARG, "ILU(0): missing diagonal");
+  }
+  hipStream_t st = s.ctx->stream;
+  s.d_ilu_rowptr.upload(prow, st);
+  s.d_ilu_colind.upload(pcol, st);
+  s.d_ilu_diag.upload(pdiag, st);
+  s.d_ilu_perm.upload(perm, st);
+  s.d_ilu_src.upload(psrc, st);
+  s.d_ilu_srcrow.upload(psrcrow, st);
+  VFVM_HIP_CHECK(hipStreamSynchronize(st));
+  s.ilu.ncolors = nc;
+  s.ilu.cptr = cptr;
+  s.ilu.rowptr = s.d_ilu_rowptr.p;
+  s.ilu.colind = s.d_ilu_colind.p;
+  s.ilu.diag = s.d_ilu_diag.p;
+  s.ilu.perm = s.d_ilu_perm.p;
+  s.ilu.src = s.d_ilu_src.p;
+  s.ilu.srcrow = s.d_ilu_srcrow.p;
+  s.ilu_built = true;
+}
+
+void setup_linear(vfvm_sys &s, const vfvm_control_t &c) {
+  if (c.precon == VFVM_PRECON_ILU0) build_ilu(s);
+  LinSetup L;
+  L.ndof = s.ndof;
+  L.nnodes = s.nnodes;
+  L.nnz = s.nnz;
+  L.nspec = s.nspec;
+  L.rowptr = s.d_rowptr.p;
+  L.colind = s.d_colind.p;
+  L.dofdiag = s.d_dofdiag.p;
+  L.ilu = s.ilu_built ? &s.ilu : nullptr;
+  L.spmv_group = s.spmv_group;
+  s.lin.setup(L, s.ctx->stream);
+}
+
+void check_control(const vfvm_control_t &c, const vfvm_sys &s) {
+  if (c.krylov != VFVM_KRYLOV_BICGSTAB && c.krylov != VFVM_KRYLOV_GMRES) throw Error(VFVM_ERR_ARG, "bad krylov");
+  if (c.precon < 0 || c.precon > 2) throw Error(VFVM_ERR_ARG, "bad precon");
+  if (c.precon == VFVM_PRECON_ILU0 && s.nspec != 1)
+    throw Error(VFVM_ERR_ARG, "ILU(0) is provided for single-species systems; use block-Jacobi");
+  if (c.gmres_restart < 1 || c.gmres_restart > 64) throw Error(VFVM_ERR_ARG, "gmres_restart must be in 1..64");
+  if (!c.updatecontrol)
+    throw Error(VFVM_ERR_ARG, "updatecontrol = false (residual-norm control) is not provided by the device loop");
+}
+
+// readback of the small result area: [0..1] norms, [2] flags
+struct Readback {
+  double a, b;
+  int32_t flags;
+};
+
+Readback read_norms(vfvm_sys &s) {
+  hipStream_t st = s.ctx->stream;
+  VFVM_HIP_CHECK(hipMemcpyAsync(s.h_pinned, s.d_red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  VFVM_HIP_CHECK(hipMemcpyAsync(s.h_pinned + 2, s.d_flags.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  VFVM_HIP_CHECK(hipStreamSynchronize(st));
+  Readback r;
+  r.a = s.h_pinned[0];
+  r.b = s.h_pinned[1];
+  std::memcpy(&r.flags, s.h_pinned + 2, sizeof(int32_t));
+  return r;
+}
+
+double penalty_free_norm(vfvm_sys &s, const double *dF) {
+  hipStream_t st = s.ctx->stream;
+  launch_fill_u64(1, reinterpret_cast<unsigned long long *>(s.d_red.p + 3), st);
+  launch_penalty_free_norm(s.nnodes, s.nspec, dF, s.d_bn_ptr.p, s.d_bn_reg.p, nullptr, s.d_inst.p, s.d_red.p + 3, 1,
+                           st);
+  double v = 0;
+  VFVM_HIP_CHECK(hipMemcpyAsync(&v, s.d_red.p + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+  VFVM_HIP_CHECK(hipStreamSynchronize(st));
+  return v;
+}
+
+// Device-resident solve_step!: d_Uold -> d_U
+int newton_device(vfvm_sys &s, double time, double tstep, double lambda, const vfvm_control_t &c,
+                  vfvm_history_t *hist) {
+  (void)time;
+  hipStream_t st = s.ctx->stream;
+  vfvm_history_t h{};
+  auto t0 = std::chrono::steady_clock::now();
+  launch_copy(s.ndof, s.d_Uold.p, s.d_U.p, st);  // solution .= oldsol
+  launch_init_dirichlet((int64_t)s.hs.bnodes.size(), s.d_bnodes.p, s.d_bn_ptr.p, s.d_bn_reg.p, nullptr, s.d_inst.p,
+                        s.nspec, lambda, s.d_U.p, st);
+  launch_fill_i32(1, s.d_flags.p, 0, st);
+  double oldnorm = 1.0, damp = 1.0, rnorm = 0.0, norm = 0.0, tolx = 0.0;
+  int converged = 0, nround = 0, niter = 1, rc = 0;
+  if (!c.is_linear) {
+    damp = c.damp_initial;
+    launch_norm1(s.ndof, s.d_U.p, s.d_partials.p, s.d_ticket.p, s.d_red.p + 1, st);
+    rnorm = read_norms(s).b;
+  }
+  while (niter <= c.maxiters) {
+    AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_F.p, s.d_J.p, tstep, lambda);
+    launch_assemble(a, s.asm_group, st);
+    const bool refactor = !(c.reuse_precon && niter > 1);
+    s.lin.prepare(s.d_J.p, c, refactor);
+    vfvm_linstats_t ls{};
+    int lrc = s.lin.solve(s.d_J.p, s.d_F.p, s.d_dU.p, c, &ls);
+    (void)lrc;  // like the reference, an unconverged Krylov solve still yields an update
+    h.nlin += ls.iters;
+    launch_newton_update(s.ndof, s.d_U.p, s.d_dU.p, damp, s.d_partials.p, s.d_ticket.p, s.d_red.p, st);
+    Readback r = read_norms(s);
+    if (r.flags & 1) {
+      rc = VFVM_ERR_ASSEMBLY;
+      s.ctx->err = "trying to assemble NaN";
+      break;
+    }
+    h.niter = niter;
+    if (c.is_linear) {
+      converged = 1;
+      break;
+    }
+    damp = std::min(damp * c.damp_growth, 1.0);
+    norm = r.a;
+    if (!std::isfinite(norm)) {
+      converged = 0;
+      break;
+    }
+    if (tolx == 0.0) tolx = norm * c.reltol;
+    double dnorm = 1.0, rnorm_new = r.b;
+    if (rnorm > 1.0e-50) dnorm = std::fabs((rnorm - rnorm_new) / rnorm);
+    if (dnorm < c.tol_round)
+      nround++;
+    else
+      nround = 0;
+    if (niter - 1 < 64) h.updatenorm[niter - 1] = norm;
+    h.last_update_norm = norm;
+    if (niter > 1 && norm / oldnorm > 1.0 / c.tol_mono) {
+      converged = 0;
+      break;
+    }
+    if (norm < c.abstol || norm < tolx) {
+      converged = 1;
+      break;
+    }
+    oldnorm = norm;
+    rnorm = rnorm_new;
+    if (nround > c.max_round) {
+      converged = 1;
+      break;
+    }
+    niter++;
+  }
+  // ||F||_inf over non-penalty rows at the returned solution (residual-only assembly)
+  if (rc == 0) {
+    AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_tmp.p, nullptr, tstep, lambda);
+    launch_assemble(a, s.asm_group, st);
+    h.residual_norm = penalty_free_norm(s, s.d_tmp.p);
+  }
+  h.converged = converged;
+  h.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (hist) *hist = h;
+  if (rc) return rc;
+  if (!converged) {
+    s.ctx->err = "Newton did not converge (ConvergenceError) after " + std::to_string(niter) + " iterations";
+    return VFVM_ERR_CONVERGENCE;
+  }
+  return VFVM_OK;
+}
+
+void autotune_impl(vfvm_sys &s, int32_t *ag, int32_t *sg) {
+  hipStream_t st = s.ctx->stream;
+  hipEvent_t e0, e1;
+  VFVM_HIP_CHECK(hipEventCreate(&e0));
+  VFVM_HIP_CHECK(hipEventCreate(&e1));
+  // assembly group: time Jacobian assembly at the current U for each legal group
+  float best = 1e30f;
+  int bestg = s.asm_group;
+  for (int g : {4, 8, 16, 32}) {
+    AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_F.p, s.d_J.p, 1.0, 0.0);
+    launch_assemble(a, g, st);
+    VFVM_HIP_CHECK(hipEventRecord(e0, st));
+    for (int r = 0; r < 5; r++) launch_assemble(a, g, st);
+    VFVM_HIP_CHECK(hipEventRecord(e1, st));
+    VFVM_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      bestg = g;
+    }
+  }
+  s.asm_group = bestg;
+  // SpMV group
+  LinSetup L;
+  L.ndof = s.ndof;
+  L.nnodes = s.nnodes;
+  L.nnz = s.nnz;
+  L.nspec = s.nspec;
+  L.rowptr = s.d_rowptr.p;
+  L.colind = s.d_colind.p;
+  best = 1e30f;
+  bestg = s.spmv_group;
+  for (int g : {2, 4, 8, 16, 32}) {
+    time_spmv(L, g, s.d_J.p, s.d_U.p, s.d_tmp.p, st);
+    VFVM_HIP_CHECK(hipEventRecord(e0, st));
+    for (int r = 0; r < 5; r++) time_spmv(L, g, s.d_J.p, s.d_U.p, s.d_tmp.p, st);
+    VFVM_HIP_CHECK(hipEventRecord(e1, st));
+    VFVM_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      bestg = g;
+    }
+  }
+  s.spmv_group = bestg;
+  s.lin.set_group(bestg);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ag) *ag = s.asm_group;
+  if (sg) *sg = s.spmv_group;
+}
+
+}  // namespace
+
+// =============================================================================== C ABI
+extern "C" {
+
+void vfvm_default_control(vfvm_control_t *c) {
+  std::memset(c, 0, sizeof(*c));
+  // src/vfvm_solvercontrol.jl:36-131
+  c->abstol = 1.0e-10;
+  c->reltol = 1.0e-10;
+  c->tol_round = 1.0e-10;
+  c->tol_mono = 1.0e-3;
+  c->damp_initial = 1.0;
+  c->damp_growth = 1.2;
+  c->maxiters = 100;
+  c->max_round = 1000;
+  c->updatecontrol = 1;
+  c->is_linear = 0;
+  c->reltol_linear = 1.0e-4;
+  c->abstol_linear = 1.0e-8;
+  c->maxiters_linear = 100;
+  c->krylov = VFVM_KRYLOV_BICGSTAB;
+  c->precon = VFVM_PRECON_JACOBI;
+  c->precision = VFVM_PREC_FP64;
+  c->gmres_restart = 30;
+  c->max_refinements = 20;
+  c->reuse_precon = 0;
+  c->check_every = 4;
+  c->inner_reltol = 1.0e-3;
+}
+
+const char *vfvm_last_error(vfvm_ctx *ctx) {
+  if (ctx) return ctx->err.c_str();
+  std::lock_guard<std::mutex> lk(g_err_mutex);
+  return g_last_create_error.c_str();
+}
+
+int vfvm_ctx_create(int device, vfvm_ctx **out) {
+  return guarded(nullptr, [&]() {
+    if (!out) throw Error(VFVM_ERR_ARG, "out is null");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(VFVM_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= n) throw Error(VFVM_ERR_ARG, "device index out of range");
+    VFVM_HIP_CHECK(hipSetDevice(device));
+    auto *c = new vfvm_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      throw Error(VFVM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return VFVM_OK;
+  });
+}
+
+int vfvm_ctx_destroy(vfvm_ctx *ctx) {
+  if (!ctx) return VFVM_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return VFVM_OK;
+}
+
+int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coord, int64_t ncells,
+                       const int64_t *cellnodes, const int64_t *cellregions, int64_t nbfaces, const int64_t *bfacenodes,
+                       const int64_t *bfaceregions, int nspec, const vfvm_physics_t *physics, int nbregions,
+                       const vfvm_bc_t *bc, vfvm_sys **out) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !out || !coord || !cellnodes || !physics) throw Error(VFVM_ERR_ARG, "null argument");
+    if (nspec < 1 || nspec > MAXSPEC) throw Error(VFVM_ERR_ARG, "nspec must be 1..3");
+    if (nbregions < 0 || nbregions > MAXBREG) throw Error(VFVM_ERR_ARG, "at most 8 boundary regions");
+    if (physics->flux == VFVM_FLUX_DD && nspec != 3) throw Error(VFVM_ERR_ARG, "DD physics needs 3 species");
+    if (physics->reaction == VFVM_REACT_DD && nspec != 3) throw Error(VFVM_ERR_ARG, "DD physics needs 3 species");
+    if (physics->reaction == VFVM_REACT_EXCHANGE && nspec < 2) throw Error(VFVM_ERR_ARG, "EXCHANGE needs 2 species");
+    (void)cellregions;
+    VFVM_HIP_CHECK(hipSetDevice(ctx->device));
+    std::unique_ptr<vfvm_sys> s(new vfvm_sys());
+    s->ctx = ctx;
+    s->dim = dim;
+    s->nspec = nspec;
+    s->nbreg = nbregions;
+    s->nnodes = nnodes;
+    s->ndof = nnodes * nspec;
+    s->phys = *physics;
+    s->bc.assign((size_t)std::max(1, nbregions) * nspec, vfvm_bc_t{});
+    if (bc && nbregions > 0) std::copy(bc, bc + (size_t)nbregions * nspec, s->bc.begin());
+    build_setup(dim, nnodes, coord, ncells, cellnodes, nbfaces, bfacenodes, bfaceregions, s->hs);
+    for (int64_t b = 0; b < (int64_t)s->hs.bn_reg.size(); b++)
+      if (s->hs.bn_reg[b] > MAXBREG) throw Error(VFVM_ERR_ARG, "boundary region number above 8");
+    s->masks = make_masks(nspec, s->phys, s->bc.data(), nbregions);
+    build_dof_csr(s->hs, nspec, s->masks, s->rowptr, s->colind);
+    s->nnz = s->rowptr[s->ndof];
+    verify_slot_map(s->hs, nspec, s->masks, s->rowptr, s->colind);
+    build_csc_perm(s->ndof, s->rowptr, s->colind, s->csc_colptr, s->csc_rowval, s->csc_perm);
+    s->dofdiag.resize((size_t)s->ndof);
+    for (int64_t r = 0; r < s->ndof; r++) {
+      int32_t d = -1;
+      for (int32_t k = s->rowptr[r]; k < s->rowptr[r + 1]; k++)
+        if (s->colind[k] == r) d = k;
+      if (d < 0) throw Error(VFVM_ERR_ARG, "pattern lacks a diagonal entry");
+      s->dofdiag[r] = d;
+    }
+    hipStream_t st = ctx->stream;
+    const HostSetup &h = s->hs;
+    s->d_adj_ptr.upload(h.adj_ptr, st);
+    s->d_adj_col.upload(h.adj_col, st);
+    s->d_adj_diag.upload(h.adj_diag, st);
+    s->d_adj_fac.upload(h.adj_fac, st);
+    s->d_nodevol.upload(h.nodevol, st);
+    s->d_bn_ptr.upload(h.bn_ptr, st);
+    s->d_bn_reg.upload(h.bn_reg.empty() ? std::vector<int32_t>{0} : h.bn_reg, st);
+    s->d_bn_fac.upload(h.bn_fac.empty() ? std::vector<double>{0.0} : h.bn_fac, st);
+    s->d_bnodes.upload(h.bnodes.empty() ? std::vector<int32_t>{0} : h.bnodes, st);
+    s->d_rowptr.upload(s->rowptr, st);
+    s->d_colind.upload(s->colind, st);
+    s->d_csc_perm.upload(s->csc_perm, st);
+    s->d_dofdiag.upload(s->dofdiag, st);
+    s->d_U.alloc((size_t)s->ndof);
+    s->d_Uold.alloc((size_t)s->ndof);
+    s->d_F.alloc((size_t)s->ndof);
+    s->d_dU.alloc((size_t)s->ndof);
+    s->d_tmp.alloc((size_t)std::max<int64_t>(s->ndof, 1));
+    s->d_J.alloc((size_t)s->nnz);
+    s->d_flags.alloc(4);
+    s->d_red.alloc(8);
+    s->d_partials.alloc((size_t)4 * MAX_RED_BLOCKS);
+    s->d_ticket.alloc(4);
+    VFVM_HIP_CHECK(hipMemsetAsync(s->d_ticket.p, 0, 4 * sizeof(unsigned), st));
+    VFVM_HIP_CHECK(hipMemsetAsync(s->d_U.p, 0, s->ndof * sizeof(double), st));
+    VFVM_HIP_CHECK(hipMemsetAsync(s->d_Uold.p, 0, s->ndof * sizeof(double), st));
+    VFVM_HIP_CHECK(hipHostMalloc((void **)&s->h_pinned, 8 * sizeof(double), hipHostMallocDefault));
+    upload_inst(*s);
+    // default group sizes from the longest row; vfvm_autotune refines them on the device
+    int maxdeg = 1, maxrow = 1;
+    for (int64_t K = 0; K < nnodes; K++) maxdeg = std::max(maxdeg, h.adj_ptr[K + 1] - h.adj_ptr[K]);
+    for (int64_t r = 0; r < s->ndof; r++) maxrow = std::max(maxrow, s->rowptr[r + 1] - s->rowptr[r]);
+    s->asm_group = next_pow2_clamp(maxdeg, 4, 32);
+    s->spmv_group = next_pow2_clamp(maxrow, 2, 32);
+    vfvm_control_t c;
+    vfvm_default_control(&c);
+    setup_linear(*s, c);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    *out = s.release();
+    return VFVM_OK;
+  });
+}
+
+int vfvm_system_destroy(vfvm_sys *s) {
+  if (!s) return VFVM_OK;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->h_pinned) (void)hipHostFree(s->h_pinned);
+  delete s;
+  return VFVM_OK;
+}
+
+int vfvm_system_set_source(vfvm_sys *s, const double *src) {
+  if (!s) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    if (src) {
+      s->d_src.upload(src, (size_t)s->ndof, s->ctx->stream);
+      s->has_src = true;
+    } else {
+      s->has_src = false;
+    }
+    VFVM_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    return VFVM_OK;
+  });
+}
+
+int vfvm_system_set_bc(vfvm_sys *s, const vfvm_bc_t *bc) {
+  if (!s || !bc) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    std::copy(bc, bc + (size_t)s->nbreg * s->nspec, s->bc.begin());
+    upload_inst(*s);
+    return VFVM_OK;
+  });
+}
+
+int vfvm_system_sizes(vfvm_sys *s, int64_t *ndof, int64_t *nnz, int64_t *nedges) {
+  if (!s) return VFVM_ERR_ARG;
+  if (ndof) *ndof = s->ndof;
+  if (nnz) *nnz = s->nnz;
+  if (nedges) *nedges = s->hs.nedges;
+  return VFVM_OK;
+}
+
+int vfvm_nodevolumes(vfvm_sys *s, double *vol) {
+  if (!s || !vol) return VFVM_ERR_ARG;
+  std::copy(s->hs.nodevol.begin(), s->hs.nodevol.end(), vol);
+  return VFVM_OK;
+}
+
+int vfvm_pattern(vfvm_sys *s, int64_t *colptr, int64_t *rowval) {
+  if (!s) return VFVM_ERR_ARG;
+  if (colptr)
+    for (int64_t j = 0; j <= s->ndof; j++) colptr[j] = s->csc_colptr[j] + 1;
+  if (rowval)
+    for (int64_t k = 0; k < s->nnz; k++) rowval[k] = (int64_t)s->csc_rowval[k] + 1;
+  return VFVM_OK;
+}
+
+int vfvm_pattern_csr(vfvm_sys *s, int32_t *rowptr, int32_t *colind) {
+  if (!s) return VFVM_ERR_ARG;
+  if (rowptr) std::copy(s->rowptr.begin(), s->rowptr.end(), rowptr);
+  if (colind) std::copy(s->colind.begin(), s->colind.end(), colind);
+  return VFVM_OK;
+}
+
+int vfvm_initialize_dirichlet(vfvm_sys *s, double *U, double time, double lambda) {
+  if (!s || !U) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    (void)time;
+    hipStream_t st = s->ctx->stream;
+    s->d_U.upload(U, (size_t)s->ndof, st);
+    launch_init_dirichlet((int64_t)s->hs.bnodes.size(), s->d_bnodes.p, s->d_bn_ptr.p, s->d_bn_reg.p, nullptr,
+                          s->d_inst.p, s->nspec, lambda, s->d_U.p, st);
+    s->d_U.download(U, (size_t)s->ndof, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return VFVM_OK;
+  });
+}
+
+int vfvm_assemble_device(vfvm_sys *s, const double *dU, const double *dUold, double time, double tstep,
+                         double lambda, double *dF, double *dnzval_csr) {
+  if (!s) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    (void)time;
+    hipStream_t st = s->ctx->stream;
+    launch_fill_i32(1, s->d_flags.p, 0, st);
+    AsmArgs a = asm_args(*s, dU, dUold, dF, dnzval_csr, tstep, lambda);
+    launch_assemble(a, s->asm_group, st);
+    int32_t fl = 0;
+    VFVM_HIP_CHECK(hipMemcpyAsync(&fl, s->d_flags.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    if (fl & 1) throw Error(VFVM_ERR_ASSEMBLY, "trying to assemble NaN");
+    return VFVM_OK;
+  });
+}
+
+int vfvm_assemble(vfvm_sys *s, const double *U, const double *Uold, double time, double tstep, double lambda, double *F,
+                  double *nzval) {
+  if (!s || !U || !F) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    hipStream_t st = s->ctx->stream;
+    s->d_U.upload(U, (size_t)s->ndof, st);
+    s->d_Uold.upload(Uold ? Uold : U, (size_t)s->ndof, st);
+    int rc = vfvm_assemble_device(s, s->d_U.p, s->d_Uold.p, time, tstep, lambda, s->d_F.p, s->d_J.p);
+    if (rc) return rc;
+    s->d_F.download(F, (size_t)s->ndof, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    if (nzval) {
+      DevBuf<double> tmp;
+      tmp.alloc((size_t)s->nnz);
+      launch_gather(s->nnz, s->d_csc_perm.p, s->d_J.p, tmp.p, st);
+      tmp.download(nzval, (size_t)s->nnz, st);
+      VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    return VFVM_OK;
+  });
+}
+
+int vfvm_linsolve(vfvm_sys *s, const double *b, double *x, const vfvm_control_t *control, vfvm_linstats_t *stats) {
+  if (!s || !b || !x) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    vfvm_control_t c;
+    if (control)
+      c = *control;
+    else
+      vfvm_default_control(&c);
+    check_control(c, *s);
+    if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
+    hipStream_t st = s->ctx->stream;
+    s->d_F.upload(b, (size_t)s->ndof, st);
+    s->lin.prepare(s->d_J.p, c, true);
+    int rc = s->lin.solve(s->d_J.p, s->d_F.p, s->d_dU.p, c, stats);
+    s->d_dU.download(x, (size_t)s->ndof, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    if (rc) s->ctx->err = "linear solver did not reach its tolerance";
+    return rc;
+  });
+}
+
+int vfvm_newton(vfvm_sys *s, double *U, const double *Uold, double time, double tstep, double lambda,
+                const vfvm_control_t *control, vfvm_history_t *hist) {
+  if (!s || !U || !Uold) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    vfvm_control_t c;
+    if (control)
+      c = *control;
+    else
+      vfvm_default_control(&c);
+    check_control(c, *s);
+    if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
+    hipStream_t st = s->ctx->stream;
+    s->d_Uold.upload(Uold, (size_t)s->ndof, st);
+    int rc = newton_device(*s, time, tstep, lambda, c, hist);
+    s->d_U.download(U, (size_t)s->ndof, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return rc;
+  });
+}
+
+int vfvm_solve_fixed_steps(vfvm_sys *s, const double *inival, int64_t nl, const double *lambdas, int transient,
+                           const vfvm_control_t *control, double *U_final, int64_t *newton_total) {
+  if (!s || !inival || !lambdas || nl < 1) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    vfvm_control_t c;
+    if (control)
+      c = *control;
+    else
+      vfvm_default_control(&c);
+    check_control(c, *s);
+    if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
+    hipStream_t st = s->ctx->stream;
+    s->d_Uold.upload(inival, (size_t)s->ndof, st);
+    int64_t total = 0;
+    int rc = VFVM_OK;
+    vfvm_history_t h{};
+    if (!transient) {
+      rc = newton_device(*s, 0.0, INFINITY, lambdas[0], c, &h);
+      total += h.niter;
+      if (!rc) launch_copy(s->ndof, s->d_U.p, s->d_Uold.p, st);
+    } else {
+      launch_init_dirichlet((int64_t)s->hs.bnodes.size(), s->d_bnodes.p, s->d_bn_ptr.p, s->d_bn_reg.p, nullptr,
+                            s->d_inst.p, s->nspec, lambdas[0], s->d_Uold.p, st);
+    }
+    for (int64_t k = 1; k < nl && rc == VFVM_OK; k++) {
+      const double dl = lambdas[k] - lambdas[k - 1];
+      if (transient)
+        rc = newton_device(*s, lambdas[k], dl, 0.0, c, &h);
+      else
+        rc = newton_device(*s, 0.0, INFINITY, lambdas[k], c, &h);
+      total += h.niter;
+      if (!rc) launch_copy(s->ndof, s->d_U.p, s->d_Uold.p, st);
+    }
+    if (U_final) s->d_Uold.download(U_final, (size_t)s->ndof, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    if (newton_total) *newton_total = total;
+    return rc;
+  });
+}
+
+int vfvm_residual_norm(vfvm_sys *s, const double *U, const double *Uold, double time, double tstep, double lambda,
+                       double *norm) {
+  if (!s || !U || !norm) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    (void)time;
+    hipStream_t st = s->ctx->stream;
+    s->d_U.upload(U, (size_t)s->ndof, st);
+    s->d_Uold.upload(Uold ? Uold : U, (size_t)s->ndof, st);
+    AsmArgs a = asm_args(*s, s->d_U.p, s->d_Uold.p, s->d_tmp.p, nullptr, tstep, lambda);
+    launch_assemble(a, s->asm_group, st);
+    *norm = penalty_free_norm(*s, s->d_tmp.p);
+    return VFVM_OK;
+  });
+}
+
+int vfvm_autotune(vfvm_sys *s, int32_t *asm_group, int32_t *spmv_group) {
+  if (!s) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    autotune_impl(*s, asm_group, spmv_group);
+    return VFVM_OK;
+  });
+}
+
+}  // extern "C"

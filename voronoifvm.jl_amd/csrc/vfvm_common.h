@@ -1,0 +1,156 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/vfvm_hip.h"
+
+namespace vfvm {
+
+constexpr int MAXSPEC = 3;
+constexpr int MAXBREG = 8;
+constexpr double DIRICHLET_PENALTY = 1.0e30;  // src/vfvm_system.jl:329
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define VFVM_HIP_CHECK(expr)                                                                    \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      throw ::vfvm::Error(VFVM_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));      \
+  } while (0)
+
+// Device buffer owned by a handle (freed on destroy).
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    VFVM_HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  // synchronous: the host buffer may be a temporary of the caller
+  void upload(const T *h, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) {
+      VFVM_HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+      VFVM_HIP_CHECK(hipStreamSynchronize(s));
+    }
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  void download(T *h, size_t count, hipStream_t s) const {
+    if (count) VFVM_HIP_CHECK(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+};
+
+// Per-instance physics parameters and BC table as the kernels read them.
+struct BCDev {
+  int32_t kind, pad;
+  double value, value_lambda, factor;
+};
+struct InstParams {
+  int32_t flux, reaction, storage, has_legacy;
+  double p[16];
+  BCDev bc[MAXBREG][MAXSPEC];
+};
+
+// Structural sparsity of the registry callbacks (which u_j a term depends on).
+struct Masks {
+  uint8_t emask[MAXSPEC];  // flux g_i depends on u_j at both ends
+  uint8_t dmask[MAXSPEC];  // diagonal node block: flux (K side) | reaction | storage | boundary | i
+  int ecnt[MAXSPEC], dcnt[MAXSPEC], eoff[MAXSPEC], doff[MAXSPEC];
+  int etot, dtot;
+};
+
+// Host-side discretisation data produced once per grid (vfvm_setup.cpp).
+struct HostSetup {
+  int dim = 0;
+  int64_t nnodes = 0;
+  std::vector<int32_t> adj_ptr, adj_col, adj_diag;  // node graph incl. self, sorted per row
+  std::vector<double> adj_fac;                      // aggregated sigma/h per entry (0 on self)
+  std::vector<double> nodevol;                      // sum of node factors
+  std::vector<int32_t> bn_ptr, bn_reg;              // boundary node items in bface order
+  std::vector<double> bn_fac;
+  std::vector<int32_t> bnodes;                      // nodes having boundary items
+  int64_t nedges = 0;
+};
+
+void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, const int64_t *cellnodes,
+                 int64_t nbfaces, const int64_t *bfacenodes, const int64_t *bfaceregions, HostSetup &out);
+Masks make_masks(int nspec, const vfvm_physics_t &ph, const vfvm_bc_t *bc, int nbreg);
+void build_dof_csr(const HostSetup &hs, int nspec, const Masks &m, std::vector<int32_t> &rowptr,
+                   std::vector<int32_t> &colind);
+void verify_slot_map(const HostSetup &hs, int nspec, const Masks &m, const std::vector<int32_t> &rowptr,
+                     const std::vector<int32_t> &colind);
+void build_csc_perm(int64_t ndof, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
+                    std::vector<int64_t> &colptr, std::vector<int32_t> &rowval, std::vector<int32_t> &perm);
+int greedy_colors(const HostSetup &hs, std::vector<int32_t> &color);
+
+// ---- kernel launch interfaces (vfvm_assemble.hip, vfvm_linalg.hip) ----
+struct AsmArgs {
+  int64_t nnodes;
+  const int32_t *adj_ptr, *adj_col, *adj_diag;
+  const double *adj_fac, *nodevol;
+  const int32_t *bn_ptr, *bn_reg;
+  const double *bn_fac;
+  const int32_t *node_inst;      // per-node instance id (batched), nullptr: instance 0
+  const InstParams *inst;        // device array of instance parameters
+  const double *U, *Uold, *src;  // src may be nullptr
+  double *F, *J;                 // J may be nullptr (residual only)
+  double tstepinv, lambda;
+  int32_t *flags;                // [0] NaN seen
+  Masks m;
+  int nspec;
+};
+void launch_assemble(const AsmArgs &a, int group, hipStream_t s);
+// U[K,i] = Dirichlet value for boundary nodes (last item in bface order wins)
+void launch_init_dirichlet(int64_t nbnodes, const int32_t *bnodes, const int32_t *bn_ptr, const int32_t *bn_reg,
+                           const int32_t *node_inst, const InstParams *inst, int nspec, double lambda, double *U,
+                           hipStream_t s);
+// ||F||_inf over rows without a Dirichlet penalty, per instance (inst may be null -> 1 instance)
+void launch_penalty_free_norm(int64_t nnodes, int nspec, const double *F, const int32_t *bn_ptr,
+                              const int32_t *bn_reg, const int32_t *node_inst, const InstParams *inst,
+                              double *out_partial, int64_t nparts, hipStream_t s);
+
+}  // namespace vfvm
+
+struct vfvm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};

@@ -1,0 +1,163 @@
+// Device linear solver state and host orchestration (see vfvm_linalg.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <type_traits>
+
+#include "vfvm_common.h"
+
+namespace vfvm {
+
+constexpr int MAX_RED_BLOCKS = 2048;
+
+// Krylov scalars kept in device memory; updated by the last block of each
+// reducing kernel.  status: 0 running, 1 converged, 2 breakdown, 3 max iterations.
+struct KState {
+  double rho, rho_old, alpha, omega, beta;
+  double rr, rr0, ss, tol2, rtol, atol;
+  double dots[4];
+  double gm_inv, gm_beta;
+  int32_t iters, maxit, done, status, half, gm_k, gm_conv, gm_stop;
+  unsigned ticket;
+  unsigned pad;
+};
+
+enum { STAGE_NONE = 0, STAGE_INIT, STAGE_ALPHA, STAGE_SNORM, STAGE_OMEGA, STAGE_XR };
+
+template <int STAGE>
+__device__ __forceinline__ void run_stage(KState *st, const double *tot) {
+  if constexpr (STAGE == STAGE_INIT) {
+    st->rr = tot[0];
+    st->rr0 = tot[0];
+    const double tol = st->atol + st->rtol * sqrt(tot[0]);
+    st->tol2 = tol * tol;
+    st->rho = tot[0];
+    st->rho_old = 1.0;
+    st->alpha = 1.0;
+    st->omega = 1.0;
+    st->beta = 0.0;
+    st->iters = 0;
+    st->half = 0;
+    if (tot[0] <= st->tol2) {
+      st->done = 1;
+      st->status = 1;
+    }
+  } else if constexpr (STAGE == STAGE_ALPHA) {
+    const double d = tot[0];
+    if (d == 0.0 || !(fabs(d) < INFINITY)) {
+      st->done = 1;
+      st->status = 2;
+    } else {
+      st->alpha = st->rho / d;
+    }
+  } else if constexpr (STAGE == STAGE_SNORM) {
+    st->ss = tot[0];
+    if (tot[0] <= st->tol2) st->half = 1;
+  } else if constexpr (STAGE == STAGE_OMEGA) {
+    st->omega = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
+  } else if constexpr (STAGE == STAGE_XR) {
+    st->iters += 1;
+    if (st->half) {
+      st->rr = st->ss;
+      st->done = 1;
+      st->status = 1;
+      return;
+    }
+    const double rho_new = tot[0];
+    st->rr = tot[1];
+    if (tot[1] <= st->tol2) {
+      st->done = 1;
+      st->status = 1;
+    } else if (!(tot[1] < INFINITY)) {
+      st->done = 1;
+      st->status = 2;
+    } else if (st->iters >= st->maxit) {
+      st->done = 1;
+      st->status = 3;
+    } else if (rho_new == 0.0 || st->omega == 0.0) {
+      st->done = 1;
+      st->status = 2;
+    } else {
+      st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
+      st->rho_old = st->rho;
+      st->rho = rho_new;
+    }
+  }
+}
+
+// multicolour ILU(0) structure (permuted CSR); cptr is a host array
+struct IluStruct {
+  int ncolors = 0;
+  std::vector<int32_t> cptr;
+  const int32_t *rowptr = nullptr, *colind = nullptr, *diag = nullptr, *perm = nullptr;
+  const int32_t *src = nullptr, *srcrow = nullptr;
+};
+
+struct LinSetup {
+  int64_t ndof = 0, nnodes = 0, nnz = 0;
+  int nspec = 1;
+  const int32_t *rowptr = nullptr, *colind = nullptr, *dofdiag = nullptr;
+  const IluStruct *ilu = nullptr;
+  int spmv_group = 8;
+};
+
+class LinearSolver {
+ public:
+  void setup(const LinSetup &ls, hipStream_t s);
+  void prepare(const double *J, const vfvm_control_t &c, bool refactor);
+  int solve(const double *J, const double *b, double *x, const vfvm_control_t &c, vfvm_linstats_t *stats);
+  void set_group(int g) { L.spmv_group = g; }
+  const double *row_scale() const { return rowscale.p; }
+
+  template <class V>
+  struct Work {
+    std::vector<DevBuf<V>> vecs;
+    int64_t n = 0;
+    void alloc(int64_t n, int nvec);
+  };
+
+ private:
+  template <class V>
+  Work<V> &work() {
+    if constexpr (sizeof(V) == 4) return w32;
+    else return w64;
+  }
+  void ensure_work(int precision, int krylov, int m);
+  template <class V>
+  const V *precond(const V *r, V *z, KState *stp, int skip_if_half, const vfvm_control_t &c);
+  template <class V>
+  V *iluy_ptr();
+  template <class V, int NDOT, int STAGE>
+  void apply_op(const V *x, V *y, const V *w0, const V *w1, KState *stp, const double *J);
+  template <class V, class VBin, class VXo>
+  int bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_control_t &c, double rtol, double atol, int maxit,
+               int &iters);
+  template <class V, class VBin, class VXo>
+  int gmres(const VBin *b, VXo *x, const double *J, const vfvm_control_t &c, double rtol, double atol, int maxit,
+            int &iters);
+  void read_state(KState &h);
+
+  LinSetup L;
+  hipStream_t stream = nullptr;
+  DevBuf<KState> st;
+  DevBuf<double> partials, rowscale, binv, bhat, rres, ilu64, iluy64, vbasis64;
+  DevBuf<float> A32, ilu32, iluy32, vbasis32, rhs32, d32;
+  DevBuf<double> gmH, gmh1, gmh2, gmcs, gmsn, gmg, gmy, gmpart;
+  Work<float> w32;
+  Work<double> w64;
+  bool ilu_ready = false;
+  double last_rr = 0, last_rr0 = 0;
+};
+
+void launch_newton_update(int64_t n, double *U, const double *dU, double damp, double *partials, unsigned *ticket,
+                          double *out, hipStream_t s);
+void launch_norm1(int64_t n, const double *U, double *partials, unsigned *ticket, double *out, hipStream_t s);
+void launch_gather(int64_t n, const int32_t *perm, const double *J, double *out, hipStream_t s);
+void launch_fill_u64(int64_t n, unsigned long long *p, hipStream_t s);
+void launch_fill_i32(int64_t n, int32_t *p, int32_t v, hipStream_t s);
+void launch_copy(int64_t n, const double *a, double *b, hipStream_t s);
+void time_spmv(const LinSetup &L, int G, const double *A, const double *x, double *y, hipStream_t s);
+
+}  // namespace vfvm

@@ -1,0 +1,432 @@
+// Host setup: once per grid.  Voronoi form factors, edgewise aggregation into a
+// node graph, boundary node items, dof-level CSR pattern, CSC export map and a
+// greedy colouring for the multicolour ILU(0).
+//
+// Restates (paths relative to the reference root):
+//   cellfactors!/bfacefactors!   src/vfvm_formfactors.jl:12-23, 64-100, 163-235, 244-247, 263-274, 293-332
+//   update_grid_edgewise!         src/vfvm_system.jl:690-755 (factor sums per edge / node in cell order;
+//                                 ExtendableSparse `+=` creates an entry only for a nonzero value)
+//   boundary assembly data        src/vfvm_system.jl:667-677, src/vfvm_assemblydata.jl:140-152
+//   dof numbering                 src/vfvm_system.jl:808-811 (node-major, species interleaved)
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "vfvm_common.h"
+
+namespace vfvm {
+
+namespace {
+
+// ExtendableGrids local element tables: triangle edge i is opposite node i;
+// tetrahedron edges in the order 12, 13, 14, 23, 24, 34.
+const int TRI_EN[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+const int TET_EN[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+
+inline int edges_per_cell(int dim) { return dim == 1 ? 1 : (dim == 2 ? 3 : 6); }
+
+inline void local_edge(int dim, int e, int &a, int &b) {
+  if (dim == 1) {
+    a = 0;
+    b = 1;
+  } else if (dim == 2) {
+    a = TRI_EN[e][0];
+    b = TRI_EN[e][1];
+  } else {
+    a = TET_EN[e][0];
+    b = TET_EN[e][1];
+  }
+}
+
+// coord is dim x nnodes column-major; n holds 0-based node numbers of the cell
+void cell_factors(int dim, const double *coord, const int64_t *n, double *npar, double *epar) {
+  if (dim == 1) {
+    double d = std::fabs(coord[n[1]] - coord[n[0]]);
+    npar[0] = d / 2;
+    npar[1] = d / 2;
+    epar[0] = 1 / d;
+    return;
+  }
+  if (dim == 2) {
+    double V0[3], V1[3], dd[3];
+    for (int e = 0; e < 3; e++) {
+      V0[e] = coord[2 * n[TRI_EN[e][0]]] - coord[2 * n[TRI_EN[e][1]]];
+      V1[e] = coord[2 * n[TRI_EN[e][0]] + 1] - coord[2 * n[TRI_EN[e][1]] + 1];
+    }
+    double det = V0[2] * V1[1] - V0[1] * V1[2];
+    double vol = std::fabs(0.5 * det);
+    double ivol = 1.0 / vol;
+    for (int e = 0; e < 3; e++) dd[e] = V0[e] * V0[e] + V1[e] * V1[e];
+    epar[0] = (dd[1] + dd[2] - dd[0]) * 0.125 * ivol;
+    epar[1] = (dd[2] + dd[0] - dd[1]) * 0.125 * ivol;
+    epar[2] = (dd[0] + dd[1] - dd[2]) * 0.125 * ivol;
+    npar[0] = npar[1] = npar[2] = 0.0;
+    for (int i = 0; i < 3; i++) {
+      npar[TRI_EN[i][0]] += epar[i] * dd[i] * 0.25;
+      npar[TRI_EN[i][1]] += epar[i] * dd[i] * 0.25;
+    }
+    return;
+  }
+  // pdelib tetrahedron formula; face tables of src/vfvm_formfactors.jl:170-176 (1-based edges)
+  static const int pi1[4] = {5, 6, 5, 1}, pi2[4] = {6, 3, 1, 4}, pi3[4] = {4, 2, 3, 2};
+  static const int po1[4] = {2, 1, 2, 6}, po2[4] = {1, 4, 6, 3}, po3[4] = {3, 5, 4, 5};
+  double dd[6];
+  for (int i = 0; i < 6; i++) {
+    const double *p1 = coord + 3 * n[TET_EN[i][0]], *p2 = coord + 3 * n[TET_EN[i][1]];
+    double dx = p1[0] - p2[0], dy = p1[1] - p2[1], dz = p1[2] - p2[2];
+    dd[i] = dx * dx + dy * dy + dz * dz;
+  }
+  for (int i = 0; i < 6; i++) epar[i] = 0.0;
+  const double *c0 = coord + 3 * n[0], *c1 = coord + 3 * n[1], *c2 = coord + 3 * n[2], *c3 = coord + 3 * n[3];
+  double x1 = c1[0] - c0[0], y1 = c1[1] - c0[1], z1 = c1[2] - c0[2];
+  double x2 = c2[0] - c0[0], y2 = c2[1] - c0[1], z2 = c2[2] - c0[2];
+  double x3 = c3[0] - c0[0], y3 = c3[1] - c0[1], z3 = c3[2] - c0[2];
+  double det = (x1 * (y2 * z3 - y3 * z2) + x2 * (y3 * z1 - y1 * z3) + x3 * (y1 * z2 - y2 * z1));
+  if (det < 0) det = -det;
+  double vol = det / 6;
+  double vv = 96 * 6 * vol;
+  for (int i = 0; i < 4; i++) {
+    npar[i] = 0.0;
+    int i1 = pi1[i] - 1, i2 = pi2[i] - 1, i3 = pi3[i] - 1;
+    double h1 = dd[i1] * (dd[i2] + dd[i3] - dd[i1]);
+    double h2 = dd[i2] * (dd[i3] + dd[i1] - dd[i2]);
+    double h3 = dd[i3] * (dd[i1] + dd[i2] - dd[i3]);
+    double df = h1 + h2 + h3;
+    double vf = (h1 * dd[po1[i] - 1] + h2 * dd[po2[i] - 1] + h3 * dd[po3[i] - 1] - 2 * dd[i1] * dd[i2] * dd[i3]) /
+                (vv * df);
+    epar[i1] += h1 * vf;
+    epar[i2] += h2 * vf;
+    epar[i3] += h3 * vf;
+  }
+  for (int i = 0; i < 6; i++) {
+    npar[TET_EN[i][0]] += epar[i];
+    npar[TET_EN[i][1]] += epar[i];
+    epar[i] = 6 * epar[i] / dd[i];
+  }
+}
+
+void bface_factors(int dim, const double *coord, const int64_t *n, double *npar) {
+  if (dim == 1) {
+    npar[0] = 1.0;
+    return;
+  }
+  if (dim == 2) {
+    double dx = coord[2 * n[0]] - coord[2 * n[1]];
+    double dy = coord[2 * n[0] + 1] - coord[2 * n[1] + 1];
+    double d = std::sqrt(dx * dx + dy * dy);
+    npar[0] = d / 2;
+    npar[1] = d / 2;
+    return;
+  }
+  double epar[3] = {0, 0, 0};
+  for (int j = 0; j < 3; j++)
+    for (int e = 0; e < 3; e++) {
+      double d = coord[3 * n[TRI_EN[e][0]] + j] - coord[3 * n[TRI_EN[e][1]] + j];
+      epar[e] += d * d;
+    }
+  double dd[3] = {epar[0], epar[1], epar[2]};
+  epar[0] = (dd[1] + dd[2] - dd[0]) * dd[0];
+  epar[1] = (dd[2] + dd[0] - dd[1]) * dd[1];
+  epar[2] = (dd[0] + dd[1] - dd[2]) * dd[2];
+  double vol = std::sqrt(epar[0] + epar[1] + epar[2]) * 0.25;
+  double d = 1.0 / (8 * vol);
+  npar[0] = npar[1] = npar[2] = 0.0;
+  for (int i = 0; i < 3; i++) {
+    npar[TRI_EN[i][0]] += epar[i] * d * 0.25;
+    npar[TRI_EN[i][1]] += epar[i] * d * 0.25;
+  }
+}
+
+}  // namespace
+
+void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, const int64_t *cellnodes1,
+                 int64_t nbfaces, const int64_t *bfacenodes1, const int64_t *bfaceregions, HostSetup &out) {
+  if (dim < 1 || dim > 3) throw Error(VFVM_ERR_ARG, "dim must be 1, 2 or 3");
+  if (nnodes >= (int64_t(1) << 31) - 1) throw Error(VFVM_ERR_ARG, "too many nodes for int32 indices");
+  const int nv = dim + 1, ne = edges_per_cell(dim);
+  out.dim = dim;
+  out.nnodes = nnodes;
+  // 1-based Julia numbering -> 0-based, with bounds checks
+  std::vector<int64_t> cn((size_t)nv * ncells);
+  for (size_t k = 0; k < cn.size(); k++) {
+    int64_t v = cellnodes1[k] - 1;
+    if (v < 0 || v >= nnodes) throw Error(VFVM_ERR_ARG, "cellnodes out of range");
+    cn[k] = v;
+  }
+  // edge contributions per row in cell order
+  std::vector<int64_t> cnt((size_t)nnodes + 1, 0);
+  for (int64_t c = 0; c < ncells; c++)
+    for (int e = 0; e < ne; e++) {
+      int a, b;
+      local_edge(dim, e, a, b);
+      cnt[cn[nv * c + a] + 1]++;
+      cnt[cn[nv * c + b] + 1]++;
+    }
+  for (int64_t i = 0; i < nnodes; i++) cnt[i + 1] += cnt[i];
+  std::vector<int32_t> tcol((size_t)cnt[nnodes]);
+  std::vector<double> tfac((size_t)cnt[nnodes]);
+  std::vector<int64_t> next(cnt.begin(), cnt.end() - 1);
+  out.nodevol.assign((size_t)nnodes, 0.0);
+  double npar[4], epar[6];
+  for (int64_t c = 0; c < ncells; c++) {
+    cell_factors(dim, coord, &cn[nv * c], npar, epar);
+    for (int i = 0; i < nv; i++) out.nodevol[cn[nv * c + i]] += npar[i];
+    for (int e = 0; e < ne; e++) {
+      int a, b;
+      local_edge(dim, e, a, b);
+      int64_t na = cn[nv * c + a], nb = cn[nv * c + b];
+      int64_t pa = next[na]++, pb = next[nb]++;
+      tcol[pa] = (int32_t)nb;
+      tfac[pa] = epar[e];
+      tcol[pb] = (int32_t)na;
+      tfac[pb] = epar[e];
+    }
+  }
+  cn.clear();
+  cn.shrink_to_fit();
+  // merge duplicates per row (cell order), keep entries that received a nonzero value
+  std::vector<int32_t> rlen((size_t)nnodes, 0);
+  std::vector<int32_t> ucol((size_t)cnt[nnodes]);
+  std::vector<double> ufac((size_t)cnt[nnodes]);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t i = 0; i < nnodes; i++) {
+    int64_t b0 = cnt[i], b1 = cnt[i + 1];
+    int32_t nu = 0;
+    int32_t *uc = &ucol[b0];
+    double *uf = &ufac[b0];
+    for (int64_t k = b0; k < b1; k++) {
+      int32_t col = tcol[k];
+      int32_t q = 0;
+      while (q < nu && uc[q] != col) q++;
+      if (q < nu) {
+        uf[q] = uf[q] + tfac[k];
+      } else if (tfac[k] != 0.0) {
+        uc[nu] = col;
+        uf[nu] = tfac[k];
+        nu++;
+      }
+    }
+    // insertion sort by column
+    for (int32_t a = 1; a < nu; a++) {
+      int32_t c0 = uc[a];
+      double f0 = uf[a];
+      int32_t q = a - 1;
+      while (q >= 0 && uc[q] > c0) {
+        uc[q + 1] = uc[q];
+        uf[q + 1] = uf[q];
+        q--;
+      }
+      uc[q + 1] = c0;
+      uf[q + 1] = f0;
+    }
+    rlen[i] = nu;
+  }
+  tcol.clear();
+  tcol.shrink_to_fit();
+  tfac.clear();
+  tfac.shrink_to_fit();
+  // node graph with self entry
+  out.adj_ptr.assign((size_t)nnodes + 1, 0);
+  for (int64_t i = 0; i < nnodes; i++) out.adj_ptr[i + 1] = out.adj_ptr[i] + rlen[i] + 1;
+  int64_t nadj = out.adj_ptr[nnodes];
+  if (nadj >= (int64_t(1) << 31) - 1) throw Error(VFVM_ERR_ARG, "node graph too large for int32 offsets");
+  out.adj_col.resize((size_t)nadj);
+  out.adj_fac.resize((size_t)nadj);
+  out.adj_diag.resize((size_t)nnodes);
+  int64_t nedge2 = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nedge2)
+  for (int64_t i = 0; i < nnodes; i++) {
+    int64_t p = out.adj_ptr[i];
+    const int32_t *uc = &ucol[cnt[i]];
+    const double *uf = &ufac[cnt[i]];
+    bool placed = false;
+    for (int32_t q = 0; q < rlen[i]; q++) {
+      if (!placed && uc[q] > i) {
+        out.adj_diag[i] = (int32_t)(p - out.adj_ptr[i]);
+        out.adj_col[p] = (int32_t)i;
+        out.adj_fac[p] = 0.0;
+        p++;
+        placed = true;
+      }
+      out.adj_col[p] = uc[q];
+      out.adj_fac[p] = uf[q];
+      p++;
+    }
+    if (!placed) {
+      out.adj_diag[i] = (int32_t)(p - out.adj_ptr[i]);
+      out.adj_col[p] = (int32_t)i;
+      out.adj_fac[p] = 0.0;
+    }
+    nedge2 += rlen[i];
+  }
+  out.nedges = nedge2 / 2;
+  // boundary node items in bface order
+  std::vector<int64_t> bcnt((size_t)nnodes + 1, 0);
+  for (int64_t ib = 0; ib < nbfaces; ib++)
+    for (int i = 0; i < dim; i++) {
+      int64_t v = bfacenodes1[dim * ib + i] - 1;
+      if (v < 0 || v >= nnodes) throw Error(VFVM_ERR_ARG, "bfacenodes out of range");
+      bcnt[v + 1]++;
+    }
+  for (int64_t i = 0; i < nnodes; i++) bcnt[i + 1] += bcnt[i];
+  out.bn_ptr.resize((size_t)nnodes + 1);
+  for (int64_t i = 0; i <= nnodes; i++) out.bn_ptr[i] = (int32_t)bcnt[i];
+  out.bn_reg.resize((size_t)bcnt[nnodes]);
+  out.bn_fac.resize((size_t)bcnt[nnodes]);
+  std::vector<int64_t> bnext(bcnt.begin(), bcnt.end() - 1);
+  for (int64_t ib = 0; ib < nbfaces; ib++) {
+    int64_t bn[3];
+    for (int i = 0; i < dim; i++) bn[i] = bfacenodes1[dim * ib + i] - 1;
+    double f[3];
+    bface_factors(dim, coord, bn, f);
+    for (int i = 0; i < dim; i++) {
+      int64_t p = bnext[bn[i]]++;
+      out.bn_reg[p] = (int32_t)bfaceregions[ib];
+      out.bn_fac[p] = f[i];
+    }
+  }
+  out.bnodes.clear();
+  for (int64_t i = 0; i < nnodes; i++)
+    if (out.bn_ptr[i + 1] > out.bn_ptr[i]) out.bnodes.push_back((int32_t)i);
+}
+
+// Registry sparsity: which unknowns each callback component depends on.
+Masks make_masks(int nspec, const vfvm_physics_t &ph, const vfvm_bc_t *bc, int nbreg) {
+  Masks m{};
+  for (int i = 0; i < nspec; i++) {
+    uint8_t e = 0, r = 0;
+    switch (ph.flux) {
+      case VFVM_FLUX_LINEAR: e = (uint8_t)(1u << i); break;
+      case VFVM_FLUX_POWER:
+      case VFVM_FLUX_NLDIFF: e = (i == 0) ? 1 : 0; break;
+      case VFVM_FLUX_DD: e = (i == 0) ? 1 : (uint8_t)(1u | (1u << i)); break;
+      default: break;
+    }
+    switch (ph.reaction) {
+      case VFVM_REACT_SINH:
+      case VFVM_REACT_POW: r = (i == 0) ? 1 : 0; break;
+      case VFVM_REACT_EXCHANGE: r = 3; break;
+      case VFVM_REACT_DD: r = 6; break;  // psi: n, p; n and p rows: n, p
+      default: break;
+    }
+    if (ph.storage == VFVM_STOR_IDENTITY) r |= (uint8_t)(1u << i);
+    m.emask[i] = e;
+    m.dmask[i] = (uint8_t)(e | r | (1u << i));
+  }
+  (void)bc;
+  (void)nbreg;
+  m.etot = m.dtot = 0;
+  for (int i = 0; i < nspec; i++) {
+    m.ecnt[i] = __builtin_popcount(m.emask[i]);
+    m.dcnt[i] = __builtin_popcount(m.dmask[i]);
+    m.eoff[i] = m.etot;
+    m.doff[i] = m.dtot;
+    m.etot += m.ecnt[i];
+    m.dtot += m.dcnt[i];
+  }
+  return m;
+}
+
+// Dof-level CSR: rows (K, i); blocks in node-graph order; within a block the
+// species columns selected by the masks (diagonal block: dmask, others: emask).
+void build_dof_csr(const HostSetup &hs, int nspec, const Masks &m, std::vector<int32_t> &rowptr,
+                   std::vector<int32_t> &colind) {
+  int64_t n = hs.nnodes, ndof = n * nspec;
+  rowptr.assign((size_t)ndof + 1, 0);
+  int64_t pos = 0;
+  for (int64_t K = 0; K < n; K++) {
+    int64_t deg = hs.adj_ptr[K + 1] - hs.adj_ptr[K] - 1;
+    for (int i = 0; i < nspec; i++) {
+      rowptr[K * nspec + i] = (int32_t)pos;
+      pos += deg * m.ecnt[i] + m.dcnt[i];
+    }
+  }
+  if (pos >= (int64_t(1) << 31) - 1) throw Error(VFVM_ERR_ARG, "Jacobian too large for int32 offsets");
+  rowptr[ndof] = (int32_t)pos;
+  colind.resize((size_t)pos);
+#pragma omp parallel for schedule(static)
+  for (int64_t K = 0; K < n; K++) {
+    for (int i = 0; i < nspec; i++) {
+      int64_t p = rowptr[K * nspec + i];
+      for (int64_t e = hs.adj_ptr[K]; e < hs.adj_ptr[K + 1]; e++) {
+        int64_t M = hs.adj_col[e];
+        uint8_t mask = (M == K) ? m.dmask[i] : m.emask[i];
+        for (int j = 0; j < nspec; j++)
+          if (mask & (1u << j)) colind[p++] = (int32_t)(M * nspec + j);
+      }
+    }
+  }
+}
+
+// Checks, for every (node, species, graph entry, species column), that the slot the
+// assembly kernel computes from adj_ptr / adj_diag / masks holds that column in the
+// CSR pattern.  Run once per system so a slot-map error is an argument error on the
+// host, never an out-of-range device write.
+void verify_slot_map(const HostSetup &hs, int nspec, const Masks &m, const std::vector<int32_t> &rowptr,
+                     const std::vector<int32_t> &colind) {
+  const int64_t n = hs.nnodes;
+  int64_t bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+  for (int64_t K = 0; K < n; K++) {
+    const int64_t a0 = hs.adj_ptr[K], deg1 = hs.adj_ptr[K + 1] - a0, diag = hs.adj_diag[K];
+    for (int i = 0; i < nspec; i++) {
+      const int64_t rowbase = (a0 - K) * m.etot + K * m.dtot + (deg1 - 1) * m.eoff[i] + m.doff[i];
+      if (rowbase != rowptr[K * nspec + i]) bad++;
+      for (int64_t e = 0; e < deg1; e++) {
+        const int64_t M = hs.adj_col[a0 + e];
+        const bool self = (e == diag);
+        if (self != (M == K)) bad++;
+        const uint8_t mask = self ? m.dmask[i] : m.emask[i];
+        const int64_t off = self ? e * m.ecnt[i] : (e < diag ? e * m.ecnt[i] : (e - 1) * m.ecnt[i] + m.dcnt[i]);
+        for (int j = 0; j < nspec; j++) {
+          if (!(mask & (1u << j))) continue;
+          const int64_t slot = rowbase + off + __builtin_popcount((unsigned)mask & ((1u << j) - 1u));
+          if (slot < rowptr[K * nspec + i] || slot >= rowptr[K * nspec + i + 1] || colind[slot] != M * nspec + j) bad++;
+        }
+      }
+    }
+  }
+  if (bad) throw Error(VFVM_ERR_ARG, "internal error: assembly slot map disagrees with the CSR pattern");
+}
+
+// CSC view of the CSR pattern (rows sorted within columns) and the map csc slot -> csr slot.
+void build_csc_perm(int64_t ndof, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
+                    std::vector<int64_t> &colptr, std::vector<int32_t> &rowval, std::vector<int32_t> &perm) {
+  int64_t nnz = rowptr[ndof];
+  colptr.assign((size_t)ndof + 1, 0);
+  for (int64_t k = 0; k < nnz; k++) colptr[colind[k] + 1]++;
+  for (int64_t j = 0; j < ndof; j++) colptr[j + 1] += colptr[j];
+  std::vector<int64_t> next(colptr.begin(), colptr.end() - 1);
+  rowval.resize((size_t)nnz);
+  perm.resize((size_t)nnz);
+  for (int64_t r = 0; r < ndof; r++)
+    for (int64_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
+      int64_t p = next[colind[k]]++;
+      rowval[p] = (int32_t)r;
+      perm[p] = (int32_t)k;
+    }
+}
+
+// Greedy colouring of the node graph in natural order (distance-1).
+int greedy_colors(const HostSetup &hs, std::vector<int32_t> &color) {
+  int64_t n = hs.nnodes;
+  color.assign((size_t)n, -1);
+  int ncol = 0;
+  std::vector<int64_t> mark(64, -1);
+  for (int64_t K = 0; K < n; K++) {
+    for (int64_t e = hs.adj_ptr[K]; e < hs.adj_ptr[K + 1]; e++) {
+      int32_t c = color[hs.adj_col[e]];
+      if (c >= 0) {
+        if ((size_t)c >= mark.size()) mark.resize((size_t)c + 1, -1);
+        mark[c] = K;
+      }
+    }
+    int c = 0;
+    while ((size_t)c < mark.size() && mark[c] == K) c++;
+    color[K] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  return ncol;
+}
+
+}  // namespace vfvm

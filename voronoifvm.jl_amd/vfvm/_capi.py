@@ -1,0 +1,159 @@
+"""ctypes binding of libvfvm_hip.so (the C ABI declared in include/vfvm_hip.h).
+
+The library is the product; if it is missing or no GPU is present, every
+compute call raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBPATH = os.path.join(_PKG, "libvfvm_hip.so")
+_LIB = None
+
+OK, ERR_CONVERGENCE, ERR_ASSEMBLY, ERR_LINEAR, ERR_ARG, ERR_HIP = range(6)
+
+
+class PhysicsT(C.Structure):
+    _fields_ = [("flux", C.c_int32), ("reaction", C.c_int32), ("storage", C.c_int32), ("pad", C.c_int32),
+                ("p", C.c_double * 16)]
+
+
+class BCT(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("value", C.c_double), ("value_lambda", C.c_double),
+                ("factor", C.c_double)]
+
+
+class ControlT(C.Structure):
+    _fields_ = [("abstol", C.c_double), ("reltol", C.c_double), ("tol_round", C.c_double), ("tol_mono", C.c_double),
+                ("damp_initial", C.c_double), ("damp_growth", C.c_double), ("maxiters", C.c_int32),
+                ("max_round", C.c_int32), ("updatecontrol", C.c_int32), ("is_linear", C.c_int32),
+                ("reltol_linear", C.c_double), ("abstol_linear", C.c_double), ("maxiters_linear", C.c_int32),
+                ("krylov", C.c_int32), ("precon", C.c_int32), ("precision", C.c_int32),
+                ("gmres_restart", C.c_int32), ("max_refinements", C.c_int32), ("reuse_precon", C.c_int32),
+                ("check_every", C.c_int32), ("inner_reltol", C.c_double)]
+
+
+class HistoryT(C.Structure):
+    _fields_ = [("niter", C.c_int32), ("nlin", C.c_int32), ("nlu", C.c_int32), ("converged", C.c_int32),
+                ("time", C.c_double), ("tasm", C.c_double), ("tlinsolve", C.c_double),
+                ("last_update_norm", C.c_double), ("residual_norm", C.c_double), ("updatenorm", C.c_double * 64)]
+
+
+class LinStatsT(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("refinements", C.c_int32), ("converged", C.c_int32), ("pad", C.c_int32),
+                ("resnorm0", C.c_double), ("resnorm", C.c_double)]
+
+
+# exported entry points and their ctypes signatures (kept in the order of include/vfvm_hip.h)
+_vp, _dp, _lp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int32)
+SIGNATURES = {
+    "vfvm_default_control": (None, [C.POINTER(ControlT)]),
+    "vfvm_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "vfvm_ctx_destroy": (C.c_int, [_vp]),
+    "vfvm_last_error": (C.c_char_p, [_vp]),
+    "vfvm_system_create": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_int64, _lp, _lp, C.c_int64, _lp, _lp, C.c_int,
+                                     C.POINTER(PhysicsT), C.c_int, C.POINTER(BCT), C.POINTER(_vp)]),
+    "vfvm_system_destroy": (C.c_int, [_vp]),
+    "vfvm_system_set_source": (C.c_int, [_vp, _dp]),
+    "vfvm_system_set_bc": (C.c_int, [_vp, C.POINTER(BCT)]),
+    "vfvm_system_sizes": (C.c_int, [_vp, _lp, _lp, _lp]),
+    "vfvm_nodevolumes": (C.c_int, [_vp, _dp]),
+    "vfvm_pattern": (C.c_int, [_vp, _lp, _lp]),
+    "vfvm_initialize_dirichlet": (C.c_int, [_vp, _dp, C.c_double, C.c_double]),
+    "vfvm_assemble": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),
+    "vfvm_linsolve": (C.c_int, [_vp, _dp, _dp, C.POINTER(ControlT), C.POINTER(LinStatsT)]),
+    "vfvm_newton": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.POINTER(ControlT),
+                              C.POINTER(HistoryT)]),
+    "vfvm_solve_fixed_steps": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int, C.POINTER(ControlT), _dp, _lp]),
+    "vfvm_residual_norm": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
+    "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
+    "vfvm_pattern_csr": (C.c_int, [_vp, _ip, _ip]),
+    "vfvm_assemble_device": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp, _vp]),
+}
+
+
+class VFVMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class ConvergenceError(VFVMError):
+    """Mirrors VoronoiFVM.ConvergenceError (src/vfvm_logging_exceptions.jl:6)."""
+
+
+class AssemblyError(VFVMError):
+    """Mirrors VoronoiFVM.AssemblyError (src/vfvm_logging_exceptions.jl:14)."""
+
+
+class LinearSolverError(VFVMError):
+    """Mirrors VoronoiFVM.LinearSolverError (src/vfvm_logging_exceptions.jl:22)."""
+
+
+_ERRTYPES = {ERR_CONVERGENCE: ConvergenceError, ERR_ASSEMBLY: AssemblyError, ERR_LINEAR: LinearSolverError}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"HIP library not built: {LIBPATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc, ctx=None):
+    if rc != OK:
+        msg = lib().vfvm_last_error(ctx).decode(errors="replace")
+        raise _ERRTYPES.get(rc, VFVMError)(rc, msg)
+
+
+def dptr(a: np.ndarray):
+    if a.dtype != np.float64 or not (a.flags["C_CONTIGUOUS"] or a.flags["F_CONTIGUOUS"]):
+        raise TypeError("expected a contiguous float64 array")
+    return a.ctypes.data_as(_dp)
+
+
+def lptr(a: np.ndarray):
+    assert a.dtype == np.int64
+    return a.ctypes.data_as(_lp)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32
+    return a.ctypes.data_as(_ip)
+
+
+class Context:
+    """One device and one HIP stream (vfvm_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        rc = lib().vfvm_ctx_create(device, C.byref(h))
+        if rc != OK:
+            raise VFVMError(rc, lib().vfvm_last_error(None).decode())
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _LIB is not None:
+            _LIB.vfvm_ctx_destroy(self.handle)
+            self.handle = None
+
+
+_DEFAULT_CTX = None
+
+
+def default_context() -> Context:
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX

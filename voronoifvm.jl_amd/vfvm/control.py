@@ -1,0 +1,66 @@
+"""SolverControl (src/vfvm_solvercontrol.jl:13-270): the fields on the path plus
+the device solver choice."""
+from __future__ import annotations
+
+from dataclasses import dataclass, fields
+
+_KRYLOV = {"bicgstab": 0, "gmres": 1}
+_PRECON = {"jacobi": 0, "blockjacobi": 1, "ilu0": 2}
+_PRECISION = {"fp64": 0, "fp32": 1}
+
+
+@dataclass
+class SolverControl:
+    abstol: float = 1.0e-10
+    reltol: float = 1.0e-10
+    maxiters: int = 100
+    tol_round: float = 1.0e-10
+    tol_mono: float = 1.0e-3
+    damp_initial: float = 1.0
+    damp_growth: float = 1.2
+    max_round: int = 1000
+    updatecontrol: bool = True
+    reltol_linear: float = 1.0e-4
+    abstol_linear: float = 1.0e-8
+    maxiters_linear: int = 100
+    # device linear solver: what method_linear selects in the reference
+    krylov: str = "bicgstab"
+    precon: str = "jacobi"
+    precision: str = "fp64"
+    gmres_restart: int = 30
+    max_refinements: int = 20
+    inner_reltol: float = 1.0e-3
+    reuse_precon: bool = False
+    check_every: int = 4
+    # transient / embedding (src/vfvm_solvercontrol.jl:138-236); fixed steps only on the device
+    Δt: float = 0.1
+    Δt_min: float = 1.0e-3
+    Δt_max: float = 1.0
+    Δp: float = 1.0
+    Δp_min: float = 1.0e-3
+    Δp_max: float = 1.0
+    verbose: bool = False
+
+    def update(self, **kw):
+        names = {f.name for f in fields(self)}
+        for k, v in kw.items():
+            if k in names:
+                setattr(self, k, v)
+        return self
+
+    def to_c(self, is_linear=False):
+        from ._capi import ControlT, lib
+        c = ControlT()
+        lib().vfvm_default_control(c)
+        for name in ("abstol", "reltol", "tol_round", "tol_mono", "damp_initial", "damp_growth", "reltol_linear",
+                     "abstol_linear", "inner_reltol"):
+            setattr(c, name, float(getattr(self, name)))
+        for name in ("maxiters", "max_round", "maxiters_linear", "gmres_restart", "max_refinements", "check_every"):
+            setattr(c, name, int(getattr(self, name)))
+        c.updatecontrol = int(bool(self.updatecontrol))
+        c.reuse_precon = int(bool(self.reuse_precon))
+        c.is_linear = int(bool(is_linear))
+        c.krylov = _KRYLOV[self.krylov]
+        c.precon = _PRECON[self.precon]
+        c.precision = _PRECISION[self.precision]
+        return c

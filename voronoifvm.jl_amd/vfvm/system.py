@@ -1,0 +1,303 @@
+"""Host-side mirror of VoronoiFVM's System / solve surface, driving the HIP library.
+
+Mirrors (paths relative to the reference root):
+  System(grid; physics..., species)          src/vfvm_system.jl:215-261
+  enable_species!                              src/vfvm_system.jl:433-470
+  boundary_dirichlet!/neumann!/robin!(sys,..)  src/vfvm_system.jl:854-933 (legacy BC API)
+  unknowns(sys; inival)                        src/vfvm_system.jl:1086-1110
+  solve(sys; inival, control, tstep, times, embed)  src/vfvm_solver.jl:551-668
+  evaluate_residual_and_jacobian               src/vfvm_solver.jl:224-260
+Arrays of unknowns are nspec x nnodes in Fortran order, so that
+``U.ravel(order="F")`` is the reference's ``dofs(U)`` order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _capi
+from .control import SolverControl
+from .grids import Grid
+from .physics import BC_DIRICHLET, BC_DIRICHLET_LEGACY, BC_ROBIN, Dirichlet, Physics, Robin
+
+
+@dataclass
+class NewtonSolverHistory:
+    """src/vfvm_history.jl:10-37 subset."""
+    nlu: int = 0
+    nlin: int = 0
+    niter: int = 0
+    time: float = 0.0
+    updatenorm: tuple = ()
+    residual_norm: float = float("nan")  # ||F||_inf over non-penalty rows at the solution
+
+
+class System:
+    """VoronoiFVM.System(grid, physics; species, is_linear)."""
+
+    def __init__(self, grid: Grid, physics: Optional[Physics] = None, species=None, is_linear=False,
+                 context=None, nspec=None, **physics_kw):
+        self.grid = grid
+        self.physics = physics if physics is not None else Physics(**physics_kw)
+        self.is_linear = is_linear
+        self.species = sorted(set(species or []))
+        if nspec is not None:
+            self.species = list(range(1, nspec + 1))
+        self._legacy = {}  # (region, species) -> (kind, value, factor)
+        self._ctx = context
+        self._h = None
+        self._bc_dirty = True
+        self.history = None
+
+    # --- species and legacy boundary conditions -------------------------------------------------
+    def enable_species(self, ispec, regions=None):
+        if ispec not in self.species:
+            self.species = sorted(self.species + [ispec])
+            self._drop()
+
+    @property
+    def num_species(self):
+        return max(self.species) if self.species else 0
+
+    def boundary_dirichlet(self, ispec, ibc, v):
+        self.enable_species(ispec)
+        self._legacy[(ibc, ispec)] = (BC_DIRICHLET_LEGACY, float(v), 0.0)
+        self._bc_dirty = True
+
+    def boundary_robin(self, ispec, ibc, alpha, v):
+        self.enable_species(ispec)
+        self._legacy[(ibc, ispec)] = (BC_ROBIN, float(v), float(alpha))
+        self._bc_dirty = True
+
+    def boundary_neumann(self, ispec, ibc, v):
+        self.boundary_robin(ispec, ibc, 0.0, v)
+
+    # --- device handle --------------------------------------------------------------------------
+    def _nbreg(self):
+        return max(self.grid.num_bfaceregions, 1)
+
+    def _bc_table(self):
+        ns, nb = self.num_species, self._nbreg()
+        tab = (_capi.BCT * (nb * ns))()
+        for bc in self.physics.bcondition:
+            if bc.region > nb or bc.species > ns:
+                continue
+            e = tab[(bc.region - 1) * ns + bc.species - 1]
+            if isinstance(bc, Dirichlet):
+                e.kind, e.value, e.value_lambda = BC_DIRICHLET, bc.value, bc.value_lambda
+            elif isinstance(bc, Robin):
+                e.kind, e.value, e.factor = BC_ROBIN, bc.value, bc.factor
+        for (reg, sp), (kind, val, fac) in self._legacy.items():
+            if reg <= nb and sp <= ns:
+                e = tab[(reg - 1) * ns + sp - 1]
+                e.kind, e.value, e.factor = kind, val, fac
+        return tab
+
+    def _source_array(self):
+        src = self.physics.source
+        if src is None:
+            return None
+        ns, n = self.num_species, self.grid.num_nodes
+        if callable(src):
+            vals = np.array([src(*self.grid.coord[:, k]) for k in range(n)], dtype=float)
+            src = vals
+        src = np.asarray(src, dtype=float)
+        if src.ndim == 1 and src.size == n:
+            full = np.zeros((ns, n), order="F")
+            full[0] = src
+            src = full
+        return np.asfortranarray(src.reshape(ns, n, order="F") if src.ndim == 1 else src)
+
+    def _drop(self):
+        if self._h is not None and _capi._LIB is not None:
+            _capi._LIB.vfvm_system_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self._drop()
+        except Exception:
+            pass
+
+    @property
+    def ctx(self):
+        if self._ctx is None:
+            self._ctx = _capi.default_context()
+        return self._ctx
+
+    def handle(self):
+        """_complete!: build the device system on first use (src/vfvm_system.jl:530-596)."""
+        if self._h is None:
+            L = _capi.lib()
+            g = self.grid
+            ns = self.num_species
+            if ns < 1:
+                raise ValueError("No species enabled. Call enable_species at least once.")
+            flux, react, stor, p = self.physics.param_block(ns)
+            ph = _capi.PhysicsT()
+            ph.flux, ph.reaction, ph.storage = flux, react, stor
+            for k in range(16):
+                ph.p[k] = p[k]
+            coord = np.asfortranarray(g.coord, dtype=np.float64)
+            cn = np.asfortranarray(g.cellnodes.astype(np.int64) + 1)
+            cr = np.ascontiguousarray(g.cellregions.astype(np.int64))
+            bn = np.asfortranarray(g.bfacenodes.astype(np.int64) + 1)
+            br = np.ascontiguousarray(g.bfaceregions.astype(np.int64))
+            tab = self._bc_table()
+            h = C.c_void_p()
+            rc = L.vfvm_system_create(self.ctx.handle, g.dim, g.num_nodes, _capi.dptr(coord), g.num_cells,
+                                      _capi.lptr(cn), _capi.lptr(cr), g.num_bfaces, _capi.lptr(bn), _capi.lptr(br), ns,
+                                      C.byref(ph), self._nbreg(), tab, C.byref(h))
+            _capi.check(rc, self.ctx.handle)
+            self._h = h
+            src = self._source_array()
+            if src is not None:
+                _capi.check(L.vfvm_system_set_source(h, _capi.dptr(src)), self.ctx.handle)
+            self._bc_dirty = False
+        elif self._bc_dirty:
+            _capi.check(_capi.lib().vfvm_system_set_bc(self._h, self._bc_table()), self.ctx.handle)
+            self._bc_dirty = False
+        return self._h
+
+    def set_source(self, src):
+        self.physics.source = src
+        if self._h is not None:
+            arr = self._source_array()
+            _capi.check(_capi.lib().vfvm_system_set_source(self._h, _capi.dptr(arr) if arr is not None else None),
+                        self.ctx.handle)
+
+    def update_bcondition(self, bcondition):
+        self.physics.bcondition = list(bcondition)
+        self._bc_dirty = True
+
+    def sizes(self):
+        nd, nz, ne = C.c_int64(), C.c_int64(), C.c_int64()
+        _capi.check(_capi.lib().vfvm_system_sizes(self.handle(), C.byref(nd), C.byref(nz), C.byref(ne)))
+        return nd.value, nz.value, ne.value
+
+    def autotune(self):
+        a, s = C.c_int32(), C.c_int32()
+        _capi.check(_capi.lib().vfvm_autotune(self.handle(), C.byref(a), C.byref(s)), self.ctx.handle)
+        return a.value, s.value
+
+    def nodevolumes(self):
+        v = np.zeros(self.grid.num_nodes)
+        _capi.check(_capi.lib().vfvm_nodevolumes(self.handle(), _capi.dptr(v)))
+        return v
+
+    def pattern(self):
+        """CSC pattern, 1-based (colptr, rowval) as SparseMatrixCSC(state.matrix) holds it."""
+        ndof, nnz, _ = self.sizes()
+        cp = np.zeros(ndof + 1, np.int64)
+        rv = np.zeros(max(nnz, 1), np.int64)
+        _capi.check(_capi.lib().vfvm_pattern(self.handle(), _capi.lptr(cp), _capi.lptr(rv)))
+        return cp, rv[:nnz]
+
+
+def unknowns(sys: System, inival=0.0):
+    """unknowns(sys; inival): nspec x nnodes, Fortran order."""
+    U = np.zeros((sys.num_species, sys.grid.num_nodes), order="F")
+    U[...] = inival
+    return U
+
+
+def dofs(U):
+    return np.asarray(U).reshape(-1, order="F")
+
+
+def _as_dofs(sys, U):
+    a = np.asarray(U, dtype=np.float64)
+    if a.ndim == 2:
+        a = a.reshape(-1, order="F")
+    if a.size != sys.num_species * sys.grid.num_nodes:
+        raise ValueError("wrong size of unknowns")
+    return np.ascontiguousarray(a)
+
+
+def _from_dofs(sys, v):
+    return np.asfortranarray(v.reshape(sys.num_species, sys.grid.num_nodes, order="F"))
+
+
+def evaluate_residual_and_jacobian(sys: System, u, t=0.0, tstep=math.inf, embed=0.0, init_dirichlet=False):
+    """Residual and Jacobian at u (src/vfvm_solver.jl:224-260).  Returns
+    (F as nspec x nnodes, scipy CSC matrix)."""
+    import scipy.sparse as sp
+    L = _capi.lib()
+    h = sys.handle()
+    U = _as_dofs(sys, u).copy()
+    if init_dirichlet:
+        _capi.check(L.vfvm_initialize_dirichlet(h, _capi.dptr(U), t, embed), sys.ctx.handle)
+    ndof, nnz, _ = sys.sizes()
+    F = np.zeros(ndof)
+    nz = np.zeros(max(nnz, 1))
+    _capi.check(L.vfvm_assemble(h, _capi.dptr(U), _capi.dptr(U), t, tstep, embed, _capi.dptr(F), _capi.dptr(nz)),
+                 sys.ctx.handle)
+    cp, rv = sys.pattern()
+    A = sp.csc_matrix((nz[:nnz], rv - 1, cp - 1), shape=(ndof, ndof))
+    return _from_dofs(sys, F), A
+
+
+def _control(control, kwargs):
+    c = SolverControl() if control is None else control
+    c.update(**{k: v for k, v in kwargs.items() if k not in ("times", "embed")})
+    return c
+
+
+def solve(sys: System, inival=0.0, control: Optional[SolverControl] = None, time=0.0, tstep=math.inf,
+          times=None, embed=None, **kwargs):
+    """solve(sys; inival, control, time, tstep, times, embed) (src/vfvm_solver.jl:551-668).
+
+    Stationary and implicit-Euler-step solves run the device Newton loop.  `times`
+    and `embed` run with fixed steps (Δt_min = Δt_max, fixed_timesteps!), the
+    step sequence given explicitly by the `times` / `embed` values.
+    Returns the solution (nspec x nnodes); for `times`/`embed` the final one."""
+    c = _control(control, kwargs)
+    L = _capi.lib()
+    h = sys.handle()
+    if np.isscalar(inival):
+        inival = unknowns(sys, inival)
+    U0 = _as_dofs(sys, inival)
+    cc = c.to_c(sys.is_linear)
+    if times is not None or embed is not None:
+        lam = np.ascontiguousarray(np.asarray(times if times is not None else embed, dtype=np.float64))
+        out = np.zeros_like(U0)
+        nn = C.c_int64()
+        _capi.check(L.vfvm_solve_fixed_steps(h, _capi.dptr(U0), lam.size, _capi.dptr(lam), int(times is not None),
+                                             C.byref(cc), _capi.dptr(out), C.byref(nn)), sys.ctx.handle)
+        sys.history = NewtonSolverHistory(niter=int(nn.value))
+        return _from_dofs(sys, out)
+    U = np.zeros_like(U0)
+    hist = _capi.HistoryT()
+    rc = L.vfvm_newton(h, _capi.dptr(U), _capi.dptr(U0), time, tstep, 0.0, C.byref(cc), C.byref(hist))
+    sys.history = NewtonSolverHistory(nlin=hist.nlin, niter=hist.niter, time=hist.time,
+                                      updatenorm=tuple(hist.updatenorm[:min(hist.niter, 64)]),
+                                      residual_norm=hist.residual_norm)
+    _capi.check(rc, sys.ctx.handle)
+    return _from_dofs(sys, U)
+
+
+def integrate(sys: System, U):
+    """integrate(sys, U)[ispec, region] for a single-region grid: sum_K |omega_K| U[i, K]
+    (src/vfvm_postprocess.jl:18-97)."""
+    vol = sys.nodevolumes()
+    return (np.asarray(U) * vol[None, :]).sum(axis=1, keepdims=True)
+
+
+def enable_species(sys, ispec, regions=None):
+    sys.enable_species(ispec, regions)
+
+
+def boundary_dirichlet(sys, ispec, ibc, v):
+    sys.boundary_dirichlet(ispec, ibc, v)
+
+
+def boundary_neumann(sys, ispec, ibc, v):
+    sys.boundary_neumann(ispec, ibc, v)
+
+
+def boundary_robin(sys, ispec, ibc, alpha, v):
+    sys.boundary_robin(ispec, ibc, alpha, v)
