@@ -189,6 +189,17 @@ int vfvm_residual_norm(vfvm_sys *sys, const double *U, const double *Uold, doubl
  * kernels for this system (project-written: times each legal size on the device). */
 int vfvm_autotune(vfvm_sys *sys, int32_t *asm_group, int32_t *spmv_group);
 
+/* Benchmark helper (no counterpart in the reference; it has no device): average device
+ * time in milliseconds of `reps` back-to-back launches of one hot kernel on the handle's
+ * stream, measured with HIP events recorded on that stream, at the current device state
+ * (U, Uold, Jacobian).  kernel: 0 Jacobian assembly (transient form, tstep = 1),
+ * 1 residual-only assembly, 2 FP64 SpMV with the row-scaled Jacobian, 3 FP32 SpMV. */
+#define VFVM_KERNEL_ASSEMBLE 0
+#define VFVM_KERNEL_RESIDUAL 1
+#define VFVM_KERNEL_SPMV64 2
+#define VFVM_KERNEL_SPMV32 3
+int vfvm_time_kernel(vfvm_sys *sys, int kernel, int reps, double *avg_ms);
+
 /* ---- device-resident interface (pointers are device pointers on the handle's device) ----
  * For hosts that keep U on the GPU (e.g. a Julia AMDGPU.jl array): same contracts as the host
  * versions, nzval_csr in the device CSR order of vfvm_pattern_csr. */

@@ -752,4 +752,55 @@ int vfvm_autotune(vfvm_sys *s, int32_t *asm_group, int32_t *spmv_group) {
   });
 }
 
+int vfvm_time_kernel(vfvm_sys *s, int kernel, int reps, double *avg_ms) {
+  if (!s || !avg_ms || reps < 1) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    hipStream_t st = s->ctx->stream;
+    DevBuf<double> x, y;
+    DevBuf<float> x32, y32;
+    if (kernel == VFVM_KERNEL_SPMV64 || kernel == VFVM_KERNEL_SPMV32) {
+      vfvm_control_t c;
+      vfvm_default_control(&c);
+      c.precision = kernel == VFVM_KERNEL_SPMV32 ? VFVM_PREC_FP32_REFINE : VFVM_PREC_FP64;
+      s->lin.prepare(s->d_J.p, c, false);
+      if (kernel == VFVM_KERNEL_SPMV32) {
+        x32.alloc((size_t)s->ndof);
+        y32.alloc((size_t)s->ndof);
+        VFVM_HIP_CHECK(hipMemsetAsync(x32.p, 0, s->ndof * sizeof(float), st));
+      } else {
+        x.alloc((size_t)s->ndof);
+        y.alloc((size_t)s->ndof);
+        launch_copy(s->ndof, s->d_U.p, x.p, st);
+      }
+    }
+    auto once = [&]() {
+      if (kernel == VFVM_KERNEL_ASSEMBLE || kernel == VFVM_KERNEL_RESIDUAL) {
+        AsmArgs a = asm_args(*s, s->d_U.p, s->d_Uold.p, kernel == VFVM_KERNEL_ASSEMBLE ? s->d_F.p : s->d_tmp.p,
+                             kernel == VFVM_KERNEL_ASSEMBLE ? s->d_J.p : nullptr, 1.0, 0.0);
+        launch_assemble(a, s->asm_group, st);
+      } else if (kernel == VFVM_KERNEL_SPMV64) {
+        s->lin.apply_plain(false, s->d_J.p, x.p, y.p);
+      } else if (kernel == VFVM_KERNEL_SPMV32) {
+        s->lin.apply_plain(true, s->d_J.p, x32.p, y32.p);
+      } else {
+        throw Error(VFVM_ERR_ARG, "unknown kernel id");
+      }
+    };
+    once();  // warm
+    hipEvent_t e0, e1;
+    VFVM_HIP_CHECK(hipEventCreate(&e0));
+    VFVM_HIP_CHECK(hipEventCreate(&e1));
+    VFVM_HIP_CHECK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) once();
+    VFVM_HIP_CHECK(hipEventRecord(e1, st));
+    VFVM_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / reps;
+    return VFVM_OK;
+  });
+}
+
 }  // extern "C"

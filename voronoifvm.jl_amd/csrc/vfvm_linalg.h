@@ -109,6 +109,8 @@ class LinearSolver {
   void prepare(const double *J, const vfvm_control_t &c, bool refactor);
   int solve(const double *J, const double *b, double *x, const vfvm_control_t &c, vfvm_linstats_t *stats);
   void set_group(int g) { L.spmv_group = g; }
+  // one plain operator application y = Ahat x in FP64 (J, row scale) or FP32 (A32), for timing
+  void apply_plain(bool fp32, const double *J, const void *x, void *y);
   const double *row_scale() const { return rowscale.p; }
 
   template <class V>

@@ -75,8 +75,9 @@ __device__ __forceinline__ bool reduce_last(const double (&v)[NR], double *parti
 __device__ __forceinline__ bool idle(const KState *st) { return st->done != 0; }
 
 // ---------------------------------------------------------------- SpMV -----
-// y = rowscale .* (A x)   (rowscale may be null), fused dots of y with w0, w1
-// (stage runs in the last block).  G lanes per row.
+// y = rowscale .* (A x)   (rowscale may be null), fused dots (w0, y) and, for
+// NDOT = 2, (y, y) taken from the register value (stage runs in the last block).
+// G lanes per row.
 template <class VA, class VX, int G, int NDOT, int STAGE>
 __global__ void __launch_bounds__(VB) k_spmv(int64_t nrows, const int32_t *__restrict__ rowptr,
                                              const int32_t *__restrict__ colind, const VA *__restrict__ val,
@@ -100,7 +101,7 @@ __global__ void __launch_bounds__(VB) k_spmv(int64_t nrows, const int32_t *__res
       const VX yv = (VX)acc;
       y[r] = yv;
       if (NDOT >= 1) d0 += (double)w0[r] * (double)yv;
-      if (NDOT >= 2) d1 += (double)w1[r] * (double)yv;
+      if (NDOT >= 2) d1 += (double)yv * (double)yv;
     }
   }
   if constexpr (NDOT > 0) {
@@ -771,6 +772,16 @@ void LinearSolver::apply_op(const V *x, V *y, const V *w0, const V *w1, KState *
   }
 }
 
+void LinearSolver::apply_plain(bool fp32, const double *J, const void *x, void *y) {
+  if (fp32)
+    apply_op<float, 0, STAGE_NONE>((const float *)x, (float *)y, (const float *)nullptr, (const float *)nullptr,
+                                   nullptr, J);
+  else
+    apply_op<double, 0, STAGE_NONE>((const double *)x, (double *)y, (const double *)nullptr,
+                                    (const double *)nullptr, nullptr, J);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
 void LinearSolver::read_state(KState &h) {
   VFVM_HIP_CHECK(hipMemcpyAsync(&h, st.p, sizeof(KState), hipMemcpyDeviceToHost, stream));
   VFVM_HIP_CHECK(hipStreamSynchronize(stream));
@@ -801,7 +812,7 @@ int LinearSolver::bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_co
       apply_op<V, 1, STAGE_ALPHA>(php, v, r0, (const V *)nullptr, st.p, J);
       hipLaunchKernelGGL((k_bicg_s<V>), dim3(g), dim3(VB), 0, stream, n, r, v, s, partials.p, st.p);
       const V *shp = precond<V>(s, sh, st.p, 1, c);
-      apply_op<V, 2, STAGE_OMEGA>(shp, t, s, t, st.p, J);
+      apply_op<V, 2, STAGE_OMEGA>(shp, t, s, (const V *)nullptr, st.p, J);
       hipLaunchKernelGGL((k_bicg_xr<V, VXo>), dim3(g), dim3(VB), 0, stream, n, x, php, shp, s, t, r, r0, partials.p,
                          st.p);
     }

@@ -71,6 +71,7 @@ SIGNATURES = {
     "vfvm_solve_fixed_steps": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int, C.POINTER(ControlT), _dp, _lp]),
     "vfvm_residual_norm": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
     "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
+    "vfvm_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "vfvm_pattern_csr": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_assemble_device": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp, _vp]),
 }
