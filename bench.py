@@ -1,0 +1,176 @@
+"""Benchmark of the MI355X assemble-and-solve path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+
+Prints ONE JSON line (metric, value, unit, ... roofline, cpu_baseline).  A
+"step" is one converged solve of the workload (an implicit-Euler time step
+solved by Newton to ||F||_inf <= 1e-10 over non-penalty rows).  Inputs are
+resident on the device before the timed region; only the solve calls are timed.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "voronoifvm.jl_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import ctypes as C  # noqa: E402
+
+import vfvm  # noqa: E402
+from vfvm import _capi  # noqa: E402
+
+METRIC = "converged solves/s at ||F||<=1e-10: surrogate+batched+FP32 vs plain GPU Newton"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_ms(sys_, kernel, reps=20):
+    v = C.c_double()
+    _capi.check(_capi.lib().vfvm_time_kernel(sys_.handle(), kernel, reps, C.byref(v)), sys_.ctx.handle)
+    return v.value
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/pmc_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(name, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+# --------------------------------------------------------------------------- cfg1
+def build_cfg1(n):
+    """3D nonlinear diffusion D(u) = 1 + u^2 on the unit cube, storage u, u = 0 on the boundary."""
+    X = vfvm.linspace(0.0, 1.0, n)
+    g = vfvm.simplexgrid(X, X, X)
+    bcs = [vfvm.Dirichlet(species=1, region=r, value=0.0) for r in range(1, 7)]
+    ph = vfvm.Physics(flux=vfvm.NonlinearDiffusionFlux(1.0, 1.0), storage=vfvm.IdentityStorage(), bcondition=bcs)
+    s = vfvm.System(g, ph, species=[1])
+    x, y, z = g.coord
+    u0 = 0.5 * np.sin(math.pi * x) * np.sin(math.pi * y) * np.sin(math.pi * z)
+    return s, u0.reshape(1, -1)
+
+
+def cfg1_control(precision):
+    return vfvm.SolverControl(krylov="bicgstab", precon="ilu0", precision=precision, reltol_linear=1e-6,
+                              abstol_linear=0.0, maxiters_linear=1000, inner_reltol=1e-3, max_refinements=10)
+
+
+def run_cfg1(args):
+    n = 129
+    t0 = time.perf_counter()
+    s, u0 = build_cfg1(n)
+    s.handle()
+    ndof, nnz, nedges = s.sizes()
+    log(f"cfg1: {n}^3 grid, ndof={ndof} nnz={nnz} edges={nedges}, setup {time.perf_counter() - t0:.1f}s")
+    ag, sg = s.autotune()
+    log(f"autotuned groups: assembly {ag}, spmv {sg}")
+    dt = 1e-2
+    K, W = args.steps, args.warmup
+    res = {}
+    for label, prec in (("plain_fp64", "fp64"), ("fp32_refine", "fp32")):
+        ctrl = cfg1_control(prec)
+        if W > 0:
+            warm = np.cumsum([0.0] + [dt] * W)
+            vfvm.solve(s, inival=np.asfortranarray(u0), control=ctrl, times=warm)
+        times = np.cumsum([0.0] + [dt] * K)
+        t1 = time.perf_counter()
+        U = vfvm.solve(s, inival=np.asfortranarray(u0), control=ctrl, times=times)
+        el = time.perf_counter() - t1
+        # converged = every step met ||F|| <= 1e-10: check the final state's residual
+        Ud = np.ascontiguousarray(vfvm.dofs(U))
+        if not s.history.residual_norm <= 1e-10:
+            raise RuntimeError(f"{label}: a step ended with ||F||_inf = {s.history.residual_norm:.3e} > 1e-10")
+        res[label] = dict(seconds=el, solves_per_s=K / el, newton=s.history.niter, krylov=s.history.nlin,
+                          max_residual=s.history.residual_norm, U=Ud)
+        log(f"{label}: {K} steps in {el:.3f}s -> {K / el:.3f} solves/s, newton iterations {s.history.niter}, "
+            f"krylov iterations {s.history.nlin}")
+    # residual of the last step at the returned solution (tstep = dt, Uold = previous step)
+    err32 = float(np.abs(res["fp32_refine"]["U"] - res["plain_fp64"]["U"]).max())
+    # roofline of the hot kernels at the final state
+    n_, E = s.grid.num_nodes, nedges
+    b_asm = 8 * 1 * n_ * 3 + 8 * n_ + E * (8 + 8) + 8 * nnz
+    b_spmv64 = nnz * (8 + 4) + (ndof + 1) * 4 + 2 * ndof * 8
+    b_spmv32 = nnz * (4 + 4) + (ndof + 1) * 4 + 2 * ndof * 4
+    t_asm = kernel_ms(s, 0)
+    t_s64 = kernel_ms(s, 2)
+    t_s32 = kernel_ms(s, 3)
+    rl = {}
+    for name, b, t in (("assemble", b_asm, t_asm), ("spmv64", b_spmv64, t_s64), ("spmv32", b_spmv32, t_s32)):
+        gbs = b / (t * 1e-3) / 1e9
+        rl[name] = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(gbs / HBM_PEAK_GBS, 4), traffic=read_traffic("cfg1_" + name),
+                        algorithmic_bytes=int(b), avg_ms=round(t, 5))
+        log(f"{name}: {t * 1e3:.1f} us/launch, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
+    out = dict(
+        value=res["fp32_refine"]["solves_per_s"], ms_per_step=1e3 * res["fp32_refine"]["seconds"] / K,
+        plain_gpu_newton_solves_per_s=res["plain_fp64"]["solves_per_s"],
+        speedup_vs_plain=res["fp32_refine"]["solves_per_s"] / res["plain_fp64"]["solves_per_s"],
+        max_abs_diff_fp32_vs_fp64=err32, roofline=rl["spmv32"], roofline_assemble=rl["assemble"],
+        roofline_spmv64=rl["spmv64"],
+        config=dict(workload=f"cfg1: 3D nonlinear diffusion D(u)=1+u^2, tetrahedral {n}^3 grid, implicit Euler "
+                             f"dt=1e-2, BiCGStab+ILU(0) FP32 inside FP64 Newton (plain: FP64 Krylov)",
+                    nodes=n_, unknowns=ndof, nnz=nnz, steps_per_solve="one implicit-Euler step"),
+        dtype="f64+f32")
+    return out
+
+
+def cpu_cfg1(args):
+    """The oracle (CPU restatement) on a bounded sample: one implicit-Euler step of the
+    same problem on an 81^3 grid (531,441 nodes), single thread."""
+    import cases
+    import oracle as O
+    n = 81
+    sys_, osys = cases.nonlinear_diffusion_3d(n)
+    u0 = cases.initial_sine_3d(sys_.grid)
+    c = O.control(method_linear="bicgstab_ilu0", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=1000)
+    t0 = time.perf_counter()
+    osys.solve_step(u0, c, time=1e-2, tstep=1e-2)
+    el = time.perf_counter() - t0
+    return dict(value=1.0 / el, unit="solves/s", cores=1, kind="port",
+                sample=f"1 implicit-Euler Newton solve of the cfg1 problem on {n}^3 nodes "
+                       f"({sys_.grid.num_nodes} nodes, 1/4 of 129^3) with the oracle's BiCGStab+ILU(0) FP64, "
+                       f"{el:.2f} s")
+
+
+WORKLOADS = {"cfg1": (run_cfg1, cpu_cfg1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg1", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); multi-GPU runs are not part "
+              f"of this build", file=sys.stderr)
+        sys.exit(2)
+    run, cpu = WORKLOADS[args.workload]
+    r = run(args)
+    line = dict(metric=METRIC, value=r.pop("value"), unit="solves/s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=r.pop("ms_per_step"), higher_is_better=True, scaling="weak",
+                vs_baseline=None, dtype=r.pop("dtype"), data="synthetic", config=r.pop("config"))
+    line.update(r)
+    line["cpu_baseline"] = None if args.no_cpu else cpu(args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -176,10 +176,12 @@ int vfvm_newton(vfvm_sys *sys, double *U, const double *Uold, double time, doubl
  * solve_transient!, src/vfvm_solver.jl:267-537 with Δt_min = Δt_max (fixed_timesteps!,
  * src/vfvm_solvercontrol.jl:320-327).  transient = 1: lambdas are times and each step is
  * solve_step!(tstep = Δt); transient = 0: lambdas are embedding values (tstep = Inf).
- * Solutions stay on the device between steps; U_final receives the last one. */
+ * Solutions stay on the device between steps; U_final receives the last one.  hist (may be
+ * NULL) accumulates over the steps: niter and nlin are totals, residual_norm is the largest
+ * per-step ||F||_inf over non-penalty rows, converged is 1 only if every step converged. */
 int vfvm_solve_fixed_steps(vfvm_sys *sys, const double *inival, int64_t nlambdas,
                            const double *lambdas, int transient, const vfvm_control_t *control,
-                           double *U_final, int64_t *newton_total);
+                           double *U_final, vfvm_history_t *hist);
 
 /* ||F||_inf over non-penalty rows for U (residual-only assembly) */
 int vfvm_residual_norm(vfvm_sys *sys, const double *U, const double *Uold, double time,

@@ -263,27 +263,39 @@ __global__ void k_init_dirichlet(int64_t nb, const int32_t *bnodes, const int32_
   }
 }
 
-// per-block max of |F| over rows that carry no Dirichlet penalty; partials are
-// non-negative doubles, combined per instance with an integer atomicMax on their bits
+// max of |F| over rows that carry no Dirichlet penalty, per instance: non-negative
+// doubles combined with an integer atomicMax on their bits, one atomic per wave when
+// the wave's nodes belong to one instance (instances are contiguous node ranges)
 __global__ void k_penalty_free_norm(int64_t nnodes, int nspec, const double *F, const int32_t *bn_ptr,
                                     const int32_t *bn_reg, const int32_t *node_inst, const InstParams *inst,
                                     unsigned long long *out) {
-  int64_t K = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (K >= nnodes) return;
-  const int id = node_inst ? node_inst[K] : 0;
-  const InstParams *P = inst + id;
+  const int64_t K = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int id = -1;
   double m = 0.0;
-  for (int i = 0; i < nspec; i++) {
-    bool pen = false;
-    for (int32_t b = bn_ptr[K]; b < bn_ptr[K + 1]; b++) {
-      const int reg = bn_reg[b];
-      if (reg < 1 || reg > MAXBREG) continue;
-      const int kind = P->bc[reg - 1][i].kind;
-      pen |= (kind == VFVM_BC_DIRICHLET || kind == VFVM_BC_DIRICHLET_LEGACY);
+  if (K < nnodes) {
+    id = node_inst ? node_inst[K] : 0;
+    const InstParams *P = inst + id;
+    for (int i = 0; i < nspec; i++) {
+      bool pen = false;
+      for (int32_t b = bn_ptr[K]; b < bn_ptr[K + 1]; b++) {
+        const int reg = bn_reg[b];
+        if (reg < 1 || reg > MAXBREG) continue;
+        const int kind = P->bc[reg - 1][i].kind;
+        pen |= (kind == VFVM_BC_DIRICHLET || kind == VFVM_BC_DIRICHLET_LEGACY);
+      }
+      if (!pen) m = fmax(m, fabs(F[K * nspec + i]));
     }
-    if (!pen) m = fmax(m, fabs(F[K * nspec + i]));
   }
-  if (m > 0.0) atomicMax(out + id, (unsigned long long)__double_as_longlong(m));
+  const int id0 = __shfl(id, 0, 64);
+  const bool uniform = __all(id == id0 || id < 0);
+  if (uniform) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && id0 >= 0 && m > 0.0)
+      atomicMax(out + id0, (unsigned long long)__double_as_longlong(m));
+  } else if (id >= 0 && m > 0.0) {
+    atomicMax(out + id, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 }  // namespace

@@ -690,7 +690,7 @@ int vfvm_newton(vfvm_sys *s, double *U, const double *Uold, double time, double 
 }
 
 int vfvm_solve_fixed_steps(vfvm_sys *s, const double *inival, int64_t nl, const double *lambdas, int transient,
-                           const vfvm_control_t *control, double *U_final, int64_t *newton_total) {
+                           const vfvm_control_t *control, double *U_final, vfvm_history_t *hist) {
   if (!s || !inival || !lambdas || nl < 1) return VFVM_ERR_ARG;
   return guarded(s->ctx, [&]() {
     vfvm_control_t c;
@@ -701,13 +701,22 @@ int vfvm_solve_fixed_steps(vfvm_sys *s, const double *inival, int64_t nl, const 
     check_control(c, *s);
     if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
     hipStream_t st = s->ctx->stream;
+    auto t0 = std::chrono::steady_clock::now();
     s->d_Uold.upload(inival, (size_t)s->ndof, st);
-    int64_t total = 0;
+    vfvm_history_t tot{};
+    tot.converged = 1;
     int rc = VFVM_OK;
+    auto account = [&](const vfvm_history_t &h) {
+      tot.niter += h.niter;
+      tot.nlin += h.nlin;
+      tot.residual_norm = std::max(tot.residual_norm, h.residual_norm);
+      tot.last_update_norm = h.last_update_norm;
+      if (!h.converged) tot.converged = 0;
+    };
     vfvm_history_t h{};
     if (!transient) {
       rc = newton_device(*s, 0.0, INFINITY, lambdas[0], c, &h);
-      total += h.niter;
+      account(h);
       if (!rc) launch_copy(s->ndof, s->d_U.p, s->d_Uold.p, st);
     } else {
       launch_init_dirichlet((int64_t)s->hs.bnodes.size(), s->d_bnodes.p, s->d_bn_ptr.p, s->d_bn_reg.p, nullptr,
@@ -719,12 +728,14 @@ int vfvm_solve_fixed_steps(vfvm_sys *s, const double *inival, int64_t nl, const 
         rc = newton_device(*s, lambdas[k], dl, 0.0, c, &h);
       else
         rc = newton_device(*s, 0.0, INFINITY, lambdas[k], c, &h);
-      total += h.niter;
+      account(h);
       if (!rc) launch_copy(s->ndof, s->d_U.p, s->d_Uold.p, st);
     }
     if (U_final) s->d_Uold.download(U_final, (size_t)s->ndof, st);
     VFVM_HIP_CHECK(hipStreamSynchronize(st));
-    if (newton_total) *newton_total = total;
+    if (rc) tot.converged = 0;
+    tot.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (hist) *hist = tot;
     return rc;
   });
 }

@@ -151,6 +151,7 @@ class LinearSolver {
   Work<double> w64;
   bool ilu_ready = false;
   double last_rr = 0, last_rr0 = 0;
+  int last_iters = 0;
 };
 
 void launch_newton_update(int64_t n, double *U, const double *dU, double damp, double *partials, unsigned *ticket,

@@ -42,8 +42,9 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return r;  // valid in thread 0
 }
 
-// Write this block's partials; returns true in thread 0 of the last block, which
-// then holds the totals in tot[0..n-1].
+// Write this block's partials; the last block to finish (ticket) sums all
+// partials with every thread and returns true in its thread 0, which then holds
+// the totals in tot[0..NR-1].
 template <int NR>
 __device__ __forceinline__ bool reduce_last(const double (&v)[NR], double *partials, unsigned *ticket, double (&tot)[NR]) {
   __shared__ double sh[16];
@@ -60,14 +61,14 @@ __device__ __forceinline__ bool reduce_last(const double (&v)[NR], double *parti
   }
   __syncthreads();
   if (!last) return false;
-  if (threadIdx.x != 0) return false;
   __threadfence();
 #pragma unroll
   for (int k = 0; k < NR; k++) {
     double acc = 0.0;
-    for (unsigned b = 0; b < gridDim.x; b++) acc += partials[(size_t)k * gridDim.x + b];
-    tot[k] = acc;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) acc += partials[(size_t)k * gridDim.x + b];
+    tot[k] = block_sum(acc, sh);
   }
+  if (threadIdx.x != 0) return false;
   atomicExch(ticket, 0u);
   return true;
 }
@@ -566,13 +567,37 @@ __global__ void __launch_bounds__(VB) k_newton_update(int64_t n, double *__restr
   __syncthreads();
   double bm = 0.0;
   for (int k = 0; k < (int)(blockDim.x >> 6); k++) bm = fmax(bm, shm[k]);
-  double vv[2] = {s, 0.0}, tot[2];
   if (threadIdx.x == 0) partials[(size_t)2 * gridDim.x + blockIdx.x] = bm;
-  if (reduce_last<2>(vv, partials, ticket, tot)) {
-    double mx = 0.0;
-    for (unsigned b = 0; b < gridDim.x; b++) mx = fmax(mx, partials[(size_t)2 * gridDim.x + b]);
-    out[0] = mx;
-    out[1] = tot[0];
+  // the max partials are published by the same fence / ticket as the sums
+  __shared__ double shx[16];
+  __shared__ bool lastb;
+  double ssum = block_sum(s, shx);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = ssum;
+    __threadfence();
+    unsigned t = atomicAdd(ticket, 1u);
+    lastb = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!lastb) return;
+  __threadfence();
+  double acc = 0.0, mx = 0.0;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+    acc += partials[b];
+    mx = fmax(mx, partials[(size_t)2 * gridDim.x + b]);
+  }
+  acc = block_sum(acc, shx);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m2 = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) m2 = fmax(m2, shm[k]);
+    out[0] = m2;
+    out[1] = acc;
+    atomicExch(ticket, 0u);
   }
 }
 
@@ -803,7 +828,10 @@ int LinearSolver::bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_co
   unsigned g = grid_for(n);
   hipLaunchKernelGGL((k_fill<VXo>), dim3(g), dim3(VB), 0, stream, n, x, (VXo)0);
   hipLaunchKernelGGL((k_bicg_init<V, VBin>), dim3(g), dim3(VB), 0, stream, n, b, r, r0, p, v, partials.p, st.p);
-  int check = std::max(1, c.check_every);
+  // first chunk: about as many iterations as the previous solve needed (Newton steps of one
+  // solve need similar counts), then poll every check_every iterations
+  const int every = std::max(1, c.check_every);
+  int check = std::max(every, std::min(maxit, last_iters - 1));
   int launched = 0;
   while (true) {
     for (int q = 0; q < check; q++) {
@@ -821,9 +849,10 @@ int LinearSolver::bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_co
     read_state(h);
     if (h.done) break;
     if (launched >= maxit + check) break;
-    check = std::min(check * 2, 32);
+    check = every;
   }
   iters = h.iters;
+  last_iters = h.iters;
   last_rr = h.rr;
   last_rr0 = h.rr0;
   return h.status == 1 ? 0 : VFVM_ERR_LINEAR;

@@ -68,7 +68,8 @@ SIGNATURES = {
     "vfvm_linsolve": (C.c_int, [_vp, _dp, _dp, C.POINTER(ControlT), C.POINTER(LinStatsT)]),
     "vfvm_newton": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.POINTER(ControlT),
                               C.POINTER(HistoryT)]),
-    "vfvm_solve_fixed_steps": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int, C.POINTER(ControlT), _dp, _lp]),
+    "vfvm_solve_fixed_steps": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int, C.POINTER(ControlT), _dp,
+                                         C.POINTER(HistoryT)]),
     "vfvm_residual_norm": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
     "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),

@@ -265,10 +265,12 @@ def solve(sys: System, inival=0.0, control: Optional[SolverControl] = None, time
     if times is not None or embed is not None:
         lam = np.ascontiguousarray(np.asarray(times if times is not None else embed, dtype=np.float64))
         out = np.zeros_like(U0)
-        nn = C.c_int64()
-        _capi.check(L.vfvm_solve_fixed_steps(h, _capi.dptr(U0), lam.size, _capi.dptr(lam), int(times is not None),
-                                             C.byref(cc), _capi.dptr(out), C.byref(nn)), sys.ctx.handle)
-        sys.history = NewtonSolverHistory(niter=int(nn.value))
+        hist = _capi.HistoryT()
+        rc = L.vfvm_solve_fixed_steps(h, _capi.dptr(U0), lam.size, _capi.dptr(lam), int(times is not None),
+                                      C.byref(cc), _capi.dptr(out), C.byref(hist))
+        sys.history = NewtonSolverHistory(niter=hist.niter, nlin=hist.nlin, time=hist.time,
+                                          residual_norm=hist.residual_norm)
+        _capi.check(rc, sys.ctx.handle)
         return _from_dofs(sys, out)
     U = np.zeros_like(U0)
     hist = _capi.HistoryT()
