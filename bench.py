@@ -147,7 +147,110 @@ def cpu_cfg1(args):
                        f"{el:.2f} s")
 
 
-WORKLOADS = {"cfg1": (run_cfg1, cpu_cfg1)}
+# --------------------------------------------------------------------------- cfg3
+def cfg3_members(count=256):
+    import cases
+    return [cases.batch_member(seed) for seed in range(count)]
+
+
+def build_poisson(n, eps, vals, f):
+    X = vfvm.linspace(0.0, 1.0, n)
+    g = vfvm.simplexgrid(X, X)
+    bcs = [vfvm.Dirichlet(species=1, region=r + 1, value=float(v)) for r, v in enumerate(vals)]
+    src = np.full((1, g.num_nodes), f, order="F")
+    ph = vfvm.Physics(flux=vfvm.LinearFlux(eps), reaction=vfvm.SinhReaction(), source=src, bcondition=bcs)
+    return vfvm.System(g, ph, species=[1])
+
+
+def cfg3_control(precision):
+    return vfvm.SolverControl(krylov="bicgstab", precon="jacobi", precision=precision, reltol_linear=1e-6,
+                              abstol_linear=0.0, maxiters_linear=3000, inner_reltol=1e-3, max_refinements=12,
+                              check_every=8)
+
+
+def run_cfg3(args):
+    members = cfg3_members()
+    t0 = time.perf_counter()
+    systems = [build_poisson(*m) for m in members]
+    for s in systems:
+        s.handle()
+    ndof = sum(s.grid.num_nodes for s in systems)
+    nnz = sum(s.sizes()[1] for s in systems)
+    batch = vfvm.Batch(systems)
+    log(f"cfg3: {len(systems)} instances, {ndof} unknowns, nnz {nnz}, setup {time.perf_counter() - t0:.1f}s, "
+        f"autotuned groups {batch.groups}")
+    K, W = args.steps, args.warmup
+    res = {}
+    for label, prec in (("batched_fp64", "fp64"), ("batched_fp32", "fp32")):
+        ctrl = cfg3_control(prec)
+        for _ in range(W):
+            batch.solve(0.0, ctrl)
+        t1 = time.perf_counter()
+        for _ in range(K):
+            U = batch.solve(0.0, ctrl)
+        el = time.perf_counter() - t1
+        worst = max(h.residual_norm for h in batch.histories)
+        if not worst <= 1e-10:
+            raise RuntimeError(f"{label}: a member ended with ||F||_inf = {worst:.3e}")
+        res[label] = dict(seconds=el, solves_per_s=len(systems) * K / el, U=[vfvm.dofs(u).copy() for u in U],
+                          newton=sum(h.niter for h in batch.histories), krylov=sum(h.nlin for h in batch.histories))
+        log(f"{label}: {K} x {len(systems)} solves in {el:.3f}s -> {res[label]['solves_per_s']:.1f} solves/s "
+            f"(newton {res[label]['newton']}, krylov {res[label]['krylov']}, worst ||F|| {worst:.2e})")
+    # plain GPU Newton: the same 256 solves one system at a time, FP64
+    ctrl = cfg3_control("fp64")
+    vfvm.solve(systems[0], inival=0.0, control=ctrl)
+    t1 = time.perf_counter()
+    Useq = []
+    for s in systems:
+        Useq.append(vfvm.dofs(vfvm.solve(s, inival=0.0, control=ctrl)).copy())
+        if not s.history.residual_norm <= 1e-10:
+            raise RuntimeError(f"sequential: ||F||_inf = {s.history.residual_norm:.3e}")
+    el = time.perf_counter() - t1
+    plain = len(systems) / el
+    log(f"sequential plain GPU Newton: {len(systems)} solves in {el:.2f}s -> {plain:.1f} solves/s")
+    diff64 = max(float(np.abs(a - b).max()) for a, b in zip(res["batched_fp64"]["U"], Useq))
+    diff32 = max(float(np.abs(a - b).max()) for a, b in zip(res["batched_fp32"]["U"], Useq))
+    # roofline of the batched SpMV / assembly are those of the single-system kernels; report the
+    # largest member's kernels measured live
+    big = max(systems, key=lambda s: s.grid.num_nodes)
+    vfvm.solve(big, inival=0.0, control=ctrl)
+    nd, nz, ne = big.sizes()
+    b_spmv = nz * (8 + 4) + (nd + 1) * 4 + 2 * nd * 8
+    t_spmv = kernel_ms(big, 2)
+    gbs = b_spmv / (t_spmv * 1e-3) / 1e9
+    return dict(value=res["batched_fp32"]["solves_per_s"], ms_per_step=1e3 * res["batched_fp32"]["seconds"] / K,
+                plain_gpu_newton_solves_per_s=plain,
+                speedup_vs_plain=res["batched_fp32"]["solves_per_s"] / plain,
+                batched_fp64_solves_per_s=res["batched_fp64"]["solves_per_s"],
+                max_abs_diff_vs_plain=dict(batched_fp64=diff64, batched_fp32=diff32),
+                roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=round(gbs / HBM_PEAK_GBS, 4), traffic=read_traffic("cfg3_spmv64"),
+                              algorithmic_bytes=int(b_spmv), avg_ms=round(t_spmv, 5),
+                              kernel="FP64 SpMV of the largest member (257^2)"),
+                config=dict(workload="cfg3: 256 nonlinear-Poisson instances, N x N grids with N in "
+                                     "{33,65,129,257}, seeded eps/boundary values/source, batched "
+                                     "BiCGStab+Jacobi, FP32 inside FP64 Newton (plain: 256 sequential FP64 "
+                                     "GPU Newton solves)", instances=len(systems), unknowns=ndof, nnz=nnz),
+                dtype="f64+f32")
+
+
+def cpu_cfg3(args):
+    """The oracle on a bounded sample: the first 24 of the 256 instances, direct solves."""
+    import cases
+    import oracle as O
+    members = cfg3_members(24)
+    t = 0.0
+    for (n, eps, vals, f) in members:
+        s, o = cases.nonlinear_poisson_2d(n, eps, f, tuple(vals))
+        t0 = time.perf_counter()
+        o.solve_step(np.zeros(s.grid.num_nodes))
+        t += time.perf_counter() - t0
+    return dict(value=len(members) / t, unit="solves/s", cores=1, kind="port",
+                sample=f"the first {len(members)} of the 256 cfg3 instances (seeds 0..{len(members) - 1}), "
+                       f"oracle Newton with banded direct solves, {t:.2f} s")
+
+
+WORKLOADS = {"cfg1": (run_cfg1, cpu_cfg1), "cfg3": (run_cfg3, cpu_cfg3)}
 
 
 def main():

@@ -209,6 +209,25 @@ int vfvm_pattern_csr(vfvm_sys *sys, int32_t *rowptr, int32_t *colind);
 int vfvm_assemble_device(vfvm_sys *sys, const double *dU, const double *dUold, double time,
                          double tstep, double lambda, double *dF, double *dnzval_csr);
 
+/* ---- batched solves (the independent-SystemState pattern of
+ * examples/Example440_ParallelState.jl:36-45, packed into single launches) ----
+ * A batch concatenates systems created on one context into a block-diagonal device system.
+ * Members share species count and physics kinds; each keeps its own parameters, source,
+ * boundary table, damping and convergence.  Member data are copied at creation (later
+ * set_source / set_bc calls on a member do not reach the batch). */
+typedef struct vfvm_batch vfvm_batch;
+int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfvm_batch **out);
+int vfvm_batch_destroy(vfvm_batch *b);
+/* project-written autotuner for the batch's assembly and SpMV group sizes */
+int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group);
+/* solve_step! for every member (src/vfvm_solver.jl:13-222): U, Uold are nsys host pointers to
+ * nspec x nnodes arrays; hist (may be NULL) receives nsys histories.  Batched solves use
+ * BiCGStab with row-scaling Jacobi, FP64 or FP32 inside FP64 refinement.  Returns
+ * VFVM_ERR_CONVERGENCE if any member failed (the others' solutions are still returned). */
+int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold, double time,
+                      double tstep, double lambda, const vfvm_control_t *control,
+                      vfvm_history_t *hist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,3 +135,15 @@ def compare_assembly(sys, osys, U, Uold=None, tstep=math.inf, embed=0.0, rtol=1e
     for k, v in errs.items():
         assert v <= rtol, (k, v)
     return errs
+
+
+def batch_member(seed, sizes=(33, 65, 129, 257)):
+    """One instance of the batched nonlinear-Poisson family: N from `sizes`, eps
+    log-uniform in [1e-3, 1], Dirichlet values uniform in [-1, 1] per side, constant
+    source uniform in [-5, 5]; all drawn from numpy.random.default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.choice(sizes))
+    eps = float(10.0 ** rng.uniform(-3.0, 0.0))
+    vals = rng.uniform(-1.0, 1.0, 4)
+    f = float(rng.uniform(-5.0, 5.0))
+    return n, eps, vals, f

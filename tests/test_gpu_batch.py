@@ -1,0 +1,37 @@
+"""Batched solves (many systems in one launch sequence) against the oracle and
+against one-at-a-time GPU solves."""
+import numpy as np
+import pytest
+
+import cases
+import vfvm
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_matches_oracle_and_single_solves(precision):
+    print("case batch", precision, flush=True)
+    members = [cases.batch_member(seed, sizes=(9, 17, 33)) for seed in range(12)]
+    pairs = [cases.nonlinear_poisson_2d(n, eps, f, tuple(v)) for (n, eps, v, f) in members]
+    systems = [p[0] for p in pairs]
+    ctrl = vfvm.SolverControl(precision=precision, reltol_linear=1e-10, abstol_linear=0.0, maxiters_linear=2000)
+    batch = vfvm.Batch(systems)
+    Ub = batch.solve(0.0, ctrl)
+    for k, ((s, o), U) in enumerate(zip(pairs, Ub)):
+        Uo, _ = o.solve_step(np.zeros(s.grid.num_nodes))
+        err = np.abs(vfvm.dofs(U) - Uo).max()
+        assert err <= 1e-9, (k, err)
+        assert batch.histories[k].residual_norm <= 1e-10
+    # the same members one at a time through the single-system path
+    for k in (0, 5, 11):
+        Us = vfvm.solve(systems[k], inival=0.0, control=ctrl)
+        assert np.abs(vfvm.dofs(Us) - vfvm.dofs(Ub[k])).max() <= 1e-9
+
+
+def test_batch_member_failure_is_reported():
+    print("case batch failure", flush=True)
+    pairs = [cases.nonlinear_poisson_2d(9, 1e-2, 1.0, (0, 1, 1, 0)) for _ in range(3)]
+    batch = vfvm.Batch([p[0] for p in pairs])
+    with pytest.raises(vfvm.ConvergenceError):
+        batch.solve(0.0, vfvm.SolverControl(maxiters=1))

@@ -47,6 +47,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
   const int64_t K = (int64_t)blockIdx.x * (ASM_BLOCK / G) + threadIdx.x / G;
   if (K >= a.nnodes) return;  // whole group leaves together (G divides the wave)
   const int inst = a.node_inst ? a.node_inst[K] : 0;
+  if (a.inst_active && !a.inst_active[inst]) return;  // converged member of a batch
   const InstParams *P = a.inst + inst;
   const int flux_kind = P->flux;
   double p[16];

@@ -1,0 +1,903 @@
+// Batched solves: many independent systems packed into one block-diagonal device
+// system and advanced by the same launches.
+//
+// The reference solves independent problems with one SystemState per thread
+// (examples/Example440_ParallelState.jl:36-45).  Here the members of a batch are
+// concatenated: node, dof and nonzero ranges are contiguous per member, every
+// node carries its member index (node_inst) and the assembly kernel reads the
+// member's parameters and boundary table from a per-member block.  The Krylov
+// kernels work on "chunks" of at most CHUNK rows that never straddle two
+// members; every reduction writes per-chunk partials and a finalize kernel with
+// one block per member sums them and updates that member's solver scalars.  Converged members drop
+// out: their kernels return at entry and their nodes are skipped by assembly.
+// One launch therefore advances every unfinished member, whatever its size.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <string>
+
+#include "vfvm_sys.h"
+
+using namespace vfvm;
+
+namespace {
+
+constexpr int BB = 256;             // threads per block
+constexpr int64_t CHUNK = 2048;     // rows per chunk
+
+struct Seg {
+  int64_t nch;
+  const int32_t *ch_inst;
+  const int64_t *ch_beg, *ch_end;
+  const int32_t *ch_first;  // per member: first chunk, nsys + 1 entries
+  unsigned *tickets;        // per member
+  double *partials;         // 3 x nch
+};
+
+__device__ __forceinline__ double bsum(double v, double *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) r += sh[k];
+  return r;
+}
+
+__device__ __forceinline__ double bmax(double v, double *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) r = fmax(r, sh[k]);
+  return r;
+}
+
+// Each chunk writes NR sums (and optionally a max in slot 2) as partials; a
+// finalize kernel with one block per member sums the member's chunk partials in
+// chunk order and runs the stage (no cross-block fences, deterministic sums).
+template <int NR, bool WITHMAX>
+__device__ void seg_partials(const Seg &S, const double (&v)[NR], double mx) {
+  __shared__ double sh[16];
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    const double s = bsum(v[k], sh);
+    if (threadIdx.x == 0) S.partials[(size_t)k * S.nch + blockIdx.x] = s;
+  }
+  if (WITHMAX) {
+    const double m = bmax(mx, sh);
+    if (threadIdx.x == 0) S.partials[(size_t)2 * S.nch + blockIdx.x] = m;
+  }
+}
+
+enum { BF_STAGE = 0, BF_RR = 1, BF_INIT = 2, BF_UPDATE = 3 };
+
+// block m finalises member m; mode selects what the totals feed
+template <int NR, int STAGE, int MODE>
+__global__ void __launch_bounds__(BB) k_bfinalize(Seg S, KState *st, const int32_t *__restrict__ active,
+                                                  double *__restrict__ out) {
+  const int m = blockIdx.x;
+  KState *sti = st + m;
+  if (MODE == BF_STAGE) {
+    if (sti->done) return;
+    if (STAGE == STAGE_OMEGA && sti->half) return;
+  }
+  if ((MODE == BF_RR || MODE == BF_UPDATE) && active && !active[m]) return;
+  __shared__ double sh[16];
+  const int c0 = S.ch_first[m], c1 = S.ch_first[m + 1];
+  double tot[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    double acc = 0.0;
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) acc += S.partials[(size_t)k * S.nch + c];
+    tot[k] = bsum(acc, sh);
+  }
+  double tmax = 0.0;
+  if (MODE == BF_UPDATE) {
+    double acc = 0.0;
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) acc = fmax(acc, S.partials[(size_t)2 * S.nch + c]);
+    tmax = bmax(acc, sh);
+  }
+  if (threadIdx.x != 0) return;
+  if constexpr (MODE == BF_STAGE) {
+    run_stage<STAGE>(sti, tot);
+  } else if constexpr (MODE == BF_RR) {
+    sti->rr = tot[0];
+  } else if constexpr (MODE == BF_INIT) {
+    run_stage<STAGE_INIT>(sti, tot);
+    if (active && !active[m]) {
+      sti->done = 1;
+      sti->status = 1;
+    }
+  } else {
+    out[2 * m] = tmax;
+    out[2 * m + 1] = tot[0];
+  }
+}
+
+// y = rowscale .* (A x) on the chunk's rows, fused (w0, y) and (y, y)
+template <class VA, class V, int G, int NDOT, int STAGE>
+__global__ void __launch_bounds__(BB) k_bspmv(Seg S, const int32_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ colind, const VA *__restrict__ val,
+                                              const V *__restrict__ x, V *__restrict__ y,
+                                              const double *__restrict__ rowscale, const V *__restrict__ w0,
+                                              KState *st) {
+  const int inst = S.ch_inst[blockIdx.x];
+  KState *sti = st + inst;
+  if (sti->done) return;
+  if (STAGE == STAGE_OMEGA && sti->half) return;
+  const int64_t r0 = S.ch_beg[blockIdx.x], r1 = S.ch_end[blockIdx.x];
+  const int lane = threadIdx.x & (G - 1);
+  double d0 = 0.0, d1 = 0.0;
+  for (int64_t r = r0 + threadIdx.x / G; r < r1; r += BB / G) {
+    const int32_t k0 = rowptr[r], k1 = rowptr[r + 1];
+    double acc = 0.0;
+    for (int32_t k = k0 + lane; k < k1; k += G) acc += (double)val[k] * (double)x[colind[k]];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+    if (lane == 0) {
+      if (rowscale) acc *= rowscale[r];
+      const V yv = (V)acc;
+      y[r] = yv;
+      if (NDOT >= 1) d0 += (double)w0[r] * (double)yv;
+      if (NDOT >= 2) d1 += (double)yv * (double)yv;
+    }
+  }
+  if constexpr (NDOT > 0) {
+    double v[2] = {d0, d1};
+    seg_partials<2, false>(S, v, 0.0);
+  }
+}
+
+// y = S (b - A x), ||y||^2 per member -> st.rr (FP64; refinement residual and bhat)
+template <int G>
+__global__ void __launch_bounds__(BB) k_bres(Seg S, const int32_t *__restrict__ rowptr,
+                                             const int32_t *__restrict__ colind, const double *__restrict__ val,
+                                             const double *__restrict__ x, const double *__restrict__ b,
+                                             const double *__restrict__ rowscale, double *__restrict__ y,
+                                             KState *st, const int32_t *__restrict__ active) {
+  const int inst = S.ch_inst[blockIdx.x];
+  if (active && !active[inst]) return;
+  const int64_t r0 = S.ch_beg[blockIdx.x], r1 = S.ch_end[blockIdx.x];
+  const int lane = threadIdx.x & (G - 1);
+  double d0 = 0.0;
+  for (int64_t r = r0 + threadIdx.x / G; r < r1; r += BB / G) {
+    const int32_t k0 = rowptr[r], k1 = rowptr[r + 1];
+    double acc = 0.0;
+    if (x)
+      for (int32_t k = k0 + lane; k < k1; k += G) acc += val[k] * x[colind[k]];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+    if (lane == 0) {
+      const double v = rowscale[r] * (b[r] - acc);
+      y[r] = v;
+      d0 += v * v;
+    }
+  }
+  double v[1] = {d0};
+  seg_partials<1, false>(S, v, 0.0);
+}
+
+template <class V, class VBin>
+__global__ void __launch_bounds__(BB) k_binit(Seg S, const VBin *__restrict__ b, V *__restrict__ r, V *__restrict__ r0,
+                                              V *__restrict__ p, V *__restrict__ v, V *__restrict__ x, KState *st,
+                                              const int32_t *__restrict__ active) {
+  const int inst = S.ch_inst[blockIdx.x];
+  const int64_t i0 = S.ch_beg[blockIdx.x], i1 = S.ch_end[blockIdx.x];
+  const bool on = !active || active[inst];
+  double d = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += BB) {
+    const V bv = on ? (V)b[i] : (V)0;
+    r[i] = bv;
+    r0[i] = bv;
+    p[i] = (V)0;
+    v[i] = (V)0;
+    x[i] = (V)0;
+    d += (double)bv * (double)bv;
+  }
+  double vv[1] = {d};
+  seg_partials<1, false>(S, vv, 0.0);
+}
+
+template <class V>
+__global__ void __launch_bounds__(BB) k_bp(Seg S, const V *__restrict__ r, V *__restrict__ p, const V *__restrict__ v,
+                                           const KState *st) {
+  const int inst = S.ch_inst[blockIdx.x];
+  const KState *sti = st + inst;
+  if (sti->done) return;
+  const double beta = sti->beta, omega = sti->omega;
+  for (int64_t i = S.ch_beg[blockIdx.x] + threadIdx.x; i < S.ch_end[blockIdx.x]; i += BB)
+    p[i] = (V)((double)r[i] + beta * ((double)p[i] - omega * (double)v[i]));
+}
+
+template <class V>
+__global__ void __launch_bounds__(BB) k_bs(Seg S, const V *__restrict__ r, const V *__restrict__ v, V *__restrict__ s,
+                                           KState *st) {
+  const int inst = S.ch_inst[blockIdx.x];
+  KState *sti = st + inst;
+  if (sti->done) return;
+  const double alpha = sti->alpha;
+  double d = 0.0;
+  for (int64_t i = S.ch_beg[blockIdx.x] + threadIdx.x; i < S.ch_end[blockIdx.x]; i += BB) {
+    const V sv = (V)((double)r[i] - alpha * (double)v[i]);
+    s[i] = sv;
+    d += (double)sv * (double)sv;
+  }
+  double vv[1] = {d};
+  seg_partials<1, false>(S, vv, 0.0);
+}
+
+template <class V>
+__global__ void __launch_bounds__(BB) k_bxr(Seg S, V *__restrict__ x, const V *__restrict__ ph,
+                                            const V *__restrict__ sh, const V *__restrict__ s,
+                                            const V *__restrict__ t, V *__restrict__ r, const V *__restrict__ r0,
+                                            KState *st) {
+  const int inst = S.ch_inst[blockIdx.x];
+  KState *sti = st + inst;
+  if (sti->done) return;
+  const double alpha = sti->alpha, omega = sti->omega;
+  const bool half = sti->half != 0;
+  double d0 = 0.0, d1 = 0.0;
+  for (int64_t i = S.ch_beg[blockIdx.x] + threadIdx.x; i < S.ch_end[blockIdx.x]; i += BB) {
+    if (half) {
+      x[i] = (V)((double)x[i] + alpha * (double)ph[i]);
+    } else {
+      x[i] = (V)((double)x[i] + alpha * (double)ph[i] + omega * (double)sh[i]);
+      const V rv = (V)((double)s[i] - omega * (double)t[i]);
+      r[i] = rv;
+      d0 += (double)r0[i] * (double)rv;
+      d1 += (double)rv * (double)rv;
+    }
+  }
+  double vv[2] = {d0, d1};
+  seg_partials<2, false>(S, vv, 0.0);
+}
+
+// x64 += d (members with active flag), d in the Krylov precision
+template <class V>
+__global__ void __launch_bounds__(BB) k_badd(Seg S, double *__restrict__ x, const V *__restrict__ d,
+                                             const int32_t *__restrict__ active) {
+  const int inst = S.ch_inst[blockIdx.x];
+  if (active && !active[inst]) return;
+  for (int64_t i = S.ch_beg[blockIdx.x] + threadIdx.x; i < S.ch_end[blockIdx.x]; i += BB) x[i] += (double)d[i];
+}
+
+// Newton update per member: U -= damp_m dU, ||dU||_inf and ||U||_1 per member
+__global__ void __launch_bounds__(BB) k_bupdate(Seg S, double *__restrict__ U, const double *__restrict__ dU,
+                                                const double *__restrict__ damp, const int32_t *__restrict__ active,
+                                                double *__restrict__ out) {
+  const int inst = S.ch_inst[blockIdx.x];
+  if (!active[inst]) return;
+  const double dm = damp ? damp[inst] : 0.0;
+  double m = 0.0, s = 0.0;
+  for (int64_t i = S.ch_beg[blockIdx.x] + threadIdx.x; i < S.ch_end[blockIdx.x]; i += BB) {
+    double u = U[i];
+    if (damp) {
+      const double d = dU[i];
+      u -= dm * d;
+      U[i] = u;
+      m = fmax(m, fabs(d));
+    }
+    s += fabs(u);
+  }
+  double vv[1] = {s};
+  seg_partials<1, true>(S, vv, m);
+}
+
+template <class T>
+__global__ void k_bfill(int64_t n, T *p, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * BB + threadIdx.x; i < n; i += (int64_t)gridDim.x * BB) p[i] = v;
+}
+
+template <class TO, class TI>
+__global__ void k_bconvert(int64_t n, const TI *__restrict__ a, TO *__restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * BB + threadIdx.x; i < n; i += (int64_t)gridDim.x * BB) b[i] = (TO)a[i];
+}
+
+unsigned gfor(int64_t n) {
+  int64_t b = (n + BB - 1) / BB;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 4096));
+}
+
+}  // namespace
+
+struct vfvm_batch {
+  vfvm_ctx *ctx = nullptr;
+  int64_t nsys = 0;
+  int nspec = 1;
+  std::vector<vfvm_sys *> members;
+  std::vector<int64_t> node_off, dof_off;
+  int64_t nnodes = 0, ndof = 0, nnz = 0, nbnodes = 0;
+  Masks masks{};
+  int asm_group = 8, spmv_group = 8;
+  DevBuf<int32_t> adj_ptr, adj_col, adj_diag, bn_ptr, bn_reg, bnodes, rowptr, colind, dofdiag, node_inst;
+  DevBuf<double> adj_fac, nodevol, bn_fac, src, U, Uold, F, J, dU, tmp, rowscale, bhat, rres;
+  DevBuf<InstParams> inst;
+  bool has_src = false;
+  // chunks
+  int64_t nch = 0;
+  DevBuf<int32_t> ch_inst, ch_first;
+  DevBuf<int64_t> ch_beg, ch_end;
+  DevBuf<unsigned> tickets;
+  DevBuf<double> partials;
+  DevBuf<KState> st;
+  DevBuf<int32_t> active, ref_active, flags;
+  DevBuf<double> damp, norms;
+  // Krylov work
+  DevBuf<double> k64[8];
+  DevBuf<float> k32[8];
+  DevBuf<float> A32;
+  Seg seg() {
+    return Seg{nch, ch_inst.p, ch_beg.p, ch_end.p, ch_first.p, tickets.p, partials.p};
+  }
+};
+
+namespace {
+
+template <class F>
+int bguard(vfvm_ctx *ctx, F &&f) {
+  try {
+    return f();
+  } catch (const Error &e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    if (ctx) ctx->err = e.what();
+    return VFVM_ERR_HIP;
+  }
+}
+
+AsmArgs basm(vfvm_batch &b, double *F, double *J, double tstep, double lambda, const int32_t *active) {
+  AsmArgs a{};
+  a.nnodes = b.nnodes;
+  a.adj_ptr = b.adj_ptr.p;
+  a.adj_col = b.adj_col.p;
+  a.adj_diag = b.adj_diag.p;
+  a.adj_fac = b.adj_fac.p;
+  a.nodevol = b.nodevol.p;
+  a.bn_ptr = b.bn_ptr.p;
+  a.bn_reg = b.bn_reg.p;
+  a.bn_fac = b.bn_fac.p;
+  a.node_inst = b.node_inst.p;
+  a.inst = b.inst.p;
+  a.inst_active = active;
+  a.U = b.U.p;
+  a.Uold = b.Uold.p;
+  a.src = b.has_src ? b.src.p : nullptr;
+  a.F = F;
+  a.J = J;
+  a.tstepinv = 1.0 / tstep;
+  a.lambda = lambda;
+  a.flags = b.flags.p;
+  a.m = b.masks;
+  a.nspec = b.nspec;
+  return a;
+}
+
+template <class V, int NDOT, int STAGE>
+void bop(vfvm_batch &b, const V *x, V *y, const V *w0) {
+  const Seg S = b.seg();
+  auto go = [&](auto gc) {
+    constexpr int G = decltype(gc)::value;
+    if constexpr (sizeof(V) == 4)
+      hipLaunchKernelGGL((k_bspmv<float, V, G, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                         b.rowptr.p, b.colind.p, (const float *)b.A32.p, x, y, (const double *)nullptr, w0, b.st.p);
+    else
+      hipLaunchKernelGGL((k_bspmv<double, V, G, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                         b.rowptr.p, b.colind.p, (const double *)b.J.p, x, y, (const double *)b.rowscale.p, w0, b.st.p);
+  };
+  switch (b.spmv_group) {
+    case 2: go(std::integral_constant<int, 2>()); break;
+    case 4: go(std::integral_constant<int, 4>()); break;
+    case 8: go(std::integral_constant<int, 8>()); break;
+    case 16: go(std::integral_constant<int, 16>()); break;
+    default: go(std::integral_constant<int, 32>()); break;
+  }
+  if constexpr (NDOT > 0)
+    hipLaunchKernelGGL((k_bfinalize<2, STAGE, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, b.ctx->stream, S,
+                       b.st.p, (const int32_t *)nullptr, (double *)nullptr);
+}
+
+void bres(vfvm_batch &b, const double *x, const double *rhs, double *y, const int32_t *active) {
+  const Seg S = b.seg();
+  auto go = [&](auto gc) {
+    constexpr int G = decltype(gc)::value;
+    hipLaunchKernelGGL((k_bres<G>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S, b.rowptr.p, b.colind.p,
+                       (const double *)b.J.p, x, rhs, (const double *)b.rowscale.p, y, b.st.p, active);
+  };
+  switch (b.spmv_group) {
+    case 2: go(std::integral_constant<int, 2>()); break;
+    case 4: go(std::integral_constant<int, 4>()); break;
+    case 8: go(std::integral_constant<int, 8>()); break;
+    case 16: go(std::integral_constant<int, 16>()); break;
+    default: go(std::integral_constant<int, 32>()); break;
+  }
+  hipLaunchKernelGGL((k_bfinalize<1, STAGE_NONE, BF_RR>), dim3((unsigned)b.nsys), dim3(BB), 0, b.ctx->stream, S, b.st.p,
+                     active, (double *)nullptr);
+}
+
+std::vector<KState> read_states(vfvm_batch &b) {
+  std::vector<KState> h((size_t)b.nsys);
+  VFVM_HIP_CHECK(hipMemcpyAsync(h.data(), b.st.p, sizeof(KState) * b.nsys, hipMemcpyDeviceToHost, b.ctx->stream));
+  VFVM_HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
+  return h;
+}
+
+void reset_states(vfvm_batch &b, double rtol, double atol, int maxit) {
+  std::vector<KState> h((size_t)b.nsys);
+  for (auto &k : h) {
+    std::memset(&k, 0, sizeof(k));
+    k.rtol = rtol;
+    k.atol = atol;
+    k.maxit = maxit;
+  }
+  b.st.upload(h.data(), h.size(), b.ctx->stream);
+}
+
+// per-member BiCGStab (point Jacobi = identity on the row-scaled operator) on bhat -> x
+template <class V, class VBin>
+int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double rtol, double atol, int maxit,
+              int check_every, std::vector<int64_t> &nlin) {
+  hipStream_t st = b.ctx->stream;
+  const Seg S = b.seg();
+  V *r, *r0, *p, *v, *s, *t;
+  if constexpr (sizeof(V) == 4) {
+    r = b.k32[0].p; r0 = b.k32[1].p; p = b.k32[2].p; v = b.k32[3].p; s = b.k32[4].p; t = b.k32[5].p;
+  } else {
+    r = b.k64[0].p; r0 = b.k64[1].p; p = b.k64[2].p; v = b.k64[3].p; s = b.k64[4].p; t = b.k64[5].p;
+  }
+  reset_states(b, rtol, atol, maxit);
+  hipLaunchKernelGGL((k_binit<V, VBin>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, bh, r, r0, p, v, x, b.st.p, active);
+  hipLaunchKernelGGL((k_bfinalize<1, STAGE_INIT, BF_INIT>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S, b.st.p, active,
+                     (double *)nullptr);
+  const int every = std::max(1, check_every);
+  int launched = 0;
+  std::vector<KState> h;
+  while (true) {
+    for (int q = 0; q < every; q++) {
+      hipLaunchKernelGGL((k_bp<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, r, p, v, b.st.p);
+      bop<V, 1, STAGE_ALPHA>(b, p, v, r0);
+      hipLaunchKernelGGL((k_bs<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, r, v, s, b.st.p);
+      hipLaunchKernelGGL((k_bfinalize<1, STAGE_SNORM, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S, b.st.p,
+                         (const int32_t *)nullptr, (double *)nullptr);
+      bop<V, 2, STAGE_OMEGA>(b, s, t, s);
+      hipLaunchKernelGGL((k_bxr<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, x, p, s, s, t, r, r0, b.st.p);
+      hipLaunchKernelGGL((k_bfinalize<2, STAGE_XR, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S, b.st.p,
+                         (const int32_t *)nullptr, (double *)nullptr);
+    }
+    VFVM_HIP_CHECK(hipGetLastError());
+    launched += every;
+    h = read_states(b);
+    bool all = true;
+    for (auto &k : h) all &= (k.done != 0);
+    if (all || launched >= maxit + every) break;
+  }
+  int rc = 0;
+  for (size_t m = 0; m < h.size(); m++) {
+    nlin[m] += h[m].iters;
+    if (h[m].status != 1) rc = VFVM_ERR_LINEAR;
+  }
+  return rc;
+}
+
+// Solve J dU = F per active member (FP64 BiCGStab, or FP32 BiCGStab inside FP64 refinement)
+void bsolve(vfvm_batch &b, const vfvm_control_t &c, std::vector<int64_t> &nlin) {
+  hipStream_t st = b.ctx->stream;
+  const Seg S = b.seg();
+  const bool fp32 = c.precision == VFVM_PREC_FP32_REFINE;
+  launch_scale_rows(b.nspec, b.nnodes, b.rowptr.p, b.colind.p, b.J.p, b.dofdiag.p, b.rowscale.p,
+                    fp32 ? b.A32.p : nullptr, nullptr, st);
+  // bhat = S F
+  bres(b, nullptr, b.F.p, b.bhat.p, b.active.p);
+  if (!fp32) {
+    bbicgstab<double, double>(b, b.bhat.p, b.dU.p, b.active.p, c.reltol_linear, c.abstol_linear, c.maxiters_linear,
+                              c.check_every, nlin);
+    return;
+  }
+  // refinement: tolerance per member from ||bhat||
+  std::vector<KState> h0 = read_states(b);  // rr of bhat per member (written by k_bres)
+  std::vector<double> tol((size_t)b.nsys);
+  for (int64_t m = 0; m < b.nsys; m++) tol[m] = c.abstol_linear + c.reltol_linear * std::sqrt(h0[m].rr);
+  std::vector<int32_t> ref((size_t)b.nsys);
+  std::vector<int32_t> act((size_t)b.nsys);
+  VFVM_HIP_CHECK(hipMemcpyAsync(act.data(), b.active.p, sizeof(int32_t) * b.nsys, hipMemcpyDeviceToHost, st));
+  VFVM_HIP_CHECK(hipStreamSynchronize(st));
+  hipLaunchKernelGGL((k_bfill<double>), dim3(gfor(b.ndof)), dim3(BB), 0, st, b.ndof, b.dU.p, 0.0);
+  hipLaunchKernelGGL((k_bconvert<double, double>), dim3(gfor(b.ndof)), dim3(BB), 0, st, b.ndof,
+                     (const double *)b.bhat.p, b.rres.p);
+  std::vector<double> rr(h0.size());
+  for (size_t m = 0; m < h0.size(); m++) rr[m] = h0[m].rr;
+  const double inner = std::max(c.inner_reltol, 1e-6);
+  for (int it = 0; it < std::max(1, c.max_refinements); it++) {
+    bool any = false;
+    for (int64_t m = 0; m < b.nsys; m++) {
+      ref[m] = act[m] && std::sqrt(rr[m]) > tol[m];
+      any |= ref[m] != 0;
+    }
+    if (!any) break;
+    b.ref_active.upload(ref.data(), ref.size(), st);
+    float *r32 = b.k32[6].p, *d32 = b.k32[7].p;
+    hipLaunchKernelGGL((k_bconvert<float, double>), dim3(gfor(b.ndof)), dim3(BB), 0, st, b.ndof,
+                       (const double *)b.rres.p, r32);
+    int64_t before = 0, after = 0;
+    for (auto v : nlin) before += v;
+    bbicgstab<float, float>(b, r32, d32, b.ref_active.p, inner, 0.0, c.maxiters_linear, c.check_every, nlin);
+    for (auto v : nlin) after += v;
+    hipLaunchKernelGGL((k_badd<float>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, b.dU.p, (const float *)d32,
+                       b.ref_active.p);
+    bres(b, b.dU.p, b.F.p, b.rres.p, b.ref_active.p);
+    std::vector<KState> h = read_states(b);
+    for (int64_t m = 0; m < b.nsys; m++)
+      if (ref[m]) rr[m] = h[m].rr;
+    if (after == before) break;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfvm_batch **out) {
+  return bguard(ctx, [&]() {
+    if (!ctx || !systems || !out || nsys < 1) throw Error(VFVM_ERR_ARG, "bad batch arguments");
+    std::unique_ptr<vfvm_batch> b(new vfvm_batch());
+    b->ctx = ctx;
+    b->nsys = nsys;
+    const vfvm_sys *s0 = systems[0];
+    b->nspec = s0->nspec;
+    b->masks = s0->masks;
+    for (int64_t m = 0; m < nsys; m++) {
+      const vfvm_sys *s = systems[m];
+      if (!s || s->ctx != ctx) throw Error(VFVM_ERR_ARG, "batch members must live on the batch's context");
+      if (s->nspec != s0->nspec || s->phys.flux != s0->phys.flux || s->phys.reaction != s0->phys.reaction ||
+          s->phys.storage != s0->phys.storage)
+        throw Error(VFVM_ERR_ARG, "batch members must share species count and physics kinds");
+      b->members.push_back(systems[m]);
+    }
+    const int ns = b->nspec;
+    b->node_off.assign((size_t)nsys + 1, 0);
+    b->dof_off.assign((size_t)nsys + 1, 0);
+    int64_t nadj = 0, nbi = 0;
+    for (int64_t m = 0; m < nsys; m++) {
+      const vfvm_sys *s = systems[m];
+      b->node_off[m + 1] = b->node_off[m] + s->nnodes;
+      b->dof_off[m + 1] = b->dof_off[m] + s->ndof;
+      nadj += s->hs.adj_ptr.back();
+      b->nnz += s->nnz;
+      nbi += (int64_t)s->hs.bn_reg.size();
+      b->nbnodes += (int64_t)s->hs.bnodes.size();
+    }
+    b->nnodes = b->node_off[nsys];
+    b->ndof = b->dof_off[nsys];
+    if (nadj >= (int64_t(1) << 31) || b->nnz >= (int64_t(1) << 31) || b->nnodes * ns >= (int64_t(1) << 31))
+      throw Error(VFVM_ERR_ARG, "batch too large for int32 offsets");
+    // concatenate host structures
+    std::vector<int32_t> adj_ptr((size_t)b->nnodes + 1), adj_col((size_t)nadj), adj_diag((size_t)b->nnodes),
+        bn_ptr((size_t)b->nnodes + 1), bn_reg((size_t)std::max<int64_t>(nbi, 1)), bnodes, rowptr((size_t)b->ndof + 1),
+        colind((size_t)b->nnz), dofdiag((size_t)b->ndof), node_inst((size_t)b->nnodes);
+    std::vector<double> adj_fac((size_t)nadj), nodevol((size_t)b->nnodes), bn_fac((size_t)std::max<int64_t>(nbi, 1));
+    std::vector<InstParams> ip((size_t)nsys);
+    int64_t pa = 0, pb = 0, pz = 0;
+    bnodes.reserve((size_t)std::max<int64_t>(b->nbnodes, 1));
+    int maxdeg = 1, maxrow = 1;
+    for (int64_t m = 0; m < nsys; m++) {
+      const vfvm_sys *s = systems[m];
+      const HostSetup &h = s->hs;
+      const int64_t no = b->node_off[m], dof0 = b->dof_off[m];
+      for (int64_t K = 0; K < s->nnodes; K++) {
+        adj_ptr[no + K] = (int32_t)(pa + h.adj_ptr[K]);
+        adj_diag[no + K] = h.adj_diag[K];
+        nodevol[no + K] = h.nodevol[K];
+        bn_ptr[no + K] = (int32_t)(pb + h.bn_ptr[K]);
+        node_inst[no + K] = (int32_t)m;
+        maxdeg = std::max(maxdeg, h.adj_ptr[K + 1] - h.adj_ptr[K]);
+      }
+      for (int64_t k = 0; k < h.adj_ptr.back(); k++) {
+        adj_col[pa + k] = (int32_t)(h.adj_col[k] + no);
+        adj_fac[pa + k] = h.adj_fac[k];
+      }
+      for (size_t k = 0; k < h.bn_reg.size(); k++) {
+        bn_reg[pb + k] = h.bn_reg[k];
+        bn_fac[pb + k] = h.bn_fac[k];
+      }
+      for (int32_t K : h.bnodes) bnodes.push_back((int32_t)(K + no));
+      for (int64_t r = 0; r < s->ndof; r++) {
+        rowptr[dof0 + r] = (int32_t)(pz + s->rowptr[r]);
+        dofdiag[dof0 + r] = (int32_t)(pz + s->dofdiag[r]);
+        maxrow = std::max(maxrow, s->rowptr[r + 1] - s->rowptr[r]);
+      }
+      for (int64_t k = 0; k < s->nnz; k++) colind[pz + k] = (int32_t)(s->colind[k] + dof0);
+      pa += h.adj_ptr.back();
+      pb += (int64_t)h.bn_reg.size();
+      pz += s->nnz;
+      ip[m] = make_inst(*s);
+    }
+    adj_ptr[b->nnodes] = (int32_t)pa;
+    bn_ptr[b->nnodes] = (int32_t)pb;
+    rowptr[b->ndof] = (int32_t)pz;
+    // the kernel's slot arithmetic holds for the concatenation (global adj_ptr, node index)
+    {
+      HostSetup cat;
+      cat.nnodes = b->nnodes;
+      cat.adj_ptr = adj_ptr;
+      cat.adj_col = adj_col;
+      cat.adj_diag = adj_diag;
+      verify_slot_map(cat, ns, b->masks, rowptr, colind);
+    }
+    hipStream_t st = ctx->stream;
+    b->adj_ptr.upload(adj_ptr, st);
+    b->adj_col.upload(adj_col, st);
+    b->adj_diag.upload(adj_diag, st);
+    b->adj_fac.upload(adj_fac, st);
+    b->nodevol.upload(nodevol, st);
+    b->bn_ptr.upload(bn_ptr, st);
+    b->bn_reg.upload(bn_reg, st);
+    b->bn_fac.upload(bn_fac, st);
+    if (bnodes.empty()) bnodes.push_back(0);
+    b->bnodes.upload(bnodes, st);
+    b->rowptr.upload(rowptr, st);
+    b->colind.upload(colind, st);
+    b->dofdiag.upload(dofdiag, st);
+    b->node_inst.upload(node_inst, st);
+    b->inst.upload(ip, st);
+    // sources: device-to-device from each member
+    b->has_src = false;
+    for (auto *s : b->members) b->has_src |= s->has_src;
+    if (b->has_src) {
+      b->src.alloc((size_t)b->ndof);
+      hipLaunchKernelGGL((k_bfill<double>), dim3(gfor(b->ndof)), dim3(BB), 0, st, b->ndof, b->src.p, 0.0);
+      for (int64_t m = 0; m < nsys; m++)
+        if (b->members[m]->has_src)
+          VFVM_HIP_CHECK(hipMemcpyAsync(b->src.p + b->dof_off[m], b->members[m]->d_src.p,
+                                        sizeof(double) * b->members[m]->ndof, hipMemcpyDeviceToDevice, st));
+    }
+    for (auto *buf : {&b->U, &b->Uold, &b->F, &b->dU, &b->tmp, &b->rowscale, &b->bhat, &b->rres})
+      buf->alloc((size_t)b->ndof);
+    b->J.alloc((size_t)b->nnz);
+    // chunks: at most CHUNK dof rows, never across members
+    std::vector<int32_t> ci, cf((size_t)nsys + 1, 0);
+    std::vector<int64_t> cb, ce;
+    for (int64_t m = 0; m < nsys; m++) {
+      cf[m] = (int32_t)ci.size();
+      for (int64_t r = b->dof_off[m]; r < b->dof_off[m + 1]; r += CHUNK) {
+        ci.push_back((int32_t)m);
+        cb.push_back(r);
+        ce.push_back(std::min(r + CHUNK, b->dof_off[m + 1]));
+      }
+    }
+    cf[nsys] = (int32_t)ci.size();
+    b->nch = (int64_t)ci.size();
+    b->ch_inst.upload(ci, st);
+    b->ch_first.upload(cf, st);
+    b->ch_beg.upload(cb, st);
+    b->ch_end.upload(ce, st);
+    b->tickets.alloc((size_t)nsys);
+    VFVM_HIP_CHECK(hipMemsetAsync(b->tickets.p, 0, sizeof(unsigned) * nsys, st));
+    b->partials.alloc((size_t)3 * b->nch);
+    b->st.alloc((size_t)nsys);
+    b->active.alloc((size_t)nsys);
+    b->ref_active.alloc((size_t)nsys);
+    b->flags.alloc(4);
+    b->damp.alloc((size_t)nsys);
+    b->norms.alloc((size_t)2 * nsys);
+    for (auto &v : b->k64) v.alloc((size_t)b->ndof);
+    for (auto &v : b->k32) v.alloc((size_t)b->ndof);
+    b->A32.alloc((size_t)b->nnz);
+    b->asm_group = systems[0]->asm_group;
+    b->spmv_group = systems[0]->spmv_group;
+    int g = 4;
+    while (g < maxdeg && g < 32) g *= 2;
+    b->asm_group = g;
+    g = 2;
+    while (g < maxrow && g < 32) g *= 2;
+    b->spmv_group = g;
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    *out = b.release();
+    return VFVM_OK;
+  });
+}
+
+int vfvm_batch_destroy(vfvm_batch *b) {
+  if (!b) return VFVM_OK;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  delete b;
+  return VFVM_OK;
+}
+
+// project-written autotuner: time the batched Jacobian assembly and SpMV for every
+// legal group size (lanes per node / per row) and keep the fastest
+int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group) {
+  if (!b) return VFVM_ERR_ARG;
+  return bguard(b->ctx, [&]() {
+    hipStream_t st = b->ctx->stream;
+    std::vector<int32_t> all((size_t)b->nsys, 1);
+    b->active.upload(all.data(), all.size(), st);
+    hipEvent_t e0, e1;
+    VFVM_HIP_CHECK(hipEventCreate(&e0));
+    VFVM_HIP_CHECK(hipEventCreate(&e1));
+    auto timeit = [&](auto &&fn) {
+      fn();
+      VFVM_HIP_CHECK(hipEventRecord(e0, st));
+      for (int r = 0; r < 5; r++) fn();
+      VFVM_HIP_CHECK(hipEventRecord(e1, st));
+      VFVM_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      return ms;
+    };
+    float best = 1e30f;
+    int bestg = b->asm_group;
+    for (int g : {4, 8, 16, 32}) {
+      AsmArgs a = basm(*b, b->F.p, b->J.p, 1.0, 0.0, b->active.p);
+      float ms = timeit([&]() { launch_assemble(a, g, st); });
+      if (ms < best) {
+        best = ms;
+        bestg = g;
+      }
+    }
+    b->asm_group = bestg;
+    launch_scale_rows(b->nspec, b->nnodes, b->rowptr.p, b->colind.p, b->J.p, b->dofdiag.p, b->rowscale.p, nullptr,
+                      nullptr, st);
+    reset_states(*b, 0.0, 0.0, 1);
+    best = 1e30f;
+    bestg = b->spmv_group;
+    for (int g : {2, 4, 8, 16, 32}) {
+      b->spmv_group = g;
+      float ms = timeit([&]() { bop<double, 0, STAGE_NONE>(*b, b->U.p, b->tmp.p, (const double *)nullptr); });
+      if (ms < best) {
+        best = ms;
+        bestg = g;
+      }
+    }
+    b->spmv_group = bestg;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (asm_group) *asm_group = b->asm_group;
+    if (spmv_group) *spmv_group = b->spmv_group;
+    return VFVM_OK;
+  });
+}
+
+// solve_step! for every member at once (src/vfvm_solver.jl:13-222 per member)
+int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold, double time, double tstep,
+                      double lambda, const vfvm_control_t *control, vfvm_history_t *hist) {
+  if (!b || !U || !Uold) return VFVM_ERR_ARG;
+  return bguard(b->ctx, [&]() {
+    (void)time;
+    vfvm_control_t c;
+    if (control)
+      c = *control;
+    else
+      vfvm_default_control(&c);
+    if (c.krylov != VFVM_KRYLOV_BICGSTAB || c.precon != VFVM_PRECON_JACOBI)
+      throw Error(VFVM_ERR_ARG, "batched solves use BiCGStab with (row-scaling) Jacobi");
+    hipStream_t st = b->ctx->stream;
+    const int64_t M = b->nsys;
+    const Seg S = b->seg();
+    auto t0 = std::chrono::steady_clock::now();
+    // pack Uold, copy to U, Dirichlet init
+    {
+      std::vector<double> pack((size_t)b->ndof);
+      for (int64_t m = 0; m < M; m++)
+        std::memcpy(pack.data() + b->dof_off[m], Uold[m], sizeof(double) * (b->dof_off[m + 1] - b->dof_off[m]));
+      b->Uold.upload(pack.data(), pack.size(), st);
+    }
+    launch_copy(b->ndof, b->Uold.p, b->U.p, st);
+    launch_init_dirichlet(b->nbnodes, b->bnodes.p, b->bn_ptr.p, b->bn_reg.p, b->node_inst.p, b->inst.p, b->nspec,
+                          lambda, b->U.p, st);
+    launch_fill_i32(1, b->flags.p, 0, st);
+    std::vector<int32_t> act((size_t)M, 1);
+    b->active.upload(act.data(), act.size(), st);
+    std::vector<double> damp((size_t)M, c.is_linear ? 1.0 : c.damp_initial), oldnorm((size_t)M, 1.0),
+        rnorm((size_t)M, 0.0), tolx((size_t)M, 0.0), hn((size_t)2 * M);
+    std::vector<int> nround((size_t)M, 0), niter((size_t)M, 0), conv((size_t)M, 0), failed((size_t)M, 0);
+    std::vector<int64_t> nlin((size_t)M, 0);
+    std::vector<std::vector<double>> upd((size_t)M);
+    // initial ||U||_1 per member
+    hipLaunchKernelGGL(k_bupdate, dim3((unsigned)b->nch), dim3(BB), 0, st, S, b->U.p, (const double *)nullptr,
+                       (const double *)nullptr, (const int32_t *)b->active.p, b->norms.p);
+    hipLaunchKernelGGL((k_bfinalize<1, STAGE_NONE, BF_UPDATE>), dim3((unsigned)M), dim3(BB), 0, st, S, b->st.p,
+                       (const int32_t *)b->active.p, b->norms.p);
+    VFVM_HIP_CHECK(hipMemcpyAsync(hn.data(), b->norms.p, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, st));
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    for (int64_t m = 0; m < M; m++) rnorm[m] = hn[2 * m + 1];
+    int64_t nactive = M;
+    for (int it = 1; it <= c.maxiters && nactive > 0; it++) {
+      AsmArgs a = basm(*b, b->F.p, b->J.p, tstep, lambda, b->active.p);
+      launch_assemble(a, b->asm_group, st);
+      bsolve(*b, c, nlin);
+      b->damp.upload(damp.data(), damp.size(), st);
+      hipLaunchKernelGGL(k_bupdate, dim3((unsigned)b->nch), dim3(BB), 0, st, S, b->U.p, (const double *)b->dU.p,
+                         (const double *)b->damp.p, (const int32_t *)b->active.p, b->norms.p);
+      hipLaunchKernelGGL((k_bfinalize<1, STAGE_NONE, BF_UPDATE>), dim3((unsigned)M), dim3(BB), 0, st, S, b->st.p,
+                         (const int32_t *)b->active.p, b->norms.p);
+      int32_t fl = 0;
+      VFVM_HIP_CHECK(hipMemcpyAsync(hn.data(), b->norms.p, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, st));
+      VFVM_HIP_CHECK(hipMemcpyAsync(&fl, b->flags.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      VFVM_HIP_CHECK(hipStreamSynchronize(st));
+      if (fl & 1) throw Error(VFVM_ERR_ASSEMBLY, "trying to assemble NaN");
+      for (int64_t m = 0; m < M; m++) {
+        if (!act[m]) continue;
+        niter[m] = it;
+        const double norm = hn[2 * m], rnew = hn[2 * m + 1];
+        upd[m].push_back(norm);
+        int stop = 0;
+        if (c.is_linear) {
+          conv[m] = 1;
+          stop = 1;
+        } else {
+          damp[m] = std::min(damp[m] * c.damp_growth, 1.0);
+          if (tolx[m] == 0.0) tolx[m] = norm * c.reltol;
+          double dnorm = 1.0;
+          if (rnorm[m] > 1.0e-50) dnorm = std::fabs((rnorm[m] - rnew) / rnorm[m]);
+          nround[m] = dnorm < c.tol_round ? nround[m] + 1 : 0;
+          if (!std::isfinite(norm) || (it > 1 && norm / oldnorm[m] > 1.0 / c.tol_mono)) {
+            failed[m] = 1;
+            stop = 1;
+          } else if (norm < c.abstol || norm < tolx[m]) {
+            conv[m] = 1;
+            stop = 1;
+          } else {
+            oldnorm[m] = norm;
+            rnorm[m] = rnew;
+            if (nround[m] > c.max_round) {
+              conv[m] = 1;
+              stop = 1;
+            }
+          }
+        }
+        if (stop) {
+          act[m] = 0;
+          nactive--;
+        }
+      }
+      b->active.upload(act.data(), act.size(), st);
+    }
+    // residual norms per member at the returned solutions
+    std::vector<double> resn((size_t)M, 0.0);
+    {
+      std::vector<int32_t> all((size_t)M, 1);
+      b->active.upload(all.data(), all.size(), st);
+      AsmArgs a = basm(*b, b->tmp.p, nullptr, tstep, lambda, b->active.p);
+      launch_assemble(a, b->asm_group, st);
+      b->norms.alloc((size_t)std::max<int64_t>(2 * M, 2));
+      launch_fill_u64(M, reinterpret_cast<unsigned long long *>(b->norms.p), st);
+      launch_penalty_free_norm(b->nnodes, b->nspec, b->tmp.p, b->bn_ptr.p, b->bn_reg.p, b->node_inst.p, b->inst.p,
+                               b->norms.p, M, st);
+      VFVM_HIP_CHECK(hipMemcpyAsync(resn.data(), b->norms.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+    }
+    std::vector<double> out((size_t)b->ndof);
+    b->U.download(out.data(), out.size(), st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    for (int64_t m = 0; m < M; m++)
+      std::memcpy(U[m], out.data() + b->dof_off[m], sizeof(double) * (b->dof_off[m + 1] - b->dof_off[m]));
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int rc = VFVM_OK;
+    for (int64_t m = 0; m < M; m++) {
+      if (hist) {
+        vfvm_history_t h{};
+        h.niter = niter[m];
+        h.nlin = (int32_t)nlin[m];
+        h.converged = conv[m];
+        h.residual_norm = resn[m];
+        h.time = el;
+        for (size_t k = 0; k < upd[m].size() && k < 64; k++) h.updatenorm[k] = upd[m][k];
+        h.last_update_norm = upd[m].empty() ? 0.0 : upd[m].back();
+        hist[m] = h;
+      }
+      if (!conv[m]) rc = VFVM_ERR_CONVERGENCE;
+    }
+    if (rc) b->ctx->err = "Newton did not converge for at least one batch member (ConvergenceError)";
+    return rc;
+  });
+}
+
+}  // extern "C"

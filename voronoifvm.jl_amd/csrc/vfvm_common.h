@@ -130,6 +130,7 @@ struct AsmArgs {
   const double *bn_fac;
   const int32_t *node_inst;      // per-node instance id (batched), nullptr: instance 0
   const InstParams *inst;        // device array of instance parameters
+  const int32_t *inst_active;    // per-instance flag; nodes of inactive instances are skipped (nullable)
   const double *U, *Uold, *src;  // src may be nullptr
   double *F, *J;                 // J may be nullptr (residual only)
   double tstepinv, lambda;

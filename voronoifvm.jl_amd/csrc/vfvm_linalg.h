@@ -161,6 +161,8 @@ void launch_gather(int64_t n, const int32_t *perm, const double *J, double *out,
 void launch_fill_u64(int64_t n, unsigned long long *p, hipStream_t s);
 void launch_fill_i32(int64_t n, int32_t *p, int32_t v, hipStream_t s);
 void launch_copy(int64_t n, const double *a, double *b, hipStream_t s);
+void launch_scale_rows(int nspec, int64_t nnodes, const int32_t *rowptr, const int32_t *colind, const double *A,
+                       const int32_t *dofdiag, double *rowscale, float *A32, double *binv, hipStream_t s);
 void time_spmv(const LinSetup &L, int G, const double *A, const double *x, double *y, hipStream_t s);
 
 }  // namespace vfvm

@@ -10,11 +10,11 @@
 // Conventions: the operator is the row-scaled Jacobian  Ahat = S A  with
 // S = diag(1 / A_ii), which turns the 1e30 Dirichlet-penalty rows into unit
 // rows before any FP32 cast (1e30^2 would overflow an FP32 dot product).
-// Reductions: every reducing kernel writes one partial per block, then the last
-// block to finish (ticket counter) sums the partials in fixed order and runs a
-// small "stage" that updates the solver scalars in device memory.  Kernels of a
-// finished solve return at entry, so the host can enqueue several iterations
-// between status reads.
+// Reductions: every reducing kernel writes one partial per block; a one-block
+// finalize kernel sums the partials in fixed order and runs a small "stage"
+// that updates the solver scalars in device memory (no cross-block fences,
+// deterministic sums).  Kernels of a finished solve return at entry, so the
+// host can enqueue several iterations between status reads.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,38 +42,50 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return r;  // valid in thread 0
 }
 
-// Write this block's partials; the last block to finish (ticket) sums all
-// partials with every thread and returns true in its thread 0, which then holds
-// the totals in tot[0..NR-1].
+// Every reducing kernel writes one partial per block (no fence, no atomics); a
+// one-block finalize kernel launched next on the same stream sums the partials in
+// fixed block order and runs the stage.  The kernel boundary orders the two.
 template <int NR>
-__device__ __forceinline__ bool reduce_last(const double (&v)[NR], double *partials, unsigned *ticket, double (&tot)[NR]) {
+__device__ __forceinline__ void write_partials(const double (&v)[NR], double *partials) {
   __shared__ double sh[16];
-  __shared__ bool last;
-  double s[NR];
-#pragma unroll
-  for (int k = 0; k < NR; k++) s[k] = block_sum(v[k], sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NR; k++) partials[(size_t)k * gridDim.x + blockIdx.x] = s[k];
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
 #pragma unroll
   for (int k = 0; k < NR; k++) {
-    double acc = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) acc += partials[(size_t)k * gridDim.x + b];
-    tot[k] = block_sum(acc, sh);
+    const double s = block_sum(v[k], sh);
+    if (threadIdx.x == 0) partials[(size_t)k * gridDim.x + blockIdx.x] = s;
   }
-  if (threadIdx.x != 0) return false;
-  atomicExch(ticket, 0u);
-  return true;
 }
 
 __device__ __forceinline__ bool idle(const KState *st) { return st->done != 0; }
+
+// stages that only store a total
+enum { STAGE_RR = 100, STAGE_GMSTART = 101, STAGE_DOT0 = 102 };
+
+template <int NR, int STAGE>
+__global__ void __launch_bounds__(VB) k_finalize(const double *__restrict__ partials, int nparts, KState *st,
+                                                 int check_idle) {
+  if (check_idle && st->done) return;
+  if (STAGE == STAGE_OMEGA && st->half) return;
+  __shared__ double sh[16];
+  double tot[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) acc += partials[(size_t)k * nparts + b];
+    tot[k] = block_sum(acc, sh);
+  }
+  if (threadIdx.x != 0) return;
+  if constexpr (STAGE == STAGE_RR) {
+    st->rr = tot[0];
+  } else if constexpr (STAGE == STAGE_GMSTART) {
+    st->rr = tot[0];
+    st->gm_inv = tot[0] > 0 ? 1.0 / sqrt(tot[0]) : 0.0;
+    st->gm_beta = sqrt(tot[0]);
+  } else if constexpr (STAGE == STAGE_DOT0) {
+    st->dots[0] = tot[0];
+  } else {
+    run_stage<STAGE>(st, tot);
+  }
+}
 
 // ---------------------------------------------------------------- SpMV -----
 // y = rowscale .* (A x)   (rowscale may be null), fused dots (w0, y) and, for
@@ -106,8 +118,8 @@ __global__ void __launch_bounds__(VB) k_spmv(int64_t nrows, const int32_t *__res
     }
   }
   if constexpr (NDOT > 0) {
-    double v[2] = {d0, d1}, tot[2];
-    if (reduce_last<2>(v, partials, &st->ticket, tot)) run_stage<STAGE>(st, tot);
+    double v[2] = {d0, d1};
+    write_partials<2>(v, partials);
   }
 }
 
@@ -134,8 +146,8 @@ __global__ void __launch_bounds__(VB) k_scaled_residual(int64_t nrows, const int
       d0 += v * v;
     }
   }
-  double v[1] = {d0}, tot[1];
-  if (reduce_last<1>(v, partials, &st->ticket, tot)) st->rr = tot[0];
+  double v[1] = {d0};
+  write_partials<1>(v, partials);
 }
 
 // ---------------------------------------------------------- BiCGStab -------
@@ -161,8 +173,8 @@ __global__ void __launch_bounds__(VB) k_bicg_s(int64_t n, const V *__restrict__ 
     s[i] = sv;
     d += (double)sv * (double)sv;
   }
-  double vv[1] = {d}, tot[1];
-  if (reduce_last<1>(vv, partials, &st->ticket, tot)) run_stage<STAGE_SNORM>(st, tot);
+  double vv[1] = {d};
+  write_partials<1>(vv, partials);
 }
 
 // x += alpha ph + omega sh ; r = s - omega t ; (r0, r), (r, r)
@@ -186,8 +198,8 @@ __global__ void __launch_bounds__(VB) k_bicg_xr(int64_t n, VXo *__restrict__ x, 
       d1 += (double)rv * (double)rv;
     }
   }
-  double vv[2] = {d0, d1}, tot[2];
-  if (reduce_last<2>(vv, partials, &st->ticket, tot)) run_stage<STAGE_XR>(st, tot);
+  double vv[2] = {d0, d1};
+  write_partials<2>(vv, partials);
 }
 
 // start: r = r0 = b (x = 0), (r, r) -> rho, rr; tolerance from ||b||
@@ -204,8 +216,8 @@ __global__ void __launch_bounds__(VB) k_bicg_init(int64_t n, const VB_ *__restri
     v[i] = (V)0;
     d += (double)bv * (double)bv;
   }
-  double vv[1] = {d}, tot[1];
-  if (reduce_last<1>(vv, partials, &st->ticket, tot)) run_stage<STAGE_INIT>(st, tot);
+  double vv[1] = {d};
+  write_partials<1>(vv, partials);
 }
 
 template <class T>
@@ -423,10 +435,8 @@ __global__ void __launch_bounds__(VB) k_multiaxpy(int64_t n, int nv, const V *__
     w[i] = wv;
     d += (double)wv * (double)wv;
   }
-  double vv[1] = {d}, tot[1];
-  if (reduce_last<1>(vv, partials, &st->ticket, tot)) {
-    if (final_pass) st->dots[0] = tot[0];
-  }
+  double vv[1] = {d};
+  write_partials<1>(vv, partials);
 }
 
 // x_out = alpha x_in (normalise the new Krylov vector)
@@ -508,12 +518,8 @@ __global__ void __launch_bounds__(VB) k_gmres_start_norm(int64_t n, const V *__r
   double d = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB)
     d += (double)r[i] * (double)r[i];
-  double vv[1] = {d}, tot[1];
-  if (reduce_last<1>(vv, partials, &st->ticket, tot)) {
-    st->rr = tot[0];
-    st->gm_inv = tot[0] > 0 ? 1.0 / sqrt(tot[0]) : 0.0;
-    st->gm_beta = sqrt(tot[0]);
-  }
+  double vv[1] = {d};
+  write_partials<1>(vv, partials);
 }
 
 __global__ void k_gmres_reset(int m, double *g, KState *st) {
@@ -559,45 +565,43 @@ __global__ void __launch_bounds__(VB) k_newton_update(int64_t n, double *__restr
     m = fmax(m, fabs(d));
     s += fabs(u);
   }
-  // max via a separate block reduction, sum via reduce_last
+  // block max and sum -> partials [sum | max]; k_newton_finalize combines them
   __shared__ double shm[16];
+  const double bs = block_sum(s, shm);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  __syncthreads();
   if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
   __syncthreads();
-  double bm = 0.0;
-  for (int k = 0; k < (int)(blockDim.x >> 6); k++) bm = fmax(bm, shm[k]);
-  if (threadIdx.x == 0) partials[(size_t)2 * gridDim.x + blockIdx.x] = bm;
-  // the max partials are published by the same fence / ticket as the sums
-  __shared__ double shx[16];
-  __shared__ bool lastb;
-  double ssum = block_sum(s, shx);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = ssum;
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
-    lastb = (t == gridDim.x - 1);
+    double bm = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) bm = fmax(bm, shm[k]);
+    partials[blockIdx.x] = bs;
+    partials[(size_t)gridDim.x + blockIdx.x] = bm;
   }
-  __syncthreads();
-  if (!lastb) return;
-  __threadfence();
+}
+
+// out[0] = max of the max partials, out[1] = sum of the sum partials (one block)
+__global__ void __launch_bounds__(VB) k_newton_finalize(const double *__restrict__ partials, int nparts,
+                                                        double *__restrict__ out_max, double *__restrict__ out_sum) {
+  const bool with_max = out_max != nullptr;
+  __shared__ double sh[16];
   double acc = 0.0, mx = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
     acc += partials[b];
-    mx = fmax(mx, partials[(size_t)2 * gridDim.x + b]);
+    if (with_max) mx = fmax(mx, partials[(size_t)nparts + b]);
   }
-  acc = block_sum(acc, shx);
+  acc = block_sum(acc, sh);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = mx;
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
   __syncthreads();
   if (threadIdx.x == 0) {
     double m2 = 0.0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); k++) m2 = fmax(m2, shm[k]);
-    out[0] = m2;
-    out[1] = acc;
-    atomicExch(ticket, 0u);
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) m2 = fmax(m2, sh[k]);
+    if (with_max) *out_max = m2;
+    *out_sum = acc;
   }
 }
 
@@ -606,8 +610,9 @@ __global__ void __launch_bounds__(VB) k_norm1(int64_t n, const double *__restric
                                               double *out) {
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB) s += fabs(U[i]);
-  double vv[1] = {s}, tot[1];
-  if (reduce_last<1>(vv, partials, ticket, tot)) out[0] = tot[0];
+  __shared__ double sh[16];
+  const double bs = block_sum(s, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
 // CSC gather: out[k] = J[perm[k]]
@@ -795,6 +800,8 @@ void LinearSolver::apply_op(const V *x, V *y, const V *w0, const V *w1, KState *
     case 16: launch(std::integral_constant<int, 16>()); break;
     default: launch(std::integral_constant<int, 32>()); break;
   }
+  if constexpr (NDOT > 0)
+    hipLaunchKernelGGL((k_finalize<2, STAGE>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p, (int)g, stp, 1);
 }
 
 void LinearSolver::apply_plain(bool fp32, const double *J, const void *x, void *y) {
@@ -828,6 +835,8 @@ int LinearSolver::bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_co
   unsigned g = grid_for(n);
   hipLaunchKernelGGL((k_fill<VXo>), dim3(g), dim3(VB), 0, stream, n, x, (VXo)0);
   hipLaunchKernelGGL((k_bicg_init<V, VBin>), dim3(g), dim3(VB), 0, stream, n, b, r, r0, p, v, partials.p, st.p);
+  hipLaunchKernelGGL((k_finalize<1, STAGE_INIT>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p, (int)g, st.p,
+                     0);
   // first chunk: about as many iterations as the previous solve needed (Newton steps of one
   // solve need similar counts), then poll every check_every iterations
   const int every = std::max(1, c.check_every);
@@ -839,10 +848,14 @@ int LinearSolver::bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_co
       const V *php = precond<V>(p, ph, st.p, 0, c);
       apply_op<V, 1, STAGE_ALPHA>(php, v, r0, (const V *)nullptr, st.p, J);
       hipLaunchKernelGGL((k_bicg_s<V>), dim3(g), dim3(VB), 0, stream, n, r, v, s, partials.p, st.p);
+      hipLaunchKernelGGL((k_finalize<1, STAGE_SNORM>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p, (int)g,
+                         st.p, 1);
       const V *shp = precond<V>(s, sh, st.p, 1, c);
       apply_op<V, 2, STAGE_OMEGA>(shp, t, s, (const V *)nullptr, st.p, J);
       hipLaunchKernelGGL((k_bicg_xr<V, VXo>), dim3(g), dim3(VB), 0, stream, n, x, php, shp, s, t, r, r0, partials.p,
                          st.p);
+      hipLaunchKernelGGL((k_finalize<2, STAGE_XR>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p, (int)g,
+                         st.p, 1);
     }
     VFVM_HIP_CHECK(hipGetLastError());
     launched += check;
@@ -879,6 +892,8 @@ int LinearSolver::gmres(const VBin *b, VXo *x, const double *J, const vfvm_contr
   hipLaunchKernelGGL((k_convert<V, VBin>), dim3(g), dim3(VB), 0, stream, n, b, bb);
   // tolerance from ||b||
   hipLaunchKernelGGL((k_gmres_start_norm<V>), dim3(g), dim3(VB), 0, stream, n, bb, partials.p, st.p);
+  hipLaunchKernelGGL((k_finalize<1, STAGE_GMSTART>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p, (int)g,
+                     st.p, 0);
   read_state(h);
   const double tol = atol + rtol * std::sqrt(h.rr);
   h.tol2 = tol * tol;
@@ -905,6 +920,8 @@ int LinearSolver::gmres(const VBin *b, VXo *x, const double *J, const vfvm_contr
       default: resid(std::integral_constant<int, 32>()); break;
     }
     hipLaunchKernelGGL((k_gmres_start_norm<V>), dim3(g), dim3(VB), 0, stream, n, r, partials.p, st.p);
+    hipLaunchKernelGGL((k_finalize<1, STAGE_GMSTART>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p,
+                       (int)g, st.p, 0);
     hipLaunchKernelGGL(k_gmres_reset, dim3(1), dim3(1), 0, stream, m, gmg.p, st.p);
     hipLaunchKernelGGL((k_gmres_normalize<V>), dim3(g), dim3(VB), 0, stream, n, r, Vb, st.p);
     read_state(h);
@@ -925,6 +942,9 @@ int LinearSolver::gmres(const VBin *b, VXo *x, const double *J, const vfvm_contr
         hipLaunchKernelGGL(k_multidot_finish, dim3(1), dim3(64), 0, stream, k + 1, (int)g, gmpart.p, hh, st.p);
         hipLaunchKernelGGL((k_multiaxpy<V>), dim3(g), dim3(VB), 0, stream, n, k + 1, Vb, n, hh, w, partials.p, st.p,
                            pass);
+        if (pass == 1)
+          hipLaunchKernelGGL((k_finalize<1, STAGE_DOT0>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p,
+                             (int)g, st.p, 1);
       }
       hipLaunchKernelGGL(k_gmres_givens, dim3(1), dim3(1), 0, stream, k, m, gmH.p, gmh1.p, gmh2.p, gmcs.p, gmsn.p,
                          gmg.p, st.p);
@@ -969,8 +989,11 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
   auto scaled_res = [&](const double *xx, double *out) {
     auto go = [&](auto gconst) {
       constexpr int GG = decltype(gconst)::value;
-      hipLaunchKernelGGL((k_scaled_residual<GG>), dim3(grid_for(n * GG)), dim3(VB), 0, stream, n, L.rowptr, L.colind,
-                         J, xx, b, (const double *)rowscale.p, out, partials.p, st.p);
+      const unsigned gg = grid_for(n * GG);
+      hipLaunchKernelGGL((k_scaled_residual<GG>), dim3(gg), dim3(VB), 0, stream, n, L.rowptr, L.colind, J, xx, b,
+                         (const double *)rowscale.p, out, partials.p, st.p);
+      hipLaunchKernelGGL((k_finalize<1, STAGE_RR>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p, (int)gg,
+                         st.p, 0);
     };
     switch (Gs) {
       case 2: go(std::integral_constant<int, 2>()); break;
@@ -1037,11 +1060,16 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
 // ---- small utilities used by the Newton driver ----
 void launch_newton_update(int64_t n, double *U, const double *dU, double damp, double *partials, unsigned *ticket,
                           double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_newton_update, dim3(grid_for(n)), dim3(VB), 0, s, n, U, dU, damp, partials, ticket, out);
+  const unsigned g = grid_for(n);
+  hipLaunchKernelGGL(k_newton_update, dim3(g), dim3(VB), 0, s, n, U, dU, damp, partials, ticket, out);
+  hipLaunchKernelGGL(k_newton_finalize, dim3(1), dim3(VB), 0, s, (const double *)partials, (int)g, out, out + 1);
   VFVM_HIP_CHECK(hipGetLastError());
 }
 void launch_norm1(int64_t n, const double *U, double *partials, unsigned *ticket, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_norm1, dim3(grid_for(n)), dim3(VB), 0, s, n, U, partials, ticket, out);
+  const unsigned g = grid_for(n);
+  hipLaunchKernelGGL(k_norm1, dim3(g), dim3(VB), 0, s, n, U, partials, ticket, out);
+  hipLaunchKernelGGL(k_newton_finalize, dim3(1), dim3(VB), 0, s, (const double *)partials, (int)g, (double *)nullptr,
+                     out);
   VFVM_HIP_CHECK(hipGetLastError());
 }
 void launch_gather(int64_t n, const int32_t *perm, const double *J, double *out, hipStream_t s) {
@@ -1058,6 +1086,26 @@ void launch_fill_i32(int64_t n, int32_t *p, int32_t v, hipStream_t s) {
 }
 void launch_copy(int64_t n, const double *a, double *b, hipStream_t s) {
   hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_for(n)), dim3(VB), 0, s, n, a, b);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_scale_rows(int nspec, int64_t nnodes, const int32_t *rowptr, const int32_t *colind, const double *A,
+                       const int32_t *dofdiag, double *rowscale, float *A32, double *binv, hipStream_t s) {
+  unsigned g = grid_for(nnodes);
+  switch (nspec) {
+    case 1:
+      hipLaunchKernelGGL((k_scale_rows<1>), dim3(g), dim3(VB), 0, s, nnodes, rowptr, colind, A, dofdiag, rowscale, A32,
+                         binv);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_scale_rows<2>), dim3(g), dim3(VB), 0, s, nnodes, rowptr, colind, A, dofdiag, rowscale, A32,
+                         binv);
+      break;
+    default:
+      hipLaunchKernelGGL((k_scale_rows<3>), dim3(g), dim3(VB), 0, s, nnodes, rowptr, colind, A, dofdiag, rowscale, A32,
+                         binv);
+      break;
+  }
   VFVM_HIP_CHECK(hipGetLastError());
 }
 

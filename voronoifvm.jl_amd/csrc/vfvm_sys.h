@@ -1,0 +1,37 @@
+// Definition of the system handle, shared by the C-ABI and the batch translation units.
+#pragma once
+
+#include "vfvm_common.h"
+#include "vfvm_linalg.h"
+
+using namespace vfvm;
+
+struct vfvm_sys {
+  vfvm_ctx *ctx = nullptr;
+  int dim = 0, nspec = 1, nbreg = 0;
+  int64_t nnodes = 0, ndof = 0, nnz = 0;
+  vfvm_physics_t phys{};
+  std::vector<vfvm_bc_t> bc;
+  Masks masks{};
+  HostSetup hs;
+  std::vector<int32_t> rowptr, colind, csc_perm, csc_rowval, dofdiag;
+  std::vector<int64_t> csc_colptr;
+  // device data
+  DevBuf<int32_t> d_adj_ptr, d_adj_col, d_adj_diag, d_bn_ptr, d_bn_reg, d_bnodes, d_rowptr, d_colind, d_csc_perm,
+      d_dofdiag, d_flags;
+  DevBuf<double> d_adj_fac, d_nodevol, d_bn_fac, d_U, d_Uold, d_F, d_J, d_src, d_dU, d_tmp, d_red, d_partials;
+  DevBuf<unsigned> d_ticket;
+  DevBuf<InstParams> d_inst;
+  bool has_src = false;
+  // multicolour ILU(0) structure
+  IluStruct ilu;
+  bool ilu_built = false;
+  DevBuf<int32_t> d_ilu_rowptr, d_ilu_colind, d_ilu_diag, d_ilu_perm, d_ilu_src, d_ilu_srcrow;
+  LinearSolver lin;
+  int asm_group = 8, spmv_group = 8;
+  double *h_pinned = nullptr;  // small pinned readback area
+};
+
+namespace vfvm {
+InstParams make_inst(const vfvm_sys &s);
+}

@@ -9,7 +9,7 @@ from .grids import Grid, linspace, simplexgrid
 from .physics import (Dirichlet, DriftDiffusionFlux, DriftDiffusionReaction, ExchangeReaction, IdentityStorage,
                       LinearFlux, Neumann, NonlinearDiffusionFlux, Physics, PowerFlux, PowerReaction, Robin,
                       SinhReaction)
-from .system import (NewtonSolverHistory, System, boundary_dirichlet, boundary_neumann, boundary_robin, dofs,
+from .system import (Batch, NewtonSolverHistory, System, boundary_dirichlet, boundary_neumann, boundary_robin, dofs,
                      enable_species, evaluate_residual_and_jacobian, integrate, solve, unknowns)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

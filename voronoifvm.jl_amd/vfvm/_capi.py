@@ -74,6 +74,11 @@ SIGNATURES = {
     "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "vfvm_pattern_csr": (C.c_int, [_vp, _ip, _ip]),
+    "vfvm_batch_create": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(_vp)]),
+    "vfvm_batch_destroy": (C.c_int, [_vp]),
+    "vfvm_batch_autotune": (C.c_int, [_vp, _ip, _ip]),
+    "vfvm_batch_newton": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp), C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(ControlT), C.POINTER(HistoryT)]),
     "vfvm_assemble_device": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp, _vp]),
 }
 

@@ -303,3 +303,51 @@ def boundary_neumann(sys, ispec, ibc, v):
 
 def boundary_robin(sys, ispec, ibc, alpha, v):
     sys.boundary_robin(ispec, ibc, alpha, v)
+
+
+class Batch:
+    """Many independent systems solved together (vfvm_batch_*): the reference's
+    one-SystemState-per-thread pattern (examples/Example440_ParallelState.jl:36-45)
+    packed into single device launches.  Members must share species count and
+    physics kinds; parameters, sources and boundary values may differ."""
+
+    def __init__(self, systems, autotune=True):
+        self.systems = list(systems)
+        self.ctx = self.systems[0].ctx
+        L = _capi.lib()
+        hs = (C.c_void_p * len(self.systems))(*[s.handle() for s in self.systems])
+        h = C.c_void_p()
+        _capi.check(L.vfvm_batch_create(self.ctx.handle, len(self.systems), hs, C.byref(h)), self.ctx.handle)
+        self._h = h
+        self.groups = None
+        if autotune:
+            a, g = C.c_int32(), C.c_int32()
+            _capi.check(L.vfvm_batch_autotune(h, C.byref(a), C.byref(g)), self.ctx.handle)
+            self.groups = (a.value, g.value)
+        self.histories = []
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _capi._LIB is not None:
+            _capi._LIB.vfvm_batch_destroy(self._h)
+            self._h = None
+
+    def solve(self, inivals=0.0, control: Optional[SolverControl] = None, time=0.0, tstep=math.inf, embed=0.0,
+              raise_on_failure=True):
+        """solve_step! for every member; returns the list of solutions (nspec x nnodes)."""
+        c = SolverControl() if control is None else control
+        n = len(self.systems)
+        if np.isscalar(inivals):
+            inivals = [unknowns(s, inivals) for s in self.systems]
+        U0 = [np.ascontiguousarray(_as_dofs(s, u)) for s, u in zip(self.systems, inivals)]
+        U = [np.zeros_like(u) for u in U0]
+        pu = (_capi._dp * n)(*[_capi.dptr(u) for u in U])
+        p0 = (_capi._dp * n)(*[_capi.dptr(u) for u in U0])
+        hist = (_capi.HistoryT * n)()
+        cc = c.to_c(self.systems[0].is_linear)
+        rc = _capi.lib().vfvm_batch_newton(self._h, pu, p0, time, tstep, embed, C.byref(cc), hist)
+        self.histories = [NewtonSolverHistory(nlin=h.nlin, niter=h.niter, time=h.time,
+                                              updatenorm=tuple(h.updatenorm[:min(h.niter, 64)]),
+                                              residual_norm=h.residual_norm) for h in hist]
+        if raise_on_failure:
+            _capi.check(rc, self.ctx.handle)
+        return [_from_dofs(s, u) for s, u in zip(self.systems, U)]
