@@ -218,6 +218,9 @@ int vfvm_assemble_device(vfvm_sys *sys, const double *dU, const double *dUold, d
 typedef struct vfvm_batch vfvm_batch;
 int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfvm_batch **out);
 int vfvm_batch_destroy(vfvm_batch *b);
+/* replace member m's source (nspec x nnodes; NULL keeps it) and boundary table (NULL keeps
+ * it) inside the batch, for repeated solves of one grid with new data */
+int vfvm_batch_update_member(vfvm_batch *b, int64_t m, const double *src, const vfvm_bc_t *bc);
 /* project-written autotuner for the batch's assembly and SpMV group sizes */
 int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group);
 /* solve_step! for every member (src/vfvm_solver.jl:13-222): U, Uold are nsys host pointers to
@@ -227,6 +230,29 @@ int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group);
 int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold, double time,
                       double tstep, double lambda, const vfvm_control_t *control,
                       vfvm_history_t *hist);
+
+/* ---- learned Newton initial guess (no counterpart in the reference) ----
+ * A message-passing network on the Voronoi edge graph of `sys` (edge weights sigma/h,
+ * normalised per node) predicts the Newton initial iterate from per-node features.
+ * Dense layers run on the matrix cores in bfloat16 with float32 accumulation; training and
+ * inference use the library's own forward and backward kernels.  Newton always finishes the
+ * solve, so the plain method stays the fallback and the judge of the prediction.
+ * features: nsamples x nnodes x 8 float32 (sample-major, then node, then feature);
+ * targets: nsamples x nnodes float32.  width must be 64. */
+#define VFVM_MPNN_NFEAT 8
+typedef struct vfvm_mpnn vfvm_mpnn;
+int vfvm_mpnn_create(vfvm_sys *sys, int nlayers, int width, uint64_t seed, vfvm_mpnn **out);
+int vfvm_mpnn_destroy(vfvm_mpnn *m);
+int vfvm_mpnn_params(vfvm_mpnn *m, int64_t *nparams, float *params /* may be NULL */);
+int vfvm_mpnn_set_params(vfvm_mpnn *m, const float *params);
+/* Adam on the mean squared error, mini-batches of `batch` samples, `epochs` passes */
+int vfvm_mpnn_train(vfvm_mpnn *m, int64_t nsamples, const float *features, const float *targets,
+                    int64_t epochs, int64_t batch, float lr, float *final_loss);
+/* out: nsamples x nnodes predictions */
+int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, double *out);
+/* loss and gradient for one batch (checked against the CPU oracle) */
+int vfvm_mpnn_loss_grad(vfvm_mpnn *m, int64_t nsamples, const float *features, const float *targets,
+                        float *loss, float *grad);
 
 #ifdef __cplusplus
 }

@@ -712,6 +712,31 @@ int vfvm_batch_destroy(vfvm_batch *b) {
   return VFVM_OK;
 }
 
+// replace member m's source (nspec x nnodes, may be NULL to keep) and BC table (may be NULL)
+int vfvm_batch_update_member(vfvm_batch *b, int64_t m, const double *src, const vfvm_bc_t *bc) {
+  if (!b || m < 0 || m >= b->nsys) return VFVM_ERR_ARG;
+  return bguard(b->ctx, [&]() {
+    hipStream_t st = b->ctx->stream;
+    vfvm_sys *s = b->members[m];
+    if (src) {
+      if (!b->has_src) {
+        b->src.alloc((size_t)b->ndof);
+        hipLaunchKernelGGL((k_bfill<double>), dim3(gfor(b->ndof)), dim3(BB), 0, st, b->ndof, b->src.p, 0.0);
+        b->has_src = true;
+      }
+      VFVM_HIP_CHECK(hipMemcpyAsync(b->src.p + b->dof_off[m], src, sizeof(double) * s->ndof, hipMemcpyHostToDevice, st));
+    }
+    if (bc) {
+      std::copy(bc, bc + (size_t)s->nbreg * s->nspec, s->bc.begin());
+      InstParams P = make_inst(*s);
+      VFVM_HIP_CHECK(hipMemcpyAsync(b->inst.p + m, &P, sizeof(InstParams), hipMemcpyHostToDevice, st));
+      VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return VFVM_OK;
+  });
+}
+
 // project-written autotuner: time the batched Jacobian assembly and SpMV for every
 // legal group size (lanes per node / per row) and keep the fastest
 int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group) {

@@ -1,0 +1,687 @@
+// Learned Newton initial guess: a message-passing network on the Voronoi edge
+// graph of a system, with hand-written forward and backward kernels.  Dense
+// layers run on the matrix cores (v_mfma_f32_16x16x16_bf16: bfloat16 operands,
+// float32 accumulation); aggregation, activations, loss and Adam are plain
+// kernels.  The network (row r = sample s, node i of the system's grid):
+//
+//   x_r   : 8 input features, zero padded to 16
+//   h0    = relu(bf(x) We + be)                                   64 wide
+//   l = 0..L-1:
+//     a_l = sum_j w_ij h_l[j] - h_l[i]     w_ij = sigma_ij/h_ij normalised per row
+//     c_l = [bf(h_l), bf(a_l)]                                    128 wide, bfloat16
+//     r_l = relu(c_l W_l + b_l),  h_{l+1} = h_l + r_l
+//   u     = h_L wd + bd
+//
+// Parameters live in one float32 vector (layout: We 16x64, be, [W_l 128x64, b_l]*L,
+// wd 64, bd) with bfloat16 copies (W_l and its transpose) refreshed after every
+// optimiser step.  Training minimises the mean squared error to converged Newton
+// solutions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "vfvm_sys.h"
+
+using namespace vfvm;
+
+namespace {
+
+constexpr int NFEAT = 8;   // host-side input features
+constexpr int KPAD = 16;   // padded input width (one MFMA k-step)
+constexpr int WID = 64;    // hidden width
+constexpr int CW = 2 * WID;  // concatenated width
+constexpr int TB = 256;    // threads per block (4 waves)
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16_t;
+
+__device__ __forceinline__ bf16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                                          // round to nearest even
+  return (bf16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+// ---------------------------------------------------------------- GEMM -----
+// C[M x ND] = A[M x KD] * B[KD x ND], A and B bfloat16 row-major, float32
+// accumulation on the matrix cores.  Block = 4 waves = 64 rows; wave w owns
+// rows 16w..16w+15 and all ND columns (ND/16 accumulator tiles).  Both
+// operands are staged in LDS (A rows padded by 8 to spread banks).
+// Operand layout of v_mfma_f32_16x16x16_bf16 for lane l (g = l / 16, c = l % 16):
+//   A: A[c][4g + j], B: B[4g + j][c], D: D[4g + j][c], j = 0..3.
+enum { EPI_ENC = 0, EPI_RES = 1, EPI_DCAT = 2 };
+
+struct GemmEpi {
+  const float *bias;  // ND (ENC, RES)
+  float *hf;          // rows x 64 hidden state (ENC: written, RES: updated in place)
+  float *rbuf;        // rows x 64 relu(z) kept for the backward pass (RES)
+  bf16_t *cat;        // rows x 128: ENC/RES write bf(h) into cols 0..63 of the next layer input
+  float *dh;          // DCAT: dh += dC[:, 0:64]
+  float *da;          // DCAT: da  = dC[:, 64:128]
+};
+
+template <int KD, int ND, int EPI>
+__global__ void __launch_bounds__(TB) k_gemm(int64_t M, const bf16_t *__restrict__ A, const bf16_t *__restrict__ B,
+                                             GemmEpi e) {
+  constexpr int ASTR = KD + 8;
+  __shared__ bf16_t As[64 * ASTR];
+  __shared__ bf16_t Bs[KD * ND];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, g = l >> 4, c = l & 15;
+  const int64_t row0 = (int64_t)blockIdx.x * 64;
+  for (int q = tid; q < KD * ND / 4; q += TB) {  // B: 4 bf16 per load
+    const uint64_t v = reinterpret_cast<const uint64_t *>(B)[q];
+    reinterpret_cast<uint64_t *>(Bs)[q] = v;
+  }
+  for (int q = tid; q < 64 * KD / 4; q += TB) {  // A tile rows, zero beyond M
+    const int r = q / (KD / 4), kq = q % (KD / 4);
+    uint64_t v = 0;
+    if (row0 + r < M) v = reinterpret_cast<const uint64_t *>(A + (row0 + r) * KD)[kq];
+    *reinterpret_cast<uint64_t *>(&As[r * ASTR + kq * 4]) = v;
+  }
+  __syncthreads();
+  f32x4 acc[ND / 16];
+#pragma unroll
+  for (int t = 0; t < ND / 16; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k0 = 0; k0 < KD; k0 += 16) {
+    bf16x4 a;
+    const bf16_t *ap = &As[(w * 16 + c) * ASTR + k0 + 4 * g];
+#pragma unroll
+    for (int j = 0; j < 4; j++) a[j] = (short)ap[j];
+#pragma unroll
+    for (int t = 0; t < ND / 16; t++) {
+      bf16x4 b;
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = (short)Bs[(k0 + 4 * g + j) * ND + t * 16 + c];
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < ND / 16; t++) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int64_t row = row0 + w * 16 + 4 * g + j;
+      const int col = t * 16 + c;
+      if (row >= M) continue;
+      const float v = acc[t][j];
+      if constexpr (EPI == EPI_ENC) {
+        const float h = fmaxf(v + e.bias[col], 0.f);
+        e.hf[row * WID + col] = h;
+        e.cat[row * CW + col] = f2bf(h);
+      } else if constexpr (EPI == EPI_RES) {
+        const float r = fmaxf(v + e.bias[col], 0.f);
+        e.rbuf[row * WID + col] = r;
+        const float hn = e.hf[row * WID + col] + r;
+        e.hf[row * WID + col] = hn;
+        if (e.cat) e.cat[row * CW + col] = f2bf(hn);
+      } else {
+        if (col < WID)
+          e.dh[row * WID + col] += v;
+        else
+          e.da[row * WID + col - WID] = v;
+      }
+    }
+  }
+}
+
+// Weight gradient: P_b[KD x 64] = sum over the block's rows of A[rows x KD]^T Z[rows x 64],
+// A and Z bfloat16.  Reduction over rows on the matrix cores (16 rows per step):
+// A-operand A'[k][m] = A[m][k] (m = row), B-operand Z[m][n].  Output tiles
+// (KD/16) x 4 are split over the 4 waves.
+template <int KD>
+__global__ void __launch_bounds__(TB) k_wgrad(int64_t M, int64_t rows_per_block, const bf16_t *__restrict__ A,
+                                              const bf16_t *__restrict__ Z, float *__restrict__ partials) {
+  constexpr int NT = (KD / 16) * (WID / 16);
+  constexpr int PER = (NT + 3) / 4;
+  constexpr int ASTR = KD + 8, ZSTR = WID + 8;
+  __shared__ bf16_t As[64 * ASTR];
+  __shared__ bf16_t Zs[64 * ZSTR];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, g = l >> 4, c = l & 15;
+  f32x4 acc[PER];
+#pragma unroll
+  for (int p = 0; p < PER; p++) acc[p] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int64_t r_begin = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r_end = std::min<int64_t>(M, r_begin + rows_per_block);
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += 64) {
+    __syncthreads();
+    for (int q = tid; q < 64 * KD / 4; q += TB) {
+      const int r = q / (KD / 4), kq = q % (KD / 4);
+      uint64_t v = 0;
+      if (r0 + r < r_end) v = reinterpret_cast<const uint64_t *>(A + (r0 + r) * KD)[kq];
+      *reinterpret_cast<uint64_t *>(&As[r * ASTR + kq * 4]) = v;
+    }
+    for (int q = tid; q < 64 * WID / 4; q += TB) {
+      const int r = q / (WID / 4), kq = q % (WID / 4);
+      uint64_t v = 0;
+      if (r0 + r < r_end) v = reinterpret_cast<const uint64_t *>(Z + (r0 + r) * WID)[kq];
+      *reinterpret_cast<uint64_t *>(&Zs[r * ZSTR + kq * 4]) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m0 = 0; m0 < 64; m0 += 16) {
+#pragma unroll
+      for (int p = 0; p < PER; p++) {
+        const int tile = w * PER + p;
+        if (tile >= NT) break;
+        const int kt = tile / (WID / 16), nt = tile % (WID / 16);
+        bf16x4 a, b;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          a[j] = (short)As[(m0 + 4 * g + j) * ASTR + kt * 16 + c];
+          b[j] = (short)Zs[(m0 + 4 * g + j) * ZSTR + nt * 16 + c];
+        }
+        acc[p] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[p], 0, 0, 0);
+      }
+    }
+  }
+  float *out = partials + (size_t)blockIdx.x * KD * WID;
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int tile = w * PER + p;
+    if (tile >= NT) break;
+    const int kt = tile / (WID / 16), nt = tile % (WID / 16);
+#pragma unroll
+    for (int j = 0; j < 4; j++) out[(kt * 16 + 4 * g + j) * WID + nt * 16 + c] = acc[p][j];
+  }
+}
+
+// grad[k] = sum over blocks of partials[b][k]
+__global__ void k_sum_partials(int nblocks, int len, const float *__restrict__ partials, float *__restrict__ grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= len) return;
+  float acc = 0.f;
+  for (int b = 0; b < nblocks; b++) acc += partials[(size_t)b * len + k];
+  grad[k] = acc;
+}
+
+// ------------------------------------------------------------ graph ops ----
+// a_i = sum_j w_ij h_j - h_i for row r = s*n + i; one wave (64 features) per row
+__global__ void __launch_bounds__(TB) k_aggregate(int64_t rows, int64_t n, const int32_t *__restrict__ gptr,
+                                                  const int32_t *__restrict__ gcol, const float *__restrict__ gw,
+                                                  const float *__restrict__ hf, bf16_t *__restrict__ cat) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int f = threadIdx.x & 63;
+  if (row >= rows) return;
+  const int64_t s = row / n, i = row % n;
+  float acc = 0.f;
+  for (int32_t k = gptr[i]; k < gptr[i + 1]; k++) acc += gw[k] * hf[(s * n + gcol[k]) * WID + f];
+  acc -= hf[row * WID + f];
+  cat[row * CW + WID + f] = f2bf(acc);
+}
+
+// dh_k += sum_{i in N(k)} w_ik da_i - da_k  (transpose weights gwt on k's entries)
+__global__ void __launch_bounds__(TB) k_aggregate_t(int64_t rows, int64_t n, const int32_t *__restrict__ gptr,
+                                                    const int32_t *__restrict__ gcol, const float *__restrict__ gwt,
+                                                    const float *__restrict__ da, float *__restrict__ dh) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int f = threadIdx.x & 63;
+  if (row >= rows) return;
+  const int64_t s = row / n, k = row % n;
+  float acc = 0.f;
+  for (int32_t q = gptr[k]; q < gptr[k + 1]; q++) acc += gwt[q] * da[(s * n + gcol[q]) * WID + f];
+  dh[row * WID + f] += acc - da[row * WID + f];
+}
+
+// x (rows x 8 float) -> bfloat16 rows x 16 (zero padded)
+__global__ void k_pad_features(int64_t rows, const float *__restrict__ x, bf16_t *__restrict__ xb) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= rows * KPAD) return;
+  const int64_t r = q / KPAD;
+  const int k = (int)(q % KPAD);
+  xb[q] = k < NFEAT ? f2bf(x[r * NFEAT + k]) : (bf16_t)0;
+}
+
+// u_r = h_L[r] . wd + bd (one wave per row); optional loss: du = 2 (u - t) / count,
+// dh = du wd, per-block partials of sum (u-t)^2, sum du h (64) and sum du
+__global__ void __launch_bounds__(TB) k_decode(int64_t rows, const float *__restrict__ hf, const float *__restrict__ wd,
+                                               const float *__restrict__ bd, float *__restrict__ u,
+                                               const float *__restrict__ target, float inv_count,
+                                               float *__restrict__ dh, float *__restrict__ part) {
+  __shared__ float sh_loss[4], sh_du[4];
+  __shared__ float sh_g[4][WID];
+  const int w = threadIdx.x >> 6, f = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + w;
+  float hv = 0.f, du = 0.f, lo = 0.f;
+  if (row < rows) {
+    hv = hf[row * WID + f];
+    float v = hv * wd[f];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const float uu = v + bd[0];
+    if (f == 0) u[row] = uu;
+    if (target) {
+      const float d = uu - target[row];
+      du = 2.f * d * inv_count;
+      lo = d * d;
+      dh[row * WID + f] = du * wd[f];
+    }
+  }
+  if (!target) return;
+  sh_g[w][f] = du * hv;
+  if (f == 0) {
+    sh_loss[w] = lo;
+    sh_du[w] = du;
+  }
+  __syncthreads();
+  if (w == 0) {
+    float gsum = sh_g[0][f] + sh_g[1][f] + sh_g[2][f] + sh_g[3][f];
+    float *pb = part + (size_t)blockIdx.x * (WID + 2);
+    pb[f] = gsum;
+    if (f == 0) {
+      pb[WID] = sh_loss[0] + sh_loss[1] + sh_loss[2] + sh_loss[3];
+      pb[WID + 1] = sh_du[0] + sh_du[1] + sh_du[2] + sh_du[3];
+    }
+  }
+}
+
+// dz = dh * (gate > 0) -> bfloat16; per-block column sums of dz (bias gradient)
+// gate is r_l (float) or, for the encoder, bf(h0) in the first half of cat_0
+template <bool GATE_BF>
+__global__ void __launch_bounds__(TB) k_gate(int64_t rows, const float *__restrict__ dh, const float *__restrict__ gate,
+                                             const bf16_t *__restrict__ gate_bf, bf16_t *__restrict__ dzb,
+                                             float *__restrict__ part, int rows_per_block) {
+  __shared__ float sh[4][WID];
+  const int w = threadIdx.x >> 6, f = threadIdx.x & 63;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = std::min<int64_t>(rows, r0 + rows_per_block);
+  float acc = 0.f;
+  for (int64_t row = r0 + w; row < r1; row += 4) {
+    const bool on = GATE_BF ? (bf2f(gate_bf[row * CW + f]) > 0.f) : (gate[row * WID + f] > 0.f);
+    const float dz = on ? dh[row * WID + f] : 0.f;
+    dzb[row * WID + f] = f2bf(dz);
+    acc += dz;
+  }
+  sh[w][f] = acc;
+  __syncthreads();
+  if (w == 0) part[(size_t)blockIdx.x * WID + f] = sh[0][f] + sh[1][f] + sh[2][f] + sh[3][f];
+}
+
+// Adam on the float32 parameters (t = step count, bias-corrected)
+__global__ void k_adam(int64_t n, float *__restrict__ theta, const float *__restrict__ grad, float *__restrict__ m,
+                       float *__restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float g = grad[k];
+  const float mk = b1 * m[k] + (1.f - b1) * g;
+  const float vk = b2 * v[k] + (1.f - b2) * g * g;
+  m[k] = mk;
+  v[k] = vk;
+  theta[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+}
+
+// bfloat16 copies: W (K x N) as is and transposed
+__global__ void k_cast(int K, int N, const float *__restrict__ W, bf16_t *__restrict__ Wb, bf16_t *__restrict__ WTb) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= K * N) return;
+  const int k = q / N, j = q % N;
+  const bf16_t b = f2bf(W[q]);
+  Wb[q] = b;
+  if (WTb) WTb[j * K + k] = b;
+}
+
+__global__ void k_fill_f(int64_t n, float *p, float v) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+}  // namespace
+
+struct vfvm_mpnn {
+  vfvm_ctx *ctx = nullptr;
+  int nlayers = 4;
+  int64_t n = 0;  // nodes per sample
+  int64_t np = 0;
+  // graph
+  DevBuf<int32_t> gptr, gcol;
+  DevBuf<float> gw, gwt;
+  // parameters
+  DevBuf<float> theta, grad, adam_m, adam_v;
+  int64_t adam_t = 0;
+  DevBuf<bf16_t> Web, Wb, WTb;  // encoder; layers (L x 128 x 64) and transposes (L x 64 x 128)
+  // activations for a batch of `cap` rows
+  int64_t cap = 0;
+  DevBuf<float> x, hf, rbuf, dh, da, u, target, part;
+  DevBuf<bf16_t> xb, cat, dzb;
+  size_t off_We = 0, off_be = 0, off_wd = 0, off_bd = 0;
+  std::vector<size_t> off_W, off_b;
+};
+
+namespace {
+
+template <class F>
+int mguard(vfvm_ctx *ctx, F &&f) {
+  try {
+    return f();
+  } catch (const Error &e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    if (ctx) ctx->err = e.what();
+    return VFVM_ERR_HIP;
+  }
+}
+
+inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+void refresh_bf16(vfvm_mpnn &m) {
+  hipStream_t st = m.ctx->stream;
+  hipLaunchKernelGGL(k_cast, dim3(nb(KPAD * WID, 256)), dim3(256), 0, st, KPAD, WID, (const float *)m.theta.p + m.off_We,
+                     m.Web.p, (bf16_t *)nullptr);
+  for (int l = 0; l < m.nlayers; l++)
+    hipLaunchKernelGGL(k_cast, dim3(nb(CW * WID, 256)), dim3(256), 0, st, CW, WID,
+                       (const float *)m.theta.p + m.off_W[l], m.Wb.p + (size_t)l * CW * WID,
+                       m.WTb.p + (size_t)l * CW * WID);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+constexpr int64_t WG_ROWS = 4096;  // rows per block of the weight-gradient kernels
+constexpr int GATE_ROWS = 256;     // rows per block of the gate kernel
+
+void ensure_cap(vfvm_mpnn &m, int64_t rows) {
+  if (rows <= m.cap) return;
+  const int L = m.nlayers;
+  m.x.alloc((size_t)rows * NFEAT);
+  m.xb.alloc((size_t)rows * KPAD);
+  m.hf.alloc((size_t)rows * WID);
+  m.rbuf.alloc((size_t)rows * WID * L);
+  m.dh.alloc((size_t)rows * WID);
+  m.da.alloc((size_t)rows * WID);
+  m.u.alloc((size_t)rows);
+  m.target.alloc((size_t)rows);
+  m.cat.alloc((size_t)rows * CW * (L + 1));
+  m.dzb.alloc((size_t)rows * WID);
+  const int64_t p1 = nb(rows, 4) * (int64_t)(WID + 2);
+  const int64_t p2 = (int64_t)nb(rows, WG_ROWS) * CW * WID;
+  const int64_t p3 = (int64_t)nb(rows, GATE_ROWS) * WID;
+  m.part.alloc((size_t)std::max({p1, p2, p3}));
+  m.cap = rows;
+}
+
+// forward pass on `rows` rows already in m.x; keeps what the backward pass needs
+void forward(vfvm_mpnn &m, int64_t rows) {
+  hipStream_t st = m.ctx->stream;
+  const int L = m.nlayers;
+  hipLaunchKernelGGL(k_pad_features, dim3(nb(rows * KPAD, 256)), dim3(256), 0, st, rows, (const float *)m.x.p, m.xb.p);
+  GemmEpi e{};
+  e.bias = m.theta.p + m.off_be;
+  e.hf = m.hf.p;
+  e.cat = m.cat.p;  // cat_0 first half = bf(h0)
+  hipLaunchKernelGGL((k_gemm<KPAD, WID, EPI_ENC>), dim3(nb(rows, 64)), dim3(TB), 0, st, rows, (const bf16_t *)m.xb.p,
+                     (const bf16_t *)m.Web.p, e);
+  for (int l = 0; l < L; l++) {
+    bf16_t *cl = m.cat.p + (size_t)l * rows * CW;
+    hipLaunchKernelGGL(k_aggregate, dim3(nb(rows, 4)), dim3(TB), 0, st, rows, m.n, (const int32_t *)m.gptr.p,
+                       (const int32_t *)m.gcol.p, (const float *)m.gw.p, (const float *)m.hf.p, cl);
+    GemmEpi r{};
+    r.bias = m.theta.p + m.off_b[l];
+    r.hf = m.hf.p;
+    r.rbuf = m.rbuf.p + (size_t)l * rows * WID;
+    r.cat = m.cat.p + (size_t)(l + 1) * rows * CW;  // next layer's h half (cat_L unused by the decoder)
+    hipLaunchKernelGGL((k_gemm<CW, WID, EPI_RES>), dim3(nb(rows, 64)), dim3(TB), 0, st, rows, (const bf16_t *)cl,
+                       (const bf16_t *)(m.Wb.p + (size_t)l * CW * WID), r);
+  }
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+// loss and full gradient (into m.grad) for `rows` rows with targets in m.target
+float backward(vfvm_mpnn &m, int64_t rows) {
+  hipStream_t st = m.ctx->stream;
+  const int L = m.nlayers;
+  const unsigned dblk = nb(rows, 4);
+  hipLaunchKernelGGL(k_decode, dim3(dblk), dim3(TB), 0, st, rows, (const float *)m.hf.p,
+                     (const float *)m.theta.p + m.off_wd, (const float *)m.theta.p + m.off_bd, m.u.p,
+                     (const float *)m.target.p, 1.0f / (float)rows, m.dh.p, m.part.p);
+  // decoder gradient and loss from the per-block partials (64 + 2 per block)
+  std::vector<float> hpart((size_t)dblk * (WID + 2));
+  VFVM_HIP_CHECK(hipMemcpyAsync(hpart.data(), m.part.p, hpart.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  VFVM_HIP_CHECK(hipStreamSynchronize(st));
+  std::vector<double> gwd(WID, 0.0);
+  double loss = 0.0, gbd = 0.0;
+  for (unsigned b = 0; b < dblk; b++) {
+    for (int f = 0; f < WID; f++) gwd[f] += hpart[(size_t)b * (WID + 2) + f];
+    loss += hpart[(size_t)b * (WID + 2) + WID];
+    gbd += hpart[(size_t)b * (WID + 2) + WID + 1];
+  }
+  std::vector<float> gdec(WID + 1);
+  for (int f = 0; f < WID; f++) gdec[f] = (float)gwd[f];
+  gdec[WID] = (float)gbd;
+  VFVM_HIP_CHECK(hipMemcpyAsync(m.grad.p + m.off_wd, gdec.data(), (WID + 1) * sizeof(float), hipMemcpyHostToDevice, st));
+  VFVM_HIP_CHECK(hipStreamSynchronize(st));  // gdec is a local
+  const unsigned wg_blocks = nb(rows, WG_ROWS);
+  const unsigned gate_blocks = nb(rows, GATE_ROWS);
+  for (int l = L - 1; l >= 0; l--) {
+    const bf16_t *cl = m.cat.p + (size_t)l * rows * CW;
+    hipLaunchKernelGGL((k_gate<false>), dim3(gate_blocks), dim3(TB), 0, st, rows, (const float *)m.dh.p,
+                       (const float *)m.rbuf.p + (size_t)l * rows * WID, (const bf16_t *)nullptr, m.dzb.p, m.part.p,
+                       GATE_ROWS);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(WID), 0, st, (int)gate_blocks, WID, (const float *)m.part.p,
+                       m.grad.p + m.off_b[l]);
+    hipLaunchKernelGGL((k_wgrad<CW>), dim3(wg_blocks), dim3(TB), 0, st, rows, WG_ROWS, cl, (const bf16_t *)m.dzb.p,
+                       m.part.p);
+    hipLaunchKernelGGL(k_sum_partials, dim3(nb(CW * WID, 256)), dim3(256), 0, st, (int)wg_blocks, CW * WID,
+                       (const float *)m.part.p, m.grad.p + m.off_W[l]);
+    GemmEpi e{};
+    e.dh = m.dh.p;
+    e.da = m.da.p;
+    hipLaunchKernelGGL((k_gemm<WID, CW, EPI_DCAT>), dim3(nb(rows, 64)), dim3(TB), 0, st, rows, (const bf16_t *)m.dzb.p,
+                       (const bf16_t *)(m.WTb.p + (size_t)l * CW * WID), e);
+    hipLaunchKernelGGL(k_aggregate_t, dim3(nb(rows, 4)), dim3(TB), 0, st, rows, m.n, (const int32_t *)m.gptr.p,
+                       (const int32_t *)m.gcol.p, (const float *)m.gwt.p, (const float *)m.da.p, m.dh.p);
+  }
+  // encoder: gate from bf(h0) in cat_0
+  hipLaunchKernelGGL((k_gate<true>), dim3(gate_blocks), dim3(TB), 0, st, rows, (const float *)m.dh.p,
+                     (const float *)nullptr, (const bf16_t *)m.cat.p, m.dzb.p, m.part.p, GATE_ROWS);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(WID), 0, st, (int)gate_blocks, WID, (const float *)m.part.p,
+                     m.grad.p + m.off_be);
+  hipLaunchKernelGGL((k_wgrad<KPAD>), dim3(wg_blocks), dim3(TB), 0, st, rows, WG_ROWS, (const bf16_t *)m.xb.p,
+                     (const bf16_t *)m.dzb.p, m.part.p);
+  hipLaunchKernelGGL(k_sum_partials, dim3(nb(KPAD * WID, 256)), dim3(256), 0, st, (int)wg_blocks, KPAD * WID,
+                     (const float *)m.part.p, m.grad.p + m.off_We);
+  VFVM_HIP_CHECK(hipGetLastError());
+  return (float)(loss / (double)rows);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vfvm_mpnn_create(vfvm_sys *sys, int nlayers, int width, uint64_t seed, vfvm_mpnn **out) {
+  if (!sys || !out) return VFVM_ERR_ARG;
+  return mguard(sys->ctx, [&]() {
+    if (width != WID) throw Error(VFVM_ERR_ARG, "the network width is fixed at 64");
+    if (nlayers < 1 || nlayers > 16) throw Error(VFVM_ERR_ARG, "nlayers must be 1..16");
+    std::unique_ptr<vfvm_mpnn> m(new vfvm_mpnn());
+    m->ctx = sys->ctx;
+    m->nlayers = nlayers;
+    m->n = sys->nnodes;
+    // normalised graph weights from the system's node graph (sigma/h factors)
+    const HostSetup &h = sys->hs;
+    std::vector<int32_t> ptr((size_t)m->n + 1, 0), col;
+    std::vector<float> w, wt;
+    std::vector<double> rowsum((size_t)m->n, 0.0);
+    for (int64_t i = 0; i < m->n; i++)
+      for (int32_t k = h.adj_ptr[i]; k < h.adj_ptr[i + 1]; k++)
+        if (h.adj_col[k] != i) rowsum[i] += h.adj_fac[k];
+    for (int64_t i = 0; i < m->n; i++) {
+      for (int32_t k = h.adj_ptr[i]; k < h.adj_ptr[i + 1]; k++) {
+        const int32_t j = h.adj_col[k];
+        if (j == i) continue;
+        col.push_back(j);
+        w.push_back(rowsum[i] > 0 ? (float)(h.adj_fac[k] / rowsum[i]) : 0.f);
+        wt.push_back(rowsum[j] > 0 ? (float)(h.adj_fac[k] / rowsum[j]) : 0.f);  // w_ji on i's entry
+      }
+      ptr[i + 1] = (int32_t)col.size();
+    }
+    hipStream_t st = m->ctx->stream;
+    m->gptr.upload(ptr, st);
+    m->gcol.upload(col.empty() ? std::vector<int32_t>{0} : col, st);
+    m->gw.upload(w.empty() ? std::vector<float>{0.f} : w, st);
+    m->gwt.upload(wt.empty() ? std::vector<float>{0.f} : wt, st);
+    // parameter layout and He-uniform initialisation
+    size_t p = 0;
+    m->off_We = p;
+    p += KPAD * WID;
+    m->off_be = p;
+    p += WID;
+    for (int l = 0; l < nlayers; l++) {
+      m->off_W.push_back(p);
+      p += CW * WID;
+      m->off_b.push_back(p);
+      p += WID;
+    }
+    m->off_wd = p;
+    p += WID;
+    m->off_bd = p;
+    p += 1;
+    m->np = (int64_t)p;
+    std::vector<float> th(p, 0.f);
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<float> U(-1.f, 1.f);
+    for (int k = 0; k < KPAD * WID; k++)
+      th[m->off_We + k] = (k / WID < NFEAT) ? U(rng) * std::sqrt(6.f / NFEAT) : 0.f;
+    for (int l = 0; l < nlayers; l++)
+      for (int k = 0; k < CW * WID; k++) th[m->off_W[l] + k] = U(rng) * std::sqrt(6.f / CW) * 0.5f;
+    for (int k = 0; k < WID; k++) th[m->off_wd + k] = U(rng) * std::sqrt(3.f / WID);
+    m->theta.upload(th, st);
+    m->grad.alloc(p);
+    m->adam_m.alloc(p);
+    m->adam_v.alloc(p);
+    hipLaunchKernelGGL(k_fill_f, dim3(nb(p, 256)), dim3(256), 0, st, (int64_t)p, m->adam_m.p, 0.f);
+    hipLaunchKernelGGL(k_fill_f, dim3(nb(p, 256)), dim3(256), 0, st, (int64_t)p, m->adam_v.p, 0.f);
+    hipLaunchKernelGGL(k_fill_f, dim3(nb(p, 256)), dim3(256), 0, st, (int64_t)p, m->grad.p, 0.f);
+    m->Web.alloc(KPAD * WID);
+    m->Wb.alloc((size_t)nlayers * CW * WID);
+    m->WTb.alloc((size_t)nlayers * CW * WID);
+    refresh_bf16(*m);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    *out = m.release();
+    return VFVM_OK;
+  });
+}
+
+int vfvm_mpnn_destroy(vfvm_mpnn *m) {
+  if (!m) return VFVM_OK;
+  (void)hipStreamSynchronize(m->ctx->stream);
+  delete m;
+  return VFVM_OK;
+}
+
+int vfvm_mpnn_params(vfvm_mpnn *m, int64_t *nparams, float *params) {
+  if (!m) return VFVM_ERR_ARG;
+  return mguard(m->ctx, [&]() {
+    if (nparams) *nparams = m->np;
+    if (params) {
+      m->theta.download(params, (size_t)m->np, m->ctx->stream);
+      VFVM_HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
+    }
+    return VFVM_OK;
+  });
+}
+
+int vfvm_mpnn_set_params(vfvm_mpnn *m, const float *params) {
+  if (!m || !params) return VFVM_ERR_ARG;
+  return mguard(m->ctx, [&]() {
+    m->theta.upload(params, (size_t)m->np, m->ctx->stream);
+    refresh_bf16(*m);
+    VFVM_HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
+    return VFVM_OK;
+  });
+}
+
+int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, double *out) {
+  if (!m || !features || !out || nsamples < 1) return VFVM_ERR_ARG;
+  return mguard(m->ctx, [&]() {
+    hipStream_t st = m->ctx->stream;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsamples, (int64_t(1) << 20) / m->n));
+    std::vector<float> u;
+    for (int64_t s0 = 0; s0 < nsamples; s0 += chunk) {
+      const int64_t ns = std::min(chunk, nsamples - s0), rows = ns * m->n;
+      ensure_cap(*m, rows);
+      m->x.upload(features + s0 * m->n * NFEAT, (size_t)rows * NFEAT, st);
+      forward(*m, rows);
+      hipLaunchKernelGGL(k_decode, dim3(nb(rows, 4)), dim3(TB), 0, st, rows, (const float *)m->hf.p,
+                         (const float *)m->theta.p + m->off_wd, (const float *)m->theta.p + m->off_bd, m->u.p,
+                         (const float *)nullptr, 0.f, (float *)nullptr, (float *)nullptr);
+      u.resize((size_t)rows);
+      m->u.download(u.data(), (size_t)rows, st);
+      VFVM_HIP_CHECK(hipStreamSynchronize(st));
+      for (int64_t k = 0; k < rows; k++) out[s0 * m->n + k] = (double)u[k];
+    }
+    return VFVM_OK;
+  });
+}
+
+int vfvm_mpnn_loss_grad(vfvm_mpnn *m, int64_t nsamples, const float *features, const float *targets, float *loss,
+                        float *grad) {
+  if (!m || !features || !targets || nsamples < 1) return VFVM_ERR_ARG;
+  return mguard(m->ctx, [&]() {
+    hipStream_t st = m->ctx->stream;
+    const int64_t rows = nsamples * m->n;
+    ensure_cap(*m, rows);
+    m->x.upload(features, (size_t)rows * NFEAT, st);
+    m->target.upload(targets, (size_t)rows, st);
+    forward(*m, rows);
+    const float lv = backward(*m, rows);
+    if (loss) *loss = lv;
+    if (grad) {
+      m->grad.download(grad, (size_t)m->np, st);
+      VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    return VFVM_OK;
+  });
+}
+
+int vfvm_mpnn_train(vfvm_mpnn *m, int64_t nsamples, const float *features, const float *targets, int64_t epochs,
+                    int64_t batch, float lr, float *final_loss) {
+  if (!m || !features || !targets || nsamples < 1 || batch < 1 || epochs < 1) return VFVM_ERR_ARG;
+  return mguard(m->ctx, [&]() {
+    hipStream_t st = m->ctx->stream;
+    const int64_t n = m->n;
+    // the whole training set stays resident on the device
+    DevBuf<float> X, T;
+    X.upload(features, (size_t)nsamples * n * NFEAT, st);
+    T.upload(targets, (size_t)nsamples * n, st);
+    const int64_t B = std::min(batch, nsamples);
+    ensure_cap(*m, B * n);
+    std::vector<int64_t> order((size_t)nsamples);
+    for (int64_t k = 0; k < nsamples; k++) order[k] = k;
+    std::mt19937_64 rng(1234567u + (uint64_t)m->adam_t);
+    double last = 0.0;
+    for (int64_t ep = 0; ep < epochs; ep++) {
+      std::shuffle(order.begin(), order.end(), rng);
+      double acc = 0.0;
+      int64_t nbat = 0;
+      for (int64_t b0 = 0; b0 + B <= nsamples; b0 += B) {
+        for (int64_t q = 0; q < B; q++) {
+          const int64_t sidx = order[b0 + q];
+          VFVM_HIP_CHECK(hipMemcpyAsync(m->x.p + q * n * NFEAT, X.p + sidx * n * NFEAT, sizeof(float) * n * NFEAT,
+                                        hipMemcpyDeviceToDevice, st));
+          VFVM_HIP_CHECK(hipMemcpyAsync(m->target.p + q * n, T.p + sidx * n, sizeof(float) * n,
+                                        hipMemcpyDeviceToDevice, st));
+        }
+        forward(*m, B * n);
+        acc += backward(*m, B * n);
+        nbat++;
+        m->adam_t++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)m->adam_t), c2 = 1.f - std::pow(b2, (float)m->adam_t);
+        hipLaunchKernelGGL(k_adam, dim3(nb(m->np, 256)), dim3(256), 0, st, m->np, m->theta.p, (const float *)m->grad.p,
+                           m->adam_m.p, m->adam_v.p, lr, b1, b2, 1e-8f, c1, c2);
+        refresh_bf16(*m);
+      }
+      last = nbat ? acc / nbat : 0.0;
+    }
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    if (final_loss) *final_loss = (float)last;
+    return VFVM_OK;
+  });
+}
+
+}  // extern "C"

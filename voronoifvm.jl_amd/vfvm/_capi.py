@@ -74,8 +74,18 @@ SIGNATURES = {
     "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "vfvm_pattern_csr": (C.c_int, [_vp, _ip, _ip]),
+    "vfvm_mpnn_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp)]),
+    "vfvm_mpnn_destroy": (C.c_int, [_vp]),
+    "vfvm_mpnn_params": (C.c_int, [_vp, _lp, C.POINTER(C.c_float)]),
+    "vfvm_mpnn_set_params": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "vfvm_mpnn_train": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.c_int64,
+                                  C.c_float, C.POINTER(C.c_float)]),
+    "vfvm_mpnn_infer": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_float), _dp]),
+    "vfvm_mpnn_loss_grad": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                      C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "vfvm_batch_create": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(_vp)]),
     "vfvm_batch_destroy": (C.c_int, [_vp]),
+    "vfvm_batch_update_member": (C.c_int, [_vp, C.c_int64, _dp, C.POINTER(BCT)]),
     "vfvm_batch_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_batch_newton": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp), C.c_double, C.c_double, C.c_double,
                                     C.POINTER(ControlT), C.POINTER(HistoryT)]),

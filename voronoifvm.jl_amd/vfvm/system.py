@@ -331,6 +331,20 @@ class Batch:
             _capi._LIB.vfvm_batch_destroy(self._h)
             self._h = None
 
+    def update_member(self, m, source=None, bcondition=None):
+        """Replace member m's source and/or boundary conditions inside the batch."""
+        s = self.systems[m]
+        src_p = None
+        if source is not None:
+            s.physics.source = source
+            arr = s._source_array()
+            src_p = _capi.dptr(arr)
+        tab = None
+        if bcondition is not None:
+            s.physics.bcondition = list(bcondition)
+            tab = s._bc_table()
+        _capi.check(_capi.lib().vfvm_batch_update_member(self._h, m, src_p, tab), self.ctx.handle)
+
     def solve(self, inivals=0.0, control: Optional[SolverControl] = None, time=0.0, tstep=math.inf, embed=0.0,
               raise_on_failure=True):
         """solve_step! for every member; returns the list of solutions (nspec x nnodes)."""
