@@ -42,6 +42,17 @@ def kernel_ms(sys_, kernel, reps=20):
     return v.value
 
 
+def roofline_hbm(algorithmic_bytes, probe, traffic, name):
+    """Roofline entry of an HBM-bound kernel from its live per-launch durations."""
+    t = probe["avg_ms"]
+    gbs = algorithmic_bytes / (t * 1e-3) / 1e9
+    log(f"{name}: {probe['launches']} launches, {t * 1e3:.1f} us/launch, {gbs:.0f} GB/s "
+        f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
+    return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
+                traffic=traffic, algorithmic_bytes=int(algorithmic_bytes), avg_ms=round(t, 5),
+                launches=probe["launches"])
+
+
 def read_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (profiles/pmc_traffic.json), or None."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -82,15 +93,20 @@ def run_cfg1(args):
     dt = 1e-2
     K, W = args.steps, args.warmup
     res = {}
-    for label, prec in (("plain_fp64", "fp64"), ("fp32_refine", "fp32")):
+    probes = {}
+    for label, prec, kernels in (("plain_fp64", "fp64", (_capi.KERNEL_SPMV64,)),
+                                 ("fp32_refine", "fp32", (_capi.KERNEL_SPMV32, _capi.KERNEL_ASSEMBLE))):
         ctrl = cfg1_control(prec)
         if W > 0:
             warm = np.cumsum([0.0] + [dt] * W)
             vfvm.solve(s, inival=np.asfortranarray(u0), control=ctrl, times=warm)
         times = np.cumsum([0.0] + [dt] * K)
-        t1 = time.perf_counter()
-        U = vfvm.solve(s, inival=np.asfortranarray(u0), control=ctrl, times=times)
-        el = time.perf_counter() - t1
+        # the kernels' launches inside the timed region are bracketed by HIP events
+        with s.ctx.probe(*kernels) as pr:
+            t1 = time.perf_counter()
+            U = vfvm.solve(s, inival=np.asfortranarray(u0), control=ctrl, times=times)
+            el = time.perf_counter() - t1
+        probes.update(pr)
         # converged = every step met ||F|| <= 1e-10: check the final state's residual
         Ud = np.ascontiguousarray(vfvm.dofs(U))
         if not s.history.residual_norm <= 1e-10:
@@ -106,16 +122,10 @@ def run_cfg1(args):
     b_asm = 8 * 1 * n_ * 3 + 8 * n_ + E * (8 + 8) + 8 * nnz
     b_spmv64 = nnz * (8 + 4) + (ndof + 1) * 4 + 2 * ndof * 8
     b_spmv32 = nnz * (4 + 4) + (ndof + 1) * 4 + 2 * ndof * 4
-    t_asm = kernel_ms(s, 0)
-    t_s64 = kernel_ms(s, 2)
-    t_s32 = kernel_ms(s, 3)
     rl = {}
-    for name, b, t in (("assemble", b_asm, t_asm), ("spmv64", b_spmv64, t_s64), ("spmv32", b_spmv32, t_s32)):
-        gbs = b / (t * 1e-3) / 1e9
-        rl[name] = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(gbs / HBM_PEAK_GBS, 4), traffic=read_traffic("cfg1_" + name),
-                        algorithmic_bytes=int(b), avg_ms=round(t, 5))
-        log(f"{name}: {t * 1e3:.1f} us/launch, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
+    for name, b, k in (("assemble", b_asm, _capi.KERNEL_ASSEMBLE), ("spmv64", b_spmv64, _capi.KERNEL_SPMV64),
+                       ("spmv32", b_spmv32, _capi.KERNEL_SPMV32)):
+        rl[name] = roofline_hbm(b, probes[k], read_traffic("cfg1_" + name), name)
     out = dict(
         value=res["fp32_refine"]["solves_per_s"], ms_per_step=1e3 * res["fp32_refine"]["seconds"] / K,
         plain_gpu_newton_solves_per_s=res["plain_fp64"]["solves_per_s"],
@@ -145,6 +155,84 @@ def cpu_cfg1(args):
                 sample=f"1 implicit-Euler Newton solve of the cfg1 problem on {n}^3 nodes "
                        f"({sys_.grid.num_nodes} nodes, 1/4 of 129^3) with the oracle's BiCGStab+ILU(0) FP64, "
                        f"{el:.2f} s")
+
+
+# --------------------------------------------------------------------------- cfg2
+CFG2_BIAS = np.linspace(0.0, 1.0, 21)  # 0 .. 1 V in 20 continuation steps
+
+
+def cfg2_control(precision):
+    return vfvm.SolverControl(krylov="gmres", precon="blockjacobi", precision=precision, gmres_restart=30,
+                              reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=6000, inner_reltol=1e-3,
+                              max_refinements=12)
+
+
+def run_cfg2(args):
+    """Bipolar drift-diffusion on 512 x 512 nodes: a bias sweep 0..1 V in 20 continuation
+    steps (21 Newton solves, the first at zero bias), GMRES(30) + 3x3 block Jacobi."""
+    import cases
+    n = 512
+    t0 = time.perf_counter()
+    s, _ = cases.drift_diffusion_2d(n, lambda2=1e-4, oracle=False)
+    s.handle()
+    ndof, nnz, nedges = s.sizes()
+    u0 = cases.dd_initial(s.grid)
+    ag, sg = s.autotune()
+    log(f"cfg2: {n}^2 grid, ndof={ndof} nnz={nnz} edges={nedges}, setup {time.perf_counter() - t0:.1f}s, "
+        f"autotuned groups {ag}/{sg}")
+    K, W = args.steps, args.warmup
+    nsolve = CFG2_BIAS.size
+    res = {}
+    probes = {}
+    for label, prec, kernels in (("plain_fp64", "fp64", (_capi.KERNEL_SPMV64,)),
+                                 ("fp32_refine", "fp32", (_capi.KERNEL_SPMV32, _capi.KERNEL_ASSEMBLE))):
+        ctrl = cfg2_control(prec)
+        for _ in range(W):
+            vfvm.solve(s, inival=u0, control=ctrl, embed=CFG2_BIAS)
+        with s.ctx.probe(*kernels) as pr:
+            t1 = time.perf_counter()
+            for _ in range(K):
+                U = vfvm.solve(s, inival=u0, control=ctrl, embed=CFG2_BIAS)
+            el = time.perf_counter() - t1
+        probes.update(pr)
+        if not s.history.residual_norm <= 1e-10:
+            raise RuntimeError(f"{label}: a bias step ended with ||F||_inf = {s.history.residual_norm:.3e}")
+        res[label] = dict(seconds=el, solves_per_s=nsolve * K / el, U=np.ascontiguousarray(vfvm.dofs(U)),
+                          newton=s.history.niter, krylov=s.history.nlin)
+        log(f"{label}: {K} sweeps x {nsolve} solves in {el:.3f}s -> {nsolve * K / el:.2f} solves/s, newton "
+            f"{s.history.niter}, krylov {s.history.nlin} per sweep")
+    diff = float(np.abs(res["fp32_refine"]["U"] - res["plain_fp64"]["U"]).max())
+    b_spmv32 = nnz * (4 + 4) + (ndof + 1) * 4 + 2 * ndof * 4
+    b_asm = 8 * 3 * n * n * 2 + 8 * n * n + nedges * (8 + 8) + 8 * nnz
+    rl = {}
+    for name, b, k in (("spmv32", b_spmv32, _capi.KERNEL_SPMV32), ("assemble", b_asm, _capi.KERNEL_ASSEMBLE)):
+        rl[name] = roofline_hbm(b, probes[k], read_traffic("cfg2_" + name), name)
+    return dict(value=res["fp32_refine"]["solves_per_s"], ms_per_step=1e3 * res["fp32_refine"]["seconds"] / K,
+                plain_gpu_newton_solves_per_s=res["plain_fp64"]["solves_per_s"],
+                speedup_vs_plain=res["fp32_refine"]["solves_per_s"] / res["plain_fp64"]["solves_per_s"],
+                max_abs_diff_fp32_vs_fp64=diff, roofline=rl["spmv32"], roofline_assemble=rl["assemble"],
+                config=dict(workload=f"cfg2: bipolar drift-diffusion (psi, n, p), {n}x{n} nodes, lambda=1e-2, "
+                                     f"SG fluxes, bias 0..1 V in 20 continuation steps, GMRES(30) + 3x3 "
+                                     f"block Jacobi, FP32 inside FP64 Newton (plain: FP64 Krylov); a step is "
+                                     f"one sweep of {nsolve} solves", unknowns=ndof, nnz=nnz),
+                dtype="f64+f32")
+
+
+def cpu_cfg2(args):
+    """The oracle on a bounded sample: the same sweep on a 65 x 65 grid, banded direct solves."""
+    import cases
+    import oracle as O
+    n = 65
+    s, o = cases.drift_diffusion_2d(n, lambda2=1e-4)
+    u0 = cases.dd_initial(s.grid)
+    U = vfvm.dofs(u0).copy()
+    t0 = time.perf_counter()
+    for lam in CFG2_BIAS:  # fixed continuation steps: one Newton solve per bias value
+        U, _ = o.solve_step(U, O.control(), embed=float(lam))
+    el = time.perf_counter() - t0
+    return dict(value=CFG2_BIAS.size / el, unit="solves/s", cores=1, kind="port",
+                sample=f"the cfg2 bias sweep ({CFG2_BIAS.size} solves) on {n}x{n} nodes with the oracle's "
+                       f"banded direct solves, {el:.2f} s")
 
 
 # --------------------------------------------------------------------------- cfg3
@@ -250,7 +338,135 @@ def cpu_cfg3(args):
                        f"oracle Newton with banded direct solves, {t:.2f} s")
 
 
-WORKLOADS = {"cfg1": (run_cfg1, cpu_cfg1), "cfg3": (run_cfg3, cpu_cfg3)}
+# --------------------------------------------------------------------------- cfg4
+def run_cfg4(args):
+    """Learned initial guess + batching + FP32 Krylov vs plain FP64 GPU Newton from zero,
+    on the 1,024 held-out seeds of the 129 x 129 nonlinear Poisson family."""
+    from vfvm import surrogate as S
+    fam = S.PoissonFamily(size=256)
+    net = S.Surrogate(fam.systems[0])
+    meta = net.load()
+    seeds = list(S.EVAL_SEEDS)
+    K = min(args.steps, len(seeds) // 256)
+    blocks = [seeds[256 * k:256 * (k + 1)] for k in range(K)]
+    ctrl32 = vfvm.SolverControl(precision="fp32", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
+                                inner_reltol=1e-3, max_refinements=12, check_every=8)
+    # warm-up (untimed): one block through both batched variants
+    feats = fam.load_seeds(blocks[0])
+    for _ in range(max(args.warmup, 0)):
+        g = net.predict(feats)
+        fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
+        fam.batch.solve(0.0, ctrl32)
+    t_sur, t_zero, it_sur, it_zero, sur_err, worst = 0.0, 0.0, [], [], [], 0.0
+    converged = []
+    for blk in blocks:
+        feats = fam.load_seeds(blk)  # host input preparation and upload: not timed
+        t1 = time.perf_counter()
+        g = net.predict(feats)
+        U = fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
+        t_sur += time.perf_counter() - t1
+        it_sur += [h.niter for h in fam.batch.histories]
+        worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
+        t1 = time.perf_counter()
+        U0 = fam.batch.solve(0.0, ctrl32)
+        t_zero += time.perf_counter() - t1
+        it_zero += [h.niter for h in fam.batch.histories]
+        worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
+        for x, u in zip(g, U):
+            sur_err.append(float(np.abs(x - vfvm.dofs(u)).max()))
+        converged += [vfvm.dofs(u).copy() for u in U]
+    if not worst <= 1e-10:
+        raise RuntimeError(f"cfg4: a solve ended with ||F||_inf = {worst:.3e}")
+    nsol = 256 * K
+    sur_rate, zero_rate = nsol / t_sur, nsol / t_zero
+    log(f"surrogate+batched+FP32: {nsol} solves in {t_sur:.3f}s -> {sur_rate:.1f} solves/s, "
+        f"mean newton {np.mean(it_sur):.2f}; batched+FP32 from zero {zero_rate:.1f} solves/s, mean newton "
+        f"{np.mean(it_zero):.2f}; surrogate max-norm error mean {np.mean(sur_err):.3e} max {np.max(sur_err):.3e}")
+    # plain FP64 GPU Newton from zero, one system at a time, same inputs
+    single = S.make_system(fam.grid, np.zeros(fam.grid.num_nodes), np.zeros(4))
+    ctrl64 = vfvm.SolverControl(precision="fp64", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
+                                check_every=8)
+    single.handle()
+    vfvm.solve(single, inival=0.0, control=ctrl64)
+    t_plain, it_plain, diff = 0.0, [], 0.0
+    k = 0
+    for blk in blocks:
+        for sd in blk:
+            f, vals = S.draw(sd, fam.grf)
+            single.set_source(np.asfortranarray(f.reshape(1, -1)))
+            single.update_bcondition(S.bcs(vals))
+            single.handle()
+            t1 = time.perf_counter()
+            Up = vfvm.solve(single, inival=0.0, control=ctrl64)
+            t_plain += time.perf_counter() - t1
+            if not single.history.residual_norm <= 1e-10:
+                raise RuntimeError("plain solve did not reach ||F|| <= 1e-10")
+            it_plain.append(single.history.niter)
+            diff = max(diff, float(np.abs(vfvm.dofs(Up) - converged[k]).max()))
+            k += 1
+    plain_rate = nsol / t_plain
+    log(f"plain FP64 GPU Newton from zero: {nsol} solves in {t_plain:.2f}s -> {plain_rate:.1f} solves/s, "
+        f"mean newton {np.mean(it_plain):.2f}; max |U_sur - U_plain| = {diff:.2e}")
+    # roofline: batched FP32 SpMV (the Krylov hot op) and the network's layer GEMM
+    import ctypes as C
+    v = C.c_double()
+    _capi.check(_capi.lib().vfvm_batch_time_kernel(fam.batch._h, 3, 20, C.byref(v)))
+    t_spmv = v.value
+    ndof = fam.grid.num_nodes * 256
+    nnz = fam.systems[0].sizes()[1] * 256
+    b_spmv = nnz * (4 + 4) + (ndof + 1) * 4 + 2 * ndof * 4
+    gbs = b_spmv / (t_spmv * 1e-3) / 1e9
+    rows = 16 * fam.grid.num_nodes
+    _capi.check(_capi.lib().vfvm_mpnn_time_layer(net._h, rows, 20, C.byref(v)))
+    t_gemm = v.value
+    tflops = 2.0 * rows * 128 * 64 / (t_gemm * 1e-3) / 1e12
+    gemm_bytes = rows * (128 * 2 + 64 * 4 * 3 + 64 * 2)
+    return dict(value=sur_rate, ms_per_step=1e3 * t_sur / K, plain_gpu_newton_solves_per_s=plain_rate,
+                speedup_vs_plain=sur_rate / plain_rate, batched_fp32_from_zero_solves_per_s=zero_rate,
+                newton_iterations=dict(surrogate=float(np.mean(it_sur)), zero_batched=float(np.mean(it_zero)),
+                                       plain=float(np.mean(it_plain))),
+                surrogate_max_norm_error=dict(mean=float(np.mean(sur_err)), max=float(np.max(sur_err))),
+                max_abs_diff_vs_plain=diff, worst_residual=worst,
+                surrogate=dict(train_samples=int(meta["samples"]), epochs=int(meta["epochs"]),
+                               final_train_mse=float(meta["final_loss"])),
+                roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=round(gbs / HBM_PEAK_GBS, 4), traffic=read_traffic("cfg4_bspmv32"),
+                              algorithmic_bytes=int(b_spmv), avg_ms=round(t_spmv, 5),
+                              kernel="batched FP32 SpMV, 256 members of 129^2"),
+                roofline_mpnn_gemm=dict(bound="mfma", achieved=round(tflops, 2), peak=2500.0, unit="TFLOP/s",
+                                        frac=round(tflops / 2500.0, 5), traffic=read_traffic("cfg4_mpnn_gemm"),
+                                        algorithmic_bytes=int(gemm_bytes),
+                                        hbm_GBs=round(gemm_bytes / (t_gemm * 1e-3) / 1e9, 1),
+                                        avg_ms=round(t_gemm, 5), rows=rows),
+                config=dict(workload="cfg4: 129x129 nonlinear Poisson (eps=1e-2, GRF source ell=0.1 amp=5, side "
+                                     "values U[-1,1]), 1,024 held-out seeds in batches of 256: MPNN initial guess "
+                                     "(4 layers, width 64, BF16 MFMA) + batched BiCGStab/Jacobi FP32 inside FP64 "
+                                     "Newton, vs plain FP64 GPU Newton from zero one solve at a time",
+                            solves=nsol, unknowns_per_solve=fam.grid.num_nodes, batch=256),
+                dtype="f64+f32+bf16")
+
+
+def cpu_cfg4(args):
+    """The oracle on a bounded sample: 16 held-out seeds from zero, direct solves."""
+    import oracle as O
+    from vfvm import surrogate as S
+    import cases
+    grf = S.GRF()
+    t = 0.0
+    seeds = list(S.EVAL_SEEDS)[:16]
+    for sd in seeds:
+        f, vals = S.draw(sd, grf)
+        s, o = cases.nonlinear_poisson_2d(S.GRID_N, S.EPS, 0.0, tuple(vals))
+        o.set_source(f)
+        t0 = time.perf_counter()
+        o.solve_step(np.zeros(s.grid.num_nodes))
+        t += time.perf_counter() - t0
+    return dict(value=len(seeds) / t, unit="solves/s", cores=1, kind="port",
+                sample=f"{len(seeds)} of the 1,024 held-out cfg4 seeds, oracle Newton from zero with banded "
+                       f"direct solves, {t:.2f} s")
+
+
+WORKLOADS = {"cfg1": (run_cfg1, cpu_cfg1), "cfg2": (run_cfg2, cpu_cfg2), "cfg3": (run_cfg3, cpu_cfg3), "cfg4": (run_cfg4, cpu_cfg4)}
 
 
 def main():

@@ -202,6 +202,16 @@ int vfvm_autotune(vfvm_sys *sys, int32_t *asm_group, int32_t *spmv_group);
 #define VFVM_KERNEL_SPMV32 3
 int vfvm_time_kernel(vfvm_sys *sys, int kernel, int reps, double *avg_ms);
 
+/* Benchmark helper (no counterpart in the reference): live timing of kernel classes
+ * inside real calls on the context.  After vfvm_probe_arm(kernel), every launch of that
+ * class (VFVM_KERNEL_ASSEMBLE: Newton-step Jacobian assembly; _SPMV64 / _SPMV32: Krylov
+ * operator applications), single-system or batched, is bracketed by HIP events on the
+ * launching stream, up to max_launches launches.  Several classes may be armed at once.
+ * vfvm_probe_read returns one class's summed duration and number of timed launches and
+ * disarms that class. */
+int vfvm_probe_arm(vfvm_ctx *ctx, int kernel, int64_t max_launches);
+int vfvm_probe_read(vfvm_ctx *ctx, int kernel, double *total_ms, int64_t *launches);
+
 /* ---- device-resident interface (pointers are device pointers on the handle's device) ----
  * For hosts that keep U on the GPU (e.g. a Julia AMDGPU.jl array): same contracts as the host
  * versions, nzval_csr in the device CSR order of vfvm_pattern_csr. */
@@ -221,6 +231,10 @@ int vfvm_batch_destroy(vfvm_batch *b);
 /* replace member m's source (nspec x nnodes; NULL keeps it) and boundary table (NULL keeps
  * it) inside the batch, for repeated solves of one grid with new data */
 int vfvm_batch_update_member(vfvm_batch *b, int64_t m, const double *src, const vfvm_bc_t *bc);
+/* benchmark helper (no counterpart in the reference): average device time in ms of `reps`
+ * launches of one batched kernel (VFVM_KERNEL_ASSEMBLE, _SPMV64, _SPMV32) over all members,
+ * HIP events on the batch's stream */
+int vfvm_batch_time_kernel(vfvm_batch *b, int kernel, int reps, double *avg_ms);
 /* project-written autotuner for the batch's assembly and SpMV group sizes */
 int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group);
 /* solve_step! for every member (src/vfvm_solver.jl:13-222): U, Uold are nsys host pointers to
@@ -250,6 +264,8 @@ int vfvm_mpnn_train(vfvm_mpnn *m, int64_t nsamples, const float *features, const
                     int64_t epochs, int64_t batch, float lr, float *final_loss);
 /* out: nsamples x nnodes predictions */
 int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, double *out);
+/* benchmark helper: average device time in ms of one hidden-layer GEMM over `rows` rows */
+int vfvm_mpnn_time_layer(vfvm_mpnn *m, int64_t rows, int reps, double *avg_ms);
 /* loss and gradient for one batch (checked against the CPU oracle) */
 int vfvm_mpnn_loss_grad(vfvm_mpnn *m, int64_t nsamples, const float *features, const float *targets,
                         float *loss, float *grad);

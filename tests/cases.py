@@ -11,13 +11,16 @@ from vfvm.grids import linspace, simplexgrid
 ORACLE_BC = {vfvm.Dirichlet: "dirichlet", vfvm.Robin: "robin"}
 
 
-def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None):
+def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None, oracle=True):
     """phys_kw: dict(flux=..., reaction=..., storage=...) of vfvm registry objects.
-    bcs: vfvm.Dirichlet / vfvm.Robin entries; legacy: (species, region, value) Dirichlet."""
+    bcs: vfvm.Dirichlet / vfvm.Robin entries; legacy: (species, region, value) Dirichlet.
+    With oracle=False only the device system is built (second item None)."""
     ph = vfvm.Physics(bcondition=list(bcs), source=source, **phys_kw)
     sys = vfvm.System(grid, ph, species=list(range(1, nspec + 1)))
     for (sp, reg, val) in legacy:
         sys.boundary_dirichlet(sp, reg, val)
+    if not oracle:
+        return sys, None
     # oracle parameter block from the same registry objects
     flux, react, stor, p = ph.param_block(nspec)
     oph = O.Physics()
@@ -60,7 +63,7 @@ def initial_sine_3d(grid):
     return 0.5 * np.sin(math.pi * x) * np.sin(math.pi * y) * np.sin(math.pi * z)
 
 
-def drift_diffusion_2d(n=33, lambda2=1e-4):
+def drift_diffusion_2d(n=33, lambda2=1e-4, oracle=True):
     """Bipolar drift-diffusion (psi, n, p), doping +1 for x < 0.5 else -1, ohmic
     contacts on west (x = 0) and east (x = 1); east contact biased by lambda."""
     X = linspace(0.0, 1.0, n)
@@ -78,7 +81,7 @@ def drift_diffusion_2d(n=33, lambda2=1e-4):
                 vfvm.Dirichlet(species=2, region=reg, value=nn),
                 vfvm.Dirichlet(species=3, region=reg, value=pp)]
     phys = dict(flux=vfvm.DriftDiffusionFlux(lambda2, -1.0, 1.0), reaction=vfvm.DriftDiffusionReaction(1.0))
-    return pair(g, 3, phys, bcs, source=src)
+    return pair(g, 3, phys, bcs, source=src, oracle=oracle)
 
 
 def dd_initial(grid):

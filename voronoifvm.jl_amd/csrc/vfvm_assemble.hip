@@ -60,6 +60,16 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
   double uK[NS];
 #pragma unroll
   for (int i = 0; i < NS; i++) uK[i] = a.U[K * NS + i];
+  // node-item operands are loaded up front by every lane of the group (one request
+  // per group after coalescing) so they are in flight during the edge loop
+  const double vol = a.nodevol[K];
+  const int32_t bb0 = a.bn_ptr[K], bb1 = a.bn_ptr[K + 1];
+  double uo[NS], srcv[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    uo[i] = a.Uold[K * NS + i];
+    srcv[i] = a.src ? a.src[K * NS + i] : 0.0;
+  }
 
   // first slot of row (K, i): rows of earlier nodes, then earlier species of K
   int64_t rowbase[NS];
@@ -129,12 +139,9 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
 
   if (lane == 0) {
     // node item: src/vfvm_assembly.jl:81-122
-    const double vol = a.nodevol[K];
     double Fn[NS], Jn[NS][NS];
     {
-      double uo[NS], so[NS];
-#pragma unroll
-      for (int i = 0; i < NS; i++) uo[i] = a.Uold[K * NS + i];
+      double so[NS];
       eval_storage<NS>(P->storage, uo, so);
       if constexpr (JAC) {
         Dual<NS> u[NS], r[NS], st[NS];
@@ -144,7 +151,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
         eval_storage<NS>(P->storage, u, st);
 #pragma unroll
         for (int i = 0; i < NS; i++) {
-          const double src = a.src ? a.src[K * NS + i] : 0.0;
+          const double src = srcv[i];
           Fn[i] = vol * (r[i].v - src + (st[i].v - so[i]) * a.tstepinv);
 #pragma unroll
           for (int j = 0; j < NS; j++) {
@@ -159,7 +166,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
         eval_storage<NS>(P->storage, uK, st);
 #pragma unroll
         for (int i = 0; i < NS; i++) {
-          const double src = a.src ? a.src[K * NS + i] : 0.0;
+          const double src = srcv[i];
           Fn[i] = vol * (r[i] - src + (st[i] - so[i]) * a.tstepinv);
         }
       }
@@ -168,7 +175,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
     double Fb[NS], Jb[NS];
 #pragma unroll
     for (int i = 0; i < NS; i++) Fb[i] = Jb[i] = 0.0;
-    for (int32_t b = a.bn_ptr[K]; b < a.bn_ptr[K + 1]; b++) {
+    for (int32_t b = bb0; b < bb1; b++) {
       const int reg = a.bn_reg[b];
       const double fac = a.bn_fac[b];
       if (reg < 1 || reg > MAXBREG) continue;
@@ -231,11 +238,13 @@ void launch_ns_g(const AsmArgs &a, hipStream_t s) {
 template <int NS>
 void launch_ns(const AsmArgs &a, int group, hipStream_t s) {
   switch (group) {
+    case 1: launch_ns_g<NS, 1>(a, s); break;
+    case 2: launch_ns_g<NS, 2>(a, s); break;
     case 4: launch_ns_g<NS, 4>(a, s); break;
     case 8: launch_ns_g<NS, 8>(a, s); break;
     case 16: launch_ns_g<NS, 16>(a, s); break;
     case 32: launch_ns_g<NS, 32>(a, s); break;
-    default: throw Error(VFVM_ERR_ARG, "assembly group size must be 4, 8, 16 or 32");
+    default: throw Error(VFVM_ERR_ARG, "assembly group size must be 1, 2, 4, 8, 16 or 32");
   }
 }
 

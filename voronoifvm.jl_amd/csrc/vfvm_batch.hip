@@ -394,6 +394,8 @@ void bop(vfvm_batch &b, const V *x, V *y, const V *w0) {
       hipLaunchKernelGGL((k_bspmv<double, V, G, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
                          b.rowptr.p, b.colind.p, (const double *)b.J.p, x, y, (const double *)b.rowscale.p, w0, b.st.p);
   };
+  const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
+  b.ctx->probe.before(kid, b.ctx->stream);
   switch (b.spmv_group) {
     case 2: go(std::integral_constant<int, 2>()); break;
     case 4: go(std::integral_constant<int, 4>()); break;
@@ -401,6 +403,7 @@ void bop(vfvm_batch &b, const V *x, V *y, const V *w0) {
     case 16: go(std::integral_constant<int, 16>()); break;
     default: go(std::integral_constant<int, 32>()); break;
   }
+  b.ctx->probe.after(kid, b.ctx->stream);
   if constexpr (NDOT > 0)
     hipLaunchKernelGGL((k_bfinalize<2, STAGE, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, b.ctx->stream, S,
                        b.st.p, (const int32_t *)nullptr, (double *)nullptr);
@@ -712,6 +715,46 @@ int vfvm_batch_destroy(vfvm_batch *b) {
   return VFVM_OK;
 }
 
+// average device time (ms) of `reps` launches of one batched kernel at the current state:
+// 0 Jacobian assembly, 2 FP64 SpMV (row-scaled), 3 FP32 SpMV; HIP events on the batch's stream
+int vfvm_batch_time_kernel(vfvm_batch *b, int kernel, int reps, double *avg_ms) {
+  if (!b || !avg_ms || reps < 1) return VFVM_ERR_ARG;
+  return bguard(b->ctx, [&]() {
+    hipStream_t st = b->ctx->stream;
+    std::vector<int32_t> all((size_t)b->nsys, 1);
+    b->active.upload(all.data(), all.size(), st);
+    AsmArgs a = basm(*b, b->F.p, b->J.p, 1.0, 0.0, b->active.p);
+    launch_assemble(a, b->asm_group, st);
+    launch_scale_rows(b->nspec, b->nnodes, b->rowptr.p, b->colind.p, b->J.p, b->dofdiag.p, b->rowscale.p, b->A32.p,
+                      nullptr, st);
+    reset_states(*b, 0.0, 0.0, 1);
+    auto once = [&]() {
+      if (kernel == 0)
+        launch_assemble(a, b->asm_group, st);
+      else if (kernel == 2)
+        bop<double, 0, STAGE_NONE>(*b, b->U.p, b->tmp.p, (const double *)nullptr);
+      else if (kernel == 3)
+        bop<float, 0, STAGE_NONE>(*b, b->k32[0].p, b->k32[1].p, (const float *)nullptr);
+      else
+        throw Error(VFVM_ERR_ARG, "unknown kernel id");
+    };
+    once();
+    hipEvent_t e0, e1;
+    VFVM_HIP_CHECK(hipEventCreate(&e0));
+    VFVM_HIP_CHECK(hipEventCreate(&e1));
+    VFVM_HIP_CHECK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) once();
+    VFVM_HIP_CHECK(hipEventRecord(e1, st));
+    VFVM_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / reps;
+    return VFVM_OK;
+  });
+}
+
 // replace member m's source (nspec x nnodes, may be NULL to keep) and BC table (may be NULL)
 int vfvm_batch_update_member(vfvm_batch *b, int64_t m, const double *src, const vfvm_bc_t *bc) {
   if (!b || m < 0 || m >= b->nsys) return VFVM_ERR_ARG;
@@ -760,7 +803,7 @@ int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group) 
     };
     float best = 1e30f;
     int bestg = b->asm_group;
-    for (int g : {4, 8, 16, 32}) {
+    for (int g : {1, 2, 4, 8, 16, 32}) {
       AsmArgs a = basm(*b, b->F.p, b->J.p, 1.0, 0.0, b->active.p);
       float ms = timeit([&]() { launch_assemble(a, g, st); });
       if (ms < best) {
@@ -837,7 +880,9 @@ int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold
     int64_t nactive = M;
     for (int it = 1; it <= c.maxiters && nactive > 0; it++) {
       AsmArgs a = basm(*b, b->F.p, b->J.p, tstep, lambda, b->active.p);
+      b->ctx->probe.before(VFVM_KERNEL_ASSEMBLE, st);
       launch_assemble(a, b->asm_group, st);
+      b->ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st);
       bsolve(*b, c, nlin);
       b->damp.upload(damp.data(), damp.size(), st);
       hipLaunchKernelGGL(k_bupdate, dim3((unsigned)b->nch), dim3(BB), 0, st, S, b->U.p, (const double *)b->dU.p,

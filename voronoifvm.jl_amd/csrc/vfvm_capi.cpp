@@ -181,6 +181,7 @@ void setup_linear(vfvm_sys &s, const vfvm_control_t &c) {
   L.dofdiag = s.d_dofdiag.p;
   L.ilu = s.ilu_built ? &s.ilu : nullptr;
   L.spmv_group = s.spmv_group;
+  L.probe = &s.ctx->probe;
   s.lin.setup(L, s.ctx->stream);
 }
 
@@ -243,7 +244,9 @@ int newton_device(vfvm_sys &s, double time, double tstep, double lambda, const v
   }
   while (niter <= c.maxiters) {
     AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_F.p, s.d_J.p, tstep, lambda);
+    s.ctx->probe.before(VFVM_KERNEL_ASSEMBLE, st);
     launch_assemble(a, s.asm_group, st);
+    s.ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st);
     const bool refactor = !(c.reuse_precon && niter > 1);
     s.lin.prepare(s.d_J.p, c, refactor);
     vfvm_linstats_t ls{};
@@ -318,7 +321,7 @@ void autotune_impl(vfvm_sys &s, int32_t *ag, int32_t *sg) {
   // assembly group: time Jacobian assembly at the current U for each legal group
   float best = 1e30f;
   int bestg = s.asm_group;
-  for (int g : {4, 8, 16, 32}) {
+  for (int g : {1, 2, 4, 8, 16, 32}) {
     AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_F.p, s.d_J.p, 1.0, 0.0);
     launch_assemble(a, g, st);
     VFVM_HIP_CHECK(hipEventRecord(e0, st));
@@ -739,6 +742,25 @@ int vfvm_autotune(vfvm_sys *s, int32_t *asm_group, int32_t *spmv_group) {
   if (!s) return VFVM_ERR_ARG;
   return guarded(s->ctx, [&]() {
     autotune_impl(*s, asm_group, spmv_group);
+    return VFVM_OK;
+  });
+}
+
+int vfvm_probe_arm(vfvm_ctx *ctx, int kernel, int64_t max_launches) {
+  if (!ctx || max_launches < 1 || max_launches > (1 << 20) || kernel < 0 || kernel >= KernelProbe::NK)
+    return VFVM_ERR_ARG;
+  return guarded(ctx, [&]() {
+    VFVM_HIP_CHECK(hipSetDevice(ctx->device));
+    ctx->probe.arm(kernel, max_launches);
+    return VFVM_OK;
+  });
+}
+
+int vfvm_probe_read(vfvm_ctx *ctx, int kernel, double *total_ms, int64_t *launches) {
+  if (!ctx || !total_ms || kernel < 0 || kernel >= KernelProbe::NK) return VFVM_ERR_ARG;
+  return guarded(ctx, [&]() {
+    *total_ms = ctx->probe.total_ms(kernel, launches);
+    ctx->probe.disarm(kernel);
     return VFVM_OK;
   });
 }

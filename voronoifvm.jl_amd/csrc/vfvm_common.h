@@ -148,10 +148,71 @@ void launch_penalty_free_norm(int64_t nnodes, int nspec, const double *F, const 
                               const int32_t *bn_reg, const int32_t *node_inst, const InstParams *inst,
                               double *out_partial, int64_t nparts, hipStream_t s);
 
+// Live timing of kernel classes inside real solves.  While a class (VFVM_KERNEL_*)
+// is armed, every launch of it is bracketed by a pair of HIP events recorded on
+// the launching stream; the pairs come from a pool created when the class is
+// armed, launches beyond the pool are counted as dropped.
+struct KernelProbe {
+  static constexpr int NK = 4;
+  struct Slot {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    int64_t dropped = 0;
+  } k[NK];
+  void arm(int id, int64_t max_launches) {
+    disarm(id);
+    Slot &q = k[id];
+    q.ev.resize(2 * (size_t)max_launches, nullptr);
+    for (auto &e : q.ev) VFVM_HIP_CHECK(hipEventCreate(&e));
+    q.on = true;
+  }
+  void disarm(int id) {
+    Slot &q = k[id];
+    for (auto e : q.ev)
+      if (e) (void)hipEventDestroy(e);
+    q.ev.clear();
+    q.used = 0;
+    q.dropped = 0;
+    q.on = false;
+  }
+  void before(int id, hipStream_t s) {
+    Slot &q = k[id];
+    if (q.on && q.used + 2 <= q.ev.size()) VFVM_HIP_CHECK(hipEventRecord(q.ev[q.used], s));
+  }
+  void after(int id, hipStream_t s) {
+    Slot &q = k[id];
+    if (!q.on) return;
+    if (q.used + 2 <= q.ev.size()) {
+      VFVM_HIP_CHECK(hipEventRecord(q.ev[q.used + 1], s));
+      q.used += 2;
+    } else {
+      q.dropped++;
+    }
+  }
+  // summed duration (ms) of the recorded launches of one class; waits for the last one
+  double total_ms(int id, int64_t *launches) {
+    Slot &q = k[id];
+    double t = 0.0;
+    if (q.used) VFVM_HIP_CHECK(hipEventSynchronize(q.ev[q.used - 1]));
+    for (size_t i = 0; i + 1 < q.used; i += 2) {
+      float ms = 0.f;
+      VFVM_HIP_CHECK(hipEventElapsedTime(&ms, q.ev[i], q.ev[i + 1]));
+      t += ms;
+    }
+    if (launches) *launches = (int64_t)(q.used / 2);
+    return t;
+  }
+  ~KernelProbe() {
+    for (int i = 0; i < NK; i++) disarm(i);
+  }
+};
+
 }  // namespace vfvm
 
 struct vfvm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  vfvm::KernelProbe probe;
 };

@@ -101,6 +101,7 @@ struct LinSetup {
   const int32_t *rowptr = nullptr, *colind = nullptr, *dofdiag = nullptr;
   const IluStruct *ilu = nullptr;
   int spmv_group = 8;
+  KernelProbe *probe = nullptr;  // live kernel timing of the owning context
 };
 
 class LinearSolver {

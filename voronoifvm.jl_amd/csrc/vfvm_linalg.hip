@@ -793,6 +793,8 @@ void LinearSolver::apply_op(const V *x, V *y, const V *w0, const V *w1, KState *
       hipLaunchKernelGGL((k_spmv<double, V, GG, NDOT, STAGE>), dim3(g), dim3(VB), 0, stream, L.ndof, L.rowptr,
                          L.colind, J, x, y, (const double *)rowscale.p, w0, w1, partials.p, stp);
   };
+  const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
+  if (L.probe) L.probe->before(kid, stream);
   switch (G) {
     case 2: launch(std::integral_constant<int, 2>()); break;
     case 4: launch(std::integral_constant<int, 4>()); break;
@@ -800,6 +802,7 @@ void LinearSolver::apply_op(const V *x, V *y, const V *w0, const V *w1, KState *
     case 16: launch(std::integral_constant<int, 16>()); break;
     default: launch(std::integral_constant<int, 32>()); break;
   }
+  if (L.probe) L.probe->after(kid, stream);
   if constexpr (NDOT > 0)
     hipLaunchKernelGGL((k_finalize<2, STAGE>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p, (int)g, stp, 1);
 }

@@ -618,6 +618,39 @@ int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, doubl
   });
 }
 
+// average device time (ms) of one hidden-layer GEMM (rows x 128 times 128 x 64 on the matrix
+// cores, relu + residual epilogue) over `rows` rows of the current activations
+int vfvm_mpnn_time_layer(vfvm_mpnn *m, int64_t rows, int reps, double *avg_ms) {
+  if (!m || rows < 1 || reps < 1 || !avg_ms) return VFVM_ERR_ARG;
+  return mguard(m->ctx, [&]() {
+    hipStream_t st = m->ctx->stream;
+    ensure_cap(*m, rows);
+    GemmEpi r{};
+    r.bias = m->theta.p + m->off_b[0];
+    r.hf = m->hf.p;
+    r.rbuf = m->rbuf.p;
+    r.cat = m->cat.p + (size_t)rows * CW;
+    auto once = [&]() {
+      hipLaunchKernelGGL((k_gemm<CW, WID, EPI_RES>), dim3(nb(rows, 64)), dim3(TB), 0, st, rows,
+                         (const bf16_t *)m->cat.p, (const bf16_t *)m->Wb.p, r);
+    };
+    once();
+    hipEvent_t e0, e1;
+    VFVM_HIP_CHECK(hipEventCreate(&e0));
+    VFVM_HIP_CHECK(hipEventCreate(&e1));
+    VFVM_HIP_CHECK(hipEventRecord(e0, st));
+    for (int k = 0; k < reps; k++) once();
+    VFVM_HIP_CHECK(hipEventRecord(e1, st));
+    VFVM_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / reps;
+    return VFVM_OK;
+  });
+}
+
 int vfvm_mpnn_loss_grad(vfvm_mpnn *m, int64_t nsamples, const float *features, const float *targets, float *loss,
                         float *grad) {
   if (!m || !features || !targets || nsamples < 1) return VFVM_ERR_ARG;

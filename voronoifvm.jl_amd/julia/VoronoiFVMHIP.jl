@@ -1,0 +1,158 @@
+# VoronoiFVMHIP.jl -- the Julia side of the drop-in boundary: ccall bindings of
+# libvfvm_hip.so (include/vfvm_hip.h) and the two replacements a VoronoiFVM
+# maintainer plugs in.  It cannot run in this repository's environment (no
+# Julia); it documents, as code, how the reference reaches the library.
+#
+#   eval_and_assemble (src/vfvm_assembly.jl:520-643)  -> hip_eval_and_assemble!
+#   solve_step!       (src/vfvm_solver.jl:13-222)     -> hip_solve_step!
+#   _solve_linear!    (src/vfvm_linsolve.jl:6-61)     -> hip_solve_linear!
+#
+# The Physics callbacks are Julia closures the device cannot call; a system is
+# mapped to a registry entry with `hip_physics` (flux/reaction/storage kinds and
+# parameters) and its boundary conditions to a table with `hip_bc_table`.
+module VoronoiFVMHIP
+
+using SparseArrays
+const libvfvm = joinpath(@__DIR__, "..", "libvfvm_hip.so")
+
+# status codes (src/vfvm_logging_exceptions.jl:6-29)
+const VFVM_OK, VFVM_ERR_CONVERGENCE, VFVM_ERR_ASSEMBLY, VFVM_ERR_LINEAR = 0, 1, 2, 3
+
+struct PhysicsT
+    flux::Int32
+    reaction::Int32
+    storage::Int32
+    pad::Int32
+    p::NTuple{16, Float64}
+end
+
+struct BCT
+    kind::Int32
+    pad::Int32
+    value::Float64
+    value_lambda::Float64
+    factor::Float64
+end
+
+mutable struct ControlT
+    abstol::Float64; reltol::Float64; tol_round::Float64; tol_mono::Float64
+    damp_initial::Float64; damp_growth::Float64
+    maxiters::Int32; max_round::Int32; updatecontrol::Int32; is_linear::Int32
+    reltol_linear::Float64; abstol_linear::Float64
+    maxiters_linear::Int32; krylov::Int32; precon::Int32; precision::Int32
+    gmres_restart::Int32; max_refinements::Int32; reuse_precon::Int32; check_every::Int32
+    inner_reltol::Float64
+    ControlT() = new()
+end
+
+mutable struct HistoryT
+    niter::Int32; nlin::Int32; nlu::Int32; converged::Int32
+    time::Float64; tasm::Float64; tlinsolve::Float64
+    last_update_norm::Float64; residual_norm::Float64
+    updatenorm::NTuple{64, Float64}
+    HistoryT() = new()
+end
+
+function check(rc, ctx)
+    rc == VFVM_OK && return nothing
+    msg = unsafe_string(ccall((:vfvm_last_error, libvfvm), Cstring, (Ptr{Cvoid},), ctx))
+    rc == VFVM_ERR_CONVERGENCE && throw(VoronoiFVM.ConvergenceError())
+    rc == VFVM_ERR_ASSEMBLY && throw(VoronoiFVM.AssemblyError())
+    rc == VFVM_ERR_LINEAR && throw(VoronoiFVM.LinearSolverError())
+    error("vfvm_hip: $msg")
+end
+
+function context(device = 0)
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:vfvm_ctx_create, libvfvm), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, ctx)
+    rc == 0 || error(unsafe_string(ccall((:vfvm_last_error, libvfvm), Cstring, (Ptr{Cvoid},), C_NULL)))
+    return ctx[]
+end
+
+"""
+    hip_system(ctx, grid, physics::PhysicsT, bc::Vector{BCT}, nspec)
+
+Uploads an ExtendableGrid (1-based Int64 CellNodes / BFaceNodes, as stored) and
+builds the device system: what System(grid; ...) + _complete! do for the path.
+"""
+function hip_system(ctx, grid, physics::PhysicsT, bc::Vector{BCT}, nspec)
+    coord = grid[Coordinates]
+    cn = Int64.(grid[CellNodes]); cr = Int64.(grid[CellRegions])
+    bn = Int64.(grid[BFaceNodes]); br = Int64.(grid[BFaceRegions])
+    nbreg = maximum(br)
+    sys = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:vfvm_system_create, libvfvm), Cint,
+               (Ptr{Cvoid}, Cint, Int64, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64},
+                Cint, Ref{PhysicsT}, Cint, Ptr{BCT}, Ptr{Ptr{Cvoid}}),
+               ctx, size(coord, 1), size(coord, 2), coord, size(cn, 2), cn, cr, size(bn, 2), bn, br,
+               nspec, physics, nbreg, bc, sys)
+    check(rc, ctx)
+    return sys[]
+end
+
+"""
+    hip_pattern(sys) -> SparseMatrixCSC
+
+The Jacobian pattern, identical to SparseMatrixCSC(state.matrix) of the edgewise
+reference assembly (1-based CSC, rows sorted).
+"""
+function hip_pattern(sys)
+    ndof, nnz, ne = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+    ccall((:vfvm_system_sizes, libvfvm), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}), sys, ndof, nnz, ne)
+    colptr = zeros(Int64, ndof[] + 1)
+    rowval = zeros(Int64, nnz[])
+    ccall((:vfvm_pattern, libvfvm), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}), sys, colptr, rowval)
+    return SparseMatrixCSC(ndof[], ndof[], colptr, rowval, zeros(nnz[]))
+end
+
+"""
+    hip_eval_and_assemble!(ctx, sys, U, UOld, F, matrix::SparseMatrixCSC, time, tstep, λ)
+
+Replacement of eval_and_assemble (src/vfvm_assembly.jl:520-643): residual into F,
+Jacobian values into nonzeros(matrix) (pattern from hip_pattern).
+"""
+function hip_eval_and_assemble!(ctx, sys, U, UOld, F, matrix::SparseMatrixCSC, time, tstep, λ)
+    rc = ccall((:vfvm_assemble, libvfvm), Cint,
+               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Ptr{Float64}, Ptr{Float64}),
+               sys, U, UOld, time, tstep, λ, F, nonzeros(matrix))
+    check(rc, ctx)
+    return matrix
+end
+
+"""
+    hip_solve_linear!(ctx, sys, u, b, control::ControlT)
+
+Replacement of _solve_linear! (src/vfvm_linsolve.jl:6-61) on the last assembled
+Jacobian: BiCGStab/GMRES with Jacobi, block-Jacobi or ILU(0), FP64 or FP32 with
+FP64 iterative refinement.
+"""
+function hip_solve_linear!(ctx, sys, u, b, control::ControlT)
+    stats = zeros(UInt8, 32)
+    rc = ccall((:vfvm_linsolve, libvfvm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ref{ControlT}, Ptr{UInt8}),
+               sys, b, u, control, stats)
+    check(rc, ctx)
+    return u
+end
+
+"""
+    hip_solve_step!(ctx, sys, solution, oldsol, control::ControlT, time, tstep, λ)
+
+Replacement of solve_step! (src/vfvm_solver.jl:13-222): Dirichlet initialisation,
+damped Newton, update-norm / round-off / monotonicity tests, all on the device.
+"""
+function hip_solve_step!(ctx, sys, solution, oldsol, control::ControlT, time, tstep, λ)
+    hist = HistoryT()
+    rc = ccall((:vfvm_newton, libvfvm), Cint,
+               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Ref{ControlT}, Ref{HistoryT}),
+               sys, solution, oldsol, time, tstep, λ, control, hist)
+    check(rc, ctx)
+    return solution, hist
+end
+
+function default_control()
+    c = ControlT()
+    ccall((:vfvm_default_control, libvfvm), Cvoid, (Ref{ControlT},), c)
+    return c
+end
+
+end # module

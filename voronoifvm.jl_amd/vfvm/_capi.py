@@ -73,6 +73,8 @@ SIGNATURES = {
     "vfvm_residual_norm": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
     "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "vfvm_probe_arm": (C.c_int, [_vp, C.c_int, C.c_int64]),
+    "vfvm_probe_read": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int64)]),
     "vfvm_pattern_csr": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_mpnn_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp)]),
     "vfvm_mpnn_destroy": (C.c_int, [_vp]),
@@ -85,6 +87,8 @@ SIGNATURES = {
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "vfvm_batch_create": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(_vp)]),
     "vfvm_batch_destroy": (C.c_int, [_vp]),
+    "vfvm_batch_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "vfvm_mpnn_time_layer": (C.c_int, [_vp, C.c_int64, C.c_int, _dp]),
     "vfvm_batch_update_member": (C.c_int, [_vp, C.c_int64, _dp, C.POINTER(BCT)]),
     "vfvm_batch_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_batch_newton": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp), C.c_double, C.c_double, C.c_double,
@@ -164,6 +168,34 @@ class Context:
         if getattr(self, "handle", None) and _LIB is not None:
             _LIB.vfvm_ctx_destroy(self.handle)
             self.handle = None
+
+    def probe(self, *kernels: int, max_launches: int = 200000):
+        """Context manager timing every launch of the given kernel classes (KERNEL_*)
+        inside the calls it wraps, with HIP events on the launching stream.  Yields a
+        dict {kernel: {total_ms, launches, avg_ms}} filled on exit."""
+        return _Probe(self, kernels, max_launches)
+
+
+KERNEL_ASSEMBLE, KERNEL_RESIDUAL, KERNEL_SPMV64, KERNEL_SPMV32 = 0, 1, 2, 3
+
+
+class _Probe:
+    def __init__(self, ctx, kernels, max_launches):
+        self.ctx, self.kernels, self.max_launches = ctx, kernels, max_launches
+        self.result = {}
+
+    def __enter__(self):
+        for k in self.kernels:
+            check(lib().vfvm_probe_arm(self.ctx.handle, k, self.max_launches), self.ctx.handle)
+        return self.result
+
+    def __exit__(self, *exc):
+        for k in self.kernels:
+            t = C.c_double()
+            n = C.c_int64()
+            check(lib().vfvm_probe_read(self.ctx.handle, k, C.byref(t), C.byref(n)), self.ctx.handle)
+            self.result[k] = dict(total_ms=t.value, launches=n.value, avg_ms=t.value / max(n.value, 1))
+        return False
 
 
 _DEFAULT_CTX = None

@@ -23,6 +23,7 @@ def main():
     ap.add_argument("--samples", type=int, default=10000)
     ap.add_argument("--epochs", type=int, default=24)
     ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--out", default=S.WEIGHTS, help="where to write the trained parameters")
     args = ap.parse_args()
     t0 = time.time()
     fam = S.PoissonFamily(size=256)
@@ -46,8 +47,8 @@ def main():
         for e in range(ep):
             loss = net.train(X, T, epochs=1, batch=args.batch, lr=lr)
             print(f"epoch lr={lr:g} loss={loss:.3e} ({time.time() - t0:.1f}s)", flush=True)
-    net.save(samples=len(seeds), epochs=args.epochs, final_loss=loss)
-    print(f"saved {S.WEIGHTS} loss={loss:.3e} total {time.time() - t0:.1f}s", flush=True)
+    net.save(args.out, samples=len(seeds), epochs=args.epochs, final_loss=loss)
+    print(f"saved {args.out} loss={loss:.3e} total {time.time() - t0:.1f}s", flush=True)
     return 0
 
 
