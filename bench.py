@@ -2,10 +2,24 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 
-Prints ONE JSON line (metric, value, unit, ... roofline, cpu_baseline).  A
-"step" is one converged solve of the workload (an implicit-Euler time step
-solved by Newton to ||F||_inf <= 1e-10 over non-penalty rows).  Inputs are
-resident on the device before the timed region; only the solve calls are timed.
+Prints ONE JSON line (metric, value, unit, ... roofline, cpu_baseline).  The
+value counts converged solves: Newton to ||F||_inf <= 1e-10 over non-penalty
+rows.  Inputs are resident on the device before the timed region; only the
+solve calls are timed.  Workloads:
+
+  cfg4 (default)  1,024 held-out 129x129 nonlinear-Poisson inputs: learned initial
+                  guess + batches of 256 + FP32 Krylov, against the plain FP64 GPU
+                  Newton from zero on the same inputs; a step is one batch of 256
+  cfg1            129^3 nonlinear diffusion, implicit Euler; a step is one time step
+  cfg2            512^2 bipolar drift-diffusion bias sweep; a step is one sweep
+  cfg3            256 seeded nonlinear-Poisson instances of uneven size; a step is
+                  one batched solve of all 256
+
+The roofline entries come from the library's kernel probe: right after the timed
+solves, the same solves run once more with every launch of the kernel bracketed
+by HIP events on the library's stream (batched Krylov blocks then launch kernel
+by kernel instead of as HIP graphs), and the work those launches did is counted
+alongside.
 """
 from __future__ import annotations
 
@@ -36,10 +50,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_ms(sys_, kernel, reps=20):
-    v = C.c_double()
-    _capi.check(_capi.lib().vfvm_time_kernel(sys_.handle(), kernel, reps, C.byref(v)), sys_.ctx.handle)
-    return v.value
+def spmv_bytes(rows, nnz, sv):
+    """Algorithmic bytes of y = A x: values + column indices, row pointers, x and y once."""
+    return nnz * (sv + 4) + (rows + 1) * 4 + 2 * rows * sv
+
+
+def assembly_bytes(nspec, nodes, adj, slots, transient=True):
+    """Algorithmic bytes of one Jacobian assembly: U, F (and Uold), the node factor,
+    per edge its factor and two int32 endpoints, and the Jacobian values."""
+    return 8 * nspec * nodes * (3 if transient else 2) + 8 * nodes + (adj // 2) * (8 + 8) + 8 * slots
+
+
+def probe_bytes(kind, probe, sv=4, nspec=1, transient=True):
+    """Algorithmic bytes per launch from the work the timed launches did."""
+    w, n = probe["work"], max(probe["launches"], 1)
+    if kind == "spmv":
+        return spmv_bytes(w[0] / n, w[1] / n, sv)
+    return assembly_bytes(nspec, w[0] / n, w[1] / n, w[2] / n, transient)
 
 
 def roofline_hbm(algorithmic_bytes, probe, traffic, name):
@@ -101,11 +128,12 @@ def run_cfg1(args):
             warm = np.cumsum([0.0] + [dt] * W)
             vfvm.solve(s, inival=np.asfortranarray(u0), control=ctrl, times=warm)
         times = np.cumsum([0.0] + [dt] * K)
-        # the kernels' launches inside the timed region are bracketed by HIP events
+        t1 = time.perf_counter()
+        U = vfvm.solve(s, inival=np.asfortranarray(u0), control=ctrl, times=times)
+        el = time.perf_counter() - t1
+        # probe pass: the same solve again with the kernels' launches bracketed by HIP events
         with s.ctx.probe(*kernels) as pr:
-            t1 = time.perf_counter()
-            U = vfvm.solve(s, inival=np.asfortranarray(u0), control=ctrl, times=times)
-            el = time.perf_counter() - t1
+            vfvm.solve(s, inival=np.asfortranarray(u0), control=ctrl, times=times)
         probes.update(pr)
         # converged = every step met ||F|| <= 1e-10: check the final state's residual
         Ud = np.ascontiguousarray(vfvm.dofs(U))
@@ -119,9 +147,9 @@ def run_cfg1(args):
     err32 = float(np.abs(res["fp32_refine"]["U"] - res["plain_fp64"]["U"]).max())
     # roofline of the hot kernels at the final state
     n_, E = s.grid.num_nodes, nedges
-    b_asm = 8 * 1 * n_ * 3 + 8 * n_ + E * (8 + 8) + 8 * nnz
-    b_spmv64 = nnz * (8 + 4) + (ndof + 1) * 4 + 2 * ndof * 8
-    b_spmv32 = nnz * (4 + 4) + (ndof + 1) * 4 + 2 * ndof * 4
+    b_asm = probe_bytes("assemble", probes[_capi.KERNEL_ASSEMBLE])
+    b_spmv64 = probe_bytes("spmv", probes[_capi.KERNEL_SPMV64], sv=8)
+    b_spmv32 = probe_bytes("spmv", probes[_capi.KERNEL_SPMV32], sv=4)
     rl = {}
     for name, b, k in (("assemble", b_asm, _capi.KERNEL_ASSEMBLE), ("spmv64", b_spmv64, _capi.KERNEL_SPMV64),
                        ("spmv32", b_spmv32, _capi.KERNEL_SPMV32)):
@@ -189,11 +217,12 @@ def run_cfg2(args):
         ctrl = cfg2_control(prec)
         for _ in range(W):
             vfvm.solve(s, inival=u0, control=ctrl, embed=CFG2_BIAS)
-        with s.ctx.probe(*kernels) as pr:
-            t1 = time.perf_counter()
-            for _ in range(K):
-                U = vfvm.solve(s, inival=u0, control=ctrl, embed=CFG2_BIAS)
-            el = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        for _ in range(K):
+            U = vfvm.solve(s, inival=u0, control=ctrl, embed=CFG2_BIAS)
+        el = time.perf_counter() - t1
+        with s.ctx.probe(*kernels) as pr:  # probe pass: one more sweep, launches bracketed by events
+            vfvm.solve(s, inival=u0, control=ctrl, embed=CFG2_BIAS)
         probes.update(pr)
         if not s.history.residual_norm <= 1e-10:
             raise RuntimeError(f"{label}: a bias step ended with ||F||_inf = {s.history.residual_norm:.3e}")
@@ -202,8 +231,8 @@ def run_cfg2(args):
         log(f"{label}: {K} sweeps x {nsolve} solves in {el:.3f}s -> {nsolve * K / el:.2f} solves/s, newton "
             f"{s.history.niter}, krylov {s.history.nlin} per sweep")
     diff = float(np.abs(res["fp32_refine"]["U"] - res["plain_fp64"]["U"]).max())
-    b_spmv32 = nnz * (4 + 4) + (ndof + 1) * 4 + 2 * ndof * 4
-    b_asm = 8 * 3 * n * n * 2 + 8 * n * n + nedges * (8 + 8) + 8 * nnz
+    b_spmv32 = probe_bytes("spmv", probes[_capi.KERNEL_SPMV32], sv=4)
+    b_asm = probe_bytes("assemble", probes[_capi.KERNEL_ASSEMBLE], nspec=3, transient=False)
     rl = {}
     for name, b, k in (("spmv32", b_spmv32, _capi.KERNEL_SPMV32), ("assemble", b_asm, _capi.KERNEL_ASSEMBLE)):
         rl[name] = roofline_hbm(b, probes[k], read_traffic("cfg2_" + name), name)
@@ -269,7 +298,9 @@ def run_cfg3(args):
         f"autotuned groups {batch.groups}")
     K, W = args.steps, args.warmup
     res = {}
-    for label, prec in (("batched_fp64", "fp64"), ("batched_fp32", "fp32")):
+    probes = {}
+    for label, prec, kernels in (("batched_fp64", "fp64", (_capi.KERNEL_SPMV64,)),
+                                 ("batched_fp32", "fp32", (_capi.KERNEL_SPMV32, _capi.KERNEL_ASSEMBLE))):
         ctrl = cfg3_control(prec)
         for _ in range(W):
             batch.solve(0.0, ctrl)
@@ -277,6 +308,9 @@ def run_cfg3(args):
         for _ in range(K):
             U = batch.solve(0.0, ctrl)
         el = time.perf_counter() - t1
+        with systems[0].ctx.probe(*kernels) as pr:  # probe pass (launched one by one, not as graphs)
+            batch.solve(0.0, ctrl)
+        probes.update(pr)
         worst = max(h.residual_norm for h in batch.histories)
         if not worst <= 1e-10:
             raise RuntimeError(f"{label}: a member ended with ||F||_inf = {worst:.3e}")
@@ -298,23 +332,18 @@ def run_cfg3(args):
     log(f"sequential plain GPU Newton: {len(systems)} solves in {el:.2f}s -> {plain:.1f} solves/s")
     diff64 = max(float(np.abs(a - b).max()) for a, b in zip(res["batched_fp64"]["U"], Useq))
     diff32 = max(float(np.abs(a - b).max()) for a, b in zip(res["batched_fp32"]["U"], Useq))
-    # roofline of the batched SpMV / assembly are those of the single-system kernels; report the
-    # largest member's kernels measured live
-    big = max(systems, key=lambda s: s.grid.num_nodes)
-    vfvm.solve(big, inival=0.0, control=ctrl)
-    nd, nz, ne = big.sizes()
-    b_spmv = nz * (8 + 4) + (nd + 1) * 4 + 2 * nd * 8
-    t_spmv = kernel_ms(big, 2)
-    gbs = b_spmv / (t_spmv * 1e-3) / 1e9
+    p32, pasm, p64 = probes[_capi.KERNEL_SPMV32], probes[_capi.KERNEL_ASSEMBLE], probes[_capi.KERNEL_SPMV64]
+    rl = roofline_hbm(probe_bytes("spmv", p32, sv=4), p32, read_traffic("cfg3_bspmv32"), "batched FP32 SpMV")
+    rl["kernel"] = "batched FP32 SpMV over the active members (k_bspmv)"
+    rl_asm = roofline_hbm(probe_bytes("assemble", pasm, transient=False), pasm, read_traffic("cfg3_assemble"),
+                          "batched assembly")
+    rl64 = roofline_hbm(probe_bytes("spmv", p64, sv=8), p64, read_traffic("cfg3_bspmv64"), "batched FP64 SpMV")
     return dict(value=res["batched_fp32"]["solves_per_s"], ms_per_step=1e3 * res["batched_fp32"]["seconds"] / K,
                 plain_gpu_newton_solves_per_s=plain,
                 speedup_vs_plain=res["batched_fp32"]["solves_per_s"] / plain,
                 batched_fp64_solves_per_s=res["batched_fp64"]["solves_per_s"],
                 max_abs_diff_vs_plain=dict(batched_fp64=diff64, batched_fp32=diff32),
-                roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                              frac=round(gbs / HBM_PEAK_GBS, 4), traffic=read_traffic("cfg3_spmv64"),
-                              algorithmic_bytes=int(b_spmv), avg_ms=round(t_spmv, 5),
-                              kernel="FP64 SpMV of the largest member (257^2)"),
+                roofline=rl, roofline_assemble=rl_asm, roofline_spmv64=rl64,
                 config=dict(workload="cfg3: 256 nonlinear-Poisson instances, N x N grids with N in "
                                      "{33,65,129,257}, seeded eps/boundary values/source, batched "
                                      "BiCGStab+Jacobi, FP32 inside FP64 Newton (plain: 256 sequential FP64 "
@@ -352,21 +381,29 @@ def run_cfg4(args):
     ctrl32 = vfvm.SolverControl(precision="fp32", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
                                 inner_reltol=1e-3, max_refinements=12, check_every=8)
     # warm-up (untimed): one block through both batched variants
-    feats = fam.load_seeds(blocks[0])
+    raw = fam.load_seeds(blocks[0])
     for _ in range(max(args.warmup, 0)):
-        g = net.predict(feats)
+        g = net.predict_sides(*raw)
         fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
         fam.batch.solve(0.0, ctrl32)
     t_sur, t_zero, it_sur, it_zero, sur_err, worst = 0.0, 0.0, [], [], [], 0.0
     converged = []
+    ctx = fam.systems[0].ctx
+    probes = {_capi.KERNEL_SPMV32: [], _capi.KERNEL_ASSEMBLE: []}
     for blk in blocks:
-        feats = fam.load_seeds(blk)  # host input preparation and upload: not timed
+        raw = fam.load_seeds(blk)  # the inputs (source, side values) and their upload: not timed
         t1 = time.perf_counter()
-        g = net.predict(feats)
+        g = net.predict_sides(*raw)  # features (on the device) and inference are part of the method
         U = fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
         t_sur += time.perf_counter() - t1
         it_sur += [h.niter for h in fam.batch.histories]
         worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
+        # probe pass: the same batch again with the kernel launches bracketed by HIP events
+        # (launched one by one: the probe times single launches, not graphs)
+        with ctx.probe(_capi.KERNEL_SPMV32, _capi.KERNEL_ASSEMBLE) as pr:
+            fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
+        for k, v in pr.items():
+            probes[k].append(v)
         t1 = time.perf_counter()
         U0 = fam.batch.solve(0.0, ctrl32)
         t_zero += time.perf_counter() - t1
@@ -407,21 +444,27 @@ def run_cfg4(args):
     plain_rate = nsol / t_plain
     log(f"plain FP64 GPU Newton from zero: {nsol} solves in {t_plain:.2f}s -> {plain_rate:.1f} solves/s, "
         f"mean newton {np.mean(it_plain):.2f}; max |U_sur - U_plain| = {diff:.2e}")
-    # roofline: batched FP32 SpMV (the Krylov hot op) and the network's layer GEMM
+    # roofline: batched FP32 SpMV (the Krylov hot op) and the Jacobian assembly, timed live
+    # inside the surrogate solves (work counted over the members still active), and the
+    # network's layer GEMM
+    def merged(lst):
+        return dict(total_ms=sum(x["total_ms"] for x in lst), launches=sum(x["launches"] for x in lst),
+                    avg_ms=sum(x["total_ms"] for x in lst) / max(sum(x["launches"] for x in lst), 1),
+                    work=tuple(sum(x["work"][i] for x in lst) for i in range(3)))
+    pspmv, pasm = merged(probes[_capi.KERNEL_SPMV32]), merged(probes[_capi.KERNEL_ASSEMBLE])
+    rl_spmv = roofline_hbm(probe_bytes("spmv", pspmv, sv=4), pspmv, read_traffic("cfg4_bspmv32"),
+                           "batched FP32 SpMV")
+    rl_spmv["kernel"] = "batched FP32 SpMV over the active members of 256 x 129^2 (k_bspmv)"
+    rl_asm = roofline_hbm(probe_bytes("assemble", pasm, transient=False), pasm, read_traffic("cfg4_assemble"),
+                          "batched assembly")
     import ctypes as C
     v = C.c_double()
-    _capi.check(_capi.lib().vfvm_batch_time_kernel(fam.batch._h, 3, 20, C.byref(v)))
-    t_spmv = v.value
-    ndof = fam.grid.num_nodes * 256
-    nnz = fam.systems[0].sizes()[1] * 256
-    b_spmv = nnz * (4 + 4) + (ndof + 1) * 4 + 2 * ndof * 4
-    gbs = b_spmv / (t_spmv * 1e-3) / 1e9
     rows = 16 * fam.grid.num_nodes
     _capi.check(_capi.lib().vfvm_mpnn_time_layer(net._h, rows, 20, C.byref(v)))
     t_gemm = v.value
     tflops = 2.0 * rows * 128 * 64 / (t_gemm * 1e-3) / 1e12
     gemm_bytes = rows * (128 * 2 + 64 * 4 * 3 + 64 * 2)
-    return dict(value=sur_rate, ms_per_step=1e3 * t_sur / K, plain_gpu_newton_solves_per_s=plain_rate,
+    return dict(value=sur_rate, ms_per_step=1e3 * t_sur / K, steps=K, plain_gpu_newton_solves_per_s=plain_rate,
                 speedup_vs_plain=sur_rate / plain_rate, batched_fp32_from_zero_solves_per_s=zero_rate,
                 newton_iterations=dict(surrogate=float(np.mean(it_sur)), zero_batched=float(np.mean(it_zero)),
                                        plain=float(np.mean(it_plain))),
@@ -429,10 +472,7 @@ def run_cfg4(args):
                 max_abs_diff_vs_plain=diff, worst_residual=worst,
                 surrogate=dict(train_samples=int(meta["samples"]), epochs=int(meta["epochs"]),
                                final_train_mse=float(meta["final_loss"])),
-                roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                              frac=round(gbs / HBM_PEAK_GBS, 4), traffic=read_traffic("cfg4_bspmv32"),
-                              algorithmic_bytes=int(b_spmv), avg_ms=round(t_spmv, 5),
-                              kernel="batched FP32 SpMV, 256 members of 129^2"),
+                roofline=rl_spmv, roofline_assemble=rl_asm,
                 roofline_mpnn_gemm=dict(bound="mfma", achieved=round(tflops, 2), peak=2500.0, unit="TFLOP/s",
                                         frac=round(tflops / 2500.0, 5), traffic=read_traffic("cfg4_mpnn_gemm"),
                                         algorithmic_bytes=int(gemm_bytes),
@@ -466,15 +506,47 @@ def cpu_cfg4(args):
                        f"direct solves, {t:.2f} s")
 
 
-WORKLOADS = {"cfg1": (run_cfg1, cpu_cfg1), "cfg2": (run_cfg2, cpu_cfg2), "cfg3": (run_cfg3, cpu_cfg3), "cfg4": (run_cfg4, cpu_cfg4)}
+# --------------------------------------------------------------------------- copy
+def run_copy(args):
+    """Device-to-device copy of 1 GiB (FP32, torch's elementwise copy kernel): the
+    measured HBM bandwidth reference, and the kernel of known size against which the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters are scaled."""
+    import torch
+    n = 256 << 20
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(max(args.warmup, 1)):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbs = 2 * 4 * n / (ms * 1e-3) / 1e9
+    log(f"copy 1 GiB: {ms:.3f} ms -> {gbs:.0f} GB/s read+write")
+    return dict(value=gbs, ms_per_step=ms, unit="GB/s", metric="HBM copy bandwidth (read + write bytes / time)",
+                roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None, algorithmic_bytes=8 * n,
+                              avg_ms=round(ms, 5)),
+                config=dict(workload="copy: 1 GiB FP32 device-to-device (torch copy kernel)"), dtype="f32")
+
+
+def cpu_copy(args):
+    return None
+
+
+WORKLOADS = {"copy": (run_copy, cpu_copy), "cfg1": (run_cfg1, cpu_cfg1), "cfg2": (run_cfg2, cpu_cfg2), "cfg3": (run_cfg3, cpu_cfg3), "cfg4": (run_cfg4, cpu_cfg4)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg1", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cfg4", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -483,7 +555,8 @@ def main():
         sys.exit(2)
     run, cpu = WORKLOADS[args.workload]
     r = run(args)
-    line = dict(metric=METRIC, value=r.pop("value"), unit="solves/s", n_gpus=1, steps=args.steps,
+    line = dict(metric=r.pop("metric", METRIC), value=r.pop("value"), unit=r.pop("unit", "solves/s"), n_gpus=1,
+                steps=r.pop("steps", args.steps),
                 warmup=args.warmup, ms_per_step=r.pop("ms_per_step"), higher_is_better=True, scaling="weak",
                 vs_baseline=None, dtype=r.pop("dtype"), data="synthetic", config=r.pop("config"))
     line.update(r)

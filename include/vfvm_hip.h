@@ -188,7 +188,10 @@ int vfvm_residual_norm(vfvm_sys *sys, const double *U, const double *Uold, doubl
                        double tstep, double lambda, double *norm);
 
 /* Autotune the lanes-per-node / lanes-per-row group size of the assembly and SpMV
- * kernels for this system (project-written: times each legal size on the device). */
+ * kernels for this system (project-written: times each legal size on the device).
+ * Assembly groups: 2..32 lanes per node; 1 = one node per lane with the block's
+ * adjacency and Jacobian rows staged in LDS; 0 = one node per lane with only the
+ * Jacobian rows staged. */
 int vfvm_autotune(vfvm_sys *sys, int32_t *asm_group, int32_t *spmv_group);
 
 /* Benchmark helper (no counterpart in the reference; it has no device): average device
@@ -207,10 +210,12 @@ int vfvm_time_kernel(vfvm_sys *sys, int kernel, int reps, double *avg_ms);
  * class (VFVM_KERNEL_ASSEMBLE: Newton-step Jacobian assembly; _SPMV64 / _SPMV32: Krylov
  * operator applications), single-system or batched, is bracketed by HIP events on the
  * launching stream, up to max_launches launches.  Several classes may be armed at once.
- * vfvm_probe_read returns one class's summed duration and number of timed launches and
- * disarms that class. */
+ * vfvm_probe_read returns one class's summed duration, the number of timed launches and
+ * (work, 3 entries, nullable) the work those launches did, then disarms that class:
+ * SpMV: rows, nonzeros, 0; assembly: nodes, adjacency entries without the node itself
+ * (2 x edges), Jacobian slots.  In batches only the members still active count. */
 int vfvm_probe_arm(vfvm_ctx *ctx, int kernel, int64_t max_launches);
-int vfvm_probe_read(vfvm_ctx *ctx, int kernel, double *total_ms, int64_t *launches);
+int vfvm_probe_read(vfvm_ctx *ctx, int kernel, double *total_ms, int64_t *launches, int64_t *work);
 
 /* ---- device-resident interface (pointers are device pointers on the handle's device) ----
  * For hosts that keep U on the GPU (e.g. a Julia AMDGPU.jl array): same contracts as the host
@@ -228,6 +233,10 @@ int vfvm_assemble_device(vfvm_sys *sys, const double *dU, const double *dUold, d
 typedef struct vfvm_batch vfvm_batch;
 int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfvm_batch **out);
 int vfvm_batch_destroy(vfvm_batch *b);
+
+/* Batched Krylov iterations between status reads run as captured HIP graphs (default on);
+ * enable = 0 launches the same kernels one by one.  No counterpart in the reference. */
+int vfvm_batch_set_graphs(vfvm_batch *b, int enable);
 /* replace member m's source (nspec x nnodes; NULL keeps it) and boundary table (NULL keeps
  * it) inside the batch, for repeated solves of one grid with new data */
 int vfvm_batch_update_member(vfvm_batch *b, int64_t m, const double *src, const vfvm_bc_t *bc);
@@ -264,6 +273,14 @@ int vfvm_mpnn_train(vfvm_mpnn *m, int64_t nsamples, const float *features, const
                     int64_t epochs, int64_t batch, float lr, float *final_loss);
 /* out: nsamples x nnodes predictions */
 int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, double *out);
+
+/* Inference with the 8 features of a 2D family with one source field and a Dirichlet
+ * value per side of the grid's bounding box, computed on the device: f (nsamples x nnodes
+ * float64 sources), sides (nsamples x 4: south, east, north, west), eps the diffusion
+ * coefficient (side decay length sqrt(eps / 2)).  Features: f / 5, asinh(f / 2),
+ * side value k * exp(-distance to side k / sqrt(eps / 2)), x, y. */
+int vfvm_mpnn_infer_sides(vfvm_mpnn *m, int64_t nsamples, const double *f, const double *sides, double eps,
+                          double *out);
 /* benchmark helper: average device time in ms of one hidden-layer GEMM over `rows` rows */
 int vfvm_mpnn_time_layer(vfvm_mpnn *m, int64_t rows, int reps, double *avg_ms);
 /* loss and gradient for one batch (checked against the CPU oracle) */

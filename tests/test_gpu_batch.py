@@ -35,3 +35,27 @@ def test_batch_member_failure_is_reported():
     batch = vfvm.Batch([p[0] for p in pairs])
     with pytest.raises(vfvm.ConvergenceError):
         batch.solve(0.0, vfvm.SolverControl(maxiters=1))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_graph_replays_match_eager(precision):
+    """The captured Krylov iteration blocks give the eager launches' results, on every
+    replay, with other device work and synchronisations between the replays."""
+    print("case batch graphs", precision, flush=True)
+    members = [cases.batch_member(seed, sizes=(17, 33)) for seed in range(8)]
+    pairs = [cases.nonlinear_poisson_2d(n, eps, f, tuple(v)) for (n, eps, v, f) in members]
+    systems = [p[0] for p in pairs]
+    ctrl = vfvm.SolverControl(precision=precision, reltol_linear=1e-10, abstol_linear=0.0, maxiters_linear=2000,
+                              check_every=4)
+    batch = vfvm.Batch(systems)
+    batch.set_graphs(False)
+    Ue = [vfvm.dofs(u).copy() for u in batch.solve(0.0, ctrl)]
+    it_e = [h.nlin for h in batch.histories]
+    batch.set_graphs(True)
+    for replay in range(3):
+        Ug = batch.solve(0.0, ctrl)
+        assert [h.nlin for h in batch.histories] == it_e
+        for a, b in zip(Ue, Ug):
+            assert np.array_equal(a, vfvm.dofs(b)), replay
+        # eager work in between: a single-system solve on the same context
+        vfvm.solve(systems[0], inival=0.0, control=ctrl)

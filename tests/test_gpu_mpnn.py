@@ -96,13 +96,19 @@ def test_trained_surrogate_saves_newton_iterations():
     net = S.Surrogate(fam.systems[0])
     net.load()
     seeds = list(S.EVAL_SEEDS)[:16]
-    feats = fam.load_seeds(seeds)
+    raw = fam.load_seeds(seeds)
+    feats = fam.features(*raw)
     ctrl = vfvm.SolverControl(precision="fp32", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=2000)
     U0 = fam.batch.solve(0.0, ctrl)
     it0 = [h.niter for h in fam.batch.histories]
     guess = net.predict(feats)
     Ug = fam.batch.solve([g.reshape(1, -1) for g in guess], ctrl)
     itg = [h.niter for h in fam.batch.histories]
+    # the device feature path gives the same prediction (bfloat16 input rounding aside)
+    guess_dev = net.predict_sides(*raw)
+    dev_diff = np.abs(guess_dev - guess).max()
+    print("device-feature prediction max diff", dev_diff, flush=True)
+    assert dev_diff <= 2e-2 * np.abs(guess).max()
     err = max(np.abs(g - vfvm.dofs(u)).max() for g, u in zip(guess, U0))
     diff = max(np.abs(vfvm.dofs(a) - vfvm.dofs(b)).max() for a, b in zip(U0, Ug))
     print("newton iterations from zero", sum(it0), "from surrogate", sum(itg), "surrogate max err", err, flush=True)

@@ -142,6 +142,31 @@ def test_block_jacobi_gmres_drift_diffusion_continuation():
         assert err <= SOL_ATOL
 
 
+def test_ilu0_bicgstab_drift_diffusion_three_species():
+    """cfg2 physics (lambda = 1e-2) on 65 x 65 nodes: BiCGStab with multicolour ILU(0)
+    of the 3-species dof graph, FP64 and FP32 inside FP64 refinement, against the
+    oracle's Newton with direct solves, bias steps 0, 0.25, 0.5."""
+    _print("case dd ilu0")
+    sys, osys = cases.drift_diffusion_2d(65, lambda2=1e-4)
+    U0 = vfvm.dofs(cases.dd_initial(sys.grid)).copy()
+    lam = [0.0, 0.25, 0.5]
+    Uo = U0
+    ref = []
+    for l in lam:
+        Uo, _ = osys.solve_step(Uo, embed=l)
+        ref.append(Uo)
+    for precision in ("fp64", "fp32"):
+        ctrl = vfvm.SolverControl(krylov="bicgstab", precon="ilu0", precision=precision, reltol_linear=1e-8,
+                                  abstol_linear=0.0, maxiters_linear=3000, max_refinements=20)
+        U = U0.reshape(3, -1, order="F")
+        for l, Ur in zip(lam, ref):
+            U = vfvm.solve(sys, inival=U, control=ctrl, embed=[l])
+            err = np.abs(vfvm.dofs(U) - Ur).max()
+            _print("dd ilu0", precision, l, "err", err, "newton", sys.history.niter, "krylov", sys.history.nlin)
+            assert err <= SOL_ATOL
+            assert sys.history.residual_norm <= 1e-10
+
+
 def test_transient_nonlinear_diffusion_3d():
     """cfg1 at small size: implicit Euler, 10 steps of 1e-2, BiCGStab + ILU(0) FP32 vs oracle."""
     _print("case transient nldiff3d")

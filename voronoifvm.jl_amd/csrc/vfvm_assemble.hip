@@ -40,6 +40,78 @@ __device__ __forceinline__ int rank_in(uint8_t mask, int j) {
   return __builtin_popcount((unsigned)mask & ((1u << j) - 1u));
 }
 
+// Node item (reaction, storage, source; src/vfvm_assembly.jl:81-122) and the
+// boundary items of node K (penalty Dirichlet, Robin; src/vfvm_assembly.jl:352-407).
+template <int NS, bool JAC>
+__device__ __forceinline__ void node_terms(const AsmArgs &a, const InstParams *P, const double (&p)[16],
+                                           const double (&uK)[NS], const double (&uo)[NS],
+                                           const double (&srcv)[NS], double vol, int32_t bb0, int32_t bb1,
+                                           double (&Fn)[NS], double (&Jn)[NS][NS], double (&Fb)[NS],
+                                           double (&Jb)[NS], bool &bad) {
+  {
+    double so[NS];
+    eval_storage<NS>(P->storage, uo, so);
+    if constexpr (JAC) {
+      Dual<NS> u[NS], r[NS], st[NS];
+#pragma unroll
+      for (int i = 0; i < NS; i++) u[i] = dvar<NS>(uK[i], i);
+      eval_reaction<NS>(P->reaction, p, u, r);
+      eval_storage<NS>(P->storage, u, st);
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        const double src = srcv[i];
+        Fn[i] = vol * (r[i].v - src + (st[i].v - so[i]) * a.tstepinv);
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+          const double v = r[i].d[j] + st[i].d[j] * a.tstepinv;
+          bad |= (v != v);
+          Jn[i][j] = v * vol;
+        }
+      }
+    } else {
+      double r[NS], st[NS];
+      eval_reaction<NS>(P->reaction, p, uK, r);
+      eval_storage<NS>(P->storage, uK, st);
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        const double src = srcv[i];
+        Fn[i] = vol * (r[i] - src + (st[i] - so[i]) * a.tstepinv);
+      }
+    }
+  }
+  // boundary items: src/vfvm_assembly.jl:352-407
+#pragma unroll
+  for (int i = 0; i < NS; i++) Fb[i] = Jb[i] = 0.0;
+  for (int32_t b = bb0; b < bb1; b++) {
+    const int reg = a.bn_reg[b];
+    const double fac = a.bn_fac[b];
+    if (reg < 1 || reg > MAXBREG) continue;
+    double penalty = DIRICHLET_PENALTY;
+    if (P->has_legacy) {
+      penalty = DIRICHLET_PENALTY / fac;
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        const BCDev &bc = P->bc[reg - 1][i];
+        if (bc.kind == VFVM_BC_DIRICHLET_LEGACY) {
+          Fb[i] += DIRICHLET_PENALTY * (uK[i] - (bc.value + bc.value_lambda * a.lambda));
+          Jb[i] += DIRICHLET_PENALTY;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      const BCDev &bc = P->bc[reg - 1][i];
+      if (bc.kind == VFVM_BC_DIRICHLET) {
+        Fb[i] += fac * (penalty * (uK[i] - (bc.value + bc.value_lambda * a.lambda)) - 0.0);
+        Jb[i] += penalty * fac;
+      } else if (bc.kind == VFVM_BC_ROBIN) {
+        Fb[i] += fac * ((bc.factor * uK[i] - (bc.value + bc.value_lambda * a.lambda)) - 0.0);
+        Jb[i] += bc.factor * fac;
+      }
+    }
+  }
+}
+
 template <int NS, int G, bool JAC>
 __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
   constexpr int NP = 2 * NS;  // edge partials: u_k then u_l
@@ -138,71 +210,8 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
   }
 
   if (lane == 0) {
-    // node item: src/vfvm_assembly.jl:81-122
-    double Fn[NS], Jn[NS][NS];
-    {
-      double so[NS];
-      eval_storage<NS>(P->storage, uo, so);
-      if constexpr (JAC) {
-        Dual<NS> u[NS], r[NS], st[NS];
-#pragma unroll
-        for (int i = 0; i < NS; i++) u[i] = dvar<NS>(uK[i], i);
-        eval_reaction<NS>(P->reaction, p, u, r);
-        eval_storage<NS>(P->storage, u, st);
-#pragma unroll
-        for (int i = 0; i < NS; i++) {
-          const double src = srcv[i];
-          Fn[i] = vol * (r[i].v - src + (st[i].v - so[i]) * a.tstepinv);
-#pragma unroll
-          for (int j = 0; j < NS; j++) {
-            const double v = r[i].d[j] + st[i].d[j] * a.tstepinv;
-            bad |= (v != v);
-            Jn[i][j] = v * vol;
-          }
-        }
-      } else {
-        double r[NS], st[NS];
-        eval_reaction<NS>(P->reaction, p, uK, r);
-        eval_storage<NS>(P->storage, uK, st);
-#pragma unroll
-        for (int i = 0; i < NS; i++) {
-          const double src = srcv[i];
-          Fn[i] = vol * (r[i] - src + (st[i] - so[i]) * a.tstepinv);
-        }
-      }
-    }
-    // boundary items: src/vfvm_assembly.jl:352-407
-    double Fb[NS], Jb[NS];
-#pragma unroll
-    for (int i = 0; i < NS; i++) Fb[i] = Jb[i] = 0.0;
-    for (int32_t b = bb0; b < bb1; b++) {
-      const int reg = a.bn_reg[b];
-      const double fac = a.bn_fac[b];
-      if (reg < 1 || reg > MAXBREG) continue;
-      double penalty = DIRICHLET_PENALTY;
-      if (P->has_legacy) {
-        penalty = DIRICHLET_PENALTY / fac;
-#pragma unroll
-        for (int i = 0; i < NS; i++) {
-          const BCDev &bc = P->bc[reg - 1][i];
-          if (bc.kind == VFVM_BC_DIRICHLET_LEGACY) {
-            Fb[i] += DIRICHLET_PENALTY * (uK[i] - (bc.value + bc.value_lambda * a.lambda));
-            Jb[i] += DIRICHLET_PENALTY;
-          }
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < NS; i++) {
-        const BCDev &bc = P->bc[reg - 1][i];
-        if (bc.kind == VFVM_BC_DIRICHLET) {
-          Fb[i] += fac * (penalty * (uK[i] - (bc.value + bc.value_lambda * a.lambda)) - 0.0);
-          Jb[i] += penalty * fac;
-        } else if (bc.kind == VFVM_BC_ROBIN) {
-          Fb[i] += fac * ((bc.factor * uK[i] - (bc.value + bc.value_lambda * a.lambda)) - 0.0);
-          Jb[i] += bc.factor * fac;
-        }
-      }
-    }
+    double Fn[NS], Jn[NS][NS], Fb[NS], Jb[NS];
+    node_terms<NS, JAC>(a, P, p, uK, uo, srcv, vol, bb0, bb1, Fn, Jn, Fb, Jb, bad);
 #pragma unroll
     for (int i = 0; i < NS; i++) a.F[K * NS + i] = (Fn[i] + Fe[i]) + Fb[i];
     if constexpr (JAC) {
@@ -224,11 +233,158 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
   if (JAC && bad) atomicOr(a.flags, 1);
 }
 
-template <int NS, int G>
+// One node per lane, ASM_BLOCK consecutive nodes per block.  The rows of the
+// sorted node graph of K0..K1-1 are one contiguous adjacency range, and their
+// Jacobian rows one contiguous slot range [j0, j1).  Both (or, with SADJ false,
+// only the Jacobian rows) are staged in LDS when they fit, so the block reads its adjacency and writes its Jacobian rows as
+// contiguous coalesced sweeps while each lane walks its own row in LDS.  The
+// Jacobian is staged only when every node of the block is assembled (then every
+// slot of the range is written exactly once before the sweep).
+constexpr int STREAM_ADJ = 8 * ASM_BLOCK;  // adjacency entries staged per block
+constexpr int STREAM_J = 8 * ASM_BLOCK;    // Jacobian slots staged per block
+
+template <int NS, bool JAC, bool SADJ>
+__global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
+  constexpr int NP = 2 * NS;
+  __shared__ int32_t s_col[SADJ ? STREAM_ADJ : 1];
+  __shared__ double s_fac[SADJ ? STREAM_ADJ : 1];
+  __shared__ double s_J[JAC ? STREAM_J : 1];
+  const int64_t K0 = (int64_t)blockIdx.x * ASM_BLOCK;
+  const int64_t K1 = K0 + ASM_BLOCK < a.nnodes ? K0 + ASM_BLOCK : a.nnodes;
+  const int64_t K = K0 + threadIdx.x;
+  const bool mine = K < K1;
+  const int32_t e0 = a.adj_ptr[K0], e1 = a.adj_ptr[K1];
+  const bool sadj = SADJ && (e1 - e0) <= STREAM_ADJ;
+  const int64_t j0 = (int64_t)(e0 - K0) * a.m.etot + K0 * a.m.dtot;
+  const int64_t j1 = (int64_t)(e1 - K1) * a.m.etot + K1 * a.m.dtot;
+  int inst = 0;
+  bool active = mine;
+  if (mine) {
+    inst = a.node_inst ? a.node_inst[K] : 0;
+    if (a.inst_active && !a.inst_active[inst]) active = false;
+  }
+  const bool sJ = JAC && (j1 - j0) <= STREAM_J && __syncthreads_and(active || !mine);
+  if (sadj)
+    for (int64_t q = e0 + threadIdx.x; q < e1; q += ASM_BLOCK) {
+      s_col[q - e0] = a.adj_col[q];
+      s_fac[q - e0] = a.adj_fac[q];
+    }
+  __syncthreads();
+  bool bad = false;
+  if (active) {
+    const InstParams *P = a.inst + inst;
+    const int flux_kind = P->flux;
+    double p[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) p[k] = P->p[k];
+    const int32_t a0 = a.adj_ptr[K], a1 = a.adj_ptr[K + 1];
+    const int deg1 = a1 - a0;
+    const int diag = a.adj_diag[K];
+    double uK[NS], uo[NS], srcv[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      uK[i] = a.U[K * NS + i];
+      uo[i] = a.Uold[K * NS + i];
+      srcv[i] = a.src ? a.src[K * NS + i] : 0.0;
+    }
+    const double vol = a.nodevol[K];
+    const int32_t bb0 = a.bn_ptr[K], bb1 = a.bn_ptr[K + 1];
+    int64_t rowbase[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++)
+      rowbase[i] = (int64_t)(a0 - K) * a.m.etot + K * a.m.dtot + (int64_t)(deg1 - 1) * a.m.eoff[i] + a.m.doff[i];
+    auto put = [&](int64_t slot, double v) {
+      if (sJ)
+        s_J[slot - j0] = v;
+      else
+        a.J[slot] = v;
+    };
+    double Fe[NS], Jd[NS][NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      Fe[i] = 0.0;
+#pragma unroll
+      for (int j = 0; j < NS; j++) Jd[i][j] = 0.0;
+    }
+    for (int e = 0; e < deg1; e++) {
+      if (e == diag) continue;
+      const int32_t M = sadj ? s_col[a0 - e0 + e] : a.adj_col[a0 + e];
+      const double f = sadj ? s_fac[a0 - e0 + e] : a.adj_fac[a0 + e];
+      if constexpr (JAC) {
+        Dual<NP> uk[NS], ul[NS], g[NS];
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          uk[i] = dvar<NP>(uK[i], i);
+          ul[i] = dvar<NP>(a.U[(int64_t)M * NS + i], NS + i);
+        }
+        eval_flux<NS>(flux_kind, p, uk, ul, g);
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          Fe[i] += f * g[i].v;
+          const int64_t boff =
+              rowbase[i] + (e < diag ? (int64_t)e * a.m.ecnt[i] : (int64_t)(e - 1) * a.m.ecnt[i] + a.m.dcnt[i]);
+          const uint8_t em = a.m.emask[i];
+#pragma unroll
+          for (int j = 0; j < NS; j++) {
+            if (em & (1u << j)) {
+              const double dk = g[i].d[j], dl = g[i].d[NS + j];
+              bad |= (dk != dk) | (dl != dl);
+              Jd[i][j] += f * dk;
+              put(boff + rank_in<NS>(em, j), f * dl);
+            }
+          }
+        }
+      } else {
+        double uk[NS], ul[NS], g[NS];
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          uk[i] = uK[i];
+          ul[i] = a.U[(int64_t)M * NS + i];
+        }
+        eval_flux<NS>(flux_kind, p, uk, ul, g);
+#pragma unroll
+        for (int i = 0; i < NS; i++) Fe[i] += f * g[i];
+      }
+    }
+    double Fn[NS], Jn[NS][NS], Fb[NS], Jb[NS];
+    node_terms<NS, JAC>(a, P, p, uK, uo, srcv, vol, bb0, bb1, Fn, Jn, Fb, Jb, bad);
+#pragma unroll
+    for (int i = 0; i < NS; i++) a.F[K * NS + i] = (Fn[i] + Fe[i]) + Fb[i];
+    if constexpr (JAC) {
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        const int64_t boff = rowbase[i] + (int64_t)diag * a.m.ecnt[i];
+        const uint8_t dm = a.m.dmask[i];
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+          if (dm & (1u << j)) {
+            double v = Jn[i][j] + Jd[i][j];
+            if (i == j) v += Jb[i];
+            put(boff + rank_in<NS>(dm, j), v);
+          }
+        }
+      }
+    }
+  }
+  if (sJ) {
+    __syncthreads();
+    for (int64_t q = j0 + threadIdx.x; q < j1; q += ASM_BLOCK) a.J[q] = s_J[q - j0];
+  }
+  if (JAC && bad) atomicOr(a.flags, 1);
+}
+
+template <int NS, int G, bool SADJ = true>
 void launch_ns_g(const AsmArgs &a, hipStream_t s) {
   const int64_t nodes_per_block = ASM_BLOCK / G;
   const int64_t grid = (a.nnodes + nodes_per_block - 1) / nodes_per_block;
   if (grid == 0) return;
+  if (G == 1) {  // one node per lane: the LDS-staged kernel (SADJ: adjacency staged too)
+    if (a.J)
+      hipLaunchKernelGGL((k_assemble_stream<NS, true, SADJ>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_assemble_stream<NS, false, SADJ>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
+    return;
+  }
   if (a.J)
     hipLaunchKernelGGL((k_assemble<NS, G, true>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
   else
@@ -237,14 +393,17 @@ void launch_ns_g(const AsmArgs &a, hipStream_t s) {
 
 template <int NS>
 void launch_ns(const AsmArgs &a, int group, hipStream_t s) {
+  // group: lanes per node; 1 and 0 = one node per lane with LDS staging of the
+  // adjacency and Jacobian rows (1) or of the Jacobian rows only (0)
   switch (group) {
+    case 0: launch_ns_g<NS, 1, false>(a, s); break;
     case 1: launch_ns_g<NS, 1>(a, s); break;
     case 2: launch_ns_g<NS, 2>(a, s); break;
     case 4: launch_ns_g<NS, 4>(a, s); break;
     case 8: launch_ns_g<NS, 8>(a, s); break;
     case 16: launch_ns_g<NS, 16>(a, s); break;
     case 32: launch_ns_g<NS, 32>(a, s); break;
-    default: throw Error(VFVM_ERR_ARG, "assembly group size must be 1, 2, 4, 8, 16 or 32");
+    default: throw Error(VFVM_ERR_ARG, "assembly group size must be 0, 1, 2, 4, 8, 16 or 32");
   }
 }
 

@@ -16,8 +16,10 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 
 #include "vfvm_sys.h"
 
@@ -26,7 +28,7 @@ using namespace vfvm;
 namespace {
 
 constexpr int BB = 256;             // threads per block
-constexpr int64_t CHUNK = 2048;     // rows per chunk
+constexpr int64_t CHUNK = 512;      // rows per chunk (small: members still iterating keep the CUs busy)
 
 struct Seg {
   int64_t nch;
@@ -123,33 +125,37 @@ __global__ void __launch_bounds__(BB) k_bfinalize(Seg S, KState *st, const int32
   }
 }
 
-// y = rowscale .* (A x) on the chunk's rows, fused (w0, y) and (y, y)
-template <class VA, class V, int G, int NDOT, int STAGE>
-__global__ void __launch_bounds__(BB) k_bspmv(Seg S, const int32_t *__restrict__ rowptr,
-                                              const int32_t *__restrict__ colind, const VA *__restrict__ val,
-                                              const V *__restrict__ x, V *__restrict__ y,
-                                              const double *__restrict__ rowscale, const V *__restrict__ w0,
-                                              KState *st) {
+// y = Ahat x on the chunk's rows (SELL-64: the chunk's rows are slices
+// ch_slice[c], ch_slice[c] + 1, ... of 64 rows; one lane per row, so a wave reads
+// entry j of its 64 rows in one coalesced access), fused (w0, y) and (y, y)
+template <class VA, class V, int NDOT, int STAGE>
+__global__ void __launch_bounds__(BB) k_bspmv(Seg S, const int32_t *__restrict__ ch_slice,
+                                              const int32_t *__restrict__ sptr, const int32_t *__restrict__ col,
+                                              const VA *__restrict__ val, const V *__restrict__ x, V *__restrict__ y,
+                                              const V *__restrict__ w0, KState *st, const int32_t *__restrict__ rowptr,
+                                              unsigned long long *work) {
   const int inst = S.ch_inst[blockIdx.x];
   KState *sti = st + inst;
   if (sti->done) return;
   if (STAGE == STAGE_OMEGA && sti->half) return;
   const int64_t r0 = S.ch_beg[blockIdx.x], r1 = S.ch_end[blockIdx.x];
-  const int lane = threadIdx.x & (G - 1);
+  if (work && threadIdx.x == 0) {  // probe counters: rows and nonzeros of this live chunk
+    atomicAdd(&work[0], (unsigned long long)(r1 - r0));
+    atomicAdd(&work[1], (unsigned long long)(rowptr[r1] - rowptr[r0]));
+  }
+  const int32_t sl0 = ch_slice[blockIdx.x];
   double d0 = 0.0, d1 = 0.0;
-  for (int64_t r = r0 + threadIdx.x / G; r < r1; r += BB / G) {
-    const int32_t k0 = rowptr[r], k1 = rowptr[r + 1];
+  for (int64_t l = threadIdx.x; l < r1 - r0; l += BB) {
+    const int32_t sl = sl0 + (int32_t)(l >> 6);
+    const int32_t p1 = sptr[sl + 1];
     double acc = 0.0;
-    for (int32_t k = k0 + lane; k < k1; k += G) acc += (double)val[k] * (double)x[colind[k]];
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
-    if (lane == 0) {
-      if (rowscale) acc *= rowscale[r];
-      const V yv = (V)acc;
-      y[r] = yv;
-      if (NDOT >= 1) d0 += (double)w0[r] * (double)yv;
-      if (NDOT >= 2) d1 += (double)yv * (double)yv;
-    }
+#pragma unroll 4
+    for (int32_t q = sptr[sl] + (int32_t)(l & 63); q < p1; q += 64) acc += (double)val[q] * (double)x[col[q]];
+    const int64_t r = r0 + l;
+    const V yv = (V)acc;
+    y[r] = yv;
+    if (NDOT >= 1) d0 += (double)w0[r] * (double)yv;
+    if (NDOT >= 2) d1 += (double)yv * (double)yv;
   }
   if constexpr (NDOT > 0) {
     double v[2] = {d0, d1};
@@ -335,6 +341,16 @@ struct vfvm_batch {
   DevBuf<double> k64[8];
   DevBuf<float> k32[8];
   DevBuf<float> A32;
+  // SELL-64 operator: slices start at every chunk start; ch_slice = first slice of a chunk
+  DevBuf<int32_t> sell_ptr, sell_col, sell_rbase, ch_slice;
+  DevBuf<double> A64s;
+  int64_t sell_nnz = 0;
+  // captured Krylov iteration blocks: (value bytes, solution vector, iterations) -> graph
+  bool use_graphs = true;
+  std::map<std::tuple<int, const void *, int>, hipGraphExec_t> graphs;
+  ~vfvm_batch() {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  }
   Seg seg() {
     return Seg{nch, ch_inst.p, ch_beg.p, ch_end.p, ch_first.p, tickets.p, partials.p};
   }
@@ -385,24 +401,17 @@ AsmArgs basm(vfvm_batch &b, double *F, double *J, double tstep, double lambda, c
 template <class V, int NDOT, int STAGE>
 void bop(vfvm_batch &b, const V *x, V *y, const V *w0) {
   const Seg S = b.seg();
-  auto go = [&](auto gc) {
-    constexpr int G = decltype(gc)::value;
-    if constexpr (sizeof(V) == 4)
-      hipLaunchKernelGGL((k_bspmv<float, V, G, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                         b.rowptr.p, b.colind.p, (const float *)b.A32.p, x, y, (const double *)nullptr, w0, b.st.p);
-    else
-      hipLaunchKernelGGL((k_bspmv<double, V, G, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                         b.rowptr.p, b.colind.p, (const double *)b.J.p, x, y, (const double *)b.rowscale.p, w0, b.st.p);
-  };
   const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
+  unsigned long long *work = b.ctx->probe.counters(kid);
   b.ctx->probe.before(kid, b.ctx->stream);
-  switch (b.spmv_group) {
-    case 2: go(std::integral_constant<int, 2>()); break;
-    case 4: go(std::integral_constant<int, 4>()); break;
-    case 8: go(std::integral_constant<int, 8>()); break;
-    case 16: go(std::integral_constant<int, 16>()); break;
-    default: go(std::integral_constant<int, 32>()); break;
-  }
+  if constexpr (sizeof(V) == 4)
+    hipLaunchKernelGGL((k_bspmv<float, V, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                       b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const float *)b.A32.p, x, y, w0, b.st.p,
+                       b.rowptr.p, work);
+  else
+    hipLaunchKernelGGL((k_bspmv<double, V, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                       b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const double *)b.A64s.p, x, y, w0, b.st.p,
+                       b.rowptr.p, work);
   b.ctx->probe.after(kid, b.ctx->stream);
   if constexpr (NDOT > 0)
     hipLaunchKernelGGL((k_bfinalize<2, STAGE, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, b.ctx->stream, S,
@@ -464,7 +473,8 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
   const int every = std::max(1, check_every);
   int launched = 0;
   std::vector<KState> h;
-  while (true) {
+  // `every` iterations between status reads; kernels of finished members return at entry
+  auto body = [&]() {
     for (int q = 0; q < every; q++) {
       hipLaunchKernelGGL((k_bp<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, r, p, v, b.st.p);
       bop<V, 1, STAGE_ALPHA>(b, p, v, r0);
@@ -476,6 +486,33 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
       hipLaunchKernelGGL((k_bfinalize<2, STAGE_XR, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S, b.st.p,
                          (const int32_t *)nullptr, (double *)nullptr);
     }
+  };
+  // The block is captured once into a HIP graph (its kernel arguments are fixed for the
+  // batch: chunk tables, Krylov vectors, solver states) and replayed; it holds kernel
+  // nodes only.  Not while the kernel probe is armed: that brackets single launches.
+  hipGraphExec_t ge = nullptr;
+  if (b.use_graphs && !b.ctx->probe.any_armed()) {
+    const auto key = std::make_tuple((int)sizeof(V), (const void *)x, every);
+    auto it = b.graphs.find(key);
+    if (it != b.graphs.end()) {
+      ge = it->second;
+    } else {
+      hipGraph_t g = nullptr;
+      VFVM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      body();
+      const hipError_t ce = hipStreamEndCapture(st, &g);
+      if (ce != hipSuccess) throw Error(VFVM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+      const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) throw Error(VFVM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+      b.graphs[key] = ge;
+    }
+  }
+  while (true) {
+    if (ge)
+      VFVM_HIP_CHECK(hipGraphLaunch(ge, st));
+    else
+      body();
     VFVM_HIP_CHECK(hipGetLastError());
     launched += every;
     h = read_states(b);
@@ -496,8 +533,8 @@ void bsolve(vfvm_batch &b, const vfvm_control_t &c, std::vector<int64_t> &nlin) 
   hipStream_t st = b.ctx->stream;
   const Seg S = b.seg();
   const bool fp32 = c.precision == VFVM_PREC_FP32_REFINE;
-  launch_scale_rows(b.nspec, b.nnodes, b.rowptr.p, b.colind.p, b.J.p, b.dofdiag.p, b.rowscale.p,
-                    fp32 ? b.A32.p : nullptr, nullptr, st);
+  launch_scale_rows(b.nspec, b.nnodes, b.rowptr.p, b.colind.p, b.J.p, b.dofdiag.p, b.sell_rbase.p, b.rowscale.p,
+                    fp32 ? b.A32.p : nullptr, fp32 ? nullptr : b.A64s.p, nullptr, st);
   // bhat = S F
   bres(b, nullptr, b.F.p, b.bhat.p, b.active.p);
   if (!fp32) {
@@ -678,6 +715,15 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
     }
     cf[nsys] = (int32_t)ci.size();
     b->nch = (int64_t)ci.size();
+    {
+      std::vector<int32_t> sptr, scol, chs, rbase;
+      build_sell64(b->ndof, rowptr, colind, cb, ce, sptr, scol, chs, rbase);
+      b->sell_ptr.upload(sptr, st);
+      b->sell_col.upload(scol, st);
+      b->sell_rbase.upload(rbase, st);
+      b->ch_slice.upload(chs, st);
+      b->sell_nnz = (int64_t)scol.size();
+    }
     b->ch_inst.upload(ci, st);
     b->ch_first.upload(cf, st);
     b->ch_beg.upload(cb, st);
@@ -693,7 +739,11 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
     b->norms.alloc((size_t)2 * nsys);
     for (auto &v : b->k64) v.alloc((size_t)b->ndof);
     for (auto &v : b->k32) v.alloc((size_t)b->ndof);
-    b->A32.alloc((size_t)b->nnz);
+    // SELL-64 operator values; padding entries stay 0
+    b->A32.alloc((size_t)b->sell_nnz);
+    b->A64s.alloc((size_t)b->sell_nnz);
+    hipLaunchKernelGGL((k_bfill<float>), dim3(gfor(b->sell_nnz)), dim3(BB), 0, st, b->sell_nnz, b->A32.p, 0.0f);
+    hipLaunchKernelGGL((k_bfill<double>), dim3(gfor(b->sell_nnz)), dim3(BB), 0, st, b->sell_nnz, b->A64s.p, 0.0);
     b->asm_group = systems[0]->asm_group;
     b->spmv_group = systems[0]->spmv_group;
     int g = 4;
@@ -706,6 +756,12 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
     *out = b.release();
     return VFVM_OK;
   });
+}
+
+int vfvm_batch_set_graphs(vfvm_batch *b, int enable) {
+  if (!b) return VFVM_ERR_ARG;
+  b->use_graphs = enable != 0;
+  return VFVM_OK;
 }
 
 int vfvm_batch_destroy(vfvm_batch *b) {
@@ -725,8 +781,8 @@ int vfvm_batch_time_kernel(vfvm_batch *b, int kernel, int reps, double *avg_ms) 
     b->active.upload(all.data(), all.size(), st);
     AsmArgs a = basm(*b, b->F.p, b->J.p, 1.0, 0.0, b->active.p);
     launch_assemble(a, b->asm_group, st);
-    launch_scale_rows(b->nspec, b->nnodes, b->rowptr.p, b->colind.p, b->J.p, b->dofdiag.p, b->rowscale.p, b->A32.p,
-                      nullptr, st);
+    launch_scale_rows(b->nspec, b->nnodes, b->rowptr.p, b->colind.p, b->J.p, b->dofdiag.p, b->sell_rbase.p,
+                      b->rowscale.p, b->A32.p, b->A64s.p, nullptr, st);
     reset_states(*b, 0.0, 0.0, 1);
     auto once = [&]() {
       if (kernel == 0)
@@ -738,19 +794,8 @@ int vfvm_batch_time_kernel(vfvm_batch *b, int kernel, int reps, double *avg_ms) 
       else
         throw Error(VFVM_ERR_ARG, "unknown kernel id");
     };
-    once();
-    hipEvent_t e0, e1;
-    VFVM_HIP_CHECK(hipEventCreate(&e0));
-    VFVM_HIP_CHECK(hipEventCreate(&e1));
-    VFVM_HIP_CHECK(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; r++) once();
-    VFVM_HIP_CHECK(hipEventRecord(e1, st));
-    VFVM_HIP_CHECK(hipEventSynchronize(e1));
-    float ms = 0;
-    VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *avg_ms = (double)ms / reps;
+    ColdTimer timer(st);
+    *avg_ms = (double)timer.time(once, reps);
     return VFVM_OK;
   });
 }
@@ -788,22 +833,11 @@ int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group) 
     hipStream_t st = b->ctx->stream;
     std::vector<int32_t> all((size_t)b->nsys, 1);
     b->active.upload(all.data(), all.size(), st);
-    hipEvent_t e0, e1;
-    VFVM_HIP_CHECK(hipEventCreate(&e0));
-    VFVM_HIP_CHECK(hipEventCreate(&e1));
-    auto timeit = [&](auto &&fn) {
-      fn();
-      VFVM_HIP_CHECK(hipEventRecord(e0, st));
-      for (int r = 0; r < 5; r++) fn();
-      VFVM_HIP_CHECK(hipEventRecord(e1, st));
-      VFVM_HIP_CHECK(hipEventSynchronize(e1));
-      float ms = 0;
-      VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-      return ms;
-    };
+    ColdTimer timer(st);
+    auto timeit = [&](auto &&fn) { return timer.time(fn); };
     float best = 1e30f;
     int bestg = b->asm_group;
-    for (int g : {1, 2, 4, 8, 16, 32}) {
+    for (int g : {0, 1, 2, 4, 8, 16, 32}) {
       AsmArgs a = basm(*b, b->F.p, b->J.p, 1.0, 0.0, b->active.p);
       float ms = timeit([&]() { launch_assemble(a, g, st); });
       if (ms < best) {
@@ -812,22 +846,20 @@ int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group) 
       }
     }
     b->asm_group = bestg;
-    launch_scale_rows(b->nspec, b->nnodes, b->rowptr.p, b->colind.p, b->J.p, b->dofdiag.p, b->rowscale.p, nullptr,
-                      nullptr, st);
-    reset_states(*b, 0.0, 0.0, 1);
+    launch_scale_rows(b->nspec, b->nnodes, b->rowptr.p, b->colind.p, b->J.p, b->dofdiag.p, nullptr, b->rowscale.p,
+                      nullptr, nullptr, nullptr, st);
+    // lanes per row of the CSR refinement residual (the SELL-64 operator has none)
     best = 1e30f;
     bestg = b->spmv_group;
     for (int g : {2, 4, 8, 16, 32}) {
       b->spmv_group = g;
-      float ms = timeit([&]() { bop<double, 0, STAGE_NONE>(*b, b->U.p, b->tmp.p, (const double *)nullptr); });
+      float ms = timeit([&]() { bres(*b, b->U.p, b->F.p, b->tmp.p, b->active.p); });
       if (ms < best) {
         best = ms;
         bestg = g;
       }
     }
     b->spmv_group = bestg;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (asm_group) *asm_group = b->asm_group;
     if (spmv_group) *spmv_group = b->spmv_group;
     return VFVM_OK;
@@ -880,9 +912,17 @@ int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold
     int64_t nactive = M;
     for (int it = 1; it <= c.maxiters && nactive > 0; it++) {
       AsmArgs a = basm(*b, b->F.p, b->J.p, tstep, lambda, b->active.p);
+      int64_t wn = 0, wa = 0, wj = 0;  // probe work of the active members
+      for (int64_t m = 0; m < M; m++)
+        if (act[m]) {
+          const vfvm_sys *sm = b->members[m];
+          wn += sm->nnodes;
+          wa += (int64_t)sm->hs.adj_ptr.back() - sm->nnodes;
+          wj += sm->nnz;
+        }
       b->ctx->probe.before(VFVM_KERNEL_ASSEMBLE, st);
       launch_assemble(a, b->asm_group, st);
-      b->ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st);
+      b->ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st, wn, wa, wj);
       bsolve(*b, c, nlin);
       b->damp.upload(damp.data(), damp.size(), st);
       hipLaunchKernelGGL(k_bupdate, dim3((unsigned)b->nch), dim3(BB), 0, st, S, b->U.p, (const double *)b->dU.p,

@@ -114,10 +114,10 @@ AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, do
 
 void build_ilu(vfvm_sys &s) {
   if (s.ilu_built) return;
-  if (s.nspec != 1) throw Error(VFVM_ERR_ARG, "ILU(0) is provided for single-species systems; use block-Jacobi");
+  // multicolour ordering of the dof graph (for one species the node graph)
   std::vector<int32_t> color;
-  int nc = greedy_colors(s.hs, color);
-  int64_t n = s.nnodes;
+  int nc = greedy_colors(s.ndof, s.rowptr, s.colind, color);
+  int64_t n = s.ndof;
   std::vector<int32_t> perm((size_t)n), iperm((size_t)n);
   std::vector<int32_t> cptr((size_t)nc + 1, 0);
   for (int64_t i = 0; i < n; i++) cptr[color[i] + 1]++;
@@ -150,7 +150,17 @@ void build_ilu(vfvm_sys &s) {
     }
     if (pdiag[i] < 0) throw Error(VFVM_ERR_ARG, "ILU(0): missing diagonal");
   }
+  std::vector<int32_t> sptr, scol, sdpos((size_t)n), segsl, rbase;
+  build_sell64(n, prow, pcol, {0}, {n}, sptr, scol, segsl, rbase);
+  for (int64_t i = 0; i < n; i++) sdpos[i] = pdiag[i] - prow[i];
   hipStream_t st = s.ctx->stream;
+  s.d_ilu_sptr.upload(sptr, st);
+  s.d_ilu_scol.upload(scol, st);
+  s.d_ilu_sdpos.upload(sdpos, st);
+  s.ilu.sptr = s.d_ilu_sptr.p;
+  s.ilu.scol = s.d_ilu_scol.p;
+  s.ilu.sdpos = s.d_ilu_sdpos.p;
+  s.ilu.snnz = (int64_t)scol.size();
   s.d_ilu_rowptr.upload(prow, st);
   s.d_ilu_colind.upload(pcol, st);
   s.d_ilu_diag.upload(pdiag, st);
@@ -179,6 +189,10 @@ void setup_linear(vfvm_sys &s, const vfvm_control_t &c) {
   L.rowptr = s.d_rowptr.p;
   L.colind = s.d_colind.p;
   L.dofdiag = s.d_dofdiag.p;
+  L.sell_ptr = s.d_sell_ptr.p;
+  L.sell_col = s.d_sell_col.p;
+  L.sell_rbase = s.d_sell_rbase.p;
+  L.sell_nnz = s.sell_nnz;
   L.ilu = s.ilu_built ? &s.ilu : nullptr;
   L.spmv_group = s.spmv_group;
   L.probe = &s.ctx->probe;
@@ -188,8 +202,6 @@ void setup_linear(vfvm_sys &s, const vfvm_control_t &c) {
 void check_control(const vfvm_control_t &c, const vfvm_sys &s) {
   if (c.krylov != VFVM_KRYLOV_BICGSTAB && c.krylov != VFVM_KRYLOV_GMRES) throw Error(VFVM_ERR_ARG, "bad krylov");
   if (c.precon < 0 || c.precon > 2) throw Error(VFVM_ERR_ARG, "bad precon");
-  if (c.precon == VFVM_PRECON_ILU0 && s.nspec != 1)
-    throw Error(VFVM_ERR_ARG, "ILU(0) is provided for single-species systems; use block-Jacobi");
   if (c.gmres_restart < 1 || c.gmres_restart > 64) throw Error(VFVM_ERR_ARG, "gmres_restart must be in 1..64");
   if (!c.updatecontrol)
     throw Error(VFVM_ERR_ARG, "updatecontrol = false (residual-norm control) is not provided by the device loop");
@@ -246,7 +258,7 @@ int newton_device(vfvm_sys &s, double time, double tstep, double lambda, const v
     AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_F.p, s.d_J.p, tstep, lambda);
     s.ctx->probe.before(VFVM_KERNEL_ASSEMBLE, st);
     launch_assemble(a, s.asm_group, st);
-    s.ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st);
+    s.ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st, s.nnodes, (int64_t)s.hs.adj_ptr.back() - s.nnodes, s.nnz);
     const bool refactor = !(c.reuse_precon && niter > 1);
     s.lin.prepare(s.d_J.p, c, refactor);
     vfvm_linstats_t ls{};
@@ -315,28 +327,20 @@ int newton_device(vfvm_sys &s, double time, double tstep, double lambda, const v
 
 void autotune_impl(vfvm_sys &s, int32_t *ag, int32_t *sg) {
   hipStream_t st = s.ctx->stream;
-  hipEvent_t e0, e1;
-  VFVM_HIP_CHECK(hipEventCreate(&e0));
-  VFVM_HIP_CHECK(hipEventCreate(&e1));
-  // assembly group: time Jacobian assembly at the current U for each legal group
+  ColdTimer timer(st);
+  // assembly group: cold-cache time of the Jacobian assembly at the current U
   float best = 1e30f;
   int bestg = s.asm_group;
-  for (int g : {1, 2, 4, 8, 16, 32}) {
+  for (int g : {0, 1, 2, 4, 8, 16, 32}) {
     AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_F.p, s.d_J.p, 1.0, 0.0);
-    launch_assemble(a, g, st);
-    VFVM_HIP_CHECK(hipEventRecord(e0, st));
-    for (int r = 0; r < 5; r++) launch_assemble(a, g, st);
-    VFVM_HIP_CHECK(hipEventRecord(e1, st));
-    VFVM_HIP_CHECK(hipEventSynchronize(e1));
-    float ms = 0;
-    VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    const float ms = timer.time([&]() { launch_assemble(a, g, st); });
     if (ms < best) {
       best = ms;
       bestg = g;
     }
   }
   s.asm_group = bestg;
-  // SpMV group
+  // SpMV group of the CSR kernels (FP64 refinement residual)
   LinSetup L;
   L.ndof = s.ndof;
   L.nnodes = s.nnodes;
@@ -347,13 +351,7 @@ void autotune_impl(vfvm_sys &s, int32_t *ag, int32_t *sg) {
   best = 1e30f;
   bestg = s.spmv_group;
   for (int g : {2, 4, 8, 16, 32}) {
-    time_spmv(L, g, s.d_J.p, s.d_U.p, s.d_tmp.p, st);
-    VFVM_HIP_CHECK(hipEventRecord(e0, st));
-    for (int r = 0; r < 5; r++) time_spmv(L, g, s.d_J.p, s.d_U.p, s.d_tmp.p, st);
-    VFVM_HIP_CHECK(hipEventRecord(e1, st));
-    VFVM_HIP_CHECK(hipEventSynchronize(e1));
-    float ms = 0;
-    VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    const float ms = timer.time([&]() { time_spmv(L, g, s.d_J.p, s.d_U.p, s.d_tmp.p, st); });
     if (ms < best) {
       best = ms;
       bestg = g;
@@ -361,8 +359,30 @@ void autotune_impl(vfvm_sys &s, int32_t *ag, int32_t *sg) {
   }
   s.spmv_group = bestg;
   s.lin.set_group(bestg);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  // SELL-64 operator: plain or non-temporal matrix loads, per precision
+  {
+    DevBuf<float> x32, y32;
+    x32.alloc((size_t)s.ndof);
+    y32.alloc((size_t)s.ndof);
+    launch_fill_u64((s.ndof + 1) / 2, reinterpret_cast<unsigned long long *>(x32.p), st);
+    vfvm_control_t c;
+    vfvm_default_control(&c);
+    for (int pi = 0; pi < 2; pi++) {
+      c.precision = pi == 0 ? VFVM_PREC_FP32_REFINE : VFVM_PREC_FP64;
+      s.lin.prepare(s.d_J.p, c, false);
+      float t[2];
+      for (int nt = 0; nt < 2; nt++) {
+        s.lin.set_sell_nt(pi, nt != 0);
+        t[nt] = timer.time([&]() {
+          if (pi == 0)
+            s.lin.apply_plain(true, s.d_J.p, x32.p, y32.p);
+          else
+            s.lin.apply_plain(false, s.d_J.p, s.d_U.p, s.d_tmp.p);
+        });
+      }
+      s.lin.set_sell_nt(pi, t[1] < t[0]);
+    }
+  }
   if (ag) *ag = s.asm_group;
   if (sg) *sg = s.spmv_group;
 }
@@ -451,6 +471,7 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     s->nspec = nspec;
     s->nbreg = nbregions;
     s->nnodes = nnodes;
+    s->coord.assign(coord, coord + (size_t)dim * nnodes);
     s->ndof = nnodes * nspec;
     s->phys = *physics;
     s->bc.assign((size_t)std::max(1, nbregions) * nspec, vfvm_bc_t{});
@@ -484,6 +505,14 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     s->d_bnodes.upload(h.bnodes.empty() ? std::vector<int32_t>{0} : h.bnodes, st);
     s->d_rowptr.upload(s->rowptr, st);
     s->d_colind.upload(s->colind, st);
+    {
+      std::vector<int32_t> sptr, scol, segsl, rbase;
+      build_sell64(s->ndof, s->rowptr, s->colind, {0}, {s->ndof}, sptr, scol, segsl, rbase);
+      s->d_sell_ptr.upload(sptr, st);
+      s->d_sell_col.upload(scol, st);
+      s->d_sell_rbase.upload(rbase, st);
+      s->sell_nnz = (int64_t)scol.size();
+    }
     s->d_csc_perm.upload(s->csc_perm, st);
     s->d_dofdiag.upload(s->dofdiag, st);
     s->d_U.alloc((size_t)s->ndof);
@@ -756,10 +785,12 @@ int vfvm_probe_arm(vfvm_ctx *ctx, int kernel, int64_t max_launches) {
   });
 }
 
-int vfvm_probe_read(vfvm_ctx *ctx, int kernel, double *total_ms, int64_t *launches) {
+int vfvm_probe_read(vfvm_ctx *ctx, int kernel, double *total_ms, int64_t *launches, int64_t *work) {
   if (!ctx || !total_ms || kernel < 0 || kernel >= KernelProbe::NK) return VFVM_ERR_ARG;
   return guarded(ctx, [&]() {
-    *total_ms = ctx->probe.total_ms(kernel, launches);
+    int64_t w[KernelProbe::NW] = {0, 0, 0, 0};
+    *total_ms = ctx->probe.total_ms(kernel, launches, w);
+    if (work) std::copy(w, w + 3, work);
     ctx->probe.disarm(kernel);
     return VFVM_OK;
   });

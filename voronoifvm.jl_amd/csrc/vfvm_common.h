@@ -119,7 +119,12 @@ void verify_slot_map(const HostSetup &hs, int nspec, const Masks &m, const std::
                      const std::vector<int32_t> &colind);
 void build_csc_perm(int64_t ndof, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
                     std::vector<int64_t> &colptr, std::vector<int32_t> &rowval, std::vector<int32_t> &perm);
-int greedy_colors(const HostSetup &hs, std::vector<int32_t> &color);
+int greedy_colors(int64_t n, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
+                  std::vector<int32_t> &color);
+void build_sell64(int64_t nrows, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
+                  const std::vector<int64_t> &seg_beg, const std::vector<int64_t> &seg_end,
+                  std::vector<int32_t> &sptr, std::vector<int32_t> &scol, std::vector<int32_t> &seg_slice,
+                  std::vector<int32_t> &rbase);
 
 // ---- kernel launch interfaces (vfvm_assemble.hip, vfvm_linalg.hip) ----
 struct AsmArgs {
@@ -151,20 +156,30 @@ void launch_penalty_free_norm(int64_t nnodes, int nspec, const double *F, const 
 // Live timing of kernel classes inside real solves.  While a class (VFVM_KERNEL_*)
 // is armed, every launch of it is bracketed by a pair of HIP events recorded on
 // the launching stream; the pairs come from a pool created when the class is
-// armed, launches beyond the pool are counted as dropped.
+// armed, launches beyond the pool are counted as dropped.  Alongside, the work
+// the timed launches did is counted (rows, nonzeros or adjacency entries, slots:
+// see vfvm_probe_read): by the host where it knows the work at launch time, by
+// the batched SpMV itself (one atomic per live chunk into `dwork`), whose work
+// depends on which members are still iterating.
 struct KernelProbe {
   static constexpr int NK = 4;
+  static constexpr int NW = 4;
   struct Slot {
     bool on = false;
     std::vector<hipEvent_t> ev;
     size_t used = 0;
     int64_t dropped = 0;
+    int64_t work[NW] = {0, 0, 0, 0};
+    unsigned long long *dwork = nullptr;
   } k[NK];
   void arm(int id, int64_t max_launches) {
     disarm(id);
     Slot &q = k[id];
     q.ev.resize(2 * (size_t)max_launches, nullptr);
     for (auto &e : q.ev) VFVM_HIP_CHECK(hipEventCreate(&e));
+    VFVM_HIP_CHECK(hipMalloc(&q.dwork, NW * sizeof(unsigned long long)));
+    VFVM_HIP_CHECK(hipMemset(q.dwork, 0, NW * sizeof(unsigned long long)));
+    VFVM_HIP_CHECK(hipDeviceSynchronize());
     q.on = true;
   }
   void disarm(int id) {
@@ -172,26 +187,40 @@ struct KernelProbe {
     for (auto e : q.ev)
       if (e) (void)hipEventDestroy(e);
     q.ev.clear();
+    if (q.dwork) (void)hipFree(q.dwork);
+    q.dwork = nullptr;
     q.used = 0;
     q.dropped = 0;
+    for (auto &w : q.work) w = 0;
     q.on = false;
   }
+  bool any_armed() const {
+    for (int i = 0; i < NK; i++)
+      if (k[i].on) return true;
+    return false;
+  }
+  // device work counters of an armed class (null when not armed)
+  unsigned long long *counters(int id) const { return k[id].on ? k[id].dwork : nullptr; }
   void before(int id, hipStream_t s) {
     Slot &q = k[id];
     if (q.on && q.used + 2 <= q.ev.size()) VFVM_HIP_CHECK(hipEventRecord(q.ev[q.used], s));
   }
-  void after(int id, hipStream_t s) {
+  void after(int id, hipStream_t s, int64_t w0 = 0, int64_t w1 = 0, int64_t w2 = 0) {
     Slot &q = k[id];
     if (!q.on) return;
     if (q.used + 2 <= q.ev.size()) {
       VFVM_HIP_CHECK(hipEventRecord(q.ev[q.used + 1], s));
       q.used += 2;
+      q.work[0] += w0;
+      q.work[1] += w1;
+      q.work[2] += w2;
     } else {
       q.dropped++;
     }
   }
-  // summed duration (ms) of the recorded launches of one class; waits for the last one
-  double total_ms(int id, int64_t *launches) {
+  // summed duration (ms) of the recorded launches of one class and their work
+  // counts (host-counted plus device-counted); waits for the last launch
+  double total_ms(int id, int64_t *launches, int64_t *work) {
     Slot &q = k[id];
     double t = 0.0;
     if (q.used) VFVM_HIP_CHECK(hipEventSynchronize(q.ev[q.used - 1]));
@@ -201,6 +230,14 @@ struct KernelProbe {
       t += ms;
     }
     if (launches) *launches = (int64_t)(q.used / 2);
+    if (work) {
+      unsigned long long dw[NW] = {0, 0, 0, 0};
+      if (q.dwork) {
+        VFVM_HIP_CHECK(hipDeviceSynchronize());
+        VFVM_HIP_CHECK(hipMemcpy(dw, q.dwork, sizeof(dw), hipMemcpyDeviceToHost));
+      }
+      for (int w = 0; w < NW; w++) work[w] = q.work[w] + (int64_t)dw[w];
+    }
     return t;
   }
   ~KernelProbe() {

@@ -93,12 +93,20 @@ struct IluStruct {
   std::vector<int32_t> cptr;
   const int32_t *rowptr = nullptr, *colind = nullptr, *diag = nullptr, *perm = nullptr;
   const int32_t *src = nullptr, *srcrow = nullptr;
+  // SELL-64 copy of the permuted factor pattern for the triangular solves;
+  // sdpos[i] = position j of the diagonal inside permuted row i
+  const int32_t *sptr = nullptr, *scol = nullptr, *sdpos = nullptr;
+  int64_t snnz = 0;
 };
 
 struct LinSetup {
   int64_t ndof = 0, nnodes = 0, nnz = 0;
   int nspec = 1;
   const int32_t *rowptr = nullptr, *colind = nullptr, *dofdiag = nullptr;
+  // SELL-64 pattern of the operator (one segment: slice r / 64, lane r % 64) and the
+  // position of each row's first entry
+  const int32_t *sell_ptr = nullptr, *sell_col = nullptr, *sell_rbase = nullptr;
+  int64_t sell_nnz = 0;
   const IluStruct *ilu = nullptr;
   int spmv_group = 8;
   KernelProbe *probe = nullptr;  // live kernel timing of the owning context
@@ -110,6 +118,8 @@ class LinearSolver {
   void prepare(const double *J, const vfvm_control_t &c, bool refactor);
   int solve(const double *J, const double *b, double *x, const vfvm_control_t &c, vfvm_linstats_t *stats);
   void set_group(int g) { L.spmv_group = g; }
+  // non-temporal matrix loads in the SELL-64 operator, per precision (0 FP32, 1 FP64)
+  void set_sell_nt(int prec_index, bool nt) { sell_nt[prec_index] = nt; }
   // one plain operator application y = Ahat x in FP64 (J, row scale) or FP32 (A32), for timing
   void apply_plain(bool fp32, const double *J, const void *x, void *y);
   const double *row_scale() const { return rowscale.p; }
@@ -146,11 +156,13 @@ class LinearSolver {
   hipStream_t stream = nullptr;
   DevBuf<KState> st;
   DevBuf<double> partials, rowscale, binv, bhat, rres, ilu64, iluy64, vbasis64;
-  DevBuf<float> A32, ilu32, iluy32, vbasis32, rhs32, d32;
+  DevBuf<double> A64s, ilu64s;  // row-scaled operator / factor in SELL-64 order (FP64 Krylov)
+  DevBuf<float> A32, ilu32, ilu32s, iluy32, vbasis32, rhs32, d32;
   DevBuf<double> gmH, gmh1, gmh2, gmcs, gmsn, gmg, gmy, gmpart;
   Work<float> w32;
   Work<double> w64;
   bool ilu_ready = false;
+  bool sell_nt[2] = {false, false};
   double last_rr = 0, last_rr0 = 0;
   int last_iters = 0;
 };
@@ -162,8 +174,51 @@ void launch_gather(int64_t n, const int32_t *perm, const double *J, double *out,
 void launch_fill_u64(int64_t n, unsigned long long *p, hipStream_t s);
 void launch_fill_i32(int64_t n, int32_t *p, int32_t v, hipStream_t s);
 void launch_copy(int64_t n, const double *a, double *b, hipStream_t s);
+// row scaling S = 1/diag(A); scaled copies of A in FP32 and/or FP64 go to CSR order
+// (rbase null) or to SELL-64 order (entry j of row r at rbase[r] + 64 j)
 void launch_scale_rows(int nspec, int64_t nnodes, const int32_t *rowptr, const int32_t *colind, const double *A,
-                       const int32_t *dofdiag, double *rowscale, float *A32, double *binv, hipStream_t s);
+                       const int32_t *dofdiag, const int32_t *rbase, double *rowscale, float *A32, double *A64,
+                       double *binv, hipStream_t s);
 void time_spmv(const LinSetup &L, int G, const double *A, const double *x, double *y, hipStream_t s);
+
+// Cold-cache timing for the autotuners.  Before each timed launch a 512 MiB
+// buffer is overwritten, which evicts the 256 MB Infinity Cache and the L2s, so
+// the launch reads its operands from HBM as it does inside a solve (repeating a
+// launch back to back would time it against operands left in the caches).
+class ColdTimer {
+ public:
+  explicit ColdTimer(hipStream_t s) : st(s) {
+    flush.alloc(kFlushWords);
+    VFVM_HIP_CHECK(hipEventCreate(&e0));
+    VFVM_HIP_CHECK(hipEventCreate(&e1));
+  }
+  ~ColdTimer() {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  // average duration (ms) of fn() over reps cold launches, after one warm-up launch
+  template <class F>
+  float time(F &&fn, int reps = 3) {
+    fn();
+    float tot = 0.f;
+    for (int r = 0; r < reps; r++) {
+      launch_fill_u64((int64_t)kFlushWords, flush.p, st);
+      VFVM_HIP_CHECK(hipEventRecord(e0, st));
+      fn();
+      VFVM_HIP_CHECK(hipEventRecord(e1, st));
+      VFVM_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      VFVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      tot += ms;
+    }
+    return tot / reps;
+  }
+
+ private:
+  static constexpr size_t kFlushWords = (size_t)64 << 20;  // 512 MiB of 8-byte words
+  hipStream_t st;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  DevBuf<unsigned long long> flush;
+};
 
 }  // namespace vfvm

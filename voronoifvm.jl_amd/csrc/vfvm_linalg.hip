@@ -123,6 +123,62 @@ __global__ void __launch_bounds__(VB) k_spmv(int64_t nrows, const int32_t *__res
   }
 }
 
+// SELL-64 SpMV of the row-scaled operator: y = Ahat x, one lane per row.  A wave
+// covers one slice of 64 rows, so each load of entry j (values and column indices)
+// is one coalesced access; no row pointers, no cross-lane reduction.  Fused dots as
+// in k_spmv.  Rows past nrows exist only as padding of the last slice.
+// matrix stream loads; NT: non-temporal (the values and indices are read once per
+// application, so they need not displace the gathered vector from the caches)
+template <bool NT, class T>
+__device__ __forceinline__ T ldm(const T *p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+template <class VA, class VX, int NDOT, int STAGE, bool NT>
+__global__ void __launch_bounds__(VB) k_sell_spmv(int64_t nrows, const int32_t *__restrict__ sptr,
+                                                  const int32_t *__restrict__ col, const VA *__restrict__ val,
+                                                  const VX *__restrict__ x, VX *__restrict__ y,
+                                                  const VX *__restrict__ w0, double *partials, KState *st) {
+  if (st && idle(st)) return;
+  if (STAGE == STAGE_OMEGA && st->half) return;
+  double d0 = 0.0, d1 = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * VB + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * VB) {
+    const int64_t sl = r >> 6;
+    const int32_t p1 = sptr[sl + 1];
+    double acc = 0.0;
+#pragma unroll 4
+    for (int32_t q = sptr[sl] + (int32_t)(r & 63); q < p1; q += 64)
+      acc += (double)ldm<NT>(val + q) * (double)x[ldm<NT>(col + q)];
+    const VX yv = (VX)acc;
+    y[r] = yv;
+    if (NDOT >= 1) d0 += (double)w0[r] * (double)yv;
+    if (NDOT >= 2) d1 += (double)yv * (double)yv;
+  }
+  if constexpr (NDOT > 0) {
+    double v[2] = {d0, d1};
+    write_partials<2>(v, partials);
+  }
+}
+
+// r = b - Ahat x with the SELL-64 operator (GMRES restarts)
+template <class VA, class V, class VXo>
+__global__ void __launch_bounds__(VB) k_sell_residual(int64_t nrows, const int32_t *__restrict__ sptr,
+                                                      const int32_t *__restrict__ col, const VA *__restrict__ val,
+                                                      const VXo *__restrict__ x, const V *__restrict__ b,
+                                                      V *__restrict__ r) {
+  for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * VB) {
+    const int64_t sl = i >> 6;
+    const int32_t p1 = sptr[sl + 1];
+    double acc = 0.0;
+#pragma unroll 4
+    for (int32_t q = sptr[sl] + (int32_t)(i & 63); q < p1; q += 64) acc += (double)val[q] * (double)x[col[q]];
+    r[i] = (V)((double)b[i] - acc);
+  }
+}
+
 // residual of the refinement: y = S (b - A x) in FP64, ||y||^2 -> st->rr
 template <int G>
 __global__ void __launch_bounds__(VB) k_scaled_residual(int64_t nrows, const int32_t *__restrict__ rowptr,
@@ -242,8 +298,10 @@ __global__ void __launch_bounds__(VB) k_add_correction(int64_t n, double *__rest
 template <int NS>
 __global__ void __launch_bounds__(VB) k_scale_rows(int64_t nnodes, const int32_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ colind, const double *__restrict__ A,
-                                                   const int32_t *__restrict__ dofdiag, double *__restrict__ rowscale,
-                                                   float *__restrict__ A32, double *__restrict__ binv) {
+                                                   const int32_t *__restrict__ dofdiag,
+                                                   const int32_t *__restrict__ rbase, double *__restrict__ rowscale,
+                                                   float *__restrict__ A32, double *__restrict__ A64,
+                                                   double *__restrict__ binv) {
   for (int64_t K = (int64_t)blockIdx.x * VB + threadIdx.x; K < nnodes; K += (int64_t)gridDim.x * VB) {
     double sc[NS];
     double B[NS][NS];
@@ -256,9 +314,14 @@ __global__ void __launch_bounds__(VB) k_scale_rows(int64_t nnodes, const int32_t
 #pragma unroll
       for (int j = 0; j < NS; j++) B[i][j] = (i == j) ? 1.0 : 0.0;
       const int32_t k0 = rowptr[r], k1 = rowptr[r + 1];
+      // scaled copies go to CSR positions, or to SELL-64 positions when rbase is given
+      const int64_t q0 = rbase ? (int64_t)rbase[r] : (int64_t)k0;
+      const int64_t qs = rbase ? 64 : 1;
       for (int32_t k = k0; k < k1; k++) {
         const double v = A[k] * sc[i];
-        if (A32) A32[k] = (float)v;
+        const int64_t q = q0 + (int64_t)(k - k0) * qs;
+        if (A32) A32[q] = (float)v;
+        if (A64) A64[q] = v;
         if (NS > 1 && binv) {
           const int32_t c = colind[k];
           if (c >= K * NS && c < K * NS + NS) B[i][c - K * NS] = v;
@@ -359,36 +422,52 @@ __global__ void __launch_bounds__(VB) k_ilu_factor(int32_t r0, int32_t r1, const
   }
 }
 
-// forward solve for one colour: y_i = r_perm(i) - sum_{k<i} l_ik y_k
+// factor values from CSR order to SELL-64 order (after each factorisation)
+template <class VL>
+__global__ void __launch_bounds__(VB) k_to_sell(int64_t nrows, const int32_t *__restrict__ rowptr,
+                                                const int32_t *__restrict__ sptr, const VL *__restrict__ a,
+                                                VL *__restrict__ as) {
+  for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * VB) {
+    const int32_t k0 = rowptr[i], k1 = rowptr[i + 1];
+    const int64_t q0 = (int64_t)sptr[i >> 6] + (i & 63);
+    for (int32_t k = k0; k < k1; k++) as[q0 + (int64_t)(k - k0) * 64] = a[k];
+  }
+}
+
+// Triangular solves of one colour on the SELL-64 factor: lanes are permuted rows
+// i of the slices that overlap the colour's row range [r0, r1).  Lower part =
+// entries before the diagonal position sdpos[i], upper part = entries after it
+// (padding entries carry value 0 and the row's own column).
 template <class VL, class V>
-__global__ void __launch_bounds__(VB) k_ilu_lower(int32_t r0, int32_t r1, const int32_t *__restrict__ rowptr,
-                                                  const int32_t *__restrict__ colind, const int32_t *__restrict__ diag,
-                                                  const VL *__restrict__ lu, const int32_t *__restrict__ perm,
-                                                  const V *__restrict__ r, V *__restrict__ y, const KState *st,
-                                                  int skip_if_half) {
+__global__ void __launch_bounds__(VB) k_ilu_lower_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ sptr,
+                                                       const int32_t *__restrict__ col,
+                                                       const int32_t *__restrict__ sdpos, const VL *__restrict__ lu,
+                                                       const int32_t *__restrict__ perm, const V *__restrict__ r,
+                                                       V *__restrict__ y, const KState *st, int skip_if_half) {
   if (st && idle(st)) return;
   if (st && skip_if_half && st->half) return;
-  const int32_t i = r0 + (int32_t)(blockIdx.x * VB + threadIdx.x);
-  if (i >= r1) return;
+  const int32_t i = (r0 & ~63) + (int32_t)(blockIdx.x * VB + threadIdx.x);
+  if (i < r0 || i >= r1) return;
   double acc = (double)r[perm[i]];
-  for (int32_t k = rowptr[i]; k < diag[i]; k++) acc -= (double)lu[k] * (double)y[colind[k]];
+  const int32_t q0 = sptr[i >> 6] + (i & 63), qd = q0 + 64 * sdpos[i];
+  for (int32_t q = q0; q < qd; q += 64) acc -= (double)lu[q] * (double)y[col[q]];
   y[i] = (V)acc;
 }
 
-// backward solve for one colour: x_i = (y_i - sum_{j>i} u_ij x_j) / u_ii ; z_perm(i) = x_i
 template <class VL, class V>
-__global__ void __launch_bounds__(VB) k_ilu_upper(int32_t r0, int32_t r1, const int32_t *__restrict__ rowptr,
-                                                  const int32_t *__restrict__ colind, const int32_t *__restrict__ diag,
-                                                  const VL *__restrict__ lu, const int32_t *__restrict__ perm,
-                                                  V *__restrict__ y, V *__restrict__ z, const KState *st,
-                                                  int skip_if_half) {
+__global__ void __launch_bounds__(VB) k_ilu_upper_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ sptr,
+                                                       const int32_t *__restrict__ col,
+                                                       const int32_t *__restrict__ sdpos, const VL *__restrict__ lu,
+                                                       const int32_t *__restrict__ perm, V *__restrict__ y,
+                                                       V *__restrict__ z, const KState *st, int skip_if_half) {
   if (st && idle(st)) return;
   if (st && skip_if_half && st->half) return;
-  const int32_t i = r0 + (int32_t)(blockIdx.x * VB + threadIdx.x);
-  if (i >= r1) return;
+  const int32_t i = (r0 & ~63) + (int32_t)(blockIdx.x * VB + threadIdx.x);
+  if (i < r0 || i >= r1) return;
+  const int32_t sl = i >> 6, q0 = sptr[sl] + (i & 63), qd = q0 + 64 * sdpos[i], qe = sptr[sl + 1];
   double acc = (double)y[i];
-  for (int32_t k = diag[i] + 1; k < rowptr[i + 1]; k++) acc -= (double)lu[k] * (double)y[colind[k]];
-  const V xv = (V)(acc / (double)lu[diag[i]]);
+  for (int32_t q = qd + 64; q < qe; q += 64) acc -= (double)lu[q] * (double)y[col[q]];
+  const V xv = (V)(acc / (double)lu[qd]);
   y[i] = xv;  // x overwrites y in place (later colours read x of their upper neighbours)
   z[perm[i]] = xv;
 }
@@ -532,27 +611,6 @@ __global__ void k_gmres_reset(int m, double *g, KState *st) {
   st->done = 0;
 }
 
-// r = b_hat - Ahat x for the GMRES restart, in the solver precision (G lanes / row)
-template <class VA, class V, class VXo, int G>
-__global__ void __launch_bounds__(VB) k_residual_v(int64_t nrows, const int32_t *__restrict__ rowptr,
-                                                   const int32_t *__restrict__ colind, const VA *__restrict__ val,
-                                                   const VXo *__restrict__ x, const V *__restrict__ b,
-                                                   const double *__restrict__ rowscale, V *__restrict__ r) {
-  const int lane = threadIdx.x & (G - 1);
-  const int64_t rows_per_block = VB / G;
-  for (int64_t row = (int64_t)blockIdx.x * rows_per_block + threadIdx.x / G; row < nrows;
-       row += (int64_t)gridDim.x * rows_per_block) {
-    double acc = 0.0;
-    for (int32_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += G) acc += (double)val[k] * (double)x[colind[k]];
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
-    if (lane == 0) {
-      if (rowscale) acc *= rowscale[row];
-      r[row] = (V)((double)b[row] - acc);
-    }
-  }
-}
-
 // Newton update: U -= damp * dU ; ||dU||_inf, ||U||_1 (src/vfvm_solver.jl:116-135)
 __global__ void __launch_bounds__(VB) k_newton_update(int64_t n, double *__restrict__ U, const double *__restrict__ dU,
                                                       double damp, double *partials, unsigned *ticket,
@@ -658,11 +716,19 @@ void LinearSolver::Work<V>::alloc(int64_t n, int nvec) {
 void LinearSolver::ensure_work(int precision, int krylov, int m) {
   int64_t n = L.ndof;
   int nv = krylov == VFVM_KRYLOV_GMRES ? 4 : 9;
+  // SELL-64 operator values: padding entries must hold 0, so a new buffer is cleared once
   if (precision == VFVM_PREC_FP32_REFINE) {
     w32.alloc(n, nv);
-    A32.alloc((size_t)L.nnz);
+    if (A32.n != (size_t)L.sell_nnz) {
+      A32.alloc((size_t)L.sell_nnz);
+      hipLaunchKernelGGL((k_fill<float>), dim3(grid_for(L.sell_nnz)), dim3(VB), 0, stream, L.sell_nnz, A32.p, 0.0f);
+    }
   } else {
     w64.alloc(n, nv);
+    if (A64s.n != (size_t)L.sell_nnz) {
+      A64s.alloc((size_t)L.sell_nnz);
+      hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(L.sell_nnz)), dim3(VB), 0, stream, L.sell_nnz, A64s.p, 0.0);
+    }
   }
   if (krylov == VFVM_KRYLOV_GMRES) {
     if (precision == VFVM_PREC_FP32_REFINE)
@@ -682,31 +748,31 @@ void LinearSolver::ensure_work(int precision, int krylov, int m) {
 
 // scaling, FP32 copy and preconditioner set-up for the current Jacobian values
 void LinearSolver::prepare(const double *J, const vfvm_control_t &c, bool refactor) {
-  int64_t n = L.ndof;
   ensure_work(c.precision, c.krylov, c.gmres_restart);
   const bool fp32 = c.precision == VFVM_PREC_FP32_REFINE;
   const bool blk = c.precon == VFVM_PRECON_BLOCKJACOBI && L.nspec > 1;
   if (blk) binv.alloc((size_t)L.nnodes * L.nspec * L.nspec);
   unsigned g = grid_for(L.nnodes);
   float *a32 = fp32 ? A32.p : nullptr;
+  double *a64 = fp32 ? nullptr : A64s.p;
   double *bi = blk ? binv.p : nullptr;
   switch (L.nspec) {
     case 1:
       hipLaunchKernelGGL((k_scale_rows<1>), dim3(g), dim3(VB), 0, stream, L.nnodes, L.rowptr, L.colind, J, L.dofdiag,
-                         rowscale.p, a32, bi);
+                         L.sell_rbase, rowscale.p, a32, a64, bi);
       break;
     case 2:
       hipLaunchKernelGGL((k_scale_rows<2>), dim3(g), dim3(VB), 0, stream, L.nnodes, L.rowptr, L.colind, J, L.dofdiag,
-                         rowscale.p, a32, bi);
+                         L.sell_rbase, rowscale.p, a32, a64, bi);
       break;
     case 3:
       hipLaunchKernelGGL((k_scale_rows<3>), dim3(g), dim3(VB), 0, stream, L.nnodes, L.rowptr, L.colind, J, L.dofdiag,
-                         rowscale.p, a32, bi);
+                         L.sell_rbase, rowscale.p, a32, a64, bi);
       break;
   }
   VFVM_HIP_CHECK(hipGetLastError());
   if (c.precon == VFVM_PRECON_ILU0 && (refactor || !ilu_ready)) {
-    if (!L.ilu) throw Error(VFVM_ERR_ARG, "ILU(0) needs the multicolour structure (single species)");
+    if (!L.ilu) throw Error(VFVM_ERR_ARG, "ILU(0) needs the multicolour structure (build_ilu)");
     const IluStruct &I = *L.ilu;
     if (fp32) {
       ilu32.alloc((size_t)L.nnz);
@@ -718,6 +784,12 @@ void LinearSolver::prepare(const double *J, const vfvm_control_t &c, bool refact
           hipLaunchKernelGGL((k_ilu_factor<float>), dim3((r1 - r0 + VB - 1) / VB), dim3(VB), 0, stream, r0, r1,
                              I.rowptr, I.colind, I.diag, ilu32.p);
       }
+      if (ilu32s.n != (size_t)I.snnz) {
+        ilu32s.alloc((size_t)I.snnz);
+        hipLaunchKernelGGL((k_fill<float>), dim3(grid_for(I.snnz)), dim3(VB), 0, stream, I.snnz, ilu32s.p, 0.0f);
+      }
+      hipLaunchKernelGGL((k_to_sell<float>), dim3(grid_for(L.ndof)), dim3(VB), 0, stream, L.ndof, I.rowptr, I.sptr,
+                         (const float *)ilu32.p, ilu32s.p);
     } else {
       ilu64.alloc((size_t)L.nnz);
       hipLaunchKernelGGL((k_ilu_gather<double>), dim3(grid_for(L.nnz)), dim3(VB), 0, stream, L.nnz, I.src, I.srcrow, J,
@@ -728,6 +800,12 @@ void LinearSolver::prepare(const double *J, const vfvm_control_t &c, bool refact
           hipLaunchKernelGGL((k_ilu_factor<double>), dim3((r1 - r0 + VB - 1) / VB), dim3(VB), 0, stream, r0, r1,
                              I.rowptr, I.colind, I.diag, ilu64.p);
       }
+      if (ilu64s.n != (size_t)I.snnz) {
+        ilu64s.alloc((size_t)I.snnz);
+        hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(I.snnz)), dim3(VB), 0, stream, I.snnz, ilu64s.p, 0.0);
+      }
+      hipLaunchKernelGGL((k_to_sell<double>), dim3(grid_for(L.ndof)), dim3(VB), 0, stream, L.ndof, I.rowptr, I.sptr,
+                         (const double *)ilu64.p, ilu64s.p);
     }
     ilu_ready = true;
     VFVM_HIP_CHECK(hipGetLastError());
@@ -747,20 +825,21 @@ const V *LinearSolver::precond(const V *r, V *z, KState *stp, int skip_if_half, 
   } else if (c.precon == VFVM_PRECON_ILU0) {
     const IluStruct &I = *L.ilu;
     const V *lu = nullptr;
-    if constexpr (sizeof(V) == 4) lu = ilu32.p;
-    else lu = ilu64.p;
+    if constexpr (sizeof(V) == 4) lu = ilu32s.p;
+    else lu = ilu64s.p;
     V *y = iluy_ptr<V>();
+    auto blocks = [](int32_t r0, int32_t r1) { return (unsigned)((r1 - (r0 & ~63) + VB - 1) / VB); };
     for (int c0 = 0; c0 < I.ncolors; c0++) {
       int32_t r0 = I.cptr[c0], r1 = I.cptr[c0 + 1];
       if (r1 > r0)
-        hipLaunchKernelGGL((k_ilu_lower<V, V>), dim3((r1 - r0 + VB - 1) / VB), dim3(VB), 0, stream, r0, r1, I.rowptr,
-                           I.colind, I.diag, lu, I.perm, r, y, stp, skip_if_half);
+        hipLaunchKernelGGL((k_ilu_lower_sell<V, V>), dim3(blocks(r0, r1)), dim3(VB), 0, stream, r0, r1, I.sptr, I.scol,
+                           I.sdpos, lu, I.perm, r, y, stp, skip_if_half);
     }
     for (int c0 = I.ncolors - 1; c0 >= 0; c0--) {
       int32_t r0 = I.cptr[c0], r1 = I.cptr[c0 + 1];
       if (r1 > r0)
-        hipLaunchKernelGGL((k_ilu_upper<V, V>), dim3((r1 - r0 + VB - 1) / VB), dim3(VB), 0, stream, r0, r1, I.rowptr,
-                           I.colind, I.diag, lu, I.perm, y, z, stp, skip_if_half);
+        hipLaunchKernelGGL((k_ilu_upper_sell<V, V>), dim3(blocks(r0, r1)), dim3(VB), 0, stream, r0, r1, I.sptr, I.scol,
+                           I.sdpos, lu, I.perm, y, z, stp, skip_if_half);
     }
     return z;
   }
@@ -782,27 +861,25 @@ V *LinearSolver::iluy_ptr() {
 // y = Ahat x with optional fused dots, in the working precision V
 template <class V, int NDOT, int STAGE>
 void LinearSolver::apply_op(const V *x, V *y, const V *w0, const V *w1, KState *stp, const double *J) {
-  const int G = L.spmv_group;
-  unsigned g = grid_for(L.ndof * G);
-  auto launch = [&](auto gconst) {
-    constexpr int GG = decltype(gconst)::value;
-    if constexpr (sizeof(V) == 4)
-      hipLaunchKernelGGL((k_spmv<float, V, GG, NDOT, STAGE>), dim3(g), dim3(VB), 0, stream, L.ndof, L.rowptr, L.colind,
-                         (const float *)A32.p, x, y, (const double *)nullptr, w0, w1, partials.p, stp);
-    else
-      hipLaunchKernelGGL((k_spmv<double, V, GG, NDOT, STAGE>), dim3(g), dim3(VB), 0, stream, L.ndof, L.rowptr,
-                         L.colind, J, x, y, (const double *)rowscale.p, w0, w1, partials.p, stp);
-  };
+  (void)w1;
+  (void)J;
+  const unsigned g = grid_for(L.ndof);
   const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
   if (L.probe) L.probe->before(kid, stream);
-  switch (G) {
-    case 2: launch(std::integral_constant<int, 2>()); break;
-    case 4: launch(std::integral_constant<int, 4>()); break;
-    case 8: launch(std::integral_constant<int, 8>()); break;
-    case 16: launch(std::integral_constant<int, 16>()); break;
-    default: launch(std::integral_constant<int, 32>()); break;
-  }
-  if (L.probe) L.probe->after(kid, stream);
+  auto go = [&](auto ntc) {
+    constexpr bool NT = decltype(ntc)::value;
+    if constexpr (sizeof(V) == 4)
+      hipLaunchKernelGGL((k_sell_spmv<float, V, NDOT, STAGE, NT>), dim3(g), dim3(VB), 0, stream, L.ndof, L.sell_ptr,
+                         L.sell_col, (const float *)A32.p, x, y, w0, partials.p, stp);
+    else
+      hipLaunchKernelGGL((k_sell_spmv<double, V, NDOT, STAGE, NT>), dim3(g), dim3(VB), 0, stream, L.ndof,
+                         L.sell_ptr, L.sell_col, (const double *)A64s.p, x, y, w0, partials.p, stp);
+  };
+  if (sell_nt[sizeof(V) == 4 ? 0 : 1])
+    go(std::true_type());
+  else
+    go(std::false_type());
+  if (L.probe) L.probe->after(kid, stream, L.ndof, L.nnz);
   if constexpr (NDOT > 0)
     hipLaunchKernelGGL((k_finalize<2, STAGE>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p, (int)g, stp, 1);
 }
@@ -902,26 +979,15 @@ int LinearSolver::gmres(const VBin *b, VXo *x, const double *J, const vfvm_contr
   h.tol2 = tol * tol;
   h.rr0 = h.rr;
   VFVM_HIP_CHECK(hipMemcpyAsync(st.p, &h, sizeof(KState), hipMemcpyHostToDevice, stream));
-  const int G = L.spmv_group;
   int status = VFVM_ERR_LINEAR;
   while (true) {
     // r = bhat - Ahat x
-    auto resid = [&](auto gconst) {
-      constexpr int GG = decltype(gconst)::value;
-      if constexpr (sizeof(V) == 4)
-        hipLaunchKernelGGL((k_residual_v<float, V, VXo, GG>), dim3(grid_for(n * GG)), dim3(VB), 0, stream, n, L.rowptr,
-                           L.colind, (const float *)A32.p, x, bb, (const double *)nullptr, r);
-      else
-        hipLaunchKernelGGL((k_residual_v<double, V, VXo, GG>), dim3(grid_for(n * GG)), dim3(VB), 0, stream, n,
-                           L.rowptr, L.colind, J, x, bb, (const double *)rowscale.p, r);
-    };
-    switch (G) {
-      case 2: resid(std::integral_constant<int, 2>()); break;
-      case 4: resid(std::integral_constant<int, 4>()); break;
-      case 8: resid(std::integral_constant<int, 8>()); break;
-      case 16: resid(std::integral_constant<int, 16>()); break;
-      default: resid(std::integral_constant<int, 32>()); break;
-    }
+    if constexpr (sizeof(V) == 4)
+      hipLaunchKernelGGL((k_sell_residual<float, V, VXo>), dim3(grid_for(n)), dim3(VB), 0, stream, n, L.sell_ptr,
+                         L.sell_col, (const float *)A32.p, x, bb, r);
+    else
+      hipLaunchKernelGGL((k_sell_residual<double, V, VXo>), dim3(grid_for(n)), dim3(VB), 0, stream, n, L.sell_ptr,
+                         L.sell_col, (const double *)A64s.p, x, bb, r);
     hipLaunchKernelGGL((k_gmres_start_norm<V>), dim3(g), dim3(VB), 0, stream, n, r, partials.p, st.p);
     hipLaunchKernelGGL((k_finalize<1, STAGE_GMSTART>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p,
                        (int)g, st.p, 0);
@@ -1093,20 +1159,21 @@ void launch_copy(int64_t n, const double *a, double *b, hipStream_t s) {
 }
 
 void launch_scale_rows(int nspec, int64_t nnodes, const int32_t *rowptr, const int32_t *colind, const double *A,
-                       const int32_t *dofdiag, double *rowscale, float *A32, double *binv, hipStream_t s) {
+                       const int32_t *dofdiag, const int32_t *rbase, double *rowscale, float *A32, double *A64,
+                       double *binv, hipStream_t s) {
   unsigned g = grid_for(nnodes);
   switch (nspec) {
     case 1:
-      hipLaunchKernelGGL((k_scale_rows<1>), dim3(g), dim3(VB), 0, s, nnodes, rowptr, colind, A, dofdiag, rowscale, A32,
-                         binv);
+      hipLaunchKernelGGL((k_scale_rows<1>), dim3(g), dim3(VB), 0, s, nnodes, rowptr, colind, A, dofdiag, rbase,
+                         rowscale, A32, A64, binv);
       break;
     case 2:
-      hipLaunchKernelGGL((k_scale_rows<2>), dim3(g), dim3(VB), 0, s, nnodes, rowptr, colind, A, dofdiag, rowscale, A32,
-                         binv);
+      hipLaunchKernelGGL((k_scale_rows<2>), dim3(g), dim3(VB), 0, s, nnodes, rowptr, colind, A, dofdiag, rbase,
+                         rowscale, A32, A64, binv);
       break;
     default:
-      hipLaunchKernelGGL((k_scale_rows<3>), dim3(g), dim3(VB), 0, s, nnodes, rowptr, colind, A, dofdiag, rowscale, A32,
-                         binv);
+      hipLaunchKernelGGL((k_scale_rows<3>), dim3(g), dim3(VB), 0, s, nnodes, rowptr, colind, A, dofdiag, rbase,
+                         rowscale, A32, A64, binv);
       break;
   }
   VFVM_HIP_CHECK(hipGetLastError());

@@ -229,6 +229,26 @@ __global__ void __launch_bounds__(TB) k_aggregate_t(int64_t rows, int64_t n, con
   dh[row * WID + f] += acc - da[row * WID + f];
 }
 
+// The 8 input features of a 2D family with one source field and a Dirichlet value
+// per side of the bounding box, computed on the device (row r = sample r / n, node
+// r % n): f / 5, asinh(f / 2), side value k * exp(-distance to side k / delta)
+// (k = south, east, north, west), x, y.  The host-side definition is identical.
+__global__ void k_features_sides(int64_t rows, int64_t n, const double *__restrict__ f,
+                                 const double *__restrict__ sides, const float *__restrict__ xy,
+                                 const float *__restrict__ dist, double inv_delta, float *__restrict__ x) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const int64_t smp = r / n, i = r - smp * n;
+  const double fv = f[r];
+  float *o = x + r * NFEAT;
+  o[0] = (float)(fv / 5.0);
+  o[1] = (float)asinh(fv / 2.0);
+#pragma unroll
+  for (int k = 0; k < 4; k++) o[2 + k] = (float)sides[smp * 4 + k] * (float)exp(-(double)dist[i * 4 + k] * inv_delta);
+  o[6] = xy[2 * i];
+  o[7] = xy[2 * i + 1];
+}
+
 // x (rows x 8 float) -> bfloat16 rows x 16 (zero padded)
 __global__ void k_pad_features(int64_t rows, const float *__restrict__ x, bf16_t *__restrict__ xb) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -349,6 +369,10 @@ struct vfvm_mpnn {
   int64_t cap = 0;
   DevBuf<float> x, hf, rbuf, dh, da, u, target, part;
   DevBuf<bf16_t> xb, cat, dzb;
+  // device features of the Dirichlet-sides family (2D): per node x, y and the distances
+  // to the bounding-box sides (south, east, north, west); per-sample inputs
+  DevBuf<float> xy, dist;
+  DevBuf<double> fin, sides;
   size_t off_We = 0, off_be = 0, off_wd = 0, off_bd = 0;
   std::vector<size_t> off_W, off_b;
 };
@@ -520,6 +544,27 @@ int vfvm_mpnn_create(vfvm_sys *sys, int nlayers, int width, uint64_t seed, vfvm_
       ptr[i + 1] = (int32_t)col.size();
     }
     hipStream_t st = m->ctx->stream;
+    if (sys->dim == 2) {
+      const double *X = sys->coord.data();
+      double lo[2] = {X[0], X[1]}, hi[2] = {X[0], X[1]};
+      for (int64_t i = 0; i < m->n; i++)
+        for (int d = 0; d < 2; d++) {
+          lo[d] = std::min(lo[d], X[2 * i + d]);
+          hi[d] = std::max(hi[d], X[2 * i + d]);
+        }
+      std::vector<float> xy((size_t)m->n * 2), dist((size_t)m->n * 4);
+      for (int64_t i = 0; i < m->n; i++) {
+        const double x = X[2 * i], y = X[2 * i + 1];
+        xy[2 * i] = (float)x;
+        xy[2 * i + 1] = (float)y;
+        dist[4 * i + 0] = (float)(y - lo[1]);
+        dist[4 * i + 1] = (float)(hi[0] - x);
+        dist[4 * i + 2] = (float)(hi[1] - y);
+        dist[4 * i + 3] = (float)(x - lo[0]);
+      }
+      m->xy.upload(xy, st);
+      m->dist.upload(dist, st);
+    }
     m->gptr.upload(ptr, st);
     m->gcol.upload(col.empty() ? std::vector<int32_t>{0} : col, st);
     m->gw.upload(w.empty() ? std::vector<float>{0.f} : w, st);
@@ -605,6 +650,38 @@ int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, doubl
       const int64_t ns = std::min(chunk, nsamples - s0), rows = ns * m->n;
       ensure_cap(*m, rows);
       m->x.upload(features + s0 * m->n * NFEAT, (size_t)rows * NFEAT, st);
+      forward(*m, rows);
+      hipLaunchKernelGGL(k_decode, dim3(nb(rows, 4)), dim3(TB), 0, st, rows, (const float *)m->hf.p,
+                         (const float *)m->theta.p + m->off_wd, (const float *)m->theta.p + m->off_bd, m->u.p,
+                         (const float *)nullptr, 0.f, (float *)nullptr, (float *)nullptr);
+      u.resize((size_t)rows);
+      m->u.download(u.data(), (size_t)rows, st);
+      VFVM_HIP_CHECK(hipStreamSynchronize(st));
+      for (int64_t k = 0; k < rows; k++) out[s0 * m->n + k] = (double)u[k];
+    }
+    return VFVM_OK;
+  });
+}
+
+// inference with the Dirichlet-sides features computed on the device from the sources
+// f (nsamples x n) and the side values (nsamples x 4)
+int vfvm_mpnn_infer_sides(vfvm_mpnn *m, int64_t nsamples, const double *f, const double *sides, double eps,
+                          double *out) {
+  if (!m || !f || !sides || !out || nsamples < 1 || !(eps > 0)) return VFVM_ERR_ARG;
+  return mguard(m->ctx, [&]() {
+    if (!m->xy.p) throw Error(VFVM_ERR_ARG, "side features need a 2D system");
+    hipStream_t st = m->ctx->stream;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsamples, (int64_t(1) << 20) / m->n));
+    const double inv_delta = 1.0 / std::sqrt(eps / 2.0);
+    std::vector<float> u;
+    for (int64_t s0 = 0; s0 < nsamples; s0 += chunk) {
+      const int64_t ns = std::min(chunk, nsamples - s0), rows = ns * m->n;
+      ensure_cap(*m, rows);
+      m->fin.upload(f + s0 * m->n, (size_t)rows, st);
+      m->sides.upload(sides + s0 * 4, (size_t)ns * 4, st);
+      hipLaunchKernelGGL(k_features_sides, dim3(nb(rows, 256)), dim3(256), 0, st, rows, m->n,
+                         (const double *)m->fin.p, (const double *)m->sides.p, (const float *)m->xy.p,
+                         (const float *)m->dist.p, inv_delta, m->x.p);
       forward(*m, rows);
       hipLaunchKernelGGL(k_decode, dim3(nb(rows, 4)), dim3(TB), 0, st, rows, (const float *)m->hf.p,
                          (const float *)m->theta.p + m->off_wd, (const float *)m->theta.p + m->off_bd, m->u.p,

@@ -407,26 +407,90 @@ void build_csc_perm(int64_t ndof, const std::vector<int32_t> &rowptr, const std:
     }
 }
 
-// Greedy colouring of the node graph in natural order (distance-1).
-int greedy_colors(const HostSetup &hs, std::vector<int32_t> &color) {
-  int64_t n = hs.nnodes;
+// Greedy colouring (natural order, distance 1) of the graph of a CSR pattern made
+// symmetric: rows i and j get different colours whenever a_ij or a_ji is a stored
+// off-diagonal entry, so the rows of one colour never reference each other and
+// a colour's rows can be factored and solved in parallel.
+int greedy_colors(int64_t n, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
+                  std::vector<int32_t> &color) {
+  // transpose pattern (column -> rows) to see a_ji when colouring row i
+  std::vector<int32_t> tptr((size_t)n + 1, 0), trow((size_t)rowptr[n]);
+  for (int64_t k = 0; k < rowptr[n]; k++) tptr[(size_t)colind[k] + 1]++;
+  for (int64_t i = 0; i < n; i++) tptr[i + 1] += tptr[i];
+  {
+    std::vector<int32_t> nx(tptr.begin(), tptr.end() - 1);
+    for (int64_t i = 0; i < n; i++)
+      for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) trow[(size_t)nx[colind[k]]++] = (int32_t)i;
+  }
   color.assign((size_t)n, -1);
   int ncol = 0;
   std::vector<int64_t> mark(64, -1);
-  for (int64_t K = 0; K < n; K++) {
-    for (int64_t e = hs.adj_ptr[K]; e < hs.adj_ptr[K + 1]; e++) {
-      int32_t c = color[hs.adj_col[e]];
-      if (c >= 0) {
-        if ((size_t)c >= mark.size()) mark.resize((size_t)c + 1, -1);
-        mark[c] = K;
-      }
+  auto see = [&](int32_t j, int64_t i) {
+    const int32_t c = color[j];
+    if (c >= 0) {
+      if ((size_t)c >= mark.size()) mark.resize((size_t)c + 1, -1);
+      mark[c] = i;
     }
+  };
+  for (int64_t i = 0; i < n; i++) {
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) see(colind[k], i);
+    for (int32_t k = tptr[i]; k < tptr[i + 1]; k++) see(trow[k], i);
     int c = 0;
-    while ((size_t)c < mark.size() && mark[c] == K) c++;
-    color[K] = c;
+    while ((size_t)c < mark.size() && mark[c] == i) c++;
+    color[i] = c;
     if (c + 1 > ncol) ncol = c + 1;
   }
   return ncol;
+}
+
+// SELL-64 layout of a CSR matrix for one-lane-per-row kernels: slices of 64
+// consecutive rows; inside a slice entry j of all 64 rows is contiguous, so a
+// wave reading entry j of its rows makes one coalesced access.  The rows are
+// split into segments [seg_beg[k], seg_end[k]) (in order, covering all rows);
+// each segment starts a new slice (the batched solver's chunks).  Rows are padded
+// to the longest row of their slice with column = the row itself (value 0 is
+// written by the value kernels); lanes past a segment's end point at its last
+// row.  Entry j of row r sits at rbase[r] + 64 j, where rbase[r] = sptr[slice of
+// r] + (r - start of that slice); seg_slice[k] is the first slice of segment k.
+// With one segment [0, nrows), the slice of r is r / 64 and its lane r % 64.
+void build_sell64(int64_t nrows, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
+                  const std::vector<int64_t> &seg_beg, const std::vector<int64_t> &seg_end,
+                  std::vector<int32_t> &sptr, std::vector<int32_t> &scol, std::vector<int32_t> &seg_slice,
+                  std::vector<int32_t> &rbase) {
+  const size_t nseg = seg_beg.size();
+  seg_slice.assign(nseg + 1, 0);
+  std::vector<int64_t> sl_beg, sl_end;
+  for (size_t k = 0; k < nseg; k++) {
+    seg_slice[k] = (int32_t)sl_beg.size();
+    for (int64_t r = seg_beg[k]; r < seg_end[k]; r += 64) {
+      sl_beg.push_back(r);
+      sl_end.push_back(std::min(r + 64, seg_end[k]));
+    }
+  }
+  seg_slice[nseg] = (int32_t)sl_beg.size();
+  const size_t ns = sl_beg.size();
+  sptr.assign(ns + 1, 0);
+  int64_t tot = 0;
+  for (size_t sl = 0; sl < ns; sl++) {
+    int32_t w = 0;
+    for (int64_t r = sl_beg[sl]; r < sl_end[sl]; r++) w = std::max(w, rowptr[r + 1] - rowptr[r]);
+    tot += (int64_t)w * 64;
+    if (tot > INT32_MAX) throw Error(VFVM_ERR_ARG, "matrix too large for 32-bit SELL offsets");
+    sptr[sl + 1] = (int32_t)tot;
+  }
+  scol.assign((size_t)tot, 0);
+  rbase.assign((size_t)nrows, 0);
+  for (size_t sl = 0; sl < ns; sl++) {
+    const int32_t w = (sptr[sl + 1] - sptr[sl]) / 64;
+    for (int l = 0; l < 64; l++) {
+      const int64_t r = sl_beg[sl] + l;
+      const bool real = r < sl_end[sl];
+      const int32_t self = (int32_t)(real ? r : sl_end[sl] - 1);
+      const int32_t k0 = real ? rowptr[r] : 0, len = real ? rowptr[r + 1] - rowptr[r] : 0;
+      if (real) rbase[(size_t)r] = sptr[sl] + l;
+      for (int32_t j = 0; j < w; j++) scol[(size_t)sptr[sl] + (size_t)j * 64 + l] = j < len ? colind[k0 + j] : self;
+    }
+  }
 }
 
 }  // namespace vfvm

@@ -14,6 +14,7 @@ struct vfvm_sys {
   std::vector<vfvm_bc_t> bc;
   Masks masks{};
   HostSetup hs;
+  std::vector<double> coord;  // dim x nnodes, as given at creation
   std::vector<int32_t> rowptr, colind, csc_perm, csc_rowval, dofdiag;
   std::vector<int64_t> csc_colptr;
   // device data
@@ -26,7 +27,11 @@ struct vfvm_sys {
   // multicolour ILU(0) structure
   IluStruct ilu;
   bool ilu_built = false;
-  DevBuf<int32_t> d_ilu_rowptr, d_ilu_colind, d_ilu_diag, d_ilu_perm, d_ilu_src, d_ilu_srcrow;
+  DevBuf<int32_t> d_ilu_rowptr, d_ilu_colind, d_ilu_diag, d_ilu_perm, d_ilu_src, d_ilu_srcrow, d_ilu_sptr,
+      d_ilu_scol, d_ilu_sdpos;
+  // SELL-64 copy of the Jacobian pattern for the Krylov operator
+  DevBuf<int32_t> d_sell_ptr, d_sell_col, d_sell_rbase;
+  int64_t sell_nnz = 0;
   LinearSolver lin;
   int asm_group = 8, spmv_group = 8;
   double *h_pinned = nullptr;  // small pinned readback area

@@ -74,7 +74,7 @@ SIGNATURES = {
     "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "vfvm_probe_arm": (C.c_int, [_vp, C.c_int, C.c_int64]),
-    "vfvm_probe_read": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int64)]),
+    "vfvm_probe_read": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "vfvm_pattern_csr": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_mpnn_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp)]),
     "vfvm_mpnn_destroy": (C.c_int, [_vp]),
@@ -83,10 +83,12 @@ SIGNATURES = {
     "vfvm_mpnn_train": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.c_int64,
                                   C.c_float, C.POINTER(C.c_float)]),
     "vfvm_mpnn_infer": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_float), _dp]),
+    "vfvm_mpnn_infer_sides": (C.c_int, [_vp, C.c_int64, _dp, _dp, C.c_double, _dp]),
     "vfvm_mpnn_loss_grad": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "vfvm_batch_create": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(_vp)]),
     "vfvm_batch_destroy": (C.c_int, [_vp]),
+    "vfvm_batch_set_graphs": (C.c_int, [_vp, C.c_int]),
     "vfvm_batch_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "vfvm_mpnn_time_layer": (C.c_int, [_vp, C.c_int64, C.c_int, _dp]),
     "vfvm_batch_update_member": (C.c_int, [_vp, C.c_int64, _dp, C.POINTER(BCT)]),
@@ -169,10 +171,12 @@ class Context:
             _LIB.vfvm_ctx_destroy(self.handle)
             self.handle = None
 
-    def probe(self, *kernels: int, max_launches: int = 200000):
+    def probe(self, *kernels: int, max_launches: int = 50000):
         """Context manager timing every launch of the given kernel classes (KERNEL_*)
         inside the calls it wraps, with HIP events on the launching stream.  Yields a
-        dict {kernel: {total_ms, launches, avg_ms}} filled on exit."""
+        dict {kernel: {total_ms, launches, avg_ms, work}} filled on exit; work is
+        (rows, nonzeros, 0) for SpMV and (nodes, adjacency entries, Jacobian slots)
+        for assembly, summed over the timed launches."""
         return _Probe(self, kernels, max_launches)
 
 
@@ -193,8 +197,10 @@ class _Probe:
         for k in self.kernels:
             t = C.c_double()
             n = C.c_int64()
-            check(lib().vfvm_probe_read(self.ctx.handle, k, C.byref(t), C.byref(n)), self.ctx.handle)
-            self.result[k] = dict(total_ms=t.value, launches=n.value, avg_ms=t.value / max(n.value, 1))
+            w = (C.c_int64 * 3)()
+            check(lib().vfvm_probe_read(self.ctx.handle, k, C.byref(t), C.byref(n), w), self.ctx.handle)
+            self.result[k] = dict(total_ms=t.value, launches=n.value, avg_ms=t.value / max(n.value, 1),
+                                  work=tuple(w))
         return False
 
 

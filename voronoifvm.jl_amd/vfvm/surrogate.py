@@ -56,19 +56,29 @@ def draw(seed, grf: GRF):
     return f, vals
 
 
-def features(grid, f, vals, eps=EPS):
-    """8 features per node: f/5, asinh(f/2) (the local reaction balance 2 sinh u = f),
-    each side value damped by exp(-distance / sqrt(eps/2)), x, y."""
-    x, y = grid.coord
-    delta = math.sqrt(eps / 2.0)
-    X = np.empty((grid.num_nodes, NFEAT), dtype=np.float32)
-    X[:, 0] = f / 5.0
-    X[:, 1] = np.arcsinh(f / 2.0)
-    for k, d in enumerate((y, 1.0 - x, 1.0 - y, x)):
-        X[:, 2 + k] = vals[k] * np.exp(-d / delta)
-    X[:, 6] = x
-    X[:, 7] = y
-    return X
+class Features:
+    """8 per-node input features of the network for sources F (m x n) and side values
+    V (m x 4: south, east, north, west): f / 5; asinh(f / 2) (the local balance
+    2 sinh u = f); each side value damped by exp(-distance / sqrt(eps / 2)); x; y."""
+
+    def __init__(self, grid, eps=EPS):
+        x, y = grid.coord
+        delta = math.sqrt(eps / 2.0)
+        self.decay = np.stack([np.exp(-d / delta) for d in (y, 1.0 - x, 1.0 - y, x)]).T.astype(np.float32)  # (n, 4)
+        self.x, self.y = x.astype(np.float32), y.astype(np.float32)
+        self.n = grid.num_nodes
+
+    def __call__(self, F, V):
+        F = np.asarray(F, dtype=np.float32).reshape(-1, self.n)
+        V = np.asarray(V, dtype=np.float32).reshape(-1, 4)
+        m = F.shape[0]
+        X = np.empty((m, self.n, NFEAT), dtype=np.float32)
+        X[:, :, 0] = F / 5.0
+        X[:, :, 1] = np.arcsinh(F / 2.0)
+        X[:, :, 2:6] = V[:, None, :] * self.decay[None, :, :]
+        X[:, :, 6] = self.x
+        X[:, :, 7] = self.y
+        return X
 
 
 def bcs(vals):
@@ -136,6 +146,16 @@ class Surrogate:
                                                 _capi.dptr(out)), self.ctx.handle)
         return out
 
+    def predict_sides(self, F, V, eps=EPS):
+        """Prediction with the features computed on the device from the sources F
+        (m x n) and side values V (m x 4); the same features as `Features`."""
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        out = np.zeros(F.shape)
+        _capi.check(_capi.lib().vfvm_mpnn_infer_sides(self._h, F.shape[0], _capi.dptr(F), _capi.dptr(V), eps,
+                                                      _capi.dptr(out)), self.ctx.handle)
+        return out
+
     def save(self, path=WEIGHTS, **meta):
         os.makedirs(os.path.dirname(path), exist_ok=True)
         np.savez(path, params=self.params(), nlayers=self.nlayers, **meta)
@@ -157,14 +177,17 @@ class PoissonFamily:
         self.systems = [make_system(self.grid, f0, np.zeros(4)) for _ in range(size)]
         self.batch = Batch(self.systems)
         self.size = size
+        self.features = Features(self.grid)
 
     def load_seeds(self, seeds):
-        """Fill the batch members with the seeds' data; returns the feature array."""
+        """Fill the batch members with the seeds' data; returns the sources (m x n) and
+        side values (m x 4) from which the network's features are computed."""
         seeds = list(seeds)
         assert len(seeds) <= self.size
-        feats = np.empty((len(seeds), self.grid.num_nodes, NFEAT), np.float32)
+        F = np.empty((len(seeds), self.grid.num_nodes))
+        V = np.empty((len(seeds), 4))
         for m, sd in enumerate(seeds):
             f, vals = draw(sd, self.grf)
             self.batch.update_member(m, source=np.asfortranarray(f.reshape(1, -1)), bcondition=bcs(vals))
-            feats[m] = features(self.grid, f, vals)
-        return feats
+            F[m], V[m] = f, vals
+        return F, V

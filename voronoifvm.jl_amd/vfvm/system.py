@@ -331,6 +331,11 @@ class Batch:
             _capi._LIB.vfvm_batch_destroy(self._h)
             self._h = None
 
+    def set_graphs(self, enable: bool):
+        """Run the batched Krylov iteration blocks as captured HIP graphs (default) or
+        launch their kernels one by one."""
+        _capi.check(_capi.lib().vfvm_batch_set_graphs(self._h, int(enable)), self.ctx.handle)
+
     def update_member(self, m, source=None, bcondition=None):
         """Replace member m's source and/or boundary conditions inside the batch."""
         s = self.systems[m]

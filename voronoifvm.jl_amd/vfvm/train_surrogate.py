@@ -33,7 +33,7 @@ def main():
     ctrl = SolverControl(precision="fp64", reltol_linear=1e-8, abstol_linear=0.0, maxiters_linear=3000)
     for b0 in range(0, len(seeds), 256):
         blk = seeds[b0:b0 + 256]
-        X[b0:b0 + len(blk)] = fam.load_seeds(blk)
+        X[b0:b0 + len(blk)] = fam.features(*fam.load_seeds(blk))
         U = fam.batch.solve(0.0, ctrl)
         worst = max(h.residual_norm for h in fam.batch.histories[:len(blk)])
         assert worst <= 1e-10, worst
