@@ -112,7 +112,8 @@ typedef struct {
 } vfvm_history_t;
 
 typedef struct {
-  int32_t iters, refinements, converged, pad;
+  int32_t iters, refinements, converged;
+  int32_t fp64_fallback; /* 1 when an FP32 refinement step failed and the solve finished in FP64 */
   double resnorm0, resnorm;
 } vfvm_linstats_t;
 
@@ -193,6 +194,9 @@ int vfvm_residual_norm(vfvm_sys *sys, const double *U, const double *Uold, doubl
  * adjacency and Jacobian rows staged in LDS; 0 = one node per lane with only the
  * Jacobian rows staged. */
 int vfvm_autotune(vfvm_sys *sys, int32_t *asm_group, int32_t *spmv_group);
+
+/* Set the group sizes directly (assembly 0, 1, 2, 4, 8, 16 or 32; CSR residual 2..32). */
+int vfvm_set_groups(vfvm_sys *sys, int32_t asm_group, int32_t spmv_group);
 
 /* Benchmark helper (no counterpart in the reference; it has no device): average device
  * time in milliseconds of `reps` back-to-back launches of one hot kernel on the handle's

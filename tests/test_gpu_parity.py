@@ -52,6 +52,23 @@ def test_assembly_drift_diffusion_2d():
     cases.compare_assembly(sys, osys, U, embed=0.3)
 
 
+@pytest.mark.parametrize("group", [0, 1, 2, 4, 8, 16, 32])
+def test_assembly_kernel_variants(group):
+    """Every assembly kernel variant the autotuner may pick (lanes per node; 1 and 0: one
+    node per lane with LDS staging) against the oracle, for 1 and 3 species."""
+    _print("case assembly variant", group)
+    sys, osys = cases.nonlinear_diffusion_3d(9)
+    sys.set_groups(group)
+    u0 = cases.initial_sine_3d(sys.grid).reshape(1, -1)
+    cases.compare_assembly(sys, osys, 1.3 * u0, Uold=u0, tstep=1e-2)
+    sys, osys = cases.drift_diffusion_2d(17, lambda2=1e-2)
+    sys.set_groups(group)
+    rng = np.random.default_rng(5)
+    U = cases.dd_initial(sys.grid)
+    U[0] += rng.uniform(-2, 2, sys.grid.num_nodes)
+    cases.compare_assembly(sys, osys, U, embed=0.3)
+
+
 def test_assembly_two_species_exchange_legacy_bc():
     _print("case assembly exchange")
     X = linspace(0, 1, 12)

@@ -306,44 +306,63 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
 #pragma unroll
       for (int j = 0; j < NS; j++) Jd[i][j] = 0.0;
     }
-    for (int e = 0; e < deg1; e++) {
-      if (e == diag) continue;
-      const int32_t M = sadj ? s_col[a0 - e0 + e] : a.adj_col[a0 + e];
-      const double f = sadj ? s_fac[a0 - e0 + e] : a.adj_fac[a0 + e];
-      if constexpr (JAC) {
-        Dual<NP> uk[NS], ul[NS], g[NS];
+    // the row is walked in blocks of EB entries: all adjacency and neighbour loads of a
+    // block are issued before any of its Jacobian stores, so they are in flight together
+    constexpr int EB = NS == 1 ? 8 : (NS == 2 ? 4 : 2);
+    for (int eb = 0; eb < deg1; eb += EB) {
+      int32_t Mv[EB];
+      double fv[EB], ulv[EB][NS];
 #pragma unroll
-        for (int i = 0; i < NS; i++) {
-          uk[i] = dvar<NP>(uK[i], i);
-          ul[i] = dvar<NP>(a.U[(int64_t)M * NS + i], NS + i);
-        }
-        eval_flux<NS>(flux_kind, p, uk, ul, g);
+      for (int q = 0; q < EB; q++) {
+        const int e = eb + q;
+        const bool ok = e < deg1;
+        Mv[q] = ok ? (sadj ? s_col[a0 - e0 + e] : a.adj_col[a0 + e]) : (int32_t)K;
+        fv[q] = ok ? (sadj ? s_fac[a0 - e0 + e] : a.adj_fac[a0 + e]) : 0.0;
+      }
 #pragma unroll
-        for (int i = 0; i < NS; i++) {
-          Fe[i] += f * g[i].v;
-          const int64_t boff =
-              rowbase[i] + (e < diag ? (int64_t)e * a.m.ecnt[i] : (int64_t)(e - 1) * a.m.ecnt[i] + a.m.dcnt[i]);
-          const uint8_t em = a.m.emask[i];
+      for (int q = 0; q < EB; q++)
 #pragma unroll
-          for (int j = 0; j < NS; j++) {
-            if (em & (1u << j)) {
-              const double dk = g[i].d[j], dl = g[i].d[NS + j];
-              bad |= (dk != dk) | (dl != dl);
-              Jd[i][j] += f * dk;
-              put(boff + rank_in<NS>(em, j), f * dl);
+        for (int i = 0; i < NS; i++) ulv[q][i] = a.U[(int64_t)Mv[q] * NS + i];
+#pragma unroll
+      for (int q = 0; q < EB; q++) {
+        const int e = eb + q;
+        if (e >= deg1 || e == diag) continue;
+        const double f = fv[q];
+        if constexpr (JAC) {
+          Dual<NP> uk[NS], ul[NS], g[NS];
+#pragma unroll
+          for (int i = 0; i < NS; i++) {
+            uk[i] = dvar<NP>(uK[i], i);
+            ul[i] = dvar<NP>(ulv[q][i], NS + i);
+          }
+          eval_flux<NS>(flux_kind, p, uk, ul, g);
+#pragma unroll
+          for (int i = 0; i < NS; i++) {
+            Fe[i] += f * g[i].v;
+            const int64_t boff =
+                rowbase[i] + (e < diag ? (int64_t)e * a.m.ecnt[i] : (int64_t)(e - 1) * a.m.ecnt[i] + a.m.dcnt[i]);
+            const uint8_t em = a.m.emask[i];
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+              if (em & (1u << j)) {
+                const double dk = g[i].d[j], dl = g[i].d[NS + j];
+                bad |= (dk != dk) | (dl != dl);
+                Jd[i][j] += f * dk;
+                put(boff + rank_in<NS>(em, j), f * dl);
+              }
             }
           }
-        }
-      } else {
-        double uk[NS], ul[NS], g[NS];
+        } else {
+          double uk[NS], ul[NS], g[NS];
 #pragma unroll
-        for (int i = 0; i < NS; i++) {
-          uk[i] = uK[i];
-          ul[i] = a.U[(int64_t)M * NS + i];
-        }
-        eval_flux<NS>(flux_kind, p, uk, ul, g);
+          for (int i = 0; i < NS; i++) {
+            uk[i] = uK[i];
+            ul[i] = ulv[q][i];
+          }
+          eval_flux<NS>(flux_kind, p, uk, ul, g);
 #pragma unroll
-        for (int i = 0; i < NS; i++) Fe[i] += f * g[i];
+          for (int i = 0; i < NS; i++) Fe[i] += f * g[i];
+        }
       }
     }
     double Fn[NS], Jn[NS][NS], Fb[NS], Jb[NS];
@@ -452,14 +471,14 @@ __global__ void k_penalty_free_norm(int64_t nnodes, int nspec, const double *F, 
         const int kind = P->bc[reg - 1][i].kind;
         pen |= (kind == VFVM_BC_DIRICHLET || kind == VFVM_BC_DIRICHLET_LEGACY);
       }
-      if (!pen) m = fmax(m, fabs(F[K * nspec + i]));
+      if (!pen) m = nanmax(m, fabs(F[K * nspec + i]));
     }
   }
   const int id0 = __shfl(id, 0, 64);
   const bool uniform = __all(id == id0 || id < 0);
   if (uniform) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    for (int o = 32; o > 0; o >>= 1) m = nanmax(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0 && id0 >= 0 && m > 0.0)
       atomicMax(out + id0, (unsigned long long)__double_as_longlong(m));
   } else if (id >= 0 && m > 0.0) {

@@ -53,13 +53,13 @@ __device__ __forceinline__ double bsum(double v, double *sh) {
 
 __device__ __forceinline__ double bmax(double v, double *sh) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
   double r = 0.0;
   if (threadIdx.x == 0)
-    for (int k = 0; k < (int)(blockDim.x >> 6); k++) r = fmax(r, sh[k]);
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) r = nanmax(r, sh[k]);
   return r;
 }
 
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(BB) k_bfinalize(Seg S, KState *st, const int32
   double tmax = 0.0;
   if (MODE == BF_UPDATE) {
     double acc = 0.0;
-    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) acc = fmax(acc, S.partials[(size_t)2 * S.nch + c]);
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) acc = nanmax(acc, S.partials[(size_t)2 * S.nch + c]);
     tmax = bmax(acc, sh);
   }
   if (threadIdx.x != 0) return;
@@ -132,16 +132,19 @@ template <class VA, class V, int NDOT, int STAGE>
 __global__ void __launch_bounds__(BB) k_bspmv(Seg S, const int32_t *__restrict__ ch_slice,
                                               const int32_t *__restrict__ sptr, const int32_t *__restrict__ col,
                                               const VA *__restrict__ val, const V *__restrict__ x, V *__restrict__ y,
-                                              const V *__restrict__ w0, KState *st, const int32_t *__restrict__ rowptr,
-                                              unsigned long long *work) {
+                                              const V *__restrict__ w0, KState *st,
+                                              unsigned long long *__restrict__ mwork,
+                                              const int64_t *__restrict__ msize) {
   const int inst = S.ch_inst[blockIdx.x];
   KState *sti = st + inst;
   if (sti->done) return;
   if (STAGE == STAGE_OMEGA && sti->half) return;
   const int64_t r0 = S.ch_beg[blockIdx.x], r1 = S.ch_end[blockIdx.x];
-  if (work && threadIdx.x == 0) {  // probe counters: rows and nonzeros of this live chunk
-    atomicAdd(&work[0], (unsigned long long)(r1 - r0));
-    atomicAdd(&work[1], (unsigned long long)(rowptr[r1] - rowptr[r0]));
+  // probe counters: the member's first chunk adds the member's rows and nonzeros (one
+  // writer per member and launch, no atomics)
+  if (mwork && threadIdx.x == 0 && blockIdx.x == S.ch_first[inst]) {
+    mwork[2 * inst] += (unsigned long long)msize[2 * inst];
+    mwork[2 * inst + 1] += (unsigned long long)msize[2 * inst + 1];
   }
   const int32_t sl0 = ch_slice[blockIdx.x];
   double d0 = 0.0, d1 = 0.0;
@@ -290,7 +293,7 @@ __global__ void __launch_bounds__(BB) k_bupdate(Seg S, double *__restrict__ U, c
       const double d = dU[i];
       u -= dm * d;
       U[i] = u;
-      m = fmax(m, fabs(d));
+      m = nanmax(m, fabs(d));
     }
     s += fabs(u);
   }
@@ -306,6 +309,29 @@ __global__ void k_bfill(int64_t n, T *p, T v) {
 template <class TO, class TI>
 __global__ void k_bconvert(int64_t n, const TI *__restrict__ a, TO *__restrict__ b) {
   for (int64_t i = (int64_t)blockIdx.x * BB + threadIdx.x; i < n; i += (int64_t)gridDim.x * BB) b[i] = (TO)a[i];
+}
+
+// adds the per-member probe counters (rows, nonzeros) to the probe's totals and clears them
+__global__ void k_collect_work(int64_t nsys, unsigned long long *__restrict__ mwork, unsigned long long *dwork) {
+  __shared__ unsigned long long sh[2][BB];
+  unsigned long long a = 0, c = 0;
+  for (int64_t m = threadIdx.x; m < nsys; m += BB) {
+    a += mwork[2 * m];
+    c += mwork[2 * m + 1];
+    mwork[2 * m] = 0;
+    mwork[2 * m + 1] = 0;
+  }
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < BB; k++) {
+      a += sh[0][k];
+      c += sh[1][k];
+    }
+    dwork[0] += a;
+    dwork[1] += c;
+  }
 }
 
 unsigned gfor(int64_t n) {
@@ -345,6 +371,9 @@ struct vfvm_batch {
   DevBuf<int32_t> sell_ptr, sell_col, sell_rbase, ch_slice;
   DevBuf<double> A64s;
   int64_t sell_nnz = 0;
+  // probe work of the batched SpMV: per member (rows, nonzeros) and per-member counters
+  DevBuf<int64_t> msize;
+  DevBuf<unsigned long long> mwork;
   // captured Krylov iteration blocks: (value bytes, solution vector, iterations) -> graph
   bool use_graphs = true;
   std::map<std::tuple<int, const void *, int>, hipGraphExec_t> graphs;
@@ -402,16 +431,16 @@ template <class V, int NDOT, int STAGE>
 void bop(vfvm_batch &b, const V *x, V *y, const V *w0) {
   const Seg S = b.seg();
   const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
-  unsigned long long *work = b.ctx->probe.counters(kid);
+  unsigned long long *mwork = b.ctx->probe.counters(kid) ? b.mwork.p : nullptr;
   b.ctx->probe.before(kid, b.ctx->stream);
   if constexpr (sizeof(V) == 4)
     hipLaunchKernelGGL((k_bspmv<float, V, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                       b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const float *)b.A32.p, x, y, w0, b.st.p,
-                       b.rowptr.p, work);
+                       b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const float *)b.A32.p, x, y, w0, b.st.p, mwork,
+                       (const int64_t *)b.msize.p);
   else
     hipLaunchKernelGGL((k_bspmv<double, V, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                       b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const double *)b.A64s.p, x, y, w0, b.st.p,
-                       b.rowptr.p, work);
+                       b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const double *)b.A64s.p, x, y, w0, b.st.p, mwork,
+                       (const int64_t *)b.msize.p);
   b.ctx->probe.after(kid, b.ctx->stream);
   if constexpr (NDOT > 0)
     hipLaunchKernelGGL((k_bfinalize<2, STAGE, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, b.ctx->stream, S,
@@ -519,6 +548,11 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
     bool all = true;
     for (auto &k : h) all &= (k.done != 0);
     if (all || launched >= maxit + every) break;
+  }
+  {
+    const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
+    if (unsigned long long *dw = b.ctx->probe.counters(kid))
+      hipLaunchKernelGGL(k_collect_work, dim3(1), dim3(BB), 0, st, b.nsys, b.mwork.p, dw);
   }
   int rc = 0;
   for (size_t m = 0; m < h.size(); m++) {
@@ -739,6 +773,16 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
     b->norms.alloc((size_t)2 * nsys);
     for (auto &v : b->k64) v.alloc((size_t)b->ndof);
     for (auto &v : b->k32) v.alloc((size_t)b->ndof);
+    {
+      std::vector<int64_t> ms((size_t)2 * nsys);
+      for (int64_t m = 0; m < nsys; m++) {
+        ms[2 * m] = b->dof_off[m + 1] - b->dof_off[m];
+        ms[2 * m + 1] = systems[m]->nnz;
+      }
+      b->msize.upload(ms, st);
+      std::vector<unsigned long long> z((size_t)2 * nsys, 0ull);
+      b->mwork.upload(z, st);
+    }
     // SELL-64 operator values; padding entries stay 0
     b->A32.alloc((size_t)b->sell_nnz);
     b->A64s.alloc((size_t)b->sell_nnz);

@@ -767,6 +767,17 @@ int vfvm_residual_norm(vfvm_sys *s, const double *U, const double *Uold, double 
   });
 }
 
+int vfvm_set_groups(vfvm_sys *s, int32_t asm_group, int32_t spmv_group) {
+  if (!s) return VFVM_ERR_ARG;
+  const bool ag = asm_group >= 0 && asm_group <= 32 && (asm_group & (asm_group - 1)) == 0;
+  const bool sg = spmv_group >= 2 && spmv_group <= 32 && (spmv_group & (spmv_group - 1)) == 0;
+  if (!ag || !sg) return VFVM_ERR_ARG;
+  s->asm_group = asm_group;
+  s->spmv_group = spmv_group;
+  s->lin.set_group(spmv_group);
+  return VFVM_OK;
+}
+
 int vfvm_autotune(vfvm_sys *s, int32_t *asm_group, int32_t *spmv_group) {
   if (!s) return VFVM_ERR_ARG;
   return guarded(s->ctx, [&]() {

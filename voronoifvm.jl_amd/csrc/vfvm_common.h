@@ -29,6 +29,10 @@ struct Error : std::runtime_error {
       throw ::vfvm::Error(VFVM_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));      \
   } while (0)
 
+// max that keeps a NaN (fmax drops it): used by the norms that decide convergence, so
+// a NaN update or residual can never look small
+__device__ __forceinline__ double nanmax(double a, double b) { return (a != a) ? a : ((b != b || b > a) ? b : a); }
+
 // Device buffer owned by a handle (freed on destroy).
 template <class T>
 struct DevBuf {

@@ -157,11 +157,17 @@ class LinearSolver {
   DevBuf<KState> st;
   DevBuf<double> partials, rowscale, binv, bhat, rres, ilu64, iluy64, vbasis64;
   DevBuf<double> A64s, ilu64s;  // row-scaled operator / factor in SELL-64 order (FP64 Krylov)
-  DevBuf<float> A32, ilu32, ilu32s, iluy32, vbasis32, rhs32, d32;
+  DevBuf<float> A32, ilu32s, iluy32, vbasis32, rhs32, d32;
   DevBuf<double> gmH, gmh1, gmh2, gmcs, gmsn, gmg, gmy, gmpart;
   Work<float> w32;
   Work<double> w64;
   bool ilu_ready = false;
+  bool ilu_sell_ready[2] = {false, false};  // SELL-64 copies of the factor (FP32, FP64)
+  DevBuf<double> fallback64;  // FP64 correction when an FP32 refinement step fails
+ public:
+  int64_t fallbacks = 0;      // linear solves finished in FP64 after an FP32 failure
+
+ private:
   bool sell_nt[2] = {false, false};
   double last_rr = 0, last_rr0 = 0;
   int last_iters = 0;

@@ -287,9 +287,11 @@ __global__ void __launch_bounds__(VB) k_convert(int64_t n, const TI *__restrict_
 }
 
 // x64 += d32
-__global__ void __launch_bounds__(VB) k_add_correction(int64_t n, double *__restrict__ x, const float *__restrict__ d) {
+template <class VD>
+__global__ void __launch_bounds__(VB) k_add_correction(int64_t n, double *__restrict__ x, const VD *__restrict__ d,
+                                                       double sign) {
   for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB)
-    x[i] += (double)d[i];
+    x[i] += sign * (double)d[i];
 }
 
 // -------------------------------------------------- scaling / conversion ---
@@ -423,14 +425,14 @@ __global__ void __launch_bounds__(VB) k_ilu_factor(int32_t r0, int32_t r1, const
 }
 
 // factor values from CSR order to SELL-64 order (after each factorisation)
-template <class VL>
+template <class VI, class VO>
 __global__ void __launch_bounds__(VB) k_to_sell(int64_t nrows, const int32_t *__restrict__ rowptr,
-                                                const int32_t *__restrict__ sptr, const VL *__restrict__ a,
-                                                VL *__restrict__ as) {
+                                                const int32_t *__restrict__ sptr, const VI *__restrict__ a,
+                                                VO *__restrict__ as) {
   for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * VB) {
     const int32_t k0 = rowptr[i], k1 = rowptr[i + 1];
     const int64_t q0 = (int64_t)sptr[i >> 6] + (i & 63);
-    for (int32_t k = k0; k < k1; k++) as[q0 + (int64_t)(k - k0) * 64] = a[k];
+    for (int32_t k = k0; k < k1; k++) as[q0 + (int64_t)(k - k0) * 64] = (VO)a[k];
   }
 }
 
@@ -620,20 +622,20 @@ __global__ void __launch_bounds__(VB) k_newton_update(int64_t n, double *__restr
     const double d = dU[i];
     const double u = U[i] - damp * d;
     U[i] = u;
-    m = fmax(m, fabs(d));
+    m = nanmax(m, fabs(d));
     s += fabs(u);
   }
   // block max and sum -> partials [sum | max]; k_newton_finalize combines them
   __shared__ double shm[16];
   const double bs = block_sum(s, shm);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  for (int o = 32; o > 0; o >>= 1) m = nanmax(m, __shfl_xor(m, o, 64));
   __syncthreads();
   if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     double bm = 0.0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); k++) bm = fmax(bm, shm[k]);
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) bm = nanmax(bm, shm[k]);
     partials[blockIdx.x] = bs;
     partials[(size_t)gridDim.x + blockIdx.x] = bm;
   }
@@ -647,17 +649,17 @@ __global__ void __launch_bounds__(VB) k_newton_finalize(const double *__restrict
   double acc = 0.0, mx = 0.0;
   for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
     acc += partials[b];
-    if (with_max) mx = fmax(mx, partials[(size_t)nparts + b]);
+    if (with_max) mx = nanmax(mx, partials[(size_t)nparts + b]);
   }
   acc = block_sum(acc, sh);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  for (int o = 32; o > 0; o >>= 1) mx = nanmax(mx, __shfl_xor(mx, o, 64));
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
   __syncthreads();
   if (threadIdx.x == 0) {
     double m2 = 0.0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); k++) m2 = fmax(m2, sh[k]);
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) m2 = nanmax(m2, sh[k]);
     if (with_max) *out_max = m2;
     *out_sum = acc;
   }
@@ -774,40 +776,38 @@ void LinearSolver::prepare(const double *J, const vfvm_control_t &c, bool refact
   if (c.precon == VFVM_PRECON_ILU0 && (refactor || !ilu_ready)) {
     if (!L.ilu) throw Error(VFVM_ERR_ARG, "ILU(0) needs the multicolour structure (build_ilu)");
     const IluStruct &I = *L.ilu;
+    // the factorisation runs in FP64 for both precisions (pivots of non-M-matrices can
+    // shrink); each precision's solve gets a SELL-64 copy of the factor
+    ilu64.alloc((size_t)L.nnz);
+    hipLaunchKernelGGL((k_ilu_gather<double>), dim3(grid_for(L.nnz)), dim3(VB), 0, stream, L.nnz, I.src, I.srcrow, J,
+                       rowscale.p, ilu64.p);
+    for (int c0 = 0; c0 < I.ncolors; c0++) {
+      int32_t r0 = I.cptr[c0], r1 = I.cptr[c0 + 1];
+      if (r1 > r0)
+        hipLaunchKernelGGL((k_ilu_factor<double>), dim3((r1 - r0 + VB - 1) / VB), dim3(VB), 0, stream, r0, r1,
+                           I.rowptr, I.colind, I.diag, ilu64.p);
+    }
+    ilu_ready = true;
+    ilu_sell_ready[0] = ilu_sell_ready[1] = false;
+  }
+  if (c.precon == VFVM_PRECON_ILU0 && !ilu_sell_ready[fp32 ? 0 : 1]) {
+    const IluStruct &I = *L.ilu;
     if (fp32) {
-      ilu32.alloc((size_t)L.nnz);
-      hipLaunchKernelGGL((k_ilu_gather<float>), dim3(grid_for(L.nnz)), dim3(VB), 0, stream, L.nnz, I.src, I.srcrow, J,
-                         rowscale.p, ilu32.p);
-      for (int c0 = 0; c0 < I.ncolors; c0++) {
-        int32_t r0 = I.cptr[c0], r1 = I.cptr[c0 + 1];
-        if (r1 > r0)
-          hipLaunchKernelGGL((k_ilu_factor<float>), dim3((r1 - r0 + VB - 1) / VB), dim3(VB), 0, stream, r0, r1,
-                             I.rowptr, I.colind, I.diag, ilu32.p);
-      }
       if (ilu32s.n != (size_t)I.snnz) {
         ilu32s.alloc((size_t)I.snnz);
         hipLaunchKernelGGL((k_fill<float>), dim3(grid_for(I.snnz)), dim3(VB), 0, stream, I.snnz, ilu32s.p, 0.0f);
       }
-      hipLaunchKernelGGL((k_to_sell<float>), dim3(grid_for(L.ndof)), dim3(VB), 0, stream, L.ndof, I.rowptr, I.sptr,
-                         (const float *)ilu32.p, ilu32s.p);
+      hipLaunchKernelGGL((k_to_sell<double, float>), dim3(grid_for(L.ndof)), dim3(VB), 0, stream, L.ndof, I.rowptr,
+                         I.sptr, (const double *)ilu64.p, ilu32s.p);
     } else {
-      ilu64.alloc((size_t)L.nnz);
-      hipLaunchKernelGGL((k_ilu_gather<double>), dim3(grid_for(L.nnz)), dim3(VB), 0, stream, L.nnz, I.src, I.srcrow, J,
-                         rowscale.p, ilu64.p);
-      for (int c0 = 0; c0 < I.ncolors; c0++) {
-        int32_t r0 = I.cptr[c0], r1 = I.cptr[c0 + 1];
-        if (r1 > r0)
-          hipLaunchKernelGGL((k_ilu_factor<double>), dim3((r1 - r0 + VB - 1) / VB), dim3(VB), 0, stream, r0, r1,
-                             I.rowptr, I.colind, I.diag, ilu64.p);
-      }
       if (ilu64s.n != (size_t)I.snnz) {
         ilu64s.alloc((size_t)I.snnz);
         hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(I.snnz)), dim3(VB), 0, stream, I.snnz, ilu64s.p, 0.0);
       }
-      hipLaunchKernelGGL((k_to_sell<double>), dim3(grid_for(L.ndof)), dim3(VB), 0, stream, L.ndof, I.rowptr, I.sptr,
-                         (const double *)ilu64.p, ilu64s.p);
+      hipLaunchKernelGGL((k_to_sell<double, double>), dim3(grid_for(L.ndof)), dim3(VB), 0, stream, L.ndof,
+                         I.rowptr, I.sptr, (const double *)ilu64.p, ilu64s.p);
     }
-    ilu_ready = true;
+    ilu_sell_ready[fp32 ? 0 : 1] = true;
     VFVM_HIP_CHECK(hipGetLastError());
   }
 }
@@ -1078,6 +1078,7 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
   const double bnorm = std::sqrt(h.rr);
   const double tol = c.abstol_linear + c.reltol_linear * bnorm;
   int iters = 0, rc = 0, nref = 0;
+  bool used_fallback = false;
   double rn = bnorm;
   if (c.precision == VFVM_PREC_FP32_REFINE) {
     // FP64 iterative refinement around FP32 Krylov solves of Ahat d = rhat
@@ -1088,6 +1089,7 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
     double *rh = rres.p;
     hipLaunchKernelGGL((k_convert<double, double>), dim3(g), dim3(VB), 0, stream, n, (const double *)bhat.p, rh);
     const int maxref = std::max(1, c.max_refinements);
+    bool fp32_failed = false;
     while (rn > tol && nref < maxref) {
       hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(VB), 0, stream, n, (const double *)rh, rhs32.p);
       int it = 0;
@@ -1097,14 +1099,51 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
       else
         bicgstab<float, float, float>(rhs32.p, r32, nullptr, c, inner, 0.0, c.maxiters_linear, it);
       iters += it;
-      hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(VB), 0, stream, n, x, (const float *)r32);
+      hipLaunchKernelGGL((k_add_correction<float>), dim3(g), dim3(VB), 0, stream, n, x, (const float *)r32, 1.0);
+      KState z{};
+      VFVM_HIP_CHECK(hipMemcpyAsync(st.p, &z, sizeof(KState), hipMemcpyHostToDevice, stream));
+      scaled_res(x, rh);
+      read_state(h);
+      const double rnew = std::sqrt(h.rr);
+      nref++;
+      if (!(rnew < rn)) {
+        // the FP32 solve did not reduce the FP64 residual (stagnation or breakdown at
+        // this conditioning): take the correction back and finish in FP64
+        hipLaunchKernelGGL((k_add_correction<float>), dim3(g), dim3(VB), 0, stream, n, x, (const float *)r32, -1.0);
+        VFVM_HIP_CHECK(hipMemcpyAsync(st.p, &z, sizeof(KState), hipMemcpyHostToDevice, stream));
+        scaled_res(x, rh);
+        read_state(h);
+        rn = std::sqrt(h.rr);
+        fp32_failed = true;
+        break;
+      }
+      rn = rnew;
+      if (it == 0) break;
+    }
+    if (fp32_failed && rn > tol) {
+      // FP64 fallback on the current refinement residual rh = S (b - A x): the FP64
+      // operator and preconditioner copies of the same Jacobian (no refactorisation)
+      vfvm_control_t c64 = c;
+      c64.precision = VFVM_PREC_FP64;
+      prepare(J, c64, false);
+      DevBuf<double> &d64 = fallback64;
+      d64.alloc((size_t)n);
+      int it = 0;
+      const double rtol = tol / std::max(rn, 1e-300);
+      if (c.krylov == VFVM_KRYLOV_GMRES)
+        gmres<double, double, double>(rh, d64.p, J, c, rtol, 0.0, c.maxiters_linear, it);
+      else
+        bicgstab<double, double, double>(rh, d64.p, J, c, rtol, 0.0, c.maxiters_linear, it);
+      iters += it;
+      fallbacks++;
+      used_fallback = true;
+      hipLaunchKernelGGL((k_add_correction<double>), dim3(g), dim3(VB), 0, stream, n, x, (const double *)d64.p, 1.0);
       KState z{};
       VFVM_HIP_CHECK(hipMemcpyAsync(st.p, &z, sizeof(KState), hipMemcpyHostToDevice, stream));
       scaled_res(x, rh);
       read_state(h);
       rn = std::sqrt(h.rr);
       nref++;
-      if (it == 0) break;
     }
     rc = rn <= tol ? 0 : VFVM_ERR_LINEAR;
   } else {
@@ -1120,6 +1159,7 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
     stats->iters = iters;
     stats->refinements = nref;
     stats->converged = rc == 0;
+    stats->fp64_fallback = used_fallback ? 1 : 0;
     stats->resnorm0 = bnorm;
     stats->resnorm = rn;
   }

@@ -44,7 +44,8 @@ class HistoryT(C.Structure):
 
 
 class LinStatsT(C.Structure):
-    _fields_ = [("iters", C.c_int32), ("refinements", C.c_int32), ("converged", C.c_int32), ("pad", C.c_int32),
+    _fields_ = [("iters", C.c_int32), ("refinements", C.c_int32), ("converged", C.c_int32),
+                ("fp64_fallback", C.c_int32),
                 ("resnorm0", C.c_double), ("resnorm", C.c_double)]
 
 
@@ -72,6 +73,7 @@ SIGNATURES = {
                                          C.POINTER(HistoryT)]),
     "vfvm_residual_norm": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
     "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
+    "vfvm_set_groups": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "vfvm_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "vfvm_probe_arm": (C.c_int, [_vp, C.c_int, C.c_int64]),
     "vfvm_probe_read": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -184,6 +184,11 @@ class System:
         _capi.check(_capi.lib().vfvm_autotune(self.handle(), C.byref(a), C.byref(s)), self.ctx.handle)
         return a.value, s.value
 
+    def set_groups(self, asm_group: int, spmv_group: int = 8):
+        """Choose the assembly kernel variant (lanes per node; 1 and 0: one node per lane
+        with LDS staging) and the lanes per row of the CSR residual kernels."""
+        _capi.check(_capi.lib().vfvm_set_groups(self.handle(), asm_group, spmv_group), self.ctx.handle)
+
     def nodevolumes(self):
         v = np.zeros(self.grid.num_nodes)
         _capi.check(_capi.lib().vfvm_nodevolumes(self.handle(), _capi.dptr(v)))
