@@ -294,6 +294,7 @@ def run_cfg3(args):
     ndof = sum(s.grid.num_nodes for s in systems)
     nnz = sum(s.sizes()[1] for s in systems)
     batch = vfvm.Batch(systems)
+    batch.set_graphs(not args.no_graphs)
     log(f"cfg3: {len(systems)} instances, {ndof} unknowns, nnz {nnz}, setup {time.perf_counter() - t0:.1f}s, "
         f"autotuned groups {batch.groups}")
     K, W = args.steps, args.warmup
@@ -347,7 +348,8 @@ def run_cfg3(args):
                 config=dict(workload="cfg3: 256 nonlinear-Poisson instances, N x N grids with N in "
                                      "{33,65,129,257}, seeded eps/boundary values/source, batched "
                                      "BiCGStab+Jacobi, FP32 inside FP64 Newton (plain: 256 sequential FP64 "
-                                     "GPU Newton solves)", instances=len(systems), unknowns=ndof, nnz=nnz),
+                                     "GPU Newton solves)", instances=len(systems), unknowns=ndof, nnz=nnz,
+                            hip_graphs=not args.no_graphs),
                 dtype="f64+f32")
 
 
@@ -373,6 +375,7 @@ def run_cfg4(args):
     on the 1,024 held-out seeds of the 129 x 129 nonlinear Poisson family."""
     from vfvm import surrogate as S
     fam = S.PoissonFamily(size=256)
+    fam.batch.set_graphs(not args.no_graphs)
     net = S.Surrogate(fam.systems[0])
     meta = net.load()
     seeds = list(S.EVAL_SEEDS)
@@ -482,7 +485,8 @@ def run_cfg4(args):
                                      "values U[-1,1]), 1,024 held-out seeds in batches of 256: MPNN initial guess "
                                      "(4 layers, width 64, BF16 MFMA) + batched BiCGStab/Jacobi FP32 inside FP64 "
                                      "Newton, vs plain FP64 GPU Newton from zero one solve at a time",
-                            solves=nsol, unknowns_per_solve=fam.grid.num_nodes, batch=256),
+                            solves=nsol, unknowns_per_solve=fam.grid.num_nodes, batch=256,
+                            hip_graphs=not args.no_graphs),
                 dtype="f64+f32+bf16")
 
 
@@ -548,6 +552,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg4", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="batched Krylov blocks launched kernel by kernel instead of as HIP graphs "
+                         "(for kernel-trace profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); multi-GPU runs are not part "

@@ -6,7 +6,9 @@ The reference has no learned component; this network is defined by this
 project (the "learned Newton initial guess" of the hot-path specification),
 so the oracle restates the project's own definition in plain numpy:
 
-    X  : n x 16 node features (8 used, zero padded)
+    X  : n x 16 network input: 8 node features, then context channels 8..15 =
+         features 1 and 0 smoothed by K = 16, 64, 160, 324 steps of
+         y <- (y + W y) / 2, W y_i = sum_j w_ij y_j (snapshots in that order)
     h0 = relu(bf(X) @ bf(We) + be)                                  (n x 64)
     for l in 0..L-1:
         a_l  = Agg(h_l),  Agg(h)_i = sum_j w_ij h_j - h_i,  w_ij = s_ij / sum_k s_ik
@@ -79,7 +81,7 @@ def init_params(seed, nlayers=4, width=WIDTH):
     """He-uniform initialisation (the library uses the same rule, its own RNG)."""
     rng = np.random.default_rng(seed)
     parts = []
-    parts.append(rng.uniform(-1, 1, (KPAD, width)) * np.sqrt(6.0 / NFEAT))
+    parts.append(rng.uniform(-1, 1, (KPAD, width)) * np.sqrt(6.0 / KPAD))
     parts.append(np.zeros(width))
     for _ in range(nlayers):
         parts.append(rng.uniform(-1, 1, (2 * width, width)) * np.sqrt(6.0 / (2 * width)) * 0.5)
@@ -105,6 +107,30 @@ def graph_weights(adj_ptr, adj_col, adj_fac):
     return np.concatenate(rows), np.concatenate(cols), np.concatenate(w)
 
 
+SMOOTH_STEPS = (16, 64, 160, 324)
+SMOOTH_SRC = (1, 0)
+
+
+def expand(X, graph):
+    """(nsamples, n, 8) features -> (nsamples, n, 16) network input with the smoothed
+    context channels (float64 here; the device smooths in float32)."""
+    import scipy.sparse as sp
+    rows, cols, w = graph
+    ns, n, _ = X.shape
+    Xp = np.zeros((ns, n, KPAD))
+    Xp[:, :, :NFEAT] = X
+    Wm = sp.csr_matrix((w, (rows, cols)), shape=(n, n))
+    for c, src in enumerate(SMOOTH_SRC):
+        y = Xp[:, :, src].T.copy()
+        snap = 0
+        for step in range(1, SMOOTH_STEPS[-1] + 1):
+            y = 0.5 * (y + Wm @ y)
+            if step == SMOOTH_STEPS[snap]:
+                Xp[:, :, NFEAT + 4 * c + snap] = y.T
+                snap += 1
+    return Xp
+
+
 def _agg(h, rows, cols, w, n):
     out = np.zeros_like(h)
     np.add.at(out, rows, w[:, None] * h[cols])
@@ -123,8 +149,7 @@ def forward(theta, X, graph, nlayers=4, emulate_bf16=True, keep=False):
     We, be, layers, wd, bd = unpack(theta, nlayers)
     rows, cols, w = graph
     ns, n, _ = X.shape
-    Xp = np.zeros((ns, n, KPAD))
-    Xp[:, :, :NFEAT] = X
+    Xp = expand(X, graph)
     us, caches = [], []
     for s in range(ns):
         x = bf(Xp[s])

@@ -164,6 +164,7 @@ class LinearSolver {
   bool ilu_ready = false;
   bool ilu_sell_ready[2] = {false, false};  // SELL-64 copies of the factor (FP32, FP64)
   DevBuf<double> fallback64;  // FP64 correction when an FP32 refinement step fails
+  DevBuf<double> xsave;       // solution before the latest FP32 correction
  public:
   int64_t fallbacks = 0;      // linear solves finished in FP64 after an FP32 failure
 

@@ -1099,6 +1099,8 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
       else
         bicgstab<float, float, float>(rhs32.p, r32, nullptr, c, inner, 0.0, c.maxiters_linear, it);
       iters += it;
+      xsave.alloc((size_t)n);
+      hipLaunchKernelGGL((k_convert<double, double>), dim3(g), dim3(VB), 0, stream, n, (const double *)x, xsave.p);
       hipLaunchKernelGGL((k_add_correction<float>), dim3(g), dim3(VB), 0, stream, n, x, (const float *)r32, 1.0);
       KState z{};
       VFVM_HIP_CHECK(hipMemcpyAsync(st.p, &z, sizeof(KState), hipMemcpyHostToDevice, stream));
@@ -1107,9 +1109,9 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
       const double rnew = std::sqrt(h.rr);
       nref++;
       if (!(rnew < rn)) {
-        // the FP32 solve did not reduce the FP64 residual (stagnation or breakdown at
-        // this conditioning): take the correction back and finish in FP64
-        hipLaunchKernelGGL((k_add_correction<float>), dim3(g), dim3(VB), 0, stream, n, x, (const float *)r32, -1.0);
+        // the FP32 solve did not reduce the FP64 residual (stagnation, breakdown or an
+        // overflow at this conditioning): restore x and finish in FP64
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(g), dim3(VB), 0, stream, n, (const double *)xsave.p, x);
         VFVM_HIP_CHECK(hipMemcpyAsync(st.p, &z, sizeof(KState), hipMemcpyHostToDevice, stream));
         scaled_res(x, rh);
         read_state(h);

@@ -4,7 +4,9 @@
 // float32 accumulation); aggregation, activations, loss and Adam are plain
 // kernels.  The network (row r = sample s, node i of the system's grid):
 //
-//   x_r   : 8 input features, zero padded to 16
+//   x_r   : 8 input features; channels 8..15 are multi-scale context: the features
+//           a = x_1 (asinh(f/2)) and x_0 (f/5) smoothed by K = 16, 64, 160, 324 steps
+//           of y <- (y + W y) / 2 on the graph (Gaussian-like, width sqrt(K)/2 edges)
 //   h0    = relu(bf(x) We + be)                                   64 wide
 //   l = 0..L-1:
 //     a_l = sum_j w_ij h_l[j] - h_l[i]     w_ij = sigma_ij/h_ij normalised per row
@@ -249,13 +251,78 @@ __global__ void k_features_sides(int64_t rows, int64_t n, const double *__restri
   o[7] = xy[2 * i + 1];
 }
 
-// x (rows x 8 float) -> bfloat16 rows x 16 (zero padded)
-__global__ void k_pad_features(int64_t rows, const float *__restrict__ x, bf16_t *__restrict__ xb) {
+// network input (rows x 16 float) -> bfloat16
+__global__ void k_pad_features(int64_t rows, const float *__restrict__ x16, bf16_t *__restrict__ xb) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= rows * KPAD) return;
+  xb[q] = f2bf(x16[q]);
+}
+
+// ---------------------------------------------------------- context -------
+// Channels 8..15 of the network input: two features smoothed on the sample's graph.
+// One block per (sample, feature); the sample's field lives in LDS (two buffers of n
+// floats) for all steps, so only the graph is read from memory.
+constexpr int SMOOTH_MAXN = 20000;  // nodes per sample held in LDS (2 x 80 KB)
+constexpr int SMOOTH_TB = 1024;
+__device__ constexpr int SMOOTH_STEPS[4] = {16, 64, 160, 324};
+constexpr int SMOOTH_SRC[2] = {1, 0};  // asinh(f/2), f/5
+
+// x16[r, 0..7] = x[r, 0..7]
+__global__ void k_expand_copy(int64_t rows, const float *__restrict__ x, float *__restrict__ x16) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= rows * KPAD) return;
   const int64_t r = q / KPAD;
   const int k = (int)(q % KPAD);
-  xb[q] = k < NFEAT ? f2bf(x[r * NFEAT + k]) : (bf16_t)0;
+  if (k < NFEAT) x16[q] = x[r * NFEAT + k];
+}
+
+__global__ void __launch_bounds__(SMOOTH_TB) k_smooth_lds(int64_t n, const int32_t *__restrict__ gptr,
+                                                          const int32_t *__restrict__ gcol,
+                                                          const float *__restrict__ gw, float *__restrict__ x16) {
+  __shared__ float buf[2][SMOOTH_MAXN];
+  const int64_t s = blockIdx.x;
+  const int c = blockIdx.y;
+  const int src = c == 0 ? SMOOTH_SRC[0] : SMOOTH_SRC[1];
+  float *xs = x16 + s * n * KPAD;
+  for (int64_t i = threadIdx.x; i < n; i += SMOOTH_TB) buf[0][i] = xs[i * KPAD + src];
+  __syncthreads();
+  int cur = 0, snap = 0;
+  for (int step = 1; step <= SMOOTH_STEPS[3]; step++) {
+    for (int64_t i = threadIdx.x; i < n; i += SMOOTH_TB) {
+      float acc = 0.f;
+      for (int32_t k = gptr[i]; k < gptr[i + 1]; k++) acc += gw[k] * buf[cur][gcol[k]];
+      buf[1 - cur][i] = 0.5f * (buf[cur][i] + acc);
+    }
+    __syncthreads();
+    cur = 1 - cur;
+    if (step == SMOOTH_STEPS[snap]) {
+      for (int64_t i = threadIdx.x; i < n; i += SMOOTH_TB) xs[i * KPAD + NFEAT + 4 * c + snap] = buf[cur][i];
+      snap++;
+    }
+  }
+}
+
+// the same on graphs too large for LDS: one launch per step, ping-pong in memory
+__global__ void k_smooth_step(int64_t rows, int64_t n, const int32_t *__restrict__ gptr,
+                              const int32_t *__restrict__ gcol, const float *__restrict__ gw,
+                              const float *__restrict__ yin, float *__restrict__ yout) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const int64_t s = r / n, i = r - s * n;
+  const float *ys = yin + s * n;
+  float acc = 0.f;
+  for (int32_t k = gptr[i]; k < gptr[i + 1]; k++) acc += gw[k] * ys[gcol[k]];
+  yout[r] = 0.5f * (yin[r] + acc);
+}
+
+__global__ void k_channel(int64_t rows, const float *__restrict__ x16, int from, float *__restrict__ y, float *x16o,
+                          int to) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  if (x16o)
+    x16o[r * KPAD + to] = y[r];
+  else
+    y[r] = x16[r * KPAD + from];
 }
 
 // u_r = h_L[r] . wd + bd (one wave per row); optional loss: du = 2 (u - t) / count,
@@ -367,7 +434,7 @@ struct vfvm_mpnn {
   DevBuf<bf16_t> Web, Wb, WTb;  // encoder; layers (L x 128 x 64) and transposes (L x 64 x 128)
   // activations for a batch of `cap` rows
   int64_t cap = 0;
-  DevBuf<float> x, hf, rbuf, dh, da, u, target, part;
+  DevBuf<float> x, x16, hf, rbuf, dh, da, u, target, part, sm0, sm1;
   DevBuf<bf16_t> xb, cat, dzb;
   // device features of the Dirichlet-sides family (2D): per node x, y and the distances
   // to the bounding-box sides (south, east, north, west); per-sample inputs
@@ -412,6 +479,7 @@ void ensure_cap(vfvm_mpnn &m, int64_t rows) {
   if (rows <= m.cap) return;
   const int L = m.nlayers;
   m.x.alloc((size_t)rows * NFEAT);
+  m.x16.alloc((size_t)rows * KPAD);
   m.xb.alloc((size_t)rows * KPAD);
   m.hf.alloc((size_t)rows * WID);
   m.rbuf.alloc((size_t)rows * WID * L);
@@ -428,11 +496,44 @@ void ensure_cap(vfvm_mpnn &m, int64_t rows) {
   m.cap = rows;
 }
 
-// forward pass on `rows` rows already in m.x; keeps what the backward pass needs
+// network input of `ns` samples: x16 = [x (8 features), smoothed context (8 channels)]
+void expand(vfvm_mpnn &m, const float *x, float *x16, int64_t ns) {
+  hipStream_t st = m.ctx->stream;
+  const int64_t rows = ns * m.n;
+  hipLaunchKernelGGL(k_expand_copy, dim3(nb(rows * KPAD, 256)), dim3(256), 0, st, rows, x, x16);
+  if (m.n <= SMOOTH_MAXN) {
+    hipLaunchKernelGGL(k_smooth_lds, dim3((unsigned)ns, 2), dim3(SMOOTH_TB), 0, st, m.n, (const int32_t *)m.gptr.p,
+                       (const int32_t *)m.gcol.p, (const float *)m.gw.p, x16);
+  } else {
+    m.sm0.alloc((size_t)std::max<int64_t>(rows, (int64_t)m.sm0.n));
+    m.sm1.alloc((size_t)std::max<int64_t>(rows, (int64_t)m.sm1.n));
+    const int steps[4] = {16, 64, 160, 324};
+    for (int c = 0; c < 2; c++) {
+      hipLaunchKernelGGL(k_channel, dim3(nb(rows, 256)), dim3(256), 0, st, rows, (const float *)x16, SMOOTH_SRC[c],
+                         m.sm0.p, (float *)nullptr, 0);
+      float *a = m.sm0.p, *b = m.sm1.p;
+      int snap = 0;
+      for (int step = 1; step <= steps[3]; step++) {
+        hipLaunchKernelGGL(k_smooth_step, dim3(nb(rows, 256)), dim3(256), 0, st, rows, m.n, (const int32_t *)m.gptr.p,
+                           (const int32_t *)m.gcol.p, (const float *)m.gw.p, (const float *)a, b);
+        std::swap(a, b);
+        if (step == steps[snap]) {
+          hipLaunchKernelGGL(k_channel, dim3(nb(rows, 256)), dim3(256), 0, st, rows, (const float *)x16, 0, a, x16,
+                             NFEAT + 4 * c + snap);
+          snap++;
+        }
+      }
+    }
+  }
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+// forward pass on `rows` rows whose network input is in m.x16; keeps what the backward pass needs
 void forward(vfvm_mpnn &m, int64_t rows) {
   hipStream_t st = m.ctx->stream;
   const int L = m.nlayers;
-  hipLaunchKernelGGL(k_pad_features, dim3(nb(rows * KPAD, 256)), dim3(256), 0, st, rows, (const float *)m.x.p, m.xb.p);
+  hipLaunchKernelGGL(k_pad_features, dim3(nb(rows * KPAD, 256)), dim3(256), 0, st, rows, (const float *)m.x16.p,
+                     m.xb.p);
   GemmEpi e{};
   e.bias = m.theta.p + m.off_be;
   e.hf = m.hf.p;
@@ -590,7 +691,7 @@ int vfvm_mpnn_create(vfvm_sys *sys, int nlayers, int width, uint64_t seed, vfvm_
     std::mt19937_64 rng(seed);
     std::uniform_real_distribution<float> U(-1.f, 1.f);
     for (int k = 0; k < KPAD * WID; k++)
-      th[m->off_We + k] = (k / WID < NFEAT) ? U(rng) * std::sqrt(6.f / NFEAT) : 0.f;
+      th[m->off_We + k] = U(rng) * std::sqrt(6.f / KPAD);
     for (int l = 0; l < nlayers; l++)
       for (int k = 0; k < CW * WID; k++) th[m->off_W[l] + k] = U(rng) * std::sqrt(6.f / CW) * 0.5f;
     for (int k = 0; k < WID; k++) th[m->off_wd + k] = U(rng) * std::sqrt(3.f / WID);
@@ -644,12 +745,13 @@ int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, doubl
   if (!m || !features || !out || nsamples < 1) return VFVM_ERR_ARG;
   return mguard(m->ctx, [&]() {
     hipStream_t st = m->ctx->stream;
-    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsamples, (int64_t(1) << 20) / m->n));
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsamples, (int64_t(1) << 23) / m->n));
     std::vector<float> u;
     for (int64_t s0 = 0; s0 < nsamples; s0 += chunk) {
       const int64_t ns = std::min(chunk, nsamples - s0), rows = ns * m->n;
       ensure_cap(*m, rows);
       m->x.upload(features + s0 * m->n * NFEAT, (size_t)rows * NFEAT, st);
+      expand(*m, m->x.p, m->x16.p, ns);
       forward(*m, rows);
       hipLaunchKernelGGL(k_decode, dim3(nb(rows, 4)), dim3(TB), 0, st, rows, (const float *)m->hf.p,
                          (const float *)m->theta.p + m->off_wd, (const float *)m->theta.p + m->off_bd, m->u.p,
@@ -671,7 +773,7 @@ int vfvm_mpnn_infer_sides(vfvm_mpnn *m, int64_t nsamples, const double *f, const
   return mguard(m->ctx, [&]() {
     if (!m->xy.p) throw Error(VFVM_ERR_ARG, "side features need a 2D system");
     hipStream_t st = m->ctx->stream;
-    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsamples, (int64_t(1) << 20) / m->n));
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsamples, (int64_t(1) << 23) / m->n));
     const double inv_delta = 1.0 / std::sqrt(eps / 2.0);
     std::vector<float> u;
     for (int64_t s0 = 0; s0 < nsamples; s0 += chunk) {
@@ -682,6 +784,7 @@ int vfvm_mpnn_infer_sides(vfvm_mpnn *m, int64_t nsamples, const double *f, const
       hipLaunchKernelGGL(k_features_sides, dim3(nb(rows, 256)), dim3(256), 0, st, rows, m->n,
                          (const double *)m->fin.p, (const double *)m->sides.p, (const float *)m->xy.p,
                          (const float *)m->dist.p, inv_delta, m->x.p);
+      expand(*m, m->x.p, m->x16.p, ns);
       forward(*m, rows);
       hipLaunchKernelGGL(k_decode, dim3(nb(rows, 4)), dim3(TB), 0, st, rows, (const float *)m->hf.p,
                          (const float *)m->theta.p + m->off_wd, (const float *)m->theta.p + m->off_bd, m->u.p,
@@ -737,6 +840,7 @@ int vfvm_mpnn_loss_grad(vfvm_mpnn *m, int64_t nsamples, const float *features, c
     ensure_cap(*m, rows);
     m->x.upload(features, (size_t)rows * NFEAT, st);
     m->target.upload(targets, (size_t)rows, st);
+    expand(*m, m->x.p, m->x16.p, nsamples);
     forward(*m, rows);
     const float lv = backward(*m, rows);
     if (loss) *loss = lv;
@@ -754,9 +858,14 @@ int vfvm_mpnn_train(vfvm_mpnn *m, int64_t nsamples, const float *features, const
   return mguard(m->ctx, [&]() {
     hipStream_t st = m->ctx->stream;
     const int64_t n = m->n;
-    // the whole training set stays resident on the device
-    DevBuf<float> X, T;
+    // the whole training set (with its smoothed context channels) stays resident on the device
+    DevBuf<float> X, X16, T;
     X.upload(features, (size_t)nsamples * n * NFEAT, st);
+    X16.alloc((size_t)nsamples * n * KPAD);
+    const int64_t ech = std::max<int64_t>(1, (int64_t(1) << 22) / n);
+    for (int64_t s0 = 0; s0 < nsamples; s0 += ech)
+      expand(*m, X.p + s0 * n * NFEAT, X16.p + s0 * n * KPAD, std::min(ech, nsamples - s0));
+    X.release();
     T.upload(targets, (size_t)nsamples * n, st);
     const int64_t B = std::min(batch, nsamples);
     ensure_cap(*m, B * n);
@@ -771,7 +880,7 @@ int vfvm_mpnn_train(vfvm_mpnn *m, int64_t nsamples, const float *features, const
       for (int64_t b0 = 0; b0 + B <= nsamples; b0 += B) {
         for (int64_t q = 0; q < B; q++) {
           const int64_t sidx = order[b0 + q];
-          VFVM_HIP_CHECK(hipMemcpyAsync(m->x.p + q * n * NFEAT, X.p + sidx * n * NFEAT, sizeof(float) * n * NFEAT,
+          VFVM_HIP_CHECK(hipMemcpyAsync(m->x16.p + q * n * KPAD, X16.p + sidx * n * KPAD, sizeof(float) * n * KPAD,
                                         hipMemcpyDeviceToDevice, st));
           VFVM_HIP_CHECK(hipMemcpyAsync(m->target.p + q * n, T.p + sidx * n, sizeof(float) * n,
                                         hipMemcpyDeviceToDevice, st));
