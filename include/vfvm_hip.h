@@ -184,6 +184,34 @@ int vfvm_solve_fixed_steps(vfvm_sys *sys, const double *inival, int64_t nlambdas
                            const double *lambdas, int transient, const vfvm_control_t *control,
                            double *U_final, vfvm_history_t *hist);
 
+/* Step-size control of solve_transient! (src/vfvm_solver.jl:267-537); for embedding
+ * the Δp fields of SolverControl, for time stepping the Δt fields
+ * (src/vfvm_solvercontrol.jl:138-211).  vfvm_default_stepcontrol gives the reference's
+ * defaults (transient != 0: Δt = 0.1, Δt_min = 1e-3, Δt_max = 1, grow 1.2; embedding:
+ * Δp = 1, Δp_min = 1e-3, Δp_max = 1, grow 1; decrease 0.5, Δu_opt = 0.1,
+ * Δu_max_factor = 1.2, num_final_steps = 5).  fixed_timesteps! corresponds to
+ * d = d_min = d_max and du_opt = DBL_MAX. */
+typedef struct {
+  double d, d_min, d_max, d_grow, d_decrease;
+  double du_opt, du_max_factor;
+  int32_t num_final_steps, force_first_step;
+} vfvm_stepcontrol_t;
+void vfvm_default_stepcontrol(int transient, vfvm_stepcontrol_t *sc);
+
+/* solve_transient! with its step-size control, on the device Newton loop: Δu =
+ * ||U - Uold||_inf; a step with Δu > du_max_factor * du_opt is retried with the step
+ * times d_decrease down to d_min, where the call fails (VFVM_ERR_CONVERGENCE) unless the
+ * first step is forced; a failing Newton solve fails the call (handle_exceptions =
+ * false).  Accepted steps grow as min(d_max, d d_grow, d du_opt / Δu, ...) with the
+ * num_final_steps rule.  Outputs: U_final; up to max_out entries (lambda, solution) in
+ * out_lambdas / out_U (max_out x ndof; both nullable): entry 0 is the start (the initial
+ * value for time stepping, the solution at lambdas[0] for embedding), then one entry per
+ * accepted step, as the reference's TransientSolution; *nsteps = accepted steps. */
+int vfvm_solve_transient(vfvm_sys *sys, const double *inival, int64_t nlambdas, const double *lambdas, int transient,
+                         const vfvm_control_t *control, const vfvm_stepcontrol_t *steps, double *U_final,
+                         int64_t max_out, double *out_lambdas, double *out_U, int64_t *nsteps,
+                         vfvm_history_t *hist);
+
 /* ||F||_inf over non-penalty rows for U (residual-only assembly) */
 int vfvm_residual_norm(vfvm_sys *sys, const double *U, const double *Uold, double time,
                        double tstep, double lambda, double *norm);

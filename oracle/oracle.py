@@ -83,7 +83,7 @@ def lib():
         L.orc_solve_step.argtypes = [C.c_void_p, dp, dp, C.POINTER(Control), C.c_double, C.c_double,
                                      C.c_double, C.POINTER(History)]
         L.orc_solve_transient.argtypes = [C.c_void_p, dp, C.c_int, dp, C.c_int, C.POINTER(Control),
-                                          C.c_double, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+                                          C.c_double, dp, C.POINTER(C.c_int), C.POINTER(C.c_int), dp, C.c_int]
         L.orc_linsolve.argtypes = [C.c_void_p, C.POINTER(Control), dp, dp, C.POINTER(C.c_int)]
         L.orc_default_control.argtypes = [C.POINTER(Control)]
         L.orc_bernoulli.restype = C.c_double
@@ -223,14 +223,21 @@ class OracleSystem:
         self._check(lib().orc_solve_step(self._h, U, Uold, C.byref(control), time, tstep, embed, C.byref(h)))
         return U, h
 
-    def solve_transient(self, inival, lambdas, control=None, transient=True, time=0.0):
+    def solve_transient(self, inival, lambdas, control=None, transient=True, time=0.0, return_steps=False,
+                        max_steps=100000):
+        """solve_transient! (step control of src/vfvm_solver.jl:267-537).  Returns the
+        final solution, the number of accepted steps, the Newton iterations in total
+        [and the accepted times / parameters]."""
         control = control or globals()["control"]()
         inival = np.ascontiguousarray(np.asarray(inival, dtype=np.float64).ravel())
         lam = np.ascontiguousarray(np.asarray(lambdas, dtype=np.float64))
         out = np.zeros(self.ndof)
+        steps = np.zeros(max_steps)
         ns, nn = C.c_int(0), C.c_int(0)
         self._check(lib().orc_solve_transient(self._h, inival, lam.size, lam, int(transient), C.byref(control),
-                                              time, out, C.byref(ns), C.byref(nn)))
+                                              time, out, C.byref(ns), C.byref(nn), steps, max_steps))
+        if return_steps:
+            return out, ns.value, nn.value, steps[:min(ns.value, max_steps)].copy()
         return out, ns.value, nn.value
 
     def linsolve(self, b, control=None):

@@ -1466,7 +1466,8 @@ int orc_solve_step(orc_sys *s, double *U, const double *Uold, const orc_control 
 
 /* solve_transient!, src/vfvm_solver.jl:267-537 (handle_exceptions = false) */
 int orc_solve_transient(orc_sys *s, const double *inival, int nl, const double *lambdas, int transient,
-                        const orc_control *c, double time, double *U_final, int *nsteps, int *newton_total) {
+                        const orc_control *c, double time, double *U_final, int *nsteps, int *newton_total,
+                        double *steps_out, int max_steps) {
   int ndof = s->nspec * s->nnodes;
   double dl, dl_min, dl_max, dl_grow;
   if (transient) {
@@ -1498,35 +1499,42 @@ int orc_solve_transient(orc_sys *s, const double *inival, int nl, const double *
     dl = fmax(dl, dl_min);
     double lend = lambdas[i + 1], l = lambdas[i];
     while (l < lend) {
-      double l0 = l, du = 0.0;
-      l = l0 + dl;
-      double t_, ts_, ep_;
-      if (transient) {
-        t_ = l;
-        ts_ = dl;
-        ep_ = 0.0;
-      } else {
-        t_ = time;
-        ts_ = INFINITY;
-        ep_ = l;
+      const double l0 = l;
+      double du = 0.0;
+      /* src/vfvm_solver.jl:359-455: try the step; a step whose update exceeds
+         Δu_max_factor Δu_opt is retried with Δλ Δλ_decrease until Δλ ≈ Δλ_min, where
+         the run stops (handle_exceptions = false) unless the first step is forced.
+         A Newton failure stops the run (handle_exceptions = false throws). Every
+         retry restarts Newton from the old solution (solve_step! copies oldsol). */
+      for (;;) {
+        l = l0 + dl;
+        double t_, ts_, ep_;
+        if (transient) {
+          t_ = l;
+          ts_ = dl;
+          ep_ = 0.0;
+        } else {
+          t_ = time;
+          ts_ = INFINITY;
+          ep_ = l;
+        }
+        rc = orc_solve_step(s, tmp, old, c, t_, ts_, ep_, &h);
+        nnewton += h.niter;
+        if (rc) goto done;
+        memcpy(sol, tmp, sizeof(double) * (size_t)ndof);
+        du = 0.0;
+        for (int k = 0; k < ndof; k++) du = fmax(du, fabs(sol[k] - old[k]));
+        if (!(du > c->du_max_factor * c->du_opt)) break;
+        /* Δλ ≈ Δλ_min (isapprox, rtol sqrt(eps)) */
+        if (fabs(dl - dl_min) <= 1.4901161193847656e-08 * fmax(fabs(dl), fabs(dl_min))) {
+          if (c->force_first_step && istep == 0) break;
+          set_err(s, "step rejected at minimal step size (du=%g)", du);
+          rc = ORC_ERR_CONVERGENCE;
+          goto done;
+        }
+        dl = fmax(dl_min, dl * (transient ? c->dt_decrease : c->dp_decrease));
       }
-      /* Newton starts from `solution` and uses `oldsolution` as Uold; with
-         handle_exceptions=false and a step-size rejection we stop (no retry
-         is needed by the fixed-step cases this restatement serves) */
-      rc = orc_solve_step(s, tmp, old, c, t_, ts_, ep_, &h);
-      nnewton += h.niter;
-      if (rc) goto done;
-      memcpy(sol, tmp, sizeof(double) * (size_t)ndof);
-      du = 0.0;
-      for (int k = 0; k < ndof; k++) du = fmax(du, fabs(sol[k] - old[k]));
-      if (du > c->du_max_factor * c->du_opt) {
-        /* rejected step: with handle_exceptions = false the reference either
-           throws (Δλ ≈ Δλ_min) or retries with a smaller step; the retry is not
-           restated, so both end the run here */
-        set_err(s, "time/embedding step rejected (du=%g)", du);
-        rc = ORC_ERR_CONVERGENCE;
-        goto done;
-      }
+      if (steps_out && istep < max_steps) steps_out[istep] = l;
       istep++;
       memcpy(old, sol, sizeof(double) * (size_t)ndof);
       double steps_to_go = ceil((lend - l) / dl);

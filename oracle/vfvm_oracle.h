@@ -124,10 +124,11 @@ int orc_solve_step(orc_sys *s, double *U, const double *Uold, const orc_control 
                    double time, double tstep, double lambda, orc_history *h);
 
 /* a4: solve_transient!  transient=1: lambdas are times; 0: embedding.
-   Final solution in U_final; number of accepted steps in *nsteps. */
+   Final solution in U_final; number of accepted steps in *nsteps; the accepted
+   times / parameters in steps_out (up to max_steps; may be NULL). */
 int orc_solve_transient(orc_sys *s, const double *inival, int nlambdas, const double *lambdas,
                         int transient, const orc_control *c, double time, double *U_final,
-                        int *nsteps, int *newton_total);
+                        int *nsteps, int *newton_total, double *steps_out, int max_steps);
 
 /* linear solve on the currently assembled matrix: x = A^{-1} b */
 int orc_linsolve(orc_sys *s, const orc_control *c, const double *b, double *x, int *iters);

@@ -152,8 +152,9 @@ def test_block_jacobi_gmres_drift_diffusion_continuation():
     Uo, _, _ = osys.solve_transient(vfvm.dofs(U0), lam, c, transient=False)
     for precision in ("fp64", "fp32"):
         ctrl = vfvm.SolverControl(krylov="gmres", precon="blockjacobi", precision=precision, reltol_linear=1e-10,
-                                  abstol_linear=0.0, maxiters_linear=3000, gmres_restart=30)
-        U = vfvm.solve(sys, inival=U0, control=ctrl, embed=lam)
+                                  abstol_linear=0.0, maxiters_linear=3000, gmres_restart=30, Δp=0.2, Δp_min=0.2,
+                                  Δp_max=0.2, Δu_opt=10.0)
+        U = vfvm.solve(sys, inival=U0, control=ctrl, embed=lam).u[-1]
         err = np.abs(vfvm.dofs(U) - Uo).max()
         _print("dd", precision, "err", err)
         assert err <= SOL_ATOL
@@ -177,7 +178,7 @@ def test_ilu0_bicgstab_drift_diffusion_three_species():
                                   abstol_linear=0.0, maxiters_linear=3000, max_refinements=20)
         U = U0.reshape(3, -1, order="F")
         for l, Ur in zip(lam, ref):
-            U = vfvm.solve(sys, inival=U, control=ctrl, embed=[l])
+            U = vfvm.solve(sys, inival=U, control=ctrl, embed=[l]).u[-1]
             err = np.abs(vfvm.dofs(U) - Ur).max()
             _print("dd ilu0", precision, l, "err", err, "newton", sys.history.niter, "krylov", sys.history.nlin)
             assert err <= SOL_ATOL
@@ -195,9 +196,9 @@ def test_transient_nonlinear_diffusion_3d():
     for k in range(1, times.size):
         Uo, _ = osys.solve_step(Uo, time=times[k], tstep=times[k] - times[k - 1])
     for precision in ("fp64", "fp32"):
-        ctrl = vfvm.SolverControl(precon="ilu0", precision=precision, reltol_linear=1e-10, abstol_linear=0.0,
-                                  maxiters_linear=2000)
-        U = vfvm.solve(sys, inival=np.asfortranarray(u0), control=ctrl, times=times)
+        ctrl = vfvm.fixed_timesteps(vfvm.SolverControl(precon="ilu0", precision=precision, reltol_linear=1e-10,
+                                                       abstol_linear=0.0, maxiters_linear=2000), 1e-2)
+        U = vfvm.solve(sys, inival=np.asfortranarray(u0), control=ctrl, times=times).u[-1]
         assert np.abs(vfvm.dofs(U) - Uo).max() <= SOL_ATOL
 
 
@@ -238,3 +239,37 @@ def test_maxiters_raises_convergence_error():
     sys, _ = cases.nonlinear_poisson_2d(21)
     with pytest.raises(vfvm.ConvergenceError):
         vfvm.solve(sys, inival=0.0, control=vfvm.SolverControl(maxiters=1))
+
+
+def test_adaptive_time_stepping_matches_oracle():
+    """solve_transient! step control on the device Newton loop against the oracle: the
+    same accepted times (rejections and growth included) and the same solutions."""
+    _print("case adaptive steps decay")
+    X = linspace(0.0, 1.0, 6)
+    g = simplexgrid(X)
+    sys, osys = cases.pair(g, 1, dict(flux=vfvm.LinearFlux(0.0), reaction=vfvm.PowerReaction(1, 5.0),
+                                      storage=vfvm.IdentityStorage()))
+    ctrl = vfvm.SolverControl(reltol_linear=1e-12, abstol_linear=0.0)
+    tsol = vfvm.solve(sys, inival=1.0, control=ctrl, times=[0.0, 1.0])
+    Uo, n, _, steps = osys.solve_transient(np.ones(g.num_nodes), [0.0, 1.0], O.control(), return_steps=True)
+    assert len(tsol.t) == n + 1 and tsol.t[0] == 0.0
+    assert np.abs(np.asarray(tsol.t[1:]) - steps).max() <= 1e-14
+    assert np.abs(vfvm.dofs(tsol.u[-1]) - Uo).max() <= SOL_ATOL
+    _print("case adaptive steps nldiff 2d")
+    X = linspace(0.0, 1.0, 17)
+    g = simplexgrid(X, X)
+    x, y = g.coord
+    u0 = np.exp(-30.0 * ((x - 0.5) ** 2 + (y - 0.5) ** 2))
+    bcs = [vfvm.Dirichlet(species=1, region=r, value=0.0) for r in (1, 2, 3, 4)]
+    sys, osys = cases.pair(g, 1, dict(flux=vfvm.NonlinearDiffusionFlux(1e-2, 1.0), storage=vfvm.IdentityStorage()),
+                           bcs)
+    for precision in ("fp64", "fp32"):
+        ctrl = vfvm.SolverControl(precon="ilu0", precision=precision, reltol_linear=1e-12, abstol_linear=0.0,
+                                  maxiters_linear=2000, Δt=1e-3, Δt_min=1e-5, Δt_max=0.5, Δu_opt=0.05)
+        tsol = vfvm.solve(sys, inival=u0.reshape(1, -1), control=ctrl, times=[0.0, 0.2, 1.0])
+        oc = O.control(dt=1e-3, dt_min=1e-5, dt_max=0.5, du_opt=0.05)
+        Uo, n, _, steps = osys.solve_transient(u0, [0.0, 0.2, 1.0], oc, return_steps=True)
+        _print("adaptive", precision, "steps", n, "gpu", len(tsol.t) - 1)
+        assert len(tsol.t) == n + 1
+        assert np.abs(np.asarray(tsol.t[1:]) - steps).max() <= 1e-12
+        assert np.abs(vfvm.dofs(tsol.u[-1]) - Uo).max() <= SOL_ATOL

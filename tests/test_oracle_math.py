@@ -169,3 +169,40 @@ def test_direct_solver_matches_scipy():
     x, _ = s.linsolve(F)
     y = sla.spsolve(s.matrix_scipy().tocsc(), F)
     assert np.abs(x - y).max() <= 1e-10 * np.abs(y).max()
+
+
+def test_step_control_on_exact_decay():
+    """solve_transient! step control (src/vfvm_solver.jl:359-500) on u' = -k u at every
+    node (no flux, reaction k u, storage u): implicit Euler gives u_n = prod 1/(1 + k dt_i)
+    over the accepted steps, so the accepted sequence the oracle reports must reproduce
+    the final value; with the reference's defaults the first step (0.1) is rejected
+    twice (du = 1/3, 1/5 > 1.2 * 0.1) and accepted at 0.025; accepted updates stay below
+    du_max_factor * du_opt, growth stays below dt_grow, and the run ends at t = 1."""
+    X = linspace(0.0, 1.0, 6)
+    g = simplexgrid(X)
+    k = 5.0
+    s = O.OracleSystem(g, 1, O.physics("linear", "pow", "identity", D0=0.0, k=k, rm=1), {}, O.EDGEWISE)
+    c = O.control()
+    U, nsteps, _, steps = s.solve_transient(np.ones(g.num_nodes), [0.0, 1.0], c, return_steps=True)
+    assert steps[-1] == 1.0
+    dts = np.diff(np.concatenate([[0.0], steps]))
+    assert dts[0] == pytest.approx(0.025, rel=1e-15)
+    u = np.cumprod(1.0 / (1.0 + k * dts))
+    assert np.abs(U - u[-1]).max() <= 1e-13
+    du = np.abs(np.diff(np.concatenate([[1.0], u])))
+    assert (du <= 1.2 * 0.1 + 1e-15).all()
+    assert (dts[1:] <= 1.2 * dts[:-1] * (1 + 1e-12)).all()
+    assert (dts >= 1e-3 * (1 - 1e-12)).all()
+
+
+def test_fixed_timesteps_give_exact_sequence():
+    """fixed_timesteps! (dt_min = dt_max = dt, du_opt = floatmax): exactly (t1 - t0) / dt
+    steps of dt, whatever the update size."""
+    X = linspace(0.0, 1.0, 6)
+    g = simplexgrid(X)
+    s = O.OracleSystem(g, 1, O.physics("linear", "pow", "identity", D0=0.0, k=50.0, rm=1), {}, O.EDGEWISE)
+    c = O.control(dt=0.05, dt_min=0.05, dt_max=0.05, dt_grow=1.0, du_opt=np.finfo(float).max)
+    U, nsteps, _, steps = s.solve_transient(np.ones(g.num_nodes), [0.0, 0.5], c, return_steps=True)
+    assert nsteps == 10
+    assert np.allclose(np.diff(np.concatenate([[0.0], steps])), 0.05, rtol=1e-12)
+    assert np.abs(U - (1.0 / 3.5) ** 10).max() <= 1e-13

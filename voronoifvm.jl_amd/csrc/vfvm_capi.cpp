@@ -752,6 +752,131 @@ int vfvm_solve_fixed_steps(vfvm_sys *s, const double *inival, int64_t nl, const 
   });
 }
 
+void vfvm_default_stepcontrol(int transient, vfvm_stepcontrol_t *sc) {
+  // src/vfvm_solvercontrol.jl:138-211
+  if (!sc) return;
+  if (transient) {
+    sc->d = 0.1;
+    sc->d_min = 1.0e-3;
+    sc->d_max = 1.0;
+    sc->d_grow = 1.2;
+  } else {
+    sc->d = 1.0;
+    sc->d_min = 1.0e-3;
+    sc->d_max = 1.0;
+    sc->d_grow = 1.0;
+  }
+  sc->d_decrease = 0.5;
+  sc->du_opt = 0.1;
+  sc->du_max_factor = 1.2;
+  sc->num_final_steps = 5;
+  sc->force_first_step = 0;
+}
+
+int vfvm_solve_transient(vfvm_sys *s, const double *inival, int64_t nl, const double *lambdas, int transient,
+                         const vfvm_control_t *control, const vfvm_stepcontrol_t *steps, double *U_final,
+                         int64_t max_out, double *out_lambdas, double *out_U, int64_t *nsteps,
+                         vfvm_history_t *hist) {
+  if (!s || !inival || !lambdas || nl < 1 || !steps || max_out < 0) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    vfvm_control_t c;
+    if (control)
+      c = *control;
+    else
+      vfvm_default_control(&c);
+    check_control(c, *s);
+    const vfvm_stepcontrol_t sc = *steps;
+    if (!(sc.d_min > 0) || !(sc.d_max >= sc.d_min) || !(sc.d_decrease > 0 && sc.d_decrease < 1))
+      throw Error(VFVM_ERR_ARG, "bad step control");
+    if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
+    hipStream_t st = s->ctx->stream;
+    auto t0 = std::chrono::steady_clock::now();
+    s->d_Uold.upload(inival, (size_t)s->ndof, st);
+    vfvm_history_t tot{};
+    tot.converged = 1;
+    auto account = [&](const vfvm_history_t &h) {
+      tot.niter += h.niter;
+      tot.nlin += h.nlin;
+      tot.residual_norm = std::max(tot.residual_norm, h.residual_norm);
+      tot.last_update_norm = h.last_update_norm;
+    };
+    int64_t istep = 0, nout = 0;
+    int rc = VFVM_OK;
+    vfvm_history_t h{};
+    // entry k of the outputs: (lambda, solution); entry 0 is the start (src/vfvm_solver.jl:304-337)
+    auto record = [&](double lam, const DevBuf<double> &u) {
+      if (nout < max_out) {
+        if (out_lambdas) out_lambdas[nout] = lam;
+        if (out_U) u.download(out_U + nout * s->ndof, (size_t)s->ndof, st);
+      }
+      nout++;
+    };
+    if (!transient) {
+      // solve for the first parameter value first; the start entry is that solution
+      rc = newton_device(*s, 0.0, INFINITY, lambdas[0], c, &h);
+      account(h);
+      if (!rc) {
+        launch_copy(s->ndof, s->d_U.p, s->d_Uold.p, st);
+        record(lambdas[0], s->d_Uold);
+      }
+    } else {
+      record(lambdas[0], s->d_Uold);  // the initial value as given
+      launch_init_dirichlet((int64_t)s->hs.bnodes.size(), s->d_bnodes.p, s->d_bn_ptr.p, s->d_bn_reg.p, nullptr,
+                            s->d_inst.p, s->nspec, lambdas[0], s->d_Uold.p, st);
+    }
+    double dl = sc.d;
+    auto approx = [](double a, double b) { return std::fabs(a - b) <= 1.4901161193847656e-08 * std::max(std::fabs(a), std::fabs(b)); };
+    for (int64_t i = 0; i + 1 < nl && rc == VFVM_OK; i++) {
+      dl = std::max(dl, sc.d_min);
+      const double lend = lambdas[i + 1];
+      double l = lambdas[i];
+      while (l < lend && rc == VFVM_OK) {
+        const double l0 = l;
+        double du = 0.0;
+        for (;;) {
+          l = l0 + dl;
+          rc = transient ? newton_device(*s, l, dl, 0.0, c, &h) : newton_device(*s, 0.0, INFINITY, l, c, &h);
+          account(h);
+          if (rc) break;  // handle_exceptions = false: a failing Newton solve ends the run
+          launch_maxdiff(s->ndof, s->d_U.p, s->d_Uold.p, s->d_partials.p, s->d_red.p + 4, st);
+          VFVM_HIP_CHECK(hipMemcpyAsync(s->h_pinned, s->d_red.p + 4, sizeof(double), hipMemcpyDeviceToHost, st));
+          VFVM_HIP_CHECK(hipStreamSynchronize(st));
+          du = s->h_pinned[0];
+          if (!(du > sc.du_max_factor * sc.du_opt)) break;
+          if (approx(dl, sc.d_min)) {
+            if (sc.force_first_step && istep == 0) break;
+            s->ctx->err = "step size minimum reached while du/du_opt = " + std::to_string(du / sc.du_opt);
+            rc = VFVM_ERR_CONVERGENCE;
+            break;
+          }
+          dl = std::max(sc.d_min, dl * sc.d_decrease);
+        }
+        if (rc) break;
+        record(l, s->d_U);
+        istep++;
+        launch_copy(s->ndof, s->d_U.p, s->d_Uold.p, st);
+        // src/vfvm_solver.jl:478-500: next step, avoiding a tiny last one
+        const double steps_to_go = std::ceil((lend - l) / dl);
+        double l_predict = lend - l;
+        if (steps_to_go < sc.num_final_steps) l_predict = (lend - l) / steps_to_go;
+        if (approx(sc.d_max, sc.d_min)) l_predict = sc.d_max;
+        if (l < lend) {
+          dl = std::min({sc.d_max, dl * sc.d_grow, dl * sc.du_opt / (du + 1.0e-14), l_predict, lend - l});
+          if (std::fabs((l + dl) - lend) <= std::max(1.0e-15, 1.0e-15 * std::max(std::fabs(l + dl), std::fabs(lend))))
+            dl = lend - l;
+        }
+      }
+    }
+    if (U_final) s->d_Uold.download(U_final, (size_t)s->ndof, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    if (nsteps) *nsteps = istep;
+    tot.converged = rc == VFVM_OK;
+    tot.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (hist) *hist = tot;
+    return rc;
+  });
+}
+
 int vfvm_residual_norm(vfvm_sys *s, const double *U, const double *Uold, double time, double tstep, double lambda,
                        double *norm) {
   if (!s || !U || !norm) return VFVM_ERR_ARG;

@@ -178,6 +178,8 @@ void launch_newton_update(int64_t n, double *U, const double *dU, double damp, d
                           double *out, hipStream_t s);
 void launch_norm1(int64_t n, const double *U, double *partials, unsigned *ticket, double *out, hipStream_t s);
 void launch_gather(int64_t n, const int32_t *perm, const double *J, double *out, hipStream_t s);
+// out[0] = max |a - b| (out[1] is overwritten)
+void launch_maxdiff(int64_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s);
 void launch_fill_u64(int64_t n, unsigned long long *p, hipStream_t s);
 void launch_fill_i32(int64_t n, int32_t *p, int32_t v, hipStream_t s);
 void launch_copy(int64_t n, const double *a, double *b, hipStream_t s);

@@ -675,6 +675,25 @@ __global__ void __launch_bounds__(VB) k_norm1(int64_t n, const double *__restric
   if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
+// max |a - b| (the step-size control's update norm); partials [0 | max] for k_newton_finalize
+__global__ void __launch_bounds__(VB) k_maxdiff(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                                                double *partials) {
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB)
+    m = nanmax(m, fabs(a[i] - b[i]));
+  __shared__ double shm[16];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = nanmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) bm = nanmax(bm, shm[k]);
+    partials[blockIdx.x] = 0.0;
+    partials[(size_t)gridDim.x + blockIdx.x] = bm;
+  }
+}
+
 // CSC gather: out[k] = J[perm[k]]
 __global__ void __launch_bounds__(VB) k_gather(int64_t n, const int32_t *__restrict__ perm, const double *__restrict__ J,
                                                double *__restrict__ out) {
@@ -1181,6 +1200,12 @@ void launch_norm1(int64_t n, const double *U, double *partials, unsigned *ticket
   hipLaunchKernelGGL(k_norm1, dim3(g), dim3(VB), 0, s, n, U, partials, ticket, out);
   hipLaunchKernelGGL(k_newton_finalize, dim3(1), dim3(VB), 0, s, (const double *)partials, (int)g, (double *)nullptr,
                      out);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+void launch_maxdiff(int64_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s) {
+  const unsigned g = grid_for(n);
+  hipLaunchKernelGGL(k_maxdiff, dim3(g), dim3(VB), 0, s, n, a, b, partials);
+  hipLaunchKernelGGL(k_newton_finalize, dim3(1), dim3(VB), 0, s, (const double *)partials, (int)g, out, out + 1);
   VFVM_HIP_CHECK(hipGetLastError());
 }
 void launch_gather(int64_t n, const int32_t *perm, const double *J, double *out, hipStream_t s) {

@@ -4,12 +4,12 @@ Host-side mirror of VoronoiFVM.jl's System / Physics / solve surface; the work
 is done by libvfvm_hip.so (HIP kernels behind the C ABI in include/vfvm_hip.h).
 """
 from ._capi import AssemblyError, ConvergenceError, LinearSolverError, VFVMError
-from .control import SolverControl
+from .control import SolverControl, fixed_timesteps
 from .grids import Grid, linspace, simplexgrid
 from .physics import (Dirichlet, DriftDiffusionFlux, DriftDiffusionReaction, ExchangeReaction, IdentityStorage,
                       LinearFlux, Neumann, NonlinearDiffusionFlux, Physics, PowerFlux, PowerReaction, Robin,
                       SinhReaction)
-from .system import (Batch, NewtonSolverHistory, System, boundary_dirichlet, boundary_neumann, boundary_robin, dofs,
-                     enable_species, evaluate_residual_and_jacobian, integrate, solve, unknowns)
+from .system import (Batch, NewtonSolverHistory, System, TransientSolution, boundary_dirichlet, boundary_neumann,
+                     boundary_robin, dofs, enable_species, evaluate_residual_and_jacobian, integrate, solve, unknowns)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

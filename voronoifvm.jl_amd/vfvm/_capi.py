@@ -43,6 +43,12 @@ class HistoryT(C.Structure):
                 ("last_update_norm", C.c_double), ("residual_norm", C.c_double), ("updatenorm", C.c_double * 64)]
 
 
+class StepControlT(C.Structure):
+    _fields_ = [("d", C.c_double), ("d_min", C.c_double), ("d_max", C.c_double), ("d_grow", C.c_double),
+                ("d_decrease", C.c_double), ("du_opt", C.c_double), ("du_max_factor", C.c_double),
+                ("num_final_steps", C.c_int32), ("force_first_step", C.c_int32)]
+
+
 class LinStatsT(C.Structure):
     _fields_ = [("iters", C.c_int32), ("refinements", C.c_int32), ("converged", C.c_int32),
                 ("fp64_fallback", C.c_int32),
@@ -71,6 +77,10 @@ SIGNATURES = {
                               C.POINTER(HistoryT)]),
     "vfvm_solve_fixed_steps": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int, C.POINTER(ControlT), _dp,
                                          C.POINTER(HistoryT)]),
+    "vfvm_default_stepcontrol": (None, [C.c_int, C.POINTER(StepControlT)]),
+    "vfvm_solve_transient": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int, C.POINTER(ControlT),
+                                       C.POINTER(StepControlT), _dp, C.c_int64, _dp, _dp, C.POINTER(C.c_int64),
+                                       C.POINTER(HistoryT)]),
     "vfvm_residual_norm": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
     "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_set_groups": (C.c_int, [_vp, C.c_int32, C.c_int32]),

@@ -32,13 +32,21 @@ class SolverControl:
     inner_reltol: float = 1.0e-3
     reuse_precon: bool = False
     check_every: int = 4
-    # transient / embedding (src/vfvm_solvercontrol.jl:138-236); fixed steps only on the device
-    Δt: float = 0.1
-    Δt_min: float = 1.0e-3
-    Δt_max: float = 1.0
+    # time stepping / embedding step control (src/vfvm_solvercontrol.jl:138-211)
     Δp: float = 1.0
-    Δp_min: float = 1.0e-3
     Δp_max: float = 1.0
+    Δp_min: float = 1.0e-3
+    Δp_grow: float = 1.0
+    Δp_decrease: float = 0.5
+    Δt: float = 0.1
+    Δt_max: float = 1.0
+    Δt_min: float = 1.0e-3
+    Δt_grow: float = 1.2
+    Δt_decrease: float = 0.5
+    Δu_opt: float = 0.1
+    Δu_max_factor: float = 1.2
+    force_first_step: bool = False
+    num_final_steps: int = 5
     verbose: bool = False
 
     def update(self, **kw):
@@ -64,3 +72,31 @@ class SolverControl:
         c.precon = _PRECON[self.precon]
         c.precision = _PRECISION[self.precision]
         return c
+
+    def to_step_c(self, transient: bool):
+        """vfvm_stepcontrol_t from the Δt fields (time stepping) or the Δp fields (embedding)."""
+        from ._capi import StepControlT
+        sc = StepControlT()
+        pre = "Δt" if transient else "Δp"
+        sc.d = float(getattr(self, pre))
+        sc.d_min = float(getattr(self, pre + "_min"))
+        sc.d_max = float(getattr(self, pre + "_max"))
+        sc.d_grow = float(getattr(self, pre + "_grow"))
+        sc.d_decrease = float(getattr(self, pre + "_decrease"))
+        sc.du_opt = float(self.Δu_opt)
+        sc.du_max_factor = float(self.Δu_max_factor)
+        sc.num_final_steps = int(self.num_final_steps)
+        sc.force_first_step = int(bool(self.force_first_step))
+        return sc
+
+
+def fixed_timesteps(control: SolverControl, Δt: float, grow: float = 1.0) -> SolverControl:
+    """fixed_timesteps!(control, Δt; grow) (src/vfvm_solvercontrol.jl:320-327): time
+    steps Δt, Δt grow, Δt grow^2, ... (Δu_opt = floatmax disables the Δu control)."""
+    import sys
+    control.Δt = Δt
+    control.Δt_max = Δt
+    control.Δt_min = Δt
+    control.Δt_grow = grow
+    control.Δu_opt = sys.float_info.max
+    return control

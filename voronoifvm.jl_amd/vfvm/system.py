@@ -252,14 +252,32 @@ def _control(control, kwargs):
     return c
 
 
+class TransientSolution:
+    """Accepted times (or embedding parameters) t and solutions u (nspec x nnodes
+    each), as the reference's TransientSolution (src/vfvm_transientsolution.jl)."""
+
+    def __init__(self, t, u, history):
+        self.t = np.asarray(t)
+        self.u = list(u)
+        self.history = history
+
+    def __len__(self):
+        return len(self.t)
+
+    def __getitem__(self, i):
+        return self.u[i]
+
+
 def solve(sys: System, inival=0.0, control: Optional[SolverControl] = None, time=0.0, tstep=math.inf,
-          times=None, embed=None, **kwargs):
+          times=None, embed=None, store_all=True, **kwargs):
     """solve(sys; inival, control, time, tstep, times, embed) (src/vfvm_solver.jl:551-668).
 
-    Stationary and implicit-Euler-step solves run the device Newton loop.  `times`
-    and `embed` run with fixed steps (Δt_min = Δt_max, fixed_timesteps!), the
-    step sequence given explicitly by the `times` / `embed` values.
-    Returns the solution (nspec x nnodes); for `times`/`embed` the final one."""
+    Stationary and implicit-Euler-step solves run the device Newton loop and return
+    the solution (nspec x nnodes).  `times` (implicit Euler) and `embed` (parameter
+    embedding) run solve_transient! with its step-size control (the Δt resp. Δp
+    fields of the control; `fixed_timesteps` gives fixed steps) and return a
+    TransientSolution with the initial value and every accepted step (only the last
+    one with store_all=False)."""
     c = _control(control, kwargs)
     L = _capi.lib()
     h = sys.handle()
@@ -268,15 +286,30 @@ def solve(sys: System, inival=0.0, control: Optional[SolverControl] = None, time
     U0 = _as_dofs(sys, inival)
     cc = c.to_c(sys.is_linear)
     if times is not None or embed is not None:
-        lam = np.ascontiguousarray(np.asarray(times if times is not None else embed, dtype=np.float64))
+        transient = times is not None
+        lam = np.ascontiguousarray(np.asarray(times if transient else embed, dtype=np.float64))
+        sc = c.to_step_c(transient)
+        # output entries: the start and every accepted step (bounded by the smallest step)
+        span = float(lam[-1] - lam[0]) if lam.size > 1 else 0.0
+        max_out = int(min(1_000_000, span / max(sc.d_min, 1e-300) + 2 * lam.size + 8)) if store_all else 1
         out = np.zeros_like(U0)
+        out_t = np.zeros(max_out)
+        out_u = np.zeros((max_out, U0.size)) if store_all else None
+        nst = C.c_int64()
         hist = _capi.HistoryT()
-        rc = L.vfvm_solve_fixed_steps(h, _capi.dptr(U0), lam.size, _capi.dptr(lam), int(times is not None),
-                                      C.byref(cc), _capi.dptr(out), C.byref(hist))
+        rc = L.vfvm_solve_transient(h, _capi.dptr(U0), lam.size, _capi.dptr(lam), int(transient), C.byref(cc),
+                                    C.byref(sc), _capi.dptr(out), max_out, _capi.dptr(out_t),
+                                    _capi.dptr(out_u) if store_all else None, C.byref(nst), C.byref(hist))
         sys.history = NewtonSolverHistory(niter=hist.niter, nlin=hist.nlin, time=hist.time,
                                           residual_norm=hist.residual_norm)
         _capi.check(rc, sys.ctx.handle)
-        return _from_dofs(sys, out)
+        n = min(int(nst.value) + 1, max_out)  # entries: start + accepted steps
+        if store_all:
+            ts = list(out_t[:n])
+            us = [_from_dofs(sys, out_u[k]) for k in range(n)]
+        else:
+            ts, us = [float(lam[-1])], [_from_dofs(sys, out)]
+        return TransientSolution(ts, us, sys.history)
     U = np.zeros_like(U0)
     hist = _capi.HistoryT()
     rc = L.vfvm_newton(h, _capi.dptr(U), _capi.dptr(U0), time, tstep, 0.0, C.byref(cc), C.byref(hist))
