@@ -128,7 +128,15 @@ __global__ void __launch_bounds__(BB) k_bfinalize(Seg S, KState *st, const int32
 // y = Ahat x on the chunk's rows (SELL-64: the chunk's rows are slices
 // ch_slice[c], ch_slice[c] + 1, ... of 64 rows; one lane per row, so a wave reads
 // entry j of its 64 rows in one coalesced access), fused (w0, y) and (y, y)
-template <class VA, class V, int NDOT, int STAGE>
+template <bool NT, class T>
+__device__ __forceinline__ T bld(const T *p) {  // matrix stream load, optionally non-temporal
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+template <class VA, class V, int NDOT, int STAGE, bool NT>
 __global__ void __launch_bounds__(BB) k_bspmv(Seg S, const int32_t *__restrict__ ch_slice,
                                               const int32_t *__restrict__ sptr, const int32_t *__restrict__ col,
                                               const VA *__restrict__ val, const V *__restrict__ x, V *__restrict__ y,
@@ -153,7 +161,8 @@ __global__ void __launch_bounds__(BB) k_bspmv(Seg S, const int32_t *__restrict__
     const int32_t p1 = sptr[sl + 1];
     double acc = 0.0;
 #pragma unroll 4
-    for (int32_t q = sptr[sl] + (int32_t)(l & 63); q < p1; q += 64) acc += (double)val[q] * (double)x[col[q]];
+    for (int32_t q = sptr[sl] + (int32_t)(l & 63); q < p1; q += 64)
+      acc += (double)bld<NT>(val + q) * (double)x[bld<NT>(col + q)];
     const int64_t r = r0 + l;
     const V yv = (V)acc;
     y[r] = yv;
@@ -374,6 +383,7 @@ struct vfvm_batch {
   // probe work of the batched SpMV: per member (rows, nonzeros) and per-member counters
   DevBuf<int64_t> msize;
   DevBuf<unsigned long long> mwork;
+  bool sell_nt[2] = {false, false};  // non-temporal matrix loads in the SpMV (FP32, FP64)
   // captured Krylov iteration blocks: (value bytes, solution vector, iterations) -> graph
   bool use_graphs = true;
   std::map<std::tuple<int, const void *, int>, hipGraphExec_t> graphs;
@@ -433,14 +443,21 @@ void bop(vfvm_batch &b, const V *x, V *y, const V *w0) {
   const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
   unsigned long long *mwork = b.ctx->probe.counters(kid) ? b.mwork.p : nullptr;
   b.ctx->probe.before(kid, b.ctx->stream);
-  if constexpr (sizeof(V) == 4)
-    hipLaunchKernelGGL((k_bspmv<float, V, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                       b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const float *)b.A32.p, x, y, w0, b.st.p, mwork,
-                       (const int64_t *)b.msize.p);
+  auto go = [&](auto ntc) {
+    constexpr bool NT = decltype(ntc)::value;
+    if constexpr (sizeof(V) == 4)
+      hipLaunchKernelGGL((k_bspmv<float, V, NDOT, STAGE, NT>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                         b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const float *)b.A32.p, x, y, w0, b.st.p, mwork,
+                         (const int64_t *)b.msize.p);
+    else
+      hipLaunchKernelGGL((k_bspmv<double, V, NDOT, STAGE, NT>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream,
+                         S, b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const double *)b.A64s.p, x, y, w0, b.st.p,
+                         mwork, (const int64_t *)b.msize.p);
+  };
+  if (b.sell_nt[sizeof(V) == 4 ? 0 : 1])
+    go(std::true_type());
   else
-    hipLaunchKernelGGL((k_bspmv<double, V, NDOT, STAGE>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                       b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const double *)b.A64s.p, x, y, w0, b.st.p, mwork,
-                       (const int64_t *)b.msize.p);
+    go(std::false_type());
   b.ctx->probe.after(kid, b.ctx->stream);
   if constexpr (NDOT > 0)
     hipLaunchKernelGGL((k_bfinalize<2, STAGE, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, b.ctx->stream, S,
@@ -805,6 +822,11 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
 int vfvm_batch_set_graphs(vfvm_batch *b, int enable) {
   if (!b) return VFVM_ERR_ARG;
   b->use_graphs = enable != 0;
+  if (!b->use_graphs) {
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+  }
   return VFVM_OK;
 }
 
@@ -904,6 +926,26 @@ int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group) 
       }
     }
     b->spmv_group = bestg;
+    // non-temporal matrix loads in the SELL-64 SpMV, per precision (operator values from J)
+    launch_scale_rows(b->nspec, b->nnodes, b->rowptr.p, b->colind.p, b->J.p, b->dofdiag.p, b->sell_rbase.p,
+                      b->rowscale.p, b->A32.p, b->A64s.p, nullptr, st);
+    reset_states(*b, 0.0, 0.0, 1);
+    for (int pi = 0; pi < 2; pi++) {
+      float t[2];
+      for (int nt = 0; nt < 2; nt++) {
+        b->sell_nt[pi] = nt != 0;
+        t[nt] = timeit([&]() {
+          if (pi == 0)
+            bop<float, 0, STAGE_NONE>(*b, b->k32[0].p, b->k32[1].p, (const float *)nullptr);
+          else
+            bop<double, 0, STAGE_NONE>(*b, b->U.p, b->tmp.p, (const double *)nullptr);
+        });
+      }
+      b->sell_nt[pi] = t[1] < t[0];
+    }
+    // captured graphs hold the previous kernel choices
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
     if (asm_group) *asm_group = b->asm_group;
     if (spmv_group) *spmv_group = b->spmv_group;
     return VFVM_OK;

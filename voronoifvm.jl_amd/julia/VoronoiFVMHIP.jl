@@ -6,6 +6,7 @@
 #   eval_and_assemble (src/vfvm_assembly.jl:520-643)  -> hip_eval_and_assemble!
 #   solve_step!       (src/vfvm_solver.jl:13-222)     -> hip_solve_step!
 #   _solve_linear!    (src/vfvm_linsolve.jl:6-61)     -> hip_solve_linear!
+#   solve_transient!  (src/vfvm_solver.jl:267-537)    -> hip_solve_transient!
 #
 # The Physics callbacks are Julia closures the device cannot call; a system is
 # mapped to a registry entry with `hip_physics` (flux/reaction/storage kinds and
@@ -147,6 +148,50 @@ function hip_solve_step!(ctx, sys, solution, oldsol, control::ControlT, time, ts
                sys, solution, oldsol, time, tstep, λ, control, hist)
     check(rc, ctx)
     return solution, hist
+end
+
+struct StepControlT
+    d::Float64; d_min::Float64; d_max::Float64; d_grow::Float64; d_decrease::Float64
+    du_opt::Float64; du_max_factor::Float64
+    num_final_steps::Int32; force_first_step::Int32
+end
+
+"""
+    step_control(control::VoronoiFVM.SolverControl, transient) -> StepControlT
+
+The Δt (time stepping) or Δp (embedding) fields of a SolverControl
+(src/vfvm_solvercontrol.jl:138-211).
+"""
+function step_control(c, transient)
+    transient ?
+        StepControlT(c.Δt, c.Δt_min, c.Δt_max, c.Δt_grow, c.Δt_decrease, c.Δu_opt, c.Δu_max_factor,
+                     c.num_final_steps, c.force_first_step) :
+        StepControlT(c.Δp, c.Δp_min, c.Δp_max, c.Δp_grow, c.Δp_decrease, c.Δu_opt, c.Δu_max_factor,
+                     c.num_final_steps, c.force_first_step)
+end
+
+"""
+    hip_solve_transient!(ctx, sys, inival, lambdas, control::ControlT, steps::StepControlT; transient = true)
+
+Replacement of solve_transient! (src/vfvm_solver.jl:267-537): returns the accepted
+times (or parameters) and solutions, the first entry being the start.
+"""
+function hip_solve_transient!(ctx, sys, inival, lambdas, control::ControlT, steps::StepControlT; transient = true,
+                              max_out = 100_000)
+    ndof = length(inival)
+    U = similar(inival)
+    ts = zeros(max_out)
+    us = zeros(ndof, max_out)
+    nsteps = Ref{Int64}(0)
+    hist = HistoryT()
+    rc = ccall((:vfvm_solve_transient, libvfvm), Cint,
+               (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Float64}, Cint, Ref{ControlT}, Ref{StepControlT}, Ptr{Float64},
+                Int64, Ptr{Float64}, Ptr{Float64}, Ref{Int64}, Ref{HistoryT}),
+               sys, inival, length(lambdas), collect(Float64, lambdas), transient, control, steps, U, max_out, ts, us,
+               nsteps, hist)
+    check(rc, ctx)
+    n = min(nsteps[] + 1, max_out)
+    return ts[1:n], [us[:, k] for k in 1:n], hist
 end
 
 function default_control()
