@@ -10,10 +10,13 @@ solve calls are timed.  Workloads:
   cfg4 (default)  1,024 held-out 129x129 nonlinear-Poisson inputs: learned initial
                   guess + batches of 256 + FP32 Krylov, against the plain FP64 GPU
                   Newton from zero on the same inputs; a step is one batch of 256
+  cfg0            101^2 nonlinear Poisson (the reference's CPU-runnable case); a step
+                  is one solve
   cfg1            129^3 nonlinear diffusion, implicit Euler; a step is one time step
   cfg2            512^2 bipolar drift-diffusion bias sweep; a step is one sweep
   cfg3            256 seeded nonlinear-Poisson instances of uneven size; a step is
                   one batched solve of all 256
+  copy            1 GiB device-to-device copy: the measured HBM bandwidth reference
 
 The roofline entries come from the library's kernel probe: right after the timed
 solves, the same solves run once more with every launch of the kernel bracketed
@@ -88,6 +91,47 @@ def read_traffic(name):
             return json.load(f).get(name, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+# --------------------------------------------------------------------------- cfg0
+def run_cfg0(args):
+    """The reference's CPU-runnable case: 101 x 101 nonlinear Poisson (eps = 1e-2, f = 1,
+    Dirichlet 0 on west/south and 1 on east/north, zero initial value), plain FP64 device
+    Newton with BiCGStab + ILU(0); a step is one solve."""
+    import cases
+    s, _ = cases.nonlinear_poisson_2d(101, oracle=False)
+    s.handle()
+    s.autotune()
+    ctrl = vfvm.SolverControl(precon="ilu0", reltol_linear=1e-8, abstol_linear=0.0, maxiters_linear=1000)
+    K, W = args.steps, args.warmup
+    for _ in range(max(W, 1)):
+        vfvm.solve(s, inival=0.0, control=ctrl)
+    t1 = time.perf_counter()
+    for _ in range(K):
+        vfvm.solve(s, inival=0.0, control=ctrl)
+        if not s.history.residual_norm <= 1e-10:
+            raise RuntimeError(f"cfg0: ||F||_inf = {s.history.residual_norm:.3e}")
+    el = time.perf_counter() - t1
+    log(f"cfg0: {K} solves in {el:.3f}s -> {K / el:.1f} solves/s, newton {s.history.niter}, krylov {s.history.nlin}")
+    return dict(value=K / el, ms_per_step=1e3 * el / K, roofline=None,
+                config=dict(workload="cfg0: 2D nonlinear Poisson 101x101 (10,201 unknowns), eps=1e-2, f=1, FP64 "
+                                     "device Newton with BiCGStab + ILU(0) (the reference solves it with UMFPACK)",
+                            unknowns=s.grid.num_nodes),
+                dtype="f64")
+
+
+def cpu_cfg0(args):
+    """The oracle on the same problem: Newton with banded direct solves (UMFPACK's role)."""
+    import cases
+    s, o = cases.nonlinear_poisson_2d(101)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 10.0:
+        o.solve_step(np.zeros(s.grid.num_nodes))
+        n += 1
+    el = time.perf_counter() - t0
+    return dict(value=n / el, unit="solves/s", cores=1, kind="port",
+                sample=f"{n} solves of the cfg0 problem with the oracle's banded direct solver, {el:.2f} s")
 
 
 # --------------------------------------------------------------------------- cfg1
@@ -392,7 +436,10 @@ def run_cfg4(args):
         g = net.predict_sides(*raw)
         fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
         fam.batch.solve(0.0, ctrl32)
-    t_sur, t_zero, it_sur, it_zero, sur_err, worst = 0.0, 0.0, [], [], [], 0.0
+    t_sur, t_zero, t_zero64, it_sur, it_zero, sur_err, worst = 0.0, 0.0, 0.0, [], [], [], 0.0
+    ctrl64b = vfvm.SolverControl(precision="fp64", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
+                                 check_every=8)
+    fam.batch.solve(0.0, ctrl64b)  # warm-up of the FP64 batched path
     converged = []
     ctx = fam.systems[0].ctx
     probes = {_capi.KERNEL_SPMV32: [], _capi.KERNEL_ASSEMBLE: []}
@@ -414,6 +461,10 @@ def run_cfg4(args):
         U0 = fam.batch.solve(0.0, ctrl32)
         t_zero += time.perf_counter() - t1
         it_zero += [h.niter for h in fam.batch.histories]
+        worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
+        t1 = time.perf_counter()  # batching alone: FP64 Krylov from zero
+        fam.batch.solve(0.0, ctrl64b)
+        t_zero64 += time.perf_counter() - t1
         worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
         for x, u in zip(g, U):
             sur_err.append(float(np.abs(x - vfvm.dofs(u)).max()))
@@ -472,6 +523,7 @@ def run_cfg4(args):
     gemm_bytes = rows * (128 * 2 + 64 * 4 * 3 + 64 * 2)
     return dict(value=sur_rate, ms_per_step=1e3 * t_sur / K, steps=K, plain_gpu_newton_solves_per_s=plain_rate,
                 speedup_vs_plain=sur_rate / plain_rate, batched_fp32_from_zero_solves_per_s=zero_rate,
+                batched_fp64_from_zero_solves_per_s=nsol / t_zero64,
                 newton_iterations=dict(surrogate=float(np.mean(it_sur)), zero_batched=float(np.mean(it_zero)),
                                        plain=float(np.mean(it_plain))),
                 surrogate_max_norm_error=dict(mean=float(np.mean(sur_err)), max=float(np.max(sur_err))),
@@ -545,7 +597,7 @@ def cpu_copy(args):
     return None
 
 
-WORKLOADS = {"copy": (run_copy, cpu_copy), "cfg1": (run_cfg1, cpu_cfg1), "cfg2": (run_cfg2, cpu_cfg2), "cfg3": (run_cfg3, cpu_cfg3), "cfg4": (run_cfg4, cpu_cfg4)}
+WORKLOADS = {"copy": (run_copy, cpu_copy), "cfg0": (run_cfg0, cpu_cfg0), "cfg1": (run_cfg1, cpu_cfg1), "cfg2": (run_cfg2, cpu_cfg2), "cfg3": (run_cfg3, cpu_cfg3), "cfg4": (run_cfg4, cpu_cfg4)}
 
 
 def main():

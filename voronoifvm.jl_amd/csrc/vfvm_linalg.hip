@@ -440,20 +440,31 @@ __global__ void __launch_bounds__(VB) k_to_sell(int64_t nrows, const int32_t *__
 // i of the slices that overlap the colour's row range [r0, r1).  Lower part =
 // entries before the diagonal position sdpos[i], upper part = entries after it
 // (padding entries carry value 0 and the row's own column).
-template <class VL, class V>
+// LAST: the colour is the last one, whose rows have no upper part, so the backward
+// solve of these rows (x = y / u_ii) is done here and the backward sweep starts at
+// the previous colour
+template <class VL, class V, bool LAST>
 __global__ void __launch_bounds__(VB) k_ilu_lower_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ sptr,
                                                        const int32_t *__restrict__ col,
                                                        const int32_t *__restrict__ sdpos, const VL *__restrict__ lu,
                                                        const int32_t *__restrict__ perm, const V *__restrict__ r,
-                                                       V *__restrict__ y, const KState *st, int skip_if_half) {
+                                                       V *__restrict__ y, V *__restrict__ z, const KState *st,
+                                                       int skip_if_half) {
   if (st && idle(st)) return;
   if (st && skip_if_half && st->half) return;
   const int32_t i = (r0 & ~63) + (int32_t)(blockIdx.x * VB + threadIdx.x);
   if (i < r0 || i >= r1) return;
-  double acc = (double)r[perm[i]];
+  const int32_t pi = perm[i];
+  double acc = (double)r[pi];
   const int32_t q0 = sptr[i >> 6] + (i & 63), qd = q0 + 64 * sdpos[i];
   for (int32_t q = q0; q < qd; q += 64) acc -= (double)lu[q] * (double)y[col[q]];
-  y[i] = (V)acc;
+  if constexpr (LAST) {
+    const V xv = (V)(acc / (double)lu[qd]);
+    y[i] = xv;
+    z[pi] = xv;
+  } else {
+    y[i] = (V)acc;
+  }
 }
 
 template <class VL, class V>
@@ -850,11 +861,15 @@ const V *LinearSolver::precond(const V *r, V *z, KState *stp, int skip_if_half, 
     auto blocks = [](int32_t r0, int32_t r1) { return (unsigned)((r1 - (r0 & ~63) + VB - 1) / VB); };
     for (int c0 = 0; c0 < I.ncolors; c0++) {
       int32_t r0 = I.cptr[c0], r1 = I.cptr[c0 + 1];
-      if (r1 > r0)
-        hipLaunchKernelGGL((k_ilu_lower_sell<V, V>), dim3(blocks(r0, r1)), dim3(VB), 0, stream, r0, r1, I.sptr, I.scol,
-                           I.sdpos, lu, I.perm, r, y, stp, skip_if_half);
+      if (r1 <= r0) continue;
+      if (c0 == I.ncolors - 1)
+        hipLaunchKernelGGL((k_ilu_lower_sell<V, V, true>), dim3(blocks(r0, r1)), dim3(VB), 0, stream, r0, r1, I.sptr,
+                           I.scol, I.sdpos, lu, I.perm, r, y, z, stp, skip_if_half);
+      else
+        hipLaunchKernelGGL((k_ilu_lower_sell<V, V, false>), dim3(blocks(r0, r1)), dim3(VB), 0, stream, r0, r1, I.sptr,
+                           I.scol, I.sdpos, lu, I.perm, r, y, z, stp, skip_if_half);
     }
-    for (int c0 = I.ncolors - 1; c0 >= 0; c0--) {
+    for (int c0 = I.ncolors - 2; c0 >= 0; c0--) {
       int32_t r0 = I.cptr[c0], r1 = I.cptr[c0 + 1];
       if (r1 > r0)
         hipLaunchKernelGGL((k_ilu_upper_sell<V, V>), dim3(blocks(r0, r1)), dim3(VB), 0, stream, r0, r1, I.sptr, I.scol,
