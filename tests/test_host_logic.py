@@ -1,0 +1,31 @@
+"""Host-side logic of the Python mirror (no device calls): SolverControl defaults
+and the step-control fields it hands to the library."""
+import sys
+
+import vfvm
+
+
+def test_solver_control_defaults_follow_reference():
+    # src/vfvm_solvercontrol.jl:36-211
+    c = vfvm.SolverControl()
+    assert (c.abstol, c.reltol, c.maxiters, c.tol_mono, c.damp_initial, c.damp_growth, c.max_round) == \
+        (1e-10, 1e-10, 100, 1e-3, 1.0, 1.2, 1000)
+    assert (c.Δt, c.Δt_min, c.Δt_max, c.Δt_grow, c.Δt_decrease) == (0.1, 1e-3, 1.0, 1.2, 0.5)
+    assert (c.Δp, c.Δp_min, c.Δp_max, c.Δp_grow, c.Δp_decrease) == (1.0, 1e-3, 1.0, 1.0, 0.5)
+    assert (c.Δu_opt, c.Δu_max_factor, c.num_final_steps, c.force_first_step) == (0.1, 1.2, 5, False)
+
+
+def test_step_control_fields():
+    c = vfvm.SolverControl(Δp=0.05, Δp_min=0.01, Δp_max=0.2)
+    t = c.to_step_c(True)
+    e = c.to_step_c(False)
+    assert (t.d, t.d_min, t.d_max, t.d_grow) == (0.1, 1e-3, 1.0, 1.2)
+    assert (e.d, e.d_min, e.d_max, e.d_grow) == (0.05, 0.01, 0.2, 1.0)
+    assert e.du_opt == 0.1 and e.num_final_steps == 5
+
+
+def test_fixed_timesteps():
+    # src/vfvm_solvercontrol.jl:320-327
+    c = vfvm.fixed_timesteps(vfvm.SolverControl(), 0.25, grow=1.5)
+    assert (c.Δt, c.Δt_min, c.Δt_max, c.Δt_grow) == (0.25, 0.25, 0.25, 1.5)
+    assert c.Δu_opt == sys.float_info.max
