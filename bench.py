@@ -501,6 +501,34 @@ def run_cfg4(args):
     plain_rate = nsol / t_plain
     log(f"plain FP64 GPU Newton from zero: {nsol} solves in {t_plain:.2f}s -> {plain_rate:.1f} solves/s, "
         f"mean newton {np.mean(it_plain):.2f}; max |U_sur - U_plain| = {diff:.2e}")
+    # continuation on a sample: source and side values ramped by the embedding parameter
+    # over [0, 1] (solve(...; embed)), plain FP64 GPU Newton at every accepted step
+    ncont = min(32, nsol)
+    from vfvm.physics import Physics, LinearFlux, SinhReaction, Dirichlet
+    cph = Physics(flux=LinearFlux(S.EPS), reaction=SinhReaction(), source=np.zeros((1, fam.grid.num_nodes), order="F"),
+                  bcondition=[Dirichlet(species=1, region=r + 1, value=0.0, value_lambda=0.0) for r in range(4)],
+                  embed_source=True)
+    cont = vfvm.System(fam.grid, cph, species=[1])
+    cctrl = vfvm.SolverControl(precision="fp64", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
+                               check_every=8, Δp=0.25, Δp_min=1e-2, Δp_max=0.5, Δp_grow=1.5, Δu_opt=0.5)
+    t_cont, it_cont, st_cont, cdiff = 0.0, [], [], 0.0
+    for k, sd in enumerate(blocks[0][:ncont]):
+        f, vals = S.draw(sd, fam.grf)
+        cont.set_source(np.asfortranarray(f.reshape(1, -1)))
+        cont.update_bcondition([Dirichlet(species=1, region=r + 1, value=0.0, value_lambda=float(v))
+                                for r, v in enumerate(vals)])
+        cont.handle()
+        t1 = time.perf_counter()
+        sol = vfvm.solve(cont, inival=0.0, control=cctrl, embed=[0.0, 1.0])
+        t_cont += time.perf_counter() - t1
+        if not cont.history.residual_norm <= 1e-10:
+            raise RuntimeError("continuation did not reach ||F|| <= 1e-10")
+        it_cont.append(cont.history.niter)  # Newton iterations over all steps
+        st_cont.append(len(sol.t) - 1)
+        cdiff = max(cdiff, float(np.abs(vfvm.dofs(sol.u[-1]) - converged[k]).max()))
+    cont_rate = ncont / t_cont
+    log(f"continuation (plain FP64 GPU, {ncont} seeds): {cont_rate:.1f} solves/s, mean accepted steps "
+        f"{np.mean(st_cont):.1f}, mean newton {np.mean(it_cont):.1f}; max |U_cont - U_sur| = {cdiff:.2e}")
     # roofline: batched FP32 SpMV (the Krylov hot op) and the Jacobian assembly, timed live
     # inside the surrogate solves (work counted over the members still active), and the
     # network's layer GEMM
@@ -524,6 +552,10 @@ def run_cfg4(args):
     return dict(value=sur_rate, ms_per_step=1e3 * t_sur / K, steps=K, plain_gpu_newton_solves_per_s=plain_rate,
                 speedup_vs_plain=sur_rate / plain_rate, batched_fp32_from_zero_solves_per_s=zero_rate,
                 batched_fp64_from_zero_solves_per_s=nsol / t_zero64,
+                continuation_plain_solves_per_s=cont_rate,
+                continuation=dict(seeds=ncont, mean_accepted_steps=float(np.mean(st_cont)),
+                                  mean_newton_iterations=float(np.mean(it_cont)),
+                                  max_abs_diff_vs_surrogate=cdiff),
                 newton_iterations=dict(surrogate=float(np.mean(it_sur)), zero_batched=float(np.mean(it_zero)),
                                        plain=float(np.mean(it_plain))),
                 surrogate_max_norm_error=dict(mean=float(np.mean(sur_err)), max=float(np.max(sur_err))),

@@ -48,7 +48,9 @@ extern "C" {
  *   p[0..2] D_i (LINEAR per species; POWER and NLDIFF use p[0])
  *   p[3] d2 (NLDIFF: D(u) = p[0] + p[3]*u^2 at the edge mean), p[4] m (POWER exponent)
  *   p[5] k (SINH/POW/EXCHANGE factor), p[6] POW exponent
- *   p[7] lambda^2, p[8] z_n, p[9] z_p, p[10] n_i^2, p[11] n_i (DD)            */
+ *   p[7] lambda^2, p[8] z_n, p[9] z_p, p[10] n_i^2, p[11] n_i (DD)
+ *   p[12] != 0: the node source is scaled by the embedding parameter lambda
+ *         (a source callback returning embedparam(node) * f, src/vfvm_geometryitems.jl:13) */
 #define VFVM_FLUX_NONE 0
 #define VFVM_FLUX_LINEAR 1 /* D_i (u_k - u_l)          Example105_NonlinearPoisson1D.jl:41-44 */
 #define VFVM_FLUX_POWER 2  /* D (u_k^m - u_l^m)        Example106_NonlinearDiffusion1D.jl:57-60 */

@@ -109,7 +109,7 @@ def control(**kw) -> Control:
 def physics(flux="none", reaction="none", storage="none", **params) -> Physics:
     """Parameter slots follow ORC_P_* in vfvm_oracle.h."""
     slots = {"D0": 0, "D1": 1, "D2sp": 2, "d2": 3, "m": 4, "k": 5, "rm": 6, "lambda2": 7,
-             "zn": 8, "zp": 9, "ni2": 10, "ni": 11}
+             "zn": 8, "zp": 9, "ni2": 10, "ni": 11, "embed_source": 12}
     ph = Physics()
     ph.flux, ph.reaction, ph.storage = FLUX[flux], REACT[reaction], STOR[storage]
     for k, v in params.items():

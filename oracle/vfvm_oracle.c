@@ -842,9 +842,11 @@ static int addnz(orc_sys *s, int i, int j, double v, double fac) {
 
 static double bc_value(const orc_bc *b, double lambda) { return b->value + b->value_lambda * lambda; }
 
-/* one node item: src/vfvm_assembly.jl:81-122 + src/vfvm_assemblydata.jl:245-270 */
+/* one node item: src/vfvm_assembly.jl:81-122 + src/vfvm_assemblydata.jl:245-270.
+ * With p[12] != 0 the source callback returns embedparam(node) * f (the embedding
+ * parameter lambda of solve(...; embed), src/vfvm_geometryitems.jl:13). */
 static int asm_node(orc_sys *s, int K, double fac, const double *U, const double *Uold, double tstepinv,
-                    double *F) {
+                    double lambda, double *F) {
   int ns = s->nspec;
   dual u[ORC_MAXSPEC], uo[ORC_MAXSPEC], r[ORC_MAXSPEC], st[ORC_MAXSPEC], so[ORC_MAXSPEC];
   for (int i = 0; i < ns; i++) {
@@ -856,6 +858,7 @@ static int asm_node(orc_sys *s, int K, double fac, const double *U, const double
   phys_storage(&s->phys, ns, uo, so);
   for (int i = 0; i < ns; i++) {
     double src = s->source ? s->source[K * ns + i] : 0.0;
+    if (s->phys.p[ORC_P_EMBED_SOURCE] != 0.0) src *= lambda;
     F[K * ns + i] += fac * (r[i].v - src + (st[i].v - so[i].v) * tstepinv);
     for (int j = 0; j < ns; j++) {
       int rc = addnz(s, K * ns + i, K * ns + j, r[i].d[j] + st[i].d[j] * tstepinv, fac);
@@ -951,7 +954,7 @@ int orc_assemble(orc_sys *s, const double *U, const double *Uold, double time, d
   if (s->assembly == ORC_CELLWISE) {
     for (int c = 0; c < s->ncells; c++)
       for (int i = 0; i < nv; i++)
-        if ((rc = asm_node(s, s->cellnodes[nv * c + i], s->cnodefac[nv * c + i], U, Uold, tstepinv, F))) return rc;
+        if ((rc = asm_node(s, s->cellnodes[nv * c + i], s->cnodefac[nv * c + i], U, Uold, tstepinv, lambda, F))) return rc;
     for (int c = 0; c < s->ncells; c++)
       for (int e = 0; e < ne; e++) {
         int a, b;
@@ -962,7 +965,7 @@ int orc_assemble(orc_sys *s, const double *U, const double *Uold, double time, d
   } else {
     for (int K = 0; K < s->nnodes; K++)
       for (int64_t k = s->enf.ptr[K]; k < s->enf.ptr[K + 1]; k++)
-        if ((rc = asm_node(s, K, s->enf.fac[k], U, Uold, tstepinv, F))) return rc;
+        if ((rc = asm_node(s, K, s->enf.fac[k], U, Uold, tstepinv, lambda, F))) return rc;
     for (int e = 0; e < s->nedges; e++)
       for (int64_t k = s->eef.ptr[e]; k < s->eef.ptr[e + 1]; k++)
         if ((rc = asm_edge(s, s->edgenodes[2 * e], s->edgenodes[2 * e + 1], s->eef.fac[k], U, F))) return rc;

@@ -57,7 +57,8 @@ enum {
   ORC_P_ZN = 8,      /* DD: electron charge number (-1) */
   ORC_P_ZP = 9,      /* DD: hole charge number (+1) */
   ORC_P_NI2 = 10,    /* DD: n_i^2 in (np - ni^2)/(n + p + 2 n_i) */
-  ORC_P_NI = 11
+  ORC_P_NI = 11,
+  ORC_P_EMBED_SOURCE = 12 /* != 0: source scaled by the embedding parameter lambda */
 };
 
 typedef struct {

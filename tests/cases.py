@@ -40,14 +40,15 @@ def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None, oracle=True):
     return sys, osys
 
 
-def nonlinear_poisson_2d(n=101, eps=1e-2, f=1.0, values=(0.0, 1.0, 1.0, 0.0)):
+def nonlinear_poisson_2d(n=101, eps=1e-2, f=1.0, values=(0.0, 1.0, 1.0, 0.0), oracle=True):
     """Nonlinear Poisson: flux eps (u_k - u_l), reaction e^u - e^-u - f; Dirichlet
     values on south, east, north, west (regions 1..4)."""
     X = linspace(0.0, 1.0, n)
     g = simplexgrid(X, X)
     bcs = [vfvm.Dirichlet(species=1, region=r + 1, value=v) for r, v in enumerate(values)]
     src = np.full((1, g.num_nodes), f, order="F")
-    return pair(g, 1, dict(flux=vfvm.LinearFlux(eps), reaction=vfvm.SinhReaction()), bcs, source=src)
+    return pair(g, 1, dict(flux=vfvm.LinearFlux(eps), reaction=vfvm.SinhReaction()), bcs, source=src,
+                oracle=oracle)
 
 
 def nonlinear_diffusion_3d(n=17):

@@ -57,6 +57,21 @@ def test_forward_matches_oracle():
     assert np.sqrt((err ** 2).mean()) <= 2e-3 * scale
 
 
+def test_forward_matches_oracle_many_tiles():
+    """169,000 rows: more 64-row tiles than inference blocks (blocks loop over tiles),
+    a partial last tile, and 80 smoothing columns (two column blocks)."""
+    print("case mpnn forward, 65x65 grid, 40 samples", flush=True)
+    net, graph, X, T = _setup(n=65, nsamples=40, seed=3)
+    theta = net.params().astype(np.float64)
+    u_gpu = net.predict(X)
+    u_cpu = M.forward(theta, X.astype(np.float64), graph, nlayers=4, emulate_bf16=True)
+    scale = np.abs(u_cpu).max()
+    err = np.abs(u_gpu - u_cpu)
+    print("forward max rel err", err.max() / scale, "rms", np.sqrt((err ** 2).mean()) / scale, flush=True)
+    assert err.max() <= 2e-2 * scale
+    assert np.sqrt((err ** 2).mean()) <= 2e-3 * scale
+
+
 def test_gradient_matches_oracle():
     print("case mpnn gradient", flush=True)
     net, graph, X, T = _setup(seed=1)

@@ -273,3 +273,37 @@ def test_adaptive_time_stepping_matches_oracle():
         assert len(tsol.t) == n + 1
         assert np.abs(np.asarray(tsol.t[1:]) - steps).max() <= 1e-12
         assert np.abs(vfvm.dofs(tsol.u[-1]) - Uo).max() <= SOL_ATOL
+
+
+def test_continuation_with_embedded_source_matches_oracle():
+    """Continuation (solve(...; embed)) on the nonlinear Poisson problem with source and
+    Dirichlet values ramped by the embedding parameter: assembly at lambda = 0.37 to
+    1e-12, the accepted parameter sequence of the step control and the final solution
+    against the oracle, and the end point against the direct solve of the unramped
+    system."""
+    _print("case continuation embedded source")
+    X = linspace(0.0, 1.0, 33)
+    g = simplexgrid(X, X)
+    rng = np.random.default_rng(11)
+    f = rng.uniform(-5.0, 5.0, g.num_nodes)
+    vals = (0.3, -0.7, 0.9, -0.2)
+    bcs = [vfvm.Dirichlet(species=1, region=r + 1, value=0.0, value_lambda=v) for r, v in enumerate(vals)]
+    sys, osys = cases.pair(g, 1, dict(flux=vfvm.LinearFlux(1e-2), reaction=vfvm.SinhReaction(), embed_source=True),
+                           bcs, source=f.reshape(1, -1))
+    U = rng.uniform(-1.0, 1.0, (1, g.num_nodes))
+    cases.compare_assembly(sys, osys, U, embed=0.37)
+    ctrl = vfvm.SolverControl(precon="ilu0", reltol_linear=1e-12, abstol_linear=0.0, maxiters_linear=2000,
+                              Δp=0.25, Δp_min=1e-3, Δp_max=0.5, Δp_grow=1.5)
+    sol = vfvm.solve(sys, inival=0.0, control=ctrl, embed=[0.0, 1.0])
+    Uo, n, _, steps = osys.solve_transient(np.zeros(g.num_nodes), [0.0, 1.0],
+                                           O.control(dp=0.25, dp_min=1e-3, dp_max=0.5, dp_grow=1.5),
+                                           transient=False, return_steps=True)
+    _print("continuation steps", n, "gpu", len(sol.t) - 1)
+    assert len(sol.t) == n + 1 and sol.t[0] == 0.0
+    assert np.abs(np.asarray(sol.t[1:]) - steps).max() <= 1e-12
+    assert np.abs(vfvm.dofs(sol.u[-1]) - Uo).max() <= SOL_ATOL
+    plain, _ = cases.pair(g, 1, dict(flux=vfvm.LinearFlux(1e-2), reaction=vfvm.SinhReaction()),
+                          [vfvm.Dirichlet(species=1, region=r + 1, value=v) for r, v in enumerate(vals)],
+                          source=f.reshape(1, -1), oracle=False)
+    Ud = vfvm.solve(plain, inival=0.0, control=ctrl)
+    assert np.abs(vfvm.dofs(sol.u[-1]) - vfvm.dofs(Ud)).max() <= SOL_ATOL

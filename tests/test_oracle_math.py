@@ -206,3 +206,31 @@ def test_fixed_timesteps_give_exact_sequence():
     assert nsteps == 10
     assert np.allclose(np.diff(np.concatenate([[0.0], steps])), 0.05, rtol=1e-12)
     assert np.abs(U - (1.0 / 3.5) ** 10).max() <= 1e-13
+
+
+def test_embedded_source_scales_with_lambda():
+    """Source embedding (p[12] != 0): the node source is lambda * f.  Only the source
+    and the Dirichlet values (value_lambda) depend on lambda, so the residual is affine
+    in lambda; at lambda = 1 it equals the system with the plain source and values; at
+    lambda = 0 the solution is u = 0; continuation over [0, 1] ends at the direct
+    solution of the lambda = 1 system."""
+    g = simplexgrid(linspace(0, 1, 9), linspace(0, 1, 9))
+    rng = np.random.default_rng(7)
+    f = rng.uniform(-5.0, 5.0, g.num_nodes)
+    vals = (0.3, -0.7, 0.9, -0.2)
+    emb = O.OracleSystem(g, 1, O.physics("linear", "sinh", D0=1e-2, k=1.0, embed_source=1.0),
+                         {(r + 1, 0): ("dirichlet", 0.0, 0.0, v) for r, v in enumerate(vals)}, O.EDGEWISE, f)
+    plain = O.OracleSystem(g, 1, O.physics("linear", "sinh", D0=1e-2, k=1.0),
+                           {(r + 1, 0): ("dirichlet", v) for r, v in enumerate(vals)}, O.EDGEWISE, f)
+    U = rng.uniform(-1.0, 1.0, g.num_nodes)
+    F0, F1, Fl = (emb.assemble(U, U, embed=l) for l in (0.0, 1.0, 0.37))
+    free = np.abs(F1) < 1e20
+    assert np.abs(Fl[free] - (F0[free] + 0.37 * (F1[free] - F0[free]))).max() <= 1e-14 * np.abs(F1[free]).max()
+    assert np.array_equal(F1, plain.assemble(U, U, embed=0.0))
+    Z, _ = emb.solve_step(np.zeros(g.num_nodes), embed=0.0)
+    assert np.abs(Z).max() == 0.0
+    ref, _ = plain.solve_step(np.zeros(g.num_nodes))
+    c = O.control(dp=0.25, dp_min=1e-3, dp_max=0.5, dp_grow=1.5)
+    Ue, nsteps, _ = emb.solve_transient(np.zeros(g.num_nodes), [0.0, 1.0], c, transient=False)
+    assert nsteps >= 3
+    assert np.abs(Ue - ref).max() <= 1e-9

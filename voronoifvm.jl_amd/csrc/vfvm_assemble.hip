@@ -49,6 +49,8 @@ __device__ __forceinline__ void node_terms(const AsmArgs &a, const InstParams *P
                                            double (&Fn)[NS], double (&Jn)[NS][NS], double (&Fb)[NS],
                                            double (&Jb)[NS], bool &bad) {
   {
+    // p[12] != 0: the source is embedparam(node) * f (src/vfvm_geometryitems.jl:13)
+    const double sfac = p[12] != 0.0 ? a.lambda : 1.0;
     double so[NS];
     eval_storage<NS>(P->storage, uo, so);
     if constexpr (JAC) {
@@ -59,7 +61,7 @@ __device__ __forceinline__ void node_terms(const AsmArgs &a, const InstParams *P
       eval_storage<NS>(P->storage, u, st);
 #pragma unroll
       for (int i = 0; i < NS; i++) {
-        const double src = srcv[i];
+        const double src = srcv[i] * sfac;
         Fn[i] = vol * (r[i].v - src + (st[i].v - so[i]) * a.tstepinv);
 #pragma unroll
         for (int j = 0; j < NS; j++) {
@@ -74,7 +76,7 @@ __device__ __forceinline__ void node_terms(const AsmArgs &a, const InstParams *P
       eval_storage<NS>(P->storage, uK, st);
 #pragma unroll
       for (int i = 0; i < NS; i++) {
-        const double src = srcv[i];
+        const double src = srcv[i] * sfac;
         Fn[i] = vol * (r[i] - src + (st[i] - so[i]) * a.tstepinv);
       }
     }

@@ -51,6 +51,12 @@ __device__ __forceinline__ bf16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8).  For a
+// grid of G = 8k workgroups, logical block (b % 8) * k + b / 8 gives each XCD one
+// contiguous range, so the graph neighbours a block gathers (nearby rows) are mostly
+// in the same XCD's L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) { return (b & 7) * (G >> 3) + (b >> 3); }
+
 // ---------------------------------------------------------------- GEMM -----
 // C[M x ND] = A[M x KD] * B[KD x ND], A and B bfloat16 row-major, float32
 // accumulation on the matrix cores.  Block = 4 waves = 64 rows; wave w owns
@@ -116,7 +122,7 @@ __global__ void __launch_bounds__(TB) k_gemm(int64_t M, const bf16_t *__restrict
       if constexpr (EPI == EPI_ENC) {
         const float h = fmaxf(v + e.bias[col], 0.f);
         e.hf[row * WID + col] = h;
-        e.cat[row * CW + col] = f2bf(h);
+        if (e.cat) e.cat[row * CW + col] = f2bf(h);
       } else if constexpr (EPI == EPI_RES) {
         const float r = fmaxf(v + e.bias[col], 0.f);
         e.rbuf[row * WID + col] = r;
@@ -231,6 +237,161 @@ __global__ void __launch_bounds__(TB) k_aggregate_t(int64_t rows, int64_t n, con
   dh[row * WID + f] += acc - da[row * WID + f];
 }
 
+// One hidden layer for inference, aggregation fused into the GEMM: for a tile of 64
+// consecutive rows, c = [bf(h_l), bf(sum_j w_ij h_l[j] - h_l[i])] is built in LDS
+// (same arithmetic as k_aggregate), multiplied on the matrix cores by W_l, and
+// h_{l+1} = h_l + relu(c W_l + b_l) is written to `hout` (a buffer other than `hin`:
+// neighbouring tiles still read h_l).  With `wd` set (last layer) the tile writes
+// u = h_{l+1} . wd + bd instead.  Blocks loop over tiles; W_l sits in LDS in operand
+// order (one 8-byte fragment per lane, k-step and column tile).  Each thread stages
+// four (row, 4-column chunk) items and loads the first NB0 neighbours of two items
+// before summing them, so 8 neighbour rows are in flight per thread.
+constexpr int NB0 = 4;
+__global__ void __launch_bounds__(TB, 4) k_layer_infer(int64_t rows, int64_t n, const int32_t *__restrict__ gptr,
+                                                    const int32_t *__restrict__ gcol, const float *__restrict__ gw,
+                                                    const float *__restrict__ hin, const bf16_t *__restrict__ W,
+                                                    const float *__restrict__ bias, float *__restrict__ hout,
+                                                    const float *__restrict__ wd, const float *__restrict__ bd,
+                                                    float *__restrict__ u) {
+  constexpr int ASTR = CW + 8;  // bf16 per LDS row of the layer input
+  constexpr int KS = CW / 16, NTL = WID / 16;
+  __shared__ bf16_t As[64 * ASTR];
+  __shared__ uint64_t Bf[KS * NTL * 64];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, g = l >> 4, c = l & 15;
+  for (int q = tid; q < KS * NTL * 64; q += TB) {
+    const int k = q / (NTL * 64), t = (q / 64) % NTL, ln = q % 64, gg = ln >> 4, cc = ln & 15;
+    uint64_t v = 0;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) v |= (uint64_t)W[(k * 16 + 4 * gg + jj) * WID + t * 16 + cc] << (16 * jj);
+    Bf[q] = v;
+  }
+  float bcol[NTL], wdc[NTL];
+#pragma unroll
+  for (int t = 0; t < NTL; t++) {
+    bcol[t] = bias[t * 16 + c];
+    wdc[t] = wd ? wd[t * 16 + c] : 0.f;
+  }
+  // XCD x takes the contiguous tile range [x T, (x + 1) T), T = ceil(ntiles / 8), in
+  // steps of the grid's blocks per XCD
+  const int64_t ntiles = (rows + 63) / 64, per = (ntiles + 7) / 8, G8 = gridDim.x / 8;
+  const int64_t t_end = std::min<int64_t>(ntiles, (int64_t)(blockIdx.x & 7) * per + per);
+  for (int64_t tile = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3); tile < t_end; tile += G8) {
+    const int64_t row0 = tile * 64;
+    __syncthreads();  // Bf is written; the previous tile's As reads are done
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 hv[4], av[4];
+    int64_t base[4];
+    int32_t kb[4], ke[4];
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      const int q = tid + it * TB, r = q >> 4, ch = q & 15;
+      const int64_t row = row0 + r;
+      hv[it] = z4;
+      av[it] = z4;
+      base[it] = ch * 4;
+      kb[it] = 0;
+      ke[it] = 0;
+      if (row < rows) {
+        const int64_t smp = row / n, i = row - smp * n;
+        base[it] = smp * n * WID + ch * 4;
+        hv[it] = *reinterpret_cast<const float4 *>(hin + row * WID + ch * 4);
+        kb[it] = gptr[i];
+        ke[it] = gptr[i + 1];
+      }
+    }
+#pragma unroll
+    for (int p2 = 0; p2 < 4; p2 += 2) {
+      // first NB0 neighbours of two items: indices clamped to entry 0 where absent
+      // (a valid node of the same sample), skipped in the sum
+      float4 v[2][NB0];
+      float wk[2][NB0];
+#pragma unroll
+      for (int it = 0; it < 2; it++)
+#pragma unroll
+        for (int jj = 0; jj < NB0; jj++) {
+          const int32_t kk = kb[p2 + it] + jj;
+          const int32_t kc = kk < ke[p2 + it] ? kk : 0;
+          wk[it][jj] = gw[kc];
+          v[it][jj] = *reinterpret_cast<const float4 *>(hin + base[p2 + it] + (int64_t)gcol[kc] * WID);
+        }
+#pragma unroll
+      for (int it = 0; it < 2; it++)
+#pragma unroll
+        for (int jj = 0; jj < NB0; jj++)
+          if (kb[p2 + it] + jj < ke[p2 + it]) {
+            av[p2 + it].x += wk[it][jj] * v[it][jj].x;
+            av[p2 + it].y += wk[it][jj] * v[it][jj].y;
+            av[p2 + it].z += wk[it][jj] * v[it][jj].z;
+            av[p2 + it].w += wk[it][jj] * v[it][jj].w;
+          }
+    }
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      for (int32_t k = kb[it] + NB0; k < ke[it]; k++) {
+        const float wv = gw[k];
+        const float4 v = *reinterpret_cast<const float4 *>(hin + base[it] + (int64_t)gcol[k] * WID);
+        av[it].x += wv * v.x;
+        av[it].y += wv * v.y;
+        av[it].z += wv * v.z;
+        av[it].w += wv * v.w;
+      }
+      const int q = tid + it * TB, r = q >> 4, ch = q & 15;
+      bf16_t *ar = &As[r * ASTR + ch * 4];
+      ar[0] = f2bf(hv[it].x);
+      ar[1] = f2bf(hv[it].y);
+      ar[2] = f2bf(hv[it].z);
+      ar[3] = f2bf(hv[it].w);
+      ar[WID + 0] = f2bf(av[it].x - hv[it].x);
+      ar[WID + 1] = f2bf(av[it].y - hv[it].y);
+      ar[WID + 2] = f2bf(av[it].z - hv[it].z);
+      ar[WID + 3] = f2bf(av[it].w - hv[it].w);
+    }
+    __syncthreads();
+    f32x4 acc[NTL];
+#pragma unroll
+    for (int t = 0; t < NTL; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < KS; k++) {
+      bf16x4 a;
+      const bf16_t *ap = &As[(w * 16 + c) * ASTR + k * 16 + 4 * g];
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) a[jj] = (short)ap[jj];
+#pragma unroll
+      for (int t = 0; t < NTL; t++) {
+        const uint64_t bv = Bf[(k * NTL + t) * 64 + l];
+        bf16x4 b;
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) b[jj] = (short)(uint16_t)(bv >> (16 * jj));
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      }
+    }
+    // epilogue: lane holds rows w*16 + 4g + jj, columns t*16 + c; h_l re-read from
+    // memory (the tile was just staged, so from cache)
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      const int64_t row = row0 + w * 16 + 4 * g + jj;
+      const bool ok = row < rows;
+      float ud = 0.f;
+#pragma unroll
+      for (int t = 0; t < NTL; t++) {
+        const int col = t * 16 + c;
+        const float h = ok ? hin[row * WID + col] : 0.f;
+        const float hn = h + fmaxf(acc[t][jj] + bcol[t], 0.f);
+        if (!wd) {
+          if (ok) hout[row * WID + col] = hn;
+        } else {
+          ud += hn * wdc[t];
+        }
+      }
+      if (wd) {
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) ud += __shfl_xor(ud, o, 16);
+        if (c == 0 && ok) u[row] = ud + bd[0];
+      }
+    }
+  }
+}
+
 // The 8 input features of a 2D family with one source field and a Dirichlet value
 // per side of the bounding box, computed on the device (row r = sample r / n, node
 // r % n): f / 5, asinh(f / 2), side value k * exp(-distance to side k / delta)
@@ -259,12 +420,12 @@ __global__ void k_pad_features(int64_t rows, const float *__restrict__ x16, bf16
 }
 
 // ---------------------------------------------------------- context -------
-// Channels 8..15 of the network input: two features smoothed on the sample's graph.
-// One block per (sample, feature); the sample's field lives in LDS (two buffers of n
-// floats) for all steps, so only the graph is read from memory.
-constexpr int SMOOTH_MAXN = 20000;  // nodes per sample held in LDS (2 x 80 KB)
-constexpr int SMOOTH_TB = 1024;
-__device__ constexpr int SMOOTH_STEPS[4] = {16, 64, 160, 324};
+// Channels 8..15 of the network input: two features smoothed on the graph by
+// K = 16, 64, 160, 324 steps of y <- (y + W y) / 2.  All samples share the graph, so
+// the fields of a launch are kept node-major, Y[i][col] with col = c * ns + s (feature
+// c, sample s): a wave takes one node and 64 columns, the node's graph entries are
+// wave-uniform and every field load is a coalesced row segment.  One launch per step.
+constexpr int SMOOTH_STEPS_H[4] = {16, 64, 160, 324};
 constexpr int SMOOTH_SRC[2] = {1, 0};  // asinh(f/2), f/5
 
 // x16[r, 0..7] = x[r, 0..7]
@@ -276,53 +437,69 @@ __global__ void k_expand_copy(int64_t rows, const float *__restrict__ x, float *
   if (k < NFEAT) x16[q] = x[r * NFEAT + k];
 }
 
-__global__ void __launch_bounds__(SMOOTH_TB) k_smooth_lds(int64_t n, const int32_t *__restrict__ gptr,
-                                                          const int32_t *__restrict__ gcol,
-                                                          const float *__restrict__ gw, float *__restrict__ x16) {
-  __shared__ float buf[2][SMOOTH_MAXN];
-  const int64_t s = blockIdx.x;
-  const int c = blockIdx.y;
-  const int src = c == 0 ? SMOOTH_SRC[0] : SMOOTH_SRC[1];
-  float *xs = x16 + s * n * KPAD;
-  for (int64_t i = threadIdx.x; i < n; i += SMOOTH_TB) buf[0][i] = xs[i * KPAD + src];
-  __syncthreads();
-  int cur = 0, snap = 0;
-  for (int step = 1; step <= SMOOTH_STEPS[3]; step++) {
-    for (int64_t i = threadIdx.x; i < n; i += SMOOTH_TB) {
-      float acc = 0.f;
-      for (int32_t k = gptr[i]; k < gptr[i + 1]; k++) acc += gw[k] * buf[cur][gcol[k]];
-      buf[1 - cur][i] = 0.5f * (buf[cur][i] + acc);
-    }
-    __syncthreads();
-    cur = 1 - cur;
-    if (step == SMOOTH_STEPS[snap]) {
-      for (int64_t i = threadIdx.x; i < n; i += SMOOTH_TB) xs[i * KPAD + NFEAT + 4 * c + snap] = buf[cur][i];
-      snap++;
-    }
+// Y[i][c ns + s] = x16[s n + i][src_c]  (to node-major; row stride Cs >= 2 ns, a multiple
+// of 4, padding columns zero), or back: x16[..][slot_c] = Y
+template <bool TO_NODE_MAJOR>
+__global__ void k_smooth_io(int64_t n, int64_t ns, int64_t Cs, float *__restrict__ x16, float *__restrict__ Y,
+                            int snap) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * Cs) return;
+  const int64_t i = q / Cs, col = q - i * Cs;
+  if (col >= 2 * ns) {
+    if (TO_NODE_MAJOR) Y[q] = 0.f;
+    return;
   }
-}
-
-// the same on graphs too large for LDS: one launch per step, ping-pong in memory
-__global__ void k_smooth_step(int64_t rows, int64_t n, const int32_t *__restrict__ gptr,
-                              const int32_t *__restrict__ gcol, const float *__restrict__ gw,
-                              const float *__restrict__ yin, float *__restrict__ yout) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  const int64_t s = r / n, i = r - s * n;
-  const float *ys = yin + s * n;
-  float acc = 0.f;
-  for (int32_t k = gptr[i]; k < gptr[i + 1]; k++) acc += gw[k] * ys[gcol[k]];
-  yout[r] = 0.5f * (yin[r] + acc);
-}
-
-__global__ void k_channel(int64_t rows, const float *__restrict__ x16, int from, float *__restrict__ y, float *x16o,
-                          int to) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  if (x16o)
-    x16o[r * KPAD + to] = y[r];
+  const int c = (int)(col / ns);
+  const int64_t s = col - (int64_t)c * ns;
+  float *xr = x16 + (s * n + i) * KPAD;
+  if (TO_NODE_MAJOR)
+    Y[q] = xr[c == 0 ? SMOOTH_SRC[0] : SMOOTH_SRC[1]];
   else
-    y[r] = x16[r * KPAD + from];
+    xr[NFEAT + 4 * c + snap] = Y[q];
+}
+
+// one step: Yo[i][:] = (Yi[i][:] + sum_j w_ij Yi[j][:]) / 2.  A wave takes one node (its
+// graph entries are wave-uniform) and walks the row in float4 columns; the neighbour
+// rows are loaded four at a time and summed in entry order.
+__global__ void __launch_bounds__(256) k_smooth_step(int64_t n, int64_t C4, const int32_t *__restrict__ gptr,
+                                                     const int32_t *__restrict__ gcol,
+                                                     const float *__restrict__ gw, const float4 *__restrict__ Yi,
+                                                     float4 *__restrict__ Yo) {
+  const int64_t i = __builtin_amdgcn_readfirstlane(
+      (int)(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)));
+  if (i >= n) return;
+  const int32_t k0 = gptr[i], k1 = gptr[i + 1];
+  for (int64_t c4 = (int64_t)blockIdx.y * 64 + (threadIdx.x & 63); c4 < C4; c4 += (int64_t)gridDim.y * 64) {
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    int32_t k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      float wk[4];
+      float4 v[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        wk[jj] = gw[k + jj];
+        v[jj] = Yi[(int64_t)gcol[k + jj] * C4 + c4];
+      }
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        acc.x += wk[jj] * v[jj].x;
+        acc.y += wk[jj] * v[jj].y;
+        acc.z += wk[jj] * v[jj].z;
+        acc.w += wk[jj] * v[jj].w;
+      }
+    }
+    for (; k < k1; k++) {
+      const float wv = gw[k];
+      const float4 v = Yi[(int64_t)gcol[k] * C4 + c4];
+      acc.x += wv * v.x;
+      acc.y += wv * v.y;
+      acc.z += wv * v.z;
+      acc.w += wv * v.w;
+    }
+    const float4 y = Yi[i * C4 + c4];
+    Yo[i * C4 + c4] = make_float4(0.5f * (y.x + acc.x), 0.5f * (y.y + acc.y), 0.5f * (y.z + acc.z),
+                                  0.5f * (y.w + acc.w));
+  }
 }
 
 // u_r = h_L[r] . wd + bd (one wave per row); optional loss: du = 2 (u - t) / count,
@@ -499,30 +676,22 @@ void ensure_cap(vfvm_mpnn &m, int64_t rows) {
 // network input of `ns` samples: x16 = [x (8 features), smoothed context (8 channels)]
 void expand(vfvm_mpnn &m, const float *x, float *x16, int64_t ns) {
   hipStream_t st = m.ctx->stream;
-  const int64_t rows = ns * m.n;
+  const int64_t rows = ns * m.n, Cs = (2 * ns + 3) / 4 * 4, C4 = Cs / 4;
   hipLaunchKernelGGL(k_expand_copy, dim3(nb(rows * KPAD, 256)), dim3(256), 0, st, rows, x, x16);
-  if (m.n <= SMOOTH_MAXN) {
-    hipLaunchKernelGGL(k_smooth_lds, dim3((unsigned)ns, 2), dim3(SMOOTH_TB), 0, st, m.n, (const int32_t *)m.gptr.p,
-                       (const int32_t *)m.gcol.p, (const float *)m.gw.p, x16);
-  } else {
-    m.sm0.alloc((size_t)std::max<int64_t>(rows, (int64_t)m.sm0.n));
-    m.sm1.alloc((size_t)std::max<int64_t>(rows, (int64_t)m.sm1.n));
-    const int steps[4] = {16, 64, 160, 324};
-    for (int c = 0; c < 2; c++) {
-      hipLaunchKernelGGL(k_channel, dim3(nb(rows, 256)), dim3(256), 0, st, rows, (const float *)x16, SMOOTH_SRC[c],
-                         m.sm0.p, (float *)nullptr, 0);
-      float *a = m.sm0.p, *b = m.sm1.p;
-      int snap = 0;
-      for (int step = 1; step <= steps[3]; step++) {
-        hipLaunchKernelGGL(k_smooth_step, dim3(nb(rows, 256)), dim3(256), 0, st, rows, m.n, (const int32_t *)m.gptr.p,
-                           (const int32_t *)m.gcol.p, (const float *)m.gw.p, (const float *)a, b);
-        std::swap(a, b);
-        if (step == steps[snap]) {
-          hipLaunchKernelGGL(k_channel, dim3(nb(rows, 256)), dim3(256), 0, st, rows, (const float *)x16, 0, a, x16,
-                             NFEAT + 4 * c + snap);
-          snap++;
-        }
-      }
+  m.sm0.alloc((size_t)std::max<int64_t>(m.n * Cs, (int64_t)m.sm0.n));
+  m.sm1.alloc((size_t)std::max<int64_t>(m.n * Cs, (int64_t)m.sm1.n));
+  hipLaunchKernelGGL((k_smooth_io<true>), dim3(nb(m.n * Cs, 256)), dim3(256), 0, st, m.n, ns, Cs, x16, m.sm0.p, 0);
+  float *a = m.sm0.p, *b = m.sm1.p;
+  const unsigned gy = (unsigned)std::max<int64_t>(1, std::min<int64_t>(C4 / 256, 64));
+  int snap = 0;
+  for (int step = 1; step <= SMOOTH_STEPS_H[3]; step++) {
+    hipLaunchKernelGGL(k_smooth_step, dim3((nb(m.n, 4) + 7) / 8 * 8, gy), dim3(256), 0, st, m.n, C4, (const int32_t *)m.gptr.p,
+                       (const int32_t *)m.gcol.p, (const float *)m.gw.p, (const float4 *)a, (float4 *)b);
+    std::swap(a, b);
+    if (step == SMOOTH_STEPS_H[snap]) {
+      hipLaunchKernelGGL((k_smooth_io<false>), dim3(nb(m.n * Cs, 256)), dim3(256), 0, st, m.n, ns, Cs, x16, a,
+                         snap);
+      snap++;
     }
   }
   VFVM_HIP_CHECK(hipGetLastError());
@@ -551,6 +720,31 @@ void forward(vfvm_mpnn &m, int64_t rows) {
     r.cat = m.cat.p + (size_t)(l + 1) * rows * CW;  // next layer's h half (cat_L unused by the decoder)
     hipLaunchKernelGGL((k_gemm<CW, WID, EPI_RES>), dim3(nb(rows, 64)), dim3(TB), 0, st, rows, (const bf16_t *)cl,
                        (const bf16_t *)(m.Wb.p + (size_t)l * CW * WID), r);
+  }
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+// inference: encoder, then the fused layers ping-ponging hf / dh; u = network output
+void forward_infer(vfvm_mpnn &m, int64_t rows) {
+  hipStream_t st = m.ctx->stream;
+  const int L = m.nlayers;
+  hipLaunchKernelGGL(k_pad_features, dim3(nb(rows * KPAD, 256)), dim3(256), 0, st, rows, (const float *)m.x16.p,
+                     m.xb.p);
+  GemmEpi e{};
+  e.bias = m.theta.p + m.off_be;
+  e.hf = m.hf.p;
+  hipLaunchKernelGGL((k_gemm<KPAD, WID, EPI_ENC>), dim3(nb(rows, 64)), dim3(TB), 0, st, rows, (const bf16_t *)m.xb.p,
+                     (const bf16_t *)m.Web.p, e);
+  const unsigned grid = (unsigned)std::min<int64_t>((nb(rows, 64) + 7) / 8 * 8, 2048);  // a multiple of 8
+  float *hin = m.hf.p, *hout = m.dh.p;
+  for (int l = 0; l < L; l++) {
+    const bool last = l == L - 1;
+    hipLaunchKernelGGL(k_layer_infer, dim3(grid), dim3(TB), 0, st, rows, m.n, (const int32_t *)m.gptr.p,
+                       (const int32_t *)m.gcol.p, (const float *)m.gw.p, (const float *)hin,
+                       (const bf16_t *)(m.Wb.p + (size_t)l * CW * WID), (const float *)m.theta.p + m.off_b[l],
+                       last ? (float *)nullptr : hout, last ? (const float *)m.theta.p + m.off_wd : nullptr,
+                       last ? (const float *)m.theta.p + m.off_bd : nullptr, last ? m.u.p : (float *)nullptr);
+    std::swap(hin, hout);
   }
   VFVM_HIP_CHECK(hipGetLastError());
 }
@@ -752,10 +946,7 @@ int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, doubl
       ensure_cap(*m, rows);
       m->x.upload(features + s0 * m->n * NFEAT, (size_t)rows * NFEAT, st);
       expand(*m, m->x.p, m->x16.p, ns);
-      forward(*m, rows);
-      hipLaunchKernelGGL(k_decode, dim3(nb(rows, 4)), dim3(TB), 0, st, rows, (const float *)m->hf.p,
-                         (const float *)m->theta.p + m->off_wd, (const float *)m->theta.p + m->off_bd, m->u.p,
-                         (const float *)nullptr, 0.f, (float *)nullptr, (float *)nullptr);
+      forward_infer(*m, rows);
       u.resize((size_t)rows);
       m->u.download(u.data(), (size_t)rows, st);
       VFVM_HIP_CHECK(hipStreamSynchronize(st));
@@ -785,10 +976,7 @@ int vfvm_mpnn_infer_sides(vfvm_mpnn *m, int64_t nsamples, const double *f, const
                          (const double *)m->fin.p, (const double *)m->sides.p, (const float *)m->xy.p,
                          (const float *)m->dist.p, inv_delta, m->x.p);
       expand(*m, m->x.p, m->x16.p, ns);
-      forward(*m, rows);
-      hipLaunchKernelGGL(k_decode, dim3(nb(rows, 4)), dim3(TB), 0, st, rows, (const float *)m->hf.p,
-                         (const float *)m->theta.p + m->off_wd, (const float *)m->theta.p + m->off_bd, m->u.p,
-                         (const float *)nullptr, 0.f, (float *)nullptr, (float *)nullptr);
+      forward_infer(*m, rows);
       u.resize((size_t)rows);
       m->u.download(u.data(), (size_t)rows, st);
       VFVM_HIP_CHECK(hipStreamSynchronize(st));

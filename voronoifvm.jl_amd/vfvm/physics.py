@@ -144,16 +144,22 @@ def Neumann(species=1, region=1, value=0.0):
 
 @dataclass
 class Physics:
-    """VoronoiFVM.Physics(; flux, reaction, storage, source, bcondition)."""
+    """VoronoiFVM.Physics(; flux, reaction, storage, source, bcondition).
+
+    embed_source: the source callback returns embedparam(node) * f, so that
+    solve(...; embed) ramps the source with the embedding parameter."""
     flux: Optional[object] = None
     reaction: Optional[object] = None
     storage: Optional[object] = None
     source: Optional[Union[np.ndarray, Callable]] = None
     bcondition: list = field(default_factory=list)
+    embed_source: bool = False
 
     def param_block(self, nspec):
         p = np.zeros(16)
         flux = self.flux.fill(p, nspec) if self.flux is not None else FLUX_NONE
         react = self.reaction.fill(p, nspec) if self.reaction is not None else REACT_NONE
         stor = self.storage.fill(p, nspec) if self.storage is not None else STOR_NONE
+        if self.embed_source:
+            p[12] = 1.0
         return flux, react, stor, p
