@@ -244,7 +244,8 @@ def cfg2_control(precision):
 
 def run_cfg2(args):
     """Bipolar drift-diffusion on 512 x 512 nodes: a bias sweep 0..1 V in 20 continuation
-    steps (21 Newton solves, the first at zero bias), GMRES(30) + 3x3 block Jacobi."""
+    steps (21 Newton solves, the first at zero bias), BiCGStab + 3x3 block Jacobi (GMRES(30)
+    with block Jacobi stalls at this size: its restarted Krylov space is too small)."""
     import cases
     n = 512
     t0 = time.perf_counter()
@@ -274,9 +275,10 @@ def run_cfg2(args):
         if not s.history.residual_norm <= 1e-10:
             raise RuntimeError(f"{label}: a bias step ended with ||F||_inf = {s.history.residual_norm:.3e}")
         res[label] = dict(seconds=el, solves_per_s=nsolve * K / el, U=np.ascontiguousarray(vfvm.dofs(U)),
-                          newton=s.history.niter, krylov=s.history.nlin)
+                          newton=s.history.niter, krylov=s.history.nlin, fp64_fallbacks=s.history.fp64_fallbacks)
         log(f"{label}: {K} sweeps x {nsolve} solves in {el:.3f}s -> {nsolve * K / el:.2f} solves/s, newton "
-            f"{s.history.niter}, krylov {s.history.nlin} per sweep")
+            f"{s.history.niter}, krylov {s.history.nlin}, linear solves done in FP64 after an FP32 failure "
+            f"{s.history.fp64_fallbacks} per sweep")
     diff = float(np.abs(res["fp32_refine"]["U"] - res["plain_fp64"]["U"]).max())
     b_spmv32 = probe_bytes("spmv", probes[_capi.KERNEL_SPMV32], sv=4)
     b_asm = probe_bytes("assemble", probes[_capi.KERNEL_ASSEMBLE], nspec=3, transient=False)
@@ -287,8 +289,10 @@ def run_cfg2(args):
                 plain_gpu_newton_solves_per_s=res["plain_fp64"]["solves_per_s"],
                 speedup_vs_plain=res["fp32_refine"]["solves_per_s"] / res["plain_fp64"]["solves_per_s"],
                 max_abs_diff_fp32_vs_fp64=diff, roofline=rl["spmv32"], roofline_assemble=rl["assemble"],
+                newton_iterations_per_sweep=res["fp32_refine"]["newton"],
+                fp64_fallbacks_per_sweep=res["fp32_refine"]["fp64_fallbacks"],
                 config=dict(workload=f"cfg2: bipolar drift-diffusion (psi, n, p), {n}x{n} nodes, lambda=1e-2, "
-                                     f"SG fluxes, bias 0..1 V in 20 continuation steps, GMRES(30) + 3x3 "
+                                     f"SG fluxes, bias 0..1 V in 20 continuation steps, BiCGStab + 3x3 "
                                      f"block Jacobi, FP32 inside FP64 Newton (plain: FP64 Krylov); a step is "
                                      f"one sweep of {nsolve} solves", unknowns=ndof, nnz=nnz),
                 dtype="f64+f32")
@@ -544,11 +548,14 @@ def run_cfg4(args):
                           "batched assembly")
     import ctypes as C
     v = C.c_double()
-    rows = 16 * fam.grid.num_nodes
+    rows = 256 * fam.grid.num_nodes  # one inference batch
     _capi.check(_capi.lib().vfvm_mpnn_time_layer(net._h, rows, 20, C.byref(v)))
-    t_gemm = v.value
-    tflops = 2.0 * rows * 128 * 64 / (t_gemm * 1e-3) / 1e12
-    gemm_bytes = rows * (128 * 2 + 64 * 4 * 3 + 64 * 2)
+    t_layer = v.value
+    layer_bytes = rows * 64 * 4 * 2  # h_l read once, h_{l+1} written once (FP32, 64 wide)
+    layer_gbs = layer_bytes / (t_layer * 1e-3) / 1e9
+    layer_tflops = 2.0 * rows * 128 * 64 / (t_layer * 1e-3) / 1e12
+    log(f"inference layer (aggregation + 128x64 MFMA GEMM) over {rows} rows: {t_layer * 1e3:.1f} us, "
+        f"{layer_gbs:.0f} GB/s, {layer_tflops:.1f} TFLOP/s")
     return dict(value=sur_rate, ms_per_step=1e3 * t_sur / K, steps=K, plain_gpu_newton_solves_per_s=plain_rate,
                 speedup_vs_plain=sur_rate / plain_rate, batched_fp32_from_zero_solves_per_s=zero_rate,
                 batched_fp64_from_zero_solves_per_s=nsol / t_zero64,
@@ -563,11 +570,13 @@ def run_cfg4(args):
                 surrogate=dict(train_samples=int(meta["samples"]), epochs=int(meta["epochs"]),
                                final_train_mse=float(meta["final_loss"])),
                 roofline=rl_spmv, roofline_assemble=rl_asm,
-                roofline_mpnn_gemm=dict(bound="mfma", achieved=round(tflops, 2), peak=2500.0, unit="TFLOP/s",
-                                        frac=round(tflops / 2500.0, 5), traffic=read_traffic("cfg4_mpnn_gemm"),
-                                        algorithmic_bytes=int(gemm_bytes),
-                                        hbm_GBs=round(gemm_bytes / (t_gemm * 1e-3) / 1e9, 1),
-                                        avg_ms=round(t_gemm, 5), rows=rows),
+                roofline_mpnn_layer=dict(bound="hbm", achieved=round(layer_gbs, 1), peak=8000.0, unit="GB/s",
+                                         frac=round(layer_gbs / 8000.0, 4), traffic=read_traffic("cfg4_mpnn_layer"),
+                                         traffic_note="mean over the four layers of one inference; the last "
+                                                      "writes u only",
+                                         algorithmic_bytes=int(layer_bytes), avg_ms=round(t_layer, 5),
+                                         mfma_tflops=round(layer_tflops, 2), rows=rows,
+                                         kernel="k_layer_infer: graph aggregation + 128x64 BF16 MFMA GEMM + residual"),
                 config=dict(workload="cfg4: 129x129 nonlinear Poisson (eps=1e-2, GRF source ell=0.1 amp=5, side "
                                      "values U[-1,1]), 1,024 held-out seeds in batches of 256: MPNN initial guess "
                                      "(4 layers, width 64, BF16 MFMA) + batched BiCGStab/Jacobi FP32 inside FP64 "

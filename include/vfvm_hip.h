@@ -111,6 +111,10 @@ typedef struct {
   double last_update_norm;
   double residual_norm; /* ||F||_inf over non-penalty rows at the returned solution */
   double updatenorm[64];
+  /* linear solves run or finished in FP64 although FP32 refinement was requested: an
+   * FP32 refinement that fails hands over to FP64, and the rest of the call (Newton
+   * solve, or time / parameter sweep) then solves in FP64 directly */
+  int32_t fp64_fallbacks, pad;
 } vfvm_history_t;
 
 typedef struct {
@@ -315,7 +319,9 @@ int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, doubl
  * side value k * exp(-distance to side k / sqrt(eps / 2)), x, y. */
 int vfvm_mpnn_infer_sides(vfvm_mpnn *m, int64_t nsamples, const double *f, const double *sides, double eps,
                           double *out);
-/* benchmark helper: average device time in ms of one hidden-layer GEMM over `rows` rows */
+/* benchmark helper: average device time in ms of one inference hidden layer (graph
+ * aggregation fused with the 128 x 64 GEMM, residual add; the kernel inference runs)
+ * over `rows` rows (whole samples of the bound grid: rows = samples x nodes) */
 int vfvm_mpnn_time_layer(vfvm_mpnn *m, int64_t rows, int reps, double *avg_ms);
 /* loss and gradient for one batch (checked against the CPU oracle) */
 int vfvm_mpnn_loss_grad(vfvm_mpnn *m, int64_t nsamples, const float *features, const float *targets,

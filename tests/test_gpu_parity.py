@@ -307,3 +307,25 @@ def test_continuation_with_embedded_source_matches_oracle():
                           source=f.reshape(1, -1), oracle=False)
     Ud = vfvm.solve(plain, inival=0.0, control=ctrl)
     assert np.abs(vfvm.dofs(sol.u[-1]) - vfvm.dofs(Ud)).max() <= SOL_ATOL
+
+
+def test_fp32_refinement_hands_over_to_fp64():
+    """cfg2 physics (lambda = 1e-2) on 257 x 257 nodes, bias 0..0.25 V: the FP32 inner
+    solves of the block-Jacobi BiCGStab stagnate at this conditioning.  The failed
+    refinement restores x and finishes in FP64, the rest of the sweep solves in FP64
+    directly, and the result equals the plain FP64 sweep."""
+    _print("case fp32 hand-over dd 257")
+    sys, _ = cases.drift_diffusion_2d(257, lambda2=1e-4, oracle=False)
+    U0 = cases.dd_initial(sys.grid)
+    out = {}
+    for precision in ("fp64", "fp32"):
+        ctrl = vfvm.SolverControl(krylov="bicgstab", precon="blockjacobi", precision=precision, reltol_linear=1e-6,
+                                  abstol_linear=0.0, maxiters_linear=6000, inner_reltol=1e-3, max_refinements=12,
+                                  Δp=0.05, Δp_min=0.05, Δp_max=0.05)
+        sol = vfvm.solve(sys, inival=U0, control=ctrl, embed=np.linspace(0.0, 0.25, 6), store_all=False)
+        h = sys.history
+        _print(precision, "newton", h.niter, "krylov", h.nlin, "fp64 fallbacks", h.fp64_fallbacks)
+        assert h.residual_norm <= 1e-10
+        out[precision] = (vfvm.dofs(sol.u[-1]).copy(), h.fp64_fallbacks)
+    assert out["fp64"][1] == 0 and out["fp32"][1] >= 1
+    assert np.abs(out["fp32"][0] - out["fp64"][0]).max() <= SOL_ATOL

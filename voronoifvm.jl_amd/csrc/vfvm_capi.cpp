@@ -265,6 +265,7 @@ int newton_device(vfvm_sys &s, double time, double tstep, double lambda, const v
     int lrc = s.lin.solve(s.d_J.p, s.d_F.p, s.d_dU.p, c, &ls);
     (void)lrc;  // like the reference, an unconverged Krylov solve still yields an update
     h.nlin += ls.iters;
+    h.fp64_fallbacks += ls.fp64_fallback;
     launch_newton_update(s.ndof, s.d_U.p, s.d_dU.p, damp, s.d_partials.p, s.d_ticket.p, s.d_red.p, st);
     Readback r = read_norms(s);
     if (r.flags & 1) {
@@ -669,6 +670,7 @@ int vfvm_linsolve(vfvm_sys *s, const double *b, double *x, const vfvm_control_t 
     else
       vfvm_default_control(&c);
     check_control(c, *s);
+    s->lin.fp32_off = false;  // each solve or sweep starts by trying FP32 again
     if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
     hipStream_t st = s->ctx->stream;
     s->d_F.upload(b, (size_t)s->ndof, st);
@@ -691,6 +693,7 @@ int vfvm_newton(vfvm_sys *s, double *U, const double *Uold, double time, double 
     else
       vfvm_default_control(&c);
     check_control(c, *s);
+    s->lin.fp32_off = false;  // each solve or sweep starts by trying FP32 again
     if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
     hipStream_t st = s->ctx->stream;
     s->d_Uold.upload(Uold, (size_t)s->ndof, st);
@@ -711,6 +714,7 @@ int vfvm_solve_fixed_steps(vfvm_sys *s, const double *inival, int64_t nl, const 
     else
       vfvm_default_control(&c);
     check_control(c, *s);
+    s->lin.fp32_off = false;  // each solve or sweep starts by trying FP32 again
     if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
     hipStream_t st = s->ctx->stream;
     auto t0 = std::chrono::steady_clock::now();
@@ -721,6 +725,7 @@ int vfvm_solve_fixed_steps(vfvm_sys *s, const double *inival, int64_t nl, const 
     auto account = [&](const vfvm_history_t &h) {
       tot.niter += h.niter;
       tot.nlin += h.nlin;
+      tot.fp64_fallbacks += h.fp64_fallbacks;
       tot.residual_norm = std::max(tot.residual_norm, h.residual_norm);
       tot.last_update_norm = h.last_update_norm;
       if (!h.converged) tot.converged = 0;
@@ -785,6 +790,7 @@ int vfvm_solve_transient(vfvm_sys *s, const double *inival, int64_t nl, const do
     else
       vfvm_default_control(&c);
     check_control(c, *s);
+    s->lin.fp32_off = false;  // each solve or sweep starts by trying FP32 again
     const vfvm_stepcontrol_t sc = *steps;
     if (!(sc.d_min > 0) || !(sc.d_max >= sc.d_min) || !(sc.d_decrease > 0 && sc.d_decrease < 1))
       throw Error(VFVM_ERR_ARG, "bad step control");
@@ -797,6 +803,7 @@ int vfvm_solve_transient(vfvm_sys *s, const double *inival, int64_t nl, const do
     auto account = [&](const vfvm_history_t &h) {
       tot.niter += h.niter;
       tot.nlin += h.nlin;
+      tot.fp64_fallbacks += h.fp64_fallbacks;
       tot.residual_norm = std::max(tot.residual_norm, h.residual_norm);
       tot.last_update_norm = h.last_update_norm;
     };

@@ -144,9 +144,11 @@ class LinearSolver {
   V *iluy_ptr();
   template <class V, int NDOT, int STAGE>
   void apply_op(const V *x, V *y, const V *w0, const V *w1, KState *stp, const double *J);
+  // stag_window > 0: stop (unconverged) once the residual has not fallen below 0.9 x its
+  // best value for stag_window iterations
   template <class V, class VBin, class VXo>
   int bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_control_t &c, double rtol, double atol, int maxit,
-               int &iters);
+               int &iters, int stag_window = 0);
   template <class V, class VBin, class VXo>
   int gmres(const VBin *b, VXo *x, const double *J, const vfvm_control_t &c, double rtol, double atol, int maxit,
             int &iters);
@@ -167,6 +169,9 @@ class LinearSolver {
   DevBuf<double> xsave;       // solution before the latest FP32 correction
  public:
   int64_t fallbacks = 0;      // linear solves finished in FP64 after an FP32 failure
+  // set when an FP32 refinement fails: later solves with FP32 requested run in FP64 until
+  // the owner clears it (at the start of each Newton solve or time/parameter sweep)
+  bool fp32_off = false;
 
  private:
   bool sell_nt[2] = {false, false};

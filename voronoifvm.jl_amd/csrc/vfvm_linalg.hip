@@ -780,6 +780,12 @@ void LinearSolver::ensure_work(int precision, int krylov, int m) {
 
 // scaling, FP32 copy and preconditioner set-up for the current Jacobian values
 void LinearSolver::prepare(const double *J, const vfvm_control_t &c, bool refactor) {
+  if (fp32_off && c.precision == VFVM_PREC_FP32_REFINE) {  // FP32 switched off: the FP64 operator
+    vfvm_control_t c64 = c;
+    c64.precision = VFVM_PREC_FP64;
+    prepare(J, c64, refactor);
+    return;
+  }
   ensure_work(c.precision, c.krylov, c.gmres_restart);
   const bool fp32 = c.precision == VFVM_PREC_FP32_REFINE;
   const bool blk = c.precon == VFVM_PRECON_BLOCKJACOBI && L.nspec > 1;
@@ -936,7 +942,7 @@ void LinearSolver::read_state(KState &h) {
 // BiCGStab on Ahat x = bhat in precision V; b is given in precision VB_ (already scaled)
 template <class V, class VBin, class VXo>
 int LinearSolver::bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_control_t &c, double rtol, double atol,
-                           int maxit, int &iters) {
+                           int maxit, int &iters, int stag_window) {
   int64_t n = L.ndof;
   Work<V> &W = work<V>();
   V *r = W.vecs[0].p, *r0 = W.vecs[1].p, *p = W.vecs[2].p, *v = W.vecs[3].p, *s = W.vecs[4].p, *t = W.vecs[5].p,
@@ -955,7 +961,8 @@ int LinearSolver::bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_co
   // solve need similar counts), then poll every check_every iterations
   const int every = std::max(1, c.check_every);
   int check = std::max(every, std::min(maxit, last_iters - 1));
-  int launched = 0;
+  int launched = 0, best_at = 0;
+  double best = -1.0;
   while (true) {
     for (int q = 0; q < check; q++) {
       hipLaunchKernelGGL((k_bicg_p<V>), dim3(g), dim3(VB), 0, stream, n, r, p, v, st.p);
@@ -976,6 +983,14 @@ int LinearSolver::bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_co
     read_state(h);
     if (h.done) break;
     if (launched >= maxit + check) break;
+    if (stag_window > 0) {
+      if (best < 0.0 || h.rr < 0.81 * best) {
+        best = h.rr;
+        best_at = h.iters;
+      } else if (h.iters - best_at >= stag_window) {
+        break;  // stagnated: the kernels of the remaining iterations are never launched
+      }
+    }
     check = every;
   }
   iters = h.iters;
@@ -1123,15 +1138,20 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
     double *rh = rres.p;
     hipLaunchKernelGGL((k_convert<double, double>), dim3(g), dim3(VB), 0, stream, n, (const double *)bhat.p, rh);
     const int maxref = std::max(1, c.max_refinements);
-    bool fp32_failed = false;
-    while (rn > tol && nref < maxref) {
+    // after an FP32 failure earlier in this Newton solve or sweep, go to FP64 directly
+    const bool fp32_skipped = fp32_off;  // prepare() already built the FP64 operator
+    bool fp32_failed = fp32_off;
+    // an FP32 inner solve that stops improving is cut short (the FP64 residual then
+    // decides between another round and the FP64 finish)
+    const int stag = std::max(200, c.maxiters_linear / 20);
+    while (!fp32_failed && rn > tol && nref < maxref) {
       hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(VB), 0, stream, n, (const double *)rh, rhs32.p);
       int it = 0;
       const double inner = std::max(c.inner_reltol, 1e-6);
       if (c.krylov == VFVM_KRYLOV_GMRES)
         gmres<float, float, float>(rhs32.p, r32, nullptr, c, inner, 0.0, c.maxiters_linear, it);
       else
-        bicgstab<float, float, float>(rhs32.p, r32, nullptr, c, inner, 0.0, c.maxiters_linear, it);
+        bicgstab<float, float, float>(rhs32.p, r32, nullptr, c, inner, 0.0, c.maxiters_linear, it, stag);
       iters += it;
       xsave.alloc((size_t)n);
       hipLaunchKernelGGL((k_convert<double, double>), dim3(g), dim3(VB), 0, stream, n, (const double *)x, xsave.p);
@@ -1151,6 +1171,7 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
         read_state(h);
         rn = std::sqrt(h.rr);
         fp32_failed = true;
+        fp32_off = true;
         break;
       }
       rn = rnew;
@@ -1161,7 +1182,7 @@ int LinearSolver::solve(const double *J, const double *b, double *x, const vfvm_
       // operator and preconditioner copies of the same Jacobian (no refactorisation)
       vfvm_control_t c64 = c;
       c64.precision = VFVM_PREC_FP64;
-      prepare(J, c64, false);
+      if (!fp32_skipped) prepare(J, c64, false);
       DevBuf<double> &d64 = fallback64;
       d64.alloc((size_t)n);
       int it = 0;

@@ -986,22 +986,22 @@ int vfvm_mpnn_infer_sides(vfvm_mpnn *m, int64_t nsamples, const double *f, const
   });
 }
 
-// average device time (ms) of one hidden-layer GEMM (rows x 128 times 128 x 64 on the matrix
-// cores, relu + residual epilogue) over `rows` rows of the current activations
+// average device time (ms) of one inference hidden layer (k_layer_infer, not the last)
+// over `rows` rows of the current hidden state
 int vfvm_mpnn_time_layer(vfvm_mpnn *m, int64_t rows, int reps, double *avg_ms) {
   if (!m || rows < 1 || reps < 1 || !avg_ms) return VFVM_ERR_ARG;
   return mguard(m->ctx, [&]() {
+    if (rows % m->n != 0) throw Error(VFVM_ERR_ARG, "rows must be a whole number of samples");
     hipStream_t st = m->ctx->stream;
     ensure_cap(*m, rows);
-    GemmEpi r{};
-    r.bias = m->theta.p + m->off_b[0];
-    r.hf = m->hf.p;
-    r.rbuf = m->rbuf.p;
-    r.cat = m->cat.p + (size_t)rows * CW;
+    const unsigned grid = (unsigned)std::min<int64_t>((nb(rows, 64) + 7) / 8 * 8, 2048);
     auto once = [&]() {
-      hipLaunchKernelGGL((k_gemm<CW, WID, EPI_RES>), dim3(nb(rows, 64)), dim3(TB), 0, st, rows,
-                         (const bf16_t *)m->cat.p, (const bf16_t *)m->Wb.p, r);
+      hipLaunchKernelGGL(k_layer_infer, dim3(grid), dim3(TB), 0, st, rows, m->n, (const int32_t *)m->gptr.p,
+                         (const int32_t *)m->gcol.p, (const float *)m->gw.p, (const float *)m->hf.p,
+                         (const bf16_t *)m->Wb.p, (const float *)m->theta.p + m->off_b[0], m->dh.p,
+                         (const float *)nullptr, (const float *)nullptr, (float *)nullptr);
     };
+    hipLaunchKernelGGL(k_fill_f, dim3(nb(rows * WID, 256)), dim3(256), 0, st, rows * WID, m->hf.p, 0.5f);
     once();
     hipEvent_t e0, e1;
     VFVM_HIP_CHECK(hipEventCreate(&e0));

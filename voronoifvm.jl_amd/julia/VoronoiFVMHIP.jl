@@ -51,6 +51,7 @@ mutable struct HistoryT
     time::Float64; tasm::Float64; tlinsolve::Float64
     last_update_norm::Float64; residual_norm::Float64
     updatenorm::NTuple{64, Float64}
+    fp64_fallbacks::Int32; pad::Int32
     HistoryT() = new()
 end
 

@@ -40,7 +40,8 @@ class ControlT(C.Structure):
 class HistoryT(C.Structure):
     _fields_ = [("niter", C.c_int32), ("nlin", C.c_int32), ("nlu", C.c_int32), ("converged", C.c_int32),
                 ("time", C.c_double), ("tasm", C.c_double), ("tlinsolve", C.c_double),
-                ("last_update_norm", C.c_double), ("residual_norm", C.c_double), ("updatenorm", C.c_double * 64)]
+                ("last_update_norm", C.c_double), ("residual_norm", C.c_double), ("updatenorm", C.c_double * 64),
+                ("fp64_fallbacks", C.c_int32), ("pad", C.c_int32)]
 
 
 class StepControlT(C.Structure):

@@ -34,6 +34,7 @@ class NewtonSolverHistory:
     time: float = 0.0
     updatenorm: tuple = ()
     residual_norm: float = float("nan")  # ||F||_inf over non-penalty rows at the solution
+    fp64_fallbacks: int = 0  # linear solves done in FP64 although FP32 refinement was requested
 
 
 class System:
@@ -301,7 +302,7 @@ def solve(sys: System, inival=0.0, control: Optional[SolverControl] = None, time
                                     C.byref(sc), _capi.dptr(out), max_out, _capi.dptr(out_t),
                                     _capi.dptr(out_u) if store_all else None, C.byref(nst), C.byref(hist))
         sys.history = NewtonSolverHistory(niter=hist.niter, nlin=hist.nlin, time=hist.time,
-                                          residual_norm=hist.residual_norm)
+                                          residual_norm=hist.residual_norm, fp64_fallbacks=hist.fp64_fallbacks)
         _capi.check(rc, sys.ctx.handle)
         n = min(int(nst.value) + 1, max_out)  # entries: start + accepted steps
         if store_all:
@@ -315,7 +316,7 @@ def solve(sys: System, inival=0.0, control: Optional[SolverControl] = None, time
     rc = L.vfvm_newton(h, _capi.dptr(U), _capi.dptr(U0), time, tstep, 0.0, C.byref(cc), C.byref(hist))
     sys.history = NewtonSolverHistory(nlin=hist.nlin, niter=hist.niter, time=hist.time,
                                       updatenorm=tuple(hist.updatenorm[:min(hist.niter, 64)]),
-                                      residual_norm=hist.residual_norm)
+                                      residual_norm=hist.residual_norm, fp64_fallbacks=hist.fp64_fallbacks)
     _capi.check(rc, sys.ctx.handle)
     return _from_dofs(sys, U)
 
