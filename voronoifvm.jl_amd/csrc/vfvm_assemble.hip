@@ -401,15 +401,15 @@ void launch_ns_g(const AsmArgs &a, hipStream_t s) {
   if (grid == 0) return;
   if (G == 1) {  // one node per lane: the LDS-staged kernel (SADJ: adjacency staged too)
     if (a.J)
-      hipLaunchKernelGGL((k_assemble_stream<NS, true, SADJ>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
+      VFVM_LAUNCH((k_assemble_stream<NS, true, SADJ>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
     else
-      hipLaunchKernelGGL((k_assemble_stream<NS, false, SADJ>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
+      VFVM_LAUNCH((k_assemble_stream<NS, false, SADJ>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
     return;
   }
   if (a.J)
-    hipLaunchKernelGGL((k_assemble<NS, G, true>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
+    VFVM_LAUNCH((k_assemble<NS, G, true>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
   else
-    hipLaunchKernelGGL((k_assemble<NS, G, false>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
+    VFVM_LAUNCH((k_assemble<NS, G, false>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
 }
 
 template <int NS>

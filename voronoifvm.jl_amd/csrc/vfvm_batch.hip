@@ -446,13 +446,13 @@ void bop(vfvm_batch &b, const V *x, V *y, const V *w0) {
   auto go = [&](auto ntc) {
     constexpr bool NT = decltype(ntc)::value;
     if constexpr (sizeof(V) == 4)
-      hipLaunchKernelGGL((k_bspmv<float, V, NDOT, STAGE, NT>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                         b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const float *)b.A32.p, x, y, w0, b.st.p, mwork,
-                         (const int64_t *)b.msize.p);
+      VFVM_LAUNCH((k_bspmv<float, V, NDOT, STAGE, NT>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                  b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const float *)b.A32.p, x, y, w0, b.st.p, mwork,
+                  (const int64_t *)b.msize.p);
     else
-      hipLaunchKernelGGL((k_bspmv<double, V, NDOT, STAGE, NT>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream,
-                         S, b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const double *)b.A64s.p, x, y, w0, b.st.p,
-                         mwork, (const int64_t *)b.msize.p);
+      VFVM_LAUNCH((k_bspmv<double, V, NDOT, STAGE, NT>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                  b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const double *)b.A64s.p, x, y, w0, b.st.p, mwork,
+                  (const int64_t *)b.msize.p);
   };
   if (b.sell_nt[sizeof(V) == 4 ? 0 : 1])
     go(std::true_type());

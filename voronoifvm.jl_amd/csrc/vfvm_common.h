@@ -1,12 +1,14 @@
 // Internal declarations shared by the library's translation units.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/vfvm_hip.h"
@@ -165,6 +167,14 @@ void launch_penalty_free_norm(int64_t nnodes, int nspec, const double *F, const 
 // see vfvm_probe_read): by the host where it knows the work at launch time, by
 // the batched SpMV itself (one atomic per live chunk into `dwork`), whose work
 // depends on which members are still iterating.
+// Event pair the next VFVM_LAUNCH hands to hipExtLaunchKernelGGL, which sets them to
+// the dispatch's own start and end times (set by KernelProbe::before, consumed by the
+// launch).  Host-thread local: a context's calls are serialised on one thread.
+inline std::pair<hipEvent_t, hipEvent_t> &pending_launch_events() {
+  static thread_local std::pair<hipEvent_t, hipEvent_t> p{nullptr, nullptr};
+  return p;
+}
+
 struct KernelProbe {
   static constexpr int NK = 4;
   static constexpr int NW = 4;
@@ -205,15 +215,22 @@ struct KernelProbe {
   }
   // device work counters of an armed class (null when not armed)
   unsigned long long *counters(int id) const { return k[id].on ? k[id].dwork : nullptr; }
+  // before/after bracket one launch of class `id`.  A launch made with VFVM_LAUNCH in
+  // between takes the pair and the events hold the kernel's own start and end; any other
+  // launch is bracketed by stream-ordered event records.
   void before(int id, hipStream_t s) {
     Slot &q = k[id];
-    if (q.on && q.used + 2 <= q.ev.size()) VFVM_HIP_CHECK(hipEventRecord(q.ev[q.used], s));
+    if (q.on && q.used + 2 <= q.ev.size()) {
+      VFVM_HIP_CHECK(hipEventRecord(q.ev[q.used], s));
+      pending_launch_events() = {q.ev[q.used], q.ev[q.used + 1]};
+    }
   }
   void after(int id, hipStream_t s, int64_t w0 = 0, int64_t w1 = 0, int64_t w2 = 0) {
     Slot &q = k[id];
     if (!q.on) return;
     if (q.used + 2 <= q.ev.size()) {
-      VFVM_HIP_CHECK(hipEventRecord(q.ev[q.used + 1], s));
+      if (pending_launch_events().first) VFVM_HIP_CHECK(hipEventRecord(q.ev[q.used + 1], s));  // not consumed
+      pending_launch_events() = {nullptr, nullptr};
       q.used += 2;
       q.work[0] += w0;
       q.work[1] += w1;
@@ -250,6 +267,19 @@ struct KernelProbe {
 };
 
 }  // namespace vfvm
+
+// hipLaunchKernelGGL, or, while a probe pair is pending, hipExtLaunchKernelGGL with that
+// pair as the dispatch's start and stop events
+#define VFVM_LAUNCH(K, G, B, SHM, ST, ...)                                                       \
+  do {                                                                                            \
+    auto &pe_ = vfvm::pending_launch_events();                                                    \
+    if (pe_.first) {                                                                              \
+      hipExtLaunchKernelGGL(K, G, B, SHM, ST, pe_.first, pe_.second, 0, __VA_ARGS__);             \
+      pe_ = {nullptr, nullptr};                                                                   \
+    } else {                                                                                      \
+      hipLaunchKernelGGL(K, G, B, SHM, ST, __VA_ARGS__);                                          \
+    }                                                                                             \
+  } while (0)
 
 struct vfvm_ctx {
   int device = 0;

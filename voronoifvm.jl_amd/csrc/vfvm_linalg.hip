@@ -909,11 +909,11 @@ void LinearSolver::apply_op(const V *x, V *y, const V *w0, const V *w1, KState *
   auto go = [&](auto ntc) {
     constexpr bool NT = decltype(ntc)::value;
     if constexpr (sizeof(V) == 4)
-      hipLaunchKernelGGL((k_sell_spmv<float, V, NDOT, STAGE, NT>), dim3(g), dim3(VB), 0, stream, L.ndof, L.sell_ptr,
-                         L.sell_col, (const float *)A32.p, x, y, w0, partials.p, stp);
+      VFVM_LAUNCH((k_sell_spmv<float, V, NDOT, STAGE, NT>), dim3(g), dim3(VB), 0, stream, L.ndof, L.sell_ptr,
+                  L.sell_col, (const float *)A32.p, x, y, w0, partials.p, stp);
     else
-      hipLaunchKernelGGL((k_sell_spmv<double, V, NDOT, STAGE, NT>), dim3(g), dim3(VB), 0, stream, L.ndof,
-                         L.sell_ptr, L.sell_col, (const double *)A64s.p, x, y, w0, partials.p, stp);
+      VFVM_LAUNCH((k_sell_spmv<double, V, NDOT, STAGE, NT>), dim3(g), dim3(VB), 0, stream, L.ndof, L.sell_ptr,
+                  L.sell_col, (const double *)A64s.p, x, y, w0, partials.p, stp);
   };
   if (sell_nt[sizeof(V) == 4 ? 0 : 1])
     go(std::true_type());
