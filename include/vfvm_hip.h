@@ -134,6 +134,12 @@ int vfvm_ctx_create(int device, vfvm_ctx **out);
 int vfvm_ctx_destroy(vfvm_ctx *ctx);
 /* last error message of this context (or of the last failed create when ctx is NULL) */
 const char *vfvm_last_error(vfvm_ctx *ctx);
+/* Device memory on the context's device, for callers that keep inputs and results in HBM
+ * between calls (the *_device entry points).  No counterpart in the reference.  copy:
+ * kind 0 host to device, 1 device to host, 2 device to device; returns when done. */
+int vfvm_device_alloc(vfvm_ctx *ctx, int64_t bytes, void **ptr);
+int vfvm_device_free(vfvm_ctx *ctx, void *ptr);
+int vfvm_device_copy(vfvm_ctx *ctx, void *dst, const void *src, int64_t bytes, int kind);
 
 /* System creation: grid upload + discretisation setup (dof map, Voronoi
  * factors, edgewise aggregation, boundary node factors, CSR pattern and slot
@@ -291,6 +297,13 @@ int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group);
 int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold, double time,
                       double tstep, double lambda, const vfvm_control_t *control,
                       vfvm_history_t *hist);
+/* The same with start values and solutions in device memory, packed in member order
+ * (member m's dofs at offsets dof_offsets[m] .. dof_offsets[m+1] of vfvm_batch_sizes):
+ * no host transfer of the unknowns.  No counterpart in the reference. */
+int vfvm_batch_newton_device(vfvm_batch *b, double *U_dev, const double *Uold_dev, double time, double tstep,
+                             double lambda, const vfvm_control_t *control, vfvm_history_t *hist);
+/* total dofs of the batch and (if not NULL) the nsys + 1 member offsets */
+int vfvm_batch_sizes(vfvm_batch *b, int64_t *ndof, int64_t *dof_offsets);
 
 /* ---- learned Newton initial guess (no counterpart in the reference) ----
  * A message-passing network on the Voronoi edge graph of `sys` (edge weights sigma/h,
@@ -319,6 +332,9 @@ int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, doubl
  * side value k * exp(-distance to side k / sqrt(eps / 2)), x, y. */
 int vfvm_mpnn_infer_sides(vfvm_mpnn *m, int64_t nsamples, const double *f, const double *sides, double eps,
                           double *out);
+/* the same with f, sides and out in device memory (out: nsamples x nnodes float64) */
+int vfvm_mpnn_infer_sides_device(vfvm_mpnn *m, int64_t nsamples, const double *f_dev, const double *sides_dev,
+                                 double eps, double *out_dev);
 /* benchmark helper: average device time in ms of one inference hidden layer (graph
  * aggregation fused with the 128 x 64 GEMM, residual add; the kernel inference runs)
  * over `rows` rows (whole samples of the bound grid: rows = samples x nodes) */

@@ -59,3 +59,29 @@ def test_batch_graph_replays_match_eager(precision):
             assert np.array_equal(a, vfvm.dofs(b)), replay
         # eager work in between: a single-system solve on the same context
         vfvm.solve(systems[0], inival=0.0, control=ctrl)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_device_resident_entry_point(precision):
+    """vfvm_batch_newton_device (start values and solutions packed in device memory, member
+    order) gives the host-buffer entry point's solutions and histories, on uneven members."""
+    print("case batch device-resident", precision, flush=True)
+    members = [cases.batch_member(seed, sizes=(17, 33)) for seed in range(6)]
+    systems = [cases.nonlinear_poisson_2d(n, eps, f, tuple(v), oracle=False)[0] for (n, eps, v, f) in members]
+    ctrl = vfvm.SolverControl(precision=precision, reltol_linear=1e-10, abstol_linear=0.0, maxiters_linear=2000)
+    batch = vfvm.Batch(systems)
+    rng = np.random.default_rng(3)
+    U0 = [rng.uniform(-0.5, 0.5, (1, s.grid.num_nodes)) for s in systems]
+    Uh = [vfvm.dofs(u).copy() for u in batch.solve(U0, ctrl)]
+    hh = [(h.niter, h.nlin) for h in batch.histories]
+    ndof, off = batch.sizes()
+    assert ndof == off[-1] and list(np.diff(off)) == [s.grid.num_nodes for s in systems]
+    ctx = systems[0].ctx
+    Uold_dev = vfvm.DeviceArray.from_numpy(ctx, np.concatenate([u.ravel() for u in U0]))
+    U_dev = vfvm.DeviceArray(ctx, ndof)
+    batch.solve_device(U_dev, Uold_dev, ctrl)
+    Ud = U_dev.numpy()
+    assert [(h.niter, h.nlin) for h in batch.histories] == hh
+    for m in range(len(systems)):
+        assert np.array_equal(Ud[off[m]:off[m + 1]], Uh[m])
+        assert batch.histories[m].residual_norm <= 1e-10

@@ -124,6 +124,13 @@ def test_trained_surrogate_saves_newton_iterations():
     dev_diff = np.abs(guess_dev - guess).max()
     print("device-feature prediction max diff", dev_diff, flush=True)
     assert dev_diff <= 2e-2 * np.abs(guess).max()
+    # inputs and prediction in device memory: the same numbers as the host-buffer call
+    ctx = fam.systems[0].ctx
+    F_dev = vfvm.DeviceArray.from_numpy(ctx, raw[0])
+    V_dev = vfvm.DeviceArray.from_numpy(ctx, raw[1])
+    out_dev = vfvm.DeviceArray(ctx, raw[0].size)
+    net.predict_sides_device(F_dev, V_dev, out_dev, len(seeds))
+    assert np.array_equal(out_dev.numpy().reshape(guess_dev.shape), guess_dev)
     err = max(np.abs(g - vfvm.dofs(u)).max() for g, u in zip(guess, U0))
     diff = max(np.abs(vfvm.dofs(a) - vfvm.dofs(b)).max() for a, b in zip(U0, Ug))
     print("newton iterations from zero", sum(it0), "from surrogate", sum(itg), "surrogate max err", err, flush=True)

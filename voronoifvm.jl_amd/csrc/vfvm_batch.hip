@@ -387,8 +387,35 @@ struct vfvm_batch {
   // captured Krylov iteration blocks: (value bytes, solution vector, iterations) -> graph
   bool use_graphs = true;
   std::map<std::tuple<int, const void *, int>, hipGraphExec_t> graphs;
+  // pinned host staging of the per-member solver states, flags and norms (small
+  // transfers every few iterations), and two status slots with their events for the
+  // pipelined Krylov loop
+  KState *hst[2] = {nullptr, nullptr};
+  hipEvent_t sev[2] = {nullptr, nullptr};
+  KState *hreset = nullptr;
+  int32_t *hact = nullptr, *href = nullptr, *hflag = nullptr;
+  double *hdamp = nullptr, *hnorm = nullptr;
+  void alloc_staging() {
+    for (int i = 0; i < 2; i++) {
+      VFVM_HIP_CHECK(hipHostMalloc((void **)&hst[i], sizeof(KState) * nsys, hipHostMallocDefault));
+      VFVM_HIP_CHECK(hipEventCreateWithFlags(&sev[i], hipEventDisableTiming));
+    }
+    VFVM_HIP_CHECK(hipHostMalloc((void **)&hreset, sizeof(KState) * nsys, hipHostMallocDefault));
+    VFVM_HIP_CHECK(hipHostMalloc((void **)&hact, sizeof(int32_t) * nsys, hipHostMallocDefault));
+    VFVM_HIP_CHECK(hipHostMalloc((void **)&href, sizeof(int32_t) * nsys, hipHostMallocDefault));
+    VFVM_HIP_CHECK(hipHostMalloc((void **)&hflag, sizeof(int32_t), hipHostMallocDefault));
+    VFVM_HIP_CHECK(hipHostMalloc((void **)&hdamp, sizeof(double) * nsys, hipHostMallocDefault));
+    VFVM_HIP_CHECK(hipHostMalloc((void **)&hnorm, sizeof(double) * 2 * nsys, hipHostMallocDefault));
+  }
   ~vfvm_batch() {
     for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 2; i++) {
+      if (hst[i]) (void)hipHostFree(hst[i]);
+      if (sev[i]) (void)hipEventDestroy(sev[i]);
+    }
+    for (void *q : {(void *)hreset, (void *)hact, (void *)href, (void *)hflag, (void *)hdamp, (void *)hnorm})
+      if (q) (void)hipHostFree(q);
   }
   Seg seg() {
     return Seg{nch, ch_inst.p, ch_beg.p, ch_end.p, ch_first.p, tickets.p, partials.p};
@@ -483,21 +510,34 @@ void bres(vfvm_batch &b, const double *x, const double *rhs, double *y, const in
 }
 
 std::vector<KState> read_states(vfvm_batch &b) {
-  std::vector<KState> h((size_t)b.nsys);
-  VFVM_HIP_CHECK(hipMemcpyAsync(h.data(), b.st.p, sizeof(KState) * b.nsys, hipMemcpyDeviceToHost, b.ctx->stream));
+  VFVM_HIP_CHECK(hipMemcpyAsync(b.hst[0], b.st.p, sizeof(KState) * b.nsys, hipMemcpyDeviceToHost, b.ctx->stream));
   VFVM_HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
-  return h;
+  return std::vector<KState>(b.hst[0], b.hst[0] + b.nsys);
+}
+
+// Stream-ordered upload from a pinned staging array.  The caller rewrites that array only
+// after a later synchronisation on the stream, so the copy has been made by then.
+template <class T>
+void stage_upload(DevBuf<T> &dst, const T *pinned, int64_t count, hipStream_t s) {
+  VFVM_HIP_CHECK(hipMemcpyAsync(dst.p, pinned, sizeof(T) * count, hipMemcpyHostToDevice, s));
+}
+
+// active flags per member: host copy in b.hact (read by the refinement loop), device copy
+// by a stream-ordered upload
+void set_active(vfvm_batch &b, const int32_t *flags, hipStream_t s) {
+  if (flags != b.hact) std::memcpy(b.hact, flags, sizeof(int32_t) * b.nsys);
+  stage_upload(b.active, (const int32_t *)b.hact, b.nsys, s);
 }
 
 void reset_states(vfvm_batch &b, double rtol, double atol, int maxit) {
-  std::vector<KState> h((size_t)b.nsys);
-  for (auto &k : h) {
+  for (int64_t m = 0; m < b.nsys; m++) {
+    KState &k = b.hreset[m];
     std::memset(&k, 0, sizeof(k));
     k.rtol = rtol;
     k.atol = atol;
     k.maxit = maxit;
   }
-  b.st.upload(h.data(), h.size(), b.ctx->stream);
+  stage_upload(b.st, b.hreset, b.nsys, b.ctx->stream);
 }
 
 // per-member BiCGStab (point Jacobi = identity on the row-scaled operator) on bhat -> x
@@ -554,18 +594,37 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
       b.graphs[key] = ge;
     }
   }
-  while (true) {
+  // Pipelined: the next block of `every` iterations is queued before the states after
+  // the current block are examined, so the device never waits for the host's decision.
+  // A block queued after every member has finished only runs kernels that return at
+  // entry.  Each block ends with an asynchronous copy of the states into its own pinned
+  // slot.
+  auto enqueue = [&](int slot) {
     if (ge)
       VFVM_HIP_CHECK(hipGraphLaunch(ge, st));
     else
       body();
     VFVM_HIP_CHECK(hipGetLastError());
     launched += every;
-    h = read_states(b);
+    VFVM_HIP_CHECK(hipMemcpyAsync(b.hst[slot], b.st.p, sizeof(KState) * b.nsys, hipMemcpyDeviceToHost, st));
+    VFVM_HIP_CHECK(hipEventRecord(b.sev[slot], st));
+  };
+  int slot = 0, last = 0;
+  enqueue(0);
+  while (true) {
+    const bool more = launched < maxit + every;
+    if (more) {
+      enqueue(1 - slot);
+      last = 1 - slot;
+    }
+    VFVM_HIP_CHECK(hipEventSynchronize(b.sev[slot]));
     bool all = true;
-    for (auto &k : h) all &= (k.done != 0);
-    if (all || launched >= maxit + every) break;
+    for (int64_t m = 0; m < b.nsys; m++) all &= (b.hst[slot][m].done != 0);
+    if (all || !more) break;
+    slot = 1 - slot;
   }
+  VFVM_HIP_CHECK(hipEventSynchronize(b.sev[last]));  // states after the last queued block
+  h.assign(b.hst[last], b.hst[last] + b.nsys);
   {
     const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
     if (unsigned long long *dw = b.ctx->probe.counters(kid))
@@ -597,10 +656,8 @@ void bsolve(vfvm_batch &b, const vfvm_control_t &c, std::vector<int64_t> &nlin) 
   std::vector<KState> h0 = read_states(b);  // rr of bhat per member (written by k_bres)
   std::vector<double> tol((size_t)b.nsys);
   for (int64_t m = 0; m < b.nsys; m++) tol[m] = c.abstol_linear + c.reltol_linear * std::sqrt(h0[m].rr);
-  std::vector<int32_t> ref((size_t)b.nsys);
-  std::vector<int32_t> act((size_t)b.nsys);
-  VFVM_HIP_CHECK(hipMemcpyAsync(act.data(), b.active.p, sizeof(int32_t) * b.nsys, hipMemcpyDeviceToHost, st));
-  VFVM_HIP_CHECK(hipStreamSynchronize(st));
+  int32_t *ref = b.href;
+  const int32_t *act = b.hact;  // the active flags last uploaded from the host
   hipLaunchKernelGGL((k_bfill<double>), dim3(gfor(b.ndof)), dim3(BB), 0, st, b.ndof, b.dU.p, 0.0);
   hipLaunchKernelGGL((k_bconvert<double, double>), dim3(gfor(b.ndof)), dim3(BB), 0, st, b.ndof,
                      (const double *)b.bhat.p, b.rres.p);
@@ -614,7 +671,7 @@ void bsolve(vfvm_batch &b, const vfvm_control_t &c, std::vector<int64_t> &nlin) 
       any |= ref[m] != 0;
     }
     if (!any) break;
-    b.ref_active.upload(ref.data(), ref.size(), st);
+    stage_upload(b.ref_active, (const int32_t *)ref, b.nsys, st);
     float *r32 = b.k32[6].p, *d32 = b.k32[7].p;
     hipLaunchKernelGGL((k_bconvert<float, double>), dim3(gfor(b.ndof)), dim3(BB), 0, st, b.ndof,
                        (const double *)b.rres.p, r32);
@@ -813,6 +870,7 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
     g = 2;
     while (g < maxrow && g < 32) g *= 2;
     b->spmv_group = g;
+    b->alloc_staging();
     VFVM_HIP_CHECK(hipStreamSynchronize(st));
     *out = b.release();
     return VFVM_OK;
@@ -844,7 +902,7 @@ int vfvm_batch_time_kernel(vfvm_batch *b, int kernel, int reps, double *avg_ms) 
   return bguard(b->ctx, [&]() {
     hipStream_t st = b->ctx->stream;
     std::vector<int32_t> all((size_t)b->nsys, 1);
-    b->active.upload(all.data(), all.size(), st);
+    set_active(*b, all.data(), st);
     AsmArgs a = basm(*b, b->F.p, b->J.p, 1.0, 0.0, b->active.p);
     launch_assemble(a, b->asm_group, st);
     launch_scale_rows(b->nspec, b->nnodes, b->rowptr.p, b->colind.p, b->J.p, b->dofdiag.p, b->sell_rbase.p,
@@ -898,7 +956,7 @@ int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group) 
   return bguard(b->ctx, [&]() {
     hipStream_t st = b->ctx->stream;
     std::vector<int32_t> all((size_t)b->nsys, 1);
-    b->active.upload(all.data(), all.size(), st);
+    set_active(*b, all.data(), st);
     ColdTimer timer(st);
     auto timeit = [&](auto &&fn) { return timer.time(fn); };
     float best = 1e30f;
@@ -953,9 +1011,15 @@ int vfvm_batch_autotune(vfvm_batch *b, int32_t *asm_group, int32_t *spmv_group) 
 }
 
 // solve_step! for every member at once (src/vfvm_solver.jl:13-222 per member)
-int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold, double time, double tstep,
-                      double lambda, const vfvm_control_t *control, vfvm_history_t *hist) {
-  if (!b || !U || !Uold) return VFVM_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// The batched Newton solve.  Start values and solutions are either per-member host arrays
+// (Uold, U) or packed device arrays in member order (Uold_dev, U_dev).
+int batch_newton_impl(vfvm_batch *b, double *const *U, const double *const *Uold, double *U_dev,
+                      const double *Uold_dev, double time, double tstep, double lambda,
+                      const vfvm_control_t *control, vfvm_history_t *hist) {
   return bguard(b->ctx, [&]() {
     (void)time;
     vfvm_control_t c;
@@ -970,7 +1034,10 @@ int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold
     const Seg S = b->seg();
     auto t0 = std::chrono::steady_clock::now();
     // pack Uold, copy to U, Dirichlet init
-    {
+    if (Uold_dev) {
+      VFVM_HIP_CHECK(
+          hipMemcpyAsync(b->Uold.p, Uold_dev, sizeof(double) * b->ndof, hipMemcpyDeviceToDevice, st));
+    } else {
       std::vector<double> pack((size_t)b->ndof);
       for (int64_t m = 0; m < M; m++)
         std::memcpy(pack.data() + b->dof_off[m], Uold[m], sizeof(double) * (b->dof_off[m + 1] - b->dof_off[m]));
@@ -981,7 +1048,7 @@ int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold
                           lambda, b->U.p, st);
     launch_fill_i32(1, b->flags.p, 0, st);
     std::vector<int32_t> act((size_t)M, 1);
-    b->active.upload(act.data(), act.size(), st);
+    set_active(*b, act.data(), st);
     std::vector<double> damp((size_t)M, c.is_linear ? 1.0 : c.damp_initial), oldnorm((size_t)M, 1.0),
         rnorm((size_t)M, 0.0), tolx((size_t)M, 0.0), hn((size_t)2 * M);
     std::vector<int> nround((size_t)M, 0), niter((size_t)M, 0), conv((size_t)M, 0), failed((size_t)M, 0);
@@ -1010,15 +1077,17 @@ int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold
       launch_assemble(a, b->asm_group, st);
       b->ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st, wn, wa, wj);
       bsolve(*b, c, nlin);
-      b->damp.upload(damp.data(), damp.size(), st);
+      std::memcpy(b->hdamp, damp.data(), sizeof(double) * M);
+      stage_upload(b->damp, (const double *)b->hdamp, M, st);
       hipLaunchKernelGGL(k_bupdate, dim3((unsigned)b->nch), dim3(BB), 0, st, S, b->U.p, (const double *)b->dU.p,
                          (const double *)b->damp.p, (const int32_t *)b->active.p, b->norms.p);
       hipLaunchKernelGGL((k_bfinalize<1, STAGE_NONE, BF_UPDATE>), dim3((unsigned)M), dim3(BB), 0, st, S, b->st.p,
                          (const int32_t *)b->active.p, b->norms.p);
-      int32_t fl = 0;
-      VFVM_HIP_CHECK(hipMemcpyAsync(hn.data(), b->norms.p, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, st));
-      VFVM_HIP_CHECK(hipMemcpyAsync(&fl, b->flags.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      VFVM_HIP_CHECK(hipMemcpyAsync(b->hnorm, b->norms.p, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, st));
+      VFVM_HIP_CHECK(hipMemcpyAsync(b->hflag, b->flags.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
       VFVM_HIP_CHECK(hipStreamSynchronize(st));
+      std::memcpy(hn.data(), b->hnorm, sizeof(double) * 2 * M);
+      const int32_t fl = *b->hflag;
       if (fl & 1) throw Error(VFVM_ERR_ASSEMBLY, "trying to assemble NaN");
       for (int64_t m = 0; m < M; m++) {
         if (!act[m]) continue;
@@ -1055,13 +1124,13 @@ int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold
           nactive--;
         }
       }
-      b->active.upload(act.data(), act.size(), st);
+      set_active(*b, act.data(), st);
     }
     // residual norms per member at the returned solutions
     std::vector<double> resn((size_t)M, 0.0);
     {
       std::vector<int32_t> all((size_t)M, 1);
-      b->active.upload(all.data(), all.size(), st);
+      set_active(*b, all.data(), st);
       AsmArgs a = basm(*b, b->tmp.p, nullptr, tstep, lambda, b->active.p);
       launch_assemble(a, b->asm_group, st);
       b->norms.alloc((size_t)std::max<int64_t>(2 * M, 2));
@@ -1070,11 +1139,16 @@ int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold
                                b->norms.p, M, st);
       VFVM_HIP_CHECK(hipMemcpyAsync(resn.data(), b->norms.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
     }
-    std::vector<double> out((size_t)b->ndof);
-    b->U.download(out.data(), out.size(), st);
-    VFVM_HIP_CHECK(hipStreamSynchronize(st));
-    for (int64_t m = 0; m < M; m++)
-      std::memcpy(U[m], out.data() + b->dof_off[m], sizeof(double) * (b->dof_off[m + 1] - b->dof_off[m]));
+    if (U_dev) {
+      VFVM_HIP_CHECK(hipMemcpyAsync(U_dev, b->U.p, sizeof(double) * b->ndof, hipMemcpyDeviceToDevice, st));
+      VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    } else {
+      std::vector<double> out((size_t)b->ndof);
+      b->U.download(out.data(), out.size(), st);
+      VFVM_HIP_CHECK(hipStreamSynchronize(st));
+      for (int64_t m = 0; m < M; m++)
+        std::memcpy(U[m], out.data() + b->dof_off[m], sizeof(double) * (b->dof_off[m + 1] - b->dof_off[m]));
+    }
     const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     int rc = VFVM_OK;
     for (int64_t m = 0; m < M; m++) {
@@ -1094,6 +1168,30 @@ int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold
     if (rc) b->ctx->err = "Newton did not converge for at least one batch member (ConvergenceError)";
     return rc;
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int vfvm_batch_newton(vfvm_batch *b, double *const *U, const double *const *Uold, double time, double tstep,
+                      double lambda, const vfvm_control_t *control, vfvm_history_t *hist) {
+  if (!b || !U || !Uold) return VFVM_ERR_ARG;
+  return batch_newton_impl(b, U, Uold, nullptr, nullptr, time, tstep, lambda, control, hist);
+}
+
+int vfvm_batch_newton_device(vfvm_batch *b, double *U_dev, const double *Uold_dev, double time, double tstep,
+                             double lambda, const vfvm_control_t *control, vfvm_history_t *hist) {
+  if (!b || !U_dev || !Uold_dev) return VFVM_ERR_ARG;
+  return batch_newton_impl(b, nullptr, nullptr, U_dev, Uold_dev, time, tstep, lambda, control, hist);
+}
+
+int vfvm_batch_sizes(vfvm_batch *b, int64_t *ndof, int64_t *dof_offsets) {
+  if (!b || !ndof) return VFVM_ERR_ARG;
+  *ndof = b->ndof;
+  if (dof_offsets)
+    for (int64_t m = 0; m <= b->nsys; m++) dof_offsets[m] = b->dof_off[m];
+  return VFVM_OK;
 }
 
 }  // extern "C"

@@ -453,6 +453,34 @@ int vfvm_ctx_destroy(vfvm_ctx *ctx) {
   return VFVM_OK;
 }
 
+int vfvm_device_alloc(vfvm_ctx *ctx, int64_t bytes, void **ptr) {
+  if (!ctx || !ptr || bytes < 0) return VFVM_ERR_ARG;
+  return guarded(ctx, [&]() {
+    *ptr = nullptr;
+    if (bytes > 0) VFVM_HIP_CHECK(hipMalloc(ptr, (size_t)bytes));
+    return VFVM_OK;
+  });
+}
+
+int vfvm_device_free(vfvm_ctx *ctx, void *ptr) {
+  if (!ctx) return VFVM_ERR_ARG;
+  return guarded(ctx, [&]() {
+    VFVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ptr) VFVM_HIP_CHECK(hipFree(ptr));
+    return VFVM_OK;
+  });
+}
+
+int vfvm_device_copy(vfvm_ctx *ctx, void *dst, const void *src, int64_t bytes, int kind) {
+  if (!ctx || bytes < 0 || kind < 0 || kind > 2 || (bytes > 0 && (!dst || !src))) return VFVM_ERR_ARG;
+  return guarded(ctx, [&]() {
+    static const hipMemcpyKind kinds[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+    if (bytes > 0) VFVM_HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)bytes, kinds[kind], ctx->stream));
+    VFVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return VFVM_OK;
+  });
+}
+
 int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coord, int64_t ncells,
                        const int64_t *cellnodes, const int64_t *cellregions, int64_t nbfaces, const int64_t *bfacenodes,
                        const int64_t *bfaceregions, int nspec, const vfvm_physics_t *physics, int nbregions,

@@ -153,6 +153,9 @@ class LinearSolver {
   int gmres(const VBin *b, VXo *x, const double *J, const vfvm_control_t &c, double rtol, double atol, int maxit,
             int &iters);
   void read_state(KState &h);
+  // pinned status slots and their events for the pipelined Krylov loop
+  KState *hst[2] = {nullptr, nullptr};
+  hipEvent_t sev[2] = {nullptr, nullptr};
 
   LinSetup L;
   hipStream_t stream = nullptr;
@@ -168,6 +171,12 @@ class LinearSolver {
   DevBuf<double> fallback64;  // FP64 correction when an FP32 refinement step fails
   DevBuf<double> xsave;       // solution before the latest FP32 correction
  public:
+  ~LinearSolver() {
+    for (int i = 0; i < 2; i++) {
+      if (hst[i]) (void)hipHostFree(hst[i]);
+      if (sev[i]) (void)hipEventDestroy(sev[i]);
+    }
+  }
   int64_t fallbacks = 0;      // linear solves finished in FP64 after an FP32 failure
   // set when an FP32 refinement fails: later solves with FP32 requested run in FP64 until
   // the owner clears it (at the start of each Newton solve or time/parameter sweep)

@@ -727,6 +727,10 @@ void LinearSolver::setup(const LinSetup &ls, hipStream_t s) {
   stream = s;
   int64_t n = L.ndof;
   st.alloc(1);
+  for (int i = 0; i < 2; i++) {
+    if (!hst[i]) VFVM_HIP_CHECK(hipHostMalloc((void **)&hst[i], sizeof(KState), hipHostMallocDefault));
+    if (!sev[i]) VFVM_HIP_CHECK(hipEventCreateWithFlags(&sev[i], hipEventDisableTiming));
+  }
   partials.alloc((size_t)4 * MAX_RED_BLOCKS);
   rowscale.alloc((size_t)n);
   hipLaunchKernelGGL((k_fill<unsigned char>), dim3(grid_for((int64_t)sizeof(KState))), dim3(VB), 0, s,
@@ -963,8 +967,11 @@ int LinearSolver::bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_co
   int check = std::max(every, std::min(maxit, last_iters - 1));
   int launched = 0, best_at = 0;
   double best = -1.0;
-  while (true) {
-    for (int q = 0; q < check; q++) {
+  // Pipelined: the next block of iterations is queued before the state after the current
+  // block is examined (a block queued after convergence runs kernels that return at
+  // entry); each block ends with an asynchronous copy of the state into its pinned slot.
+  auto enqueue = [&](int nit, int slot) {
+    for (int q = 0; q < nit; q++) {
       hipLaunchKernelGGL((k_bicg_p<V>), dim3(g), dim3(VB), 0, stream, n, r, p, v, st.p);
       const V *php = precond<V>(p, ph, st.p, 0, c);
       apply_op<V, 1, STAGE_ALPHA>(php, v, r0, (const V *)nullptr, st.p, J);
@@ -979,20 +986,33 @@ int LinearSolver::bicgstab(const VBin *b, VXo *x, const double *J, const vfvm_co
                          st.p, 1);
     }
     VFVM_HIP_CHECK(hipGetLastError());
-    launched += check;
-    read_state(h);
-    if (h.done) break;
-    if (launched >= maxit + check) break;
+    launched += nit;
+    VFVM_HIP_CHECK(hipMemcpyAsync(hst[slot], st.p, sizeof(KState), hipMemcpyDeviceToHost, stream));
+    VFVM_HIP_CHECK(hipEventRecord(sev[slot], stream));
+  };
+  int slot = 0, last = 0;
+  enqueue(check, 0);
+  while (true) {
+    const bool more = launched < maxit + every;
+    if (more) {
+      enqueue(every, 1 - slot);
+      last = 1 - slot;
+    }
+    VFVM_HIP_CHECK(hipEventSynchronize(sev[slot]));
+    h = *hst[slot];
+    if (h.done || !more) break;
     if (stag_window > 0) {
       if (best < 0.0 || h.rr < 0.81 * best) {
         best = h.rr;
         best_at = h.iters;
       } else if (h.iters - best_at >= stag_window) {
-        break;  // stagnated: the kernels of the remaining iterations are never launched
+        break;  // stagnated: no further blocks are queued
       }
     }
-    check = every;
+    slot = 1 - slot;
   }
+  VFVM_HIP_CHECK(hipEventSynchronize(sev[last]));  // state after the last queued block
+  h = *hst[last];
   iters = h.iters;
   last_iters = h.iters;
   last_rr = h.rr;

@@ -595,6 +595,12 @@ __global__ void k_fill_f(int64_t n, float *p, float v) {
   if (k < n) p[k] = v;
 }
 
+// u (float, rows) -> out (double)
+__global__ void k_to_double(int64_t rows, const float *__restrict__ u, double *__restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows) out[r] = (double)u[r];
+}
+
 }  // namespace
 
 struct vfvm_mpnn {
@@ -807,6 +813,46 @@ float backward(vfvm_mpnn &m, int64_t rows) {
   return (float)(loss / (double)rows);
 }
 
+// inference with the Dirichlet-sides features computed on the device from the sources f
+// (nsamples x n) and the side values (nsamples x 4); host or device arrays
+int infer_sides_impl(vfvm_mpnn *m, int64_t nsamples, const double *f, const double *sides, double eps, double *out,
+                     bool on_device) {
+  return mguard(m->ctx, [&]() {
+    if (!m->xy.p) throw Error(VFVM_ERR_ARG, "side features need a 2D system");
+    hipStream_t st = m->ctx->stream;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsamples, (int64_t(1) << 23) / m->n));
+    const double inv_delta = 1.0 / std::sqrt(eps / 2.0);
+    std::vector<float> u;
+    for (int64_t s0 = 0; s0 < nsamples; s0 += chunk) {
+      const int64_t ns = std::min(chunk, nsamples - s0), rows = ns * m->n;
+      ensure_cap(*m, rows);
+      const double *fp = f + s0 * m->n, *sp = sides + s0 * 4;
+      if (!on_device) {
+        m->fin.upload(fp, (size_t)rows, st);
+        m->sides.upload(sp, (size_t)ns * 4, st);
+        fp = m->fin.p;
+        sp = m->sides.p;
+      }
+      hipLaunchKernelGGL(k_features_sides, dim3(nb(rows, 256)), dim3(256), 0, st, rows, m->n, fp, sp,
+                         (const float *)m->xy.p, (const float *)m->dist.p, inv_delta, m->x.p);
+      expand(*m, m->x.p, m->x16.p, ns);
+      forward_infer(*m, rows);
+      if (on_device) {
+        hipLaunchKernelGGL(k_to_double, dim3(nb(rows, 256)), dim3(256), 0, st, rows, (const float *)m->u.p,
+                           out + s0 * m->n);
+        VFVM_HIP_CHECK(hipGetLastError());
+      } else {
+        u.resize((size_t)rows);
+        m->u.download(u.data(), (size_t)rows, st);
+        VFVM_HIP_CHECK(hipStreamSynchronize(st));
+        for (int64_t k = 0; k < rows; k++) out[s0 * m->n + k] = (double)u[k];
+      }
+    }
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return VFVM_OK;
+  });
+}
+
 }  // namespace
 
 extern "C" {
@@ -956,34 +1002,16 @@ int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, doubl
   });
 }
 
-// inference with the Dirichlet-sides features computed on the device from the sources
-// f (nsamples x n) and the side values (nsamples x 4)
 int vfvm_mpnn_infer_sides(vfvm_mpnn *m, int64_t nsamples, const double *f, const double *sides, double eps,
                           double *out) {
   if (!m || !f || !sides || !out || nsamples < 1 || !(eps > 0)) return VFVM_ERR_ARG;
-  return mguard(m->ctx, [&]() {
-    if (!m->xy.p) throw Error(VFVM_ERR_ARG, "side features need a 2D system");
-    hipStream_t st = m->ctx->stream;
-    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsamples, (int64_t(1) << 23) / m->n));
-    const double inv_delta = 1.0 / std::sqrt(eps / 2.0);
-    std::vector<float> u;
-    for (int64_t s0 = 0; s0 < nsamples; s0 += chunk) {
-      const int64_t ns = std::min(chunk, nsamples - s0), rows = ns * m->n;
-      ensure_cap(*m, rows);
-      m->fin.upload(f + s0 * m->n, (size_t)rows, st);
-      m->sides.upload(sides + s0 * 4, (size_t)ns * 4, st);
-      hipLaunchKernelGGL(k_features_sides, dim3(nb(rows, 256)), dim3(256), 0, st, rows, m->n,
-                         (const double *)m->fin.p, (const double *)m->sides.p, (const float *)m->xy.p,
-                         (const float *)m->dist.p, inv_delta, m->x.p);
-      expand(*m, m->x.p, m->x16.p, ns);
-      forward_infer(*m, rows);
-      u.resize((size_t)rows);
-      m->u.download(u.data(), (size_t)rows, st);
-      VFVM_HIP_CHECK(hipStreamSynchronize(st));
-      for (int64_t k = 0; k < rows; k++) out[s0 * m->n + k] = (double)u[k];
-    }
-    return VFVM_OK;
-  });
+  return infer_sides_impl(m, nsamples, f, sides, eps, out, false);
+}
+
+int vfvm_mpnn_infer_sides_device(vfvm_mpnn *m, int64_t nsamples, const double *f_dev, const double *sides_dev,
+                                 double eps, double *out_dev) {
+  if (!m || !f_dev || !sides_dev || !out_dev || nsamples < 1 || !(eps > 0)) return VFVM_ERR_ARG;
+  return infer_sides_impl(m, nsamples, f_dev, sides_dev, eps, out_dev, true);
 }
 
 // average device time (ms) of one inference hidden layer (k_layer_infer, not the last)

@@ -3,7 +3,7 @@
 Host-side mirror of VoronoiFVM.jl's System / Physics / solve surface; the work
 is done by libvfvm_hip.so (HIP kernels behind the C ABI in include/vfvm_hip.h).
 """
-from ._capi import AssemblyError, ConvergenceError, LinearSolverError, VFVMError
+from ._capi import AssemblyError, ConvergenceError, DeviceArray, LinearSolverError, VFVMError
 from .control import SolverControl, fixed_timesteps
 from .grids import Grid, linspace, simplexgrid
 from .physics import (Dirichlet, DriftDiffusionFlux, DriftDiffusionReaction, ExchangeReaction, IdentityStorage,

@@ -63,6 +63,9 @@ SIGNATURES = {
     "vfvm_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "vfvm_ctx_destroy": (C.c_int, [_vp]),
     "vfvm_last_error": (C.c_char_p, [_vp]),
+    "vfvm_device_alloc": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp)]),
+    "vfvm_device_free": (C.c_int, [_vp, _vp]),
+    "vfvm_device_copy": (C.c_int, [_vp, _vp, _vp, C.c_int64, C.c_int]),
     "vfvm_system_create": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_int64, _lp, _lp, C.c_int64, _lp, _lp, C.c_int,
                                      C.POINTER(PhysicsT), C.c_int, C.POINTER(BCT), C.POINTER(_vp)]),
     "vfvm_system_destroy": (C.c_int, [_vp]),
@@ -97,6 +100,7 @@ SIGNATURES = {
                                   C.c_float, C.POINTER(C.c_float)]),
     "vfvm_mpnn_infer": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_float), _dp]),
     "vfvm_mpnn_infer_sides": (C.c_int, [_vp, C.c_int64, _dp, _dp, C.c_double, _dp]),
+    "vfvm_mpnn_infer_sides_device": (C.c_int, [_vp, C.c_int64, _vp, _vp, C.c_double, _vp]),
     "vfvm_mpnn_loss_grad": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "vfvm_batch_create": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(_vp)]),
@@ -108,6 +112,9 @@ SIGNATURES = {
     "vfvm_batch_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_batch_newton": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp), C.c_double, C.c_double, C.c_double,
                                     C.POINTER(ControlT), C.POINTER(HistoryT)]),
+    "vfvm_batch_newton_device": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, C.POINTER(ControlT),
+                                           C.POINTER(HistoryT)]),
+    "vfvm_batch_sizes": (C.c_int, [_vp, _lp, _lp]),
     "vfvm_assemble_device": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp, _vp]),
 }
 
@@ -225,3 +232,43 @@ def default_context() -> Context:
     if _DEFAULT_CTX is None:
         _DEFAULT_CTX = Context(0)
     return _DEFAULT_CTX
+
+
+class DeviceArray:
+    """A float64 array in the device memory of a context (vfvm_device_alloc), for the
+    *_device entry points: inputs stay resident between calls and results are read
+    back only when wanted."""
+
+    def __init__(self, ctx, n):
+        self.ctx, self.n = ctx, int(n)
+        p = C.c_void_p()
+        check(lib().vfvm_device_alloc(ctx.handle, 8 * self.n, C.byref(p)), ctx.handle)
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, ctx, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        d = cls(ctx, a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if a.size != self.n:
+            raise ValueError("size mismatch")
+        check(lib().vfvm_device_copy(self.ctx.handle, self.ptr, a.ctypes.data_as(_vp), 8 * self.n, 0),
+              self.ctx.handle)
+
+    def numpy(self):
+        out = np.empty(self.n)
+        check(lib().vfvm_device_copy(self.ctx.handle, out.ctypes.data_as(_vp), self.ptr, 8 * self.n, 1),
+              self.ctx.handle)
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value and _LIB is not None:
+            _LIB.vfvm_device_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        self.free()

@@ -156,6 +156,13 @@ class Surrogate:
                                                       _capi.dptr(out)), self.ctx.handle)
         return out
 
+    def predict_sides_device(self, F_dev, V_dev, out_dev, nsamples, eps=EPS):
+        """predict_sides with the sources, side values and the prediction in device
+        memory (vfvm.DeviceArray; out_dev holds nsamples x n float64)."""
+        _capi.check(_capi.lib().vfvm_mpnn_infer_sides_device(self._h, int(nsamples), F_dev.ptr, V_dev.ptr, eps,
+                                                             out_dev.ptr), self.ctx.handle)
+        return out_dev
+
     def save(self, path=WEIGHTS, **meta):
         os.makedirs(os.path.dirname(path), exist_ok=True)
         np.savez(path, params=self.params(), nlayers=self.nlayers, **meta)

@@ -375,6 +375,31 @@ class Batch:
         launch their kernels one by one."""
         _capi.check(_capi.lib().vfvm_batch_set_graphs(self._h, int(enable)), self.ctx.handle)
 
+    def sizes(self):
+        """Total dofs and the member offsets of the packed (member-order) dof vector."""
+        nd = C.c_int64()
+        off = np.zeros(len(self.systems) + 1, dtype=np.int64)
+        _capi.check(_capi.lib().vfvm_batch_sizes(self._h, C.byref(nd), off.ctypes.data_as(_capi._lp)),
+                    self.ctx.handle)
+        return nd.value, off
+
+    def solve_device(self, U_dev, Uold_dev, control: Optional[SolverControl] = None, time=0.0, tstep=math.inf,
+                     embed=0.0, raise_on_failure=True, **kwargs):
+        """solve for every member with start values Uold_dev and solutions U_dev in device
+        memory (vfvm.DeviceArray of the packed member-order dofs, see sizes())."""
+        c = _control(control, kwargs)
+        n = len(self.systems)
+        hist = (_capi.HistoryT * n)()
+        cc = c.to_c(self.systems[0].is_linear)
+        rc = _capi.lib().vfvm_batch_newton_device(self._h, U_dev.ptr, Uold_dev.ptr, time, tstep, embed, C.byref(cc),
+                                                  hist)
+        self.histories = [NewtonSolverHistory(nlin=h.nlin, niter=h.niter, time=h.time,
+                                              updatenorm=tuple(h.updatenorm[:min(h.niter, 64)]),
+                                              residual_norm=h.residual_norm) for h in hist]
+        if raise_on_failure:
+            _capi.check(rc, self.ctx.handle)
+        return U_dev
+
     def update_member(self, m, source=None, bcondition=None):
         """Replace member m's source and/or boundary conditions inside the batch."""
         s = self.systems[m]
