@@ -195,6 +195,41 @@ function hip_solve_transient!(ctx, sys, inival, lambdas, control::ControlT, step
     return ts[1:n], [us[:, k] for k in 1:n], hist
 end
 
+"""
+    hip_batch(ctx, systems) -> handle
+
+Packs independent systems (the one-SystemState-per-thread pattern of
+examples/Example440_ParallelState.jl:36-45) into one batch.
+"""
+function hip_batch(ctx, systems::Vector{Ptr{Cvoid}})
+    b = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:vfvm_batch_create, libvfvm), Cint, (Ptr{Cvoid}, Int64, Ptr{Ptr{Cvoid}}, Ref{Ptr{Cvoid}}),
+               ctx, length(systems), systems, b)
+    check(rc, ctx)
+    return b[]
+end
+
+"""
+    hip_batch_solve_device!(ctx, batch, U, Uold, control; time = 0.0, tstep = Inf, λ = 0.0)
+
+solve_step! for every member with start values and solutions in device memory:
+`U` and `Uold` are device pointers (e.g. `pointer(A)` of an AMDGPU.jl ROCArray, its
+stream synchronised) to the members' dofs packed in member order.
+"""
+function hip_batch_solve_device!(ctx, batch, U::Ptr{Float64}, Uold::Ptr{Float64}, control::ControlT, nsys;
+                                 time = 0.0, tstep = Inf, λ = 0.0)
+    hist = [HistoryT() for _ in 1:nsys]
+    hbuf = Vector{UInt8}(undef, sizeof(HistoryT) * nsys)
+    rc = ccall((:vfvm_batch_newton_device, libvfvm), Cint,
+               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Ref{ControlT}, Ptr{UInt8}),
+               batch, U, Uold, time, tstep, λ, control, hbuf)
+    for m in 1:nsys
+        hist[m] = unsafe_load(Ptr{HistoryT}(pointer(hbuf) + (m - 1) * sizeof(HistoryT)))
+    end
+    check(rc, ctx)
+    return hist
+end
+
 function default_control()
     c = ControlT()
     ccall((:vfvm_default_control, libvfvm), Cvoid, (Ref{ControlT},), c)
