@@ -216,18 +216,15 @@ solve_step! for every member with start values and solutions in device memory:
 `U` and `Uold` are device pointers (e.g. `pointer(A)` of an AMDGPU.jl ROCArray, its
 stream synchronised) to the members' dofs packed in member order.
 """
-function hip_batch_solve_device!(ctx, batch, U::Ptr{Float64}, Uold::Ptr{Float64}, control::ControlT, nsys;
+function hip_batch_solve_device!(ctx, batch, U::Ptr{Float64}, Uold::Ptr{Float64}, control::ControlT;
                                  time = 0.0, tstep = Inf, λ = 0.0)
-    hist = [HistoryT() for _ in 1:nsys]
-    hbuf = Vector{UInt8}(undef, sizeof(HistoryT) * nsys)
+    # per-member histories are not requested here (NULL); the return code reports a
+    # member that failed to converge (ConvergenceError)
     rc = ccall((:vfvm_batch_newton_device, libvfvm), Cint,
-               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Ref{ControlT}, Ptr{UInt8}),
-               batch, U, Uold, time, tstep, λ, control, hbuf)
-    for m in 1:nsys
-        hist[m] = unsafe_load(Ptr{HistoryT}(pointer(hbuf) + (m - 1) * sizeof(HistoryT)))
-    end
+               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Ref{ControlT}, Ptr{Cvoid}),
+               batch, U, Uold, time, tstep, λ, control, C_NULL)
     check(rc, ctx)
-    return hist
+    return nothing
 end
 
 function default_control()
