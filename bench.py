@@ -93,6 +93,12 @@ def read_traffic(name):
         return None
 
 
+# linear tolerance of every form: SolverControl's default reltol_linear
+# (src/vfvm_solvercontrol.jl:111-116); FP32 refinement rounds reduce the residual by
+# INNER_RELTOL each
+RELTOL_LINEAR = 1e-4
+INNER_RELTOL = 1e-2
+
 # --------------------------------------------------------------------------- cfg0
 def run_cfg0(args):
     """The reference's CPU-runnable case: 101 x 101 nonlinear Poisson (eps = 1e-2, f = 1,
@@ -149,8 +155,8 @@ def build_cfg1(n):
 
 def cfg1_control(precision):
     """BiCGStab + ILU(0), fixed implicit-Euler steps of 1e-2 (fixed_timesteps!)."""
-    c = vfvm.SolverControl(krylov="bicgstab", precon="ilu0", precision=precision, reltol_linear=1e-6,
-                           abstol_linear=0.0, maxiters_linear=1000, inner_reltol=1e-3, max_refinements=10)
+    c = vfvm.SolverControl(krylov="bicgstab", precon="ilu0", precision=precision, reltol_linear=RELTOL_LINEAR,
+                           abstol_linear=0.0, maxiters_linear=1000, inner_reltol=INNER_RELTOL, max_refinements=10)
     return vfvm.fixed_timesteps(c, 1e-2)
 
 
@@ -221,7 +227,8 @@ def cpu_cfg1(args):
     n = 81
     sys_, osys = cases.nonlinear_diffusion_3d(n)
     u0 = cases.initial_sine_3d(sys_.grid)
-    c = O.control(method_linear="bicgstab_ilu0", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=1000)
+    c = O.control(method_linear="bicgstab_ilu0", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0,
+                  maxiters_linear=1000)
     t0 = time.perf_counter()
     osys.solve_step(u0, c, time=1e-2, tstep=1e-2)
     el = time.perf_counter() - t0
@@ -237,9 +244,9 @@ CFG2_BIAS = np.linspace(0.0, 1.0, 21)  # 0 .. 1 V in 20 continuation steps
 
 def cfg2_control(precision):
     """Block-Jacobi BiCGStab; fixed embedding steps of 0.05 V (Δp = Δp_min = Δp_max)."""
-    return vfvm.SolverControl(krylov="bicgstab", precon="blockjacobi", precision=precision, reltol_linear=1e-6,
-                              abstol_linear=0.0, maxiters_linear=6000, inner_reltol=1e-3, max_refinements=12,
-                              Δp=0.05, Δp_min=0.05, Δp_max=0.05)
+    return vfvm.SolverControl(krylov="bicgstab", precon="blockjacobi", precision=precision,
+                              reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=6000,
+                              inner_reltol=INNER_RELTOL, max_refinements=12, Δp=0.05, Δp_min=0.05, Δp_max=0.05)
 
 
 def run_cfg2(args):
@@ -331,8 +338,8 @@ def build_poisson(n, eps, vals, f):
 
 
 def cfg3_control(precision):
-    return vfvm.SolverControl(krylov="bicgstab", precon="jacobi", precision=precision, reltol_linear=1e-6,
-                              abstol_linear=0.0, maxiters_linear=3000, inner_reltol=1e-3, max_refinements=12,
+    return vfvm.SolverControl(krylov="bicgstab", precon="jacobi", precision=precision, reltol_linear=RELTOL_LINEAR,
+                              abstol_linear=0.0, maxiters_linear=3000, inner_reltol=INNER_RELTOL, max_refinements=12,
                               check_every=8)
 
 
@@ -432,8 +439,8 @@ def run_cfg4(args):
     seeds = list(S.EVAL_SEEDS)
     K = min(args.steps, len(seeds) // 256)
     blocks = [seeds[256 * k:256 * (k + 1)] for k in range(K)]
-    ctrl32 = vfvm.SolverControl(precision="fp32", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
-                                inner_reltol=1e-3, max_refinements=12, check_every=8)
+    ctrl32 = vfvm.SolverControl(precision="fp32", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=3000,
+                                inner_reltol=INNER_RELTOL, max_refinements=12, check_every=8)
     ctx = fam.systems[0].ctx
     n = fam.grid.num_nodes
     ndof, off = fam.batch.sizes()
@@ -458,7 +465,7 @@ def run_cfg4(args):
         g = net.predict_sides(*raw)  # the host-buffer path too
         fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
     t_sur, t_sur_host, t_zero, t_zero64, it_sur, it_zero, sur_err, worst = 0.0, 0.0, 0.0, 0.0, [], [], [], 0.0
-    ctrl64b = vfvm.SolverControl(precision="fp64", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
+    ctrl64b = vfvm.SolverControl(precision="fp64", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=3000,
                                  check_every=8)
     fam.batch.solve_device(U_dev, Z_dev, ctrl64b)  # warm-up of the FP64 batched path
     converged = []
@@ -510,7 +517,7 @@ def run_cfg4(args):
         f"{np.mean(it_zero):.2f}; surrogate max-norm error mean {np.mean(sur_err):.3e} max {np.max(sur_err):.3e}")
     # plain FP64 GPU Newton from zero, one system at a time, same inputs
     single = S.make_system(fam.grid, np.zeros(fam.grid.num_nodes), np.zeros(4))
-    ctrl64 = vfvm.SolverControl(precision="fp64", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
+    ctrl64 = vfvm.SolverControl(precision="fp64", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=3000,
                                 check_every=8)
     single.handle()
     vfvm.solve(single, inival=0.0, control=ctrl64)
@@ -541,7 +548,7 @@ def run_cfg4(args):
                   bcondition=[Dirichlet(species=1, region=r + 1, value=0.0, value_lambda=0.0) for r in range(4)],
                   embed_source=True)
     cont = vfvm.System(fam.grid, cph, species=[1])
-    cctrl = vfvm.SolverControl(precision="fp64", reltol_linear=1e-6, abstol_linear=0.0, maxiters_linear=3000,
+    cctrl = vfvm.SolverControl(precision="fp64", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=3000,
                                check_every=8, Δp=0.25, Δp_min=1e-2, Δp_max=0.5, Δp_grow=1.5, Δu_opt=0.5)
     t_cont, it_cont, st_cont, cdiff = 0.0, [], [], 0.0
     for k, sd in enumerate(blocks[0][:ncont]):
@@ -609,7 +616,8 @@ def run_cfg4(args):
                 config=dict(workload="cfg4: 129x129 nonlinear Poisson (eps=1e-2, GRF source ell=0.1 amp=5, side "
                                      "values U[-1,1]), 1,024 held-out seeds in batches of 256: MPNN initial guess "
                                      "(4 layers, width 64, BF16 MFMA) + batched BiCGStab/Jacobi FP32 inside FP64 "
-                                     "Newton, vs plain FP64 GPU Newton from zero one solve at a time",
+                                     "Newton, vs plain FP64 GPU Newton from zero one solve at a time; "
+                                     "reltol_linear 1e-4 (the reference's default) in every form",
                             solves=nsol, unknowns_per_solve=fam.grid.num_nodes, batch=256,
                             hip_graphs=not args.no_graphs,
                             resident="sources, side values, guesses and solutions in HBM (the *_device "
