@@ -237,6 +237,10 @@ int vfvm_autotune(vfvm_sys *sys, int32_t *asm_group, int32_t *spmv_group);
 
 /* Set the group sizes directly (assembly 0, 1, 2, 4, 8, 16 or 32; CSR residual 2..32). */
 int vfvm_set_groups(vfvm_sys *sys, int32_t asm_group, int32_t spmv_group);
+/* Column indices the SELL-64 SpMV reads: 16 = offsets from the row in 16 bits (the
+ * default wherever every row's columns lie within 32767 rows of it), 32 = absolute
+ * 32-bit columns.  Same arithmetic either way; 16 bits move 2 bytes less per entry. */
+int vfvm_set_sell_index(vfvm_sys *sys, int bits);
 
 /* Benchmark helper (no counterpart in the reference; it has no device): average device
  * time in milliseconds of `reps` back-to-back launches of one hot kernel on the handle's
@@ -281,6 +285,8 @@ int vfvm_batch_destroy(vfvm_batch *b);
 /* Batched Krylov iterations between status reads run as captured HIP graphs (default on);
  * enable = 0 launches the same kernels one by one.  No counterpart in the reference. */
 int vfvm_batch_set_graphs(vfvm_batch *b, int enable);
+/* the batch's SpMV column width, as vfvm_set_sell_index */
+int vfvm_batch_set_sell_index(vfvm_batch *b, int bits);
 /* replace member m's source (nspec x nnodes; NULL keeps it) and boundary table (NULL keeps
  * it) inside the batch, for repeated solves of one grid with new data */
 int vfvm_batch_update_member(vfvm_batch *b, int64_t m, const double *src, const vfvm_bc_t *bc);

@@ -85,3 +85,23 @@ def test_batch_device_resident_entry_point(precision):
     for m in range(len(systems)):
         assert np.array_equal(Ud[off[m]:off[m + 1]], Uh[m])
         assert batch.histories[m].residual_norm <= 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_sell_index_widths_agree(precision):
+    """Batched SpMV with 16-bit column offsets and with 32-bit columns: bitwise the same
+    solutions and iteration counts on uneven members (graphs recaptured on the switch)."""
+    print("case batch sell index widths", precision, flush=True)
+    members = [cases.batch_member(seed, sizes=(17, 33, 65)) for seed in range(6)]
+    systems = [cases.nonlinear_poisson_2d(n, eps, f, tuple(v), oracle=False)[0] for (n, eps, v, f) in members]
+    ctrl = vfvm.SolverControl(precision=precision, reltol_linear=1e-10, abstol_linear=0.0, maxiters_linear=2000)
+    batch = vfvm.Batch(systems)
+    out = []
+    for bits in (16, 32, 16):
+        batch.set_sell_index(bits)
+        U = [vfvm.dofs(u).copy() for u in batch.solve(0.0, ctrl)]
+        out.append((U, [h.nlin for h in batch.histories]))
+    for U, it in out[1:]:
+        assert it == out[0][1]
+        for a, b in zip(U, out[0][0]):
+            assert np.array_equal(a, b)

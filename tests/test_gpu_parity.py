@@ -329,3 +329,20 @@ def test_fp32_refinement_hands_over_to_fp64():
         out[precision] = (vfvm.dofs(sol.u[-1]).copy(), h.fp64_fallbacks)
     assert out["fp64"][1] == 0 and out["fp32"][1] >= 1
     assert np.abs(out["fp32"][0] - out["fp64"][0]).max() <= SOL_ATOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_sell_index_widths_agree(precision):
+    """The SpMV with 16-bit column offsets and with 32-bit columns: the same arithmetic,
+    so the same Newton iterates bit for bit (2D and 3D patterns, single system)."""
+    _print("case sell index widths", precision)
+    for sys, osys in (cases.nonlinear_poisson_2d(41, eps=1e-2, f=2.0, values=(-0.5, 1.0, 0.2, 0.0)),
+                      cases.nonlinear_diffusion_3d(9)):
+        c = vfvm.SolverControl(precision=precision, reltol_linear=1e-10, abstol_linear=0.0, maxiters_linear=3000)
+        out = []
+        for bits in (16, 32):
+            sys.set_sell_index(bits)
+            U = vfvm.solve(sys, inival=0.5, control=c)
+            out.append((vfvm.dofs(U).copy(), sys.history.nlin))
+            assert sys.history.residual_norm <= 1e-10
+        assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]

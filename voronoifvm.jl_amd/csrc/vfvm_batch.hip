@@ -20,6 +20,7 @@
 #include <memory>
 #include <string>
 #include <tuple>
+#include <type_traits>
 
 #include "vfvm_sys.h"
 
@@ -136,9 +137,9 @@ __device__ __forceinline__ T bld(const T *p) {  // matrix stream load, optionall
     return *p;
 }
 
-template <class VA, class V, int NDOT, int STAGE, bool NT>
+template <class VA, class V, int NDOT, int STAGE, bool NT, class CI>
 __global__ void __launch_bounds__(BB) k_bspmv(Seg S, const int32_t *__restrict__ ch_slice,
-                                              const int32_t *__restrict__ sptr, const int32_t *__restrict__ col,
+                                              const int32_t *__restrict__ sptr, const CI *__restrict__ col,
                                               const VA *__restrict__ val, const V *__restrict__ x, V *__restrict__ y,
                                               const V *__restrict__ w0, KState *st,
                                               unsigned long long *__restrict__ mwork,
@@ -159,11 +160,11 @@ __global__ void __launch_bounds__(BB) k_bspmv(Seg S, const int32_t *__restrict__
   for (int64_t l = threadIdx.x; l < r1 - r0; l += BB) {
     const int32_t sl = sl0 + (int32_t)(l >> 6);
     const int32_t p1 = sptr[sl + 1];
+    const int64_t r = r0 + l;
     double acc = 0.0;
 #pragma unroll 4
     for (int32_t q = sptr[sl] + (int32_t)(l & 63); q < p1; q += 64)
-      acc += (double)bld<NT>(val + q) * (double)x[bld<NT>(col + q)];
-    const int64_t r = r0 + l;
+      acc += (double)bld<NT>(val + q) * (double)x[sell_column(bld<NT>(col + q), r)];
     const V yv = (V)acc;
     y[r] = yv;
     if (NDOT >= 1) d0 += (double)w0[r] * (double)yv;
@@ -378,6 +379,8 @@ struct vfvm_batch {
   DevBuf<float> A32;
   // SELL-64 operator: slices start at every chunk start; ch_slice = first slice of a chunk
   DevBuf<int32_t> sell_ptr, sell_col, sell_rbase, ch_slice;
+  DevBuf<int16_t> sell_off16;  // columns as 16-bit offsets from the row (empty: too far)
+  bool sell16 = true;
   DevBuf<double> A64s;
   int64_t sell_nnz = 0;
   // probe work of the batched SpMV: per member (rows, nonzeros) and per-member counters
@@ -470,21 +473,30 @@ void bop(vfvm_batch &b, const V *x, V *y, const V *w0) {
   const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
   unsigned long long *mwork = b.ctx->probe.counters(kid) ? b.mwork.p : nullptr;
   b.ctx->probe.before(kid, b.ctx->stream);
-  auto go = [&](auto ntc) {
+  auto go = [&](auto ntc, const auto *col) {
     constexpr bool NT = decltype(ntc)::value;
+    using CI = std::remove_const_t<std::remove_pointer_t<decltype(col)>>;
     if constexpr (sizeof(V) == 4)
-      VFVM_LAUNCH((k_bspmv<float, V, NDOT, STAGE, NT>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                  b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const float *)b.A32.p, x, y, w0, b.st.p, mwork,
+      VFVM_LAUNCH((k_bspmv<float, V, NDOT, STAGE, NT, CI>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                  b.ch_slice.p, b.sell_ptr.p, col, (const float *)b.A32.p, x, y, w0, b.st.p, mwork,
                   (const int64_t *)b.msize.p);
     else
-      VFVM_LAUNCH((k_bspmv<double, V, NDOT, STAGE, NT>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                  b.ch_slice.p, b.sell_ptr.p, b.sell_col.p, (const double *)b.A64s.p, x, y, w0, b.st.p, mwork,
+      VFVM_LAUNCH((k_bspmv<double, V, NDOT, STAGE, NT, CI>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                  b.ch_slice.p, b.sell_ptr.p, col, (const double *)b.A64s.p, x, y, w0, b.st.p, mwork,
                   (const int64_t *)b.msize.p);
   };
-  if (b.sell_nt[sizeof(V) == 4 ? 0 : 1])
-    go(std::true_type());
-  else
-    go(std::false_type());
+  const bool nt = b.sell_nt[sizeof(V) == 4 ? 0 : 1];
+  if (b.sell_off16.p && b.sell16) {
+    if (nt)
+      go(std::true_type(), (const int16_t *)b.sell_off16.p);
+    else
+      go(std::false_type(), (const int16_t *)b.sell_off16.p);
+  } else {
+    if (nt)
+      go(std::true_type(), (const int32_t *)b.sell_col.p);
+    else
+      go(std::false_type(), (const int32_t *)b.sell_col.p);
+  }
   b.ctx->probe.after(kid, b.ctx->stream);
   if constexpr (NDOT > 0)
     hipLaunchKernelGGL((k_bfinalize<2, STAGE, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, b.ctx->stream, S,
@@ -825,9 +837,11 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
     b->nch = (int64_t)ci.size();
     {
       std::vector<int32_t> sptr, scol, chs, rbase;
-      build_sell64(b->ndof, rowptr, colind, cb, ce, sptr, scol, chs, rbase);
+      std::vector<int16_t> off16;
+      build_sell64(b->ndof, rowptr, colind, cb, ce, sptr, scol, chs, rbase, &off16);
       b->sell_ptr.upload(sptr, st);
       b->sell_col.upload(scol, st);
+      if (!off16.empty()) b->sell_off16.upload(off16, st);
       b->sell_rbase.upload(rbase, st);
       b->ch_slice.upload(chs, st);
       b->sell_nnz = (int64_t)scol.size();
@@ -886,6 +900,19 @@ int vfvm_batch_set_graphs(vfvm_batch *b, int enable) {
     b->graphs.clear();
   }
   return VFVM_OK;
+}
+
+int vfvm_batch_set_sell_index(vfvm_batch *b, int bits) {
+  if (!b || (bits != 16 && bits != 32)) return VFVM_ERR_ARG;
+  return bguard(b->ctx, [&]() {
+    if (b->sell16 != (bits == 16)) {
+      VFVM_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+      for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);  // captured with the other kernels
+      b->graphs.clear();
+    }
+    b->sell16 = bits == 16;
+    return VFVM_OK;
+  });
 }
 
 int vfvm_batch_destroy(vfvm_batch *b) {

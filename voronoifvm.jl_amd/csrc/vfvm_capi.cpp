@@ -191,6 +191,7 @@ void setup_linear(vfvm_sys &s, const vfvm_control_t &c) {
   L.dofdiag = s.d_dofdiag.p;
   L.sell_ptr = s.d_sell_ptr.p;
   L.sell_col = s.d_sell_col.p;
+  L.sell_off16 = s.d_sell_off16.p;
   L.sell_rbase = s.d_sell_rbase.p;
   L.sell_nnz = s.sell_nnz;
   L.ilu = s.ilu_built ? &s.ilu : nullptr;
@@ -536,10 +537,12 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     s->d_colind.upload(s->colind, st);
     {
       std::vector<int32_t> sptr, scol, segsl, rbase;
-      build_sell64(s->ndof, s->rowptr, s->colind, {0}, {s->ndof}, sptr, scol, segsl, rbase);
+      std::vector<int16_t> off16;
+      build_sell64(s->ndof, s->rowptr, s->colind, {0}, {s->ndof}, sptr, scol, segsl, rbase, &off16);
       s->d_sell_ptr.upload(sptr, st);
       s->d_sell_col.upload(scol, st);
       s->d_sell_rbase.upload(rbase, st);
+      if (!off16.empty()) s->d_sell_off16.upload(off16, st);
       s->sell_nnz = (int64_t)scol.size();
     }
     s->d_csc_perm.upload(s->csc_perm, st);
@@ -936,6 +939,14 @@ int vfvm_set_groups(vfvm_sys *s, int32_t asm_group, int32_t spmv_group) {
   s->spmv_group = spmv_group;
   s->lin.set_group(spmv_group);
   return VFVM_OK;
+}
+
+int vfvm_set_sell_index(vfvm_sys *s, int bits) {
+  if (!s || (bits != 16 && bits != 32)) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    s->lin.sell16 = bits == 16;
+    return VFVM_OK;
+  });
 }
 
 int vfvm_autotune(vfvm_sys *s, int32_t *asm_group, int32_t *spmv_group) {

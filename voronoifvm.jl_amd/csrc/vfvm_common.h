@@ -127,10 +127,21 @@ void build_csc_perm(int64_t ndof, const std::vector<int32_t> &rowptr, const std:
                     std::vector<int64_t> &colptr, std::vector<int32_t> &rowval, std::vector<int32_t> &perm);
 int greedy_colors(int64_t n, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
                   std::vector<int32_t> &color);
+// column of a SELL-64 entry: an absolute int32 index, or an int16 offset from the row
+template <class CI>
+__device__ __forceinline__ int64_t sell_column(CI c, int64_t row) {
+  if constexpr (sizeof(CI) == 2)
+    return row + (int64_t)c;
+  else
+    return (int64_t)c;
+}
+
+// off16 (optional): every entry's column as an offset from its row, 16 bits; left
+// empty if some row reaches further than 32767 rows
 void build_sell64(int64_t nrows, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
                   const std::vector<int64_t> &seg_beg, const std::vector<int64_t> &seg_end,
                   std::vector<int32_t> &sptr, std::vector<int32_t> &scol, std::vector<int32_t> &seg_slice,
-                  std::vector<int32_t> &rbase);
+                  std::vector<int32_t> &rbase, std::vector<int16_t> *off16 = nullptr);
 
 // ---- kernel launch interfaces (vfvm_assemble.hip, vfvm_linalg.hip) ----
 struct AsmArgs {

@@ -106,6 +106,7 @@ struct LinSetup {
   // SELL-64 pattern of the operator (one segment: slice r / 64, lane r % 64) and the
   // position of each row's first entry
   const int32_t *sell_ptr = nullptr, *sell_col = nullptr, *sell_rbase = nullptr;
+  const int16_t *sell_off16 = nullptr;  // columns as 16-bit offsets from the row (null: too far)
   int64_t sell_nnz = 0;
   const IluStruct *ilu = nullptr;
   int spmv_group = 8;
@@ -181,6 +182,8 @@ class LinearSolver {
   // set when an FP32 refinement fails: later solves with FP32 requested run in FP64 until
   // the owner clears it (at the start of each Newton solve or time/parameter sweep)
   bool fp32_off = false;
+  // SpMV reads 16-bit column offsets when the pattern allows (false: 32-bit columns)
+  bool sell16 = true;
 
  private:
   bool sell_nt[2] = {false, false};

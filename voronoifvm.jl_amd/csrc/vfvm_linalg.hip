@@ -17,6 +17,8 @@
 // host can enqueue several iterations between status reads.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 
@@ -137,9 +139,9 @@ __device__ __forceinline__ T ldm(const T *p) {
     return *p;
 }
 
-template <class VA, class VX, int NDOT, int STAGE, bool NT>
+template <class VA, class VX, int NDOT, int STAGE, bool NT, class CI>
 __global__ void __launch_bounds__(VB) k_sell_spmv(int64_t nrows, const int32_t *__restrict__ sptr,
-                                                  const int32_t *__restrict__ col, const VA *__restrict__ val,
+                                                  const CI *__restrict__ col, const VA *__restrict__ val,
                                                   const VX *__restrict__ x, VX *__restrict__ y,
                                                   const VX *__restrict__ w0, double *partials, KState *st) {
   if (st && idle(st)) return;
@@ -151,7 +153,7 @@ __global__ void __launch_bounds__(VB) k_sell_spmv(int64_t nrows, const int32_t *
     double acc = 0.0;
 #pragma unroll 4
     for (int32_t q = sptr[sl] + (int32_t)(r & 63); q < p1; q += 64)
-      acc += (double)ldm<NT>(val + q) * (double)x[ldm<NT>(col + q)];
+      acc += (double)ldm<NT>(val + q) * (double)x[sell_column(ldm<NT>(col + q), r)];
     const VX yv = (VX)acc;
     y[r] = yv;
     if (NDOT >= 1) d0 += (double)w0[r] * (double)yv;
@@ -910,19 +912,28 @@ void LinearSolver::apply_op(const V *x, V *y, const V *w0, const V *w1, KState *
   const unsigned g = grid_for(L.ndof);
   const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
   if (L.probe) L.probe->before(kid, stream);
-  auto go = [&](auto ntc) {
+  auto go = [&](auto ntc, const auto *col) {
     constexpr bool NT = decltype(ntc)::value;
+    using CI = std::remove_const_t<std::remove_pointer_t<decltype(col)>>;
     if constexpr (sizeof(V) == 4)
-      VFVM_LAUNCH((k_sell_spmv<float, V, NDOT, STAGE, NT>), dim3(g), dim3(VB), 0, stream, L.ndof, L.sell_ptr,
-                  L.sell_col, (const float *)A32.p, x, y, w0, partials.p, stp);
+      VFVM_LAUNCH((k_sell_spmv<float, V, NDOT, STAGE, NT, CI>), dim3(g), dim3(VB), 0, stream, L.ndof, L.sell_ptr, col,
+                  (const float *)A32.p, x, y, w0, partials.p, stp);
     else
-      VFVM_LAUNCH((k_sell_spmv<double, V, NDOT, STAGE, NT>), dim3(g), dim3(VB), 0, stream, L.ndof, L.sell_ptr,
-                  L.sell_col, (const double *)A64s.p, x, y, w0, partials.p, stp);
+      VFVM_LAUNCH((k_sell_spmv<double, V, NDOT, STAGE, NT, CI>), dim3(g), dim3(VB), 0, stream, L.ndof, L.sell_ptr,
+                  col, (const double *)A64s.p, x, y, w0, partials.p, stp);
   };
-  if (sell_nt[sizeof(V) == 4 ? 0 : 1])
-    go(std::true_type());
-  else
-    go(std::false_type());
+  const bool nt = sell_nt[sizeof(V) == 4 ? 0 : 1];
+  if (L.sell_off16 && sell16) {
+    if (nt)
+      go(std::true_type(), L.sell_off16);
+    else
+      go(std::false_type(), L.sell_off16);
+  } else {
+    if (nt)
+      go(std::true_type(), L.sell_col);
+    else
+      go(std::false_type(), L.sell_col);
+  }
   if (L.probe) L.probe->after(kid, stream, L.ndof, L.nnz);
   if constexpr (NDOT > 0)
     hipLaunchKernelGGL((k_finalize<2, STAGE>), dim3(1), dim3(VB), 0, stream, (const double *)partials.p, (int)g, stp, 1);

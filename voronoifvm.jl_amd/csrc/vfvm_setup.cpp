@@ -456,7 +456,7 @@ int greedy_colors(int64_t n, const std::vector<int32_t> &rowptr, const std::vect
 void build_sell64(int64_t nrows, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
                   const std::vector<int64_t> &seg_beg, const std::vector<int64_t> &seg_end,
                   std::vector<int32_t> &sptr, std::vector<int32_t> &scol, std::vector<int32_t> &seg_slice,
-                  std::vector<int32_t> &rbase) {
+                  std::vector<int32_t> &rbase, std::vector<int16_t> *off16) {
   const size_t nseg = seg_beg.size();
   seg_slice.assign(nseg + 1, 0);
   std::vector<int64_t> sl_beg, sl_end;
@@ -480,6 +480,8 @@ void build_sell64(int64_t nrows, const std::vector<int32_t> &rowptr, const std::
   }
   scol.assign((size_t)tot, 0);
   rbase.assign((size_t)nrows, 0);
+  bool fits16 = off16 != nullptr;
+  if (off16) off16->assign((size_t)tot, 0);
   for (size_t sl = 0; sl < ns; sl++) {
     const int32_t w = (sptr[sl + 1] - sptr[sl]) / 64;
     for (int l = 0; l < 64; l++) {
@@ -488,9 +490,20 @@ void build_sell64(int64_t nrows, const std::vector<int32_t> &rowptr, const std::
       const int32_t self = (int32_t)(real ? r : sl_end[sl] - 1);
       const int32_t k0 = real ? rowptr[r] : 0, len = real ? rowptr[r + 1] - rowptr[r] : 0;
       if (real) rbase[(size_t)r] = sptr[sl] + l;
-      for (int32_t j = 0; j < w; j++) scol[(size_t)sptr[sl] + (size_t)j * 64 + l] = j < len ? colind[k0 + j] : self;
+      for (int32_t j = 0; j < w; j++) {
+        const size_t q = (size_t)sptr[sl] + (size_t)j * 64 + l;
+        scol[q] = j < len ? colind[k0 + j] : self;
+        if (fits16 && real) {
+          const int64_t d = (int64_t)scol[q] - r;  // padding entries: the row itself, 0
+          if (d < -32768 || d > 32767)
+            fits16 = false;
+          else
+            (*off16)[q] = (int16_t)d;
+        }
+      }
     }
   }
+  if (off16 && !fits16) off16->clear();
 }
 
 }  // namespace vfvm

@@ -31,6 +31,7 @@ struct vfvm_sys {
       d_ilu_scol, d_ilu_sdpos;
   // SELL-64 copy of the Jacobian pattern for the Krylov operator
   DevBuf<int32_t> d_sell_ptr, d_sell_col, d_sell_rbase;
+  DevBuf<int16_t> d_sell_off16;  // empty when some row reaches further than 32767 rows
   int64_t sell_nnz = 0;
   LinearSolver lin;
   int asm_group = 8, spmv_group = 8;

@@ -185,6 +185,11 @@ class System:
         _capi.check(_capi.lib().vfvm_autotune(self.handle(), C.byref(a), C.byref(s)), self.ctx.handle)
         return a.value, s.value
 
+    def set_sell_index(self, bits: int):
+        """SpMV column indices: 16-bit offsets from the row (default where they fit) or
+        32-bit absolute columns."""
+        _capi.check(_capi.lib().vfvm_set_sell_index(self.handle(), int(bits)), self.ctx.handle)
+
     def set_groups(self, asm_group: int, spmv_group: int = 8):
         """Choose the assembly kernel variant (lanes per node; 1 and 0: one node per lane
         with LDS staging) and the lanes per row of the CSR residual kernels."""
@@ -374,6 +379,10 @@ class Batch:
         """Run the batched Krylov iteration blocks as captured HIP graphs (default) or
         launch their kernels one by one."""
         _capi.check(_capi.lib().vfvm_batch_set_graphs(self._h, int(enable)), self.ctx.handle)
+
+    def set_sell_index(self, bits: int):
+        """The batched SpMV's column indices: 16-bit offsets or 32-bit columns."""
+        _capi.check(_capi.lib().vfvm_batch_set_sell_index(self._h, int(bits)), self.ctx.handle)
 
     def sizes(self):
         """Total dofs and the member offsets of the packed (member-order) dof vector."""
