@@ -428,31 +428,34 @@ def cpu_cfg3(args):
 
 
 # --------------------------------------------------------------------------- cfg4
+CFG4_BATCH = 512  # members per batched solve (the 1,024 held-out seeds form 2 blocks)
 def run_cfg4(args):
     """Learned initial guess + batching + FP32 Krylov vs plain FP64 GPU Newton from zero,
     on the 1,024 held-out seeds of the 129 x 129 nonlinear Poisson family."""
     from vfvm import surrogate as S
-    fam = S.PoissonFamily(size=256)
+    B = CFG4_BATCH
+    fam = S.PoissonFamily(size=B)
     fam.batch.set_graphs(not args.no_graphs)
     net = S.Surrogate(fam.systems[0])
     meta = net.load()
     seeds = list(S.EVAL_SEEDS)
-    K = min(args.steps, len(seeds) // 256)
-    blocks = [seeds[256 * k:256 * (k + 1)] for k in range(K)]
+    K = args.steps
+    # the held-out seeds in blocks of B; step k solves block k mod (number of blocks)
+    blocks = [seeds[B * k:B * (k + 1)] for k in range(len(seeds) // B)]
     ctrl32 = vfvm.SolverControl(precision="fp32", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=3000,
                                 inner_reltol=INNER_RELTOL, max_refinements=12, check_every=8)
     ctx = fam.systems[0].ctx
     n = fam.grid.num_nodes
     ndof, off = fam.batch.sizes()
-    assert ndof == 256 * n and (np.diff(off) == n).all()  # packed member order = sample-major
+    assert ndof == B * n and (np.diff(off) == n).all()  # packed member order = sample-major
     # device-resident inputs and results (the timed region starts with the sources and
     # side values in HBM and leaves the solutions there)
-    F_dev, V_dev = vfvm.DeviceArray(ctx, 256 * n), vfvm.DeviceArray(ctx, 256 * 4)
+    F_dev, V_dev = vfvm.DeviceArray(ctx, B * n), vfvm.DeviceArray(ctx, B * 4)
     G_dev, U_dev = vfvm.DeviceArray(ctx, ndof), vfvm.DeviceArray(ctx, ndof)
     Z_dev = vfvm.DeviceArray.from_numpy(ctx, np.zeros(ndof))
 
     def surrogate_solve():
-        net.predict_sides_device(F_dev, V_dev, G_dev, 256)  # features, smoothing, network
+        net.predict_sides_device(F_dev, V_dev, G_dev, B)  # features, smoothing, network
         fam.batch.solve_device(U_dev, G_dev, ctrl32)
 
     # warm-up (untimed): one block through both batched variants
@@ -470,7 +473,9 @@ def run_cfg4(args):
     fam.batch.solve_device(U_dev, Z_dev, ctrl64b)  # warm-up of the FP64 batched path
     converged = []
     probes = {_capi.KERNEL_SPMV32: [], _capi.KERNEL_ASSEMBLE: []}
-    for blk in blocks:
+    for step in range(K):
+        blk = blocks[step % len(blocks)]
+        first_pass = step < len(blocks)
         raw = fam.load_seeds(blk)  # the inputs (source, side values) and their upload: not timed
         F_dev.upload(raw[0])
         V_dev.upload(raw[1])
@@ -479,8 +484,8 @@ def run_cfg4(args):
         t_sur += time.perf_counter() - t1
         it_sur += [h.niter for h in fam.batch.histories]
         worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
-        g = G_dev.numpy().reshape(256, n)
-        U = U_dev.numpy().reshape(256, n)
+        g = G_dev.numpy().reshape(B, n)
+        U = U_dev.numpy().reshape(B, n)
         # the same through the host-buffer entry points (PCIe transfers and packing included)
         t1 = time.perf_counter()
         gh = net.predict_sides(*raw)
@@ -501,14 +506,15 @@ def run_cfg4(args):
         fam.batch.solve_device(U_dev, Z_dev, ctrl64b)
         t_zero64 += time.perf_counter() - t1
         worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
-        for m in range(256):
+        for m in range(B):
             sur_err.append(float(np.abs(g[m] - U[m]).max()))
             if not np.abs(vfvm.dofs(Uh[m]) - U[m]).max() <= 1e-9:
                 raise RuntimeError("cfg4: host-buffer and device-resident solutions differ")
-        converged += [U[m].copy() for m in range(256)]
+        if first_pass:
+            converged += [U[m].copy() for m in range(B)]
     if not worst <= 1e-10:
         raise RuntimeError(f"cfg4: a solve ended with ||F||_inf = {worst:.3e}")
-    nsol = 256 * K
+    nsol = B * K
     sur_rate, zero_rate = nsol / t_sur, nsol / t_zero
     log(f"surrogate+batched+FP32 through the host-buffer entry points (PCIe and packing included): "
         f"{nsol / t_sur_host:.1f} solves/s")
@@ -523,7 +529,7 @@ def run_cfg4(args):
     vfvm.solve(single, inival=0.0, control=ctrl64)
     t_plain, it_plain, diff = 0.0, [], 0.0
     k = 0
-    for blk in blocks:
+    for blk in blocks[:min(K, len(blocks))]:  # every distinct seed the timed steps solved
         for sd in blk:
             f, vals = S.draw(sd, fam.grf)
             single.set_source(np.asfortranarray(f.reshape(1, -1)))
@@ -537,8 +543,8 @@ def run_cfg4(args):
             it_plain.append(single.history.niter)
             diff = max(diff, float(np.abs(vfvm.dofs(Up) - converged[k]).max()))
             k += 1
-    plain_rate = nsol / t_plain
-    log(f"plain FP64 GPU Newton from zero: {nsol} solves in {t_plain:.2f}s -> {plain_rate:.1f} solves/s, "
+    plain_rate = k / t_plain
+    log(f"plain FP64 GPU Newton from zero: {k} solves in {t_plain:.2f}s -> {plain_rate:.1f} solves/s, "
         f"mean newton {np.mean(it_plain):.2f}; max |U_sur - U_plain| = {diff:.2e}")
     # continuation on a sample: source and side values ramped by the embedding parameter
     # over [0, 1] (solve(...; embed)), plain FP64 GPU Newton at every accepted step
@@ -578,12 +584,12 @@ def run_cfg4(args):
     pspmv, pasm = merged(probes[_capi.KERNEL_SPMV32]), merged(probes[_capi.KERNEL_ASSEMBLE])
     rl_spmv = roofline_hbm(probe_bytes("spmv", pspmv, sv=4), pspmv, read_traffic("cfg4_bspmv32"),
                            "batched FP32 SpMV")
-    rl_spmv["kernel"] = "batched FP32 SpMV over the active members of 256 x 129^2 (k_bspmv)"
+    rl_spmv["kernel"] = f"batched FP32 SpMV over the active members of {B} x 129^2 (k_bspmv)"
     rl_asm = roofline_hbm(probe_bytes("assemble", pasm, transient=False), pasm, read_traffic("cfg4_assemble"),
                           "batched assembly")
     import ctypes as C
     v = C.c_double()
-    rows = 256 * fam.grid.num_nodes  # one inference batch
+    rows = B * fam.grid.num_nodes  # one inference batch
     _capi.check(_capi.lib().vfvm_mpnn_time_layer(net._h, rows, 20, C.byref(v)))
     t_layer = v.value
     layer_bytes = rows * 64 * 4 * 2  # h_l read once, h_{l+1} written once (FP32, 64 wide)
@@ -614,11 +620,12 @@ def run_cfg4(args):
                                          mfma_tflops=round(layer_tflops, 2), rows=rows,
                                          kernel="k_layer_infer: graph aggregation + 128x64 BF16 MFMA GEMM + residual"),
                 config=dict(workload="cfg4: 129x129 nonlinear Poisson (eps=1e-2, GRF source ell=0.1 amp=5, side "
-                                     "values U[-1,1]), 1,024 held-out seeds in batches of 256: MPNN initial guess "
+                                     "values U[-1,1]), 1,024 held-out seeds in 2 batches of 512 (step k solves "
+                                     "batch k mod 2): MPNN initial guess "
                                      "(4 layers, width 64, BF16 MFMA) + batched BiCGStab/Jacobi FP32 inside FP64 "
                                      "Newton, vs plain FP64 GPU Newton from zero one solve at a time; "
                                      "reltol_linear 1e-4 (the reference's default) in every form",
-                            solves=nsol, unknowns_per_solve=fam.grid.num_nodes, batch=256,
+                            solves=nsol, unknowns_per_solve=fam.grid.num_nodes, batch=B,
                             hip_graphs=not args.no_graphs,
                             resident="sources, side values, guesses and solutions in HBM (the *_device "
                                      "entry points); host_buffer_path_solves_per_s includes the PCIe "
