@@ -176,6 +176,50 @@ __global__ void __launch_bounds__(BB) k_bspmv(Seg S, const int32_t *__restrict__
   }
 }
 
+// The BiCGStab half step and the second operator application in one launch:
+// s = r - alpha v for the chunk's rows (written) and, for every column the SpMV
+// gathers, recomputed from r and v with the same fused multiply-add, t = Ahat s,
+// and the dots (s, t), (t, t), (s, s) for STAGE_SOMEGA.
+template <class V>
+__device__ __forceinline__ V bicg_s(V r, V v, double alpha) {
+  return (V)__builtin_fma(-alpha, (double)v, (double)r);
+}
+
+template <class VA, class V, bool NT, class CI>
+__global__ void __launch_bounds__(BB) k_bspmv_s(Seg S, const int32_t *__restrict__ ch_slice,
+                                                const int32_t *__restrict__ sptr, const CI *__restrict__ col,
+                                                const VA *__restrict__ val, const V *__restrict__ r,
+                                                const V *__restrict__ v, V *__restrict__ s, V *__restrict__ t,
+                                                KState *st) {
+  const int inst = S.ch_inst[blockIdx.x];
+  const KState *sti = st + inst;
+  if (sti->done) return;
+  const double alpha = sti->alpha;
+  const int64_t r0 = S.ch_beg[blockIdx.x], r1 = S.ch_end[blockIdx.x];
+  const int32_t sl0 = ch_slice[blockIdx.x];
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  for (int64_t l = threadIdx.x; l < r1 - r0; l += BB) {
+    const int32_t sl = sl0 + (int32_t)(l >> 6);
+    const int32_t p1 = sptr[sl + 1];
+    const int64_t row = r0 + l;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int32_t q = sptr[sl] + (int32_t)(l & 63); q < p1; q += 64) {
+      const int64_t j = sell_column(bld<NT>(col + q), row);
+      acc += (double)bld<NT>(val + q) * (double)bicg_s(r[j], v[j], alpha);
+    }
+    const V tv = (V)acc;
+    const V sv = bicg_s(r[row], v[row], alpha);
+    t[row] = tv;
+    s[row] = sv;
+    d0 += (double)sv * (double)tv;
+    d1 += (double)tv * (double)tv;
+    d2 += (double)sv * (double)sv;
+  }
+  double vv[3] = {d0, d1, d2};
+  seg_partials<3, false>(S, vv, 0.0);
+}
+
 // y = S (b - A x), ||y||^2 per member -> st.rr (FP64; refinement residual and bhat)
 template <int G>
 __global__ void __launch_bounds__(BB) k_bres(Seg S, const int32_t *__restrict__ rowptr,
@@ -503,6 +547,38 @@ void bop(vfvm_batch &b, const V *x, V *y, const V *w0) {
                        b.st.p, (const int32_t *)nullptr, (double *)nullptr);
 }
 
+// s = r - alpha v, t = Ahat s, and STAGE_SOMEGA (see k_bspmv_s)
+template <class V>
+void bop_s(vfvm_batch &b, const V *r, const V *v, V *s, V *t) {
+  const Seg S = b.seg();
+  auto go = [&](auto ntc, const auto *col) {
+    constexpr bool NT = decltype(ntc)::value;
+    using CI = std::remove_const_t<std::remove_pointer_t<decltype(col)>>;
+    if constexpr (sizeof(V) == 4)
+      hipLaunchKernelGGL((k_bspmv_s<float, V, NT, CI>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                         (const int32_t *)b.ch_slice.p, (const int32_t *)b.sell_ptr.p, col, (const float *)b.A32.p, r,
+                         v, s, t, b.st.p);
+    else
+      hipLaunchKernelGGL((k_bspmv_s<double, V, NT, CI>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                         (const int32_t *)b.ch_slice.p, (const int32_t *)b.sell_ptr.p, col, (const double *)b.A64s.p,
+                         r, v, s, t, b.st.p);
+  };
+  const bool nt = b.sell_nt[sizeof(V) == 4 ? 0 : 1];
+  if (b.sell_off16.p && b.sell16) {
+    if (nt)
+      go(std::true_type(), (const int16_t *)b.sell_off16.p);
+    else
+      go(std::false_type(), (const int16_t *)b.sell_off16.p);
+  } else {
+    if (nt)
+      go(std::true_type(), (const int32_t *)b.sell_col.p);
+    else
+      go(std::false_type(), (const int32_t *)b.sell_col.p);
+  }
+  hipLaunchKernelGGL((k_bfinalize<3, STAGE_SOMEGA, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, b.ctx->stream, S,
+                     b.st.p, (const int32_t *)nullptr, (double *)nullptr);
+}
+
 void bres(vfvm_batch &b, const double *x, const double *rhs, double *y, const int32_t *active) {
   const Seg S = b.seg();
   auto go = [&](auto gc) {
@@ -576,10 +652,7 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
     for (int q = 0; q < every; q++) {
       hipLaunchKernelGGL((k_bp<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, r, p, v, b.st.p);
       bop<V, 1, STAGE_ALPHA>(b, p, v, r0);
-      hipLaunchKernelGGL((k_bs<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, r, v, s, b.st.p);
-      hipLaunchKernelGGL((k_bfinalize<1, STAGE_SNORM, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S, b.st.p,
-                         (const int32_t *)nullptr, (double *)nullptr);
-      bop<V, 2, STAGE_OMEGA>(b, s, t, s);
+      bop_s<V>(b, r, v, s, t);  // s = r - alpha v, t = Ahat s, half-step test and omega
       hipLaunchKernelGGL((k_bxr<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, x, p, s, s, t, r, r0, b.st.p);
       hipLaunchKernelGGL((k_bfinalize<2, STAGE_XR, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S, b.st.p,
                          (const int32_t *)nullptr, (double *)nullptr);

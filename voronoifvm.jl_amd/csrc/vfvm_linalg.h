@@ -24,7 +24,9 @@ struct KState {
   unsigned pad;
 };
 
-enum { STAGE_NONE = 0, STAGE_INIT, STAGE_ALPHA, STAGE_SNORM, STAGE_OMEGA, STAGE_XR };
+// STAGE_SOMEGA: SNORM and OMEGA from one reduction ((s, t), (t, t), (s, s)), for the
+// solver that computes s inside the second operator application
+enum { STAGE_NONE = 0, STAGE_INIT, STAGE_ALPHA, STAGE_SNORM, STAGE_OMEGA, STAGE_XR, STAGE_SOMEGA };
 
 template <int STAGE>
 __device__ __forceinline__ void run_stage(KState *st, const double *tot) {
@@ -57,6 +59,12 @@ __device__ __forceinline__ void run_stage(KState *st, const double *tot) {
     if (tot[0] <= st->tol2) st->half = 1;
   } else if constexpr (STAGE == STAGE_OMEGA) {
     st->omega = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
+  } else if constexpr (STAGE == STAGE_SOMEGA) {
+    st->ss = tot[2];
+    if (tot[2] <= st->tol2)
+      st->half = 1;
+    else
+      st->omega = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
   } else if constexpr (STAGE == STAGE_XR) {
     st->iters += 1;
     if (st->half) {
