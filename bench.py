@@ -432,92 +432,110 @@ CFG4_BATCH = 512  # members per batched solve (the 1,024 held-out seeds form 2 b
 def run_cfg4(args):
     """Learned initial guess + batching + FP32 Krylov vs plain FP64 GPU Newton from zero,
     on the 1,024 held-out seeds of the 129 x 129 nonlinear Poisson family."""
+    from concurrent.futures import ThreadPoolExecutor
     from vfvm import surrogate as S
+    from vfvm.system import System
     B = CFG4_BATCH
-    fam = S.PoissonFamily(size=B)
-    fam.batch.set_graphs(not args.no_graphs)
-    net = S.Surrogate(fam.systems[0])
-    meta = net.load()
     seeds = list(S.EVAL_SEEDS)
     K = args.steps
-    # the held-out seeds in blocks of B; step k solves block k mod (number of blocks)
+    # the held-out seeds in blocks of B, one batch per block (each holds its block's
+    # sources and side values); step k solves block k mod (number of blocks)
     blocks = [seeds[B * k:B * (k + 1)] for k in range(len(seeds) // B)]
+    nb = len(blocks)
+    fams = [S.PoissonFamily(size=B) for _ in range(nb)]
+    fam = fams[0]
+    for f in fams:
+        f.batch.set_graphs(not args.no_graphs)
     ctrl32 = vfvm.SolverControl(precision="fp32", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=3000,
                                 inner_reltol=INNER_RELTOL, max_refinements=12, check_every=8)
     ctx = fam.systems[0].ctx
     n = fam.grid.num_nodes
     ndof, off = fam.batch.sizes()
     assert ndof == B * n and (np.diff(off) == n).all()  # packed member order = sample-major
-    # device-resident inputs and results (the timed region starts with the sources and
-    # side values in HBM and leaves the solutions there)
-    F_dev, V_dev = vfvm.DeviceArray(ctx, B * n), vfvm.DeviceArray(ctx, B * 4)
-    G_dev, U_dev = vfvm.DeviceArray(ctx, ndof), vfvm.DeviceArray(ctx, ndof)
+    # the network on a second context (its own stream): the next block's inference runs
+    # while the current block's Newton solve occupies the first
+    ctx2 = _capi.Context(0)
+    net = S.Surrogate(System(fam.grid, fam.systems[0].physics, species=[1], context=ctx2))
+    meta = net.load()
+    # device-resident inputs and results (the timed region starts with every block's
+    # sources and side values in HBM and leaves the guesses and solutions there)
+    raws = [f.load_seeds(blk) for f, blk in zip(fams, blocks)]
+    F_dev = [vfvm.DeviceArray.from_numpy(ctx, r[0]) for r in raws]
+    V_dev = [vfvm.DeviceArray.from_numpy(ctx, r[1]) for r in raws]
+    G_dev = [vfvm.DeviceArray(ctx, ndof) for _ in range(nb)]
+    U_dev = [vfvm.DeviceArray(ctx, ndof) for _ in range(nb)]
     Z_dev = vfvm.DeviceArray.from_numpy(ctx, np.zeros(ndof))
 
-    def surrogate_solve():
-        net.predict_sides_device(F_dev, V_dev, G_dev, B)  # features, smoothing, network
-        fam.batch.solve_device(U_dev, G_dev, ctrl32)
+    def infer(k):
+        net.predict_sides_device(F_dev[k % nb], V_dev[k % nb], G_dev[k % nb], B)  # features, smoothing, network
 
-    # warm-up (untimed): one block through both batched variants
-    raw = fam.load_seeds(blocks[0])
-    F_dev.upload(raw[0])
-    V_dev.upload(raw[1])
-    for _ in range(max(args.warmup, 0)):
-        surrogate_solve()
-        fam.batch.solve_device(U_dev, Z_dev, ctrl32)
-        g = net.predict_sides(*raw)  # the host-buffer path too
-        fam.batch.solve([x.reshape(1, -1) for x in g], ctrl32)
-    t_sur, t_sur_host, t_zero, t_zero64, it_sur, it_zero, sur_err, worst = 0.0, 0.0, 0.0, 0.0, [], [], [], 0.0
+    def pipeline(steps):
+        """steps batched solves; inference of step k + 1 overlaps the Newton solve of step k"""
+        hist = []
+        infer(0)
+        with ThreadPoolExecutor(max_workers=1) as ex:
+            for k in range(steps):
+                nxt = ex.submit(infer, k + 1) if k + 1 < steps else None
+                fams[k % nb].batch.solve_device(U_dev[k % nb], G_dev[k % nb], ctrl32)
+                hist.append(list(fams[k % nb].batch.histories))
+                if nxt is not None:
+                    nxt.result()
+        return hist
+
     ctrl64b = vfvm.SolverControl(precision="fp64", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=3000,
                                  check_every=8)
-    fam.batch.solve_device(U_dev, Z_dev, ctrl64b)  # warm-up of the FP64 batched path
+    for _ in range(max(args.warmup, 0)):  # warm-up (untimed): every batch, every path
+        pipeline(nb)
+        for f in fams:
+            f.batch.solve_device(U_dev[0], Z_dev, ctrl32)
+            f.batch.solve_device(U_dev[0], Z_dev, ctrl64b)
+        net.predict_sides(*raws[0])
+    # timed: K steps back to back
+    t1 = time.perf_counter()
+    hist = pipeline(K)
+    t_sur = time.perf_counter() - t1
+    it_sur = [h.niter for hs in hist for h in hs]
+    worst = max(h.residual_norm for hs in hist for h in hs)
+    # beside it, per block: the host-buffer path, a probe pass, and the batched solves from zero
+    t_sur_host, t_zero, t_zero64, it_zero, sur_err = 0.0, 0.0, 0.0, [], []
     converged = []
     probes = {_capi.KERNEL_SPMV32: [], _capi.KERNEL_ASSEMBLE: []}
-    for step in range(K):
-        blk = blocks[step % len(blocks)]
-        first_pass = step < len(blocks)
-        raw = fam.load_seeds(blk)  # the inputs (source, side values) and their upload: not timed
-        F_dev.upload(raw[0])
-        V_dev.upload(raw[1])
-        t1 = time.perf_counter()
-        surrogate_solve()
-        t_sur += time.perf_counter() - t1
-        it_sur += [h.niter for h in fam.batch.histories]
-        worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
-        g = G_dev.numpy().reshape(B, n)
-        U = U_dev.numpy().reshape(B, n)
-        # the same through the host-buffer entry points (PCIe transfers and packing included)
-        t1 = time.perf_counter()
-        gh = net.predict_sides(*raw)
-        Uh = fam.batch.solve([x.reshape(1, -1) for x in gh], ctrl32)
+    for b in range(min(K, nb)):
+        f = fams[b]
+        infer(b)
+        f.batch.solve_device(U_dev[b], G_dev[b], ctrl32)
+        g = G_dev[b].numpy().reshape(B, n)
+        U = U_dev[b].numpy().reshape(B, n)
+        t1 = time.perf_counter()  # the host-buffer entry points (PCIe transfers and packing included)
+        gh = net.predict_sides(*raws[b])
+        Uh = f.batch.solve([x.reshape(1, -1) for x in gh], ctrl32)
         t_sur_host += time.perf_counter() - t1
-        worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
-        # probe pass: the same batch again with the kernel launches timed one by one
-        with ctx.probe(_capi.KERNEL_SPMV32, _capi.KERNEL_ASSEMBLE) as pr:
-            fam.batch.solve_device(U_dev, G_dev, ctrl32)
+        worst = max(worst, max(h.residual_norm for h in f.batch.histories))
+        with ctx.probe(_capi.KERNEL_SPMV32, _capi.KERNEL_ASSEMBLE) as pr:  # kernel launches timed one by one
+            f.batch.solve_device(U_dev[b], G_dev[b], ctrl32)
         for k, v in pr.items():
             probes[k].append(v)
         t1 = time.perf_counter()
-        fam.batch.solve_device(U_dev, Z_dev, ctrl32)
+        f.batch.solve_device(U_dev[b], Z_dev, ctrl32)
         t_zero += time.perf_counter() - t1
-        it_zero += [h.niter for h in fam.batch.histories]
-        worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
+        it_zero += [h.niter for h in f.batch.histories]
+        worst = max(worst, max(h.residual_norm for h in f.batch.histories))
         t1 = time.perf_counter()  # batching alone: FP64 Krylov from zero
-        fam.batch.solve_device(U_dev, Z_dev, ctrl64b)
+        f.batch.solve_device(U_dev[b], Z_dev, ctrl64b)
         t_zero64 += time.perf_counter() - t1
-        worst = max(worst, max(h.residual_norm for h in fam.batch.histories))
+        worst = max(worst, max(h.residual_norm for h in f.batch.histories))
         for m in range(B):
             sur_err.append(float(np.abs(g[m] - U[m]).max()))
             if not np.abs(vfvm.dofs(Uh[m]) - U[m]).max() <= 1e-9:
                 raise RuntimeError("cfg4: host-buffer and device-resident solutions differ")
-        if first_pass:
-            converged += [U[m].copy() for m in range(B)]
+        converged += [U[m].copy() for m in range(B)]
     if not worst <= 1e-10:
         raise RuntimeError(f"cfg4: a solve ended with ||F||_inf = {worst:.3e}")
+    nfill = B * min(K, nb)
     nsol = B * K
-    sur_rate, zero_rate = nsol / t_sur, nsol / t_zero
-    log(f"surrogate+batched+FP32 through the host-buffer entry points (PCIe and packing included): "
-        f"{nsol / t_sur_host:.1f} solves/s")
+    sur_rate, zero_rate = nsol / t_sur, nfill / t_zero
+    log(f"surrogate+batched+FP32 through the host-buffer entry points (PCIe and packing included, no "
+        f"overlap): {nfill / t_sur_host:.1f} solves/s")
     log(f"surrogate+batched+FP32: {nsol} solves in {t_sur:.3f}s -> {sur_rate:.1f} solves/s, "
         f"mean newton {np.mean(it_sur):.2f}; batched+FP32 from zero {zero_rate:.1f} solves/s, mean newton "
         f"{np.mean(it_zero):.2f}; surrogate max-norm error mean {np.mean(sur_err):.3e} max {np.max(sur_err):.3e}")
@@ -599,8 +617,8 @@ def run_cfg4(args):
         f"{layer_gbs:.0f} GB/s, {layer_tflops:.1f} TFLOP/s")
     return dict(value=sur_rate, ms_per_step=1e3 * t_sur / K, steps=K, plain_gpu_newton_solves_per_s=plain_rate,
                 speedup_vs_plain=sur_rate / plain_rate, batched_fp32_from_zero_solves_per_s=zero_rate,
-                batched_fp64_from_zero_solves_per_s=nsol / t_zero64,
-                host_buffer_path_solves_per_s=nsol / t_sur_host,
+                batched_fp64_from_zero_solves_per_s=nfill / t_zero64,
+                host_buffer_path_solves_per_s=nfill / t_sur_host,
                 continuation_plain_solves_per_s=cont_rate,
                 continuation=dict(seeds=ncont, mean_accepted_steps=float(np.mean(st_cont)),
                                   mean_newton_iterations=float(np.mean(it_cont)),
@@ -621,7 +639,8 @@ def run_cfg4(args):
                                          kernel="k_layer_infer: graph aggregation + 128x64 BF16 MFMA GEMM + residual"),
                 config=dict(workload="cfg4: 129x129 nonlinear Poisson (eps=1e-2, GRF source ell=0.1 amp=5, side "
                                      "values U[-1,1]), 1,024 held-out seeds in 2 batches of 512 (step k solves "
-                                     "batch k mod 2): MPNN initial guess "
+                                     "batch k mod 2; the inference of step k+1 runs on a second stream during "
+                                     "the Newton solve of step k): MPNN initial guess "
                                      "(4 layers, width 64, BF16 MFMA) + batched BiCGStab/Jacobi FP32 inside FP64 "
                                      "Newton, vs plain FP64 GPU Newton from zero one solve at a time; "
                                      "reltol_linear 1e-4 (the reference's default) in every form",
