@@ -241,6 +241,10 @@ int vfvm_set_groups(vfvm_sys *sys, int32_t asm_group, int32_t spmv_group);
  * default wherever every row's columns lie within 32767 rows of it), 32 = absolute
  * 32-bit columns.  Same arithmetic either way; 16 bits move 2 bytes less per entry. */
 int vfvm_set_sell_index(vfvm_sys *sys, int bits);
+/* Node graph the assembly reads: 1 (default) = compact where it fits (neighbours as
+ * 16-bit offsets from the node, edge factors as 16-bit indices into a table of the
+ * graph's distinct values, bit-exact), 0 = int32 neighbours and FP64 factors per entry. */
+int vfvm_set_compact_graph(vfvm_sys *sys, int enable);
 
 /* Benchmark helper (no counterpart in the reference; it has no device): average device
  * time in milliseconds of `reps` back-to-back launches of one hot kernel on the handle's
@@ -287,6 +291,8 @@ int vfvm_batch_destroy(vfvm_batch *b);
 int vfvm_batch_set_graphs(vfvm_batch *b, int enable);
 /* the batch's SpMV column width, as vfvm_set_sell_index */
 int vfvm_batch_set_sell_index(vfvm_batch *b, int bits);
+/* the batch's assembly graph, as vfvm_set_compact_graph */
+int vfvm_batch_set_compact_graph(vfvm_batch *b, int enable);
 /* replace member m's source (nspec x nnodes; NULL keeps it) and boundary table (NULL keeps
  * it) inside the batch, for repeated solves of one grid with new data */
 int vfvm_batch_update_member(vfvm_batch *b, int64_t m, const double *src, const vfvm_bc_t *bc);

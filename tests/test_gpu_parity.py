@@ -346,3 +346,24 @@ def test_sell_index_widths_agree(precision):
             out.append((vfvm.dofs(U).copy(), sys.history.nlin))
             assert sys.history.residual_norm <= 1e-10
         assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+
+
+def test_compact_graph_is_bitwise_identical():
+    """The assembly reading the compact node graph (16-bit neighbour offsets, factor
+    table) gives bit for bit the residual and Jacobian of the plain graph, for every
+    kernel family and 1 and 3 species."""
+    _print("case compact graph")
+    for sys, osys in (cases.nonlinear_diffusion_3d(9), cases.drift_diffusion_2d(17, lambda2=1e-2),
+                      cases.nonlinear_poisson_2d(33)):
+        rng = np.random.default_rng(9)
+        U = rng.uniform(0.2, 1.0, (sys.num_species, sys.grid.num_nodes))
+        for group in (0, 1, 8):
+            sys.set_groups(group)
+            out = []
+            for compact in (True, False):
+                sys.set_compact_graph(compact)
+                F, A = vfvm.evaluate_residual_and_jacobian(sys, np.asfortranarray(U), embed=0.3)
+                out.append((vfvm.dofs(F).copy(), A.tocsc().data.copy()))
+            assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), group
+        sys.set_compact_graph(True)
+        cases.compare_assembly(sys, osys, U, embed=0.3)

@@ -40,10 +40,19 @@ __device__ __forceinline__ int rank_in(uint8_t mask, int j) {
   return __builtin_popcount((unsigned)mask & ((1u << j) - 1u));
 }
 
+// neighbour and edge factor of adjacency entry q of node K, from the compact graph
+// when the system has one
+__device__ __forceinline__ int32_t adj_node(const AsmArgs &a, int64_t q, int64_t K) {
+  return a.adj_off ? (int32_t)(K + a.adj_off[q]) : a.adj_col[q];
+}
+__device__ __forceinline__ double adj_factor(const AsmArgs &a, int64_t q) {
+  return a.adj_fidx ? a.fac_tab[a.adj_fidx[q]] : a.adj_fac[q];
+}
+
 // Node item (reaction, storage, source; src/vfvm_assembly.jl:81-122) and the
 // boundary items of node K (penalty Dirichlet, Robin; src/vfvm_assembly.jl:352-407).
 template <int NS, bool JAC>
-__device__ __forceinline__ void node_terms(const AsmArgs &a, const InstParams *P, const double (&p)[16],
+__device__ __forceinline__ void node_terms(const AsmArgs &a, const InstParams *P, const double *__restrict__ p,
                                            const double (&uK)[NS], const double (&uo)[NS],
                                            const double (&srcv)[NS], double vol, int32_t bb0, int32_t bb1,
                                            double (&Fn)[NS], double (&Jn)[NS][NS], double (&Fb)[NS],
@@ -124,9 +133,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
   if (a.inst_active && !a.inst_active[inst]) return;  // converged member of a batch
   const InstParams *P = a.inst + inst;
   const int flux_kind = P->flux;
-  double p[16];
-#pragma unroll
-  for (int k = 0; k < 16; k++) p[k] = P->p[k];
+  const double *__restrict__ p = P->p;  // parameters are loaded where the physics reads them
 
   const int32_t a0 = a.adj_ptr[K], a1 = a.adj_ptr[K + 1];
   const int deg1 = a1 - a0;
@@ -162,9 +169,9 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
   bool bad = false;
 
   for (int e = lane; e < deg1; e += G) {
-    const int32_t M = a.adj_col[a0 + e];
+    const int32_t M = adj_node(a, a0 + e, K);
     if (M == K) continue;
-    const double f = a.adj_fac[a0 + e];
+    const double f = adj_factor(a, a0 + e);
     if constexpr (JAC) {
       Dual<NP> uk[NS], ul[NS], g[NS];
 #pragma unroll
@@ -245,12 +252,20 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
 constexpr int STREAM_ADJ = 8 * ASM_BLOCK;  // adjacency entries staged per block
 constexpr int STREAM_J = 8 * ASM_BLOCK;    // Jacobian slots staged per block
 
+constexpr int STREAM_TAB = 256;  // factor-table entries kept in LDS
+
 template <int NS, bool JAC, bool SADJ>
 __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
   constexpr int NP = 2 * NS;
   __shared__ int32_t s_col[SADJ ? STREAM_ADJ : 1];
   __shared__ double s_fac[SADJ ? STREAM_ADJ : 1];
   __shared__ double s_J[JAC ? STREAM_J : 1];
+  __shared__ double s_tab[STREAM_TAB];
+  // a small factor table is read from LDS (one lookup per entry, no dependent global load)
+  const bool ltab = a.adj_fidx && a.ntab <= STREAM_TAB;
+  if (ltab)
+    for (int k = threadIdx.x; k < a.ntab; k += ASM_BLOCK) s_tab[k] = a.fac_tab[k];
+  auto factor = [&](int64_t q) { return ltab ? s_tab[a.adj_fidx[q]] : adj_factor(a, q); };
   const int64_t K0 = (int64_t)blockIdx.x * ASM_BLOCK;
   const int64_t K1 = K0 + ASM_BLOCK < a.nnodes ? K0 + ASM_BLOCK : a.nnodes;
   const int64_t K = K0 + threadIdx.x;
@@ -268,17 +283,15 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
   const bool sJ = JAC && (j1 - j0) <= STREAM_J && __syncthreads_and(active || !mine);
   if (sadj)
     for (int64_t q = e0 + threadIdx.x; q < e1; q += ASM_BLOCK) {
-      s_col[q - e0] = a.adj_col[q];
-      s_fac[q - e0] = a.adj_fac[q];
+      s_col[q - e0] = a.adj_off ? (int32_t)a.adj_off[q] : a.adj_col[q];  // offset from its node, or the node
+      s_fac[q - e0] = a.adj_fidx ? a.fac_tab[a.adj_fidx[q]] : a.adj_fac[q];
     }
   __syncthreads();
   bool bad = false;
   if (active) {
     const InstParams *P = a.inst + inst;
     const int flux_kind = P->flux;
-    double p[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) p[k] = P->p[k];
+    const double *__restrict__ p = P->p;  // parameters are loaded where the physics reads them
     const int32_t a0 = a.adj_ptr[K], a1 = a.adj_ptr[K + 1];
     const int deg1 = a1 - a0;
     const int diag = a.adj_diag[K];
@@ -318,8 +331,9 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
       for (int q = 0; q < EB; q++) {
         const int e = eb + q;
         const bool ok = e < deg1;
-        Mv[q] = ok ? (sadj ? s_col[a0 - e0 + e] : a.adj_col[a0 + e]) : (int32_t)K;
-        fv[q] = ok ? (sadj ? s_fac[a0 - e0 + e] : a.adj_fac[a0 + e]) : 0.0;
+        const int32_t sc = (sadj && ok) ? s_col[a0 - e0 + e] : 0;
+        Mv[q] = ok ? (sadj ? (a.adj_off ? (int32_t)(K + sc) : sc) : adj_node(a, a0 + e, K)) : (int32_t)K;
+        fv[q] = ok ? (sadj ? s_fac[a0 - e0 + e] : factor(a0 + e)) : 0.0;
       }
 #pragma unroll
       for (int q = 0; q < EB; q++)

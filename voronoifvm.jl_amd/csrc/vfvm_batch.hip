@@ -406,6 +406,11 @@ struct vfvm_batch {
   int asm_group = 8, spmv_group = 8;
   DevBuf<int32_t> adj_ptr, adj_col, adj_diag, bn_ptr, bn_reg, bnodes, rowptr, colind, dofdiag, node_inst;
   DevBuf<double> adj_fac, nodevol, bn_fac, src, U, Uold, F, J, dU, tmp, rowscale, bhat, rres;
+  // compact node graph for the assembly (empty when it does not fit)
+  DevBuf<int16_t> adj_off;
+  DevBuf<uint16_t> adj_fidx;
+  DevBuf<double> fac_tab;
+  bool compact_graph = true;
   DevBuf<InstParams> inst;
   bool has_src = false;
   // chunks
@@ -491,6 +496,12 @@ AsmArgs basm(vfvm_batch &b, double *F, double *J, double tstep, double lambda, c
   a.adj_col = b.adj_col.p;
   a.adj_diag = b.adj_diag.p;
   a.adj_fac = b.adj_fac.p;
+  if (b.compact_graph) {
+    a.adj_off = b.adj_off.p;
+    a.adj_fidx = b.adj_fidx.p;
+    a.fac_tab = b.fac_tab.p;
+    a.ntab = (int32_t)b.fac_tab.n;
+  }
   a.nodevol = b.nodevol.p;
   a.bn_ptr = b.bn_ptr.p;
   a.bn_reg = b.bn_reg.p;
@@ -870,6 +881,17 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
     b->adj_col.upload(adj_col, st);
     b->adj_diag.upload(adj_diag, st);
     b->adj_fac.upload(adj_fac, st);
+    {
+      std::vector<int16_t> off;
+      std::vector<uint16_t> fidx;
+      std::vector<double> tab;
+      compact_graph(adj_ptr, adj_col, adj_fac, off, fidx, tab);
+      if (!off.empty()) b->adj_off.upload(off, st);
+      if (!fidx.empty()) {
+        b->adj_fidx.upload(fidx, st);
+        b->fac_tab.upload(tab, st);
+      }
+    }
     b->nodevol.upload(nodevol, st);
     b->bn_ptr.upload(bn_ptr, st);
     b->bn_reg.upload(bn_reg, st);
@@ -986,6 +1008,12 @@ int vfvm_batch_set_sell_index(vfvm_batch *b, int bits) {
     b->sell16 = bits == 16;
     return VFVM_OK;
   });
+}
+
+int vfvm_batch_set_compact_graph(vfvm_batch *b, int enable) {
+  if (!b) return VFVM_ERR_ARG;
+  b->compact_graph = enable != 0;
+  return VFVM_OK;
 }
 
 int vfvm_batch_destroy(vfvm_batch *b) {

@@ -93,6 +93,12 @@ AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, do
   a.adj_col = s.d_adj_col.p;
   a.adj_diag = s.d_adj_diag.p;
   a.adj_fac = s.d_adj_fac.p;
+  if (s.compact_graph) {
+    a.adj_off = s.d_adj_off.p;
+    a.adj_fidx = s.d_adj_fidx.p;
+    a.fac_tab = s.d_fac_tab.p;
+    a.ntab = (int32_t)s.d_fac_tab.n;
+  }
   a.nodevol = s.d_nodevol.p;
   a.bn_ptr = s.d_bn_ptr.p;
   a.bn_reg = s.d_bn_reg.p;
@@ -528,6 +534,17 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     s->d_adj_col.upload(h.adj_col, st);
     s->d_adj_diag.upload(h.adj_diag, st);
     s->d_adj_fac.upload(h.adj_fac, st);
+    {
+      std::vector<int16_t> off;
+      std::vector<uint16_t> fidx;
+      std::vector<double> tab;
+      compact_graph(h.adj_ptr, h.adj_col, h.adj_fac, off, fidx, tab);
+      if (!off.empty()) s->d_adj_off.upload(off, st);
+      if (!fidx.empty()) {
+        s->d_adj_fidx.upload(fidx, st);
+        s->d_fac_tab.upload(tab, st);
+      }
+    }
     s->d_nodevol.upload(h.nodevol, st);
     s->d_bn_ptr.upload(h.bn_ptr, st);
     s->d_bn_reg.upload(h.bn_reg.empty() ? std::vector<int32_t>{0} : h.bn_reg, st);
@@ -947,6 +964,12 @@ int vfvm_set_sell_index(vfvm_sys *s, int bits) {
     s->lin.sell16 = bits == 16;
     return VFVM_OK;
   });
+}
+
+int vfvm_set_compact_graph(vfvm_sys *s, int enable) {
+  if (!s) return VFVM_ERR_ARG;
+  s->compact_graph = enable != 0;
+  return VFVM_OK;
 }
 
 int vfvm_autotune(vfvm_sys *s, int32_t *asm_group, int32_t *spmv_group) {

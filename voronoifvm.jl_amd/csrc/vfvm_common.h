@@ -148,6 +148,12 @@ struct AsmArgs {
   int64_t nnodes;
   const int32_t *adj_ptr, *adj_col, *adj_diag;
   const double *adj_fac, *nodevol;
+  // compact node graph (nullable): neighbour as a 16-bit offset from the node, edge
+  // factor as a 16-bit index into a table of the graph's distinct factor values
+  const int16_t *adj_off;
+  const uint16_t *adj_fidx;
+  const double *fac_tab;
+  int32_t ntab;  // entries of fac_tab
   const int32_t *bn_ptr, *bn_reg;
   const double *bn_fac;
   const int32_t *node_inst;      // per-node instance id (batched), nullptr: instance 0
@@ -161,6 +167,12 @@ struct AsmArgs {
   int nspec;
 };
 void launch_assemble(const AsmArgs &a, int group, hipStream_t s);
+// Compact form of a node graph (adj_ptr rows, adj_col, adj_fac): off[q] = adj_col[q] - row
+// and fidx[q] indexing tab (the distinct factor values, compared bitwise).  Each output
+// is left empty when it does not fit (an offset beyond 16 bits, more than 65536 values).
+void compact_graph(const std::vector<int32_t> &adj_ptr, const std::vector<int32_t> &adj_col,
+                   const std::vector<double> &adj_fac, std::vector<int16_t> &off, std::vector<uint16_t> &fidx,
+                   std::vector<double> &tab);
 // U[K,i] = Dirichlet value for boundary nodes (last item in bface order wins)
 void launch_init_dirichlet(int64_t nbnodes, const int32_t *bnodes, const int32_t *bn_ptr, const int32_t *bn_reg,
                            const int32_t *node_inst, const InstParams *inst, int nspec, double lambda, double *U,

@@ -9,6 +9,8 @@
 //   boundary assembly data        src/vfvm_system.jl:667-677, src/vfvm_assemblydata.jl:140-152
 //   dof numbering                 src/vfvm_system.jl:808-811 (node-major, species interleaved)
 #include <algorithm>
+#include <cstring>
+#include <map>
 #include <cmath>
 #include <numeric>
 
@@ -504,6 +506,43 @@ void build_sell64(int64_t nrows, const std::vector<int32_t> &rowptr, const std::
     }
   }
   if (off16 && !fits16) off16->clear();
+}
+
+void compact_graph(const std::vector<int32_t> &adj_ptr, const std::vector<int32_t> &adj_col,
+                   const std::vector<double> &adj_fac, std::vector<int16_t> &off, std::vector<uint16_t> &fidx,
+                   std::vector<double> &tab) {
+  const size_t nnz = adj_col.size();
+  off.assign(nnz, 0);
+  bool off_ok = true;
+  for (size_t K = 0; K + 1 < adj_ptr.size() && off_ok; K++)
+    for (int32_t q = adj_ptr[K]; q < adj_ptr[K + 1]; q++) {
+      const int64_t d = (int64_t)adj_col[(size_t)q] - (int64_t)K;
+      if (d < -32768 || d > 32767) {
+        off_ok = false;
+        break;
+      }
+      off[(size_t)q] = (int16_t)d;
+    }
+  if (!off_ok) off.clear();
+  // distinct factor values by bit pattern, in order of first appearance
+  std::map<uint64_t, uint16_t> ids;
+  tab.clear();
+  fidx.assign(nnz, 0);
+  for (size_t q = 0; q < nnz; q++) {
+    uint64_t bits;
+    std::memcpy(&bits, &adj_fac[q], sizeof(bits));
+    auto it = ids.find(bits);
+    if (it == ids.end()) {
+      if (tab.size() == 65536) {
+        fidx.clear();
+        tab.clear();
+        return;
+      }
+      it = ids.emplace(bits, (uint16_t)tab.size()).first;
+      tab.push_back(adj_fac[q]);
+    }
+    fidx[q] = it->second;
+  }
 }
 
 }  // namespace vfvm

@@ -32,6 +32,11 @@ struct vfvm_sys {
   // SELL-64 copy of the Jacobian pattern for the Krylov operator
   DevBuf<int32_t> d_sell_ptr, d_sell_col, d_sell_rbase;
   DevBuf<int16_t> d_sell_off16;  // empty when some row reaches further than 32767 rows
+  // compact node graph for the assembly (empty when it does not fit) and whether to use it
+  DevBuf<int16_t> d_adj_off;
+  DevBuf<uint16_t> d_adj_fidx;
+  DevBuf<double> d_fac_tab;
+  bool compact_graph = true;
   int64_t sell_nnz = 0;
   LinearSolver lin;
   int asm_group = 8, spmv_group = 8;

@@ -89,6 +89,7 @@ SIGNATURES = {
     "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_set_groups": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "vfvm_set_sell_index": (C.c_int, [_vp, C.c_int]),
+    "vfvm_set_compact_graph": (C.c_int, [_vp, C.c_int]),
     "vfvm_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "vfvm_probe_arm": (C.c_int, [_vp, C.c_int, C.c_int64]),
     "vfvm_probe_read": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
@@ -108,6 +109,7 @@ SIGNATURES = {
     "vfvm_batch_destroy": (C.c_int, [_vp]),
     "vfvm_batch_set_graphs": (C.c_int, [_vp, C.c_int]),
     "vfvm_batch_set_sell_index": (C.c_int, [_vp, C.c_int]),
+    "vfvm_batch_set_compact_graph": (C.c_int, [_vp, C.c_int]),
     "vfvm_batch_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "vfvm_mpnn_time_layer": (C.c_int, [_vp, C.c_int64, C.c_int, _dp]),
     "vfvm_batch_update_member": (C.c_int, [_vp, C.c_int64, _dp, C.POINTER(BCT)]),

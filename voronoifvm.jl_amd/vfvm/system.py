@@ -185,6 +185,11 @@ class System:
         _capi.check(_capi.lib().vfvm_autotune(self.handle(), C.byref(a), C.byref(s)), self.ctx.handle)
         return a.value, s.value
 
+    def set_compact_graph(self, enable: bool):
+        """Assembly reads the compact node graph (16-bit neighbour offsets, factor
+        table; default where it fits) or int32 neighbours and FP64 factors."""
+        _capi.check(_capi.lib().vfvm_set_compact_graph(self.handle(), int(bool(enable))), self.ctx.handle)
+
     def set_sell_index(self, bits: int):
         """SpMV column indices: 16-bit offsets from the row (default where they fit) or
         32-bit absolute columns."""
@@ -379,6 +384,10 @@ class Batch:
         """Run the batched Krylov iteration blocks as captured HIP graphs (default) or
         launch their kernels one by one."""
         _capi.check(_capi.lib().vfvm_batch_set_graphs(self._h, int(enable)), self.ctx.handle)
+
+    def set_compact_graph(self, enable: bool):
+        """The batched assembly's node graph: compact (default where it fits) or plain."""
+        _capi.check(_capi.lib().vfvm_batch_set_compact_graph(self._h, int(bool(enable))), self.ctx.handle)
 
     def set_sell_index(self, bits: int):
         """The batched SpMV's column indices: 16-bit offsets or 32-bit columns."""
