@@ -160,6 +160,15 @@ int vfvm_system_set_bc(vfvm_sys *sys, const vfvm_bc_t *bc);
 int vfvm_system_sizes(vfvm_sys *sys, int64_t *ndof, int64_t *nnz, int64_t *nedges);
 /* node volumes (sum of node factors), for integrate() and tests (src/vfvm_postprocess.jl:397-412) */
 int vfvm_nodevolumes(vfvm_sys *sys, double *vol);
+/* update_grid!(system; grid) for a grid with new coordinates and the same cells and
+ * boundary faces (src/vfvm_system.jl:607-612, update_grid_edgewise! 690-755): coord is
+ * dim x nnodes (as at creation).  The form factors, node volumes and boundary face
+ * factors are recomputed on the device, bit for bit those of a system created with
+ * these coordinates.  VFVM_ERR_ARG (geometry left unchanged) if an edge whose factor was
+ * exactly zero at creation, so has no Jacobian entry, gets a nonzero factor.  The
+ * compact factor table is rebuilt on the host only by vfvm_set_compact_graph(sys, 1);
+ * until then the assembly reads the FP64 factors. */
+int vfvm_update_grid(vfvm_sys *sys, const double *coord);
 
 /* Jacobian pattern as CSC, 1-based, rows sorted: what SparseMatrixCSC(state.matrix)
  * exposes (src/vfvm_linsolve.jl:3-4, src/vfvm_state.jl:136).  colptr has ndof+1 entries. */
@@ -243,7 +252,8 @@ int vfvm_set_groups(vfvm_sys *sys, int32_t asm_group, int32_t spmv_group);
 int vfvm_set_sell_index(vfvm_sys *sys, int bits);
 /* Node graph the assembly reads: 1 (default) = compact where it fits (neighbours as
  * 16-bit offsets from the node, edge factors as 16-bit indices into a table of the
- * graph's distinct values, bit-exact), 0 = int32 neighbours and FP64 factors per entry. */
+ * graph's distinct values, bit-exact), 0 = int32 neighbours and FP64 factors per entry.
+ * After vfvm_update_grid, enabling it rebuilds the factor table (a host pass). */
 int vfvm_set_compact_graph(vfvm_sys *sys, int enable);
 
 /* Benchmark helper (no counterpart in the reference; it has no device): average device

@@ -51,6 +51,26 @@ def nonlinear_poisson_2d(n=101, eps=1e-2, f=1.0, values=(0.0, 1.0, 1.0, 0.0), or
                 oracle=oracle)
 
 
+def sg_grid(name):
+    """One of the reference's unstructured grid assets, kept as fixtures in tests/golden."""
+    import os
+    from vfvm.grids import read_sg
+    return read_sg(os.path.join(os.path.dirname(__file__), "golden", name))
+
+
+def nonlinear_poisson_sg(name, eps=0.1, f=1.0):
+    """Nonlinear Poisson (as nonlinear_poisson_2d) on an unstructured .sg grid: u = 0 on
+    boundary face region 1, u = 1 on region 2, homogeneous Neumann elsewhere."""
+    return nonlinear_poisson_on(sg_grid(name), eps, f)
+
+
+def nonlinear_poisson_on(g, eps=0.1, f=1.0):
+    """Nonlinear Poisson on any grid: u = 0 on boundary face region 1, u = 1 on region 2."""
+    bcs = [vfvm.Dirichlet(species=1, region=1, value=0.0), vfvm.Dirichlet(species=1, region=2, value=1.0)]
+    src = np.full((1, g.num_nodes), f, order="F")
+    return pair(g, 1, dict(flux=vfvm.LinearFlux(eps), reaction=vfvm.SinhReaction()), bcs, source=src)
+
+
 def nonlinear_diffusion_3d(n=17):
     """3D nonlinear diffusion D(u) = 1 + u^2, storage u, u = 0 on all faces."""
     X = linspace(0.0, 1.0, n)

@@ -1,0 +1,109 @@
+"""Device geometry updates (update_grid!, src/vfvm_system.jl:607-612, 690-755):
+new coordinates on the same cells give, bit for bit, the factors, node volumes
+and assembly of a system created with those coordinates, and the oracle's
+assembly within 1e-12; a change that would alter the sparsity pattern is refused
+and leaves the system as it was."""
+import numpy as np
+import pytest
+
+import cases
+import vfvm
+from vfvm.grids import Grid, linspace, simplexgrid
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _moved(g, scale, seed, boundary=False):
+    """Interior nodes (on no boundary face; all nodes with boundary=True) moved by up to
+    `scale` times the shortest cell edge in each coordinate."""
+    rng = np.random.default_rng(seed)
+    P = g.coord[:, g.cellnodes]
+    h = min(np.sqrt(((P[:, a] - P[:, b]) ** 2).sum(axis=0)).min()
+            for a in range(g.dim + 1) for b in range(a + 1, g.dim + 1))
+    inner = np.ones(g.num_nodes, bool)
+    if not boundary:
+        inner[g.bfacenodes.ravel()] = False
+    coord = g.coord.copy()
+    coord[:, inner] += scale * h * rng.uniform(-1, 1, (g.dim, int(inner.sum())))
+    return Grid(g.dim, coord, g.cellnodes, g.cellregions, g.bfacenodes, g.bfaceregions, g.shape)
+
+
+def _asm(sys, U):
+    F, A = vfvm.evaluate_residual_and_jacobian(sys, np.asfortranarray(U))
+    A = A.tocsc()
+    return vfvm.dofs(F).copy(), A.data.copy(), A.indices.copy(), A.indptr.copy()
+
+
+def _same(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def _grids():
+    X1 = np.sort(np.r_[0.0, 1.0, np.random.default_rng(2).uniform(0, 1, 30)])
+    g1 = simplexgrid(X1)
+    X = linspace(0.0, 1.0, 9)
+    g3 = _moved(simplexgrid(X, X, X), 0.1, 7, boundary=True)  # generic: every edge factor nonzero
+    return [("1d", g1, 0.2, False), ("cyl_unstruct", cases.sg_grid("cyl_unstruct.sg"), 0.15, False),
+            ("kuhn9", g3, 0.1, True)]
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_update_grid_equals_fresh_system(case):
+    name, g0, scale, bnd = _grids()[case]
+    _print("case update_grid", name)
+    sys, _ = cases.nonlinear_poisson_on(g0)
+    sys.set_compact_graph(False)
+    rng = np.random.default_rng(3)
+    U = rng.uniform(-1, 1, (1, g0.num_nodes))
+    before = _asm(sys, U)
+    vol0 = sys.nodevolumes()
+    # the same coordinates: nothing changes
+    sys.update_grid(g0)
+    assert _same(_asm(sys, U), before)
+    assert np.array_equal(sys.nodevolumes(), vol0)
+    g1 = _moved(g0, scale, 11, bnd)
+    sys.update_grid(g1)
+    fresh, ofresh = cases.nonlinear_poisson_on(g1)
+    fresh.set_compact_graph(False)
+    got, want = _asm(sys, U), _asm(fresh, U)
+    assert _same(got, want), "factors differ from a system created with the new coordinates"
+    assert np.array_equal(sys.nodevolumes(), fresh.nodevolumes())
+    assert not np.array_equal(got[0], before[0])
+    cases.compare_assembly(sys, ofresh, U)
+    # the compact factor table is rebuilt from the device factors on request
+    sys.set_compact_graph(True)
+    assert _same(_asm(sys, U), want)
+    # and a Newton solve on the moved grid agrees with the oracle's
+    Uo, _ = ofresh.solve_step(np.zeros(g1.num_nodes))
+    c = vfvm.SolverControl(reltol_linear=1e-12, abstol_linear=1e-14, maxiters_linear=2000, precon="ilu0")
+    Ug = vfvm.solve(sys, inival=0.0, control=c)
+    err = np.abs(vfvm.dofs(Ug) - Uo).max()
+    _print("newton on moved grid err", err, "res", sys.history.residual_norm)
+    assert err <= 1e-9 and sys.history.residual_norm <= 1e-10
+
+
+def test_update_grid_refuses_pattern_change():
+    """A uniform 2D tensor grid has exactly-zero factors on its diagonals (no Jacobian
+    entry); moving interior nodes makes them nonzero: refused, system unchanged.
+    Scaling the grid by 2 keeps them exactly zero: accepted, equal to a fresh system."""
+    _print("case update_grid pattern change")
+    X = linspace(0.0, 1.0, 9)
+    g0 = simplexgrid(X, X)
+    sys, _ = cases.nonlinear_poisson_on(g0)
+    sys.set_compact_graph(False)
+    U = np.random.default_rng(4).uniform(-1, 1, (1, g0.num_nodes))
+    before = _asm(sys, U)
+    with pytest.raises(vfvm.VFVMError):
+        sys.update_grid(_moved(g0, 0.1, 5))
+    assert _same(_asm(sys, U), before)
+    assert sys.grid is g0
+    g2 = Grid(2, 2.0 * g0.coord, g0.cellnodes, g0.cellregions, g0.bfacenodes, g0.bfaceregions, g0.shape)
+    sys.update_grid(g2)
+    fresh, ofresh = cases.nonlinear_poisson_on(g2)
+    fresh.set_compact_graph(False)
+    assert _same(_asm(sys, U), _asm(fresh, U))
+    cases.compare_assembly(sys, ofresh, U)

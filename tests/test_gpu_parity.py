@@ -367,3 +367,21 @@ def test_compact_graph_is_bitwise_identical():
             assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), group
         sys.set_compact_graph(True)
         cases.compare_assembly(sys, osys, U, embed=0.3)
+
+
+@pytest.mark.parametrize("name", ["rz2d.sg", "cyl_unstruct.sg"])
+def test_unstructured_sg_grid_matches_oracle(name):
+    """Unstructured triangulations read from the reference's .sg assets: assembly at a
+    random state and the device Newton solution against the oracle."""
+    _print("case unstructured", name)
+    sys, osys = cases.nonlinear_poisson_sg(name)
+    rng = np.random.default_rng(5)
+    U = rng.uniform(-1, 1, (1, sys.grid.num_nodes))
+    cases.compare_assembly(sys, osys, U)
+    Uo, ho = osys.solve_step(np.zeros(sys.grid.num_nodes))
+    c = vfvm.SolverControl(reltol_linear=1e-12, abstol_linear=1e-14, maxiters_linear=2000, precon="ilu0")
+    Ug = vfvm.solve(sys, inival=0.0, control=c)
+    err = np.abs(vfvm.dofs(Ug) - Uo).max()
+    _print("newton iters gpu", sys.history.niter, "oracle", ho.niter, "err", err, "res", sys.history.residual_norm)
+    assert err <= SOL_ATOL
+    assert sys.history.residual_norm <= 1e-10

@@ -810,6 +810,7 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
     b->node_off.assign((size_t)nsys + 1, 0);
     b->dof_off.assign((size_t)nsys + 1, 0);
     int64_t nadj = 0, nbi = 0;
+    for (int64_t m = 0; m < nsys; m++) sync_host_geometry(*systems[m]);
     for (int64_t m = 0; m < nsys; m++) {
       const vfvm_sys *s = systems[m];
       b->node_off[m + 1] = b->node_off[m] + s->nnodes;

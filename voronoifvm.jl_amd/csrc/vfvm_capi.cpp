@@ -95,9 +95,11 @@ AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, do
   a.adj_fac = s.d_adj_fac.p;
   if (s.compact_graph) {
     a.adj_off = s.d_adj_off.p;
-    a.adj_fidx = s.d_adj_fidx.p;
-    a.fac_tab = s.d_fac_tab.p;
-    a.ntab = (int32_t)s.d_fac_tab.n;
+    if (!s.fac_tab_stale) {
+      a.adj_fidx = s.d_adj_fidx.p;
+      a.fac_tab = s.d_fac_tab.p;
+      a.ntab = (int32_t)s.d_fac_tab.n;
+    }
   }
   a.nodevol = s.d_nodevol.p;
   a.bn_ptr = s.d_bn_ptr.p;
@@ -513,6 +515,13 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     s->bc.assign((size_t)std::max(1, nbregions) * nspec, vfvm_bc_t{});
     if (bc && nbregions > 0) std::copy(bc, bc + (size_t)nbregions * nspec, s->bc.begin());
     build_setup(dim, nnodes, coord, ncells, cellnodes, nbfaces, bfacenodes, bfaceregions, s->hs);
+    // ranges checked by build_setup
+    s->ncells = ncells;
+    s->nbfaces = nbfaces;
+    s->cellnodes0.resize((size_t)(dim + 1) * ncells);
+    for (size_t k = 0; k < s->cellnodes0.size(); k++) s->cellnodes0[k] = (int32_t)(cellnodes[k] - 1);
+    s->bfacenodes0.resize((size_t)dim * nbfaces);
+    for (size_t k = 0; k < s->bfacenodes0.size(); k++) s->bfacenodes0[k] = (int32_t)(bfacenodes[k] - 1);
     for (int64_t b = 0; b < (int64_t)s->hs.bn_reg.size(); b++)
       if (s->hs.bn_reg[b] > MAXBREG) throw Error(VFVM_ERR_ARG, "boundary region number above 8");
     s->masks = make_masks(nspec, s->phys, s->bc.data(), nbregions);
@@ -636,8 +645,19 @@ int vfvm_system_sizes(vfvm_sys *s, int64_t *ndof, int64_t *nnz, int64_t *nedges)
 
 int vfvm_nodevolumes(vfvm_sys *s, double *vol) {
   if (!s || !vol) return VFVM_ERR_ARG;
-  std::copy(s->hs.nodevol.begin(), s->hs.nodevol.end(), vol);
-  return VFVM_OK;
+  return guarded(s->ctx, [&]() {
+    sync_host_geometry(*s);
+    std::copy(s->hs.nodevol.begin(), s->hs.nodevol.end(), vol);
+    return VFVM_OK;
+  });
+}
+
+int vfvm_update_grid(vfvm_sys *s, const double *coord) {
+  if (!s || !coord) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    update_geometry(*s, coord);
+    return VFVM_OK;
+  });
 }
 
 int vfvm_pattern(vfvm_sys *s, int64_t *colptr, int64_t *rowval) {
@@ -968,8 +988,25 @@ int vfvm_set_sell_index(vfvm_sys *s, int bits) {
 
 int vfvm_set_compact_graph(vfvm_sys *s, int enable) {
   if (!s) return VFVM_ERR_ARG;
-  s->compact_graph = enable != 0;
-  return VFVM_OK;
+  return guarded(s->ctx, [&]() {
+    if (enable && s->fac_tab_stale) {
+      // factors changed on the device since the table was built: rebuild it
+      sync_host_geometry(*s);
+      std::vector<int16_t> off;
+      std::vector<uint16_t> fidx;
+      std::vector<double> tab;
+      compact_graph(s->hs.adj_ptr, s->hs.adj_col, s->hs.adj_fac, off, fidx, tab);
+      s->d_adj_fidx.release();
+      s->d_fac_tab.release();
+      if (!fidx.empty()) {
+        s->d_adj_fidx.upload(fidx, s->ctx->stream);
+        s->d_fac_tab.upload(tab, s->ctx->stream);
+      }
+      s->fac_tab_stale = false;
+    }
+    s->compact_graph = enable != 0;
+    return VFVM_OK;
+  });
 }
 
 int vfvm_autotune(vfvm_sys *s, int32_t *asm_group, int32_t *spmv_group) {

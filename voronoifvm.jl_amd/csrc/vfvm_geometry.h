@@ -1,0 +1,157 @@
+// Voronoi form factors of one simplex cell / boundary face, shared by the host
+// setup and the device geometry kernels so both evaluate the same expression
+// trees (contraction into FMA is switched off: the two builds agree bit for bit).
+//
+// Restates (paths relative to the reference root):
+//   cellfactors!(Edge1D, Cartesian1D)        src/vfvm_formfactors.jl:12-23
+//   cellfactors!(Triangle2D, Cartesian2D)     src/vfvm_formfactors.jl:64-100
+//   cellfactors!(Tetrahedron3D, Cartesian3D)  src/vfvm_formfactors.jl:163-235
+//   bfacefactors!(Vertex0D / Edge1D / Triangle2D, Cartesian)
+//                                            src/vfvm_formfactors.jl:244-247, 263-274, 293-332
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace vfvm {
+namespace geom {
+
+// ExtendableGrids local element tables: triangle edge i is opposite node i;
+// tetrahedron edges in the order 12, 13, 14, 23, 24, 34.
+__host__ __device__ constexpr int edges_per_cell(int dim) { return dim == 1 ? 1 : (dim == 2 ? 3 : 6); }
+
+__host__ __device__ inline void local_edge(int dim, int e, int &a, int &b) {
+  if (dim == 1) {
+    a = 0;
+    b = 1;
+  } else if (dim == 2) {
+    a = e == 0 ? 1 : (e == 1 ? 2 : 0);
+    b = e == 0 ? 2 : (e == 1 ? 0 : 1);
+  } else {
+    constexpr int ta[6] = {0, 0, 0, 1, 1, 2}, tb[6] = {1, 2, 3, 2, 3, 3};
+    a = ta[e];
+    b = tb[e];
+  }
+}
+
+// coord is dim x nnodes column-major; n holds the 0-based node numbers of the cell.
+// npar: dim+1 node volume parts; epar: sigma/h parts of the cell's edges.
+template <class I>
+__host__ __device__ inline void cell_factors(int dim, const double *coord, const I *n, double *npar, double *epar) {
+#pragma clang fp contract(off)
+  if (dim == 1) {
+    double d = fabs(coord[n[1]] - coord[n[0]]);
+    npar[0] = d / 2;
+    npar[1] = d / 2;
+    epar[0] = 1 / d;
+    return;
+  }
+  if (dim == 2) {
+    double V0[3], V1[3], dd[3];
+    for (int e = 0; e < 3; e++) {
+      int a, b;
+      local_edge(2, e, a, b);
+      V0[e] = coord[2 * n[a]] - coord[2 * n[b]];
+      V1[e] = coord[2 * n[a] + 1] - coord[2 * n[b] + 1];
+    }
+    double det = V0[2] * V1[1] - V0[1] * V1[2];
+    double vol = fabs(0.5 * det);
+    double ivol = 1.0 / vol;
+    for (int e = 0; e < 3; e++) dd[e] = V0[e] * V0[e] + V1[e] * V1[e];
+    epar[0] = (dd[1] + dd[2] - dd[0]) * 0.125 * ivol;
+    epar[1] = (dd[2] + dd[0] - dd[1]) * 0.125 * ivol;
+    epar[2] = (dd[0] + dd[1] - dd[2]) * 0.125 * ivol;
+    npar[0] = npar[1] = npar[2] = 0.0;
+    for (int e = 0; e < 3; e++) {
+      int a, b;
+      local_edge(2, e, a, b);
+      npar[a] += epar[e] * dd[e] * 0.25;
+      npar[b] += epar[e] * dd[e] * 0.25;
+    }
+    return;
+  }
+  // pdelib tetrahedron formula; face tables of src/vfvm_formfactors.jl:170-176 (1-based edges)
+  constexpr int pi1[4] = {5, 6, 5, 1}, pi2[4] = {6, 3, 1, 4}, pi3[4] = {4, 2, 3, 2};
+  constexpr int po1[4] = {2, 1, 2, 6}, po2[4] = {1, 4, 6, 3}, po3[4] = {3, 5, 4, 5};
+  double dd[6];
+  for (int e = 0; e < 6; e++) {
+    int a, b;
+    local_edge(3, e, a, b);
+    const double *p1 = coord + 3 * n[a], *p2 = coord + 3 * n[b];
+    double dx = p1[0] - p2[0], dy = p1[1] - p2[1], dz = p1[2] - p2[2];
+    dd[e] = dx * dx + dy * dy + dz * dz;
+  }
+  for (int e = 0; e < 6; e++) epar[e] = 0.0;
+  const double *c0 = coord + 3 * n[0], *c1 = coord + 3 * n[1], *c2 = coord + 3 * n[2], *c3 = coord + 3 * n[3];
+  double x1 = c1[0] - c0[0], y1 = c1[1] - c0[1], z1 = c1[2] - c0[2];
+  double x2 = c2[0] - c0[0], y2 = c2[1] - c0[1], z2 = c2[2] - c0[2];
+  double x3 = c3[0] - c0[0], y3 = c3[1] - c0[1], z3 = c3[2] - c0[2];
+  double det = (x1 * (y2 * z3 - y3 * z2) + x2 * (y3 * z1 - y1 * z3) + x3 * (y1 * z2 - y2 * z1));
+  if (det < 0) det = -det;
+  double vol = det / 6;
+  double vv = 96 * 6 * vol;
+  for (int i = 0; i < 4; i++) {
+    npar[i] = 0.0;
+    int i1 = pi1[i] - 1, i2 = pi2[i] - 1, i3 = pi3[i] - 1;
+    double h1 = dd[i1] * (dd[i2] + dd[i3] - dd[i1]);
+    double h2 = dd[i2] * (dd[i3] + dd[i1] - dd[i2]);
+    double h3 = dd[i3] * (dd[i1] + dd[i2] - dd[i3]);
+    double df = h1 + h2 + h3;
+    double vf = (h1 * dd[po1[i] - 1] + h2 * dd[po2[i] - 1] + h3 * dd[po3[i] - 1] - 2 * dd[i1] * dd[i2] * dd[i3]) /
+                (vv * df);
+    epar[i1] += h1 * vf;
+    epar[i2] += h2 * vf;
+    epar[i3] += h3 * vf;
+  }
+  for (int e = 0; e < 6; e++) {
+    int a, b;
+    local_edge(3, e, a, b);
+    npar[a] += epar[e];
+    npar[b] += epar[e];
+    epar[e] = 6 * epar[e] / dd[e];
+  }
+}
+
+// Boundary face measure parts per face node (n: 0-based face nodes).
+template <class I>
+__host__ __device__ inline void bface_factors(int dim, const double *coord, const I *n, double *npar) {
+#pragma clang fp contract(off)
+  if (dim == 1) {
+    npar[0] = 1.0;
+    return;
+  }
+  if (dim == 2) {
+    double dx = coord[2 * n[0]] - coord[2 * n[1]];
+    double dy = coord[2 * n[0] + 1] - coord[2 * n[1] + 1];
+    double d = sqrt(dx * dx + dy * dy);
+    npar[0] = d / 2;
+    npar[1] = d / 2;
+    return;
+  }
+  double epar[3] = {0, 0, 0};
+  for (int j = 0; j < 3; j++)
+    for (int e = 0; e < 3; e++) {
+      int a, b;
+      local_edge(2, e, a, b);
+      double d = coord[3 * n[a] + j] - coord[3 * n[b] + j];
+      epar[e] += d * d;
+    }
+  double dd[3] = {epar[0], epar[1], epar[2]};
+  epar[0] = (dd[1] + dd[2] - dd[0]) * dd[0];
+  epar[1] = (dd[2] + dd[0] - dd[1]) * dd[1];
+  epar[2] = (dd[0] + dd[1] - dd[2]) * dd[2];
+  double vol = sqrt(epar[0] + epar[1] + epar[2]) * 0.25;
+  double d = 1.0 / (8 * vol);
+  npar[0] = npar[1] = npar[2] = 0.0;
+  for (int e = 0; e < 3; e++) {
+    int a, b;
+    local_edge(2, e, a, b);
+    npar[a] += epar[e] * d * 0.25;
+    npar[b] += epar[e] * d * 0.25;
+  }
+}
+
+}  // namespace geom
+}  // namespace vfvm

@@ -867,6 +867,7 @@ int vfvm_mpnn_create(vfvm_sys *sys, int nlayers, int width, uint64_t seed, vfvm_
     m->nlayers = nlayers;
     m->n = sys->nnodes;
     // normalised graph weights from the system's node graph (sigma/h factors)
+    sync_host_geometry(*sys);
     const HostSetup &h = sys->hs;
     std::vector<int32_t> ptr((size_t)m->n + 1, 0), col;
     std::vector<float> w, wt;

@@ -3,7 +3,7 @@
 // greedy colouring for the multicolour ILU(0).
 //
 // Restates (paths relative to the reference root):
-//   cellfactors!/bfacefactors!   src/vfvm_formfactors.jl:12-23, 64-100, 163-235, 244-247, 263-274, 293-332
+//   cellfactors!/bfacefactors!   vfvm_geometry.h (shared with the device geometry kernels)
 //   update_grid_edgewise!         src/vfvm_system.jl:690-755 (factor sums per edge / node in cell order;
 //                                 ExtendableSparse `+=` creates an entry only for a nonzero value)
 //   boundary assembly data        src/vfvm_system.jl:667-677, src/vfvm_assemblydata.jl:140-152
@@ -15,131 +15,14 @@
 #include <numeric>
 
 #include "vfvm_common.h"
+#include "vfvm_geometry.h"
 
 namespace vfvm {
 
-namespace {
-
-// ExtendableGrids local element tables: triangle edge i is opposite node i;
-// tetrahedron edges in the order 12, 13, 14, 23, 24, 34.
-const int TRI_EN[3][2] = {{1, 2}, {2, 0}, {0, 1}};
-const int TET_EN[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
-
-inline int edges_per_cell(int dim) { return dim == 1 ? 1 : (dim == 2 ? 3 : 6); }
-
-inline void local_edge(int dim, int e, int &a, int &b) {
-  if (dim == 1) {
-    a = 0;
-    b = 1;
-  } else if (dim == 2) {
-    a = TRI_EN[e][0];
-    b = TRI_EN[e][1];
-  } else {
-    a = TET_EN[e][0];
-    b = TET_EN[e][1];
-  }
-}
-
-// coord is dim x nnodes column-major; n holds 0-based node numbers of the cell
-void cell_factors(int dim, const double *coord, const int64_t *n, double *npar, double *epar) {
-  if (dim == 1) {
-    double d = std::fabs(coord[n[1]] - coord[n[0]]);
-    npar[0] = d / 2;
-    npar[1] = d / 2;
-    epar[0] = 1 / d;
-    return;
-  }
-  if (dim == 2) {
-    double V0[3], V1[3], dd[3];
-    for (int e = 0; e < 3; e++) {
-      V0[e] = coord[2 * n[TRI_EN[e][0]]] - coord[2 * n[TRI_EN[e][1]]];
-      V1[e] = coord[2 * n[TRI_EN[e][0]] + 1] - coord[2 * n[TRI_EN[e][1]] + 1];
-    }
-    double det = V0[2] * V1[1] - V0[1] * V1[2];
-    double vol = std::fabs(0.5 * det);
-    double ivol = 1.0 / vol;
-    for (int e = 0; e < 3; e++) dd[e] = V0[e] * V0[e] + V1[e] * V1[e];
-    epar[0] = (dd[1] + dd[2] - dd[0]) * 0.125 * ivol;
-    epar[1] = (dd[2] + dd[0] - dd[1]) * 0.125 * ivol;
-    epar[2] = (dd[0] + dd[1] - dd[2]) * 0.125 * ivol;
-    npar[0] = npar[1] = npar[2] = 0.0;
-    for (int i = 0; i < 3; i++) {
-      npar[TRI_EN[i][0]] += epar[i] * dd[i] * 0.25;
-      npar[TRI_EN[i][1]] += epar[i] * dd[i] * 0.25;
-    }
-    return;
-  }
-  // pdelib tetrahedron formula; face tables of src/vfvm_formfactors.jl:170-176 (1-based edges)
-  static const int pi1[4] = {5, 6, 5, 1}, pi2[4] = {6, 3, 1, 4}, pi3[4] = {4, 2, 3, 2};
-  static const int po1[4] = {2, 1, 2, 6}, po2[4] = {1, 4, 6, 3}, po3[4] = {3, 5, 4, 5};
-  double dd[6];
-  for (int i = 0; i < 6; i++) {
-    const double *p1 = coord + 3 * n[TET_EN[i][0]], *p2 = coord + 3 * n[TET_EN[i][1]];
-    double dx = p1[0] - p2[0], dy = p1[1] - p2[1], dz = p1[2] - p2[2];
-    dd[i] = dx * dx + dy * dy + dz * dz;
-  }
-  for (int i = 0; i < 6; i++) epar[i] = 0.0;
-  const double *c0 = coord + 3 * n[0], *c1 = coord + 3 * n[1], *c2 = coord + 3 * n[2], *c3 = coord + 3 * n[3];
-  double x1 = c1[0] - c0[0], y1 = c1[1] - c0[1], z1 = c1[2] - c0[2];
-  double x2 = c2[0] - c0[0], y2 = c2[1] - c0[1], z2 = c2[2] - c0[2];
-  double x3 = c3[0] - c0[0], y3 = c3[1] - c0[1], z3 = c3[2] - c0[2];
-  double det = (x1 * (y2 * z3 - y3 * z2) + x2 * (y3 * z1 - y1 * z3) + x3 * (y1 * z2 - y2 * z1));
-  if (det < 0) det = -det;
-  double vol = det / 6;
-  double vv = 96 * 6 * vol;
-  for (int i = 0; i < 4; i++) {
-    npar[i] = 0.0;
-    int i1 = pi1[i] - 1, i2 = pi2[i] - 1, i3 = pi3[i] - 1;
-    double h1 = dd[i1] * (dd[i2] + dd[i3] - dd[i1]);
-    double h2 = dd[i2] * (dd[i3] + dd[i1] - dd[i2]);
-    double h3 = dd[i3] * (dd[i1] + dd[i2] - dd[i3]);
-    double df = h1 + h2 + h3;
-    double vf = (h1 * dd[po1[i] - 1] + h2 * dd[po2[i] - 1] + h3 * dd[po3[i] - 1] - 2 * dd[i1] * dd[i2] * dd[i3]) /
-                (vv * df);
-    epar[i1] += h1 * vf;
-    epar[i2] += h2 * vf;
-    epar[i3] += h3 * vf;
-  }
-  for (int i = 0; i < 6; i++) {
-    npar[TET_EN[i][0]] += epar[i];
-    npar[TET_EN[i][1]] += epar[i];
-    epar[i] = 6 * epar[i] / dd[i];
-  }
-}
-
-void bface_factors(int dim, const double *coord, const int64_t *n, double *npar) {
-  if (dim == 1) {
-    npar[0] = 1.0;
-    return;
-  }
-  if (dim == 2) {
-    double dx = coord[2 * n[0]] - coord[2 * n[1]];
-    double dy = coord[2 * n[0] + 1] - coord[2 * n[1] + 1];
-    double d = std::sqrt(dx * dx + dy * dy);
-    npar[0] = d / 2;
-    npar[1] = d / 2;
-    return;
-  }
-  double epar[3] = {0, 0, 0};
-  for (int j = 0; j < 3; j++)
-    for (int e = 0; e < 3; e++) {
-      double d = coord[3 * n[TRI_EN[e][0]] + j] - coord[3 * n[TRI_EN[e][1]] + j];
-      epar[e] += d * d;
-    }
-  double dd[3] = {epar[0], epar[1], epar[2]};
-  epar[0] = (dd[1] + dd[2] - dd[0]) * dd[0];
-  epar[1] = (dd[2] + dd[0] - dd[1]) * dd[1];
-  epar[2] = (dd[0] + dd[1] - dd[2]) * dd[2];
-  double vol = std::sqrt(epar[0] + epar[1] + epar[2]) * 0.25;
-  double d = 1.0 / (8 * vol);
-  npar[0] = npar[1] = npar[2] = 0.0;
-  for (int i = 0; i < 3; i++) {
-    npar[TRI_EN[i][0]] += epar[i] * d * 0.25;
-    npar[TRI_EN[i][1]] += epar[i] * d * 0.25;
-  }
-}
-
-}  // namespace
+using geom::bface_factors;
+using geom::cell_factors;
+using geom::edges_per_cell;
+using geom::local_edge;
 
 void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, const int64_t *cellnodes1,
                  int64_t nbfaces, const int64_t *bfacenodes1, const int64_t *bfaceregions, HostSetup &out) {

@@ -14,7 +14,17 @@ struct vfvm_sys {
   std::vector<vfvm_bc_t> bc;
   Masks masks{};
   HostSetup hs;
-  std::vector<double> coord;  // dim x nnodes, as given at creation
+  std::vector<double> coord;  // dim x nnodes: the coordinates the factors belong to
+  // topology kept for device geometry updates (0-based; (dim+1) x ncells, dim x nbfaces)
+  int64_t ncells = 0, nbfaces = 0;
+  std::vector<int32_t> cellnodes0, bfacenodes0;
+  struct Geometry {
+    DevBuf<int32_t> cn, inc_ptr, inc_k, bfn, bslot, flag;
+    DevBuf<double> coord, cf;
+    bool built = false;
+  } geo;
+  bool hs_stale = false;       // hs.adj_fac / nodevol / bn_fac older than the device copies
+  bool fac_tab_stale = false;  // compact factor table older than d_adj_fac
   std::vector<int32_t> rowptr, colind, csc_perm, csc_rowval, dofdiag;
   std::vector<int64_t> csc_colptr;
   // device data
@@ -45,4 +55,8 @@ struct vfvm_sys {
 
 namespace vfvm {
 InstParams make_inst(const vfvm_sys &s);
+// new coordinates, same topology: factors recomputed on the device (vfvm_geometry.hip)
+void update_geometry(vfvm_sys &s, const double *coord);
+// brings hs.adj_fac / nodevol / bn_fac up to date after update_geometry
+void sync_host_geometry(vfvm_sys &s);
 }

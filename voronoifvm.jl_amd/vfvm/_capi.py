@@ -73,6 +73,7 @@ SIGNATURES = {
     "vfvm_system_set_bc": (C.c_int, [_vp, C.POINTER(BCT)]),
     "vfvm_system_sizes": (C.c_int, [_vp, _lp, _lp, _lp]),
     "vfvm_nodevolumes": (C.c_int, [_vp, _dp]),
+    "vfvm_update_grid": (C.c_int, [_vp, _dp]),
     "vfvm_pattern": (C.c_int, [_vp, _lp, _lp]),
     "vfvm_initialize_dirichlet": (C.c_int, [_vp, _dp, C.c_double, C.c_double]),
     "vfvm_assemble": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),

@@ -172,3 +172,55 @@ def simplexgrid(*axes) -> Grid:
     if len(axes) == 3:
         return _grid3d(*axes)
     raise ValueError("simplexgrid takes 1, 2 or 3 coordinate vectors")
+
+
+def read_sg(path: str) -> Grid:
+    """``simplexgrid(file)`` for pdelib's ``.sg`` text format ("SimplexGrid 2.1", as the
+    reference's assets/rz2d.sg and assets/cyl_unstruct.sg, read in
+    examples/Example203_CoordinateSystems.jl:105 and Example205_StagnationPoint.jl:107).
+    The reader belongs to ExtendableGrids.jl, which is not part of the reference tree;
+    the format, restated from those files: ``#`` comment lines; ``DIMENSION`` d;
+    ``NODES`` n d, then n*d coordinates (node-major); ``CELLS`` c, then per cell d+1
+    1-based node numbers and a region; ``FACES`` f, then per boundary face d node
+    numbers and a region; ``END``."""
+    with open(path) as fh:
+        toks = [t for line in fh if not line.lstrip().startswith("#") for t in line.split()]
+    if len(toks) < 2 or toks[0] != "SimplexGrid":
+        raise ValueError(f"{path}: not a SimplexGrid file")
+    pos = 2  # "SimplexGrid" and the version
+    dim = None
+    coord = cells = faces = None
+    cregs = fregs = None
+    while pos < len(toks):
+        key = toks[pos]
+        pos += 1
+        if key == "DIMENSION":
+            dim = int(toks[pos])
+            pos += 1
+        elif key == "NODES":
+            n, d = int(toks[pos]), int(toks[pos + 1])
+            pos += 2
+            coord = np.array(toks[pos:pos + n * d], dtype=np.float64).reshape(n, d).T.copy()
+            pos += n * d
+        elif key == "CELLS":
+            c = int(toks[pos])
+            pos += 1
+            a = np.array(toks[pos:pos + c * (dim + 2)], dtype=np.int64).reshape(c, dim + 2)
+            pos += c * (dim + 2)
+            cells, cregs = a[:, :dim + 1].T - 1, a[:, dim + 1]
+        elif key == "FACES":
+            f = int(toks[pos])
+            pos += 1
+            a = np.array(toks[pos:pos + f * (dim + 1)], dtype=np.int64).reshape(f, dim + 1)
+            pos += f * (dim + 1)
+            faces, fregs = a[:, :dim].T - 1, a[:, dim]
+        elif key == "END":
+            break
+        else:
+            raise ValueError(f"{path}: unknown section {key}")
+    if dim is None or coord is None or cells is None:
+        raise ValueError(f"{path}: incomplete grid")
+    if faces is None:
+        faces, fregs = np.zeros((dim, 0), np.int64), np.zeros(0, np.int64)
+    return Grid(dim, coord, np.ascontiguousarray(cells.astype(np.int32)), cregs.astype(np.int32),
+                np.ascontiguousarray(faces.astype(np.int32)), fregs.astype(np.int32))

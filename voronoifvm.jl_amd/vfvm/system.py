@@ -202,8 +202,23 @@ class System:
 
     def nodevolumes(self):
         v = np.zeros(self.grid.num_nodes)
-        _capi.check(_capi.lib().vfvm_nodevolumes(self.handle(), _capi.dptr(v)))
+        _capi.check(_capi.lib().vfvm_nodevolumes(self.handle(), _capi.dptr(v)), self.ctx.handle)
         return v
+
+    def update_grid(self, grid: Grid):
+        """update_grid!(system; grid) (src/vfvm_system.jl:607-612): `grid` has the cells and
+        boundary faces of the system's grid and new coordinates; the form factors are
+        recomputed on the device.  Raises VFVMError (geometry unchanged) when an edge with
+        no Jacobian entry (zero factor at creation) would get a nonzero factor."""
+        g = self.grid
+        if (grid.dim != g.dim or grid.num_nodes != g.num_nodes or not np.array_equal(grid.cellnodes, g.cellnodes)
+                or not np.array_equal(grid.bfacenodes, g.bfacenodes)
+                or not np.array_equal(grid.bfaceregions, g.bfaceregions)):
+            raise ValueError("update_grid needs the same cells and boundary faces (only coordinates may change)")
+        coord = np.asfortranarray(grid.coord, dtype=np.float64)
+        if self._h is not None:
+            _capi.check(_capi.lib().vfvm_update_grid(self._h, _capi.dptr(coord)), self.ctx.handle)
+        self.grid = grid
 
     def pattern(self):
         """CSC pattern, 1-based (colptr, rowval) as SparseMatrixCSC(state.matrix) holds it."""
