@@ -42,16 +42,28 @@ def _same(a, b):
     return all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
+def _fan(m=70):
+    """A centre node joined to m rim nodes (node graph row of m+1 entries: more than
+    one 64-lane group); rim faces region 1 on the first half, region 2 on the rest."""
+    t = 2 * np.pi * np.arange(m) / m
+    coord = np.vstack([np.r_[0.0, np.cos(t)], np.r_[0.0, np.sin(t)]])
+    k = np.arange(m)
+    cells = np.vstack([np.zeros(m, np.int32), 1 + k, 1 + (k + 1) % m]).astype(np.int32)
+    faces = np.vstack([1 + k, 1 + (k + 1) % m]).astype(np.int32)
+    regs = np.where(k < m // 2, 1, 2).astype(np.int32)
+    return Grid(2, coord, cells, np.ones(m, np.int32), faces, regs)
+
+
 def _grids():
     X1 = np.sort(np.r_[0.0, 1.0, np.random.default_rng(2).uniform(0, 1, 30)])
     g1 = simplexgrid(X1)
     X = linspace(0.0, 1.0, 9)
     g3 = _moved(simplexgrid(X, X, X), 0.1, 7, boundary=True)  # generic: every edge factor nonzero
     return [("1d", g1, 0.2, False), ("cyl_unstruct", cases.sg_grid("cyl_unstruct.sg"), 0.15, False),
-            ("kuhn9", g3, 0.1, True)]
+            ("kuhn9", g3, 0.1, True), ("fan70", _fan(), 0.3, False)]
 
 
-@pytest.mark.parametrize("case", [0, 1, 2])
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
 def test_update_grid_equals_fresh_system(case):
     name, g0, scale, bnd = _grids()[case]
     _print("case update_grid", name)
@@ -107,3 +119,9 @@ def test_update_grid_refuses_pattern_change():
     fresh.set_compact_graph(False)
     assert _same(_asm(sys, U), _asm(fresh, U))
     cases.compare_assembly(sys, ofresh, U)
+    # non-finite coordinates: refused, factors kept
+    bad = g2.coord.copy()
+    bad[0, 3] = np.nan
+    with pytest.raises(vfvm.VFVMError):
+        sys.update_grid(Grid(2, bad, g0.cellnodes, g0.cellregions, g0.bfacenodes, g0.bfaceregions, g0.shape))
+    assert _same(_asm(sys, U), _asm(fresh, U))

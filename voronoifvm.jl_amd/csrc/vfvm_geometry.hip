@@ -4,10 +4,10 @@
 //
 // Three launches, no atomics on values:
 //   k_geom_cells   one thread per cell: node and edge factors into a cell-major table
-//   k_geom_nodes   one thread per node: walks the node's cells in cell order and sums
-//                  the node volume and, per neighbour, sigma/h in the slot the node
-//                  graph already holds -- the order the host setup sums them in, so
-//                  the factors are bit for bit those of a system created with these
+//   k_geom_nodes   a lane group per node: walks the node's cells in cell order and
+//                  sums the node volume and, per neighbour, sigma/h in the slot the
+//                  node graph already holds -- the order the host setup sums them in,
+//                  so the factors are bit for bit those of a system created with these
 //                  coordinates
 //   k_geom_bfaces  one thread per boundary face: face measure parts into the boundary
 //                  node items (positions fixed at creation)
@@ -42,8 +42,14 @@ __global__ __launch_bounds__(GEOM_BLOCK) void k_geom_cells(int64_t ncells, const
   for (int e = 0; e < NE; e++) o[NV + e] = epar[e];
 }
 
-template <int DIM>
-__global__ __launch_bounds__(GEOM_BLOCK) void k_geom_nodes(int64_t nnodes, const int32_t *__restrict__ inc_ptr,
+// G lanes per node, lane q owning the row's slots q, q+G, ...: every lane walks the
+// node's incidences in cell order (the group reads the same cell record, one
+// transaction for G lanes) and adds the contributions whose far node is its slot's
+// column, so each slot's sum keeps the cell order of the host setup.  flag bit 0: a
+// nonzero contribution found no slot; bit 1: a non-finite coordinate.
+template <int DIM, int G>
+__global__ __launch_bounds__(GEOM_BLOCK) void k_geom_nodes(int64_t nnodes, const double *__restrict__ coord,
+                                                           const int32_t *__restrict__ inc_ptr,
                                                            const int32_t *__restrict__ inc_k,
                                                            const int32_t *__restrict__ cn,
                                                            const double *__restrict__ cf,
@@ -52,42 +58,49 @@ __global__ __launch_bounds__(GEOM_BLOCK) void k_geom_nodes(int64_t nnodes, const
                                                            double *__restrict__ adj_fac, double *__restrict__ nodevol,
                                                            int32_t *__restrict__ flag) {
   constexpr int NV = DIM + 1, NE = geom::edges_per_cell(DIM), ST = NV + NE;
-  const int64_t i = (int64_t)blockIdx.x * GEOM_BLOCK + threadIdx.x;
-  if (i >= nnodes) return;
+  const int lane = threadIdx.x % G;
+  const int64_t i = ((int64_t)blockIdx.x * GEOM_BLOCK + threadIdx.x) / G;
+  if (i >= nnodes) return;  // whole groups leave together: G divides the block
   const int32_t a0 = adj_ptr[i], a1 = adj_ptr[i + 1];
-  for (int32_t q = a0; q < a1; q++) adj_fac[q] = 0.0;
-  double vol = 0.0;
-  bool missing = false;
-  for (int32_t k = inc_ptr[i]; k < inc_ptr[i + 1]; k++) {
-    const int32_t ck = inc_k[k];
-    const int64_t c = ck / NV;
-    const int l = ck - (int)(c * NV);
-    const double *o = cf + c * ST;
-    vol += o[l];
-#pragma unroll
-    for (int e = 0; e < NE; e++) {
-      int a, b;
-      geom::local_edge(DIM, e, a, b);
-      if (a != l && b != l) continue;
-      const int32_t j = cn[c * NV + (a == l ? b : a)];
-      const double f = o[NV + e];
-      // the row's columns are sorted: binary search for j
-      int32_t lo = a0, hi = a1 - 1;
-      while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (adj_col[mid] < j)
-          lo = mid + 1;
-        else
-          hi = mid;
-      }
-      if (lo < a1 && adj_col[lo] == j)
-        adj_fac[lo] = adj_fac[lo] + f;
-      else if (f != 0.0)
-        missing = true;
-    }
+  const int32_t k0 = inc_ptr[i], k1 = inc_ptr[i + 1];
+  int32_t bad = 0;
+  if (lane == 0) {
+    for (int d = 0; d < DIM; d++)
+      if (!isfinite(coord[i * DIM + d])) bad |= 2;
   }
-  nodevol[i] = vol;
-  if (missing) atomicOr(flag, 1);
+  int nz = 0, hit = 0;  // nonzero contributions seen (lane 0) / taken by this lane
+  for (int32_t q0 = a0; q0 < a1; q0 += G) {
+    const int32_t q = q0 + lane;
+    const int32_t col = q < a1 ? adj_col[q] : -1;
+    double acc = 0.0, vol = 0.0;
+    for (int32_t k = k0; k < k1; k++) {
+      const int32_t ck = inc_k[k];
+      const int64_t c = ck / NV;
+      const int l = ck - (int)(c * NV);
+      const double *o = cf + c * ST;
+      vol += o[l];
+#pragma unroll
+      for (int e = 0; e < NE; e++) {
+        int ea, eb;
+        geom::local_edge(DIM, e, ea, eb);
+        if (ea != l && eb != l) continue;
+        const int32_t j = cn[c * NV + (ea == l ? eb : ea)];
+        const double f = o[NV + e];
+        if (j == col) {
+          acc = acc + f;
+          if (f != 0.0) hit++;
+        }
+        if (q0 == a0 && lane == 0 && f != 0.0) nz++;
+      }
+    }
+    if (q < a1) adj_fac[q] = acc;
+    if (q0 == a0 && lane == 0) nodevol[i] = vol;
+  }
+  // every nonzero contribution must have found its slot
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) hit += __shfl_xor(hit, off, G);
+  if (lane == 0 && hit != nz) bad |= 1;
+  if (bad) atomicOr(flag, bad);
 }
 
 template <int DIM>
@@ -130,6 +143,10 @@ void build_geometry_program(vfvm_sys &s) {
   g.coord.alloc((size_t)s.dim * nn);
   g.cf.alloc((size_t)std::max<int64_t>(nc, 1) * (nv + geom::edges_per_cell(s.dim)));
   g.flag.alloc(1);
+  int maxrow = 1;
+  for (int64_t i = 0; i < nn; i++) maxrow = std::max(maxrow, s.hs.adj_ptr[(size_t)i + 1] - s.hs.adj_ptr[(size_t)i]);
+  g.group = 4;
+  while (g.group < maxrow && g.group < 64) g.group *= 2;
   g.built = true;
 }
 
@@ -141,8 +158,22 @@ void launch_geometry(vfvm_sys &s) {
   if (s.ncells > 0)
     hipLaunchKernelGGL(k_geom_cells<DIM>, blocks(s.ncells), dim3(GEOM_BLOCK), 0, st, s.ncells, g.coord.p, g.cn.p,
                        g.cf.p);
-  hipLaunchKernelGGL(k_geom_nodes<DIM>, blocks(s.nnodes), dim3(GEOM_BLOCK), 0, st, s.nnodes, g.inc_ptr.p, g.inc_k.p,
-                     g.cn.p, g.cf.p, s.d_adj_ptr.p, s.d_adj_col.p, s.d_adj_fac.p, s.d_nodevol.p, g.flag.p);
+  const int G = g.group;
+  const dim3 nb = blocks(s.nnodes * G);
+#define VFVM_GEOM_NODES(GG)                                                                                     \
+  hipLaunchKernelGGL((k_geom_nodes<DIM, GG>), nb, dim3(GEOM_BLOCK), 0, st, s.nnodes, g.coord.p, g.inc_ptr.p,     \
+                     g.inc_k.p, g.cn.p, g.cf.p, s.d_adj_ptr.p, s.d_adj_col.p, s.d_adj_fac.p, s.d_nodevol.p, g.flag.p)
+  if (G == 4)
+    VFVM_GEOM_NODES(4);
+  else if (G == 8)
+    VFVM_GEOM_NODES(8);
+  else if (G == 16)
+    VFVM_GEOM_NODES(16);
+  else if (G == 32)
+    VFVM_GEOM_NODES(32);
+  else
+    VFVM_GEOM_NODES(64);
+#undef VFVM_GEOM_NODES
   if (s.nbfaces > 0)
     hipLaunchKernelGGL(k_geom_bfaces<DIM>, blocks(s.nbfaces), dim3(GEOM_BLOCK), 0, st, s.nbfaces, g.coord.p, g.bfn.p,
                        g.bslot.p, s.d_bn_fac.p);
@@ -175,12 +206,11 @@ namespace vfvm {
 void update_geometry(vfvm_sys &s, const double *coord) {
   VFVM_HIP_CHECK(hipSetDevice(s.ctx->device));
   const size_t nc = (size_t)s.dim * s.nnodes;
-  for (size_t k = 0; k < nc; k++)
-    if (!std::isfinite(coord[k])) throw Error(VFVM_ERR_ARG, "coordinates must be finite");
   if (!s.geo.built) build_geometry_program(s);
-  if (run_geometry(s, coord)) {
+  if (const int32_t bad = run_geometry(s, coord)) {
     // restore the factors of the coordinates in use
     run_geometry(s, s.coord.data());
+    if (bad & 2) throw Error(VFVM_ERR_ARG, "coordinates must be finite");
     throw Error(VFVM_ERR_ARG,
                 "the new coordinates give an edge that has no Jacobian entry a nonzero factor "
                 "(the sparsity pattern would change); create a new system for them");

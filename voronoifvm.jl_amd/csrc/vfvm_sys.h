@@ -21,6 +21,7 @@ struct vfvm_sys {
   struct Geometry {
     DevBuf<int32_t> cn, inc_ptr, inc_k, bfn, bslot, flag;
     DevBuf<double> coord, cf;
+    int group = 4;  // lanes per node of k_geom_nodes: power of two >= the longest graph row, at most 64
     bool built = false;
   } geo;
   bool hs_stale = false;       // hs.adj_fac / nodevol / bn_fac older than the device copies

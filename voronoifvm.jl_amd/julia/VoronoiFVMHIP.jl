@@ -7,6 +7,7 @@
 #   solve_step!       (src/vfvm_solver.jl:13-222)     -> hip_solve_step!
 #   _solve_linear!    (src/vfvm_linsolve.jl:6-61)     -> hip_solve_linear!
 #   solve_transient!  (src/vfvm_solver.jl:267-537)    -> hip_solve_transient!
+#   update_grid!      (src/vfvm_system.jl:607-612)    -> hip_update_grid!
 #
 # The Physics callbacks are Julia closures the device cannot call; a system is
 # mapped to a registry entry with `hip_physics` (flux/reaction/storage kinds and
@@ -149,6 +150,21 @@ function hip_solve_step!(ctx, sys, solution, oldsol, control::ControlT, time, ts
                sys, solution, oldsol, time, tstep, λ, control, hist)
     check(rc, ctx)
     return solution, hist
+end
+
+"""
+    hip_update_grid!(ctx, sys, grid)
+
+Replacement of update_grid!(system; grid) (src/vfvm_system.jl:607-612, 690-755) for a
+grid with the system's cells and boundary faces and new coordinates: the form factors
+are recomputed on the device.  Throws if the new coordinates would need a different
+sparsity pattern (the system keeps its previous geometry then).
+"""
+function hip_update_grid!(ctx, sys, grid)
+    coord = Float64.(grid[Coordinates])
+    rc = ccall((:vfvm_update_grid, libvfvm), Cint, (Ptr{Cvoid}, Ptr{Float64}), sys, coord)
+    check(rc, ctx)
+    return sys
 end
 
 struct StepControlT

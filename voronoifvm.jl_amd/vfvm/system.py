@@ -211,9 +211,12 @@ class System:
         recomputed on the device.  Raises VFVMError (geometry unchanged) when an edge with
         no Jacobian entry (zero factor at creation) would get a nonzero factor."""
         g = self.grid
-        if (grid.dim != g.dim or grid.num_nodes != g.num_nodes or not np.array_equal(grid.cellnodes, g.cellnodes)
-                or not np.array_equal(grid.bfacenodes, g.bfacenodes)
-                or not np.array_equal(grid.bfaceregions, g.bfaceregions)):
+
+        def same(a, b):
+            return a is b or np.array_equal(a, b)
+
+        if (grid.dim != g.dim or grid.num_nodes != g.num_nodes or not same(grid.cellnodes, g.cellnodes)
+                or not same(grid.bfacenodes, g.bfacenodes) or not same(grid.bfaceregions, g.bfaceregions)):
             raise ValueError("update_grid needs the same cells and boundary faces (only coordinates may change)")
         coord = np.asfortranarray(grid.coord, dtype=np.float64)
         if self._h is not None:
