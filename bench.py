@@ -216,12 +216,36 @@ def run_cfg1(args):
     for name, b, k in (("assemble", b_asm, _capi.KERNEL_ASSEMBLE), ("spmv64", b_spmv64, _capi.KERNEL_SPMV64),
                        ("spmv32", b_spmv32, _capi.KERNEL_SPMV32)):
         rl[name] = roofline_hbm(b, probes[k], read_traffic("cfg1_" + name), name)
+    # ODE/DAE form on device memory (eval_rhs!, eval_jacobian!): the assembly at
+    # tstep = Inf, then the negation; assembly kernels timed by the probe
+    ud = vfvm.DeviceArray.from_numpy(s.ctx, res["fp32_refine"]["U"])
+    dud, jd = vfvm.DeviceArray(s.ctx, ndof), vfvm.DeviceArray(s.ctx, nnz)
+    s.eval_rhs_device(ud, dud)
+    s.eval_jacobian_device(ud, jd)
+    t1 = time.perf_counter()
+    for _ in range(10):
+        s.eval_rhs_device(ud, dud)
+        s.eval_jacobian_device(ud, jd)
+    ode_pair_ms = 1e3 * (time.perf_counter() - t1) / 10
+    with s.ctx.probe(_capi.KERNEL_RESIDUAL, _capi.KERNEL_ASSEMBLE) as pode:
+        for _ in range(10):
+            s.eval_rhs_device(ud, dud)
+            s.eval_jacobian_device(ud, jd)
+    for name, k, kind in (("ode_rhs", _capi.KERNEL_RESIDUAL, "residual"), ("ode_jacobian", _capi.KERNEL_ASSEMBLE,
+                                                                           "assemble")):
+        w, nl = pode[k]["work"], max(pode[k]["launches"], 1)
+        b = assembly_bytes(1, w[0] / nl, w[1] / nl, w[2] / nl, transient=False)
+        rl[name] = roofline_hbm(b, pode[k], None, name)
+    for x in (ud, dud, jd):
+        x.free()
+    log(f"ODE form: eval_rhs + eval_jacobian on device memory {ode_pair_ms:.2f} ms per pair (wall clock)")
     out = dict(
         value=res["fp32_refine"]["solves_per_s"], ms_per_step=1e3 * res["fp32_refine"]["seconds"] / K,
         plain_gpu_newton_solves_per_s=res["plain_fp64"]["solves_per_s"],
         speedup_vs_plain=res["fp32_refine"]["solves_per_s"] / res["plain_fp64"]["solves_per_s"],
         max_abs_diff_fp32_vs_fp64=err32, roofline=rl["spmv32"], roofline_assemble=rl["assemble"],
         roofline_spmv64=rl["spmv64"], geometry_update_s=min(tg), system_create_s=create_s,
+        roofline_ode_rhs=rl["ode_rhs"], roofline_ode_jacobian=rl["ode_jacobian"], ode_rhs_jacobian_pair_ms=ode_pair_ms,
         config=dict(workload=f"cfg1: 3D nonlinear diffusion D(u)=1+u^2, tetrahedral {n}^3 grid, implicit Euler "
                              f"dt=1e-2, BiCGStab+ILU(0) FP32 inside FP64 Newton (plain: FP64 Krylov)",
                     nodes=n_, unknowns=ndof, nnz=nnz, steps_per_solve="one implicit-Euler step"),

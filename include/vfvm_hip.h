@@ -183,6 +183,23 @@ int vfvm_initialize_dirichlet(vfvm_sys *sys, double *U, double time, double lamb
 int vfvm_assemble(vfvm_sys *sys, const double *U, const double *Uold, double time, double tstep,
                   double lambda, double *F, double *nzval);
 
+/* ODE/DAE form of the discrete system (src/vfvm_diffeq_interface.jl): with
+ * M du/dt = rhs(u), rhs(u) = -F(u) and jacobian = -dF/du, F assembled at U = Uold = u,
+ * tstep = Inf, embedding 0 (_eval_res_jac!, lines 10-23).
+ *   vfvm_eval_rhs       eval_rhs!(du, u, state, t)       lines 31-36   (u, du: ndof)
+ *   vfvm_eval_jacobian  eval_jacobian!(J, u, state, t)   lines 44-50   (values in the
+ *                       order of vfvm_pattern, i.e. nonzeros of the CSC matrix)
+ *   vfvm_mass_matrix    mass_matrix(state)               lines 58-96   (the diagonal, ndof
+ *                       entries: storage derivative times node volume; the registry's
+ *                       storage terms are local per species, so the matrix is diagonal)
+ * The _device forms take and return device memory: ddu has ndof entries, dnzval_csr
+ * the Jacobian values in the row-major order of vfvm_pattern_csr. */
+int vfvm_eval_rhs(vfvm_sys *sys, const double *u, double time, double *du);
+int vfvm_eval_jacobian(vfvm_sys *sys, const double *u, double time, double *nzval);
+int vfvm_mass_matrix(vfvm_sys *sys, double *diag);
+int vfvm_eval_rhs_device(vfvm_sys *sys, const double *du, double time, double *ddu);
+int vfvm_eval_jacobian_device(vfvm_sys *sys, const double *du, double time, double *dnzval_csr);
+
 /* Linear solve A x = b on the currently assembled Jacobian (after vfvm_assemble or inside
  * vfvm_newton): _solve_linear!, src/vfvm_linsolve.jl:6-61. */
 int vfvm_linsolve(vfvm_sys *sys, const double *b, double *x, const vfvm_control_t *control,

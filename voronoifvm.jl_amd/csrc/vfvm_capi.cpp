@@ -729,6 +729,95 @@ int vfvm_assemble(vfvm_sys *s, const double *U, const double *Uold, double time,
   });
 }
 
+// ODE/DAE form (src/vfvm_diffeq_interface.jl): the assembly at U = Uold = u with
+// tstep = Inf and embedding 0, negated: rhs = -F(u), jacobian = -dF/du.
+namespace {
+void ode_rhs_device(vfvm_sys &s, const double *du, double *dout) {
+  hipStream_t st = s.ctx->stream;
+  AsmArgs a = asm_args(s, du, du, dout, nullptr, INFINITY, 0.0);
+  s.ctx->probe.before(VFVM_KERNEL_RESIDUAL, st);
+  launch_assemble(a, s.asm_group, st);
+  s.ctx->probe.after(VFVM_KERNEL_RESIDUAL, st, s.nnodes, (int64_t)s.hs.adj_ptr.back() - s.nnodes, 0);
+  launch_negate(s.ndof, dout, st);
+}
+
+void ode_jacobian_device(vfvm_sys &s, const double *du, double *dnz_csr) {
+  hipStream_t st = s.ctx->stream;
+  launch_fill_i32(1, s.d_flags.p, 0, st);
+  AsmArgs a = asm_args(s, du, du, s.d_F.p, dnz_csr, INFINITY, 0.0);
+  s.ctx->probe.before(VFVM_KERNEL_ASSEMBLE, st);
+  launch_assemble(a, s.asm_group, st);
+  s.ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st, s.nnodes, (int64_t)s.hs.adj_ptr.back() - s.nnodes, s.nnz);
+  launch_negate(s.nnz, dnz_csr, st);
+  int32_t fl = 0;
+  VFVM_HIP_CHECK(hipMemcpyAsync(&fl, s.d_flags.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  VFVM_HIP_CHECK(hipStreamSynchronize(st));
+  if (fl & 1) throw Error(VFVM_ERR_ASSEMBLY, "trying to assemble NaN");
+}
+}  // namespace
+
+int vfvm_eval_rhs_device(vfvm_sys *s, const double *du, double time, double *ddu) {
+  if (!s || !du || !ddu) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    (void)time;
+    ode_rhs_device(*s, du, ddu);
+    VFVM_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    return VFVM_OK;
+  });
+}
+
+int vfvm_eval_jacobian_device(vfvm_sys *s, const double *du, double time, double *dnzval_csr) {
+  if (!s || !du || !dnzval_csr) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    (void)time;
+    ode_jacobian_device(*s, du, dnzval_csr);
+    return VFVM_OK;
+  });
+}
+
+int vfvm_eval_rhs(vfvm_sys *s, const double *u, double time, double *du) {
+  if (!s || !u || !du) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    (void)time;
+    hipStream_t st = s->ctx->stream;
+    s->d_U.upload(u, (size_t)s->ndof, st);
+    ode_rhs_device(*s, s->d_U.p, s->d_tmp.p);
+    s->d_tmp.download(du, (size_t)s->ndof, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return VFVM_OK;
+  });
+}
+
+int vfvm_eval_jacobian(vfvm_sys *s, const double *u, double time, double *nzval) {
+  if (!s || !u || !nzval) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    (void)time;
+    hipStream_t st = s->ctx->stream;
+    s->d_U.upload(u, (size_t)s->ndof, st);
+    ode_jacobian_device(*s, s->d_U.p, s->d_J.p);
+    DevBuf<double> tmp;
+    tmp.alloc((size_t)s->nnz);
+    launch_gather(s->nnz, s->d_csc_perm.p, s->d_J.p, tmp.p, st);
+    tmp.download(nzval, (size_t)s->nnz, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return VFVM_OK;
+  });
+}
+
+int vfvm_mass_matrix(vfvm_sys *s, double *diag) {
+  if (!s || !diag) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    // the registry's storage terms are per species and local: s_i(u) = u_i or none, so
+    // the storage Jacobian is diagonal with entries 1 or 0 (at any u, in particular
+    // the reference's u = 0), times the node volume
+    sync_host_geometry(*s);
+    const double ds = s->phys.storage == VFVM_STOR_IDENTITY ? 1.0 : 0.0;
+    for (int64_t K = 0; K < s->nnodes; K++)
+      for (int i = 0; i < s->nspec; i++) diag[K * s->nspec + i] = ds * s->hs.nodevol[(size_t)K];
+    return VFVM_OK;
+  });
+}
+
 int vfvm_linsolve(vfvm_sys *s, const double *b, double *x, const vfvm_control_t *control, vfvm_linstats_t *stats) {
   if (!s || !b || !x) return VFVM_ERR_ARG;
   return guarded(s->ctx, [&]() {

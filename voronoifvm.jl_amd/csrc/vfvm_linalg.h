@@ -208,6 +208,7 @@ void launch_maxdiff(int64_t n, const double *a, const double *b, double *partial
 void launch_fill_u64(int64_t n, unsigned long long *p, hipStream_t s);
 void launch_fill_i32(int64_t n, int32_t *p, int32_t v, hipStream_t s);
 void launch_copy(int64_t n, const double *a, double *b, hipStream_t s);
+void launch_negate(int64_t n, double *x, hipStream_t s);  // x = -x
 // row scaling S = 1/diag(A); scaled copies of A in FP32 and/or FP64 go to CSR order
 // (rbase null) or to SELL-64 order (entry j of row r at rbase[r] + 64 j)
 void launch_scale_rows(int nspec, int64_t nnodes, const int32_t *rowptr, const int32_t *colind, const double *A,

@@ -288,6 +288,10 @@ __global__ void __launch_bounds__(VB) k_convert(int64_t n, const TI *__restrict_
   for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB) b[i] = (TO)a[i];
 }
 
+__global__ void __launch_bounds__(VB) k_negate(int64_t n, double *__restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB) x[i] = -x[i];
+}
+
 // x64 += d32
 template <class VD>
 __global__ void __launch_bounds__(VB) k_add_correction(int64_t n, double *__restrict__ x, const VD *__restrict__ d,
@@ -1285,6 +1289,11 @@ void launch_fill_u64(int64_t n, unsigned long long *p, hipStream_t s) {
 }
 void launch_fill_i32(int64_t n, int32_t *p, int32_t v, hipStream_t s) {
   hipLaunchKernelGGL((k_fill<int32_t>), dim3(grid_for(n)), dim3(VB), 0, s, n, p, v);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+void launch_negate(int64_t n, double *x, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_negate, dim3(grid_for(n)), dim3(VB), 0, s, n, x);
   VFVM_HIP_CHECK(hipGetLastError());
 }
 void launch_copy(int64_t n, const double *a, double *b, hipStream_t s) {

@@ -8,13 +8,15 @@
 #   _solve_linear!    (src/vfvm_linsolve.jl:6-61)     -> hip_solve_linear!
 #   solve_transient!  (src/vfvm_solver.jl:267-537)    -> hip_solve_transient!
 #   update_grid!      (src/vfvm_system.jl:607-612)    -> hip_update_grid!
+#   eval_rhs!, eval_jacobian!, mass_matrix
+#                     (src/vfvm_diffeq_interface.jl)  -> hip_eval_rhs!, hip_eval_jacobian!, hip_mass_matrix
 #
 # The Physics callbacks are Julia closures the device cannot call; a system is
 # mapped to a registry entry with `hip_physics` (flux/reaction/storage kinds and
 # parameters) and its boundary conditions to a table with `hip_bc_table`.
 module VoronoiFVMHIP
 
-using SparseArrays
+using LinearAlgebra, SparseArrays
 const libvfvm = joinpath(@__DIR__, "..", "libvfvm_hip.so")
 
 # status codes (src/vfvm_logging_exceptions.jl:6-29)
@@ -165,6 +167,35 @@ function hip_update_grid!(ctx, sys, grid)
     rc = ccall((:vfvm_update_grid, libvfvm), Cint, (Ptr{Cvoid}, Ptr{Float64}), sys, coord)
     check(rc, ctx)
     return sys
+end
+
+"""
+    hip_eval_rhs!(ctx, sys, du, u, t)
+    hip_eval_jacobian!(ctx, sys, J::SparseMatrixCSC, u, t)
+    hip_mass_matrix(ctx, sys) -> Diagonal
+
+The ODE/DAE form (src/vfvm_diffeq_interface.jl:31-96) for an ODEFunction: du = -F(u)
+and J = -dF/du assembled with tstep = Inf; the mass matrix is diagonal for every
+registry physics.  J has the pattern of hip_pattern(sys).
+"""
+function hip_eval_rhs!(ctx, sys, du, u, t)
+    rc = ccall((:vfvm_eval_rhs, libvfvm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Float64, Ptr{Float64}), sys, u, t, du)
+    check(rc, ctx)
+    return du
+end
+
+function hip_eval_jacobian!(ctx, sys, J::SparseMatrixCSC, u, t)
+    rc = ccall((:vfvm_eval_jacobian, libvfvm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Float64, Ptr{Float64}), sys, u, t,
+               nonzeros(J))
+    check(rc, ctx)
+    return J
+end
+
+function hip_mass_matrix(ctx, sys, ndof)
+    d = zeros(ndof)
+    rc = ccall((:vfvm_mass_matrix, libvfvm), Cint, (Ptr{Cvoid}, Ptr{Float64}), sys, d)
+    check(rc, ctx)
+    return Diagonal(d)
 end
 
 struct StepControlT

@@ -10,6 +10,7 @@ from .physics import (Dirichlet, DriftDiffusionFlux, DriftDiffusionReaction, Exc
                       LinearFlux, Neumann, NonlinearDiffusionFlux, Physics, PowerFlux, PowerReaction, Robin,
                       SinhReaction)
 from .system import (Batch, NewtonSolverHistory, System, TransientSolution, boundary_dirichlet, boundary_neumann,
-                     boundary_robin, dofs, enable_species, evaluate_residual_and_jacobian, integrate, solve, unknowns)
+                     boundary_robin, dofs, enable_species, eval_jacobian, eval_rhs, evaluate_residual_and_jacobian,
+                     integrate, mass_matrix, solve, unknowns)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

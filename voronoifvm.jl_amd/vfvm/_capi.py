@@ -121,6 +121,11 @@ SIGNATURES = {
                                            C.POINTER(HistoryT)]),
     "vfvm_batch_sizes": (C.c_int, [_vp, _lp, _lp]),
     "vfvm_assemble_device": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp, _vp]),
+    "vfvm_eval_rhs": (C.c_int, [_vp, _dp, C.c_double, _dp]),
+    "vfvm_eval_jacobian": (C.c_int, [_vp, _dp, C.c_double, _dp]),
+    "vfvm_mass_matrix": (C.c_int, [_vp, _dp]),
+    "vfvm_eval_rhs_device": (C.c_int, [_vp, _vp, C.c_double, _vp]),
+    "vfvm_eval_jacobian_device": (C.c_int, [_vp, _vp, C.c_double, _vp]),
 }
 
 

@@ -205,6 +205,16 @@ class System:
         _capi.check(_capi.lib().vfvm_nodevolumes(self.handle(), _capi.dptr(v)), self.ctx.handle)
         return v
 
+    def eval_rhs_device(self, u, du, t=0.0):
+        """eval_rhs! on device memory: u and du are DeviceArrays of ndof entries."""
+        _capi.check(_capi.lib().vfvm_eval_rhs_device(self.handle(), u.ptr, float(t), du.ptr), self.ctx.handle)
+
+    def eval_jacobian_device(self, u, nzval_csr, t=0.0):
+        """eval_jacobian! on device memory: the values of -dF/du in the row-major order of
+        pattern_csr() into the DeviceArray nzval_csr (nnz entries)."""
+        _capi.check(_capi.lib().vfvm_eval_jacobian_device(self.handle(), u.ptr, float(t), nzval_csr.ptr),
+                    self.ctx.handle)
+
     def update_grid(self, grid: Grid):
         """update_grid!(system; grid) (src/vfvm_system.jl:607-612): `grid` has the cells and
         boundary faces of the system's grid and new coordinates; the form factors are
@@ -230,6 +240,17 @@ class System:
         rv = np.zeros(max(nnz, 1), np.int64)
         _capi.check(_capi.lib().vfvm_pattern(self.handle(), _capi.lptr(cp), _capi.lptr(rv)))
         return cp, rv[:nnz]
+
+    def pattern_csr(self):
+        """The device CSR pattern, 0-based (rowptr, colind): the order of the *_device
+        Jacobian values."""
+        import ctypes as C
+        ndof, nnz, _ = self.sizes()
+        rp = np.zeros(ndof + 1, np.int32)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        ip = C.POINTER(C.c_int32)
+        _capi.check(_capi.lib().vfvm_pattern_csr(self.handle(), rp.ctypes.data_as(ip), ci.ctypes.data_as(ip)))
+        return rp, ci[:nnz]
 
 
 def unknowns(sys: System, inival=0.0):
@@ -273,6 +294,44 @@ def evaluate_residual_and_jacobian(sys: System, u, t=0.0, tstep=math.inf, embed=
     cp, rv = sys.pattern()
     A = sp.csc_matrix((nz[:nnz], rv - 1, cp - 1), shape=(ndof, ndof))
     return _from_dofs(sys, F), A
+
+
+def eval_rhs(sys: System, u, t=0.0):
+    """eval_rhs!(du, u, state, t) (src/vfvm_diffeq_interface.jl:31-36): du = -F(u) with
+    F assembled at U = Uold = u, tstep = Inf.  u and the result are dof vectors."""
+    U = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+    ndof, _, _ = sys.sizes()
+    if U.size != ndof:
+        raise ValueError(f"u must have {ndof} entries")
+    du = np.zeros(ndof)
+    _capi.check(_capi.lib().vfvm_eval_rhs(sys.handle(), _capi.dptr(U), float(t), _capi.dptr(du)), sys.ctx.handle)
+    return du
+
+
+def eval_jacobian(sys: System, u, t=0.0):
+    """eval_jacobian!(J, u, state, t) (src/vfvm_diffeq_interface.jl:44-50): -dF/du at u as a
+    scipy CSC matrix with the system's pattern."""
+    import scipy.sparse as sp
+    U = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+    ndof, nnz, _ = sys.sizes()
+    if U.size != ndof:
+        raise ValueError(f"u must have {ndof} entries")
+    nz = np.zeros(max(nnz, 1))
+    _capi.check(_capi.lib().vfvm_eval_jacobian(sys.handle(), _capi.dptr(U), float(t), _capi.dptr(nz)),
+                sys.ctx.handle)
+    cp, rv = sys.pattern()
+    return sp.csc_matrix((nz[:nnz], rv - 1, cp - 1), shape=(ndof, ndof))
+
+
+def mass_matrix(sys: System):
+    """mass_matrix(state) (src/vfvm_diffeq_interface.jl:58-96): diagonal for every physics
+    of the registry, returned as a scipy sparse diagonal matrix (the reference returns a
+    Diagonal when the matrix is diagonal)."""
+    import scipy.sparse as sp
+    ndof, _, _ = sys.sizes()
+    d = np.zeros(ndof)
+    _capi.check(_capi.lib().vfvm_mass_matrix(sys.handle(), _capi.dptr(d)), sys.ctx.handle)
+    return sp.diags(d, format="dia")
 
 
 def _control(control, kwargs):
