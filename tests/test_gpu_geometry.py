@@ -125,3 +125,53 @@ def test_update_grid_refuses_pattern_change():
     with pytest.raises(vfvm.VFVMError):
         sys.update_grid(Grid(2, bad, g0.cellnodes, g0.cellregions, g0.bfacenodes, g0.bfaceregions, g0.shape))
     assert _same(_asm(sys, U), _asm(fresh, U))
+
+
+def _setup_grids():
+    X = linspace(0.0, 1.0, 9)
+    X7 = linspace(0.0, 1.0, 7)
+    return [("1d", _grids()[0][1]), ("tensor2d", simplexgrid(X, X)), ("kuhn7", simplexgrid(X7, X7, X7)),
+            ("rz2d", cases.sg_grid("rz2d.sg")), ("cyl_unstruct", cases.sg_grid("cyl_unstruct.sg")),
+            ("kuhn9_moved", _grids()[2][1]), ("fan70", _fan())]
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_device_setup_equals_host_setup(case):
+    """System creation with the node graph, form factors and node volumes built on the
+    GPU gives bit for bit the system the host build gives: pattern, edge count, node
+    volumes, residual and Jacobian; grids with exactly-zero factors (dropped edges)
+    included.  A geometry update of the device-built system matches too."""
+    name, g = _setup_grids()[case]
+    _print("case device setup", name)
+    ctx = vfvm.System(g, nspec=1).ctx
+    out = []
+    try:
+        for on_device in (False, True):
+            ctx.set_setup(on_device)
+            sys, _ = cases.nonlinear_poisson_on(g)
+            sys.handle()
+            U = np.random.default_rng(6).uniform(-1, 1, (1, g.num_nodes))
+            out.append((sys, sys.pattern(), sys.sizes(), sys.nodevolumes(), _asm(sys, U), U))
+    finally:
+        ctx.set_setup(True)
+    (sh, ph, zh, vh, ah, U), (sd, pd, zd, vd, ad, _) = out
+    assert all(np.array_equal(a, b) for a, b in zip(ph, pd)) and zh == zd
+    assert np.array_equal(vh, vd)
+    assert _same(ah, ad)
+    g1 = _moved(g, 0.1, 3, boundary=True) if name in ("kuhn9_moved", "fan70", "1d", "cyl_unstruct") else None
+    if g1 is not None:
+        sh.update_grid(g1)
+        sd.update_grid(g1)
+        assert _same(_asm(sh, U), _asm(sd, U))
+
+
+def test_device_setup_rejects_bad_cells():
+    _print("case device setup bad cells")
+    X = linspace(0.0, 1.0, 5)
+    g = simplexgrid(X, X)
+    cn = g.cellnodes.copy()
+    cn[1, 3] = g.num_nodes  # one past the last node
+    bad = Grid(2, g.coord, cn, g.cellregions, g.bfacenodes, g.bfaceregions, g.shape)
+    sys = vfvm.System(bad, flux=vfvm.LinearFlux(1.0), species=[1])
+    with pytest.raises(vfvm.VFVMError):
+        sys.handle()

@@ -514,14 +514,25 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     s->phys = *physics;
     s->bc.assign((size_t)std::max(1, nbregions) * nspec, vfvm_bc_t{});
     if (bc && nbregions > 0) std::copy(bc, bc + (size_t)nbregions * nspec, s->bc.begin());
-    build_setup(dim, nnodes, coord, ncells, cellnodes, nbfaces, bfacenodes, bfaceregions, s->hs);
-    // ranges checked by build_setup
+    if (dim < 1 || dim > 3) throw Error(VFVM_ERR_ARG, "dim must be 1, 2 or 3");
+    if (ncells < 0 || nbfaces < 0) throw Error(VFVM_ERR_ARG, "negative item count");
+    VFVM_HIP_CHECK(hipHostMalloc((void **)&s->h_pinned, 8 * sizeof(double), hipHostMallocDefault));
     s->ncells = ncells;
     s->nbfaces = nbfaces;
-    s->cellnodes0.resize((size_t)(dim + 1) * ncells);
-    for (size_t k = 0; k < s->cellnodes0.size(); k++) s->cellnodes0[k] = (int32_t)(cellnodes[k] - 1);
+    const bool dev_setup = ctx->setup_on_device;
+    if (dev_setup) {
+      // cells part on the device (form factors, node graph), boundary items on the host
+      build_node_graph_device(*s, coord, cellnodes, ncells);
+      build_boundary(dim, nnodes, coord, nbfaces, bfacenodes, bfaceregions, s->hs);
+    } else {
+      build_setup(dim, nnodes, coord, ncells, cellnodes, nbfaces, bfacenodes, bfaceregions, s->hs);
+      // ranges checked by build_setup
+      s->cellnodes0.resize((size_t)(dim + 1) * ncells);
+      for (size_t k = 0; k < s->cellnodes0.size(); k++) s->cellnodes0[k] = (int32_t)(cellnodes[k] - 1);
+    }
     s->bfacenodes0.resize((size_t)dim * nbfaces);
     for (size_t k = 0; k < s->bfacenodes0.size(); k++) s->bfacenodes0[k] = (int32_t)(bfacenodes[k] - 1);
+    if (dev_setup) finish_device_geometry(*s);
     for (int64_t b = 0; b < (int64_t)s->hs.bn_reg.size(); b++)
       if (s->hs.bn_reg[b] > MAXBREG) throw Error(VFVM_ERR_ARG, "boundary region number above 8");
     s->masks = make_masks(nspec, s->phys, s->bc.data(), nbregions);
@@ -539,10 +550,13 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     }
     hipStream_t st = ctx->stream;
     const HostSetup &h = s->hs;
-    s->d_adj_ptr.upload(h.adj_ptr, st);
-    s->d_adj_col.upload(h.adj_col, st);
-    s->d_adj_diag.upload(h.adj_diag, st);
-    s->d_adj_fac.upload(h.adj_fac, st);
+    if (!dev_setup) {  // the device build left them in place
+      s->d_adj_ptr.upload(h.adj_ptr, st);
+      s->d_adj_col.upload(h.adj_col, st);
+      s->d_adj_diag.upload(h.adj_diag, st);
+      s->d_adj_fac.upload(h.adj_fac, st);
+      s->d_nodevol.upload(h.nodevol, st);
+    }
     {
       std::vector<int16_t> off;
       std::vector<uint16_t> fidx;
@@ -554,7 +568,6 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
         s->d_fac_tab.upload(tab, st);
       }
     }
-    s->d_nodevol.upload(h.nodevol, st);
     s->d_bn_ptr.upload(h.bn_ptr, st);
     s->d_bn_reg.upload(h.bn_reg.empty() ? std::vector<int32_t>{0} : h.bn_reg, st);
     s->d_bn_fac.upload(h.bn_fac.empty() ? std::vector<double>{0.0} : h.bn_fac, st);
@@ -586,7 +599,6 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     VFVM_HIP_CHECK(hipMemsetAsync(s->d_ticket.p, 0, 4 * sizeof(unsigned), st));
     VFVM_HIP_CHECK(hipMemsetAsync(s->d_U.p, 0, s->ndof * sizeof(double), st));
     VFVM_HIP_CHECK(hipMemsetAsync(s->d_Uold.p, 0, s->ndof * sizeof(double), st));
-    VFVM_HIP_CHECK(hipHostMalloc((void **)&s->h_pinned, 8 * sizeof(double), hipHostMallocDefault));
     upload_inst(*s);
     // default group sizes from the longest row; vfvm_autotune refines them on the device
     int maxdeg = 1, maxrow = 1;
@@ -607,7 +619,6 @@ int vfvm_system_destroy(vfvm_sys *s) {
   if (!s) return VFVM_OK;
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
-  if (s->h_pinned) (void)hipHostFree(s->h_pinned);
   delete s;
   return VFVM_OK;
 }
@@ -650,6 +661,12 @@ int vfvm_nodevolumes(vfvm_sys *s, double *vol) {
     std::copy(s->hs.nodevol.begin(), s->hs.nodevol.end(), vol);
     return VFVM_OK;
   });
+}
+
+int vfvm_ctx_set_setup(vfvm_ctx *ctx, int on_device) {
+  if (!ctx) return VFVM_ERR_ARG;
+  ctx->setup_on_device = on_device != 0;
+  return VFVM_OK;
 }
 
 int vfvm_update_grid(vfvm_sys *s, const double *coord) {

@@ -118,6 +118,9 @@ struct HostSetup {
 
 void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, const int64_t *cellnodes,
                  int64_t nbfaces, const int64_t *bfacenodes, const int64_t *bfaceregions, HostSetup &out);
+// the boundary node items of build_setup alone (bn_ptr, bn_reg, bn_fac, bnodes)
+void build_boundary(int dim, int64_t nnodes, const double *coord, int64_t nbfaces, const int64_t *bfacenodes,
+                    const int64_t *bfaceregions, HostSetup &out);
 Masks make_masks(int nspec, const vfvm_physics_t &ph, const vfvm_bc_t *bc, int nbreg);
 void build_dof_csr(const HostSetup &hs, int nspec, const Masks &m, std::vector<int32_t> &rowptr,
                    std::vector<int32_t> &colind);
@@ -306,6 +309,7 @@ struct KernelProbe {
 
 struct vfvm_ctx {
   int device = 0;
+  bool setup_on_device = true;  // node graph and form factors of new systems built by the GPU
   hipStream_t stream = nullptr;
   std::string err;
   vfvm::KernelProbe probe;

@@ -145,6 +145,11 @@ void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, c
     nedge2 += rlen[i];
   }
   out.nedges = nedge2 / 2;
+  build_boundary(dim, nnodes, coord, nbfaces, bfacenodes1, bfaceregions, out);
+}
+
+void build_boundary(int dim, int64_t nnodes, const double *coord, int64_t nbfaces, const int64_t *bfacenodes1,
+                    const int64_t *bfaceregions, HostSetup &out) {
   // boundary node items in bface order
   std::vector<int64_t> bcnt((size_t)nnodes + 1, 0);
   for (int64_t ib = 0; ib < nbfaces; ib++)

@@ -52,10 +52,21 @@ struct vfvm_sys {
   LinearSolver lin;
   int asm_group = 8, spmv_group = 8;
   double *h_pinned = nullptr;  // small pinned readback area
+  vfvm_sys() = default;
+  vfvm_sys(const vfvm_sys &) = delete;
+  vfvm_sys &operator=(const vfvm_sys &) = delete;
+  ~vfvm_sys() {
+    if (h_pinned) (void)hipHostFree(h_pinned);
+  }
 };
 
 namespace vfvm {
 InstParams make_inst(const vfvm_sys &s);
+// the cells part of build_setup on the device: fills hs (node graph, factors, node
+// volumes; host copies) and the device node graph, and keeps the incidences for
+// update_geometry; finish_device_geometry completes that after build_boundary
+void build_node_graph_device(vfvm_sys &s, const double *coord, const int64_t *cellnodes1, int64_t ncells);
+void finish_device_geometry(vfvm_sys &s);
 // new coordinates, same topology: factors recomputed on the device (vfvm_geometry.hip)
 void update_geometry(vfvm_sys &s, const double *coord);
 // brings hs.adj_fac / nodevol / bn_fac up to date after update_geometry

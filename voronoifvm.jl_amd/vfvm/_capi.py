@@ -74,6 +74,7 @@ SIGNATURES = {
     "vfvm_system_sizes": (C.c_int, [_vp, _lp, _lp, _lp]),
     "vfvm_nodevolumes": (C.c_int, [_vp, _dp]),
     "vfvm_update_grid": (C.c_int, [_vp, _dp]),
+    "vfvm_ctx_set_setup": (C.c_int, [_vp, C.c_int]),
     "vfvm_pattern": (C.c_int, [_vp, _lp, _lp]),
     "vfvm_initialize_dirichlet": (C.c_int, [_vp, _dp, C.c_double, C.c_double]),
     "vfvm_assemble": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),
@@ -200,6 +201,11 @@ class Context:
         if getattr(self, "handle", None) and _LIB is not None:
             _LIB.vfvm_ctx_destroy(self.handle)
             self.handle = None
+
+    def set_setup(self, on_device: bool):
+        """Where new systems build their node graph and form factors: the GPU (default)
+        or the host; the two give the same system bit for bit."""
+        check(lib().vfvm_ctx_set_setup(self.handle, int(bool(on_device))), self.handle)
 
     def probe(self, *kernels: int, max_launches: int = 50000):
         """Context manager timing every launch of the given kernel classes (KERNEL_*)
