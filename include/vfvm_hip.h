@@ -206,6 +206,33 @@ int vfvm_mass_matrix(vfvm_sys *sys, double *diag);
 int vfvm_eval_rhs_device(vfvm_sys *sys, const double *du, double time, double *ddu);
 int vfvm_eval_jacobian_device(vfvm_sys *sys, const double *du, double time, double *dnzval_csr);
 
+/* Small-signal impedance (src/vfvm_impedance.jl).
+ * vfvm_impedance_create: ImpedanceSystem(system, U0, excited_spec, excited_bc), lines
+ *   159-186 with 49-152: J(U0) assembled with tstep = 1e30 and Uold = U0, the storage
+ *   derivative, and the excitation of species excited_spec (1-based) on boundary region
+ *   excited_bc: 1e30 per boundary item for a legacy Dirichlet condition
+ *   (boundary_dirichlet!(sys, ...)), factor times the item's measure for a Robin one,
+ *   nothing otherwise (as the reference, which reads only the legacy boundary factors).
+ * vfvm_measurement_derivatives: measurement_derivative (lines 235-252) of
+ *   U -> -integrate_stdy(sys, T, U)[spec] and U -> -integrate_tran(sys, T, U)[spec]
+ *   (src/vfvm_testfunctions.jl:398-420) at U0 for a test function T (nnodes values):
+ *   dstdy, dtran have ndof entries.
+ * vfvm_impedance_sweep: solve!(UZ, isys, w) and impedance(isys, w, U0, dstdy, dtran)
+ *   (lines 205-232, 268-288) for nomega frequencies at once on the GPU (batched BiCGStab,
+ *   relative residual of the row-scaled system <= reltol).  UZ (nullable): nomega x ndof
+ *   complex values, interleaved (re, im); Z (nullable, needs dstdy and dtran): nomega
+ *   complex impedances 1 / (dstdy.UZ + i w dtran.UZ); iters (nullable): iterations per
+ *   frequency.  VFVM_ERR_LINEAR if a frequency does not converge. */
+typedef struct vfvm_impedance vfvm_impedance;
+int vfvm_impedance_create(vfvm_sys *sys, const double *U0, int excited_spec, int excited_bc,
+                          vfvm_impedance **out);
+int vfvm_impedance_destroy(vfvm_impedance *imp);
+int vfvm_measurement_derivatives(vfvm_sys *sys, const double *U0, const double *T, int spec,
+                                 double *dstdy, double *dtran);
+int vfvm_impedance_sweep(vfvm_impedance *imp, int64_t nomega, const double *omega, const double *dstdy,
+                         const double *dtran, double reltol, int32_t maxiters, double *UZ, double *Z,
+                         int32_t *iters);
+
 /* Linear solve A x = b on the currently assembled Jacobian (after vfvm_assemble or inside
  * vfvm_newton): _solve_linear!, src/vfvm_linsolve.jl:6-61. */
 int vfvm_linsolve(vfvm_sys *sys, const double *b, double *x, const vfvm_control_t *control,

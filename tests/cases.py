@@ -11,6 +11,16 @@ from vfvm.grids import linspace, simplexgrid
 ORACLE_BC = {vfvm.Dirichlet: "dirichlet", vfvm.Robin: "robin"}
 
 
+def oracle_physics(ph, nspec):
+    """The oracle's parameter block from the same registry objects."""
+    flux, react, stor, p = ph.param_block(nspec)
+    oph = O.Physics()
+    oph.flux, oph.reaction, oph.storage = flux, react, stor
+    for k in range(16):
+        oph.p[k] = p[k]
+    return oph
+
+
 def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None, oracle=True):
     """phys_kw: dict(flux=..., reaction=..., storage=...) of vfvm registry objects.
     bcs: vfvm.Dirichlet / vfvm.Robin entries; legacy: (species, region, value) Dirichlet.
@@ -21,12 +31,7 @@ def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None, oracle=True):
         sys.boundary_dirichlet(sp, reg, val)
     if not oracle:
         return sys, None
-    # oracle parameter block from the same registry objects
-    flux, react, stor, p = ph.param_block(nspec)
-    oph = O.Physics()
-    oph.flux, oph.reaction, oph.storage = flux, react, stor
-    for k in range(16):
-        oph.p[k] = p[k]
+    oph = oracle_physics(ph, nspec)
     obcs = {}
     for b in bcs:
         if isinstance(b, vfvm.Dirichlet):

@@ -1,0 +1,114 @@
+"""Small-signal impedance (src/vfvm_impedance.jl, Example150_Impedance1D.jl) on the
+GPU against the oracle (its assembly, a direct complex solve per frequency, the
+test function and measurement rows formed from the oracle's matrices) and against
+the closed-form impedance of the continuous Example150 problem."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import cases
+import oracle as O
+import vfvm
+from vfvm.grids import linspace, simplexgrid
+
+pytestmark = pytest.mark.gpu
+
+OMEGAS = 1e-3 * 1.1 ** np.arange(113)  # Example150: w from 1e-3 by factors 1.1 below 50
+ZTOL = 1e-7  # impedance vs the oracle's direct solve (Krylov tolerance 1e-12 x condition)
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _problem(g, C=1.0, R=1.0, D=1.0, excited=1, meas=2):
+    """C u_t - (D u_x)_x + R u = 0, u = 1 on region `excited`, u = 0 on region `meas`
+    (Example150_Impedance1D.jl:38-88; storage coefficient C = 1)."""
+    assert C == 1.0
+    phys = dict(flux=vfvm.LinearFlux(D), reaction=vfvm.PowerReaction(m=1, k=R), storage=vfvm.IdentityStorage())
+    sys, osys = cases.pair(g, 1, phys, legacy=[(1, excited, 1.0), (1, meas, 0.0)])
+    return sys, osys, vfvm.Physics(**phys)
+
+
+def _oracle(g, osys, ph, omegas, excited=1, meas=2):
+    n = g.num_nodes
+    U0, _ = osys.solve_step(np.zeros(n))
+    osys.assemble(U0, U0, tstep=1e30)
+    J0 = osys.matrix_scipy().tocsc()
+    vol = osys.nodevolumes()
+    b = np.zeros(n)
+    for f, r in zip(g.bfacenodes.T, g.bfaceregions):
+        if r == excited:
+            for k in f:
+                b[k] += 1e30
+    tsys = O.OracleSystem(g, 1, O.physics("linear", D0=1.0),
+                          {(excited, 0): ("dirichlet_legacy", 0.0), (meas, 0): ("dirichlet_legacy", 1.0)}, O.EDGEWISE)
+    T, _ = tsys.solve_step(np.zeros(n))
+    nsys = O.OracleSystem(g, 1, cases.oracle_physics(ph, 1), {}, O.EDGEWISE)
+    nsys.assemble(U0, U0, tstep=math.inf)
+    Jn = nsys.matrix_scipy().tocsc()
+    dstdy = -(Jn.T @ T)
+    dtran = -T * vol
+    Z, UZ = [], []
+    for w in omegas:
+        x = spla.spsolve((J0 + 1j * w * sp.diags(vol)).tocsc(), b.astype(complex))
+        UZ.append(x)
+        Z.append(1.0 / (dstdy @ x + 1j * w * (dtran @ x)))
+    return U0, T, dstdy, dtran, np.array(Z), UZ
+
+
+def _gpu(sys, omegas, excited=1, meas=2):
+    c = vfvm.SolverControl(reltol_linear=1e-13, abstol_linear=0.0, maxiters_linear=5000, precon="ilu0")
+    U0 = vfvm.solve(sys, inival=0.0, control=c)
+    T = vfvm.testfunction(sys, [excited], [meas])
+    ds, dt = vfvm.measurement_derivatives(sys, U0, T, 1)
+    isys = vfvm.ImpedanceSystem(sys, U0, 1, excited)
+    Z = isys.impedances(omegas, ds, dt)
+    return U0, T, ds, dt, Z, isys
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("n", [41, 201])
+def test_impedance_1d_matches_oracle_and_closed_form(n):
+    _print("case impedance 1d", n)
+    g = simplexgrid(linspace(0.0, 1.0, n))
+    sys, osys, ph = _problem(g)
+    U0o, To, dso, dto, Zo, UZo = _oracle(g, osys, ph, OMEGAS)
+    U0, T, ds, dt, Z, isys = _gpu(sys, OMEGAS)
+    _print("iterations", isys.iterations.min(), isys.iterations.max())
+    assert _rel(vfvm.dofs(U0), U0o) <= 1e-9
+    assert _rel(T, To) <= 1e-9
+    assert _rel(ds, dso) <= 1e-9 and _rel(dt, dto) <= 1e-9
+    err = np.abs(Z - Zo) / np.abs(Zo)
+    _print("max rel err vs oracle", err.max())
+    # BiCGStab stops at a relative residual of 1e-12 of the row-scaled system; the error
+    # bound is that times its condition number, about 2 / (h^2 (R + w C)) <= 1e5 here
+    assert err.max() <= ZTOL
+    # one frequency's solution vector
+    uz = isys.solve(OMEGAS[40])
+    assert _rel(uz, UZo[40]) <= ZTOL
+    # the continuous problem: 1 / (2 D z / (e^zL - e^-zL)), z = sqrt(i w C / D + R / D)
+    zz = np.sqrt(1j * OMEGAS + 1.0)
+    exact = 1.0 / (2.0 * zz / (np.exp(zz) - np.exp(-zz)))
+    derr = np.abs(Z - exact) / np.abs(exact)
+    _print("max rel err vs closed form", derr.max())
+    assert derr.max() <= (2e-2 if n == 41 else 1e-3)
+
+
+def test_impedance_unstructured_matches_oracle():
+    _print("case impedance rz2d")
+    g = cases.sg_grid("rz2d.sg")
+    sys, osys, ph = _problem(g, D=0.5, R=2.0)
+    om = OMEGAS[::8]
+    _, _, ds, dt, Z, isys = _gpu(sys, om)
+    _, _, dso, dto, Zo, _ = _oracle(g, osys, ph, om)
+    assert _rel(ds, dso) <= 1e-9 and _rel(dt, dto) <= 1e-9
+    err = np.abs(Z - Zo) / np.abs(Zo)
+    _print("max rel err vs oracle", err.max(), "iterations", isys.iterations.max())
+    assert err.max() <= ZTOL

@@ -77,7 +77,7 @@ InstParams vfvm::make_inst(const vfvm_sys &s) {
   return P;
 }
 
-namespace {
+namespace vfvm {
 
 void upload_inst(vfvm_sys &s) {
   InstParams P = make_inst(s);
@@ -119,6 +119,10 @@ AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, do
   a.nspec = s.nspec;
   return a;
 }
+
+}  // namespace vfvm
+
+namespace {
 
 void build_ilu(vfvm_sys &s) {
   if (s.ilu_built) return;

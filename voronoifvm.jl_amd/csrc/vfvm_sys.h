@@ -62,6 +62,11 @@ struct vfvm_sys {
 
 namespace vfvm {
 InstParams make_inst(const vfvm_sys &s);
+// the instance parameters and BC table of the system to the device
+void upload_inst(vfvm_sys &s);
+// kernel arguments of one assembly of the system
+AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, double *J, double tstep,
+                 double lambda);
 // the cells part of build_setup on the device: fills hs (node graph, factors, node
 // volumes; host copies) and the device node graph, and keeps the incidences for
 // update_geometry; finish_device_geometry completes that after build_boundary

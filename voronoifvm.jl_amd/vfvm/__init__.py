@@ -5,12 +5,14 @@ is done by libvfvm_hip.so (HIP kernels behind the C ABI in include/vfvm_hip.h).
 """
 from ._capi import AssemblyError, ConvergenceError, DeviceArray, LinearSolverError, VFVMError
 from .control import SolverControl, fixed_timesteps
-from .grids import Grid, linspace, simplexgrid
+from .grids import Grid, linspace, read_sg, simplexgrid
 from .physics import (Dirichlet, DriftDiffusionFlux, DriftDiffusionReaction, ExchangeReaction, IdentityStorage,
                       LinearFlux, Neumann, NonlinearDiffusionFlux, Physics, PowerFlux, PowerReaction, Robin,
                       SinhReaction)
 from .system import (Batch, NewtonSolverHistory, System, TransientSolution, boundary_dirichlet, boundary_neumann,
                      boundary_robin, dofs, enable_species, eval_jacobian, eval_rhs, evaluate_residual_and_jacobian,
                      integrate, mass_matrix, solve, unknowns)
+
+from .impedance import ImpedanceSystem, impedance, measurement_derivatives, testfunction
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -75,6 +75,11 @@ SIGNATURES = {
     "vfvm_nodevolumes": (C.c_int, [_vp, _dp]),
     "vfvm_update_grid": (C.c_int, [_vp, _dp]),
     "vfvm_ctx_set_setup": (C.c_int, [_vp, C.c_int]),
+    "vfvm_impedance_create": (C.c_int, [_vp, _dp, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "vfvm_impedance_destroy": (C.c_int, [_vp]),
+    "vfvm_measurement_derivatives": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, _dp]),
+    "vfvm_impedance_sweep": (C.c_int, [_vp, C.c_int64, _dp, _vp, _vp, C.c_double, C.c_int32, _vp, _vp,
+                                       C.POINTER(C.c_int32)]),
     "vfvm_pattern": (C.c_int, [_vp, _lp, _lp]),
     "vfvm_initialize_dirichlet": (C.c_int, [_vp, _dp, C.c_double, C.c_double]),
     "vfvm_assemble": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),

@@ -1,0 +1,96 @@
+"""Small-signal impedance: host mirror of src/vfvm_impedance.jl and the test
+functions of src/vfvm_testfunctions.jl.  The work is done by libvfvm_hip.so
+(vfvm_impedance_*, vfvm_measurement_derivatives): the Jacobian and the
+measurement rows are assembled on the GPU and the frequency-domain systems of a
+whole sweep are solved there together."""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _capi
+from .control import SolverControl
+from .physics import LinearFlux
+from .system import System, dofs, solve
+
+
+def testfunction(sys: System, bc0, bc1, reltol_linear=1e-12):
+    """testfunction(TestFunctionFactory(sys), bc0, bc1) (src/vfvm_testfunctions.jl:46-106):
+    T with -Laplace T = 0, T = 0 on the boundary regions bc0, T = 1 on bc1, homogeneous
+    Neumann elsewhere; nnodes values."""
+    tf = System(sys.grid, flux=LinearFlux(1.0), species=[1], is_linear=True, context=sys.ctx)
+    for r in bc1:
+        tf.boundary_dirichlet(1, r, 1.0)
+    for r in bc0:
+        tf.boundary_dirichlet(1, r, 0.0)
+    c = SolverControl(reltol_linear=reltol_linear, abstol_linear=0.0, maxiters_linear=20000, precon="ilu0")
+    T = solve(tf, inival=0.0, control=c)
+    return np.ascontiguousarray(dofs(T), dtype=np.float64)
+
+
+def measurement_derivatives(sys: System, U0, T, spec):
+    """measurement_derivative (src/vfvm_impedance.jl:235-252) of
+    U -> -integrate_stdy(sys, T, U)[spec] and U -> -integrate_tran(sys, T, U)[spec] at U0:
+    two dof vectors (dmeas_stdy, dmeas_tran)."""
+    ndof, _, _ = sys.sizes()
+    u = np.ascontiguousarray(dofs(U0) if np.ndim(U0) == 2 else np.asarray(U0, dtype=np.float64).reshape(-1))
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    ds, dt = np.zeros(ndof), np.zeros(ndof)
+    _capi.check(_capi.lib().vfvm_measurement_derivatives(sys.handle(), _capi.dptr(u), _capi.dptr(T), int(spec),
+                                                         _capi.dptr(ds), _capi.dptr(dt)), sys.ctx.handle)
+    return ds, dt
+
+
+class ImpedanceSystem:
+    """ImpedanceSystem(system, U0, excited_spec, excited_bc) (src/vfvm_impedance.jl:159-186)."""
+
+    def __init__(self, sys: System, U0, excited_spec, excited_bc):
+        self.sys = sys
+        self.U0 = U0
+        u = np.ascontiguousarray(dofs(U0) if np.ndim(U0) == 2 else np.asarray(U0, dtype=np.float64).reshape(-1))
+        h = C.c_void_p()
+        _capi.check(_capi.lib().vfvm_impedance_create(sys.handle(), _capi.dptr(u), int(excited_spec),
+                                                      int(excited_bc), C.byref(h)), sys.ctx.handle)
+        self._h = h
+        self.ndof = u.size
+        self.iterations = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value and _capi._LIB is not None:
+            _capi._LIB.vfvm_impedance_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _sweep(self, omegas, dstdy=None, dtran=None, reltol=1e-12, maxiters=20000, want_uz=False):
+        om = np.ascontiguousarray(np.atleast_1d(np.asarray(omegas, dtype=np.float64)))
+        nw = om.size
+        uz = np.zeros((nw, self.ndof), dtype=np.complex128) if want_uz else None
+        z = np.zeros(nw, dtype=np.complex128) if dstdy is not None else None
+        its = np.zeros(nw, dtype=np.int32)
+        ds = None if dstdy is None else np.ascontiguousarray(dstdy, dtype=np.float64)
+        dt = None if dtran is None else np.ascontiguousarray(dtran, dtype=np.float64)
+        vp = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        _capi.check(_capi.lib().vfvm_impedance_sweep(self._h, nw, _capi.dptr(om), vp(ds), vp(dt), float(reltol),
+                                                     int(maxiters), vp(uz), vp(z),
+                                                     its.ctypes.data_as(C.POINTER(C.c_int32))),
+                    self.sys.ctx.handle)
+        self.iterations = its
+        return uz, z
+
+    def solve(self, omega, reltol=1e-12, maxiters=20000):
+        """solve!(UZ, isys, w) (src/vfvm_impedance.jl:205-232): the complex dof vector UZ."""
+        uz, _ = self._sweep([omega], reltol=reltol, maxiters=maxiters, want_uz=True)
+        return uz[0]
+
+    def impedances(self, omegas, dmeas_stdy, dmeas_tran, reltol=1e-12, maxiters=20000):
+        """impedance(isys, w, U0, dstdy, dtran) (src/vfvm_impedance.jl:268-288) for every w
+        of the sweep, solved together on the GPU."""
+        _, z = self._sweep(omegas, dmeas_stdy, dmeas_tran, reltol=reltol, maxiters=maxiters)
+        return z
+
+
+def impedance(isys: ImpedanceSystem, omega, U0, dmeas_stdy, dmeas_tran):
+    """impedance(impedance_system, w, U0, dmeas_stdy, dmeas_tran) for one frequency."""
+    return isys.impedances([omega], dmeas_stdy, dmeas_tran)[0]
