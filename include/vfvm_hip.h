@@ -226,12 +226,27 @@ int vfvm_eval_jacobian_device(vfvm_sys *sys, const double *du, double time, doub
 typedef struct vfvm_impedance vfvm_impedance;
 int vfvm_impedance_create(vfvm_sys *sys, const double *U0, int excited_spec, int excited_bc,
                           vfvm_impedance **out);
+/* ImpedanceSystem(system, U0; lambda0) for a system whose parameter is lambda (lines
+ * 49-112): the excitation is F = -dF/dlambda (vfvm_assemble_dudp). */
+int vfvm_impedance_create_param(vfvm_sys *sys, const double *U0, double lambda0, vfvm_impedance **out);
 int vfvm_impedance_destroy(vfvm_impedance *imp);
 int vfvm_measurement_derivatives(vfvm_sys *sys, const double *U0, const double *T, int spec,
                                  double *dstdy, double *dtran);
 int vfvm_impedance_sweep(vfvm_impedance *imp, int64_t nomega, const double *omega, const double *dstdy,
                          const double *dtran, double reltol, int32_t maxiters, double *UZ, double *Z,
                          int32_t *iters);
+
+/* Parameter derivatives.  The registry's system parameter is lambda: it enters the
+ * boundary values value + value_lambda * lambda and, with p[12] set, the source
+ * lambda * f -- the registry's form of `nparams = 1, params = [lambda]`.
+ * vfvm_assemble_dudp: state.dudp[1] = dF/dlambda (src/vfvm_assembly.jl:116-120,
+ *   194-198, 398-405), ndof values (exact; independent of U and lambda here).
+ * vfvm_parameter_derivative: the sensitivity du/dlambda = -J(U)^-1 dF/dlambda of a
+ *   stationary solution U at lambda (what Example430_ParameterDerivativesStationary.jl
+ *   obtains by dual numbers through the solve), with the linear solver of `control`. */
+int vfvm_assemble_dudp(vfvm_sys *sys, double *dudp);
+int vfvm_parameter_derivative(vfvm_sys *sys, const double *U, double lambda, const vfvm_control_t *control,
+                              double *dudl);
 
 /* Linear solve A x = b on the currently assembled Jacobian (after vfvm_assemble or inside
  * vfvm_newton): _solve_linear!, src/vfvm_linsolve.jl:6-61. */

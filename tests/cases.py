@@ -21,11 +21,11 @@ def oracle_physics(ph, nspec):
     return oph
 
 
-def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None, oracle=True):
+def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None, oracle=True, embed_source=False):
     """phys_kw: dict(flux=..., reaction=..., storage=...) of vfvm registry objects.
     bcs: vfvm.Dirichlet / vfvm.Robin entries; legacy: (species, region, value) Dirichlet.
     With oracle=False only the device system is built (second item None)."""
-    ph = vfvm.Physics(bcondition=list(bcs), source=source, **phys_kw)
+    ph = vfvm.Physics(bcondition=list(bcs), source=source, embed_source=embed_source, **phys_kw)
     sys = vfvm.System(grid, ph, species=list(range(1, nspec + 1)))
     for (sp, reg, val) in legacy:
         sys.boundary_dirichlet(sp, reg, val)

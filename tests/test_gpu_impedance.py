@@ -112,3 +112,78 @@ def test_impedance_unstructured_matches_oracle():
     err = np.abs(Z - Zo) / np.abs(Zo)
     _print("max rel err vs oracle", err.max(), "iterations", isys.iterations.max())
     assert err.max() <= ZTOL
+
+
+def test_dudp_matches_oracle_difference():
+    """dF/dlambda (bias value_lambda of drift-diffusion contacts; an embedded source)
+    against the oracle's residual difference (F is affine in lambda)."""
+    _print("case dudp")
+    sys1, osys1 = cases.drift_diffusion_2d(9, lambda2=1e-2)
+    X = linspace(0.0, 1.0, 17)
+    g = simplexgrid(X, X)
+    bcs = [vfvm.Dirichlet(species=1, region=1, value=0.0, value_lambda=0.5),
+           vfvm.Dirichlet(species=1, region=3, value=1.0, value_lambda=-2.0)]
+    src = np.full((1, g.num_nodes), 3.0, order="F")
+    sys2, osys2 = cases.pair(g, 1, dict(flux=vfvm.LinearFlux(0.1), reaction=vfvm.SinhReaction()), bcs, source=src,
+                             embed_source=True)
+    for sys, osys in ((sys1, osys1), (sys2, osys2)):
+        n = sys.grid.num_nodes * sys.num_species
+        u = np.random.default_rng(2).uniform(0.3, 1.0, n)
+        F1 = osys.assemble(u, u, tstep=math.inf, embed=0.2)
+        F2 = osys.assemble(u, u, tstep=math.inf, embed=0.7)
+        want = (F2 - F1) / 0.5
+        got = vfvm.assemble_dudp(sys)
+        pen = np.abs(want) > 1e20
+        for mask in (pen, ~pen):
+            if mask.any():
+                sc = np.abs(want[mask]).max()
+                assert np.abs(got[mask] - want[mask]).max() <= 1e-9 * sc
+
+
+def test_parameter_excited_impedance_equals_boundary_excited():
+    """Example151 (boundary value = the parameter, ImpedanceSystem(sys, U0; lambda0))
+    and Example150 (legacy Dirichlet excitation) give the same impedance."""
+    _print("case impedance parameter excitation")
+    g = simplexgrid(linspace(0.0, 1.0, 101))
+    sysA, _, _ = _problem(g)
+    _, _, dsA, dtA, ZA, _ = _gpu(sysA, OMEGAS)
+    phys = dict(flux=vfvm.LinearFlux(1.0), reaction=vfvm.PowerReaction(m=1, k=1.0), storage=vfvm.IdentityStorage())
+    bcs = [vfvm.Dirichlet(species=1, region=1, value=1.0, value_lambda=1.0),
+           vfvm.Dirichlet(species=1, region=2, value=0.0)]
+    sysB, _ = cases.pair(g, 1, phys, bcs, oracle=False)
+    c = vfvm.SolverControl(reltol_linear=1e-13, abstol_linear=0.0, maxiters_linear=5000, precon="ilu0")
+    U0 = vfvm.solve(sysB, inival=0.0, control=c)
+    T = vfvm.testfunction(sysB, [1], [2])
+    ds, dt = vfvm.measurement_derivatives(sysB, U0, T, 1)
+    isys = vfvm.ImpedanceSystem(sysB, U0, lambda0=0.0)
+    ZB = isys.impedances(OMEGAS, ds, dt)
+    err = np.abs(ZB - ZA) / np.abs(ZA)
+    _print("max rel diff", err.max())
+    assert err.max() <= 1e-10
+
+
+def test_parameter_derivative_matches_central_difference():
+    """du/dlambda of a stationary nonlinear Poisson solution (boundary value on region 2
+    = 1 + 0.5 lambda) against the central difference of solutions at lambda = +-h."""
+    _print("case parameter derivative")
+    X = linspace(0.0, 1.0, 33)
+    g = simplexgrid(X, X)
+    phys = dict(flux=vfvm.LinearFlux(0.5), reaction=vfvm.SinhReaction())
+    src = np.full((1, g.num_nodes), 1.0, order="F")
+    c = vfvm.SolverControl(reltol_linear=1e-13, abstol_linear=0.0, maxiters_linear=5000, precon="ilu0")
+
+    def build(v):
+        bcs = [vfvm.Dirichlet(species=1, region=4, value=0.0),
+               vfvm.Dirichlet(species=1, region=2, value=v, value_lambda=0.5)]
+        return cases.pair(g, 1, phys, bcs, source=src, oracle=False)[0]
+
+    h = 1e-3
+    s0, sp_, sm = build(1.0), build(1.0 + 0.5 * h), build(1.0 - 0.5 * h)
+    U0 = vfvm.solve(s0, inival=0.0, control=c)
+    Up = vfvm.solve(sp_, inival=0.0, control=c)
+    Um = vfvm.solve(sm, inival=0.0, control=c)
+    fd = (vfvm.dofs(Up) - vfvm.dofs(Um)) / (2 * h)
+    d = vfvm.parameter_derivative(s0, U0, 0.0, c)
+    err = np.abs(d - fd).max() / np.abs(fd).max()
+    _print("sensitivity vs central difference", err)
+    assert err <= 1e-5

@@ -467,6 +467,41 @@ __global__ void k_init_dirichlet(int64_t nb, const int32_t *bnodes, const int32_
   }
 }
 
+// dF/dlambda of the assembly for the system's parameter lambda (the reference's
+// state.dudp for params = [lambda], src/vfvm_assembly.jl:116-120, 194-198, 398-405):
+// lambda enters the boundary values value + value_lambda * lambda and, with p[12] set,
+// the source embedparam * f -- both linearly -- so the derivative is exact and does
+// not depend on U.  One thread per node, the terms of node_terms differentiated.
+__global__ void k_dudp(int64_t nnodes, int nspec, const int32_t *__restrict__ bn_ptr,
+                       const int32_t *__restrict__ bn_reg, const double *__restrict__ bn_fac,
+                       const double *__restrict__ nodevol, const double *__restrict__ src,
+                       const InstParams *__restrict__ inst, double *__restrict__ out) {
+  const int64_t K = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (K >= nnodes) return;
+  const InstParams *P = inst;
+  const double vol = nodevol[K];
+  for (int i = 0; i < nspec; i++) {
+    double d = 0.0;
+    if (P->p[12] != 0.0 && src) d = -(vol * src[K * nspec + i]);
+    for (int32_t b = bn_ptr[K]; b < bn_ptr[K + 1]; b++) {
+      const int reg = bn_reg[b];
+      const double fac = bn_fac[b];
+      if (reg < 1 || reg > MAXBREG) continue;
+      const BCDev &bc = P->bc[reg - 1][i];
+      double penalty = DIRICHLET_PENALTY;
+      if (P->has_legacy) {
+        penalty = DIRICHLET_PENALTY / fac;
+        if (bc.kind == VFVM_BC_DIRICHLET_LEGACY) d += DIRICHLET_PENALTY * -bc.value_lambda;
+      }
+      if (bc.kind == VFVM_BC_DIRICHLET)
+        d += fac * (penalty * -bc.value_lambda);
+      else if (bc.kind == VFVM_BC_ROBIN)
+        d += fac * -bc.value_lambda;
+    }
+    out[K * nspec + i] = d;
+  }
+}
+
 // max of |F| over rows that carry no Dirichlet penalty, per instance: non-negative
 // doubles combined with an integer atomicMax on their bits, one atomic per wave when
 // the wave's nodes belong to one instance (instances are contiguous node ranges)
@@ -520,6 +555,14 @@ void launch_init_dirichlet(int64_t nb, const int32_t *bnodes, const int32_t *bn_
   if (nb == 0) return;
   hipLaunchKernelGGL(k_init_dirichlet, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, nb, bnodes, bn_ptr, bn_reg,
                      node_inst, inst, nspec, lambda, U);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_dudp(int64_t nnodes, int nspec, const int32_t *bn_ptr, const int32_t *bn_reg, const double *bn_fac,
+                 const double *nodevol, const double *src, const InstParams *inst, double *out, hipStream_t s) {
+  if (nnodes == 0) return;
+  hipLaunchKernelGGL(k_dudp, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, s, nnodes, nspec, bn_ptr, bn_reg,
+                     bn_fac, nodevol, src, inst, out);
   VFVM_HIP_CHECK(hipGetLastError());
 }
 

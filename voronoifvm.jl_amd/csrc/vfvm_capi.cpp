@@ -861,6 +861,52 @@ int vfvm_linsolve(vfvm_sys *s, const double *b, double *x, const vfvm_control_t 
   });
 }
 
+int vfvm_assemble_dudp(vfvm_sys *s, double *dudp) {
+  if (!s || !dudp) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    hipStream_t st = s->ctx->stream;
+    launch_dudp(s->nnodes, s->nspec, s->d_bn_ptr.p, s->d_bn_reg.p, s->d_bn_fac.p, s->d_nodevol.p,
+                s->has_src ? s->d_src.p : nullptr, s->d_inst.p, s->d_tmp.p, st);
+    s->d_tmp.download(dudp, (size_t)s->ndof, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return VFVM_OK;
+  });
+}
+
+int vfvm_parameter_derivative(vfvm_sys *s, const double *U, double lambda, const vfvm_control_t *control,
+                              double *dudl) {
+  if (!s || !U || !dudl) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    vfvm_control_t c;
+    if (control)
+      c = *control;
+    else
+      vfvm_default_control(&c);
+    check_control(c, *s);
+    s->lin.fp32_off = false;
+    if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
+    hipStream_t st = s->ctx->stream;
+    // J(U) of the stationary problem, then J x = dF/dlambda, du/dlambda = -x
+    s->d_U.upload(U, (size_t)s->ndof, st);
+    launch_fill_i32(1, s->d_flags.p, 0, st);
+    AsmArgs a = asm_args(*s, s->d_U.p, s->d_U.p, s->d_tmp.p, s->d_J.p, INFINITY, lambda);
+    launch_assemble(a, s->asm_group, st);
+    launch_dudp(s->nnodes, s->nspec, s->d_bn_ptr.p, s->d_bn_reg.p, s->d_bn_fac.p, s->d_nodevol.p,
+                s->has_src ? s->d_src.p : nullptr, s->d_inst.p, s->d_F.p, st);
+    s->lin.prepare(s->d_J.p, c, true);
+    vfvm_linstats_t ls{};
+    int rc = s->lin.solve(s->d_J.p, s->d_F.p, s->d_dU.p, c, &ls);
+    launch_negate(s->ndof, s->d_dU.p, st);
+    s->d_dU.download(dudl, (size_t)s->ndof, st);
+    int32_t fl = 0;
+    VFVM_HIP_CHECK(hipMemcpyAsync(&fl, s->d_flags.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    if (fl & 1) throw Error(VFVM_ERR_ASSEMBLY, "trying to assemble NaN");
+    if (rc) s->ctx->err = "linear solver did not reach its tolerance";
+    return rc;
+  });
+}
+
 int vfvm_newton(vfvm_sys *s, double *U, const double *Uold, double time, double tstep, double lambda,
                 const vfvm_control_t *control, vfvm_history_t *hist) {
   if (!s || !U || !Uold) return VFVM_ERR_ARG;

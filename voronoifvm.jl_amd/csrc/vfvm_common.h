@@ -181,6 +181,9 @@ void launch_init_dirichlet(int64_t nbnodes, const int32_t *bnodes, const int32_t
                            const int32_t *node_inst, const InstParams *inst, int nspec, double lambda, double *U,
                            hipStream_t s);
 // ||F||_inf over rows without a Dirichlet penalty, per instance (inst may be null -> 1 instance)
+// dF/dlambda (exact: lambda enters linearly), one value per dof
+void launch_dudp(int64_t nnodes, int nspec, const int32_t *bn_ptr, const int32_t *bn_reg, const double *bn_fac,
+                 const double *nodevol, const double *src, const InstParams *inst, double *out, hipStream_t s);
 void launch_penalty_free_norm(int64_t nnodes, int nspec, const double *F, const int32_t *bn_ptr,
                               const int32_t *bn_reg, const int32_t *node_inst, const InstParams *inst,
                               double *out_partial, int64_t nparts, hipStream_t s);

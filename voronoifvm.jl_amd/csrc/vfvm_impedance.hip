@@ -303,11 +303,13 @@ void check_dofs(const vfvm_sys &s, int spec) {
 
 extern "C" {
 
+int vfvm_impedance_destroy(vfvm_impedance *z);
+
 int vfvm_impedance_create(vfvm_sys *s, const double *U0, int excited_spec, int excited_bc, vfvm_impedance **out) {
   if (!s || !U0 || !out) return VFVM_ERR_ARG;
   try {
     check_dofs(*s, excited_spec);
-    if (excited_bc < 1 || excited_bc > MAXBREG) throw Error(VFVM_ERR_ARG, "boundary region out of range");
+    if (excited_bc < 1 || excited_bc > MAXBREG + 1) throw Error(VFVM_ERR_ARG, "boundary region out of range");
     VFVM_HIP_CHECK(hipSetDevice(s->ctx->device));
     std::unique_ptr<vfvm_impedance> z(new vfvm_impedance());
     z->sys = s;
@@ -355,6 +357,29 @@ int vfvm_impedance_create(vfvm_sys *s, const double *U0, int excited_spec, int e
   } catch (const std::exception &e) {
     s->ctx->err = e.what();
     return VFVM_ERR_HIP;
+  }
+}
+
+int vfvm_impedance_create_param(vfvm_sys *s, const double *U0, double lambda0, vfvm_impedance **out) {
+  if (!s || !U0 || !out) return VFVM_ERR_ARG;
+  // as the boundary-excited form with no boundary excitation, then F = -dF/dlambda
+  // (src/vfvm_impedance.jl:49-112: params = [lambda0], F .-= dudp[1])
+  int rc = vfvm_impedance_create(s, U0, 1, MAXBREG + 1, out);
+  (void)lambda0;  // dF/dlambda does not depend on lambda for the registry
+  if (rc) return rc;
+  try {
+    vfvm_impedance *z = *out;
+    hipStream_t st = s->ctx->stream;
+    launch_dudp(s->nnodes, s->nspec, s->d_bn_ptr.p, s->d_bn_reg.p, s->d_bn_fac.p, s->d_nodevol.p,
+                s->has_src ? s->d_src.p : nullptr, s->d_inst.p, z->b.p, st);
+    launch_negate(s->ndof, z->b.p, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return VFVM_OK;
+  } catch (const Error &e) {
+    vfvm_impedance_destroy(*out);
+    *out = nullptr;
+    s->ctx->err = e.what();
+    return e.code;
   }
 }
 

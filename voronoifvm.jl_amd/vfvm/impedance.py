@@ -41,16 +41,46 @@ def measurement_derivatives(sys: System, U0, T, spec):
     return ds, dt
 
 
-class ImpedanceSystem:
-    """ImpedanceSystem(system, U0, excited_spec, excited_bc) (src/vfvm_impedance.jl:159-186)."""
+def _dofvec(U):
+    return np.ascontiguousarray(dofs(U) if np.ndim(U) == 2 else np.asarray(U, dtype=np.float64).reshape(-1))
 
-    def __init__(self, sys: System, U0, excited_spec, excited_bc):
+
+def assemble_dudp(sys: System):
+    """dF/dlambda for the system's parameter lambda (the reference's state.dudp[1] with
+    params = [lambda]; src/vfvm_assembly.jl:116-120, 194-198, 398-405): a dof vector."""
+    ndof, _, _ = sys.sizes()
+    d = np.zeros(ndof)
+    _capi.check(_capi.lib().vfvm_assemble_dudp(sys.handle(), _capi.dptr(d)), sys.ctx.handle)
+    return d
+
+
+def parameter_derivative(sys: System, U, lam=0.0, control=None):
+    """du/dlambda = -J(U)^-1 dF/dlambda at a stationary solution U for parameter lam."""
+    c = SolverControl() if control is None else control
+    cc = c.to_c(sys.is_linear)
+    u = _dofvec(U)
+    out = np.zeros(u.size)
+    _capi.check(_capi.lib().vfvm_parameter_derivative(sys.handle(), _capi.dptr(u), float(lam), C.byref(cc),
+                                                      _capi.dptr(out)), sys.ctx.handle)
+    return out
+
+
+class ImpedanceSystem:
+    """ImpedanceSystem(system, U0, excited_spec, excited_bc) (src/vfvm_impedance.jl:159-186),
+    or, without excited_spec / excited_bc, ImpedanceSystem(system, U0; lambda0) (lines
+    49-152) excited through the system parameter lambda (F = -dF/dlambda)."""
+
+    def __init__(self, sys: System, U0, excited_spec=None, excited_bc=None, lambda0=0.0):
         self.sys = sys
         self.U0 = U0
-        u = np.ascontiguousarray(dofs(U0) if np.ndim(U0) == 2 else np.asarray(U0, dtype=np.float64).reshape(-1))
+        u = _dofvec(U0)
         h = C.c_void_p()
-        _capi.check(_capi.lib().vfvm_impedance_create(sys.handle(), _capi.dptr(u), int(excited_spec),
-                                                      int(excited_bc), C.byref(h)), sys.ctx.handle)
+        if excited_spec is None and excited_bc is None:
+            rc = _capi.lib().vfvm_impedance_create_param(sys.handle(), _capi.dptr(u), float(lambda0), C.byref(h))
+        else:
+            rc = _capi.lib().vfvm_impedance_create(sys.handle(), _capi.dptr(u), int(excited_spec), int(excited_bc),
+                                                   C.byref(h))
+        _capi.check(rc, sys.ctx.handle)
         self._h = h
         self.ndof = u.size
         self.iterations = None
