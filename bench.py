@@ -176,7 +176,7 @@ def run_cfg1(args):
         s_host.handle()
         create_host_s = time.perf_counter() - t1
     finally:
-        s.ctx.set_setup(True)
+        s.ctx.set_setup("auto")
     del s_host
     log(f"system creation: {create_s:.2f} s with the device setup, {create_host_s:.2f} s with the host setup")
     # update_grid on the device: every form factor, node volume and boundary factor

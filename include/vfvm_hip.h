@@ -171,10 +171,11 @@ int vfvm_nodevolumes(vfvm_sys *sys, double *vol);
 int vfvm_update_grid(vfvm_sys *sys, const double *coord);
 /* Where vfvm_system_create builds the node graph, edge factors and node volumes
  * (the cells part of _complete! / update_grid_edgewise!, src/vfvm_system.jl:690-755):
- * 1 (default) on the GPU, 0 on the host.  Both give bit for bit the same system; the
- * boundary items, the dof pattern and the solver structures are built on the host
+ * 0 on the host, 1 on the GPU, 2 (default) on the GPU for grids of at least 2^18
+ * cell-node incidences and on the host below.  All give bit for bit the same system;
+ * the boundary items, the dof pattern and the solver structures are built on the host
  * either way.  No counterpart in the reference (a choice of where setup runs). */
-int vfvm_ctx_set_setup(vfvm_ctx *ctx, int on_device);
+int vfvm_ctx_set_setup(vfvm_ctx *ctx, int mode);
 
 /* Jacobian pattern as CSC, 1-based, rows sorted: what SparseMatrixCSC(state.matrix)
  * exposes (src/vfvm_linsolve.jl:3-4, src/vfvm_state.jl:136).  colptr has ndof+1 entries. */

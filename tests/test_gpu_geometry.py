@@ -153,7 +153,7 @@ def test_device_setup_equals_host_setup(case):
             U = np.random.default_rng(6).uniform(-1, 1, (1, g.num_nodes))
             out.append((sys, sys.pattern(), sys.sizes(), sys.nodevolumes(), _asm(sys, U), U))
     finally:
-        ctx.set_setup(True)
+        ctx.set_setup("auto")
     (sh, ph, zh, vh, ah, U), (sd, pd, zd, vd, ad, _) = out
     assert all(np.array_equal(a, b) for a, b in zip(ph, pd)) and zh == zd
     assert np.array_equal(vh, vd)

@@ -523,7 +523,8 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     VFVM_HIP_CHECK(hipHostMalloc((void **)&s->h_pinned, 8 * sizeof(double), hipHostMallocDefault));
     s->ncells = ncells;
     s->nbfaces = nbfaces;
-    const bool dev_setup = ctx->setup_on_device;
+    const bool dev_setup =
+        ctx->setup_mode == 1 || (ctx->setup_mode == 2 && ncells * (dim + 1) >= SETUP_DEVICE_MIN_INCIDENCES);
     if (dev_setup) {
       // cells part on the device (form factors, node graph), boundary items on the host
       build_node_graph_device(*s, coord, cellnodes, ncells);
@@ -667,9 +668,9 @@ int vfvm_nodevolumes(vfvm_sys *s, double *vol) {
   });
 }
 
-int vfvm_ctx_set_setup(vfvm_ctx *ctx, int on_device) {
-  if (!ctx) return VFVM_ERR_ARG;
-  ctx->setup_on_device = on_device != 0;
+int vfvm_ctx_set_setup(vfvm_ctx *ctx, int mode) {
+  if (!ctx || mode < 0 || mode > 2) return VFVM_ERR_ARG;
+  ctx->setup_mode = mode;
   return VFVM_OK;
 }
 

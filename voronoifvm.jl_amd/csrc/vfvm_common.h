@@ -18,6 +18,7 @@ namespace vfvm {
 constexpr int MAXSPEC = 3;
 constexpr int MAXBREG = 8;
 constexpr double DIRICHLET_PENALTY = 1.0e30;  // src/vfvm_system.jl:329
+constexpr int64_t SETUP_DEVICE_MIN_INCIDENCES = int64_t(1) << 18;
 
 struct Error : std::runtime_error {
   int code;
@@ -312,7 +313,10 @@ struct KernelProbe {
 
 struct vfvm_ctx {
   int device = 0;
-  bool setup_on_device = true;  // node graph and form factors of new systems built by the GPU
+  // where new systems build their node graph and form factors: 0 host, 1 GPU, 2 the GPU
+  // from SETUP_DEVICE_MIN_INCIDENCES cell-node incidences on (below, launch and
+  // synchronisation costs outweigh the host loop)
+  int setup_mode = 2;
   hipStream_t stream = nullptr;
   std::string err;
   vfvm::KernelProbe probe;

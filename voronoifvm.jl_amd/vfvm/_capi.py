@@ -210,10 +210,12 @@ class Context:
             _LIB.vfvm_ctx_destroy(self.handle)
             self.handle = None
 
-    def set_setup(self, on_device: bool):
-        """Where new systems build their node graph and form factors: the GPU (default)
-        or the host; the two give the same system bit for bit."""
-        check(lib().vfvm_ctx_set_setup(self.handle, int(bool(on_device))), self.handle)
+    def set_setup(self, where):
+        """Where new systems build their node graph and form factors: True (the GPU),
+        False (the host) or "auto" (default: the GPU for grids of at least 2^18 cell-node
+        incidences); all give the same system bit for bit."""
+        mode = 2 if where == "auto" else int(bool(where))
+        check(lib().vfvm_ctx_set_setup(self.handle, mode), self.handle)
 
     def probe(self, *kernels: int, max_launches: int = 50000):
         """Context manager timing every launch of the given kernel classes (KERNEL_*)
