@@ -749,7 +749,77 @@ def cpu_copy(args):
     return None
 
 
-WORKLOADS = {"copy": (run_copy, cpu_copy), "cfg0": (run_cfg0, cpu_cfg0), "cfg1": (run_cfg1, cpu_cfg1), "cfg2": (run_cfg2, cpu_cfg2), "cfg3": (run_cfg3, cpu_cfg3), "cfg4": (run_cfg4, cpu_cfg4)}
+# --------------------------------------------------------------------------- impedance
+IMP_N = 257
+IMP_OMEGAS = 1e-3 * 1.1 ** np.arange(113)  # Example150's sweep: w from 1e-3 by factors 1.1 below 50
+
+
+def build_impedance(n, oracle=False):
+    """Example150's problem on the unit square: C u_t - div(D grad u) + R u = 0 (C = D = R = 1),
+    u = 1 on the west side (region 4, excited), u = 0 on the east side (region 2, measured)."""
+    import cases
+    X = vfvm.linspace(0.0, 1.0, n)
+    g = vfvm.simplexgrid(X, X)
+    phys = dict(flux=vfvm.LinearFlux(1.0), reaction=vfvm.PowerReaction(m=1, k=1.0), storage=vfvm.IdentityStorage())
+    s, o = cases.pair(g, 1, phys, legacy=[(1, 4, 1.0), (1, 2, 0.0)], oracle=oracle)
+    return g, s, o, vfvm.Physics(**phys)
+
+
+def run_impedance(args):
+    """Impedance spectrum of Example150's problem on a 257 x 257 grid over Example150's
+    113 frequencies: steady state, test function and measurement rows, then the whole
+    sweep solved as one batch (complex BiCGStab, relative residual 1e-10)."""
+    n = IMP_N
+    g, s, _, _ = build_impedance(n)
+    c = vfvm.SolverControl(reltol_linear=1e-12, abstol_linear=0.0, maxiters_linear=20000, precon="ilu0")
+    U0 = vfvm.solve(s, inival=0.0, control=c)
+    T = vfvm.testfunction(s, [4], [2])
+    ds, dt = vfvm.measurement_derivatives(s, U0, T, 1)
+    isys = vfvm.ImpedanceSystem(s, U0, 1, 4)
+    for _ in range(max(args.warmup, 0)):
+        isys.impedances(IMP_OMEGAS, ds, dt, reltol=1e-10)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        Z = isys.impedances(IMP_OMEGAS, ds, dt, reltol=1e-10)
+    el = (time.perf_counter() - t0) / args.steps
+    it = isys.iterations
+    log(f"impedance: {g.num_nodes} nodes, {IMP_OMEGAS.size} frequencies in {el:.3f} s per sweep -> "
+        f"{IMP_OMEGAS.size / el:.1f} frequencies/s, BiCGStab iterations {it.min()}..{it.max()}")
+    return dict(value=IMP_OMEGAS.size / el, unit="frequencies/s", ms_per_step=1e3 * el,
+                metric="impedance spectrum: frequencies solved per second (one sweep of 113 as a batch)",
+                iterations_min=int(it.min()), iterations_max=int(it.max()),
+                Z_first=[float(Z[0].real), float(Z[0].imag)], Z_last=[float(Z[-1].real), float(Z[-1].imag)],
+                config=dict(workload=f"impedance: Example150 problem on a {n}x{n} grid, 113 frequencies "
+                                     f"1e-3..50, reltol 1e-10", nodes=g.num_nodes), dtype="c128")
+
+
+def cpu_impedance(args):
+    """The oracle's form of the same sweep (scipy's sparse LU per frequency, as the
+    reference's solve!) on a bounded sample: 8 of the 113 frequencies."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    n = IMP_N
+    g, _, osys, _ = build_impedance(n, oracle=True)
+    U0, _ = osys.solve_step(np.zeros(g.num_nodes))
+    osys.assemble(U0, U0, tstep=1e30)
+    J0 = osys.matrix_scipy().tocsc()
+    M = sp.diags(osys.nodevolumes())
+    b = np.zeros(g.num_nodes, dtype=complex)
+    for f, r in zip(g.bfacenodes.T, g.bfaceregions):
+        if r == 4:
+            for k in f:
+                b[k] += 1e30
+    om = IMP_OMEGAS[::14][:8]
+    t0 = time.perf_counter()
+    for w in om:
+        spla.spsolve((J0 + 1j * w * M).tocsc(), b)
+    el = time.perf_counter() - t0
+    return dict(value=om.size / el, unit="frequencies/s", cores=1, kind="port",
+                sample=f"{om.size} of the 113 frequencies on the {n}x{n} grid, scipy sparse LU per frequency "
+                       f"(the reference's solve! factorises per frequency), {el:.2f} s")
+
+
+WORKLOADS = {"impedance": (run_impedance, cpu_impedance), "copy": (run_copy, cpu_copy), "cfg0": (run_cfg0, cpu_cfg0), "cfg1": (run_cfg1, cpu_cfg1), "cfg2": (run_cfg2, cpu_cfg2), "cfg3": (run_cfg3, cpu_cfg3), "cfg4": (run_cfg4, cpu_cfg4)}
 
 
 def main():

@@ -1,10 +1,11 @@
 """Builds the same problem twice: as a vfvm.System (HIP path) and as an
-OracleSystem (CPU restatement), from one description."""
+OracleSystem (CPU restatement), from one description.  The oracle is imported
+only when an oracle system is asked for (oracle=True), so that the GPU legs of
+the benchmark can build their problems here without loading it."""
 import math
 
 import numpy as np
 
-import oracle as O
 import vfvm
 from vfvm.grids import linspace, simplexgrid
 
@@ -13,6 +14,7 @@ ORACLE_BC = {vfvm.Dirichlet: "dirichlet", vfvm.Robin: "robin"}
 
 def oracle_physics(ph, nspec):
     """The oracle's parameter block from the same registry objects."""
+    import oracle as O  # imported only where an oracle system is built
     flux, react, stor, p = ph.param_block(nspec)
     oph = O.Physics()
     oph.flux, oph.reaction, oph.storage = flux, react, stor
@@ -41,6 +43,7 @@ def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None, oracle=True, embe
     for (sp, reg, val) in legacy:
         obcs[(reg, sp - 1)] = ("dirichlet_legacy", val)
     osrc = None if source is None else np.asarray(source, dtype=float).reshape(-1, order="F")
+    import oracle as O
     osys = O.OracleSystem(grid, nspec, oph, obcs, O.EDGEWISE, osrc)
     return sys, osys
 

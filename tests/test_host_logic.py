@@ -29,3 +29,18 @@ def test_fixed_timesteps():
     c = vfvm.fixed_timesteps(vfvm.SolverControl(), 0.25, grow=1.5)
     assert (c.Δt, c.Δt_min, c.Δt_max, c.Δt_grow) == (0.25, 0.25, 0.25, 1.5)
     assert c.Δu_opt == sys.float_info.max
+
+
+def test_product_and_gpu_problem_builders_do_not_load_the_oracle():
+    """The package and the problem builders the benchmark's GPU legs use must not import
+    the oracle (it is the checker, loaded only by tests and the CPU baselines)."""
+    import subprocess
+    import sys as _sys
+    root = __import__("os").path.dirname(__import__("os").path.dirname(__import__("os").path.abspath(__file__)))
+    code = ("import sys; sys.path[:0] = ['tests', 'voronoifvm.jl_amd', 'oracle']\n"
+            "import vfvm, vfvm.surrogate, vfvm.impedance, cases\n"
+            "cases.nonlinear_poisson_2d(9, oracle=False)\n"
+            "print('oracle' in sys.modules, 'mpnn_oracle' in sys.modules)\n")
+    out = subprocess.run([_sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "False False"
