@@ -74,44 +74,65 @@ __device__ __forceinline__ void block_partials(double2 (&v)[ND], double2 *out) {
     for (int k = 0; k < ND; k++) out[k] = sh[k][0];
 }
 
-// y = D_w^-1 (J + i w M) x for member w = blockIdx.y; ND = 1: partial (r0, y);
+// y = D_w^-1 (J + i w M) x for the ZMB members w = blockIdx.y * ZMB + m of a block:
+// each thread applies its row to every member of the group in turn, so the row's
+// matrix entries come from cache after the first member.  ND = 1: partial (r0, y);
 // ND = 2: partials (y, s) and (y, y)
+constexpr int ZMB = 8;
+
 template <int ND>
-__global__ __launch_bounds__(ZB) void k_zop(int64_t n, const int32_t *__restrict__ rowptr,
+__global__ __launch_bounds__(ZB) void k_zop(int64_t n, int nw, const int32_t *__restrict__ rowptr,
                                             const int32_t *__restrict__ colind, const double *__restrict__ J,
                                             const int32_t *__restrict__ dofdiag, const double *__restrict__ M,
                                             const double *__restrict__ omega, const double2 *__restrict__ X,
                                             double2 *__restrict__ Y, const double2 *__restrict__ R,
                                             double2 *__restrict__ part, const ZState *__restrict__ st) {
-  const int w = blockIdx.y;
-  if (st[w].done) return;
-  const double om = omega[w];
-  const double2 *x = X + (int64_t)w * n;
-  double2 *y = Y + (int64_t)w * n;
-  const double2 *r = R + (int64_t)w * n;
-  double2 acc[ND];
+  const int w0 = blockIdx.y * ZMB;
+  bool live[ZMB];
+  bool any = false;
 #pragma unroll
-  for (int k = 0; k < ND; k++) acc[k] = make_double2(0.0, 0.0);
+  for (int m = 0; m < ZMB; m++) {
+    live[m] = w0 + m < nw && !st[w0 + m].done;
+    any |= live[m];
+  }
+  if (!any) return;
+  double2 acc[ZMB][ND];
+#pragma unroll
+  for (int m = 0; m < ZMB; m++)
+#pragma unroll
+    for (int k = 0; k < ND; k++) acc[m][k] = make_double2(0.0, 0.0);
   for (int64_t i = (int64_t)blockIdx.x * ZB + threadIdx.x; i < n; i += (int64_t)gridDim.x * ZB) {
-    double2 sum = make_double2(0.0, 0.0);
-    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
-      const double a = J[k];
-      const double2 xv = x[colind[k]];
-      sum.x += a * xv.x;
-      sum.y += a * xv.y;
-    }
-    const double2 xi = x[i];
-    sum = zadd(sum, make_double2(-om * M[i] * xi.y, om * M[i] * xi.x));
-    const double2 yi = zdiv(sum, zdiag(J, dofdiag, M, om, i));
-    y[i] = yi;
-    if constexpr (ND == 1) {
-      acc[0] = zadd(acc[0], zcmul(r[i], yi));
-    } else {
-      acc[0] = zadd(acc[0], zcmul(yi, r[i]));
-      acc[1].x += yi.x * yi.x + yi.y * yi.y;
+    const int32_t k0 = rowptr[i], k1 = rowptr[i + 1];
+    const double Mi = M[i], Jii = J[dofdiag[i]];
+#pragma unroll
+    for (int m = 0; m < ZMB; m++) {
+      if (!live[m]) continue;
+      const int w = w0 + m;
+      const double om = omega[w];
+      const double2 *x = X + (int64_t)w * n;
+      double2 sum = make_double2(0.0, 0.0);
+      for (int32_t k = k0; k < k1; k++) {
+        const double a = J[k];
+        const double2 xv = x[colind[k]];
+        sum.x += a * xv.x;
+        sum.y += a * xv.y;
+      }
+      const double2 xi = x[i];
+      sum = zadd(sum, make_double2(-om * Mi * xi.y, om * Mi * xi.x));
+      const double2 yi = zdiv(sum, make_double2(Jii, om * Mi));
+      Y[(int64_t)w * n + i] = yi;
+      const double2 ri = R[(int64_t)w * n + i];
+      if constexpr (ND == 1) {
+        acc[m][0] = zadd(acc[m][0], zcmul(ri, yi));
+      } else {
+        acc[m][0] = zadd(acc[m][0], zcmul(yi, ri));
+        acc[m][1].x += yi.x * yi.x + yi.y * yi.y;
+      }
     }
   }
-  block_partials<ND>(acc, part + ((int64_t)w * gridDim.x + blockIdx.x) * ND);
+#pragma unroll
+  for (int m = 0; m < ZMB; m++)
+    if (live[m]) block_partials<ND>(acc[m], part + ((int64_t)(w0 + m) * gridDim.x + blockIdx.x) * ND);
 }
 
 // x = 0, r = r0 = p = D_w^-1 b; partials (r0, r) and (r, r)
@@ -478,6 +499,7 @@ int vfvm_impedance_sweep(vfvm_impedance *z, int64_t nw, const double *omega, con
     DevBuf<ZState> zst;
     zst.alloc((size_t)nw);
     const dim3 g2((unsigned)nblk, (unsigned)nw);
+    const dim3 gop((unsigned)nblk, (unsigned)((nw + ZMB - 1) / ZMB));
     const double tol2 = reltol * reltol;
     hipLaunchKernelGGL(k_zinit, g2, dim3(ZB), 0, st, n, z->b.p, z->J.p, s->d_dofdiag.p, z->M.p, dom.p, X.p, R.p,
                        R0.p, P.p, part.p);
@@ -489,11 +511,11 @@ int vfvm_impedance_sweep(vfvm_impedance *z, int64_t nw, const double *omega, con
     while (it < maxiters && !all) {
       const int nb = std::min(CHECK, maxiters - it);
       for (int k = 0; k < nb; k++) {
-        hipLaunchKernelGGL((k_zop<1>), g2, dim3(ZB), 0, st, n, s->d_rowptr.p, s->d_colind.p, z->J.p,
+        hipLaunchKernelGGL((k_zop<1>), gop, dim3(ZB), 0, st, n, (int)nw, s->d_rowptr.p, s->d_colind.p, z->J.p,
                            s->d_dofdiag.p, z->M.p, dom.p, P.p, V.p, R0.p, part.p, zst.p);
         hipLaunchKernelGGL((k_zfin<ZF_ALPHA, 1>), dim3((unsigned)nw), dim3(ZB), 0, st, nblk, part.p, zst.p, tol2, 0);
         hipLaunchKernelGGL(k_zs, g2, dim3(ZB), 0, st, n, R.p, V.p, S.p, zst.p);
-        hipLaunchKernelGGL((k_zop<2>), g2, dim3(ZB), 0, st, n, s->d_rowptr.p, s->d_colind.p, z->J.p,
+        hipLaunchKernelGGL((k_zop<2>), gop, dim3(ZB), 0, st, n, (int)nw, s->d_rowptr.p, s->d_colind.p, z->J.p,
                            s->d_dofdiag.p, z->M.p, dom.p, S.p, T.p, S.p, part.p, zst.p);
         hipLaunchKernelGGL((k_zfin<ZF_OMEGA, 2>), dim3((unsigned)nw), dim3(ZB), 0, st, nblk, part.p, zst.p, tol2, 0);
         hipLaunchKernelGGL(k_zx, g2, dim3(ZB), 0, st, n, X.p, R.p, R0.p, P.p, S.p, T.p, part.p, zst.p);
