@@ -151,6 +151,20 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
                        const int64_t *bfacenodes, const int64_t *bfaceregions, int nspec,
                        const vfvm_physics_t *physics, int nbregions, const vfvm_bc_t *bc,
                        vfvm_sys **out);
+/* The same for a grid in a symmetric coordinate system (ExtendableGrids'
+ * circular_symmetric! / spherical_symmetric!, used by
+ * examples/Example203_CoordinateSystems.jl): the form factors of
+ * src/vfvm_formfactors.jl:25-61 (Polar1D, Spherical1D: dim 1, the coordinate is the
+ * radius) and 102-161, 276-291 (Cylindrical2D: dim 2, coordinates r, z). */
+#define VFVM_COORD_CARTESIAN 0
+#define VFVM_COORD_POLAR1D 1
+#define VFVM_COORD_SPHERICAL1D 2
+#define VFVM_COORD_CYLINDRICAL2D 3
+int vfvm_system_create_cs(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coord, int64_t ncells,
+                          const int64_t *cellnodes, const int64_t *cellregions, int64_t nbfaces,
+                          const int64_t *bfacenodes, const int64_t *bfaceregions, int nspec,
+                          const vfvm_physics_t *physics, int nbregions, const vfvm_bc_t *bc, int coordsys,
+                          vfvm_sys **out);
 int vfvm_system_destroy(vfvm_sys *sys);
 /* per-node source term, nspec x nnodes (Physics(source = array), src/vfvm_system.jl:312-319); NULL clears */
 int vfvm_system_set_source(vfvm_sys *sys, const double *src);

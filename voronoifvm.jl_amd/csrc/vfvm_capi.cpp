@@ -498,7 +498,20 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
                        const int64_t *cellnodes, const int64_t *cellregions, int64_t nbfaces, const int64_t *bfacenodes,
                        const int64_t *bfaceregions, int nspec, const vfvm_physics_t *physics, int nbregions,
                        const vfvm_bc_t *bc, vfvm_sys **out) {
+  return vfvm_system_create_cs(ctx, dim, nnodes, coord, ncells, cellnodes, cellregions, nbfaces, bfacenodes,
+                               bfaceregions, nspec, physics, nbregions, bc, VFVM_COORD_CARTESIAN, out);
+}
+
+int vfvm_system_create_cs(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coord, int64_t ncells,
+                          const int64_t *cellnodes, const int64_t *cellregions, int64_t nbfaces,
+                          const int64_t *bfacenodes, const int64_t *bfaceregions, int nspec,
+                          const vfvm_physics_t *physics, int nbregions, const vfvm_bc_t *bc, int coordsys,
+                          vfvm_sys **out) {
   return guarded(ctx, [&]() {
+    const bool cs_ok = coordsys == VFVM_COORD_CARTESIAN ||
+                       ((coordsys == VFVM_COORD_POLAR1D || coordsys == VFVM_COORD_SPHERICAL1D) && dim == 1) ||
+                       (coordsys == VFVM_COORD_CYLINDRICAL2D && dim == 2);
+    if (!cs_ok) throw Error(VFVM_ERR_ARG, "coordinate system does not fit the grid dimension");
     if (!ctx || !out || !coord || !cellnodes || !physics) throw Error(VFVM_ERR_ARG, "null argument");
     if (nspec < 1 || nspec > MAXSPEC) throw Error(VFVM_ERR_ARG, "nspec must be 1..3");
     if (nbregions < 0 || nbregions > MAXBREG) throw Error(VFVM_ERR_ARG, "at most 8 boundary regions");
@@ -513,6 +526,7 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     s->nspec = nspec;
     s->nbreg = nbregions;
     s->nnodes = nnodes;
+    s->coordsys = coordsys;
     s->coord.assign(coord, coord + (size_t)dim * nnodes);
     s->ndof = nnodes * nspec;
     s->phys = *physics;
@@ -528,9 +542,9 @@ int vfvm_system_create(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coo
     if (dev_setup) {
       // cells part on the device (form factors, node graph), boundary items on the host
       build_node_graph_device(*s, coord, cellnodes, ncells);
-      build_boundary(dim, nnodes, coord, nbfaces, bfacenodes, bfaceregions, s->hs);
+      build_boundary(dim, nnodes, coord, nbfaces, bfacenodes, bfaceregions, s->hs, coordsys);
     } else {
-      build_setup(dim, nnodes, coord, ncells, cellnodes, nbfaces, bfacenodes, bfaceregions, s->hs);
+      build_setup(dim, nnodes, coord, ncells, cellnodes, nbfaces, bfacenodes, bfaceregions, s->hs, coordsys);
       // ranges checked by build_setup
       s->cellnodes0.resize((size_t)(dim + 1) * ncells);
       for (size_t k = 0; k < s->cellnodes0.size(); k++) s->cellnodes0[k] = (int32_t)(cellnodes[k] - 1);

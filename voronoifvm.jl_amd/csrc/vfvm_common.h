@@ -117,11 +117,12 @@ struct HostSetup {
   int64_t nedges = 0;
 };
 
+// cs: coordinate system (geom::CS_*, vfvm_geometry.h)
 void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, const int64_t *cellnodes,
-                 int64_t nbfaces, const int64_t *bfacenodes, const int64_t *bfaceregions, HostSetup &out);
+                 int64_t nbfaces, const int64_t *bfacenodes, const int64_t *bfaceregions, HostSetup &out, int cs = 0);
 // the boundary node items of build_setup alone (bn_ptr, bn_reg, bn_fac, bnodes)
 void build_boundary(int dim, int64_t nnodes, const double *coord, int64_t nbfaces, const int64_t *bfacenodes,
-                    const int64_t *bfaceregions, HostSetup &out);
+                    const int64_t *bfaceregions, HostSetup &out, int cs = 0);
 Masks make_masks(int nspec, const vfvm_physics_t &ph, const vfvm_bc_t *bc, int nbreg);
 void build_dof_csr(const HostSetup &hs, int nspec, const Masks &m, std::vector<int32_t> &rowptr,
                    std::vector<int32_t> &colind);

@@ -6,8 +6,12 @@
 //   cellfactors!(Edge1D, Cartesian1D)        src/vfvm_formfactors.jl:12-23
 //   cellfactors!(Triangle2D, Cartesian2D)     src/vfvm_formfactors.jl:64-100
 //   cellfactors!(Tetrahedron3D, Cartesian3D)  src/vfvm_formfactors.jl:163-235
+//   cellfactors!(Edge1D, Polar1D / Spherical1D) src/vfvm_formfactors.jl:25-61
+//   cellfactors!(Triangle2D, Cylindrical2D)     src/vfvm_formfactors.jl:102-161
 //   bfacefactors!(Vertex0D / Edge1D / Triangle2D, Cartesian)
 //                                            src/vfvm_formfactors.jl:244-247, 263-274, 293-332
+//   bfacefactors!(Vertex0D, Polar1D / Spherical1D), (Edge1D, Cylindrical2D)
+//                                            src/vfvm_formfactors.jl:249-261, 276-291
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,6 +21,22 @@
 
 namespace vfvm {
 namespace geom {
+
+// coordinate systems (ExtendableGrids' CoordinateSystem; circular_symmetric! gives
+// Polar1D in 1D and Cylindrical2D in 2D, spherical_symmetric! gives Spherical1D)
+enum { CS_CARTESIAN = 0, CS_POLAR1D = 1, CS_SPHERICAL1D = 2, CS_CYLINDRICAL2D = 3 };
+constexpr double PI = 3.141592653589793;
+
+// first coordinate of the circumcentre of the triangle (a, b, c), from the
+// perpendicular-bisector equations relative to a (Cramer's rule)
+__host__ __device__ inline double circumcenter_x(const double *a, const double *b, const double *c) {
+#pragma clang fp contract(off)
+  const double bx = b[0] - a[0], by = b[1] - a[1];
+  const double cx = c[0] - a[0], cy = c[1] - a[1];
+  const double d = 2.0 * (bx * cy - by * cx);
+  const double b2 = bx * bx + by * by, c2 = cx * cx + cy * cy;
+  return a[0] + (cy * b2 - by * c2) / d;
+}
 
 // ExtendableGrids local element tables: triangle edge i is opposite node i;
 // tetrahedron edges in the order 12, 13, 14, 23, 24, 34.
@@ -39,9 +59,29 @@ __host__ __device__ inline void local_edge(int dim, int e, int &a, int &b) {
 // coord is dim x nnodes column-major; n holds the 0-based node numbers of the cell.
 // npar: dim+1 node volume parts; epar: sigma/h parts of the cell's edges.
 template <class I>
-__host__ __device__ inline void cell_factors(int dim, const double *coord, const I *n, double *npar, double *epar) {
+__host__ __device__ inline void cell_factors(int dim, const double *coord, const I *n, double *npar, double *epar,
+                                             int cs = CS_CARTESIAN) {
 #pragma clang fp contract(off)
   if (dim == 1) {
+    if (cs == CS_POLAR1D || cs == CS_SPHERICAL1D) {
+      // src/vfvm_formfactors.jl:25-61 (the ordered radii; node parts by local number)
+      double r0 = coord[n[0]], r1 = coord[n[1]];
+      if (r1 < r0) {
+        r0 = coord[n[1]];
+        r1 = coord[n[0]];
+      }
+      const double rhalf = 0.5 * (r1 + r0);
+      if (cs == CS_POLAR1D) {
+        npar[0] = PI * (rhalf * rhalf - r0 * r0);
+        npar[1] = PI * (r1 * r1 - rhalf * rhalf);
+        epar[0] = 2.0 * PI * rhalf / (r1 - r0);
+      } else {
+        npar[0] = PI * (rhalf * rhalf * rhalf - r0 * r0 * r0) * 4.0 / 3.0;
+        npar[1] = PI * (r1 * r1 * r1 - rhalf * rhalf * rhalf) * 4.0 / 3.0;
+        epar[0] = 4.0 * PI * (rhalf * rhalf) / (r1 - r0);
+      }
+      return;
+    }
     double d = fabs(coord[n[1]] - coord[n[0]]);
     npar[0] = d / 2;
     npar[1] = d / 2;
@@ -64,6 +104,28 @@ __host__ __device__ inline void cell_factors(int dim, const double *coord, const
     epar[1] = (dd[2] + dd[0] - dd[1]) * 0.125 * ivol;
     epar[2] = (dd[0] + dd[1] - dd[2]) * 0.125 * ivol;
     npar[0] = npar[1] = npar[2] = 0.0;
+    if (cs == CS_CYLINDRICAL2D) {
+      // src/vfvm_formfactors.jl:102-161: areas times 2 pi times mean radii
+      const double rcc = circumcenter_x(coord + 2 * n[0], coord + 2 * n[1], coord + 2 * n[2]);
+      for (int e = 0; e < 3; e++) {
+        int a, b;
+        local_edge(2, e, a, b);
+        const double ra = coord[2 * n[a]], rb = coord[2 * n[b]];
+        const double emid = 0.5 * (ra + rb);
+        const double cyl1 = 2.0 * PI * (ra + rcc + emid) / 3.0;
+        const double cyl2 = 2.0 * PI * (rb + rcc + emid) / 3.0;
+        npar[a] += epar[e] * dd[e] * 0.25 * cyl1;
+        npar[b] += epar[e] * dd[e] * 0.25 * cyl2;
+      }
+      for (int e = 0; e < 3; e++) {
+        int a, b;
+        local_edge(2, e, a, b);
+        const double emid = 0.5 * (coord[2 * n[a]] + coord[2 * n[b]]);
+        const double rmid = (rcc + emid) / 2.0;
+        epar[e] *= 2.0 * PI * rmid;
+      }
+      return;
+    }
     for (int e = 0; e < 3; e++) {
       int a, b;
       local_edge(2, e, a, b);
@@ -116,10 +178,23 @@ __host__ __device__ inline void cell_factors(int dim, const double *coord, const
 
 // Boundary face measure parts per face node (n: 0-based face nodes).
 template <class I>
-__host__ __device__ inline void bface_factors(int dim, const double *coord, const I *n, double *npar) {
+__host__ __device__ inline void bface_factors(int dim, const double *coord, const I *n, double *npar,
+                                              int cs = CS_CARTESIAN) {
 #pragma clang fp contract(off)
   if (dim == 1) {
-    npar[0] = 1.0;
+    // src/vfvm_formfactors.jl:244-262
+    const double r = coord[n[0]];
+    npar[0] = cs == CS_POLAR1D ? 2.0 * PI * r : (cs == CS_SPHERICAL1D ? 4.0 * PI * (r * r) : 1.0);
+    return;
+  }
+  if (dim == 2 && cs == CS_CYLINDRICAL2D) {
+    // src/vfvm_formfactors.jl:276-291
+    const double r1 = coord[2 * n[0]], r2 = coord[2 * n[1]];
+    const double dr = r1 - r2, dz = coord[2 * n[0] + 1] - coord[2 * n[1] + 1];
+    const double rmid = (r1 + r2) / 2.0;
+    const double l = sqrt(dr * dr + dz * dz);
+    npar[0] = PI * (r1 + rmid) * l / 2.0;
+    npar[1] = PI * (r2 + rmid) * l / 2.0;
     return;
   }
   if (dim == 2) {

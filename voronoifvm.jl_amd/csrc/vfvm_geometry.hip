@@ -27,7 +27,8 @@ constexpr int GEOM_BLOCK = 256;
 
 template <int DIM>
 __global__ __launch_bounds__(GEOM_BLOCK) void k_geom_cells(int64_t ncells, const double *__restrict__ coord,
-                                                           const int32_t *__restrict__ cn, double *__restrict__ cf) {
+                                                           const int32_t *__restrict__ cn, double *__restrict__ cf,
+                                                           int cs) {
   constexpr int NV = DIM + 1, NE = geom::edges_per_cell(DIM), ST = NV + NE;
   const int64_t c = (int64_t)blockIdx.x * GEOM_BLOCK + threadIdx.x;
   if (c >= ncells) return;
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(GEOM_BLOCK) void k_geom_cells(int64_t ncells, const
 #pragma unroll
   for (int i = 0; i < NV; i++) n[i] = cn[c * NV + i];
   double npar[NV], epar[NE];
-  geom::cell_factors(DIM, coord, n, npar, epar);
+  geom::cell_factors(DIM, coord, n, npar, epar, cs);
   double *o = cf + c * ST;
 #pragma unroll
   for (int i = 0; i < NV; i++) o[i] = npar[i];
@@ -108,14 +109,14 @@ template <int DIM>
 __global__ __launch_bounds__(GEOM_BLOCK) void k_geom_bfaces(int64_t nbfaces, const double *__restrict__ coord,
                                                             const int32_t *__restrict__ bfn,
                                                             const int32_t *__restrict__ bslot,
-                                                            double *__restrict__ bn_fac) {
+                                                            double *__restrict__ bn_fac, int cs) {
   const int64_t b = (int64_t)blockIdx.x * GEOM_BLOCK + threadIdx.x;
   if (b >= nbfaces) return;
   int32_t n[DIM];
 #pragma unroll
   for (int i = 0; i < DIM; i++) n[i] = bfn[b * DIM + i];
   double f[DIM];
-  geom::bface_factors(DIM, coord, n, f);
+  geom::bface_factors(DIM, coord, n, f, cs);
 #pragma unroll
   for (int i = 0; i < DIM; i++) bn_fac[bslot[b * DIM + i]] = f[i];
 }
@@ -355,7 +356,8 @@ void node_graph_device(vfvm_sys &s, const double *coord, const int64_t *cellnode
   g.coord.upload(coord, (size_t)DIM * nn, st);
   g.cf.alloc((size_t)std::max<int64_t>(nc, 1) * (NV + NE));
   if (nc > 0)
-    hipLaunchKernelGGL(k_geom_cells<DIM>, blocks(nc), dim3(GEOM_BLOCK), 0, st, nc, g.coord.p, g.cn.p, g.cf.p);
+    hipLaunchKernelGGL(k_geom_cells<DIM>, blocks(nc), dim3(GEOM_BLOCK), 0, st, nc, g.coord.p, g.cn.p, g.cf.p,
+                       s.coordsys);
   // incidences in cell order
   DevBuf<int32_t> cnt, cursor;
   cnt.alloc((size_t)std::max<int64_t>(nn, 1));
@@ -452,7 +454,7 @@ void launch_geometry(vfvm_sys &s) {
   const auto blocks = [](int64_t n) { return dim3((unsigned)std::max<int64_t>(1, (n + GEOM_BLOCK - 1) / GEOM_BLOCK)); };
   if (s.ncells > 0)
     hipLaunchKernelGGL(k_geom_cells<DIM>, blocks(s.ncells), dim3(GEOM_BLOCK), 0, st, s.ncells, g.coord.p, g.cn.p,
-                       g.cf.p);
+                       g.cf.p, s.coordsys);
   const int G = g.group;
   const dim3 nb = blocks(s.nnodes * G);
 #define VFVM_GEOM_NODES(GG)                                                                                     \
@@ -471,7 +473,7 @@ void launch_geometry(vfvm_sys &s) {
 #undef VFVM_GEOM_NODES
   if (s.nbfaces > 0)
     hipLaunchKernelGGL(k_geom_bfaces<DIM>, blocks(s.nbfaces), dim3(GEOM_BLOCK), 0, st, s.nbfaces, g.coord.p, g.bfn.p,
-                       g.bslot.p, s.d_bn_fac.p);
+                       g.bslot.p, s.d_bn_fac.p, s.coordsys);
   VFVM_HIP_CHECK(hipGetLastError());
 }
 

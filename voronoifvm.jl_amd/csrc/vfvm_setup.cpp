@@ -25,7 +25,7 @@ using geom::edges_per_cell;
 using geom::local_edge;
 
 void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, const int64_t *cellnodes1,
-                 int64_t nbfaces, const int64_t *bfacenodes1, const int64_t *bfaceregions, HostSetup &out) {
+                 int64_t nbfaces, const int64_t *bfacenodes1, const int64_t *bfaceregions, HostSetup &out, int cs) {
   if (dim < 1 || dim > 3) throw Error(VFVM_ERR_ARG, "dim must be 1, 2 or 3");
   if (nnodes >= (int64_t(1) << 31) - 1) throw Error(VFVM_ERR_ARG, "too many nodes for int32 indices");
   const int nv = dim + 1, ne = edges_per_cell(dim);
@@ -54,7 +54,7 @@ void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, c
   out.nodevol.assign((size_t)nnodes, 0.0);
   double npar[4], epar[6];
   for (int64_t c = 0; c < ncells; c++) {
-    cell_factors(dim, coord, &cn[nv * c], npar, epar);
+    cell_factors(dim, coord, &cn[nv * c], npar, epar, cs);
     for (int i = 0; i < nv; i++) out.nodevol[cn[nv * c + i]] += npar[i];
     for (int e = 0; e < ne; e++) {
       int a, b;
@@ -145,11 +145,11 @@ void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, c
     nedge2 += rlen[i];
   }
   out.nedges = nedge2 / 2;
-  build_boundary(dim, nnodes, coord, nbfaces, bfacenodes1, bfaceregions, out);
+  build_boundary(dim, nnodes, coord, nbfaces, bfacenodes1, bfaceregions, out, cs);
 }
 
 void build_boundary(int dim, int64_t nnodes, const double *coord, int64_t nbfaces, const int64_t *bfacenodes1,
-                    const int64_t *bfaceregions, HostSetup &out) {
+                    const int64_t *bfaceregions, HostSetup &out, int cs) {
   // boundary node items in bface order
   std::vector<int64_t> bcnt((size_t)nnodes + 1, 0);
   for (int64_t ib = 0; ib < nbfaces; ib++)
@@ -168,7 +168,7 @@ void build_boundary(int dim, int64_t nnodes, const double *coord, int64_t nbface
     int64_t bn[3];
     for (int i = 0; i < dim; i++) bn[i] = bfacenodes1[dim * ib + i] - 1;
     double f[3];
-    bface_factors(dim, coord, bn, f);
+    bface_factors(dim, coord, bn, f, cs);
     for (int i = 0; i < dim; i++) {
       int64_t p = bnext[bn[i]]++;
       out.bn_reg[p] = (int32_t)bfaceregions[ib];

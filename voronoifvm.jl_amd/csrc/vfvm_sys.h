@@ -9,6 +9,7 @@ using namespace vfvm;
 struct vfvm_sys {
   vfvm_ctx *ctx = nullptr;
   int dim = 0, nspec = 1, nbreg = 0;
+  int coordsys = 0;  // geom::CS_* of the grid
   int64_t nnodes = 0, ndof = 0, nnz = 0;
   vfvm_physics_t phys{};
   std::vector<vfvm_bc_t> bc;

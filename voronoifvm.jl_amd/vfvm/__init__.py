@@ -5,7 +5,7 @@ is done by libvfvm_hip.so (HIP kernels behind the C ABI in include/vfvm_hip.h).
 """
 from ._capi import AssemblyError, ConvergenceError, DeviceArray, LinearSolverError, VFVMError
 from .control import SolverControl, fixed_timesteps
-from .grids import Grid, linspace, read_sg, simplexgrid
+from .grids import Grid, circular_symmetric, linspace, read_sg, simplexgrid, spherical_symmetric
 from .physics import (Dirichlet, DriftDiffusionFlux, DriftDiffusionReaction, ExchangeReaction, IdentityStorage,
                       LinearFlux, Neumann, NonlinearDiffusionFlux, Physics, PowerFlux, PowerReaction, Robin,
                       SinhReaction)

@@ -68,6 +68,9 @@ SIGNATURES = {
     "vfvm_device_copy": (C.c_int, [_vp, _vp, _vp, C.c_int64, C.c_int]),
     "vfvm_system_create": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_int64, _lp, _lp, C.c_int64, _lp, _lp, C.c_int,
                                      C.POINTER(PhysicsT), C.c_int, C.POINTER(BCT), C.POINTER(_vp)]),
+    "vfvm_system_create_cs": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_int64, _lp, _lp, C.c_int64, _lp, _lp,
+                                        C.c_int, C.POINTER(PhysicsT), C.c_int, C.POINTER(BCT), C.c_int,
+                                        C.POINTER(_vp)]),
     "vfvm_system_destroy": (C.c_int, [_vp]),
     "vfvm_system_set_source": (C.c_int, [_vp, _dp]),
     "vfvm_system_set_bc": (C.c_int, [_vp, C.POINTER(BCT)]),

@@ -39,6 +39,9 @@ class Grid:
     bfacenodes: np.ndarray  # (dim, B) int32
     bfaceregions: np.ndarray  # (B,) int32, 1-based
     shape: tuple = field(default=())
+    # coordinate system: 0 Cartesian, 1 polar (1D radius), 2 spherical (1D radius),
+    # 3 cylindrical (2D r, z); set by circular_symmetric / spherical_symmetric
+    coordsys: int = 0
 
     @property
     def num_nodes(self) -> int:
@@ -224,3 +227,27 @@ def read_sg(path: str) -> Grid:
         faces, fregs = np.zeros((dim, 0), np.int64), np.zeros(0, np.int64)
     return Grid(dim, coord, np.ascontiguousarray(cells.astype(np.int32)), cregs.astype(np.int32),
                 np.ascontiguousarray(faces.astype(np.int32)), fregs.astype(np.int32))
+
+
+COORD_CARTESIAN, COORD_POLAR1D, COORD_SPHERICAL1D, COORD_CYLINDRICAL2D = 0, 1, 2, 3
+
+
+def circular_symmetric(grid: Grid) -> Grid:
+    """circular_symmetric!(grid) (ExtendableGrids; used by
+    examples/Example203_CoordinateSystems.jl): a 1D grid becomes polar (the coordinate is
+    the radius), a 2D grid cylindrical (coordinates r, z)."""
+    if grid.dim == 1:
+        grid.coordsys = COORD_POLAR1D
+    elif grid.dim == 2:
+        grid.coordsys = COORD_CYLINDRICAL2D
+    else:
+        raise ValueError("circular symmetry needs a 1D or 2D grid")
+    return grid
+
+
+def spherical_symmetric(grid: Grid) -> Grid:
+    """spherical_symmetric!(grid): a 1D grid becomes spherical (the coordinate is the radius)."""
+    if grid.dim != 1:
+        raise ValueError("spherical symmetry needs a 1D grid")
+    grid.coordsys = COORD_SPHERICAL1D
+    return grid

@@ -150,9 +150,10 @@ class System:
             br = np.ascontiguousarray(g.bfaceregions.astype(np.int64))
             tab = self._bc_table()
             h = C.c_void_p()
-            rc = L.vfvm_system_create(self.ctx.handle, g.dim, g.num_nodes, _capi.dptr(coord), g.num_cells,
-                                      _capi.lptr(cn), _capi.lptr(cr), g.num_bfaces, _capi.lptr(bn), _capi.lptr(br), ns,
-                                      C.byref(ph), self._nbreg(), tab, C.byref(h))
+            rc = L.vfvm_system_create_cs(self.ctx.handle, g.dim, g.num_nodes, _capi.dptr(coord), g.num_cells,
+                                         _capi.lptr(cn), _capi.lptr(cr), g.num_bfaces, _capi.lptr(bn),
+                                         _capi.lptr(br), ns, C.byref(ph), self._nbreg(), tab, int(g.coordsys),
+                                         C.byref(h))
             _capi.check(rc, self.ctx.handle)
             self._h = h
             src = self._source_array()
@@ -226,7 +227,8 @@ class System:
             return a is b or np.array_equal(a, b)
 
         if (grid.dim != g.dim or grid.num_nodes != g.num_nodes or not same(grid.cellnodes, g.cellnodes)
-                or not same(grid.bfacenodes, g.bfacenodes) or not same(grid.bfaceregions, g.bfaceregions)):
+                or not same(grid.bfacenodes, g.bfacenodes) or not same(grid.bfaceregions, g.bfaceregions)
+                or grid.coordsys != g.coordsys):
             raise ValueError("update_grid needs the same cells and boundary faces (only coordinates may change)")
         coord = np.asfortranarray(grid.coord, dtype=np.float64)
         if self._h is not None:
