@@ -10,6 +10,9 @@
 #   update_grid!      (src/vfvm_system.jl:607-612)    -> hip_update_grid!
 #   eval_rhs!, eval_jacobian!, mass_matrix
 #                     (src/vfvm_diffeq_interface.jl)  -> hip_eval_rhs!, hip_eval_jacobian!, hip_mass_matrix
+#   ImpedanceSystem, impedance, measurement_derivative
+#                     (src/vfvm_impedance.jl)         -> hip_impedance, hip_measurement_derivatives
+# A grid with circular_symmetric! / spherical_symmetric! goes to vfvm_system_create_cs.
 #
 # The Physics callbacks are Julia closures the device cannot call; a system is
 # mapped to a registry entry with `hip_physics` (flux/reaction/storage kinds and
@@ -196,6 +199,45 @@ function hip_mass_matrix(ctx, sys, ndof)
     rc = ccall((:vfvm_mass_matrix, libvfvm), Cint, (Ptr{Cvoid}, Ptr{Float64}), sys, d)
     check(rc, ctx)
     return Diagonal(d)
+end
+
+"""
+    hip_impedance(ctx, sys, U0, ω, dmeas_stdy, dmeas_tran; excited_spec, excited_bc, λ0)
+
+ImpedanceSystem + impedance (src/vfvm_impedance.jl) for a whole frequency sweep at
+once: boundary-excited (excited_spec, excited_bc given) or excited through the
+parameter λ (ImpedanceSystem(sys, U0; λ0)).  Returns the complex impedances.
+"""
+function hip_impedance(ctx, sys, U0, ω::Vector{Float64}, dmeas_stdy, dmeas_tran;
+                       excited_spec = 0, excited_bc = 0, λ0 = 0.0, reltol = 1.0e-12, maxiters = 20000)
+    isys = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = excited_bc > 0 ?
+        ccall((:vfvm_impedance_create, libvfvm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint, Ref{Ptr{Cvoid}}),
+              sys, U0, excited_spec, excited_bc, isys) :
+        ccall((:vfvm_impedance_create_param, libvfvm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Float64, Ref{Ptr{Cvoid}}),
+              sys, U0, λ0, isys)
+    check(rc, ctx)
+    Z = zeros(ComplexF64, length(ω))
+    rc = ccall((:vfvm_impedance_sweep, libvfvm), Cint,
+               (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Int32, Ptr{ComplexF64},
+                Ptr{ComplexF64}, Ptr{Int32}),
+               isys[], length(ω), ω, dmeas_stdy, dmeas_tran, reltol, maxiters, C_NULL, Z, C_NULL)
+    ccall((:vfvm_impedance_destroy, libvfvm), Cint, (Ptr{Cvoid},), isys[])
+    check(rc, ctx)
+    return Z
+end
+
+"""
+    hip_measurement_derivatives(ctx, sys, U0, T, spec) -> (dmeas_stdy, dmeas_tran)
+
+measurement_derivative of -integrate_stdy / -integrate_tran with test function T.
+"""
+function hip_measurement_derivatives(ctx, sys, U0, T, spec, ndof)
+    ds, dt = zeros(ndof), zeros(ndof)
+    rc = ccall((:vfvm_measurement_derivatives, libvfvm), Cint,
+               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Float64}, Ptr{Float64}), sys, U0, T, spec, ds, dt)
+    check(rc, ctx)
+    return ds, dt
 end
 
 struct StepControlT
