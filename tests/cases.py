@@ -27,6 +27,8 @@ def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None, oracle=True, embe
     """phys_kw: dict(flux=..., reaction=..., storage=...) of vfvm registry objects.
     bcs: vfvm.Dirichlet / vfvm.Robin entries; legacy: (species, region, value) Dirichlet.
     With oracle=False only the device system is built (second item None)."""
+    phys_kw = dict(phys_kw)
+    embed_source = embed_source or phys_kw.pop("embed_source", False)
     ph = vfvm.Physics(bcondition=list(bcs), source=source, embed_source=embed_source, **phys_kw)
     sys = vfvm.System(grid, ph, species=list(range(1, nspec + 1)))
     for (sp, reg, val) in legacy:

@@ -254,11 +254,16 @@ constexpr int STREAM_J = 8 * ASM_BLOCK;    // Jacobian slots staged per block
 
 constexpr int STREAM_TAB = 256;  // factor-table entries kept in LDS
 
-template <int NS, bool JAC, bool SADJ>
+// CMP: the staged adjacency in its compact form (16-bit offsets, 16-bit factor-table
+// indices, the table in LDS): a quarter of the LDS of int32 columns and FP64 factors,
+// so more blocks fit on a CU.
+template <int NS, bool JAC, bool SADJ, bool CMP = false>
 __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
   constexpr int NP = 2 * NS;
-  __shared__ int32_t s_col[SADJ ? STREAM_ADJ : 1];
-  __shared__ double s_fac[SADJ ? STREAM_ADJ : 1];
+  __shared__ int32_t s_col[SADJ && !CMP ? STREAM_ADJ : 1];
+  __shared__ double s_fac[SADJ && !CMP ? STREAM_ADJ : 1];
+  __shared__ int16_t s_off[SADJ && CMP ? STREAM_ADJ : 1];
+  __shared__ uint16_t s_fi[SADJ && CMP ? STREAM_ADJ : 1];
   __shared__ double s_J[JAC ? STREAM_J : 1];
   __shared__ double s_tab[STREAM_TAB];
   // a small factor table is read from LDS (one lookup per entry, no dependent global load)
@@ -283,8 +288,13 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
   const bool sJ = JAC && (j1 - j0) <= STREAM_J && __syncthreads_and(active || !mine);
   if (sadj)
     for (int64_t q = e0 + threadIdx.x; q < e1; q += ASM_BLOCK) {
-      s_col[q - e0] = a.adj_off ? (int32_t)a.adj_off[q] : a.adj_col[q];  // offset from its node, or the node
-      s_fac[q - e0] = a.adj_fidx ? a.fac_tab[a.adj_fidx[q]] : a.adj_fac[q];
+      if constexpr (CMP) {
+        s_off[q - e0] = a.adj_off[q];
+        s_fi[q - e0] = a.adj_fidx[q];
+      } else {
+        s_col[q - e0] = a.adj_off ? (int32_t)a.adj_off[q] : a.adj_col[q];  // offset from its node, or the node
+        s_fac[q - e0] = a.adj_fidx ? a.fac_tab[a.adj_fidx[q]] : a.adj_fac[q];
+      }
     }
   __syncthreads();
   bool bad = false;
@@ -331,9 +341,14 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
       for (int q = 0; q < EB; q++) {
         const int e = eb + q;
         const bool ok = e < deg1;
-        const int32_t sc = (sadj && ok) ? s_col[a0 - e0 + e] : 0;
-        Mv[q] = ok ? (sadj ? (a.adj_off ? (int32_t)(K + sc) : sc) : adj_node(a, a0 + e, K)) : (int32_t)K;
-        fv[q] = ok ? (sadj ? s_fac[a0 - e0 + e] : factor(a0 + e)) : 0.0;
+        if constexpr (CMP) {
+          Mv[q] = ok ? (sadj ? (int32_t)(K + s_off[a0 - e0 + e]) : adj_node(a, a0 + e, K)) : (int32_t)K;
+          fv[q] = ok ? (sadj ? s_tab[s_fi[a0 - e0 + e]] : factor(a0 + e)) : 0.0;
+        } else {
+          const int32_t sc = (sadj && ok) ? s_col[a0 - e0 + e] : 0;
+          Mv[q] = ok ? (sadj ? (a.adj_off ? (int32_t)(K + sc) : sc) : adj_node(a, a0 + e, K)) : (int32_t)K;
+          fv[q] = ok ? (sadj ? s_fac[a0 - e0 + e] : factor(a0 + e)) : 0.0;
+        }
       }
 #pragma unroll
       for (int q = 0; q < EB; q++)
@@ -414,6 +429,14 @@ void launch_ns_g(const AsmArgs &a, hipStream_t s) {
   const int64_t grid = (a.nnodes + nodes_per_block - 1) / nodes_per_block;
   if (grid == 0) return;
   if (G == 1) {  // one node per lane: the LDS-staged kernel (SADJ: adjacency staged too)
+    const bool cmp = SADJ && a.adj_off && a.adj_fidx && a.ntab <= STREAM_TAB;
+    if (cmp) {
+      if (a.J)
+        VFVM_LAUNCH((k_assemble_stream<NS, true, true, true>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
+      else
+        VFVM_LAUNCH((k_assemble_stream<NS, false, true, true>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
+      return;
+    }
     if (a.J)
       VFVM_LAUNCH((k_assemble_stream<NS, true, SADJ>), dim3((unsigned)grid), dim3(ASM_BLOCK), 0, s, a);
     else
