@@ -106,3 +106,22 @@ def test_cylindrical_device_setup_equals_host_and_volume():
     area = 0.5 * np.abs((P[0, 1] - P[0, 0]) * (P[1, 2] - P[1, 0]) - (P[0, 2] - P[0, 0]) * (P[1, 1] - P[1, 0]))
     rc = P[0].mean(axis=0)
     assert vols[0].sum() == pytest.approx((2 * np.pi * area * rc).sum(), rel=1e-12)
+
+
+def test_cylindrical_geometry_update_equals_fresh_system():
+    """update_grid on a cylindrical system recomputes the cylindrical factors: bit for bit
+    those of a cylindrical system created with the moved coordinates."""
+    _print("case cylindrical update_grid")
+    from test_gpu_geometry import _asm, _moved, _same
+    g0 = circular_symmetric(cases.sg_grid("cyl_unstruct.sg"))
+    s0 = vfvm.System(g0, flux=vfvm.LinearFlux(1.0), species=[1])
+    s0.boundary_dirichlet(1, 2, 0.0)
+    s0.set_compact_graph(False)
+    g1 = _moved(g0, 0.15, 4)
+    s0.update_grid(g1)
+    s1 = vfvm.System(g1, flux=vfvm.LinearFlux(1.0), species=[1])
+    s1.boundary_dirichlet(1, 2, 0.0)
+    s1.set_compact_graph(False)
+    U = np.random.default_rng(8).uniform(0.0, 1.0, (1, g0.num_nodes))
+    assert _same(_asm(s0, U), _asm(s1, U))
+    assert np.array_equal(s0.nodevolumes(), s1.nodevolumes())

@@ -29,7 +29,7 @@ def _moved(g, scale, seed, boundary=False):
         inner[g.bfacenodes.ravel()] = False
     coord = g.coord.copy()
     coord[:, inner] += scale * h * rng.uniform(-1, 1, (g.dim, int(inner.sum())))
-    return Grid(g.dim, coord, g.cellnodes, g.cellregions, g.bfacenodes, g.bfaceregions, g.shape)
+    return Grid(g.dim, coord, g.cellnodes, g.cellregions, g.bfacenodes, g.bfaceregions, g.shape, g.coordsys)
 
 
 def _asm(sys, U):
