@@ -79,6 +79,7 @@ __device__ __forceinline__ void block_partials(double2 (&v)[ND], double2 *out) {
 // matrix entries come from cache after the first member.  ND = 1: partial (r0, y);
 // ND = 2: partials (y, s) and (y, y)
 constexpr int ZMB = 8;
+constexpr int ZRMAX = 8;  // rows up to this length are kept in registers
 
 template <int ND>
 __global__ __launch_bounds__(ZB) void k_zop(int64_t n, int nw, const int32_t *__restrict__ rowptr,
@@ -104,6 +105,16 @@ __global__ __launch_bounds__(ZB) void k_zop(int64_t n, int nw, const int32_t *__
   for (int64_t i = (int64_t)blockIdx.x * ZB + threadIdx.x; i < n; i += (int64_t)gridDim.x * ZB) {
     const int32_t k0 = rowptr[i], k1 = rowptr[i + 1];
     const double Mi = M[i], Jii = J[dofdiag[i]];
+    // a short row is held in registers for all members of the group
+    const bool short_row = k1 - k0 <= ZRMAX;
+    double jv[ZRMAX];
+    int32_t jc[ZRMAX];
+#pragma unroll
+    for (int q = 0; q < ZRMAX; q++) {
+      const bool in = short_row && k0 + q < k1;
+      jv[q] = in ? J[k0 + q] : 0.0;
+      jc[q] = in ? colind[k0 + q] : (int32_t)i;
+    }
 #pragma unroll
     for (int m = 0; m < ZMB; m++) {
       if (!live[m]) continue;
@@ -111,11 +122,22 @@ __global__ __launch_bounds__(ZB) void k_zop(int64_t n, int nw, const int32_t *__
       const double om = omega[w];
       const double2 *x = X + (int64_t)w * n;
       double2 sum = make_double2(0.0, 0.0);
-      for (int32_t k = k0; k < k1; k++) {
-        const double a = J[k];
-        const double2 xv = x[colind[k]];
-        sum.x += a * xv.x;
-        sum.y += a * xv.y;
+      if (short_row) {
+#pragma unroll
+        for (int q = 0; q < ZRMAX; q++) {
+          if (k0 + q < k1) {
+            const double2 xv = x[jc[q]];
+            sum.x += jv[q] * xv.x;
+            sum.y += jv[q] * xv.y;
+          }
+        }
+      } else {
+        for (int32_t k = k0; k < k1; k++) {
+          const double a = J[k];
+          const double2 xv = x[colind[k]];
+          sum.x += a * xv.x;
+          sum.y += a * xv.y;
+        }
       }
       const double2 xi = x[i];
       sum = zadd(sum, make_double2(-om * Mi * xi.y, om * Mi * xi.x));
