@@ -187,3 +187,22 @@ def test_parameter_derivative_matches_central_difference():
     err = np.abs(d - fd).max() / np.abs(fd).max()
     _print("sensitivity vs central difference", err)
     assert err <= 1e-5
+
+
+def test_testfunction_integral_matches_oracle_and_closed_form():
+    """integrate(system, T, U) on Example150's steady state: against the oracle's residual
+    without boundary conditions weighted by T, and against the exact boundary flux
+    u'(1) = -1/sinh(1) of -u'' + u = 0, u(0) = 1, u(1) = 0."""
+    _print("case testfunction integral")
+    g = simplexgrid(linspace(0.0, 1.0, 201))
+    sys, osys, ph = _problem(g)
+    c = vfvm.SolverControl(reltol_linear=1e-13, abstol_linear=0.0, maxiters_linear=5000, precon="ilu0")
+    U0 = vfvm.solve(sys, inival=0.0, control=c)
+    T = vfvm.testfunction(sys, [1], [2])
+    I = vfvm.integrate_testfunction(sys, T, U0)[0]
+    nsys = O.OracleSystem(g, 1, cases.oracle_physics(ph, 1), {}, O.EDGEWISE)
+    F = nsys.assemble(vfvm.dofs(U0), vfvm.dofs(U0), tstep=math.inf, embed=1.0)
+    Io = float(T @ F)
+    _print("integral", I, "oracle", Io, "exact", -1.0 / math.sinh(1.0))
+    assert abs(I - Io) <= 1e-12 * abs(Io)
+    assert abs(I + 1.0 / math.sinh(1.0)) <= 1e-3

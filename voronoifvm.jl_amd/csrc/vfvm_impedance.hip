@@ -338,6 +338,42 @@ __global__ __launch_bounds__(ZB) void k_meas_stdy(int64_t n, int ns, int spec, c
   y[j] = -acc;
 }
 
+// per-species sums of T_K F[K * ns + i]: one block, fixed order
+__global__ __launch_bounds__(ZB) void k_tdot(int64_t nnodes, int ns, const double *__restrict__ T,
+                                             const double *__restrict__ F, double *__restrict__ out) {
+  __shared__ double sh[ZB];
+  for (int i = 0; i < ns; i++) {
+    double acc = 0.0;
+    for (int64_t K = threadIdx.x; K < nnodes; K += ZB) acc += T[K] * F[K * ns + i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = ZB / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = sh[0];
+    __syncthreads();
+  }
+}
+
+// the residual of flux, reaction and source terms alone (every boundary condition
+// switched off) at U, tstep = Inf, into F
+void residual_without_bc(vfvm_sys &s, const double *dU, double lambda, double *dF) {
+  std::vector<vfvm_bc_t> saved = s.bc;
+  for (auto &c : s.bc) c.kind = VFVM_BC_NONE;
+  try {
+    upload_inst(s);
+    AsmArgs a = asm_args(s, dU, dU, dF, nullptr, INFINITY, lambda);
+    launch_assemble(a, s.asm_group, s.ctx->stream);
+  } catch (...) {
+    s.bc = saved;
+    upload_inst(s);
+    throw;
+  }
+  s.bc = saved;
+  upload_inst(s);
+}
+
 void check_dofs(const vfvm_sys &s, int spec) {
   if (spec < 1 || spec > s.nspec) throw Error(VFVM_ERR_ARG, "species number out of range");
 }
@@ -486,6 +522,32 @@ int vfvm_measurement_derivatives(vfvm_sys *s, const double *U0, const double *T,
       } catch (...) {
       }
     }
+    s->ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    s->ctx->err = e.what();
+    return VFVM_ERR_HIP;
+  }
+}
+
+int vfvm_integrate_testfunction(vfvm_sys *s, const double *U, const double *T, double *result) {
+  if (!s || !U || !T || !result) return VFVM_ERR_ARG;
+  try {
+    VFVM_HIP_CHECK(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    s->d_U.upload(U, (size_t)s->ndof, st);
+    // integrate(system, T, U) evaluates at time 0 with embedding parameter 1
+    // (src/vfvm_testfunctions.jl:232, Node(system, 0.0, 1.0))
+    residual_without_bc(*s, s->d_U.p, 1.0, s->d_tmp.p);
+    DevBuf<double> dT, dr;
+    dT.upload(T, (size_t)s->nnodes, st);
+    dr.alloc((size_t)s->nspec);
+    hipLaunchKernelGGL(k_tdot, dim3(1), dim3(ZB), 0, st, s->nnodes, s->nspec, dT.p, s->d_tmp.p, dr.p);
+    VFVM_HIP_CHECK(hipGetLastError());
+    dr.download(result, (size_t)s->nspec, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return VFVM_OK;
+  } catch (const Error &e) {
     s->ctx->err = e.what();
     return e.code;
   } catch (const std::exception &e) {

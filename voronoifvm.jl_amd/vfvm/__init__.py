@@ -13,7 +13,7 @@ from .system import (Batch, NewtonSolverHistory, System, TransientSolution, boun
                      boundary_robin, dofs, enable_species, eval_jacobian, eval_rhs, evaluate_residual_and_jacobian,
                      integrate, mass_matrix, solve, unknowns)
 
-from .impedance import (ImpedanceSystem, assemble_dudp, impedance, measurement_derivatives, parameter_derivative,
-                        testfunction)
+from .impedance import (ImpedanceSystem, assemble_dudp, impedance, integrate_testfunction, measurement_derivatives,
+                        parameter_derivative, testfunction)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -28,6 +28,18 @@ def testfunction(sys: System, bc0, bc1, reltol_linear=1e-12):
     return np.ascontiguousarray(dofs(T), dtype=np.float64)
 
 
+def integrate_testfunction(sys: System, T, U):
+    """integrate(system, T, U) for a stationary solution (src/vfvm_testfunctions.jl:110-130):
+    per species I_flux + I_react - I_src weighted by the test function T; with T from
+    testfunction(sys, bc0, bc1) the flux through the bc1 boundary."""
+    u = _dofvec(U)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    out = np.zeros(sys.num_species)
+    _capi.check(_capi.lib().vfvm_integrate_testfunction(sys.handle(), _capi.dptr(u), _capi.dptr(T),
+                                                        _capi.dptr(out)), sys.ctx.handle)
+    return out
+
+
 def measurement_derivatives(sys: System, U0, T, spec):
     """measurement_derivative (src/vfvm_impedance.jl:235-252) of
     U -> -integrate_stdy(sys, T, U)[spec] and U -> -integrate_tran(sys, T, U)[spec] at U0:
