@@ -52,6 +52,7 @@ struct vfvm_sys {
   int64_t sell_nnz = 0;
   LinearSolver lin;
   int asm_group = 8, spmv_group = 8;
+  int res_group = 8;  // lanes per node of the residual-only assembly
   double *h_pinned = nullptr;  // small pinned readback area
   vfvm_sys() = default;
   vfvm_sys(const vfvm_sys &) = delete;
