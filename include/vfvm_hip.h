@@ -239,11 +239,14 @@ int vfvm_eval_jacobian_device(vfvm_sys *sys, const double *du, double time, doub
  *   complex impedances 1 / (dstdy.UZ + i w dtran.UZ); iters (nullable): iterations per
  *   frequency.  VFVM_ERR_LINEAR if a frequency does not converge. */
 /* integrate(system, T, U) for a stationary solution (src/vfvm_testfunctions.jl:110-130,
- * integrate_stdy 398-410): per species sum_K T_K F_K(U) of the flux, reaction and
- * source terms (no boundary conditions), i.e. I_flux + I_react - I_src; nspec values.
- * With T from testfunction(factory, bc0, bc1) this is the flux through the bc1
- * boundary (the currents of examples/Example161_BipolarDriftDiffusionCurrent.jl). */
-int vfvm_integrate_testfunction(vfvm_sys *sys, const double *U, const double *T, double *result);
+ * integrate_stdy 398-410; Uold NULL, tstep = Inf) and integrate(system, T, U, Uold,
+ * tstep) for an implicit-Euler step (lines 194-210): per species sum_K T_K F_K of the
+ * flux, reaction, source and storage terms (no boundary conditions), i.e.
+ * I_flux + I_react - I_src + (I_stor - I_oldstor) / tstep; nspec values.  With T from
+ * testfunction(factory, bc0, bc1) this is the flux through the bc1 boundary (the
+ * currents of examples/Example161_BipolarDriftDiffusionCurrent.jl). */
+int vfvm_integrate_testfunction(vfvm_sys *sys, const double *U, const double *Uold, double tstep, const double *T,
+                                double *result);
 typedef struct vfvm_impedance vfvm_impedance;
 int vfvm_impedance_create(vfvm_sys *sys, const double *U0, int excited_spec, int excited_bc,
                           vfvm_impedance **out);

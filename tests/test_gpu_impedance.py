@@ -206,3 +206,10 @@ def test_testfunction_integral_matches_oracle_and_closed_form():
     _print("integral", I, "oracle", Io, "exact", -1.0 / math.sinh(1.0))
     assert abs(I - Io) <= 1e-12 * abs(Io)
     assert abs(I + 1.0 / math.sinh(1.0)) <= 1e-3
+    # an implicit-Euler step's integral adds the storage change: oracle with tstep
+    Uold = 0.5 * vfvm.dofs(U0)
+    It = vfvm.integrate_testfunction(sys, T, U0, Uold, 0.1)[0]
+    Ft = nsys.assemble(vfvm.dofs(U0), Uold, tstep=0.1, embed=1.0)
+    Ito = float(T @ Ft)
+    _print("transient integral", It, "oracle", Ito)
+    assert abs(It - Ito) <= 1e-12 * abs(Ito) and abs(It - I) > 1e-3

@@ -358,13 +358,14 @@ __global__ __launch_bounds__(ZB) void k_tdot(int64_t nnodes, int ns, const doubl
 
 // the residual of flux, reaction and source terms alone (every boundary condition
 // switched off) at U, tstep = Inf, into F
-void residual_without_bc(vfvm_sys &s, const double *dU, double lambda, double *dF) {
+void residual_without_bc(vfvm_sys &s, const double *dU, const double *dUold, double tstep, double lambda,
+                         double *dF) {
   std::vector<vfvm_bc_t> saved = s.bc;
   for (auto &c : s.bc) c.kind = VFVM_BC_NONE;
   try {
     upload_inst(s);
-    AsmArgs a = asm_args(s, dU, dU, dF, nullptr, INFINITY, lambda);
-    launch_assemble(a, s.asm_group, s.ctx->stream);
+    AsmArgs a = asm_args(s, dU, dUold, dF, nullptr, tstep, lambda);
+    launch_assemble(a, s.res_group, s.ctx->stream);
   } catch (...) {
     s.bc = saved;
     upload_inst(s);
@@ -530,15 +531,17 @@ int vfvm_measurement_derivatives(vfvm_sys *s, const double *U0, const double *T,
   }
 }
 
-int vfvm_integrate_testfunction(vfvm_sys *s, const double *U, const double *T, double *result) {
-  if (!s || !U || !T || !result) return VFVM_ERR_ARG;
+int vfvm_integrate_testfunction(vfvm_sys *s, const double *U, const double *Uold, double tstep, const double *T,
+                                double *result) {
+  if (!s || !U || !T || !result || !(tstep > 0.0)) return VFVM_ERR_ARG;
   try {
     VFVM_HIP_CHECK(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
     s->d_U.upload(U, (size_t)s->ndof, st);
-    // integrate(system, T, U) evaluates at time 0 with embedding parameter 1
-    // (src/vfvm_testfunctions.jl:232, Node(system, 0.0, 1.0))
-    residual_without_bc(*s, s->d_U.p, 1.0, s->d_tmp.p);
+    s->d_Uold.upload(Uold ? Uold : U, (size_t)s->ndof, st);
+    // integrate(system, T, U[, Uold, tstep]) evaluates at time 0 with embedding
+    // parameter 1 (src/vfvm_testfunctions.jl:232, Node(system, 0.0, 1.0))
+    residual_without_bc(*s, s->d_U.p, s->d_Uold.p, tstep, 1.0, s->d_tmp.p);
     DevBuf<double> dT, dr;
     dT.upload(T, (size_t)s->nnodes, st);
     dr.alloc((size_t)s->nspec);

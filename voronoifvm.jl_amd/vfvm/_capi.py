@@ -84,7 +84,7 @@ SIGNATURES = {
     "vfvm_assemble_dudp": (C.c_int, [_vp, _dp]),
     "vfvm_parameter_derivative": (C.c_int, [_vp, _dp, C.c_double, C.POINTER(ControlT), _dp]),
     "vfvm_measurement_derivatives": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, _dp]),
-    "vfvm_integrate_testfunction": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "vfvm_integrate_testfunction": (C.c_int, [_vp, _dp, _dp, C.c_double, _dp, _dp]),
     "vfvm_impedance_sweep": (C.c_int, [_vp, C.c_int64, _dp, _vp, _vp, C.c_double, C.c_int32, _vp, _vp,
                                        C.POINTER(C.c_int32)]),
     "vfvm_pattern": (C.c_int, [_vp, _lp, _lp]),

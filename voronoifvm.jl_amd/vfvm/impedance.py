@@ -28,15 +28,18 @@ def testfunction(sys: System, bc0, bc1, reltol_linear=1e-12):
     return np.ascontiguousarray(dofs(T), dtype=np.float64)
 
 
-def integrate_testfunction(sys: System, T, U):
-    """integrate(system, T, U) for a stationary solution (src/vfvm_testfunctions.jl:110-130):
-    per species I_flux + I_react - I_src weighted by the test function T; with T from
-    testfunction(sys, bc0, bc1) the flux through the bc1 boundary."""
+def integrate_testfunction(sys: System, T, U, Uold=None, tstep=math.inf):
+    """integrate(system, T, U) for a stationary solution (src/vfvm_testfunctions.jl:110-130),
+    or integrate(system, T, U, Uold, tstep) for an implicit-Euler step (lines 194-210): per
+    species I_flux + I_react - I_src (+ (I_stor - I_oldstor) / tstep) weighted by the test
+    function T; with T from testfunction(sys, bc0, bc1) the flux through the bc1 boundary."""
     u = _dofvec(U)
+    uo = None if Uold is None else _dofvec(Uold)
     T = np.ascontiguousarray(T, dtype=np.float64)
     out = np.zeros(sys.num_species)
-    _capi.check(_capi.lib().vfvm_integrate_testfunction(sys.handle(), _capi.dptr(u), _capi.dptr(T),
-                                                        _capi.dptr(out)), sys.ctx.handle)
+    _capi.check(_capi.lib().vfvm_integrate_testfunction(sys.handle(), _capi.dptr(u),
+                                                        None if uo is None else _capi.dptr(uo), float(tstep),
+                                                        _capi.dptr(T), _capi.dptr(out)), sys.ctx.handle)
     return out
 
 
