@@ -240,6 +240,36 @@ function hip_measurement_derivatives(ctx, sys, U0, T, spec, ndof)
     return ds, dt
 end
 
+"""
+    hip_integrate(ctx, sys, T, U; Uold = nothing, tstep = Inf) -> Vector (one value per species)
+
+integrate(system, T, U) / integrate(system, T, U, Uold, tstep)
+(src/vfvm_testfunctions.jl:110-210): test-function integrals, e.g. terminal currents.
+"""
+function hip_integrate(ctx, sys, T, U, nspec; Uold = nothing, tstep = Inf)
+    out = zeros(nspec)
+    rc = ccall((:vfvm_integrate_testfunction, libvfvm), Cint,
+               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Ptr{Float64}, Ptr{Float64}),
+               sys, U, Uold === nothing ? C_NULL : Uold, tstep, T, out)
+    check(rc, ctx)
+    return out
+end
+
+"""
+    coordsys_code(grid) -> Cint
+
+The VFVM_COORD_* code of grid[CoordinateSystem] (circular_symmetric! /
+spherical_symmetric!); pass it to vfvm_system_create_cs instead of calling
+vfvm_system_create.
+"""
+function coordsys_code(grid)
+    cs = grid[CoordinateSystem]
+    cs <: Polar1D && return Cint(1)
+    cs <: Spherical1D && return Cint(2)
+    cs <: Cylindrical2D && return Cint(3)
+    return Cint(0)
+end
+
 struct StepControlT
     d::Float64; d_min::Float64; d_max::Float64; d_grow::Float64; d_decrease::Float64
     du_opt::Float64; du_max_factor::Float64
