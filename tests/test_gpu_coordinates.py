@@ -125,3 +125,21 @@ def test_cylindrical_geometry_update_equals_fresh_system():
     U = np.random.default_rng(8).uniform(0.0, 1.0, (1, g0.num_nodes))
     assert _same(_asm(s0, U), _asm(s1, U))
     assert np.array_equal(s0.nodevolumes(), s1.nodevolumes())
+
+
+def test_batch_of_cylindrical_systems_matches_single_solves():
+    """Cylindrical systems (Example203's unstructured cylinder with different sources)
+    solved as one batch give the single-system solutions."""
+    _print("case cylindrical batch")
+    g = circular_symmetric(cases.sg_grid("cyl_unstruct.sg"))
+    systems = []
+    for f in (1.0, 2.5, -0.7):
+        s = vfvm.System(g, flux=vfvm.LinearFlux(1.0), reaction=vfvm.SinhReaction(), species=[1],
+                        source=np.full((1, g.num_nodes), f, order="F"))
+        s.boundary_dirichlet(1, 2, 0.0)
+        systems.append(s)
+    c = vfvm.SolverControl(reltol_linear=1e-12, abstol_linear=0.0, maxiters_linear=5000)
+    Ub = vfvm.Batch(systems).solve(0.0, c)
+    for s, ub in zip(systems, Ub):
+        us = vfvm.solve(s, inival=0.0, control=c)
+        assert np.abs(vfvm.dofs(ub) - vfvm.dofs(us)).max() <= 1e-9
