@@ -183,6 +183,9 @@ int vfvm_nodevolumes(vfvm_sys *sys, double *vol);
  * compact factor table is rebuilt on the host only by vfvm_set_compact_graph(sys, 1);
  * until then the assembly reads the FP64 factors. */
 int vfvm_update_grid(vfvm_sys *sys, const double *coord);
+/* The same with the coordinates in device memory (a moving mesh computed on the GPU):
+ * dcoord is a device pointer, dim x nnodes. */
+int vfvm_update_grid_device(vfvm_sys *sys, const double *dcoord);
 /* Where vfvm_system_create builds the node graph, edge factors and node volumes
  * (the cells part of _complete! / update_grid_edgewise!, src/vfvm_system.jl:690-755):
  * 0 on the host, 1 on the GPU, 2 (default) on the GPU for grids of at least 2^18

@@ -175,3 +175,20 @@ def test_device_setup_rejects_bad_cells():
     sys = vfvm.System(bad, flux=vfvm.LinearFlux(1.0), species=[1])
     with pytest.raises(vfvm.VFVMError):
         sys.handle()
+
+
+def test_update_grid_from_device_memory():
+    """The device-memory form gives the host form's factors bit for bit."""
+    _print("case update_grid device memory")
+    g0 = cases.sg_grid("cyl_unstruct.sg")
+    g1 = _moved(g0, 0.15, 21)
+    a, _ = cases.nonlinear_poisson_on(g0)
+    b, _ = cases.nonlinear_poisson_on(g0)
+    for s in (a, b):
+        s.set_compact_graph(False)
+    a.update_grid(g1)
+    d = vfvm.DeviceArray.from_numpy(b.ctx, np.asfortranarray(g1.coord).ravel(order="F"))
+    b.update_grid_device(d)
+    U = np.random.default_rng(12).uniform(-1, 1, (1, g0.num_nodes))
+    assert _same(_asm(a, U), _asm(b, U))
+    assert np.array_equal(a.nodevolumes(), b.nodevolumes())

@@ -710,6 +710,14 @@ int vfvm_update_grid(vfvm_sys *s, const double *coord) {
   });
 }
 
+int vfvm_update_grid_device(vfvm_sys *s, const double *dcoord) {
+  if (!s || !dcoord) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    update_geometry(*s, dcoord, true);
+    return VFVM_OK;
+  });
+}
+
 int vfvm_pattern(vfvm_sys *s, int64_t *colptr, int64_t *rowval) {
   if (!s) return VFVM_ERR_ARG;
   if (colptr)

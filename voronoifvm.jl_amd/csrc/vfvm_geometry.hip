@@ -477,12 +477,12 @@ void launch_geometry(vfvm_sys &s) {
   VFVM_HIP_CHECK(hipGetLastError());
 }
 
-// uploads coord and recomputes; returns the pattern-change flag
-int32_t run_geometry(vfvm_sys &s, const double *coord) {
+// copies coord (host or device memory) and recomputes; returns the pattern-change flag
+int32_t run_geometry(vfvm_sys &s, const double *coord, bool on_device = false) {
   auto &g = s.geo;
   hipStream_t st = s.ctx->stream;
-  VFVM_HIP_CHECK(hipMemcpyAsync(g.coord.p, coord, (size_t)s.dim * s.nnodes * sizeof(double), hipMemcpyHostToDevice,
-                                st));
+  VFVM_HIP_CHECK(hipMemcpyAsync(g.coord.p, coord, (size_t)s.dim * s.nnodes * sizeof(double),
+                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
   VFVM_HIP_CHECK(hipMemsetAsync(g.flag.p, 0, sizeof(int32_t), st));
   if (s.dim == 1)
     launch_geometry<1>(s);
@@ -515,13 +515,13 @@ void build_node_graph_device(vfvm_sys &s, const double *coord, const int64_t *ce
 
 void finish_device_geometry(vfvm_sys &s) { finish_geometry_program(s); }
 
-void update_geometry(vfvm_sys &s, const double *coord) {
+void update_geometry(vfvm_sys &s, const double *coord, bool on_device) {
   VFVM_HIP_CHECK(hipSetDevice(s.ctx->device));
   const size_t nc = (size_t)s.dim * s.nnodes;
   if (!s.geo.built) build_geometry_program(s);
   s.geo.coord.alloc(nc);
   s.geo.cf.alloc((size_t)std::max<int64_t>(s.ncells, 1) * (s.dim + 1 + geom::edges_per_cell(s.dim)));
-  if (const int32_t bad = run_geometry(s, coord)) {
+  if (const int32_t bad = run_geometry(s, coord, on_device)) {
     // restore the factors of the coordinates in use
     run_geometry(s, s.coord.data());
     if (bad & 2) throw Error(VFVM_ERR_ARG, "coordinates must be finite");
@@ -529,7 +529,13 @@ void update_geometry(vfvm_sys &s, const double *coord) {
                 "the new coordinates give an edge that has no Jacobian entry a nonzero factor "
                 "(the sparsity pattern would change); create a new system for them");
   }
-  s.coord.assign(coord, coord + nc);
+  if (on_device) {
+    // the host copy of the coordinates (restores, network features) follows
+    s.geo.coord.download(s.coord.data(), nc, s.ctx->stream);
+    VFVM_HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+  } else {
+    s.coord.assign(coord, coord + nc);
+  }
   s.hs_stale = true;
   s.fac_tab_stale = true;
 }

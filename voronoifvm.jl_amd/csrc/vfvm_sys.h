@@ -75,7 +75,7 @@ AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, do
 void build_node_graph_device(vfvm_sys &s, const double *coord, const int64_t *cellnodes1, int64_t ncells);
 void finish_device_geometry(vfvm_sys &s);
 // new coordinates, same topology: factors recomputed on the device (vfvm_geometry.hip)
-void update_geometry(vfvm_sys &s, const double *coord);
+void update_geometry(vfvm_sys &s, const double *coord, bool on_device = false);
 // brings hs.adj_fac / nodevol / bn_fac up to date after update_geometry
 void sync_host_geometry(vfvm_sys &s);
 }

@@ -77,6 +77,7 @@ SIGNATURES = {
     "vfvm_system_sizes": (C.c_int, [_vp, _lp, _lp, _lp]),
     "vfvm_nodevolumes": (C.c_int, [_vp, _dp]),
     "vfvm_update_grid": (C.c_int, [_vp, _dp]),
+    "vfvm_update_grid_device": (C.c_int, [_vp, _vp]),
     "vfvm_ctx_set_setup": (C.c_int, [_vp, C.c_int]),
     "vfvm_impedance_create": (C.c_int, [_vp, _dp, C.c_int, C.c_int, C.POINTER(_vp)]),
     "vfvm_impedance_destroy": (C.c_int, [_vp]),

@@ -216,6 +216,11 @@ class System:
         _capi.check(_capi.lib().vfvm_eval_jacobian_device(self.handle(), u.ptr, float(t), nzval_csr.ptr),
                     self.ctx.handle)
 
+    def update_grid_device(self, coord):
+        """update_grid! with the new coordinates (dim x nnodes, column-major) in a
+        DeviceArray; the cells and boundary faces stay those of the system's grid."""
+        _capi.check(_capi.lib().vfvm_update_grid_device(self.handle(), coord.ptr), self.ctx.handle)
+
     def update_grid(self, grid: Grid):
         """update_grid!(system; grid) (src/vfvm_system.jl:607-612): `grid` has the cells and
         boundary faces of the system's grid and new coordinates; the form factors are
