@@ -329,6 +329,18 @@ int vfvm_solve_transient(vfvm_sys *sys, const double *inival, int64_t nlambdas, 
 int vfvm_residual_norm(vfvm_sys *sys, const double *U, const double *Uold, double time,
                        double tstep, double lambda, double *norm);
 
+/* The same norm with U and Uold in device memory (dUold NULL: dU), written to dnorm (one
+ * double in device memory), and NO allocation: no hipMalloc, hipFree or allocating library
+ * call.  No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * The residual it reduces lives in a static device array of VFVM_NOALLOC_MAX_DOF doubles
+ * (32 MiB per GPU, held until the process ends); a system with more dofs gets VFVM_ERR_ARG.
+ * The call is enqueued on the handle's stream and returns without synchronising.  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU race. */
+#define VFVM_NOALLOC_MAX_DOF ((int64_t)1 << 22)
+int vfvm_residual_norm_device_noalloc(vfvm_sys *sys, const double *dU, const double *dUold, double time,
+                                      double tstep, double lambda, double *dnorm);
+
 /* Autotune the lanes-per-node / lanes-per-row group size of the assembly and SpMV
  * kernels for this system (project-written: times each legal size on the device).
  * Assembly groups: 2..32 lanes per node; 1 = one node per lane with the block's

@@ -458,6 +458,7 @@ int vfvm_ctx_create(int device, vfvm_ctx **out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(VFVM_ERR_HIP, "no HIP device available");
     if (device < 0 || device >= n) throw Error(VFVM_ERR_ARG, "device index out of range");
     VFVM_HIP_CHECK(hipSetDevice(device));
+    noalloc_setup(device);
     auto *c = new vfvm_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -78,4 +78,6 @@ void finish_device_geometry(vfvm_sys &s);
 void update_geometry(vfvm_sys &s, const double *coord, bool on_device = false);
 // brings hs.adj_fac / nodevol / bn_fac up to date after update_geometry
 void sync_host_geometry(vfvm_sys &s);
+// device address of the no-alloc entry point's static array on `device` (current), once per device
+void noalloc_setup(int device);
 }

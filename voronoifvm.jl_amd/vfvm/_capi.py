@@ -101,6 +101,7 @@ SIGNATURES = {
                                        C.POINTER(StepControlT), _dp, C.c_int64, _dp, _dp, C.POINTER(C.c_int64),
                                        C.POINTER(HistoryT)]),
     "vfvm_residual_norm": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
+    "vfvm_residual_norm_device_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp]),
     "vfvm_autotune": (C.c_int, [_vp, _ip, _ip]),
     "vfvm_set_groups": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "vfvm_set_sell_index": (C.c_int, [_vp, C.c_int]),
