@@ -36,6 +36,7 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# oracle/ and tests/ serve the cpu_baseline legs only (the checker and its problem twins)
 for _p in (os.path.join(ROOT, "voronoifvm.jl_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):
     if _p not in sys.path:
         sys.path.insert(0, _p)
@@ -104,8 +105,8 @@ def run_cfg0(args):
     """The reference's CPU-runnable case: 101 x 101 nonlinear Poisson (eps = 1e-2, f = 1,
     Dirichlet 0 on west/south and 1 on east/north, zero initial value), plain FP64 device
     Newton with BiCGStab + ILU(0); a step is one solve."""
-    import cases
-    s, _ = cases.nonlinear_poisson_2d(101, oracle=False)
+    from vfvm import problems
+    s = problems.nonlinear_poisson_2d(101)
     s.handle()
     s.autotune()
     ctrl = vfvm.SolverControl(precon="ilu0", reltol_linear=1e-8, abstol_linear=0.0, maxiters_linear=1000)
@@ -298,13 +299,13 @@ def run_cfg2(args):
     """Bipolar drift-diffusion on 512 x 512 nodes: a bias sweep 0..1 V in 20 continuation
     steps (21 Newton solves, the first at zero bias), BiCGStab + 3x3 block Jacobi (GMRES(30)
     with block Jacobi stalls at this size: its restarted Krylov space is too small)."""
-    import cases
+    from vfvm import problems
     n = 512
     t0 = time.perf_counter()
-    s, _ = cases.drift_diffusion_2d(n, lambda2=1e-4, oracle=False)
+    s = problems.drift_diffusion_2d(n, lambda2=1e-4)
     s.handle()
     ndof, nnz, nedges = s.sizes()
-    u0 = cases.dd_initial(s.grid)
+    u0 = problems.dd_initial(s.grid)
     ag, sg = s.autotune()
     log(f"cfg2: {n}^2 grid, ndof={ndof} nnz={nnz} edges={nedges}, setup {time.perf_counter() - t0:.1f}s, "
         f"autotuned groups {ag}/{sg}")
@@ -369,8 +370,8 @@ def cpu_cfg2(args):
 
 # --------------------------------------------------------------------------- cfg3
 def cfg3_members(count=256):
-    import cases
-    return [cases.batch_member(seed) for seed in range(count)]
+    from vfvm import problems
+    return [problems.batch_member(seed) for seed in range(count)]
 
 
 def build_poisson(n, eps, vals, f):
@@ -698,23 +699,54 @@ def run_cfg4(args):
 
 
 def cpu_cfg4(args):
-    """The oracle on a bounded sample: 16 held-out seeds from zero, direct solves."""
-    import oracle as O
-    from vfvm import surrogate as S
+    """The oracle on bounded samples of the held-out seeds, Newton from zero with banded
+    direct solves: one thread, then every core of the host's share (independent solves
+    on a thread pool; the oracle's ctypes calls release the GIL).  `value` is the
+    all-core rate."""
+    from concurrent.futures import ThreadPoolExecutor
     import cases
+    from vfvm import surrogate as S
     grf = S.GRF()
-    t = 0.0
-    seeds = list(S.EVAL_SEEDS)[:16]
-    for sd in seeds:
+    seeds = list(S.EVAL_SEEDS)
+
+    def build(sd):
         f, vals = S.draw(sd, grf)
-        s, o = cases.nonlinear_poisson_2d(S.GRID_N, S.EPS, 0.0, tuple(vals))
+        _, o = cases.nonlinear_poisson_2d(S.GRID_N, S.EPS, 0.0, tuple(vals))
         o.set_source(f)
-        t0 = time.perf_counter()
-        o.solve_step(np.zeros(s.grid.num_nodes))
-        t += time.perf_counter() - t0
-    return dict(value=len(seeds) / t, unit="solves/s", cores=1, kind="port",
-                sample=f"{len(seeds)} of the 1,024 held-out cfg4 seeds, oracle Newton from zero with banded "
-                       f"direct solves, {t:.2f} s")
+        return o
+
+    def run(o):
+        o.solve_step(np.zeros(S.GRID_N * S.GRID_N))
+
+    n1 = 8
+    systems = [build(sd) for sd in seeds[:n1]]
+    t0 = time.perf_counter()
+    for o in systems:
+        run(o)
+    t1 = time.perf_counter() - t0
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))  # the host share of one GPU
+    nall = 4 * cores
+    systems = [build(sd) for sd in seeds[:nall]]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(run, systems))
+    tn = time.perf_counter() - t0
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=nall / tn, unit="solves/s", cores=cores, kind="port", single_thread_value=n1 / t1,
+                cpu_model=model,
+                sample=f"held-out cfg4 seeds, oracle Newton from zero with banded direct solves: {nall} seeds on "
+                       f"{cores} threads in {tn:.2f} s; {n1} seeds on one thread in {t1:.2f} s")
 
 
 # --------------------------------------------------------------------------- copy
@@ -757,11 +789,16 @@ IMP_OMEGAS = 1e-3 * 1.1 ** np.arange(113)  # Example150's sweep: w from 1e-3 by 
 def build_impedance(n, oracle=False):
     """Example150's problem on the unit square: C u_t - div(D grad u) + R u = 0 (C = D = R = 1),
     u = 1 on the west side (region 4, excited), u = 0 on the east side (region 2, measured)."""
-    import cases
     X = vfvm.linspace(0.0, 1.0, n)
     g = vfvm.simplexgrid(X, X)
     phys = dict(flux=vfvm.LinearFlux(1.0), reaction=vfvm.PowerReaction(m=1, k=1.0), storage=vfvm.IdentityStorage())
-    s, o = cases.pair(g, 1, phys, legacy=[(1, 4, 1.0), (1, 2, 0.0)], oracle=oracle)
+    s = vfvm.System(g, vfvm.Physics(**phys), species=[1])
+    s.boundary_dirichlet(1, 4, 1.0)
+    s.boundary_dirichlet(1, 2, 0.0)
+    o = None
+    if oracle:  # the checker's twin (the CPU baseline leg only)
+        import cases
+        o = cases.oracle_twin(s)
     return g, s, o, vfvm.Physics(**phys)
 
 

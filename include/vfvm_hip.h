@@ -16,8 +16,9 @@
  *    i.e. the reference's dofs(U) order: dof(K, i) = (K-1)*nspec + i
  *    (src/vfvm_system.jl:808-811, src/vfvm_densesolution.jl:49,68).
  *  - Matrices cross the boundary as CSC with 1-based int64 indices and rows
- *    sorted within columns, exactly what SparseMatrixCSC(state.matrix) gives
- *    (src/vfvm_linsolve.jl:3-4).  Inside, the device keeps CSR + a slot map.
+ *    sorted within columns, what SparseMatrixCSC(state.matrix) gives for the
+ *    system's assembly variant (src/vfvm_linsolve.jl:3-4; VFVM_ASSEMBLY_EDGEWISE
+ *    or _CELLWISE).  Inside, the device keeps CSR + a slot map.
  *  - Host pointers are copied in and out; no pointer is retained after a call.
  *    Device buffers belong to the handle and are freed by its destroy call.
  *  - Every call returns a status: 0 OK, or one of the VFVM_ERR_* codes that
@@ -100,6 +101,9 @@ typedef struct {
   int32_t maxiters, max_round, updatecontrol, is_linear;
   double reltol_linear, abstol_linear;
   int32_t maxiters_linear, krylov, precon, precision;
+  /* reuse_precon = !factorize_every_newtonstep (src/vfvm_solvercontrol.jl:123-131): the
+   * ILU(0) factor of a Newton solve's first step serves its later steps (reuse_precs,
+   * src/vfvm_solver.jl:99); default 1 */
   int32_t gmres_restart, max_refinements, reuse_precon, check_every;
   double inner_reltol; /* FP32 inner solve tolerance per refinement sweep */
 } vfvm_control_t;
@@ -165,6 +169,66 @@ int vfvm_system_create_cs(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *
                           const int64_t *bfacenodes, const int64_t *bfaceregions, int nspec,
                           const vfvm_physics_t *physics, int nbregions, const vfvm_bc_t *bc, int coordsys,
                           vfvm_sys **out);
+/* Species activity (enable_species! / enable_boundary_species!, src/vfvm_system.jl:433-515):
+ * BULK species are enabled in every cell region; a BOUNDARY species lives on the nodes of
+ * the boundary faces of the regions whose bits (region r: bit r-1) are set in bregions; a
+ * NONE species is enabled nowhere.  Dofs of a species not active at a node get identity
+ * rows, F = U - Uold, and are set to 0 at the start of each Newton solve
+ * (_eval_and_assemble_inactive_species / _initialize_inactive_dof!, src/vfvm_system.jl:
+ * 1010-1042, dense unknown storage).  A bulk species enabled in only some of several cell
+ * regions is not expressible here. */
+#define VFVM_SPECIES_BULK 0
+#define VFVM_SPECIES_BOUNDARY 1
+#define VFVM_SPECIES_NONE 2
+typedef struct {
+  int32_t kind;
+  uint32_t bregions;
+} vfvm_species_t;
+
+/* Boundary physics of one boundary region: the registry's breaction, bstorage, bsource and
+ * bflux callbacks (src/vfvm_physics.jl:67-182; assembled by assemble_bnodes /
+ * assemble_bedges, src/vfvm_assembly.jl:318-510):
+ *   breaction  f_i = sum_j react[i][j] u_j (+ bcoef[i,K] u_i where `fields` is set)
+ *              (examples/Example220_NonlinearPoisson2D_BoundarySpecies.jl:25-33,
+ *              Example311_HeatEquation_BoundaryDiffusion.jl:66-75)
+ *   bsource    bsrc[i,K] where `fields` is set; the item adds fac (breaction - bsource)
+ *   bstorage   f_i = u_i for the species bits i of `storage` (Example220:33-37)
+ *   bflux      f_i = bflux[i] (u_i,K - u_i,L) on the region's boundary edges, with the
+ *              boundary-face edge factors (examples/Example230_BoundaryFlux.jl:125-130)
+ * Terms act on the species active at the node (the edge: at both nodes).  These add to the
+ * boundary-condition table of vfvm_bc_t. */
+typedef struct {
+  double react[3][3];
+  double bflux[3];
+  int32_t storage, fields;
+} vfvm_bphysics_t;
+
+/* Assembly variant of System(grid; assembly), src/vfvm_system.jl:215-227: EDGEWISE sums the
+ * cell factors per edge first and stores no Jacobian entry for an edge whose summed factor
+ * is zero (update_grid_edgewise!, lines 690-755); CELLWISE (the reference's default)
+ * stores an entry for every cell edge (_addnz tests the flux derivative, not the factor,
+ * src/vfvm_assembly.jl:20-28), so its pattern is that of SparseMatrixCSC(state.matrix) of
+ * a cellwise system; its values equal the cellwise sums to rounding. */
+#define VFVM_ASSEMBLY_EDGEWISE 0
+#define VFVM_ASSEMBLY_CELLWISE 1
+
+/* System creation with species activity (species: nspec entries, NULL = all BULK), boundary
+ * physics (bphys: nbregions entries, NULL = none) and the assembly variant; the rest as
+ * vfvm_system_create_cs.  Systems with boundary species, inactive dofs or boundary physics
+ * run an extra boundary pass after the bulk assembly; they cannot join a batch. */
+int vfvm_system_create_ex(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coord, int64_t ncells,
+                          const int64_t *cellnodes, const int64_t *cellregions, int64_t nbfaces,
+                          const int64_t *bfacenodes, const int64_t *bfaceregions, int nspec,
+                          const vfvm_physics_t *physics, int nbregions, const vfvm_bc_t *bc, int coordsys,
+                          const vfvm_species_t *species, const vfvm_bphysics_t *bphys, int assembly,
+                          vfvm_sys **out);
+/* Per-node fields, nspec x nnodes each (NULL clears): rcoef adds rcoef[i,K] u_i to the bulk
+ * reaction (a reaction callback reading per-node data, examples/Example230_BoundaryFlux.jl:
+ * 63-66); bsrc and bcoef are the boundary source and boundary reaction coefficient of the
+ * regions whose vfvm_bphysics_t.fields is set. */
+int vfvm_system_set_fields(vfvm_sys *sys, const double *rcoef, const double *bsrc, const double *bcoef);
+/* node_dof activity: bit i of mask[K] is set if species i is active at node K (nnodes bytes) */
+int vfvm_node_species(vfvm_sys *sys, uint8_t *mask);
 int vfvm_system_destroy(vfvm_sys *sys);
 /* per-node source term, nspec x nnodes (Physics(source = array), src/vfvm_system.jl:312-319); NULL clears */
 int vfvm_system_set_source(vfvm_sys *sys, const double *src);
@@ -195,7 +259,9 @@ int vfvm_update_grid_device(vfvm_sys *sys, const double *dcoord);
 int vfvm_ctx_set_setup(vfvm_ctx *ctx, int mode);
 
 /* Jacobian pattern as CSC, 1-based, rows sorted: what SparseMatrixCSC(state.matrix)
- * exposes (src/vfvm_linsolve.jl:3-4, src/vfvm_state.jl:136).  colptr has ndof+1 entries. */
+ * exposes for the system's assembly variant (src/vfvm_linsolve.jl:3-4,
+ * src/vfvm_state.jl:136; vfvm_system_create and _cs give the edgewise pattern,
+ * vfvm_system_create_ex either).  colptr has ndof+1 entries. */
 int vfvm_pattern(vfvm_sys *sys, int64_t *colptr, int64_t *rowval);
 
 /* Dirichlet initialisation of U (in place): _initialize_dirichlet!, src/vfvm_system.jl:947-1003 */
@@ -267,11 +333,26 @@ int vfvm_impedance_sweep(vfvm_impedance *imp, int64_t nomega, const double *omeg
  * boundary values value + value_lambda * lambda and, with p[12] set, the source
  * lambda * f -- the registry's form of `nparams = 1, params = [lambda]`.
  * vfvm_assemble_dudp: state.dudp[1] = dF/dlambda (src/vfvm_assembly.jl:116-120,
- *   194-198, 398-405), ndof values (exact; independent of U and lambda here).
+ *   194-198, 398-405), ndof values, of the boundary values and the embedded source (exact;
+ *   independent of U and lambda); with lambda-dependent coefficients use
+ *   vfvm_assemble_dudp_at.
  * vfvm_parameter_derivative: the sensitivity du/dlambda = -J(U)^-1 dF/dlambda of a
  *   stationary solution U at lambda (what Example430_ParameterDerivativesStationary.jl
- *   obtains by dual numbers through the solve), with the linear solver of `control`. */
+ *   obtains by dual numbers through the solve), with the linear solver of `control`;
+ *   dF/dlambda includes the lambda-dependent coefficients. */
 int vfvm_assemble_dudp(vfvm_sys *sys, double *dudp);
+/* Registry coefficients that depend on the parameter lambda (the reference's params reaching
+ * the flux and reaction callbacks, src/vfvm_assembly.jl:116-120, 194-198;
+ * examples/Example430_ParameterDerivativesStationary.jl:142-160): slot j of the physics
+ * block is p[j] + dp[j] lambda in every assembly at lambda (dp: 16 values, NULL clears;
+ * the integer exponents p[4], p[6] cannot depend on lambda). */
+int vfvm_system_set_param_dependence(vfvm_sys *sys, const double *dp);
+/* state.dudp[1] = dF/dlambda at U and lambda: the boundary values and embedded source as
+ * vfvm_assemble_dudp, plus the flux and reaction terms through the lambda-dependent
+ * coefficients, evaluated with dual-number parameters.  with_bc = 0 leaves out the
+ * boundary conditions (the derivative of integrate(system, T, U; params), the
+ * test-function integral, src/vfvm_testfunctions.jl:110-130). */
+int vfvm_assemble_dudp_at(vfvm_sys *sys, const double *U, double lambda, int with_bc, double *dudp);
 int vfvm_parameter_derivative(vfvm_sys *sys, const double *U, double lambda, const vfvm_control_t *control,
                               double *dudl);
 

@@ -47,6 +47,15 @@ class Control(C.Structure):
                 ("force_first_step", C.c_int32)]
 
 
+class BPhysics(C.Structure):
+    """orc_bphysics: breaction matrix, bflux coefficients, bstorage bits, fields flag."""
+    _fields_ = [("react", (C.c_double * 3) * 3), ("bflux", C.c_double * 3), ("storage", C.c_int32),
+                ("fields", C.c_int32)]
+
+
+SPEC = {"bulk": 0, "boundary": 1, "none": 2}
+
+
 class History(C.Structure):
     _fields_ = [("niter", C.c_int32), ("nlin_total", C.c_int32), ("nlu", C.c_int32),
                 ("converged", C.c_int32), ("last_update_norm", C.c_double),
@@ -69,6 +78,12 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_set_source.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_set_bc.argtypes = [C.c_void_p, C.POINTER(BCEntry)]
+        L.orc_set_species.argtypes = [C.c_void_p, ip, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]
+        L.orc_set_bphysics.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_bfields.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_rcoef.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_param_dependence.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_node_species.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")]
         L.orc_last_error.restype = C.c_char_p
         L.orc_last_error.argtypes = [C.c_void_p]
         L.orc_num_edges.argtypes = [C.c_void_p]
@@ -120,7 +135,54 @@ def physics(flux="none", reaction="none", storage="none", **params) -> Physics:
 class OracleSystem:
     """One restated VoronoiFVM system (grid + physics + BC table)."""
 
-    def __init__(self, grid, nspec, phys: Physics, bcs=None, assembly=EDGEWISE, source=None):
+    def __init__(self, grid, nspec, phys: Physics, bcs=None, assembly=EDGEWISE, source=None, species=None,
+                 bphysics=None, bfields=None):
+        """species: per species ("bulk",) / ("boundary", [bregions]) / ("none",) (default: all
+        bulk); bphysics: {bregion: dict(react=3x3, bflux=3, storage=[species 0-based], fields=bool)};
+        bfields: (bsrc, bcoef), each nspec x nnodes species fastest or None."""
+        self._init(grid, nspec, phys, bcs, assembly, source)
+        if species is not None:
+            kind = np.zeros(nspec, np.int32)
+            breg = np.zeros(nspec, np.uint32)
+            for i, sp in enumerate(species):
+                kind[i] = SPEC[sp[0]]
+                for r in (sp[1] if len(sp) > 1 else ()):
+                    breg[i] |= np.uint32(1 << (r - 1))
+            lib().orc_set_species(self._h, kind, breg)
+        if bphysics is not None:
+            arr = (BPhysics * self.nbreg)()
+            for r, d in bphysics.items():
+                e = arr[r - 1]
+                for i, row in enumerate(d.get("react", ())):
+                    for j, v in enumerate(row):
+                        e.react[i][j] = float(v)
+                for i, v in enumerate(d.get("bflux", ())):
+                    e.bflux[i] = float(v)
+                for i in d.get("storage", ()):
+                    e.storage |= 1 << i
+                e.fields = int(bool(d.get("fields", False)))
+            self._bphys = arr
+            lib().orc_set_bphysics(self._h, C.cast(arr, C.c_void_p))
+        if bfields is not None:
+            self._bf = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel() for a in bfields]
+            lib().orc_set_bfields(self._h, *[None if a is None else a.ctypes.data for a in self._bf])
+
+    def set_rcoef(self, rcoef):
+        """per-node bulk reaction coefficient (nspec x nnodes, species fastest) or None"""
+        self._rc = None if rcoef is None else np.ascontiguousarray(rcoef, dtype=np.float64).ravel()
+        lib().orc_set_rcoef(self._h, None if self._rc is None else self._rc.ctypes.data)
+
+    def set_param_dependence(self, dp):
+        """registry parameter slots p_j(lambda) = p_j + dp[j] lambda (16 values) or None"""
+        self._dp = None if dp is None else np.ascontiguousarray(dp, dtype=np.float64)
+        lib().orc_set_param_dependence(self._h, None if self._dp is None else self._dp.ctypes.data)
+
+    def node_species(self):
+        m = np.zeros(self.grid.num_nodes, np.uint8)
+        lib().orc_node_species(self._h, m)
+        return m
+
+    def _init(self, grid, nspec, phys, bcs, assembly, source):
         L = lib()
         self.grid = grid
         self.nspec = nspec

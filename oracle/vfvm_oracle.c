@@ -359,8 +359,10 @@ void orc_cellfactors(int dim, const double *coord, const int32_t *n, double *npa
   }
 }
 
-/* bfacefactors!, src/vfvm_formfactors.jl:244-247 (1D), 263-274 (2D), 293-332 (3D) */
-void orc_bfacefactors(int dim, const double *coord, const int32_t *n, double *npar) {
+/* bfacefactors!, src/vfvm_formfactors.jl:244-247 (1D), 263-274 (2D), 293-332 (3D): node
+   factors npar and (nullable) the edge factors epar of the face's own edges (2D: the one
+   edge, 1/length; 3D: edge i opposite node i; 1D: none) */
+static void bfacefactors_all(int dim, const double *coord, const int32_t *n, double *npar, double *eout) {
   if (dim == 1) {
     npar[0] = 1.0;
     return;
@@ -371,6 +373,7 @@ void orc_bfacefactors(int dim, const double *coord, const int32_t *n, double *np
     double d = sqrt(dx * dx + dy * dy);
     npar[0] = d / 2;
     npar[1] = d / 2;
+    if (eout) eout[0] = 1 / d;
     return;
   }
   double epar[3] = {0, 0, 0};
@@ -391,6 +394,12 @@ void orc_bfacefactors(int dim, const double *coord, const int32_t *n, double *np
     npar[EN_TRI[i][0]] += epar[i] * d * 0.25;
     npar[EN_TRI[i][1]] += epar[i] * d * 0.25;
   }
+  if (eout)
+    for (int i = 0; i < 3; i++) eout[i] = epar[i] * d / dd[i];
+}
+
+void orc_bfacefactors(int dim, const double *coord, const int32_t *n, double *npar) {
+  bfacefactors_all(dim, coord, n, npar, NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -537,6 +546,20 @@ struct orc_sys {
   int nedges;
   int32_t *edgenodes;
   regfac_t enf, eef;
+  /* species activity and boundary physics (src/vfvm_system.jl:433-515, 1010-1042) */
+  int homogeneous;        /* every species BULK: no activity checks, no inactive rows */
+  uint8_t bulkmask;       /* species enabled in the cell regions (isregionspecies) */
+  uint8_t *nodemask;      /* per node: active species (node_dof) */
+  orc_bphysics *bphys;    /* nbreg entries or NULL */
+  int has_bflux, has_bstorage;
+  double *bsrc, *bcoef;   /* per-node boundary fields (nullable) */
+  double *rcoef;          /* per-node bulk reaction coefficient (nullable) */
+  /* registry parameters that depend on the system parameter lambda: p_j + dp_j lambda
+     (params of src/vfvm_assembly.jl:116-120, 194-198 for the registry's coefficients) */
+  double dp[ORC_NPARAM];
+  int has_dp;
+  orc_physics peff; /* the parameters at the lambda of the current assembly */
+  double *bedgefac;       /* boundary-face edge factors: (dim == 2 ? 1 : 3) x nbfaces */
   /* state */
   lnk_t mat;
   int have_factorization;
@@ -758,9 +781,95 @@ orc_sys *orc_create(int dim, int nnodes, const double *coord, int ncells, const 
     free(ord);
     free(newid);
   }
+  /* boundary-face edge factors for bflux (src/vfvm_system.jl:667-677) */
+  {
+    int nbe = dim == 3 ? 3 : 1;
+    double np3[3];
+    s->bedgefac = (double *)calloc((size_t)nbe * (nbfaces ? nbfaces : 1), sizeof(double));
+    if (dim >= 2)
+      for (int ib = 0; ib < nbfaces; ib++) bfacefactors_all(dim, s->coord, s->bfacenodes + dim * ib, np3, s->bedgefac + nbe * ib);
+  }
+  s->homogeneous = 1;
+  s->bulkmask = (uint8_t)((1u << nspec) - 1u);
+  s->nodemask = (uint8_t *)malloc((size_t)(nnodes ? nnodes : 1));
+  memset(s->nodemask, s->bulkmask, (size_t)(nnodes ? nnodes : 1));
   lnk_init(&s->mat, nnodes * nspec);
   return s;
 }
+
+/* enable_species! / enable_boundary_species! (src/vfvm_system.jl:433-515): node_dof[i, K]
+   is set for the nodes of the cells of the species' regions (here: every cell region) and
+   for the nodes of the boundary faces of a boundary species' regions */
+int orc_set_species(orc_sys *s, const int32_t *kind, const uint32_t *bregions) {
+  uint8_t bulk = 0;
+  int homog = 1;
+  for (int i = 0; i < s->nspec; i++) {
+    if (kind[i] == ORC_SPEC_BULK)
+      bulk |= (uint8_t)(1u << i);
+    else
+      homog = 0;
+  }
+  s->bulkmask = bulk;
+  s->homogeneous = homog;
+  memset(s->nodemask, bulk, (size_t)(s->nnodes ? s->nnodes : 1));
+  for (int i = 0; i < s->nspec; i++) {
+    if (kind[i] != ORC_SPEC_BOUNDARY) continue;
+    for (int ib = 0; ib < s->nbfaces; ib++) {
+      int r = s->bfaceregions[ib];
+      if (r < 1 || r > 32 || !((bregions[i] >> (r - 1)) & 1u)) continue;
+      for (int k = 0; k < s->dim; k++) s->nodemask[s->bfacenodes[s->dim * ib + k]] |= (uint8_t)(1u << i);
+    }
+  }
+  return 0;
+}
+
+void orc_set_bphysics(orc_sys *s, const orc_bphysics *bp) {
+  free(s->bphys);
+  s->bphys = NULL;
+  s->has_bflux = s->has_bstorage = 0;
+  if (!bp) return;
+  s->bphys = (orc_bphysics *)malloc(sizeof(orc_bphysics) * (size_t)(s->nbreg ? s->nbreg : 1));
+  memcpy(s->bphys, bp, sizeof(orc_bphysics) * (size_t)s->nbreg);
+  for (int r = 0; r < s->nbreg; r++) {
+    for (int i = 0; i < s->nspec; i++) s->has_bflux |= (bp[r].bflux[i] != 0.0);
+    s->has_bstorage |= (bp[r].storage != 0);
+  }
+}
+
+void orc_set_bfields(orc_sys *s, const double *bsrc, const double *bcoef) {
+  size_t n = sizeof(double) * (size_t)s->nspec * s->nnodes;
+  free(s->bsrc);
+  free(s->bcoef);
+  s->bsrc = s->bcoef = NULL;
+  if (bsrc) {
+    s->bsrc = (double *)malloc(n);
+    memcpy(s->bsrc, bsrc, n);
+  }
+  if (bcoef) {
+    s->bcoef = (double *)malloc(n);
+    memcpy(s->bcoef, bcoef, n);
+  }
+}
+
+void orc_set_param_dependence(orc_sys *s, const double *dp) {
+  s->has_dp = 0;
+  for (int j = 0; j < ORC_NPARAM; j++) {
+    s->dp[j] = dp ? dp[j] : 0.0;
+    s->has_dp |= s->dp[j] != 0.0;
+  }
+}
+
+void orc_set_rcoef(orc_sys *s, const double *rcoef) {
+  free(s->rcoef);
+  s->rcoef = NULL;
+  if (rcoef) {
+    size_t n = sizeof(double) * (size_t)s->nspec * s->nnodes;
+    s->rcoef = (double *)malloc(n);
+    memcpy(s->rcoef, rcoef, n);
+  }
+}
+
+void orc_node_species(const orc_sys *s, uint8_t *mask) { memcpy(mask, s->nodemask, (size_t)s->nnodes); }
 
 void orc_destroy(orc_sys *s) {
   if (!s) return;
@@ -782,6 +891,12 @@ void orc_destroy(orc_sys *s) {
   free(s->eef.reg);
   free(s->eef.fac);
   free(s->lu_band);
+  free(s->nodemask);
+  free(s->bphys);
+  free(s->bsrc);
+  free(s->bcoef);
+  free(s->rcoef);
+  free(s->bedgefac);
   lnk_free(&s->mat);
   free(s);
 }
@@ -853,14 +968,22 @@ static int asm_node(orc_sys *s, int K, double fac, const double *U, const double
     u[i] = dvar(U[K * ns + i], i);
     uo[i] = dc(Uold[K * ns + i]);
   }
-  phys_reaction(&s->phys, ns, u, r);
-  phys_storage(&s->phys, ns, u, st);
-  phys_storage(&s->phys, ns, uo, so);
+  phys_reaction(&s->peff, ns, u, r);
+  if (s->rcoef)
+    for (int i = 0; i < ns; i++) r[i] = dadd(r[i], dscale(u[i], s->rcoef[K * ns + i]));
+  phys_storage(&s->peff, ns, u, st);
+  phys_storage(&s->peff, ns, uo, so);
+  /* assemble_res_jac(node): rows and columns of the region's species only
+     (src/vfvm_assemblydata.jl:245-270); a species is in a cell region's list when
+     enable_species! named the region (here: every cell region, the BULK kind) */
+  const uint8_t bm = s->bulkmask;
   for (int i = 0; i < ns; i++) {
+    if (!((bm >> i) & 1u)) continue;
     double src = s->source ? s->source[K * ns + i] : 0.0;
     if (s->phys.p[ORC_P_EMBED_SOURCE] != 0.0) src *= lambda;
     F[K * ns + i] += fac * (r[i].v - src + (st[i].v - so[i].v) * tstepinv);
     for (int j = 0; j < ns; j++) {
+      if (!((bm >> j) & 1u)) continue;
       int rc = addnz(s, K * ns + i, K * ns + j, r[i].d[j] + st[i].d[j] * tstepinv, fac);
       if (rc) return rc;
     }
@@ -876,12 +999,17 @@ static int asm_edge(orc_sys *s, int K, int L, double fac, const double *U, doubl
     uk[i] = dvar(U[K * ns + i], i);
     ul[i] = dvar(U[L * ns + i], ns + i);
   }
-  phys_flux(&s->phys, ns, uk, ul, g);
+  phys_flux(&s->peff, ns, uk, ul, g);
+  /* assemble_res_jac(edge), src/vfvm_assemblydata.jl:349-379: region species, present
+     at both nodes (the BULK kind is present at every node) */
+  const uint8_t bm = s->bulkmask;
   for (int i = 0; i < ns; i++) {
+    if (!((bm >> i) & 1u)) continue;
     double val = fac * g[i].v;
     F[K * ns + i] += val;
     F[L * ns + i] += -val;
     for (int j = 0; j < ns; j++) {
+      if (!((bm >> j) & 1u)) continue;
       int rc = 0;
       rc |= addnz(s, K * ns + i, K * ns + j, +g[i].d[j], fac);
       rc |= addnz(s, L * ns + i, K * ns + j, -g[i].d[j], fac);
@@ -894,8 +1022,8 @@ static int asm_edge(orc_sys *s, int K, int L, double fac, const double *U, doubl
 }
 
 /* one boundary node item: src/vfvm_assembly.jl:352-407 */
-static int asm_bnode(orc_sys *s, int K, int breg, double fac, const double *U, double lambda,
-                     double *F) {
+static int asm_bnode(orc_sys *s, int K, int breg, double fac, const double *U, const double *Uold,
+                     double tstepinv, double lambda, double *F) {
   int ns = s->nspec;
   double penalty = DIRICHLET_PENALTY; /* bnode.Dirichlet, src/vfvm_geometryitems.jl:296 */
   if (s->has_legacy) {
@@ -917,10 +1045,18 @@ static int asm_bnode(orc_sys *s, int K, int breg, double fac, const double *U, d
       }
     }
   }
-  /* breaction with boundary_dirichlet! / boundary_robin! (src/vfvm_physics.jl:487-494, 552) */
+  /* breaction with boundary_dirichlet! / boundary_robin! (src/vfvm_physics.jl:487-494, 552),
+     the region's linear boundary reaction and per-node coefficient, minus the boundary
+     source (src/vfvm_assembly.jl:385-407) */
+  const orc_bphysics *bp = (s->bphys && breg >= 1 && breg <= s->nbreg) ? &s->bphys[breg - 1] : NULL;
   dual y[ORC_MAXSPEC];
+  double bsrc[ORC_MAXSPEC];
   int nontrivial = 0;
-  for (int i = 0; i < ns; i++) y[i] = dc(0.0);
+  for (int i = 0; i < ns; i++) {
+    y[i] = dc(0.0);
+    bsrc[i] = (bp && bp->fields && s->bsrc) ? s->bsrc[K * ns + i] : 0.0;
+    if (bsrc[i] != 0.0) nontrivial = 1;
+  }
   for (int i = 0; i < ns && breg >= 1 && breg <= s->nbreg; i++) {
     const orc_bc *b = &s->bc[(breg - 1) * ns + i];
     dual ui = dvar(U[K * ns + i], i);
@@ -932,11 +1068,65 @@ static int asm_bnode(orc_sys *s, int K, int breg, double fac, const double *U, d
       nontrivial = 1;
     }
   }
-  if (!nontrivial) return 0; /* nofunc breaction adds fac*(0-0) and no entries */
+  if (bp) {
+    for (int i = 0; i < ns; i++) {
+      for (int j = 0; j < ns; j++)
+        if (bp->react[i][j] != 0.0) {
+          y[i] = dadd(y[i], dscale(dvar(U[K * ns + j], j), bp->react[i][j]));
+          nontrivial = 1;
+        }
+      if (bp->fields && s->bcoef && s->bcoef[K * ns + i] != 0.0) {
+        y[i] = dadd(y[i], dscale(dvar(U[K * ns + i], i), s->bcoef[K * ns + i]));
+        nontrivial = 1;
+      }
+    }
+  }
+  /* assemble_res_jac(bnode): species present at the node (src/vfvm_assemblydata.jl:282-303) */
+  const uint8_t nm = s->nodemask[K];
+  if (nontrivial) {
+    for (int i = 0; i < ns; i++) {
+      if (!((nm >> i) & 1u)) continue;
+      F[K * ns + i] += fac * (y[i].v - bsrc[i]);
+      for (int j = 0; j < ns; j++)
+        if (((nm >> j) & 1u) && addnz(s, K * ns + i, K * ns + j, y[i].d[j], fac)) return ORC_ERR_ASSEMBLY;
+    }
+  }
+  /* bstorage, src/vfvm_assembly.jl:409-439 */
+  if (s->has_bstorage) {
+    for (int i = 0; i < ns; i++) {
+      if (!((nm >> i) & 1u)) continue;
+      const int st = bp && ((bp->storage >> i) & 1);
+      double v = st ? U[K * ns + i] : 0.0, vo = st ? Uold[K * ns + i] : 0.0;
+      F[K * ns + i] += fac * (v - vo) * tstepinv;
+      if (st && addnz(s, K * ns + i, K * ns + i, 1.0, fac * tstepinv)) return ORC_ERR_ASSEMBLY;
+    }
+  }
+  return 0;
+}
+
+/* one boundary edge item: assemble_bedges, src/vfvm_assembly.jl:444-510, with
+   assemble_res_jac(bedge), src/vfvm_assemblydata.jl:411-445: species present at both
+   nodes; the registry's bflux is f_i = d_i (u_i,K - u_i,L) on the region's edges */
+static int asm_bedge(orc_sys *s, int K, int L, int breg, double fac, const double *U, double *F) {
+  if (!s->bphys || breg < 1 || breg > s->nbreg) return 0;
+  const orc_bphysics *bp = &s->bphys[breg - 1];
+  int ns = s->nspec;
+  dual g[ORC_MAXSPEC];
+  for (int i = 0; i < ns; i++) g[i] = dscale(dsub(dvar(U[K * ns + i], i), dvar(U[L * ns + i], ns + i)), bp->bflux[i]);
+  const uint8_t mk = s->nodemask[K], ml = s->nodemask[L];
   for (int i = 0; i < ns; i++) {
-    F[K * ns + i] += fac * (y[i].v - 0.0);
-    for (int j = 0; j < ns; j++)
-      if (addnz(s, K * ns + i, K * ns + j, y[i].d[j], fac)) return ORC_ERR_ASSEMBLY;
+    if (!((mk >> i) & 1u) || !((ml >> i) & 1u)) continue;
+    F[K * ns + i] += fac * g[i].v;
+    F[L * ns + i] += -fac * g[i].v;
+    for (int j = 0; j < ns; j++) {
+      if (!((mk >> j) & 1u) || !((ml >> j) & 1u)) continue;
+      int rc = 0;
+      rc |= addnz(s, K * ns + i, K * ns + j, +g[i].d[j], fac);
+      rc |= addnz(s, L * ns + i, K * ns + j, -g[i].d[j], fac);
+      rc |= addnz(s, K * ns + i, L * ns + j, +g[i].d[ns + j], fac);
+      rc |= addnz(s, L * ns + i, L * ns + j, -g[i].d[ns + j], fac);
+      if (rc) return ORC_ERR_ASSEMBLY;
+    }
   }
   return 0;
 }
@@ -950,6 +1140,9 @@ int orc_assemble(orc_sys *s, const double *U, const double *Uold, double time, d
   lnk_zero(&s->mat);
   for (int k = 0; k < ndof; k++) F[k] = 0.0;
   double tstepinv = 1.0 / tstep;
+  s->peff = s->phys;
+  if (s->has_dp)
+    for (int j = 0; j < ORC_NPARAM; j++) s->peff.p[j] = s->phys.p[j] + s->dp[j] * lambda;
   int rc;
   if (s->assembly == ORC_CELLWISE) {
     for (int c = 0; c < s->ncells; c++)
@@ -973,8 +1166,26 @@ int orc_assemble(orc_sys *s, const double *U, const double *Uold, double time, d
   for (int ib = 0; ib < s->nbfaces; ib++)
     for (int i = 0; i < s->dim; i++)
       if ((rc = asm_bnode(s, s->bfacenodes[s->dim * ib + i], s->bfaceregions[ib], s->bnodefac[s->dim * ib + i], U,
-                          lambda, F)))
+                          Uold, tstepinv, lambda, F)))
         return rc;
+  if (s->has_bflux && s->dim >= 2) {
+    int nbe = s->dim == 3 ? 3 : 1;
+    for (int ib = 0; ib < s->nbfaces; ib++) {
+      const int32_t *bn = s->bfacenodes + s->dim * ib;
+      for (int e = 0; e < nbe; e++) {
+        int K = s->dim == 3 ? bn[EN_TRI[e][0]] : bn[0], L = s->dim == 3 ? bn[EN_TRI[e][1]] : bn[1];
+        if ((rc = asm_bedge(s, K, L, s->bfaceregions[ib], s->bedgefac[nbe * ib + e], U, F))) return rc;
+      }
+    }
+  }
+  /* _eval_and_assemble_inactive_species, src/vfvm_system.jl:1010-1026 */
+  if (!s->homogeneous)
+    for (int K = 0; K < s->nnodes; K++)
+      for (int i = 0; i < ns; i++)
+        if (!((s->nodemask[K] >> i) & 1u)) {
+          F[K * ns + i] += U[K * ns + i] - Uold[K * ns + i];
+          lnk_add(&s->mat, K * ns + i, K * ns + i, 1.0);
+        }
   s->have_factorization = 0;
   return 0;
 }
@@ -994,6 +1205,14 @@ void orc_matrix_csc(const orc_sys *s, int64_t *colptr, int32_t *rowval, double *
 void orc_matrix_reset(orc_sys *s) {
   lnk_free(&s->mat);
   lnk_init(&s->mat, s->nnodes * s->nspec);
+}
+
+/* _initialize_inactive_dof!, src/vfvm_system.jl:1028-1042 */
+static void orc_initialize_inactive(orc_sys *s, double *U) {
+  if (s->homogeneous) return;
+  for (int K = 0; K < s->nnodes; K++)
+    for (int i = 0; i < s->nspec; i++)
+      if (!((s->nodemask[K] >> i) & 1u)) U[K * s->nspec + i] = 0.0;
 }
 
 /* _initialize_dirichlet!, src/vfvm_system.jl:947-1003 */
@@ -1408,6 +1627,7 @@ int orc_solve_step(orc_sys *s, double *U, const double *Uold, const orc_control 
   memset(&hist, 0, sizeof(hist));
   for (int k = 0; k < ndof; k++) U[k] = Uold[k]; /* solution .= oldsol */
   orc_initialize_dirichlet(s, U, time, lambda);
+  orc_initialize_inactive(s, U);
   double oldnorm = 1.0, damp = 1.0, rnorm = 0.0, norm = 0.0, tolx = 0.0;
   int converged = 0, nround = 0, niter = 1, rc = 0;
   if (!c->is_linear) {

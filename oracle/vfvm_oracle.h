@@ -93,6 +93,21 @@ typedef struct {
 
 typedef struct orc_sys orc_sys;
 
+/* species activity: enable_species! (every cell region) / enable_boundary_species!
+   (src/vfvm_system.jl:433-515); NONE = enabled nowhere */
+enum { ORC_SPEC_BULK = 0, ORC_SPEC_BOUNDARY = 1, ORC_SPEC_NONE = 2 };
+
+/* boundary physics of one boundary region (the registry's breaction, bstorage, bflux):
+     breaction  f_i += sum_j react[i][j] u_j  (+ bcoef[i,K] u_i where `fields` is set)
+     bsource    bsrc[i,K] where `fields` is set (subtracted: fac (breaction - bsource))
+     bstorage   f_i = u_i for the species bits of `storage`
+     bflux      f_i = bflux[i] (u_i,K - u_i,L) on the boundary edges of the region */
+typedef struct {
+  double react[ORC_MAXSPEC][ORC_MAXSPEC];
+  double bflux[ORC_MAXSPEC];
+  int32_t storage, fields;
+} orc_bphysics;
+
 void orc_default_control(orc_control *c);
 
 orc_sys *orc_create(int dim, int nnodes, const double *coord,
@@ -101,6 +116,21 @@ orc_sys *orc_create(int dim, int nnodes, const double *coord,
                     int nspec, const orc_physics *phys, int nbregions, const orc_bc *bc,
                     int assembly);
 void orc_destroy(orc_sys *s);
+/* species activity (kind[i], and for BOUNDARY the boundary regions as bits r-1 of
+   bregions[i]); call before the first assembly.  Default: every species BULK. */
+int orc_set_species(orc_sys *s, const int32_t *kind, const uint32_t *bregions);
+/* boundary physics per boundary region (nbregions entries), NULL = none */
+void orc_set_bphysics(orc_sys *s, const orc_bphysics *bp);
+/* per-node boundary source and breaction coefficient (nspec x nnodes each, nullable) */
+void orc_set_bfields(orc_sys *s, const double *bsrc, const double *bcoef);
+/* per-node bulk reaction coefficient (nspec x nnodes, nullable): reaction f_i += rcoef[i,K] u_i
+   (a reaction callback reading node.index data, examples/Example230_BoundaryFlux.jl:63-66) */
+void orc_set_rcoef(orc_sys *s, const double *rcoef);
+/* registry parameters depending on the system parameter lambda: the assembly at lambda
+   uses p_j + dp[j] lambda (dp: ORC_NPARAM values, NULL = none) */
+void orc_set_param_dependence(orc_sys *s, const double *dp);
+/* node_dof activity mask: bit i of mask[K] set if species i is active at node K */
+void orc_node_species(const orc_sys *s, uint8_t *mask);
 /* per-node source, nspec x nnodes (species fastest); NULL = none */
 void orc_set_source(orc_sys *s, const double *src);
 void orc_set_bc(orc_sys *s, const orc_bc *bc);

@@ -7,6 +7,7 @@ import math
 import numpy as np
 
 import vfvm
+from vfvm import problems
 from vfvm.grids import linspace, simplexgrid
 
 ORACLE_BC = {vfvm.Dirichlet: "dirichlet", vfvm.Robin: "robin"}
@@ -50,15 +51,31 @@ def pair(grid, nspec, phys_kw, bcs=(), legacy=(), source=None, oracle=True, embe
     return sys, osys
 
 
+def oracle_twin(sys):
+    """The oracle's system for a vfvm.System built from registry entries (physics block,
+    bcondition entries, legacy boundary conditions, source array)."""
+    import oracle as O
+    ph = sys.physics
+    ns = sys.num_species
+    obcs = {}
+    for b in ph.bcondition:
+        if isinstance(b, vfvm.Dirichlet):
+            obcs[(b.region, b.species - 1)] = ("dirichlet", b.value, 0.0, b.value_lambda)
+        else:
+            obcs[(b.region, b.species - 1)] = ("robin", b.value, b.factor)
+    for (reg, sp), (kind, val, fac) in sys._legacy.items():
+        obcs[(reg, sp - 1)] = ("dirichlet_legacy", val) if kind == vfvm.physics.BC_DIRICHLET_LEGACY else \
+            ("robin", val, fac)
+    src = sys._source_array()
+    osrc = None if src is None else np.asarray(src, dtype=float).reshape(-1, order="F")
+    return O.OracleSystem(sys.grid, ns, oracle_physics(ph, ns), obcs, O.EDGEWISE, osrc)
+
+
 def nonlinear_poisson_2d(n=101, eps=1e-2, f=1.0, values=(0.0, 1.0, 1.0, 0.0), oracle=True):
     """Nonlinear Poisson: flux eps (u_k - u_l), reaction e^u - e^-u - f; Dirichlet
     values on south, east, north, west (regions 1..4)."""
-    X = linspace(0.0, 1.0, n)
-    g = simplexgrid(X, X)
-    bcs = [vfvm.Dirichlet(species=1, region=r + 1, value=v) for r, v in enumerate(values)]
-    src = np.full((1, g.num_nodes), f, order="F")
-    return pair(g, 1, dict(flux=vfvm.LinearFlux(eps), reaction=vfvm.SinhReaction()), bcs, source=src,
-                oracle=oracle)
+    sys = problems.nonlinear_poisson_2d(n, eps, f, values)
+    return sys, (oracle_twin(sys) if oracle else None)
 
 
 def sg_grid(name):
@@ -81,48 +98,23 @@ def nonlinear_poisson_on(g, eps=0.1, f=1.0):
     return pair(g, 1, dict(flux=vfvm.LinearFlux(eps), reaction=vfvm.SinhReaction()), bcs, source=src)
 
 
-def nonlinear_diffusion_3d(n=17):
+def nonlinear_diffusion_3d(n=17, oracle=True):
     """3D nonlinear diffusion D(u) = 1 + u^2, storage u, u = 0 on all faces."""
-    X = linspace(0.0, 1.0, n)
-    g = simplexgrid(X, X, X)
-    bcs = [vfvm.Dirichlet(species=1, region=r, value=0.0) for r in range(1, 7)]
-    return pair(g, 1, dict(flux=vfvm.NonlinearDiffusionFlux(1.0, 1.0), storage=vfvm.IdentityStorage()), bcs)
+    sys = problems.nonlinear_diffusion_3d(n)
+    return sys, (oracle_twin(sys) if oracle else None)
 
 
-def initial_sine_3d(grid):
-    x, y, z = grid.coord
-    return 0.5 * np.sin(math.pi * x) * np.sin(math.pi * y) * np.sin(math.pi * z)
+initial_sine_3d = problems.initial_sine_3d
 
 
 def drift_diffusion_2d(n=33, lambda2=1e-4, oracle=True):
     """Bipolar drift-diffusion (psi, n, p), doping +1 for x < 0.5 else -1, ohmic
     contacts on west (x = 0) and east (x = 1); east contact biased by lambda."""
-    X = linspace(0.0, 1.0, n)
-    g = simplexgrid(X, X)
-    x = g.coord[0]
-    C = np.where(x < 0.5, 1.0, -1.0)
-    src = np.zeros((3, g.num_nodes), order="F")
-    src[0] = C
-    bcs = []
-    for reg, c, bias in ((4, 1.0, 0.0), (2, -1.0, 1.0)):
-        nn = 0.5 * (c + math.sqrt(c * c + 4.0))
-        pp = 1.0 / nn
-        psi = math.asinh(c / 2.0)
-        bcs += [vfvm.Dirichlet(species=1, region=reg, value=psi, value_lambda=bias),
-                vfvm.Dirichlet(species=2, region=reg, value=nn),
-                vfvm.Dirichlet(species=3, region=reg, value=pp)]
-    phys = dict(flux=vfvm.DriftDiffusionFlux(lambda2, -1.0, 1.0), reaction=vfvm.DriftDiffusionReaction(1.0))
-    return pair(g, 3, phys, bcs, source=src, oracle=oracle)
+    sys = problems.drift_diffusion_2d(n, lambda2)
+    return sys, (oracle_twin(sys) if oracle else None)
 
 
-def dd_initial(grid):
-    x = grid.coord[0]
-    C = np.where(x < 0.5, 1.0, -1.0)
-    U = np.zeros((3, grid.num_nodes), order="F")
-    U[0] = np.arcsinh(C / 2.0)
-    U[1] = 0.5 * (C + np.sqrt(C * C + 4))
-    U[2] = 1.0 / U[1]
-    return U
+dd_initial = problems.dd_initial
 
 
 def penalty_rows(osys):
@@ -171,13 +163,4 @@ def compare_assembly(sys, osys, U, Uold=None, tstep=math.inf, embed=0.0, rtol=1e
     return errs
 
 
-def batch_member(seed, sizes=(33, 65, 129, 257)):
-    """One instance of the batched nonlinear-Poisson family: N from `sizes`, eps
-    log-uniform in [1e-3, 1], Dirichlet values uniform in [-1, 1] per side, constant
-    source uniform in [-5, 5]; all drawn from numpy.random.default_rng(seed)."""
-    rng = np.random.default_rng(seed)
-    n = int(rng.choice(sizes))
-    eps = float(10.0 ** rng.uniform(-3.0, 0.0))
-    vals = rng.uniform(-1.0, 1.0, 4)
-    f = float(rng.uniform(-5.0, 5.0))
-    return n, eps, vals, f
+batch_member = problems.batch_member

@@ -32,15 +32,35 @@ def test_fixed_timesteps():
 
 
 def test_product_and_gpu_problem_builders_do_not_load_the_oracle():
-    """The package and the problem builders the benchmark's GPU legs use must not import
-    the oracle (it is the checker, loaded only by tests and the CPU baselines)."""
+    """The package, including the benchmark configurations' problem builders
+    (vfvm.problems), must not import the oracle or the test suite's helpers: the oracle
+    is the checker, loaded only by tests and the CPU baselines."""
     import subprocess
     import sys as _sys
     root = __import__("os").path.dirname(__import__("os").path.dirname(__import__("os").path.abspath(__file__)))
-    code = ("import sys; sys.path[:0] = ['tests', 'voronoifvm.jl_amd', 'oracle']\n"
-            "import vfvm, vfvm.surrogate, vfvm.impedance, cases\n"
-            "cases.nonlinear_poisson_2d(9, oracle=False)\n"
-            "print('oracle' in sys.modules, 'mpnn_oracle' in sys.modules)\n")
+    code = ("import sys; sys.path[:0] = ['voronoifvm.jl_amd']\n"
+            "import vfvm, vfvm.surrogate, vfvm.impedance, vfvm.problems as P\n"
+            "P.nonlinear_poisson_2d(9); P.drift_diffusion_2d(9); P.nonlinear_diffusion_3d(5)\n"
+            "print('oracle' in sys.modules, 'mpnn_oracle' in sys.modules, 'cases' in sys.modules)\n")
     out = subprocess.run([_sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
-    assert out.stdout.strip() == "False False"
+    assert out.stdout.strip() == "False False False"
+
+
+def test_benchmark_gpu_legs_import_no_test_code():
+    """bench.py's GPU legs (run_*) build their problems from the package, not from the
+    test suite or the oracle (those serve the cpu_baseline legs only)."""
+    import ast
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    tree = ast.parse(open(os.path.join(root, "bench.py")).read())
+    for fn in tree.body:
+        if isinstance(fn, ast.FunctionDef) and fn.name.startswith("run_"):
+            for node in ast.walk(fn):
+                names = []
+                if isinstance(node, ast.Import):
+                    names = [a.name for a in node.names]
+                elif isinstance(node, ast.ImportFrom):
+                    names = [node.module or ""]
+                for n in names:
+                    assert n.split(".")[0] not in ("cases", "oracle", "mpnn_oracle"), (fn.name, n)

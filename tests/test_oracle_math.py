@@ -278,3 +278,47 @@ def test_sg_grids_read_and_measure(name, counts):
     np.add.at(bal, en[:, 1], ef * (x[en[:, 1]] - x[en[:, 0]]))
     inner = np.array([i for i in range(g.num_nodes) if i not in bnodes])
     assert np.abs(bal[inner]).max() <= 1e-11 * np.abs(ef).max()
+
+
+def _dd_physics(zn=-1.0, zp=1.0, lambda2=1e-2):
+    return O.physics("dd", "dd", lambda2=lambda2, zn=zn, zp=zp, ni2=1.0, ni=1.0)
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_dd_zero_bias_equilibrium(dim):
+    """n = e^psi, p = e^-psi: every Scharfetter-Gummel carrier flux and the recombination
+    vanish, so the carrier rows of F are zero to rounding for arbitrary psi.  The same
+    state with the charge numbers flipped (equivalently B(x) and B(-x) swapped) is far
+    from equilibrium, which shows the check sees such a mistake."""
+    import dd_pins
+    axes = [linspace(0, 1, 9)] * dim
+    g = simplexgrid(*axes)
+    rng = np.random.default_rng(11)
+    U = dd_pins.equilibrium_state(g.num_nodes, rng)
+    s = O.OracleSystem(g, 3, _dd_physics(), {}, O.EDGEWISE)
+    _, ef = s.edges()
+    scale = np.abs(U[1:]).max() * np.abs(ef).max()
+    F = s.assemble(U.ravel(order="F")).reshape(3, -1, order="F")
+    assert np.abs(F[1:]).max() <= 1e-13 * scale
+    flipped = O.OracleSystem(g, 3, _dd_physics(zn=1.0, zp=-1.0), {}, O.EDGEWISE)
+    Ff = flipped.assemble(U.ravel(order="F")).reshape(3, -1, order="F")
+    assert np.abs(Ff[1:]).max() >= 1e-2 * scale
+
+
+@pytest.mark.parametrize("psiK,psiL,nK,nL", [(0.3, -0.1, 1.2, 0.4), (-2.0, 1.5, 0.5, 3.0),
+                                             (0.01, 0.0, 2.0, 1.0), (40.0, -30.0, 1e-3, 2.0)])
+def test_dd_two_node_current_matches_ode(psiK, psiL, nK, nL):
+    """A two-node 1D grid: the carrier rows of F are the edge currents of the exact
+    1D boundary-value problem (integrated numerically), +j at K and -j at L; p = 1/n
+    makes the recombination vanish, so only the flux is left."""
+    import dd_pins
+    h = 0.37
+    g = simplexgrid(np.array([0.0, h]))
+    U = np.array([[psiK, psiL], [nK, nL], [1.0 / nK, 1.0 / nL]], order="F")
+    s = O.OracleSystem(g, 3, _dd_physics(), {}, O.EDGEWISE)
+    F = s.assemble(U.ravel(order="F")).reshape(3, 2, order="F")
+    for ispec, z in ((1, -1.0), (2, 1.0)):
+        j = dd_pins.sg_current_ode(U[ispec, 0], U[ispec, 1], psiK, psiL, z, h)
+        sc = max(abs(j), abs(U[ispec]).max() / h)
+        assert abs(F[ispec, 0] - j) <= 1e-9 * sc, (ispec, F[ispec, 0], j)
+        assert abs(F[ispec, 1] + j) <= 1e-9 * sc, (ispec, F[ispec, 1], -j)

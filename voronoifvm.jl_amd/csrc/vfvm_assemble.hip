@@ -8,10 +8,13 @@
 // search per entry (_addnz -> rawupdateindex!, src/vfvm_assembly.jl:20-28).
 // Here a group of G lanes owns node K and writes its rows of the CSR Jacobian
 // directly: lane l evaluates the fluxes of adjacency entries l, l+G, ...  with
-// K as the first argument (every registry flux is antisymmetric,
-// flux(u_l, u_k) = -flux(u_k, u_l) exactly in IEEE arithmetic, so row K's
-// contribution +fac*flux(u_k, u_l) equals the reference's -fac*flux(u_l, u_k)
-// when K is the edge's second node).  Off-diagonal block values go straight to
+// K as the first argument.  Every registry flux is antisymmetric,
+// flux(u_l, u_k) = -flux(u_k, u_l), so row K's contribution +fac*flux(u_k, u_l)
+// stands for the reference's -fac*flux(u_l, u_k) when K is the edge's second
+// node.  For the linear, power and nonlinear-diffusion fluxes the two are equal
+// bit for bit; for the Scharfetter-Gummel flux they differ by rounding (row L
+// evaluates B(-x) as -x/expm1(-x), the reference uses x + x/expm1(x)), well
+// inside the 1e-12 parity bound.  Off-diagonal block values go straight to
 // their slots; the diagonal block and F are reduced across the group.  No
 // atomics, no scatter, each J entry written exactly once.
 //
@@ -51,9 +54,12 @@ __device__ __forceinline__ double adj_factor(const AsmArgs &a, int64_t q) {
 
 // Node item (reaction, storage, source; src/vfvm_assembly.jl:81-122) and the
 // boundary items of node K (penalty Dirichlet, Robin; src/vfvm_assembly.jl:352-407).
+// Rows of species that are not region species (boundary species, species enabled
+// nowhere) get no node and no edge terms (assemble_res_jac, src/vfvm_assemblydata.jl:
+// 245-270, 349-379): their F and Jacobian entries stay 0 here.
 template <int NS, bool JAC>
 __device__ __forceinline__ void node_terms(const AsmArgs &a, const InstParams *P, const double *__restrict__ p,
-                                           const double (&uK)[NS], const double (&uo)[NS],
+                                           int64_t K, const double (&uK)[NS], const double (&uo)[NS],
                                            const double (&srcv)[NS], double vol, int32_t bb0, int32_t bb1,
                                            double (&Fn)[NS], double (&Jn)[NS][NS], double (&Fb)[NS],
                                            double (&Jb)[NS], bool &bad) {
@@ -67,26 +73,35 @@ __device__ __forceinline__ void node_terms(const AsmArgs &a, const InstParams *P
 #pragma unroll
       for (int i = 0; i < NS; i++) u[i] = dvar<NS>(uK[i], i);
       eval_reaction<NS>(P->reaction, p, u, r);
+      if (a.rcoef) {  // a reaction reading per-node data: + c_K u_i
+#pragma unroll
+        for (int i = 0; i < NS; i++) r[i] = r[i] + u[i] * a.rcoef[K * NS + i];
+      }
       eval_storage<NS>(P->storage, u, st);
 #pragma unroll
       for (int i = 0; i < NS; i++) {
+        const bool on = (a.m.bulk >> i) & 1u;
         const double src = srcv[i] * sfac;
-        Fn[i] = vol * (r[i].v - src + (st[i].v - so[i]) * a.tstepinv);
+        Fn[i] = on ? vol * (r[i].v - src + (st[i].v - so[i]) * a.tstepinv) : 0.0;
 #pragma unroll
         for (int j = 0; j < NS; j++) {
           const double v = r[i].d[j] + st[i].d[j] * a.tstepinv;
-          bad |= (v != v);
-          Jn[i][j] = v * vol;
+          bad |= on && (v != v);
+          Jn[i][j] = on ? v * vol : 0.0;
         }
       }
     } else {
       double r[NS], st[NS];
       eval_reaction<NS>(P->reaction, p, uK, r);
+      if (a.rcoef) {
+#pragma unroll
+        for (int i = 0; i < NS; i++) r[i] = r[i] + uK[i] * a.rcoef[K * NS + i];
+      }
       eval_storage<NS>(P->storage, uK, st);
 #pragma unroll
       for (int i = 0; i < NS; i++) {
         const double src = srcv[i] * sfac;
-        Fn[i] = vol * (r[i] - src + (st[i] - so[i]) * a.tstepinv);
+        Fn[i] = ((a.m.bulk >> i) & 1u) ? vol * (r[i] - src + (st[i] - so[i]) * a.tstepinv) : 0.0;
       }
     }
   }
@@ -182,7 +197,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
       eval_flux<NS>(flux_kind, p, uk, ul, g);
 #pragma unroll
       for (int i = 0; i < NS; i++) {
-        Fe[i] += f * g[i].v;
+        if ((a.m.bulk >> i) & 1u) Fe[i] += f * g[i].v;
         const int64_t boff =
             rowbase[i] + (e < diag ? (int64_t)e * a.m.ecnt[i] : (int64_t)(e - 1) * a.m.ecnt[i] + a.m.dcnt[i]);
         const uint8_t em = a.m.emask[i];
@@ -205,7 +220,8 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
       }
       eval_flux<NS>(flux_kind, p, uk, ul, g);
 #pragma unroll
-      for (int i = 0; i < NS; i++) Fe[i] += f * g[i];
+      for (int i = 0; i < NS; i++)
+        if ((a.m.bulk >> i) & 1u) Fe[i] += f * g[i];
     }
   }
 
@@ -220,7 +236,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble(AsmArgs a) {
 
   if (lane == 0) {
     double Fn[NS], Jn[NS][NS], Fb[NS], Jb[NS];
-    node_terms<NS, JAC>(a, P, p, uK, uo, srcv, vol, bb0, bb1, Fn, Jn, Fb, Jb, bad);
+    node_terms<NS, JAC>(a, P, p, K, uK, uo, srcv, vol, bb0, bb1, Fn, Jn, Fb, Jb, bad);
 #pragma unroll
     for (int i = 0; i < NS; i++) a.F[K * NS + i] = (Fn[i] + Fe[i]) + Fb[i];
     if constexpr (JAC) {
@@ -369,7 +385,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
           eval_flux<NS>(flux_kind, p, uk, ul, g);
 #pragma unroll
           for (int i = 0; i < NS; i++) {
-            Fe[i] += f * g[i].v;
+            if ((a.m.bulk >> i) & 1u) Fe[i] += f * g[i].v;
             const int64_t boff =
                 rowbase[i] + (e < diag ? (int64_t)e * a.m.ecnt[i] : (int64_t)(e - 1) * a.m.ecnt[i] + a.m.dcnt[i]);
             const uint8_t em = a.m.emask[i];
@@ -392,12 +408,13 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
           }
           eval_flux<NS>(flux_kind, p, uk, ul, g);
 #pragma unroll
-          for (int i = 0; i < NS; i++) Fe[i] += f * g[i];
+          for (int i = 0; i < NS; i++)
+            if ((a.m.bulk >> i) & 1u) Fe[i] += f * g[i];
         }
       }
     }
     double Fn[NS], Jn[NS][NS], Fb[NS], Jb[NS];
-    node_terms<NS, JAC>(a, P, p, uK, uo, srcv, vol, bb0, bb1, Fn, Jn, Fb, Jb, bad);
+    node_terms<NS, JAC>(a, P, p, K, uK, uo, srcv, vol, bb0, bb1, Fn, Jn, Fb, Jb, bad);
 #pragma unroll
     for (int i = 0; i < NS; i++) a.F[K * NS + i] = (Fn[i] + Fe[i]) + Fb[i];
     if constexpr (JAC) {
@@ -498,7 +515,7 @@ __global__ void k_init_dirichlet(int64_t nb, const int32_t *bnodes, const int32_
 __global__ void k_dudp(int64_t nnodes, int nspec, const int32_t *__restrict__ bn_ptr,
                        const int32_t *__restrict__ bn_reg, const double *__restrict__ bn_fac,
                        const double *__restrict__ nodevol, const double *__restrict__ src,
-                       const InstParams *__restrict__ inst, double *__restrict__ out) {
+                       const InstParams *__restrict__ inst, double *__restrict__ out, int with_bc) {
   const int64_t K = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (K >= nnodes) return;
   const InstParams *P = inst;
@@ -506,7 +523,7 @@ __global__ void k_dudp(int64_t nnodes, int nspec, const int32_t *__restrict__ bn
   for (int i = 0; i < nspec; i++) {
     double d = 0.0;
     if (P->p[12] != 0.0 && src) d = -(vol * src[K * nspec + i]);
-    for (int32_t b = bn_ptr[K]; b < bn_ptr[K + 1]; b++) {
+    for (int32_t b = with_bc ? bn_ptr[K] : 0; b < (with_bc ? bn_ptr[K + 1] : 0); b++) {
       const int reg = bn_reg[b];
       const double fac = bn_fac[b];
       if (reg < 1 || reg > MAXBREG) continue;
@@ -523,6 +540,45 @@ __global__ void k_dudp(int64_t nnodes, int nspec, const int32_t *__restrict__ bn
     }
     out[K * nspec + i] = d;
   }
+}
+
+// dF/dlambda of the flux and reaction terms whose registry coefficients depend on the
+// system parameter, p_j(lambda) = p_j + dp_j lambda (the params lanes of the reference's
+// dudp, src/vfvm_assembly.jl:116-120, 194-198): the registry evaluated with Dual<1>
+// parameters {p_j(lambda), dp_j} and constant unknowns, added to `out`.
+template <int NS>
+__global__ void __launch_bounds__(256) k_dudp_coef(AsmArgs a, ParamDeriv dpar, double *__restrict__ out) {
+  const int64_t K = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (K >= a.nnodes) return;
+  const InstParams *P = a.inst;
+  Dual<1> pd[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    pd[j].v = P->p[j];
+    pd[j].d[0] = dpar.dp[j];
+  }
+  Dual<1> uk[NS], r[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) uk[i] = dconst<1>(a.U[K * NS + i]);
+  eval_reaction<NS, Dual<1>, Dual<1>>(P->reaction, pd, uk, r);
+  const double vol = a.nodevol[K];
+  double acc[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) acc[i] = ((a.m.bulk >> i) & 1u) ? vol * r[i].d[0] : 0.0;
+  for (int32_t q = a.adj_ptr[K]; q < a.adj_ptr[K + 1]; q++) {
+    const int32_t M = adj_node(a, q, K);
+    if (M == K) continue;
+    const double f = adj_factor(a, q);
+    Dual<1> ul[NS], g[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) ul[i] = dconst<1>(a.U[(int64_t)M * NS + i]);
+    eval_flux<NS, Dual<1>, Dual<1>>(P->flux, pd, uk, ul, g);
+#pragma unroll
+    for (int i = 0; i < NS; i++)
+      if ((a.m.bulk >> i) & 1u) acc[i] += f * g[i].d[0];
+  }
+#pragma unroll
+  for (int i = 0; i < NS; i++) out[K * NS + i] += acc[i];
 }
 
 // max of |F| over rows that carry no Dirichlet penalty, per instance: non-negative
@@ -582,10 +638,23 @@ void launch_init_dirichlet(int64_t nb, const int32_t *bnodes, const int32_t *bn_
 }
 
 void launch_dudp(int64_t nnodes, int nspec, const int32_t *bn_ptr, const int32_t *bn_reg, const double *bn_fac,
-                 const double *nodevol, const double *src, const InstParams *inst, double *out, hipStream_t s) {
+                 const double *nodevol, const double *src, const InstParams *inst, double *out, hipStream_t s,
+                 int with_bc) {
   if (nnodes == 0) return;
   hipLaunchKernelGGL(k_dudp, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, s, nnodes, nspec, bn_ptr, bn_reg,
-                     bn_fac, nodevol, src, inst, out);
+                     bn_fac, nodevol, src, inst, out, with_bc);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_dudp_coef(const AsmArgs &a, const ParamDeriv &dp, double *out, hipStream_t s) {
+  if (a.nnodes == 0) return;
+  const dim3 g((unsigned)((a.nnodes + 255) / 256));
+  switch (a.nspec) {
+    case 1: hipLaunchKernelGGL((k_dudp_coef<1>), g, dim3(256), 0, s, a, dp, out); break;
+    case 2: hipLaunchKernelGGL((k_dudp_coef<2>), g, dim3(256), 0, s, a, dp, out); break;
+    case 3: hipLaunchKernelGGL((k_dudp_coef<3>), g, dim3(256), 0, s, a, dp, out); break;
+    default: throw Error(VFVM_ERR_ARG, "nspec must be 1..3");
+  }
   VFVM_HIP_CHECK(hipGetLastError());
 }
 
@@ -595,6 +664,170 @@ void launch_penalty_free_norm(int64_t nnodes, int nspec, const double *F, const 
   if (nnodes == 0) return;
   hipLaunchKernelGGL(k_penalty_free_norm, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, s, nnodes, nspec, F,
                      bn_ptr, bn_reg, node_inst, inst, reinterpret_cast<unsigned long long *>(out));
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+// ---- boundary pass: boundary species, boundary reaction / storage / source, bflux,
+// inactive species (src/vfvm_assembly.jl:318-510, src/vfvm_system.jl:1010-1026) ----
+
+// slot of column `col` in row `row` of the full pattern (rows sorted), -1 if absent
+__device__ __forceinline__ int64_t find_slot(const int32_t *rowptr, const int32_t *colind, int64_t row, int64_t col) {
+  int32_t lo = rowptr[row], hi = rowptr[row + 1];
+  const int32_t end = hi;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (colind[mid] < col)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (lo < end && colind[lo] == col) ? lo : -1;
+}
+
+// One thread per listed node K.  It owns the rows of K, so every F and J update is a
+// plain read-modify-write; the updates come in the reference's order: each boundary node
+// item of K (bface order) with its breaction and bstorage, then each boundary edge item
+// of K (bface order), then the identity rows of the species inactive at K.
+template <int NS, bool JAC>
+__global__ void __launch_bounds__(256) k_bnd(BndArgs b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.nlist) return;
+  const int64_t K = b.list[t];
+  const unsigned nm = b.nodemask[K];
+  double u[NS], uo[NS], Fk[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    u[i] = b.U[K * NS + i];
+    uo[i] = b.Uold[K * NS + i];
+    Fk[i] = b.F[K * NS + i];
+  }
+  bool missing = false;
+  auto addJ = [&](int64_t row, int64_t col, double v) {
+    const int64_t q = find_slot(b.rowptr, b.colind, row, col);
+    if (q < 0)
+      missing = true;
+    else
+      b.J[q] += v;
+  };
+  for (int32_t bi = b.bn_ptr[K]; bi < b.bn_ptr[K + 1]; bi++) {
+    const int reg = b.bn_reg[bi];
+    if (reg < 1 || reg > MAXBREG) continue;
+    const BPhysDev &P = b.bp[reg - 1];
+    const double fac = b.bn_fac[bi];
+    // breaction: f_i = sum_j react[i][j] u_j (+ c_K u_i), minus the boundary source
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      if (!((nm >> i) & 1u)) continue;
+      const double coef = (P.fields && b.bcoef) ? b.bcoef[K * NS + i] : 0.0;
+      const double bs = (P.fields && b.bsrc) ? b.bsrc[K * NS + i] : 0.0;
+      double y = 0.0;
+      bool any = bs != 0.0 || coef != 0.0;
+#pragma unroll
+      for (int j = 0; j < NS; j++)
+        if (P.react[i][j] != 0.0) {
+          y = y + u[j] * P.react[i][j];
+          any = true;
+        }
+      if (coef != 0.0) y = y + u[i] * coef;
+      if (!any) continue;
+      Fk[i] += fac * (y - bs);
+      if constexpr (JAC) {
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+          if (!((nm >> j) & 1u)) continue;
+          double v = P.react[i][j];
+          if (j == i && coef != 0.0) v = v + coef;
+          if (v != 0.0) addJ(K * NS + i, K * NS + j, v * fac);
+        }
+      }
+    }
+    // bstorage: f_i = u_i for the region's storage species
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      if (!((nm >> i) & 1u) || !((P.storage >> i) & 1)) continue;
+      Fk[i] += fac * (u[i] - uo[i]) * b.tstepinv;
+      if constexpr (JAC) addJ(K * NS + i, K * NS + i, fac * b.tstepinv);
+    }
+  }
+  // bflux: f_i = d_i (u_i,K - u_i,L), species present at both nodes
+  for (int32_t q = b.be_ptr[t]; q < b.be_ptr[t + 1]; q++) {
+    const int reg = b.be_reg[q];
+    if (reg < 1 || reg > MAXBREG) continue;
+    const BPhysDev &P = b.bp[reg - 1];
+    const int64_t L = b.be_nb[q];
+    const unsigned ml = b.nodemask[L];
+    const double fac = b.be_fac[q];
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      const double d = P.bflux[i];
+      if (d == 0.0 || !((nm >> i) & 1u) || !((ml >> i) & 1u)) continue;
+      Fk[i] += fac * ((u[i] - b.U[L * NS + i]) * d);
+      if constexpr (JAC) {
+        addJ(K * NS + i, K * NS + i, d * fac);
+        addJ(K * NS + i, L * NS + i, -d * fac);
+      }
+    }
+  }
+  // inactive species: identity rows (rawupdateindex!, always stored)
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    if ((nm >> i) & 1u) continue;
+    Fk[i] += u[i] - uo[i];
+    if constexpr (JAC) addJ(K * NS + i, K * NS + i, 1.0);
+  }
+#pragma unroll
+  for (int i = 0; i < NS; i++) b.F[K * NS + i] = Fk[i];
+  if (missing) atomicOr(b.flags, 2);
+}
+
+__global__ void k_gather_or_zero(int64_t n, const int32_t *src, const double *x, double *y) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int32_t q = src[k];
+  y[k] = q >= 0 ? x[q] : 0.0;
+}
+
+__global__ void k_init_inactive(int64_t nlist, const int32_t *list, const uint8_t *nodemask, int nspec, double *U) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlist) return;
+  const int64_t K = list[t];
+  const unsigned nm = nodemask[K];
+  for (int i = 0; i < nspec; i++)
+    if (!((nm >> i) & 1u)) U[K * nspec + i] = 0.0;
+}
+
+void launch_bnd(const BndArgs &b, hipStream_t s) {
+  if (b.nlist == 0) return;
+  const dim3 g((unsigned)((b.nlist + 255) / 256));
+  switch (b.nspec) {
+    case 1:
+      if (b.J) hipLaunchKernelGGL((k_bnd<1, true>), g, dim3(256), 0, s, b);
+      else hipLaunchKernelGGL((k_bnd<1, false>), g, dim3(256), 0, s, b);
+      break;
+    case 2:
+      if (b.J) hipLaunchKernelGGL((k_bnd<2, true>), g, dim3(256), 0, s, b);
+      else hipLaunchKernelGGL((k_bnd<2, false>), g, dim3(256), 0, s, b);
+      break;
+    case 3:
+      if (b.J) hipLaunchKernelGGL((k_bnd<3, true>), g, dim3(256), 0, s, b);
+      else hipLaunchKernelGGL((k_bnd<3, false>), g, dim3(256), 0, s, b);
+      break;
+    default: throw Error(VFVM_ERR_ARG, "nspec must be 1..3");
+  }
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_gather_or_zero(int64_t n, const int32_t *src, const double *x, double *y, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_gather_or_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, src, x, y);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_init_inactive(int64_t nlist, const int32_t *list, const uint8_t *nodemask, int nspec, double *U,
+                          hipStream_t s) {
+  if (nlist == 0) return;
+  hipLaunchKernelGGL(k_init_inactive, dim3((unsigned)((nlist + 255) / 256)), dim3(256), 0, s, nlist, list, nodemask,
+                     nspec, U);
   VFVM_HIP_CHECK(hipGetLastError());
 }
 

@@ -62,7 +62,7 @@ InstParams vfvm::make_inst(const vfvm_sys &s) {
   P.flux = s.phys.flux;
   P.reaction = s.phys.reaction;
   P.storage = s.phys.storage;
-  for (int k = 0; k < 16; k++) P.p[k] = s.phys.p[k];
+  for (int k = 0; k < 16; k++) P.p[k] = s.has_dp ? s.phys.p[k] + s.dpar.dp[k] * s.inst_lambda : s.phys.p[k];
   P.has_legacy = 0;
   for (int r = 0; r < s.nbreg && r < MAXBREG; r++)
     for (int i = 0; i < s.nspec; i++) {
@@ -87,6 +87,10 @@ void upload_inst(vfvm_sys &s) {
 
 AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, double *J, double tstep,
                  double lambda) {
+  if (s.has_dp && lambda != s.inst_lambda) {  // coefficients at this lambda
+    s.inst_lambda = lambda;
+    upload_inst(s);
+  }
   AsmArgs a{};
   a.nnodes = s.nnodes;
   a.adj_ptr = s.d_adj_ptr.p;
@@ -110,6 +114,7 @@ AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, do
   a.U = U;
   a.Uold = Uold;
   a.src = s.has_src ? s.d_src.p : nullptr;
+  a.rcoef = s.has_rcoef ? s.d_rcoef.p : nullptr;
   a.F = F;
   a.J = J;
   a.tstepinv = 1.0 / tstep;  // 1/Inf = 0: stationary (src/vfvm_assembly.jl:554)
@@ -118,6 +123,50 @@ AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, do
   a.m = s.masks;
   a.nspec = s.nspec;
   return a;
+}
+
+void run_assembly(vfvm_sys &s, AsmArgs a, int group, hipStream_t st) {
+  if (!s.bnd.hs.on) {
+    launch_assemble(a, group, st);
+    return;
+  }
+  double *J = a.J;
+  const bool extra = J && s.bnd.hs.extra;
+  if (extra) a.J = s.bnd.d_Jb.p;  // the bulk kernel fills the bulk pattern
+  launch_assemble(a, group, st);
+  if (extra) launch_gather_or_zero(s.nnz, s.bnd.d_map.p, s.bnd.d_Jb.p, J, st);
+  BndArgs b{};
+  b.nlist = (int64_t)s.bnd.hs.list.size();
+  b.list = s.bnd.d_list.p;
+  b.nodemask = s.bnd.d_nodemask.p;
+  b.bn_ptr = s.d_bn_ptr.p;
+  b.bn_reg = s.d_bn_reg.p;
+  b.bn_fac = s.d_bn_fac.p;
+  b.be_ptr = s.bnd.d_be_ptr.p;
+  b.be_nb = s.bnd.d_be_nb.p;
+  b.be_reg = s.bnd.d_be_reg.p;
+  b.be_fac = s.bnd.d_be_fac.p;
+  b.bp = s.bnd.d_bp.p;
+  b.bsrc = s.bnd.has_bsrc ? s.bnd.d_bsrc.p : nullptr;
+  b.bcoef = s.bnd.has_bcoef ? s.bnd.d_bcoef.p : nullptr;
+  b.rowptr = s.d_rowptr.p;
+  b.colind = s.d_colind.p;
+  b.U = a.U;
+  b.Uold = a.Uold;
+  b.F = a.F;
+  b.J = J;
+  b.tstepinv = a.tstepinv;
+  b.flags = a.flags;
+  b.nspec = s.nspec;
+  launch_bnd(b, st);
+}
+
+// where the bulk kernel writes its Jacobian values (the bulk pattern) when timed alone
+double *bulk_J(vfvm_sys &s) { return s.bnd.hs.extra ? s.bnd.d_Jb.p : s.d_J.p; }
+
+void init_inactive(vfvm_sys &s, double *U, hipStream_t st) {
+  if (s.bnd.hs.homogeneous) return;
+  launch_init_inactive((int64_t)s.bnd.hs.list.size(), s.bnd.d_list.p, s.bnd.d_nodemask.p, s.nspec, U, st);
 }
 
 }  // namespace vfvm
@@ -259,6 +308,7 @@ int newton_device(vfvm_sys &s, double time, double tstep, double lambda, const v
   launch_copy(s.ndof, s.d_Uold.p, s.d_U.p, st);  // solution .= oldsol
   launch_init_dirichlet((int64_t)s.hs.bnodes.size(), s.d_bnodes.p, s.d_bn_ptr.p, s.d_bn_reg.p, nullptr, s.d_inst.p,
                         s.nspec, lambda, s.d_U.p, st);
+  init_inactive(s, s.d_U.p, st);  // _initialize!, src/vfvm_system.jl:1005-1008
   launch_fill_i32(1, s.d_flags.p, 0, st);
   double oldnorm = 1.0, damp = 1.0, rnorm = 0.0, norm = 0.0, tolx = 0.0;
   int converged = 0, nround = 0, niter = 1, rc = 0;
@@ -270,8 +320,9 @@ int newton_device(vfvm_sys &s, double time, double tstep, double lambda, const v
   while (niter <= c.maxiters) {
     AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_F.p, s.d_J.p, tstep, lambda);
     s.ctx->probe.before(VFVM_KERNEL_ASSEMBLE, st);
-    launch_assemble(a, s.asm_group, st);
+    run_assembly(s, a, s.asm_group, st);
     s.ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st, s.nnodes, (int64_t)s.hs.adj_ptr.back() - s.nnodes, s.nnz);
+    // reuse_precs of src/vfvm_solver.jl:99: the first step's factor serves the later ones
     const bool refactor = !(c.reuse_precon && niter > 1);
     s.lin.prepare(s.d_J.p, c, refactor);
     vfvm_linstats_t ls{};
@@ -325,7 +376,7 @@ int newton_device(vfvm_sys &s, double time, double tstep, double lambda, const v
   // ||F||_inf over non-penalty rows at the returned solution (residual-only assembly)
   if (rc == 0) {
     AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_tmp.p, nullptr, tstep, lambda);
-    launch_assemble(a, s.asm_group, st);
+    run_assembly(s, a, s.asm_group, st);
     h.residual_norm = penalty_free_norm(s, s.d_tmp.p);
   }
   h.converged = converged;
@@ -346,7 +397,7 @@ void autotune_impl(vfvm_sys &s, int32_t *ag, int32_t *sg) {
   float best = 1e30f;
   int bestg = s.asm_group;
   for (int g : {0, 1, 2, 4, 8, 16, 32}) {
-    AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_F.p, s.d_J.p, 1.0, 0.0);
+    AsmArgs a = asm_args(s, s.d_U.p, s.d_Uold.p, s.d_F.p, bulk_J(s), 1.0, 0.0);
     const float ms = timer.time([&]() { launch_assemble(a, g, st); });
     if (ms < best) {
       best = ms;
@@ -440,7 +491,7 @@ void vfvm_default_control(vfvm_control_t *c) {
   c->precision = VFVM_PREC_FP64;
   c->gmres_restart = 30;
   c->max_refinements = 20;
-  c->reuse_precon = 0;
+  c->reuse_precon = 1;  // factorize_every_newtonstep = false (src/vfvm_solvercontrol.jl:126)
   c->check_every = 4;
   c->inner_reltol = 1.0e-3;
 }
@@ -521,7 +572,24 @@ int vfvm_system_create_cs(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *
                           const int64_t *bfacenodes, const int64_t *bfaceregions, int nspec,
                           const vfvm_physics_t *physics, int nbregions, const vfvm_bc_t *bc, int coordsys,
                           vfvm_sys **out) {
+  return vfvm_system_create_ex(ctx, dim, nnodes, coord, ncells, cellnodes, cellregions, nbfaces, bfacenodes,
+                               bfaceregions, nspec, physics, nbregions, bc, coordsys, nullptr, nullptr,
+                               VFVM_ASSEMBLY_EDGEWISE, out);
+}
+
+int vfvm_system_create_ex(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *coord, int64_t ncells,
+                          const int64_t *cellnodes, const int64_t *cellregions, int64_t nbfaces,
+                          const int64_t *bfacenodes, const int64_t *bfaceregions, int nspec,
+                          const vfvm_physics_t *physics, int nbregions, const vfvm_bc_t *bc, int coordsys,
+                          const vfvm_species_t *species, const vfvm_bphysics_t *bphys, int assembly,
+                          vfvm_sys **out) {
   return guarded(ctx, [&]() {
+    if (assembly != VFVM_ASSEMBLY_EDGEWISE && assembly != VFVM_ASSEMBLY_CELLWISE)
+      throw Error(VFVM_ERR_ARG, "assembly must be VFVM_ASSEMBLY_EDGEWISE or VFVM_ASSEMBLY_CELLWISE");
+    if (species)
+      for (int i = 0; i < nspec && i < MAXSPEC; i++)
+        if (species[i].kind < VFVM_SPECIES_BULK || species[i].kind > VFVM_SPECIES_NONE)
+          throw Error(VFVM_ERR_ARG, "unknown species kind");
     const bool cs_ok = coordsys == VFVM_COORD_CARTESIAN ||
                        ((coordsys == VFVM_COORD_POLAR1D || coordsys == VFVM_COORD_SPHERICAL1D) && dim == 1) ||
                        (coordsys == VFVM_COORD_CYLINDRICAL2D && dim == 2);
@@ -551,6 +619,7 @@ int vfvm_system_create_cs(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *
     VFVM_HIP_CHECK(hipHostMalloc((void **)&s->h_pinned, 8 * sizeof(double), hipHostMallocDefault));
     s->ncells = ncells;
     s->nbfaces = nbfaces;
+    s->cellwise = assembly == VFVM_ASSEMBLY_CELLWISE;
     const bool dev_setup =
         ctx->setup_mode == 1 || (ctx->setup_mode == 2 && ncells * (dim + 1) >= SETUP_DEVICE_MIN_INCIDENCES);
     if (dev_setup) {
@@ -558,7 +627,8 @@ int vfvm_system_create_cs(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *
       build_node_graph_device(*s, coord, cellnodes, ncells);
       build_boundary(dim, nnodes, coord, nbfaces, bfacenodes, bfaceregions, s->hs, coordsys);
     } else {
-      build_setup(dim, nnodes, coord, ncells, cellnodes, nbfaces, bfacenodes, bfaceregions, s->hs, coordsys);
+      build_setup(dim, nnodes, coord, ncells, cellnodes, nbfaces, bfacenodes, bfaceregions, s->hs, coordsys,
+                  s->cellwise);
       // ranges checked by build_setup
       s->cellnodes0.resize((size_t)(dim + 1) * ncells);
       for (size_t k = 0; k < s->cellnodes0.size(); k++) s->cellnodes0[k] = (int32_t)(cellnodes[k] - 1);
@@ -568,10 +638,16 @@ int vfvm_system_create_cs(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *
     if (dev_setup) finish_device_geometry(*s);
     for (int64_t b = 0; b < (int64_t)s->hs.bn_reg.size(); b++)
       if (s->hs.bn_reg[b] > MAXBREG) throw Error(VFVM_ERR_ARG, "boundary region number above 8");
-    s->masks = make_masks(nspec, s->phys, s->bc.data(), nbregions);
+    s->masks = make_masks(nspec, s->phys, s->bc.data(), nbregions, bulk_species(nspec, species));
     build_dof_csr(s->hs, nspec, s->masks, s->rowptr, s->colind);
-    s->nnz = s->rowptr[s->ndof];
     verify_slot_map(s->hs, nspec, s->masks, s->rowptr, s->colind);
+    s->bnd.nnz_bulk = s->rowptr[s->ndof];
+    // boundary species, inactive dofs, boundary physics: the boundary pass and the full pattern
+    build_bnd(dim, nnodes, coord, nbfaces, bfacenodes, bfaceregions, coordsys, nspec, species, bphys, nbregions,
+              s->hs, s->rowptr, s->colind, s->bnd.hs);
+    s->nnz = s->rowptr[s->ndof];
+    s->bnd.bp.assign(MAXBREG, vfvm_bphysics_t{});
+    if (bphys) std::copy(bphys, bphys + nbregions, s->bnd.bp.begin());
     build_csc_perm(s->ndof, s->rowptr, s->colind, s->csc_colptr, s->csc_rowval, s->csc_perm);
     s->dofdiag.resize((size_t)s->ndof);
     for (int64_t r = 0; r < s->ndof; r++) {
@@ -619,6 +695,20 @@ int vfvm_system_create_cs(vfvm_ctx *ctx, int dim, int64_t nnodes, const double *
     }
     s->d_csc_perm.upload(s->csc_perm, st);
     s->d_dofdiag.upload(s->dofdiag, st);
+    if (s->bnd.hs.on) {
+      const BndSetup &B = s->bnd.hs;
+      s->bnd.d_nodemask.upload(B.nodemask, st);
+      s->bnd.d_list.upload(B.list.empty() ? std::vector<int32_t>{0} : B.list, st);
+      s->bnd.d_be_ptr.upload(B.be_ptr, st);
+      s->bnd.d_be_nb.upload(B.be_nb.empty() ? std::vector<int32_t>{0} : B.be_nb, st);
+      s->bnd.d_be_reg.upload(B.be_reg.empty() ? std::vector<int32_t>{0} : B.be_reg, st);
+      s->bnd.d_be_fac.upload(B.be_fac.empty() ? std::vector<double>{0.0} : B.be_fac, st);
+      s->bnd.d_bp.upload(s->bnd.bp, st);
+      if (B.extra) {
+        s->bnd.d_map.upload(B.map, st);
+        s->bnd.d_Jb.alloc((size_t)s->bnd.nnz_bulk);
+      }
+    }
     s->d_U.alloc((size_t)s->ndof);
     s->d_Uold.alloc((size_t)s->ndof);
     s->d_F.alloc((size_t)s->ndof);
@@ -671,6 +761,29 @@ int vfvm_system_set_source(vfvm_sys *s, const double *src) {
   });
 }
 
+int vfvm_system_set_fields(vfvm_sys *s, const double *rcoef, const double *bsrc, const double *bcoef) {
+  if (!s) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    hipStream_t st = s->ctx->stream;
+    s->has_rcoef = rcoef != nullptr;
+    if (rcoef) s->d_rcoef.upload(rcoef, (size_t)s->ndof, st);
+    s->bnd.has_bsrc = bsrc != nullptr;
+    if (bsrc) s->bnd.d_bsrc.upload(bsrc, (size_t)s->ndof, st);
+    s->bnd.has_bcoef = bcoef != nullptr;
+    if (bcoef) s->bnd.d_bcoef.upload(bcoef, (size_t)s->ndof, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return VFVM_OK;
+  });
+}
+
+int vfvm_node_species(vfvm_sys *s, uint8_t *mask) {
+  if (!s || !mask) return VFVM_ERR_ARG;
+  const uint8_t all = (uint8_t)((1u << s->nspec) - 1u);
+  for (int64_t K = 0; K < s->nnodes; K++)
+    mask[K] = s->bnd.hs.nodemask.empty() ? all : s->bnd.hs.nodemask[(size_t)K];
+  return VFVM_OK;
+}
+
 int vfvm_system_set_bc(vfvm_sys *s, const vfvm_bc_t *bc) {
   if (!s || !bc) return VFVM_ERR_ARG;
   return guarded(s->ctx, [&]() {
@@ -706,6 +819,8 @@ int vfvm_ctx_set_setup(vfvm_ctx *ctx, int mode) {
 int vfvm_update_grid(vfvm_sys *s, const double *coord) {
   if (!s || !coord) return VFVM_ERR_ARG;
   return guarded(s->ctx, [&]() {
+    if (!s->bnd.hs.be_fac.empty())
+      throw Error(VFVM_ERR_ARG, "update_grid: the boundary-edge factors of bflux are not recomputed");
     update_geometry(*s, coord);
     return VFVM_OK;
   });
@@ -714,6 +829,8 @@ int vfvm_update_grid(vfvm_sys *s, const double *coord) {
 int vfvm_update_grid_device(vfvm_sys *s, const double *dcoord) {
   if (!s || !dcoord) return VFVM_ERR_ARG;
   return guarded(s->ctx, [&]() {
+    if (!s->bnd.hs.be_fac.empty())
+      throw Error(VFVM_ERR_ARG, "update_grid: the boundary-edge factors of bflux are not recomputed");
     update_geometry(*s, dcoord, true);
     return VFVM_OK;
   });
@@ -757,7 +874,7 @@ int vfvm_assemble_device(vfvm_sys *s, const double *dU, const double *dUold, dou
     hipStream_t st = s->ctx->stream;
     launch_fill_i32(1, s->d_flags.p, 0, st);
     AsmArgs a = asm_args(*s, dU, dUold, dF, dnzval_csr, tstep, lambda);
-    launch_assemble(a, s->asm_group, st);
+    run_assembly(*s, a, s->asm_group, st);
     int32_t fl = 0;
     VFVM_HIP_CHECK(hipMemcpyAsync(&fl, s->d_flags.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     VFVM_HIP_CHECK(hipStreamSynchronize(st));
@@ -795,7 +912,7 @@ void ode_rhs_device(vfvm_sys &s, const double *du, double *dout) {
   hipStream_t st = s.ctx->stream;
   AsmArgs a = asm_args(s, du, du, dout, nullptr, INFINITY, 0.0);
   s.ctx->probe.before(VFVM_KERNEL_RESIDUAL, st);
-  launch_assemble(a, s.res_group, st);
+  run_assembly(s, a, s.res_group, st);
   s.ctx->probe.after(VFVM_KERNEL_RESIDUAL, st, s.nnodes, (int64_t)s.hs.adj_ptr.back() - s.nnodes, 0);
   launch_negate(s.ndof, dout, st);
 }
@@ -805,7 +922,7 @@ void ode_jacobian_device(vfvm_sys &s, const double *du, double *dnz_csr) {
   launch_fill_i32(1, s.d_flags.p, 0, st);
   AsmArgs a = asm_args(s, du, du, s.d_F.p, dnz_csr, INFINITY, 0.0);
   s.ctx->probe.before(VFVM_KERNEL_ASSEMBLE, st);
-  launch_assemble(a, s.asm_group, st);
+  run_assembly(s, a, s.asm_group, st);
   s.ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st, s.nnodes, (int64_t)s.hs.adj_ptr.back() - s.nnodes, s.nnz);
   launch_negate(s.nnz, dnz_csr, st);
   int32_t fl = 0;
@@ -911,6 +1028,37 @@ int vfvm_assemble_dudp(vfvm_sys *s, double *dudp) {
   });
 }
 
+int vfvm_system_set_param_dependence(vfvm_sys *s, const double *dp) {
+  if (!s) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    s->has_dp = false;
+    for (int j = 0; j < 16; j++) {
+      s->dpar.dp[j] = dp ? dp[j] : 0.0;
+      s->has_dp |= s->dpar.dp[j] != 0.0;
+    }
+    if ((s->dpar.dp[4] != 0.0 || s->dpar.dp[6] != 0.0))
+      throw Error(VFVM_ERR_ARG, "the exponent slots p[4], p[6] cannot depend on lambda");
+    s->inst_lambda = 0.0;
+    upload_inst(*s);
+    return VFVM_OK;
+  });
+}
+
+int vfvm_assemble_dudp_at(vfvm_sys *s, const double *U, double lambda, int with_bc, double *dudp) {
+  if (!s || !U || !dudp) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    hipStream_t st = s->ctx->stream;
+    s->d_U.upload(U, (size_t)s->ndof, st);
+    AsmArgs a = asm_args(*s, s->d_U.p, s->d_U.p, s->d_F.p, nullptr, INFINITY, lambda);
+    launch_dudp(s->nnodes, s->nspec, s->d_bn_ptr.p, s->d_bn_reg.p, s->d_bn_fac.p, s->d_nodevol.p,
+                s->has_src ? s->d_src.p : nullptr, s->d_inst.p, s->d_tmp.p, st, with_bc ? 1 : 0);
+    if (s->has_dp) launch_dudp_coef(a, s->dpar, s->d_tmp.p, st);
+    s->d_tmp.download(dudp, (size_t)s->ndof, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    return VFVM_OK;
+  });
+}
+
 int vfvm_parameter_derivative(vfvm_sys *s, const double *U, double lambda, const vfvm_control_t *control,
                               double *dudl) {
   if (!s || !U || !dudl) return VFVM_ERR_ARG;
@@ -928,9 +1076,10 @@ int vfvm_parameter_derivative(vfvm_sys *s, const double *U, double lambda, const
     s->d_U.upload(U, (size_t)s->ndof, st);
     launch_fill_i32(1, s->d_flags.p, 0, st);
     AsmArgs a = asm_args(*s, s->d_U.p, s->d_U.p, s->d_tmp.p, s->d_J.p, INFINITY, lambda);
-    launch_assemble(a, s->asm_group, st);
+    run_assembly(*s, a, s->asm_group, st);
     launch_dudp(s->nnodes, s->nspec, s->d_bn_ptr.p, s->d_bn_reg.p, s->d_bn_fac.p, s->d_nodevol.p,
                 s->has_src ? s->d_src.p : nullptr, s->d_inst.p, s->d_F.p, st);
+    if (s->has_dp) launch_dudp_coef(a, s->dpar, s->d_F.p, st);
     s->lin.prepare(s->d_J.p, c, true);
     vfvm_linstats_t ls{};
     int rc = s->lin.solve(s->d_J.p, s->d_F.p, s->d_dU.p, c, &ls);
@@ -1155,7 +1304,7 @@ int vfvm_residual_norm(vfvm_sys *s, const double *U, const double *Uold, double 
     s->d_U.upload(U, (size_t)s->ndof, st);
     s->d_Uold.upload(Uold ? Uold : U, (size_t)s->ndof, st);
     AsmArgs a = asm_args(*s, s->d_U.p, s->d_Uold.p, s->d_tmp.p, nullptr, tstep, lambda);
-    launch_assemble(a, s->res_group, st);
+    run_assembly(*s, a, s->res_group, st);
     *norm = penalty_free_norm(*s, s->d_tmp.p);
     return VFVM_OK;
   });
@@ -1257,7 +1406,7 @@ int vfvm_time_kernel(vfvm_sys *s, int kernel, int reps, double *avg_ms) {
     auto once = [&]() {
       if (kernel == VFVM_KERNEL_ASSEMBLE || kernel == VFVM_KERNEL_RESIDUAL) {
         AsmArgs a = asm_args(*s, s->d_U.p, s->d_Uold.p, kernel == VFVM_KERNEL_ASSEMBLE ? s->d_F.p : s->d_tmp.p,
-                             kernel == VFVM_KERNEL_ASSEMBLE ? s->d_J.p : nullptr, 1.0, 0.0);
+                             kernel == VFVM_KERNEL_ASSEMBLE ? bulk_J(*s) : nullptr, 1.0, 0.0);
         launch_assemble(a, s->asm_group, st);
       } else if (kernel == VFVM_KERNEL_SPMV64) {
         s->lin.apply_plain(false, s->d_J.p, x.p, y.p);

@@ -102,7 +102,44 @@ struct Masks {
   uint8_t dmask[MAXSPEC];  // diagonal node block: flux (K side) | reaction | storage | boundary | i
   int ecnt[MAXSPEC], dcnt[MAXSPEC], eoff[MAXSPEC], doff[MAXSPEC];
   int etot, dtot;
+  // species the node and edge terms assemble: those enabled in the cell regions
+  // (isregionspecies, src/vfvm_assemblydata.jl:245-270); boundary species are not
+  uint8_t bulk;
 };
+
+// Boundary physics of one boundary region, as the boundary pass reads it (the registry's
+// breaction matrix, bflux coefficients, bstorage species bits, per-node fields flag).
+using BPhysDev = vfvm_bphysics_t;
+static_assert(MAXSPEC == 3, "vfvm_bphysics_t holds 3 species");
+
+// Arguments of the boundary pass (k_bnd): for each listed node, the terms the bulk kernel
+// leaves out, added to F and to the Jacobian of the full pattern in the reference's order
+// (boundary node items, then boundary edges, then inactive species; src/vfvm_assembly.jl:
+// 318-510, src/vfvm_system.jl:1010-1026).
+struct BndArgs {
+  int64_t nlist;
+  const int32_t *list;      // nodes with boundary-physics items, boundary edges or inactive species
+  const uint8_t *nodemask;  // species active at each node (node_dof)
+  const int32_t *bn_ptr, *bn_reg;
+  const double *bn_fac;
+  const int32_t *be_ptr;    // boundary-edge items of list entry t: be_ptr[t] .. be_ptr[t + 1]
+  const int32_t *be_nb, *be_reg;
+  const double *be_fac;
+  const BPhysDev *bp;       // MAXBREG entries
+  const double *bsrc, *bcoef;  // per-node fields, nspec x nnodes (nullable)
+  const int32_t *rowptr, *colind;  // the full pattern
+  const double *U, *Uold;
+  double *F, *J;            // J nullable: residual only
+  double tstepinv;
+  int32_t *flags;           // bit 1: a Jacobian slot the pattern lacks (cannot happen by construction)
+  int nspec;
+};
+void launch_bnd(const BndArgs &b, hipStream_t s);
+// y[k] = src[k] >= 0 ? x[src[k]] : 0 for k < n
+void launch_gather_or_zero(int64_t n, const int32_t *src, const double *x, double *y, hipStream_t s);
+// U[K, i] = 0 for the species inactive at the listed nodes (_initialize_inactive_dof!)
+void launch_init_inactive(int64_t nlist, const int32_t *list, const uint8_t *nodemask, int nspec, double *U,
+                          hipStream_t s);
 
 // Host-side discretisation data produced once per grid (vfvm_setup.cpp).
 struct HostSetup {
@@ -118,14 +155,37 @@ struct HostSetup {
 };
 
 // cs: coordinate system (geom::CS_*, vfvm_geometry.h)
+// keep_zero: keep node-graph entries whose summed factor is 0 (the cellwise pattern)
 void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, const int64_t *cellnodes,
-                 int64_t nbfaces, const int64_t *bfacenodes, const int64_t *bfaceregions, HostSetup &out, int cs = 0);
+                 int64_t nbfaces, const int64_t *bfacenodes, const int64_t *bfaceregions, HostSetup &out, int cs = 0,
+                 bool keep_zero = false);
 // the boundary node items of build_setup alone (bn_ptr, bn_reg, bn_fac, bnodes)
 void build_boundary(int dim, int64_t nnodes, const double *coord, int64_t nbfaces, const int64_t *bfacenodes,
                     const int64_t *bfaceregions, HostSetup &out, int cs = 0);
-Masks make_masks(int nspec, const vfvm_physics_t &ph, const vfvm_bc_t *bc, int nbreg);
+Masks make_masks(int nspec, const vfvm_physics_t &ph, const vfvm_bc_t *bc, int nbreg, uint8_t bulk = 0xff);
 void build_dof_csr(const HostSetup &hs, int nspec, const Masks &m, std::vector<int32_t> &rowptr,
                    std::vector<int32_t> &colind);
+
+// The boundary pass of a system (k_bnd): which species are active where, the nodes the
+// pass visits, their boundary-edge items, and the full Jacobian pattern.
+struct BndSetup {
+  bool on = false;           // the pass runs (boundary species, inactive dofs or boundary physics)
+  bool extra = false;        // the full pattern has entries the bulk pattern lacks
+  bool homogeneous = true;   // every species BULK
+  uint8_t bulk = 0;          // BULK species bits
+  std::vector<uint8_t> nodemask;
+  std::vector<int32_t> list, be_ptr, be_nb, be_reg;
+  std::vector<double> be_fac;
+  std::vector<int32_t> map;  // full-pattern slot -> bulk slot, or -1 (when extra)
+};
+// species: nspec entries or NULL; bp: nbreg entries or NULL.  rowptr / colind hold the bulk
+// pattern on entry and the full pattern on return.
+void build_bnd(int dim, int64_t nnodes, const double *coord, int64_t nbfaces, const int64_t *bfacenodes1,
+               const int64_t *bfaceregions, int cs, int nspec, const vfvm_species_t *species,
+               const vfvm_bphysics_t *bp, int nbreg, const HostSetup &hs, std::vector<int32_t> &rowptr,
+               std::vector<int32_t> &colind, BndSetup &out);
+// bulk-species bits of a species table (NULL: all)
+uint8_t bulk_species(int nspec, const vfvm_species_t *species);
 void verify_slot_map(const HostSetup &hs, int nspec, const Masks &m, const std::vector<int32_t> &rowptr,
                      const std::vector<int32_t> &colind);
 void build_csc_perm(int64_t ndof, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
@@ -165,6 +225,7 @@ struct AsmArgs {
   const InstParams *inst;        // device array of instance parameters
   const int32_t *inst_active;    // per-instance flag; nodes of inactive instances are skipped (nullable)
   const double *U, *Uold, *src;  // src may be nullptr
+  const double *rcoef;           // per-node reaction coefficient, reaction f_i += rcoef[K, i] u_i (nullable)
   double *F, *J;                 // J may be nullptr (residual only)
   double tstepinv, lambda;
   int32_t *flags;                // [0] NaN seen
@@ -184,8 +245,16 @@ void launch_init_dirichlet(int64_t nbnodes, const int32_t *bnodes, const int32_t
                            hipStream_t s);
 // ||F||_inf over rows without a Dirichlet penalty, per instance (inst may be null -> 1 instance)
 // dF/dlambda (exact: lambda enters linearly), one value per dof
+// with_bc = 0: without the boundary-condition values (the terms a test-function integral sees)
 void launch_dudp(int64_t nnodes, int nspec, const int32_t *bn_ptr, const int32_t *bn_reg, const double *bn_fac,
-                 const double *nodevol, const double *src, const InstParams *inst, double *out, hipStream_t s);
+                 const double *nodevol, const double *src, const InstParams *inst, double *out, hipStream_t s,
+                 int with_bc = 1);
+// d(registry coefficient)/dlambda of the parameter slots
+struct ParamDeriv {
+  double dp[16];
+};
+// adds dF/dlambda of the flux and reaction terms with lambda-dependent coefficients (a.U: the state)
+void launch_dudp_coef(const AsmArgs &a, const ParamDeriv &dp, double *out, hipStream_t s);
 void launch_penalty_free_norm(int64_t nnodes, int nspec, const double *F, const int32_t *bn_ptr,
                               const int32_t *bn_reg, const int32_t *node_inst, const InstParams *inst,
                               double *out_partial, int64_t nparts, hipStream_t s);

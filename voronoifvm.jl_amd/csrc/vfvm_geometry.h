@@ -176,10 +176,13 @@ __host__ __device__ inline void cell_factors(int dim, const double *coord, const
   }
 }
 
-// Boundary face measure parts per face node (n: 0-based face nodes).
+// Boundary face measure parts per face node (n: 0-based face nodes), and (eout, nullable)
+// the factors of the face's own edges for boundary-edge fluxes: 2D Cartesian 1 / length
+// (the cylindrical form sets none, so 0), 3D edge e = local_edge(2, e); 1D faces have no
+// edges (src/vfvm_formfactors.jl:263-332).
 template <class I>
 __host__ __device__ inline void bface_factors(int dim, const double *coord, const I *n, double *npar,
-                                              int cs = CS_CARTESIAN) {
+                                              int cs = CS_CARTESIAN, double *eout = nullptr) {
 #pragma clang fp contract(off)
   if (dim == 1) {
     // src/vfvm_formfactors.jl:244-262
@@ -195,6 +198,7 @@ __host__ __device__ inline void bface_factors(int dim, const double *coord, cons
     const double l = sqrt(dr * dr + dz * dz);
     npar[0] = PI * (r1 + rmid) * l / 2.0;
     npar[1] = PI * (r2 + rmid) * l / 2.0;
+    if (eout) eout[0] = 0.0;
     return;
   }
   if (dim == 2) {
@@ -203,6 +207,7 @@ __host__ __device__ inline void bface_factors(int dim, const double *coord, cons
     double d = sqrt(dx * dx + dy * dy);
     npar[0] = d / 2;
     npar[1] = d / 2;
+    if (eout) eout[0] = 1 / d;
     return;
   }
   double epar[3] = {0, 0, 0};
@@ -226,6 +231,8 @@ __host__ __device__ inline void bface_factors(int dim, const double *coord, cons
     npar[a] += epar[e] * d * 0.25;
     npar[b] += epar[e] * d * 0.25;
   }
+  if (eout)
+    for (int e = 0; e < 3; e++) eout[e] = epar[e] * d / dd[e];
 }
 
 }  // namespace geom

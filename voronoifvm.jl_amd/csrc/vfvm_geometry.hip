@@ -183,7 +183,7 @@ __global__ __launch_bounds__(GEOM_BLOCK) void k_merge_rows(int64_t nnodes, const
                                                            const int32_t *__restrict__ cn,
                                                            const double *__restrict__ cf, int32_t *__restrict__ mcol,
                                                            double *__restrict__ mfac, int32_t *__restrict__ rlen,
-                                                           double *__restrict__ nodevol) {
+                                                           double *__restrict__ nodevol, int keep_zero) {
   constexpr int NV = DIM + 1, NE = geom::edges_per_cell(DIM), ST = NV + NE;
   const int64_t i = (int64_t)blockIdx.x * GEOM_BLOCK + threadIdx.x;
   if (i >= nnodes) return;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(GEOM_BLOCK) void k_merge_rows(int64_t nnodes, const
       while (q < nu && uc[q] != j) q++;
       if (q < nu) {
         uf[q] = uf[q] + f;
-      } else if (f != 0.0) {
+      } else if (keep_zero || f != 0.0) {  // cellwise: every cell edge has an entry
         uc[nu] = j;
         uf[nu] = f;
         nu++;
@@ -379,7 +379,7 @@ void node_graph_device(vfvm_sys &s, const double *coord, const int64_t *cellnode
   mfac.alloc((size_t)std::max<int64_t>(nk * DIM, 1));
   s.d_nodevol.alloc((size_t)std::max<int64_t>(nn, 1));
   hipLaunchKernelGGL(k_merge_rows<DIM>, blocks(nn), dim3(GEOM_BLOCK), 0, st, nn, g.inc_ptr.p, g.inc_k.p, g.cn.p,
-                     g.cf.p, mcol.p, mfac.p, cnt.p, s.d_nodevol.p);
+                     g.cf.p, mcol.p, mfac.p, cnt.p, s.d_nodevol.p, (int)s.cellwise);
   s.d_adj_ptr.alloc((size_t)nn + 1);
   const int64_t nadj = exclusive_scan(nn, cnt.p, s.d_adj_ptr.p, st, h64);
   if (nadj >= (int64_t(1) << 31) - 1) throw Error(VFVM_ERR_ARG, "node graph too large for int32 offsets");

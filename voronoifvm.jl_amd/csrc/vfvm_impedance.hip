@@ -365,7 +365,7 @@ void residual_without_bc(vfvm_sys &s, const double *dU, const double *dUold, dou
   try {
     upload_inst(s);
     AsmArgs a = asm_args(s, dU, dUold, dF, nullptr, tstep, lambda);
-    launch_assemble(a, s.res_group, s.ctx->stream);
+    run_assembly(s, a, s.res_group, s.ctx->stream);
   } catch (...) {
     s.bc = saved;
     upload_inst(s);
@@ -385,7 +385,19 @@ extern "C" {
 
 int vfvm_impedance_destroy(vfvm_impedance *z);
 
+namespace {
+int impedance_create_at(vfvm_sys *s, const double *U0, int excited_spec, int excited_bc, double lambda,
+                        vfvm_impedance **out);
+}
+
 int vfvm_impedance_create(vfvm_sys *s, const double *U0, int excited_spec, int excited_bc, vfvm_impedance **out) {
+  return impedance_create_at(s, U0, excited_spec, excited_bc, 0.0, out);
+}
+
+namespace {
+// the impedance system of J(U0) at the system parameter lambda
+int impedance_create_at(vfvm_sys *s, const double *U0, int excited_spec, int excited_bc, double lambda,
+                        vfvm_impedance **out) {
   if (!s || !U0 || !out) return VFVM_ERR_ARG;
   try {
     check_dofs(*s, excited_spec);
@@ -400,8 +412,8 @@ int vfvm_impedance_create(vfvm_sys *s, const double *U0, int excited_spec, int e
     s->d_U.upload(U0, (size_t)s->ndof, st);
     z->J.alloc((size_t)s->nnz);
     launch_fill_i32(1, s->d_flags.p, 0, st);
-    AsmArgs a = asm_args(*s, s->d_U.p, s->d_U.p, s->d_F.p, z->J.p, 1.0e30, 0.0);
-    launch_assemble(a, s->asm_group, st);
+    AsmArgs a = asm_args(*s, s->d_U.p, s->d_U.p, s->d_F.p, z->J.p, 1.0e30, lambda);
+    run_assembly(*s, a, s->asm_group, st);
     int32_t fl = 0;
     VFVM_HIP_CHECK(hipMemcpyAsync(&fl, s->d_flags.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     VFVM_HIP_CHECK(hipStreamSynchronize(st));
@@ -439,19 +451,22 @@ int vfvm_impedance_create(vfvm_sys *s, const double *U0, int excited_spec, int e
     return VFVM_ERR_HIP;
   }
 }
+}  // namespace
 
 int vfvm_impedance_create_param(vfvm_sys *s, const double *U0, double lambda0, vfvm_impedance **out) {
   if (!s || !U0 || !out) return VFVM_ERR_ARG;
   // as the boundary-excited form with no boundary excitation, then F = -dF/dlambda
   // (src/vfvm_impedance.jl:49-112: params = [lambda0], F .-= dudp[1])
-  int rc = vfvm_impedance_create(s, U0, 1, MAXBREG + 1, out);
-  (void)lambda0;  // dF/dlambda does not depend on lambda for the registry
+  int rc = impedance_create_at(s, U0, 1, MAXBREG + 1, lambda0, out);
   if (rc) return rc;
   try {
     vfvm_impedance *z = *out;
     hipStream_t st = s->ctx->stream;
+    s->d_U.upload(U0, (size_t)s->ndof, st);
+    AsmArgs a = asm_args(*s, s->d_U.p, s->d_U.p, s->d_F.p, nullptr, INFINITY, lambda0);
     launch_dudp(s->nnodes, s->nspec, s->d_bn_ptr.p, s->d_bn_reg.p, s->d_bn_fac.p, s->d_nodevol.p,
                 s->has_src ? s->d_src.p : nullptr, s->d_inst.p, z->b.p, st);
+    if (s->has_dp) launch_dudp_coef(a, s->dpar, z->b.p, st);
     launch_negate(s->ndof, z->b.p, st);
     VFVM_HIP_CHECK(hipStreamSynchronize(st));
     return VFVM_OK;
@@ -488,7 +503,7 @@ int vfvm_measurement_derivatives(vfvm_sys *s, const double *U0, const double *T,
     s->d_U.upload(U0, (size_t)s->ndof, st);
     launch_fill_i32(1, s->d_flags.p, 0, st);
     AsmArgs a = asm_args(*s, s->d_U.p, s->d_U.p, s->d_F.p, s->d_J.p, INFINITY, 0.0);
-    launch_assemble(a, s->asm_group, st);
+    run_assembly(*s, a, s->asm_group, st);
     s->bc = saved;
     saved.clear();
     upload_inst(*s);

@@ -62,7 +62,7 @@ extern "C" int vfvm_residual_norm_device_noalloc(vfvm_sys *s, const double *dU, 
     double *F = base + kHead;
     AsmArgs a = asm_args(*s, dU, dUold ? dUold : dU, F, nullptr, tstep, lambda);
     a.flags = reinterpret_cast<int32_t *>(base);  // the residual-only assembly never sets it
-    launch_assemble(a, s->res_group, st);
+    run_assembly(*s, a, s->res_group, st);
     // the norm kernel max-reduces the bit patterns of non-negative doubles into dnorm
     launch_fill_u64(1, reinterpret_cast<unsigned long long *>(dnorm), st);
     launch_penalty_free_norm(s->nnodes, s->nspec, F, s->d_bn_ptr.p, s->d_bn_reg.p, nullptr, s->d_inst.p, dnorm, 1,

@@ -80,6 +80,10 @@ VD Dual<N> operator*(const Dual<N> &a, double s) {
   return r;
 }
 template <int N>
+VD Dual<N> operator*(double s, const Dual<N> &a) {
+  return a * s;
+}
+template <int N>
 VD Dual<N> operator+(const Dual<N> &a, double c) {
   Dual<N> r = a;
   r.v = a.v + c;
@@ -192,15 +196,18 @@ VD void bernoulli_pm(const T &x, T &bp, T &bm) {
 }
 
 // ---- flux registry: g = flux(u_k, u_l) ----
-template <int NS, class T>
-VD void eval_flux(int kind, const double *p, const T *uk, const T *ul, T *g) {
+// PT, the parameter type: double in the assembly; Dual<1> where the derivative with
+// respect to the system parameter is taken (the registry's params, dual lanes as the
+// reference's ForwardDiff over [u; params], src/vfvm_physics.jl:435-449)
+template <int NS, class T, class PT = double>
+VD void eval_flux(int kind, const PT *p, const T *uk, const T *ul, T *g) {
   switch (kind) {
     case VFVM_FLUX_LINEAR:
 #pragma unroll
       for (int i = 0; i < NS; i++) g[i] = (uk[i] - ul[i]) * p[i];
       break;
     case VFVM_FLUX_POWER: {
-      int m = (int)p[4];
+      int m = (int)val(p[4]);
       g[0] = (fpowi(uk[0], m) - fpowi(ul[0], m)) * p[0];
 #pragma unroll
       for (int i = 1; i < NS; i++) g[i] = mk<T>(0.0);
@@ -230,8 +237,8 @@ VD void eval_flux(int kind, const double *p, const T *uk, const T *ul, T *g) {
 }
 
 // ---- reaction registry: r = reaction(u) ----
-template <int NS, class T>
-VD void eval_reaction(int kind, const double *p, const T *u, T *r) {
+template <int NS, class T, class PT = double>
+VD void eval_reaction(int kind, const PT *p, const T *u, T *r) {
 #pragma unroll
   for (int i = 0; i < NS; i++) r[i] = mk<T>(0.0);
   switch (kind) {
@@ -239,7 +246,7 @@ VD void eval_reaction(int kind, const double *p, const T *u, T *r) {
       r[0] = (fexp(u[0]) - fexp(u[0] * -1.0)) * p[5];
       break;
     case VFVM_REACT_POW:
-      r[0] = fpowi(u[0], (int)p[6]) * p[5];
+      r[0] = fpowi(u[0], (int)val(p[6])) * p[5];
       break;
     case VFVM_REACT_EXCHANGE:
       if constexpr (NS >= 2) {

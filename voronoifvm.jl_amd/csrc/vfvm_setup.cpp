@@ -25,7 +25,8 @@ using geom::edges_per_cell;
 using geom::local_edge;
 
 void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, const int64_t *cellnodes1,
-                 int64_t nbfaces, const int64_t *bfacenodes1, const int64_t *bfaceregions, HostSetup &out, int cs) {
+                 int64_t nbfaces, const int64_t *bfacenodes1, const int64_t *bfaceregions, HostSetup &out, int cs,
+                 bool keep_zero) {
   if (dim < 1 || dim > 3) throw Error(VFVM_ERR_ARG, "dim must be 1, 2 or 3");
   if (nnodes >= (int64_t(1) << 31) - 1) throw Error(VFVM_ERR_ARG, "too many nodes for int32 indices");
   const int nv = dim + 1, ne = edges_per_cell(dim);
@@ -70,6 +71,8 @@ void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, c
   cn.clear();
   cn.shrink_to_fit();
   // merge duplicates per row (cell order), keep entries that received a nonzero value
+  // (edgewise: ExtendableSparse's += stores no zero), or every entry (cellwise: _addnz
+  // tests the flux derivative, not the factor, src/vfvm_assembly.jl:20-28)
   std::vector<int32_t> rlen((size_t)nnodes, 0);
   std::vector<int32_t> ucol((size_t)cnt[nnodes]);
   std::vector<double> ufac((size_t)cnt[nnodes]);
@@ -85,7 +88,7 @@ void build_setup(int dim, int64_t nnodes, const double *coord, int64_t ncells, c
       while (q < nu && uc[q] != col) q++;
       if (q < nu) {
         uf[q] = uf[q] + tfac[k];
-      } else if (tfac[k] != 0.0) {
+      } else if (keep_zero || tfac[k] != 0.0) {  // cellwise: every cell edge has an entry
         uc[nu] = col;
         uf[nu] = tfac[k];
         nu++;
@@ -181,9 +184,15 @@ void build_boundary(int dim, int64_t nnodes, const double *coord, int64_t nbface
 }
 
 // Registry sparsity: which unknowns each callback component depends on.
-Masks make_masks(int nspec, const vfvm_physics_t &ph, const vfvm_bc_t *bc, int nbreg) {
+Masks make_masks(int nspec, const vfvm_physics_t &ph, const vfvm_bc_t *bc, int nbreg, uint8_t bulk) {
   Masks m{};
+  m.bulk = (uint8_t)(bulk & ((1u << nspec) - 1u));
   for (int i = 0; i < nspec; i++) {
+    if (!((m.bulk >> i) & 1u)) {  // not a region species: only the self entry
+      m.emask[i] = 0;
+      m.dmask[i] = (uint8_t)(1u << i);
+      continue;
+    }
     uint8_t e = 0, r = 0;
     switch (ph.flux) {
       case VFVM_FLUX_LINEAR: e = (uint8_t)(1u << i); break;
@@ -200,6 +209,8 @@ Masks make_masks(int nspec, const vfvm_physics_t &ph, const vfvm_bc_t *bc, int n
       default: break;
     }
     if (ph.storage == VFVM_STOR_IDENTITY) r |= (uint8_t)(1u << i);
+    e &= m.bulk;  // columns of region species only
+    r &= m.bulk;
     m.emask[i] = e;
     m.dmask[i] = (uint8_t)(e | r | (1u << i));
   }
@@ -245,6 +256,150 @@ void build_dof_csr(const HostSetup &hs, int nspec, const Masks &m, std::vector<i
           if (mask & (1u << j)) colind[p++] = (int32_t)(M * nspec + j);
       }
     }
+  }
+}
+
+uint8_t bulk_species(int nspec, const vfvm_species_t *species) {
+  uint8_t b = 0;
+  for (int i = 0; i < nspec; i++)
+    if (!species || species[i].kind == VFVM_SPECIES_BULK) b |= (uint8_t)(1u << i);
+  return b;
+}
+
+// Species activity (node_dof of enable_species! / enable_boundary_species!,
+// src/vfvm_system.jl:433-515), the boundary-edge items of bflux (assemble_bedges over the
+// faces of the cellwise boundary data, src/vfvm_assembly.jl:444-510, factors of
+// bfacefactors!) and the entries their Jacobian terms add to the bulk pattern
+// (assemble_res_jac for bnode and bedge, src/vfvm_assemblydata.jl:282-303, 411-445:
+// species present at the node, resp. at both nodes; an entry exists where the callback's
+// derivative is nonzero).
+void build_bnd(int dim, int64_t nnodes, const double *coord, int64_t nbfaces, const int64_t *bfacenodes1,
+               const int64_t *bfaceregions, int cs, int nspec, const vfvm_species_t *species,
+               const vfvm_bphysics_t *bp, int nbreg, const HostSetup &hs, std::vector<int32_t> &rowptr,
+               std::vector<int32_t> &colind, BndSetup &out) {
+  const uint8_t all = (uint8_t)((1u << nspec) - 1u);
+  out = BndSetup{};
+  out.bulk = bulk_species(nspec, species);
+  out.homogeneous = out.bulk == all;
+  out.nodemask.assign((size_t)nnodes, out.bulk);
+  if (species)
+    for (int i = 0; i < nspec; i++) {
+      if (species[i].kind == VFVM_SPECIES_BULK || species[i].kind == VFVM_SPECIES_NONE) continue;
+      if (species[i].kind != VFVM_SPECIES_BOUNDARY) throw Error(VFVM_ERR_ARG, "unknown species kind");
+      for (int64_t ib = 0; ib < nbfaces; ib++) {
+        const int64_t r = bfaceregions[ib];
+        if (r < 1 || r > 32 || !((species[i].bregions >> (r - 1)) & 1u)) continue;
+        for (int k = 0; k < dim; k++) out.nodemask[(size_t)(bfacenodes1[dim * ib + k] - 1)] |= (uint8_t)(1u << i);
+      }
+    }
+  auto region_active = [&](int r) {
+    if (!bp || r < 1 || r > nbreg) return false;
+    const vfvm_bphysics_t &P = bp[r - 1];
+    bool a = P.storage != 0 || P.fields != 0;
+    for (int i = 0; i < nspec; i++)
+      for (int j = 0; j < nspec; j++) a |= P.react[i][j] != 0.0;
+    return a;
+  };
+  auto region_bflux = [&](int r) {
+    if (!bp || r < 1 || r > nbreg) return false;
+    bool a = false;
+    for (int i = 0; i < nspec; i++) a |= bp[r - 1].bflux[i] != 0.0;
+    return a;
+  };
+  bool needed = !out.homogeneous;
+  for (int r = 1; r <= nbreg; r++) needed |= region_active(r) || region_bflux(r);
+  if (!needed) return;  // the bulk kernel assembles everything
+  // boundary-edge items per node, in bface order
+  std::vector<std::vector<std::pair<int32_t, std::pair<int32_t, double>>>> items((size_t)nnodes);
+  bool any_be = false;
+  if (dim >= 2)
+    for (int64_t ib = 0; ib < nbfaces; ib++) {
+      const int r = (int)bfaceregions[ib];
+      if (!region_bflux(r)) continue;
+      int64_t bn[3];
+      for (int k = 0; k < dim; k++) bn[k] = bfacenodes1[dim * ib + k] - 1;
+      double npar[3], epar[3] = {0.0, 0.0, 0.0};
+      geom::bface_factors(dim, coord, bn, npar, cs, epar);
+      const int ne = dim == 3 ? 3 : 1;
+      for (int e = 0; e < ne; e++) {
+        int a = 0, b = 1;
+        if (dim == 3) geom::local_edge(2, e, a, b);
+        const int32_t K = (int32_t)bn[a], L = (int32_t)bn[b];
+        items[(size_t)K].push_back({L, {r, epar[e]}});
+        items[(size_t)L].push_back({K, {r, epar[e]}});
+        any_be = true;
+      }
+    }
+  // visited nodes
+  std::vector<std::vector<int32_t>> extra((size_t)nnodes * nspec);
+  bool any = !out.homogeneous || any_be;
+  for (int64_t K = 0; K < nnodes; K++) {
+    bool visit = out.nodemask[(size_t)K] != all || !items[(size_t)K].empty();
+    const uint8_t nm = out.nodemask[(size_t)K];
+    for (int32_t b = hs.bn_ptr[K]; b < hs.bn_ptr[K + 1]; b++) {
+      const int r = hs.bn_reg[b];
+      if (!region_active(r)) continue;
+      visit = true;
+      const vfvm_bphysics_t &P = bp[r - 1];
+      for (int i = 0; i < nspec; i++)
+        for (int j = 0; j < nspec; j++)
+          if (i != j && ((nm >> i) & 1u) && ((nm >> j) & 1u) && P.react[i][j] != 0.0)
+            extra[(size_t)K * nspec + i].push_back((int32_t)(K * nspec + j));
+    }
+    for (const auto &it : items[(size_t)K]) {
+      const int32_t L = it.first;
+      const vfvm_bphysics_t &P = bp[it.second.first - 1];
+      const uint8_t ml = out.nodemask[(size_t)L];
+      for (int i = 0; i < nspec; i++)
+        if (((nm >> i) & 1u) && ((ml >> i) & 1u) && P.bflux[i] != 0.0)
+          extra[(size_t)K * nspec + i].push_back((int32_t)(L * nspec + i));
+    }
+    if (visit) out.list.push_back((int32_t)K);
+    any |= visit;
+  }
+  out.on = any;
+  if (!out.on) return;
+  out.be_ptr.assign(out.list.size() + 1, 0);
+  for (size_t t = 0; t < out.list.size(); t++) {
+    for (const auto &it : items[(size_t)out.list[t]]) {
+      out.be_nb.push_back(it.first);
+      out.be_reg.push_back(it.second.first);
+      out.be_fac.push_back(it.second.second);
+    }
+    out.be_ptr[t + 1] = (int32_t)out.be_nb.size();
+  }
+  // full pattern: bulk columns and the extra ones, sorted per row
+  const int64_t ndof = nnodes * nspec;
+  std::vector<int32_t> rp((size_t)ndof + 1, 0), ci, map;
+  ci.reserve(colind.size());
+  map.reserve(colind.size());
+  for (int64_t r = 0; r < ndof; r++) {
+    auto &ex = extra[(size_t)r];
+    std::sort(ex.begin(), ex.end());
+    ex.erase(std::unique(ex.begin(), ex.end()), ex.end());
+    int32_t k = rowptr[r];
+    const int32_t k1 = rowptr[r + 1];
+    size_t x = 0;
+    while (k < k1 || x < ex.size()) {
+      if (x >= ex.size() || (k < k1 && colind[k] <= ex[x])) {
+        if (x < ex.size() && colind[k] == ex[x]) x++;
+        ci.push_back(colind[k]);
+        map.push_back(k);
+        k++;
+      } else {
+        ci.push_back(ex[x]);
+        map.push_back(-1);
+        x++;
+        out.extra = true;
+      }
+    }
+    rp[r + 1] = (int32_t)ci.size();
+  }
+  if (out.extra) {
+    if ((int64_t)ci.size() >= (int64_t(1) << 31) - 1) throw Error(VFVM_ERR_ARG, "Jacobian too large for int32 offsets");
+    rowptr.swap(rp);
+    colind.swap(ci);
+    out.map.swap(map);
   }
 }
 

@@ -49,6 +49,26 @@ struct vfvm_sys {
   DevBuf<uint16_t> d_adj_fidx;
   DevBuf<double> d_fac_tab;
   bool compact_graph = true;
+  bool cellwise = false;  // VFVM_ASSEMBLY_CELLWISE: node graph keeps zero-factor cell edges
+  // registry coefficients depending on the system parameter: p_j + dpar.dp[j] lambda; the
+  // device parameter block holds them at inst_lambda
+  ParamDeriv dpar{};
+  bool has_dp = false;
+  double inst_lambda = 0.0;
+  // per-node bulk reaction coefficient (vfvm_system_set_fields)
+  DevBuf<double> d_rcoef;
+  bool has_rcoef = false;
+  // boundary pass: boundary species, inactive dofs, boundary physics (k_bnd)
+  struct Bnd {
+    BndSetup hs;
+    int64_t nnz_bulk = 0;
+    std::vector<vfvm_bphysics_t> bp;  // MAXBREG entries
+    DevBuf<uint8_t> d_nodemask;
+    DevBuf<int32_t> d_list, d_be_ptr, d_be_nb, d_be_reg, d_map;
+    DevBuf<double> d_be_fac, d_Jb, d_bsrc, d_bcoef;
+    DevBuf<vfvm_bphysics_t> d_bp;
+    bool has_bsrc = false, has_bcoef = false;
+  } bnd;
   int64_t sell_nnz = 0;
   LinearSolver lin;
   int asm_group = 8, spmv_group = 8;
@@ -80,4 +100,9 @@ void update_geometry(vfvm_sys &s, const double *coord, bool on_device = false);
 void sync_host_geometry(vfvm_sys &s);
 // device address of the no-alloc entry point's static array on `device` (current), once per device
 void noalloc_setup(int device);
+// one assembly of the system (the bulk kernel with `group`, then the boundary pass when the
+// system has one); a.J null: residual only
+void run_assembly(vfvm_sys &s, AsmArgs a, int group, hipStream_t st);
+// _initialize_inactive_dof! on device U (no-op for systems without inactive dofs)
+void init_inactive(vfvm_sys &s, double *U, hipStream_t st);
 }

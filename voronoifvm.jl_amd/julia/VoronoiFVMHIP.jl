@@ -12,18 +12,126 @@
 #                     (src/vfvm_diffeq_interface.jl)  -> hip_eval_rhs!, hip_eval_jacobian!, hip_mass_matrix
 #   ImpedanceSystem, impedance, measurement_derivative
 #                     (src/vfvm_impedance.jl)         -> hip_impedance, hip_measurement_derivatives
-# A grid with circular_symmetric! / spherical_symmetric! goes to vfvm_system_create_cs.
+# hip_system reads the grid's coordinate system (circular_symmetric! /
+# spherical_symmetric!), the species activity (enable_species! /
+# enable_boundary_species!) and the assembly variant of the System.
 #
-# The Physics callbacks are Julia closures the device cannot call; a system is
-# mapped to a registry entry with `hip_physics` (flux/reaction/storage kinds and
-# parameters) and its boundary conditions to a table with `hip_bc_table`.
+# The Physics callbacks are Julia closures the device cannot call.  The registry terms
+# below (LinearFlux, SinhReaction, ..., BoundaryConditions) are ordinary callables with
+# the reference's callback signatures, so a Physics built from them runs unchanged on
+# the reference's CPU path; `hip_physics` and `hip_bc_table` recognise them and map the
+# system onto the device registry.  A Physics with any other callback maps to
+# `nothing`: such a system stays on the reference CPU path (`hip_enabled`).
 module VoronoiFVMHIP
 
 using LinearAlgebra, SparseArrays
+using VoronoiFVM, ExtendableGrids
 const libvfvm = joinpath(@__DIR__, "..", "libvfvm_hip.so")
 
 # status codes (src/vfvm_logging_exceptions.jl:6-29)
 const VFVM_OK, VFVM_ERR_CONVERGENCE, VFVM_ERR_ASSEMBLY, VFVM_ERR_LINEAR = 0, 1, 2, 3
+
+# ---- registry terms: callables with VoronoiFVM's callback signatures ----
+# (include/vfvm_hip.h lists the formulas and the examples they restate)
+abstract type RegistryTerm end
+
+struct LinearFlux <: RegistryTerm  # Example105_NonlinearPoisson1D.jl:41-44, per species D
+    D::Vector{Float64}
+end
+function (t::LinearFlux)(f, u, edge, data)
+    for i in eachindex(t.D)
+        f[i] = t.D[i] * (u[i, 1] - u[i, 2])
+    end
+    return nothing
+end
+
+struct PowerFlux <: RegistryTerm  # Example106_NonlinearDiffusion1D.jl:57-60
+    m::Int
+    D::Float64
+end
+(t::PowerFlux)(f, u, edge, data) = (f[1] = t.D * (u[1, 1]^t.m - u[1, 2]^t.m); nothing)
+
+struct NonlinearDiffusionFlux <: RegistryTerm  # D(ū) (u_k - u_l), D(u) = d0 + d2 u^2
+    d0::Float64
+    d2::Float64
+end
+function (t::NonlinearDiffusionFlux)(f, u, edge, data)
+    ū = (u[1, 1] + u[1, 2]) / 2
+    f[1] = (t.d2 * ū^2 + t.d0) * (u[1, 1] - u[1, 2])
+    return nothing
+end
+
+struct DriftDiffusionFlux <: RegistryTerm  # psi, n, p; Example160_UnipolarDriftDiffusion1D.jl:43-50
+    λ2::Float64
+    zn::Float64
+    zp::Float64
+end
+function (t::DriftDiffusionFlux)(f, u, edge, data)
+    dψ = u[1, 1] - u[1, 2]
+    f[1] = t.λ2 * dψ
+    bp, bm = VoronoiFVM.fbernoulli_pm(t.zn * dψ)
+    f[2] = bm * u[2, 1] - bp * u[2, 2]
+    bp, bm = VoronoiFVM.fbernoulli_pm(t.zp * dψ)
+    f[3] = bm * u[3, 1] - bp * u[3, 2]
+    return nothing
+end
+
+struct SinhReaction <: RegistryTerm  # Example105:56-59
+    k::Float64
+end
+(t::SinhReaction)(f, u, node, data) = (f[1] = t.k * (exp(u[1]) - exp(-u[1])); nothing)
+
+struct PowerReaction <: RegistryTerm  # Example207_NonlinearPoisson2D.jl:34-36
+    m::Int
+    k::Float64
+end
+(t::PowerReaction)(f, u, node, data) = (f[1] = t.k * u[1]^t.m; nothing)
+
+struct ExchangeReaction <: RegistryTerm  # Example210:25-30
+    k::Float64
+end
+(t::ExchangeReaction)(f, u, node, data) = (f[1] = t.k * (u[1] - u[2]); f[2] = t.k * (u[2] - u[1]); nothing)
+
+struct DriftDiffusionReaction <: RegistryTerm  # n - p; (np - ni^2) / (n + p + 2 ni)
+    ni::Float64
+end
+function (t::DriftDiffusionReaction)(f, u, node, data)
+    f[1] = u[2] - u[3]
+    R = (u[2] * u[3] - t.ni^2) / (u[2] + u[3] + 2 * t.ni)
+    f[2] = R
+    f[3] = R
+    return nothing
+end
+
+struct IdentityStorage <: RegistryTerm  # Example106:63-66
+end
+(t::IdentityStorage)(f, u, node, data) = (f .= u; nothing)
+
+# bcondition entries: (:dirichlet | :robin, species, region, value, factor); value(λ) =
+# value + value_lambda * embedparam
+struct BCEntry
+    kind::Symbol
+    species::Int
+    region::Int
+    value::Float64
+    value_lambda::Float64
+    factor::Float64
+end
+struct BoundaryConditions <: RegistryTerm
+    entries::Vector{BCEntry}
+end
+function (t::BoundaryConditions)(f, u, bnode, data)
+    for e in t.entries
+        v = e.value + e.value_lambda * VoronoiFVM.embedparam(bnode)
+        if e.kind == :dirichlet
+            VoronoiFVM.boundary_dirichlet!(f, u, bnode; species = e.species, region = e.region, value = v)
+        else
+            VoronoiFVM.boundary_robin!(f, u, bnode; species = e.species, region = e.region, factor = e.factor,
+                                       value = v)
+        end
+    end
+    return nothing
+end
 
 struct PhysicsT
     flux::Int32
@@ -77,25 +185,160 @@ function context(device = 0)
     return ctx[]
 end
 
-"""
-    hip_system(ctx, grid, physics::PhysicsT, bc::Vector{BCT}, nspec)
+# registry ids (include/vfvm_hip.h)
+const FLUX_IDS = Dict(LinearFlux => 1, PowerFlux => 2, NonlinearDiffusionFlux => 3, DriftDiffusionFlux => 4)
+const REACT_IDS = Dict(SinhReaction => 1, PowerReaction => 2, ExchangeReaction => 3, DriftDiffusionReaction => 4)
+isnofunc(fn) = fn === VoronoiFVM.nofunc
 
-Uploads an ExtendableGrid (1-based Int64 CellNodes / BFaceNodes, as stored) and
-builds the device system: what System(grid; ...) + _complete! do for the path.
 """
-function hip_system(ctx, grid, physics::PhysicsT, bc::Vector{BCT}, nspec)
+    hip_physics(physics::VoronoiFVM.Physics, nspec) -> Union{PhysicsT, Nothing}
+
+The registry entry of a Physics whose flux, reaction and storage are registry terms (or
+absent) and whose other callbacks are absent or a `BoundaryConditions`; `nothing`
+otherwise: that system stays on the reference's CPU path.
+"""
+function hip_physics(physics, nspec)
+    p = zeros(16)
+    flux, react, stor = 0, 0, 0
+    fl, re, st = physics.flux, physics.reaction, physics.storage
+    if fl isa LinearFlux
+        length(fl.D) == nspec || return nothing
+        p[1:nspec] .= fl.D
+    elseif fl isa PowerFlux
+        p[1] = fl.D; p[5] = fl.m
+    elseif fl isa NonlinearDiffusionFlux
+        p[1] = fl.d0; p[4] = fl.d2
+    elseif fl isa DriftDiffusionFlux
+        nspec == 3 || return nothing
+        p[8] = fl.λ2; p[9] = fl.zn; p[10] = fl.zp
+    elseif !isnofunc(fl)
+        return nothing
+    end
+    isnofunc(fl) || (flux = FLUX_IDS[typeof(fl)])
+    if re isa SinhReaction
+        p[6] = re.k
+    elseif re isa PowerReaction
+        p[6] = re.k; p[7] = re.m
+    elseif re isa ExchangeReaction
+        p[6] = re.k
+    elseif re isa DriftDiffusionReaction
+        p[11] = re.ni^2; p[12] = re.ni
+    elseif !isnofunc(re)
+        return nothing
+    end
+    isnofunc(re) || (react = REACT_IDS[typeof(re)])
+    if st isa IdentityStorage
+        stor = 1
+    elseif !isnofunc(st) && st !== VoronoiFVM.default_storage
+        return nothing
+    end
+    bc = physics.breaction
+    (isnofunc(bc) || bc isa BoundaryConditions) || return nothing
+    for fn in (physics.bflux, physics.bstorage, physics.bsource, physics.edgereaction, physics.generic)
+        isnofunc(fn) || return nothing
+    end
+    src = physics.source
+    isnofunc(src) || return nothing  # per-node sources go through vfvm_system_set_source
+    return PhysicsT(flux, react, stor, 0, Tuple(p))
+end
+
+"""
+    hip_bc_table(sys::VoronoiFVM.System, nbreg) -> Vector{BCT}
+
+The (boundary region, species) table of a system: the entries of a BoundaryConditions
+breaction, then the legacy conditions of boundary_dirichlet! / boundary_robin!
+(sys.boundary_factors / boundary_values, src/vfvm_system.jl:854-933).
+"""
+function hip_bc_table(sys, nbreg)
+    nspec = VoronoiFVM.num_species(sys)
+    tab = [BCT(0, 0, 0.0, 0.0, 0.0) for _ in 1:(nbreg * nspec)]
+    bc = sys.physics.breaction
+    if bc isa BoundaryConditions
+        for e in bc.entries
+            k = (e.region - 1) * nspec + e.species
+            tab[k] = BCT(e.kind == :dirichlet ? 1 : 3, 0, e.value, e.value_lambda, e.factor)
+        end
+    end
+    for ireg in 1:nbreg, ispec in 1:nspec
+        fac, val = sys.boundary_factors[ispec, ireg], sys.boundary_values[ispec, ireg]
+        (fac == 0 && val == 0) && continue
+        k = (ireg - 1) * nspec + ispec
+        tab[k] = fac ≈ VoronoiFVM.Dirichlet(Float64) ? BCT(2, 0, val, 0.0, 0.0) : BCT(3, 0, val, 0.0, fac)
+    end
+    return tab
+end
+
+struct SpeciesT
+    kind::Int32
+    bregions::UInt32
+end
+
+"""
+    hip_species(sys) -> Union{Vector{SpeciesT}, Nothing}
+
+Species activity: BULK (enabled in every cell region), BOUNDARY (with its boundary
+regions) or NONE; `nothing` for a bulk species restricted to some of several cell regions.
+"""
+function hip_species(sys)
+    nspec = VoronoiFVM.num_species(sys)
+    ncreg = size(sys.region_species, 2)
+    out = SpeciesT[]
+    for ispec in 1:nspec
+        if VoronoiFVM.is_boundary_species(sys, ispec)
+            bits = UInt32(0)
+            for r in 1:size(sys.bregion_species, 2)
+                sys.bregion_species[ispec, r] > 0 && (bits |= UInt32(1) << (r - 1))
+            end
+            push!(out, SpeciesT(1, bits))
+        elseif VoronoiFVM.is_bulk_species(sys, ispec)
+            all(r -> sys.region_species[ispec, r] > 0, 1:ncreg) || return nothing
+            push!(out, SpeciesT(0, 0))
+        else
+            push!(out, SpeciesT(2, 0))
+        end
+    end
+    return out
+end
+
+"""
+    hip_enabled(sys) -> Bool
+
+Whether a VoronoiFVM.System maps onto the device registry; if not, the reference's own
+CPU path runs it.
+"""
+hip_enabled(sys) = hip_physics(sys.physics, VoronoiFVM.num_species(sys)) !== nothing && hip_species(sys) !== nothing &&
+                   coordsys_code(sys.grid) !== nothing
+
+"""
+    hip_system(ctx, sys::VoronoiFVM.System) -> handle
+
+Uploads the system's ExtendableGrid (1-based Int64 CellNodes / BFaceNodes, as stored)
+and builds the device system: what System(grid; ...) + _complete! do for the path, in
+the grid's coordinate system, with the system's species activity and assembly variant.
+Throws for a system outside the registry (check `hip_enabled` first).
+"""
+function hip_system(ctx, sys)
+    grid = sys.grid
+    nspec = VoronoiFVM.num_species(sys)
+    physics = hip_physics(sys.physics, nspec)
+    species = hip_species(sys)
+    cs = coordsys_code(grid)
+    (physics === nothing || species === nothing || cs === nothing) &&
+        error("system outside the device registry: run it on the reference CPU path")
     coord = grid[Coordinates]
     cn = Int64.(grid[CellNodes]); cr = Int64.(grid[CellRegions])
     bn = Int64.(grid[BFaceNodes]); br = Int64.(grid[BFaceRegions])
     nbreg = maximum(br)
-    sys = Ref{Ptr{Cvoid}}(C_NULL)
-    rc = ccall((:vfvm_system_create, libvfvm), Cint,
+    bc = hip_bc_table(sys, nbreg)
+    assembly = sys.assembly_type == :cellwise ? Cint(1) : Cint(0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:vfvm_system_create_ex, libvfvm), Cint,
                (Ptr{Cvoid}, Cint, Int64, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64},
-                Cint, Ref{PhysicsT}, Cint, Ptr{BCT}, Ptr{Ptr{Cvoid}}),
+                Cint, Ref{PhysicsT}, Cint, Ptr{BCT}, Cint, Ptr{SpeciesT}, Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}),
                ctx, size(coord, 1), size(coord, 2), coord, size(cn, 2), cn, cr, size(bn, 2), bn, br,
-               nspec, physics, nbreg, bc, sys)
+               nspec, physics, nbreg, bc, cs, species, C_NULL, assembly, h)
     check(rc, ctx)
-    return sys[]
+    return h[]
 end
 
 """
@@ -256,18 +499,22 @@ function hip_integrate(ctx, sys, T, U, nspec; Uold = nothing, tstep = Inf)
 end
 
 """
-    coordsys_code(grid) -> Cint
+    coordsys_code(grid) -> Union{Cint, Nothing}
 
 The VFVM_COORD_* code of grid[CoordinateSystem] (circular_symmetric! /
-spherical_symmetric!); pass it to vfvm_system_create_cs instead of calling
-vfvm_system_create.
+spherical_symmetric!) for the grid's dimension; `nothing` for a coordinate system the
+device form factors do not cover (that system stays on the reference CPU path).
 """
 function coordsys_code(grid)
     cs = grid[CoordinateSystem]
-    cs <: Polar1D && return Cint(1)
-    cs <: Spherical1D && return Cint(2)
-    cs <: Cylindrical2D && return Cint(3)
-    return Cint(0)
+    dim = size(grid[Coordinates], 1)
+    cs <: Cartesian1D && dim == 1 && return Cint(0)
+    cs <: Cartesian2D && dim == 2 && return Cint(0)
+    cs <: Cartesian3D && dim == 3 && return Cint(0)
+    cs <: Polar1D && dim == 1 && return Cint(1)
+    cs <: Spherical1D && dim == 1 && return Cint(2)
+    cs <: Cylindrical2D && dim == 2 && return Cint(3)
+    return nothing
 end
 
 struct StepControlT

@@ -6,14 +6,15 @@ is done by libvfvm_hip.so (HIP kernels behind the C ABI in include/vfvm_hip.h).
 from ._capi import AssemblyError, ConvergenceError, DeviceArray, LinearSolverError, VFVMError
 from .control import SolverControl, fixed_timesteps
 from .grids import Grid, circular_symmetric, linspace, read_sg, simplexgrid, spherical_symmetric
-from .physics import (Dirichlet, DriftDiffusionFlux, DriftDiffusionReaction, ExchangeReaction, IdentityStorage,
-                      LinearFlux, Neumann, NonlinearDiffusionFlux, Physics, PowerFlux, PowerReaction, Robin,
-                      SinhReaction)
+from .physics import (BoundaryFlux, BoundaryReaction, BoundaryReactionField, BoundarySource, BoundaryStorage,
+                      Dirichlet, DriftDiffusionFlux, DriftDiffusionReaction, ExchangeReaction, IdentityStorage,
+                      LinearFlux, Neumann, NonlinearDiffusionFlux, Physics, PowerFlux, PowerReaction, ReactionField,
+                      Robin, SinhReaction)
 from .system import (Batch, NewtonSolverHistory, System, TransientSolution, boundary_dirichlet, boundary_neumann,
-                     boundary_robin, dofs, enable_species, eval_jacobian, eval_rhs, evaluate_residual_and_jacobian,
-                     integrate, mass_matrix, solve, unknowns)
+                     boundary_robin, dofs, enable_boundary_species, enable_species, eval_jacobian, eval_rhs, evaluate_residual_and_jacobian,
+                     initialize_dirichlet, integrate, mass_matrix, solve, unknowns)
 
-from .impedance import (ImpedanceSystem, assemble_dudp, impedance, integrate_testfunction, measurement_derivatives,
+from .impedance import (ImpedanceSystem, assemble_dudp, assemble_dudp_at, impedance, integrate_testfunction, measurement_derivatives,
                         parameter_derivative, testfunction)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -27,6 +27,19 @@ class BCT(C.Structure):
                 ("factor", C.c_double)]
 
 
+class SpeciesT(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("bregions", C.c_uint32)]
+
+
+class BPhysT(C.Structure):
+    _fields_ = [("react", (C.c_double * 3) * 3), ("bflux", C.c_double * 3), ("storage", C.c_int32),
+                ("fields", C.c_int32)]
+
+
+SPECIES_BULK, SPECIES_BOUNDARY, SPECIES_NONE = range(3)
+ASSEMBLY = {"edgewise": 0, "cellwise": 1}
+
+
 class ControlT(C.Structure):
     _fields_ = [("abstol", C.c_double), ("reltol", C.c_double), ("tol_round", C.c_double), ("tol_mono", C.c_double),
                 ("damp_initial", C.c_double), ("damp_growth", C.c_double), ("maxiters", C.c_int32),
@@ -71,6 +84,11 @@ SIGNATURES = {
     "vfvm_system_create_cs": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_int64, _lp, _lp, C.c_int64, _lp, _lp,
                                         C.c_int, C.POINTER(PhysicsT), C.c_int, C.POINTER(BCT), C.c_int,
                                         C.POINTER(_vp)]),
+    "vfvm_system_create_ex": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_int64, _lp, _lp, C.c_int64, _lp, _lp,
+                                        C.c_int, C.POINTER(PhysicsT), C.c_int, C.POINTER(BCT), C.c_int,
+                                        C.POINTER(SpeciesT), C.POINTER(BPhysT), C.c_int, C.POINTER(_vp)]),
+    "vfvm_system_set_fields": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "vfvm_node_species": (C.c_int, [_vp, C.POINTER(C.c_uint8)]),
     "vfvm_system_destroy": (C.c_int, [_vp]),
     "vfvm_system_set_source": (C.c_int, [_vp, _dp]),
     "vfvm_system_set_bc": (C.c_int, [_vp, C.POINTER(BCT)]),
@@ -83,6 +101,8 @@ SIGNATURES = {
     "vfvm_impedance_destroy": (C.c_int, [_vp]),
     "vfvm_impedance_create_param": (C.c_int, [_vp, _dp, C.c_double, C.POINTER(_vp)]),
     "vfvm_assemble_dudp": (C.c_int, [_vp, _dp]),
+    "vfvm_system_set_param_dependence": (C.c_int, [_vp, _dp]),
+    "vfvm_assemble_dudp_at": (C.c_int, [_vp, _dp, C.c_double, C.c_int, _dp]),
     "vfvm_parameter_derivative": (C.c_int, [_vp, _dp, C.c_double, C.POINTER(ControlT), _dp]),
     "vfvm_measurement_derivatives": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, _dp]),
     "vfvm_integrate_testfunction": (C.c_int, [_vp, _dp, _dp, C.c_double, _dp, _dp]),

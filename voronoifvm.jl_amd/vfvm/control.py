@@ -30,7 +30,9 @@ class SolverControl:
     gmres_restart: int = 30
     max_refinements: int = 20
     inner_reltol: float = 1.0e-3
-    reuse_precon: bool = False
+    # src/vfvm_solvercontrol.jl:123-131: the preconditioner (ILU(0) factor) of the first
+    # Newton step is reused by the later ones unless this is set
+    factorize_every_newtonstep: bool = False
     check_every: int = 4
     # time stepping / embedding step control (src/vfvm_solvercontrol.jl:138-211)
     Δp: float = 1.0
@@ -66,7 +68,7 @@ class SolverControl:
         for name in ("maxiters", "max_round", "maxiters_linear", "gmres_restart", "max_refinements", "check_every"):
             setattr(c, name, int(getattr(self, name)))
         c.updatecontrol = int(bool(self.updatecontrol))
-        c.reuse_precon = int(bool(self.reuse_precon))
+        c.reuse_precon = int(not self.factorize_every_newtonstep)
         c.is_linear = int(bool(is_linear))
         c.krylov = _KRYLOV[self.krylov]
         c.precon = _PRECON[self.precon]

@@ -69,6 +69,17 @@ def assemble_dudp(sys: System):
     return d
 
 
+def assemble_dudp_at(sys: System, U, lam=0.0, with_bc=True):
+    """state.dudp[1] = dF/dlambda at U and lambda, including the flux and reaction terms
+    whose coefficients depend on lambda (Physics.dflux / dreaction); with_bc=False leaves
+    out the boundary conditions (the derivative of a test-function integral)."""
+    u = _dofvec(U)
+    d = np.zeros(u.size)
+    _capi.check(_capi.lib().vfvm_assemble_dudp_at(sys.handle(), _capi.dptr(u), float(lam), int(bool(with_bc)),
+                                                  _capi.dptr(d)), sys.ctx.handle)
+    return d
+
+
 def parameter_derivative(sys: System, U, lam=0.0, control=None):
     """du/dlambda = -J(U)^-1 dF/dlambda at a stationary solution U for parameter lam."""
     c = SolverControl() if control is None else control

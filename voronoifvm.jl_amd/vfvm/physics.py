@@ -142,18 +142,101 @@ def Neumann(species=1, region=1, value=0.0):
     return Robin(species=species, region=region, factor=0.0, value=value)
 
 
+# ---- boundary physics (assemble_bnodes / assemble_bedges, src/vfvm_assembly.jl:318-510) ----
+
+@dataclass
+class BoundaryReaction:
+    """breaction on boundary region `region`: f_i = sum_j matrix[i][j] u_j (species in
+    1-based order, rows = equations), e.g. the exchange of
+    Example220_NonlinearPoisson2D_BoundarySpecies.jl:25-33 or
+    Example311_HeatEquation_BoundaryDiffusion.jl:66-75."""
+    region: int = 1
+    matrix: Sequence[Sequence[float]] = ()
+
+
+@dataclass
+class BoundaryStorage:
+    """bstorage on `region`: f_i = u_i for `species` (Example220:33-37)."""
+    region: int = 1
+    species: int = 1
+
+
+@dataclass
+class BoundaryFlux:
+    """bflux on the boundary edges of `region`: f_i = D (u_i,K - u_i,L) for `species`
+    (Example230_BoundaryFlux.jl:125-130, Example311:58-64)."""
+    region: int = 1
+    species: int = 1
+    D: float = 1.0
+
+
+@dataclass
+class BoundarySource:
+    """bsource on `region` for `species`: per-node values (array over all nodes) or a
+    callable of the node coordinates (Example230:145-156)."""
+    region: int = 1
+    species: int = 1
+    values: Union[np.ndarray, Callable, float] = 0.0
+
+
+@dataclass
+class BoundaryReactionField:
+    """breaction term c_K u_i on `region` for `species` with per-node c (array or callable
+    of the coordinates), e.g. Example230:132-143."""
+    region: int = 1
+    species: int = 1
+    values: Union[np.ndarray, Callable, float] = 0.0
+
+
+@dataclass
+class ReactionField:
+    """bulk reaction term c_K u_i for `species` with per-node c (array or callable of the
+    coordinates), a reaction callback reading node data (Example230:63-66)."""
+    species: int = 1
+    values: Union[np.ndarray, Callable, float] = 0.0
+
+
 @dataclass
 class Physics:
-    """VoronoiFVM.Physics(; flux, reaction, storage, source, bcondition).
+    """VoronoiFVM.Physics(; flux, reaction, storage, source, bcondition, breaction,
+    bstorage, bflux, bsource).
 
     embed_source: the source callback returns embedparam(node) * f, so that
-    solve(...; embed) ramps the source with the embedding parameter."""
+    solve(...; embed) ramps the source with the embedding parameter.
+    breaction: BoundaryReaction / BoundaryReactionField entries; bstorage:
+    BoundaryStorage; bflux: BoundaryFlux; bsource: BoundarySource; reaction_fields:
+    ReactionField entries added to the bulk reaction."""
     flux: Optional[object] = None
     reaction: Optional[object] = None
     storage: Optional[object] = None
     source: Optional[Union[np.ndarray, Callable]] = None
     bcondition: list = field(default_factory=list)
     embed_source: bool = False
+    breaction: list = field(default_factory=list)
+    bstorage: list = field(default_factory=list)
+    bflux: list = field(default_factory=list)
+    bsource: list = field(default_factory=list)
+    reaction_fields: list = field(default_factory=list)
+    # registry objects whose coefficients are the derivatives of flux's and reaction's
+    # coefficients with respect to the system parameter lambda (the coefficient is
+    # c + dc lambda; Example430_ParameterDerivativesStationary.jl:142-160 passes its p
+    # this way).  Exponents do not depend on lambda.
+    dflux: Optional[object] = None
+    dreaction: Optional[object] = None
+
+    def dparam_block(self, nspec):
+        """d(parameter slot)/dlambda, 16 values (zero where nothing depends on lambda)"""
+        dp = np.zeros(16)
+        if self.dflux is not None:
+            self.dflux.fill(dp, nspec)
+        if self.dreaction is not None:
+            self.dreaction.fill(dp, nspec)
+        dp[4] = dp[6] = 0.0  # integer exponents
+        dp[12] = 0.0  # the embedded-source flag
+        return dp
+
+    def has_boundary_physics(self):
+        return bool(self.breaction or self.bstorage or self.bflux or self.bsource)
 
     def param_block(self, nspec):
         p = np.zeros(16)

@@ -22,7 +22,9 @@ import numpy as np
 from . import _capi
 from .control import SolverControl
 from .grids import Grid
-from .physics import BC_DIRICHLET, BC_DIRICHLET_LEGACY, BC_ROBIN, Dirichlet, Physics, Robin
+from .physics import (BC_DIRICHLET, BC_DIRICHLET_LEGACY, BC_ROBIN, BoundaryFlux, BoundaryReaction,
+                      BoundaryReactionField, BoundarySource, BoundaryStorage, Dirichlet, Physics, ReactionField,
+                      Robin)
 
 
 @dataclass
@@ -38,16 +40,24 @@ class NewtonSolverHistory:
 
 
 class System:
-    """VoronoiFVM.System(grid, physics; species, is_linear)."""
+    """VoronoiFVM.System(grid, physics; species, is_linear, assembly).
+
+    assembly: "edgewise" (factors summed per edge; no Jacobian entry for an edge whose
+    factor is zero) or "cellwise" (the reference's default: an entry for every cell edge;
+    values equal to rounding), src/vfvm_system.jl:215-227."""
 
     def __init__(self, grid: Grid, physics: Optional[Physics] = None, species=None, is_linear=False,
-                 context=None, nspec=None, **physics_kw):
+                 context=None, nspec=None, assembly="edgewise", **physics_kw):
         self.grid = grid
         self.physics = physics if physics is not None else Physics(**physics_kw)
         self.is_linear = is_linear
         self.species = sorted(set(species or []))
         if nspec is not None:
             self.species = list(range(1, nspec + 1))
+        if assembly not in _capi.ASSEMBLY:
+            raise ValueError("assembly must be 'edgewise' or 'cellwise'")
+        self.assembly = assembly
+        self._bspecies = {}  # boundary species -> boundary regions
         self._legacy = {}  # (region, species) -> (kind, value, factor)
         self._ctx = context
         self._h = None
@@ -56,13 +66,121 @@ class System:
 
     # --- species and legacy boundary conditions -------------------------------------------------
     def enable_species(self, ispec, regions=None):
+        """enable_species!(sys, ispec, regions) (src/vfvm_system.jl:433-455).  The registry
+        path assembles a bulk species in every cell region: `regions` must name all of the
+        grid's cell regions (or be None)."""
+        if ispec in self._bspecies:
+            raise ValueError(f"species {ispec} is already a boundary species")
+        if regions is not None:
+            allreg = set(int(r) for r in np.unique(self.grid.cellregions))
+            if set(int(r) for r in regions) != allreg:
+                raise ValueError("a bulk species enabled in only some cell regions is not supported: "
+                                 f"regions {sorted(regions)} vs the grid's {sorted(allreg)}")
         if ispec not in self.species:
             self.species = sorted(self.species + [ispec])
             self._drop()
 
+    def enable_boundary_species(self, ispec, bregions):
+        """enable_boundary_species!(sys, ispec, bregions) (src/vfvm_system.jl:492-515)."""
+        if ispec in self.species:
+            raise ValueError(f"species {ispec} is already a bulk species")
+        self._bspecies[ispec] = sorted(set(int(r) for r in bregions))
+        self._drop()
+
     @property
     def num_species(self):
-        return max(self.species) if self.species else 0
+        return max(list(self.species) + list(self._bspecies)) if (self.species or self._bspecies) else 0
+
+    def _species_table(self):
+        ns = self.num_species
+        tab = (_capi.SpeciesT * ns)()
+        for i in range(1, ns + 1):
+            e = tab[i - 1]
+            if i in self._bspecies:
+                e.kind = _capi.SPECIES_BOUNDARY
+                for r in self._bspecies[i]:
+                    if not 1 <= r <= 32:
+                        raise ValueError("boundary region numbers must lie in 1..32")
+                    e.bregions |= 1 << (r - 1)
+            elif i in self.species:
+                e.kind = _capi.SPECIES_BULK
+            else:
+                e.kind = _capi.SPECIES_NONE
+        return tab
+
+    def _bphys_table(self):
+        ph = self.physics
+        if not ph.has_boundary_physics():
+            return None
+        ns, nb = self.num_species, self._nbreg()
+        tab = (_capi.BPhysT * nb)()
+
+        def entry(r):
+            if not 1 <= r <= nb:
+                raise ValueError(f"boundary region {r} outside 1..{nb}")
+            return tab[r - 1]
+
+        for b in ph.breaction:
+            if isinstance(b, BoundaryReaction):
+                e = entry(b.region)
+                M = np.asarray(b.matrix, dtype=float)
+                if M.shape != (ns, ns):
+                    raise ValueError(f"BoundaryReaction matrix must be {ns} x {ns}")
+                for i in range(ns):
+                    for j in range(ns):
+                        e.react[i][j] = M[i, j]
+            elif isinstance(b, BoundaryReactionField):
+                entry(b.region).fields = 1
+            else:
+                raise TypeError(f"not a boundary reaction of the registry: {b!r}")
+        for b in ph.bstorage:
+            entry(b.region).storage |= 1 << (b.species - 1)
+        for b in ph.bflux:
+            entry(b.region).bflux[b.species - 1] = float(b.D)
+        for b in ph.bsource:
+            entry(b.region).fields = 1
+        return tab
+
+    def _node_values(self, values):
+        n = self.grid.num_nodes
+        if callable(values):
+            return np.array([values(*self.grid.coord[:, k]) for k in range(n)], dtype=float)
+        return np.broadcast_to(np.asarray(values, dtype=float), (n,)).copy()
+
+    def _field_arrays(self):
+        """(rcoef, bsrc, bcoef), nspec x nnodes Fortran arrays or None.  Boundary fields are
+        set on the nodes of their region's boundary faces."""
+        ph = self.physics
+        ns, n = self.num_species, self.grid.num_nodes
+        g = self.grid
+
+        def region_nodes(r):
+            return np.unique(g.bfacenodes[:, g.bfaceregions == r])
+
+        rcoef = bsrc = bcoef = None
+        for f in ph.reaction_fields:
+            rcoef = np.zeros((ns, n), order="F") if rcoef is None else rcoef
+            rcoef[f.species - 1] += self._node_values(f.values)
+        for lst, kind in ((ph.bsource, BoundarySource), (ph.breaction, BoundaryReactionField)):
+            for f in lst:
+                if not isinstance(f, kind):
+                    continue
+                nodes = region_nodes(f.region)
+                vals = self._node_values(f.values)
+                if kind is BoundarySource:
+                    bsrc = np.zeros((ns, n), order="F") if bsrc is None else bsrc
+                    bsrc[f.species - 1, nodes] = vals[nodes]
+                else:
+                    bcoef = np.zeros((ns, n), order="F") if bcoef is None else bcoef
+                    bcoef[f.species - 1, nodes] = vals[nodes]
+        return rcoef, bsrc, bcoef
+
+    def node_species(self):
+        """node_dof activity: bit i - 1 of entry K is set if species i is active at node K."""
+        m = np.zeros(self.grid.num_nodes, np.uint8)
+        _capi.check(_capi.lib().vfvm_node_species(self.handle(), m.ctypes.data_as(C.POINTER(C.c_uint8))),
+                    self.ctx.handle)
+        return m
 
     def boundary_dirichlet(self, ispec, ibc, v):
         self.enable_species(ispec)
@@ -149,16 +267,26 @@ class System:
             bn = np.asfortranarray(g.bfacenodes.astype(np.int64) + 1)
             br = np.ascontiguousarray(g.bfaceregions.astype(np.int64))
             tab = self._bc_table()
+            spec = self._species_table()
+            bph = self._bphys_table()
             h = C.c_void_p()
-            rc = L.vfvm_system_create_cs(self.ctx.handle, g.dim, g.num_nodes, _capi.dptr(coord), g.num_cells,
+            rc = L.vfvm_system_create_ex(self.ctx.handle, g.dim, g.num_nodes, _capi.dptr(coord), g.num_cells,
                                          _capi.lptr(cn), _capi.lptr(cr), g.num_bfaces, _capi.lptr(bn),
                                          _capi.lptr(br), ns, C.byref(ph), self._nbreg(), tab, int(g.coordsys),
-                                         C.byref(h))
+                                         spec, bph, _capi.ASSEMBLY[self.assembly], C.byref(h))
             _capi.check(rc, self.ctx.handle)
             self._h = h
             src = self._source_array()
             if src is not None:
                 _capi.check(L.vfvm_system_set_source(h, _capi.dptr(src)), self.ctx.handle)
+            dp = self.physics.dparam_block(ns)
+            if np.any(dp != 0.0):
+                _capi.check(L.vfvm_system_set_param_dependence(h, _capi.dptr(np.ascontiguousarray(dp))),
+                            self.ctx.handle)
+            fields = self._field_arrays()
+            if any(f is not None for f in fields):
+                _capi.check(L.vfvm_system_set_fields(h, *[_capi.dptr(f) if f is not None else None for f in fields]),
+                            self.ctx.handle)
             self._bc_dirty = False
         elif self._bc_dirty:
             _capi.check(_capi.lib().vfvm_system_set_bc(self._h, self._bc_table()), self.ctx.handle)
@@ -303,6 +431,16 @@ def evaluate_residual_and_jacobian(sys: System, u, t=0.0, tstep=math.inf, embed=
     return _from_dofs(sys, F), A
 
 
+def initialize_dirichlet(sys: System, u, t=0.0, embed=0.0):
+    """_initialize_dirichlet!(U, system, data) (src/vfvm_system.jl:947-1003): a copy of u
+    with the Dirichlet values of time t and embedding parameter `embed` set on the
+    boundary nodes (the last boundary face visited sets a node on several regions)."""
+    U = _as_dofs(sys, u).copy()
+    _capi.check(_capi.lib().vfvm_initialize_dirichlet(sys.handle(), _capi.dptr(U), float(t), float(embed)),
+                sys.ctx.handle)
+    return _from_dofs(sys, U)
+
+
 def eval_rhs(sys: System, u, t=0.0):
     """eval_rhs!(du, u, state, t) (src/vfvm_diffeq_interface.jl:31-36): du = -F(u) with
     F assembled at U = Uold = u, tstep = Inf.  u and the result are dof vectors."""
@@ -364,8 +502,9 @@ class TransientSolution:
 
 
 def solve(sys: System, inival=0.0, control: Optional[SolverControl] = None, time=0.0, tstep=math.inf,
-          times=None, embed=None, store_all=True, **kwargs):
-    """solve(sys; inival, control, time, tstep, times, embed) (src/vfvm_solver.jl:551-668).
+          times=None, embed=None, store_all=True, params=None, **kwargs):
+    """solve(sys; inival, control, time, tstep, times, embed, params) (src/vfvm_solver.jl:551-668).
+    params: [lambda], the value of the system parameter for a single solve.
 
     Stationary and implicit-Euler-step solves run the device Newton loop and return
     the solution (nspec x nnodes).  `times` (implicit Euler) and `embed` (parameter
@@ -407,7 +546,8 @@ def solve(sys: System, inival=0.0, control: Optional[SolverControl] = None, time
         return TransientSolution(ts, us, sys.history)
     U = np.zeros_like(U0)
     hist = _capi.HistoryT()
-    rc = L.vfvm_newton(h, _capi.dptr(U), _capi.dptr(U0), time, tstep, 0.0, C.byref(cc), C.byref(hist))
+    lam = 0.0 if params is None else float(np.atleast_1d(params)[0])
+    rc = L.vfvm_newton(h, _capi.dptr(U), _capi.dptr(U0), time, tstep, lam, C.byref(cc), C.byref(hist))
     sys.history = NewtonSolverHistory(nlin=hist.nlin, niter=hist.niter, time=hist.time,
                                       updatenorm=tuple(hist.updatenorm[:min(hist.niter, 64)]),
                                       residual_norm=hist.residual_norm, fp64_fallbacks=hist.fp64_fallbacks)
@@ -420,6 +560,10 @@ def integrate(sys: System, U):
     (src/vfvm_postprocess.jl:18-97)."""
     vol = sys.nodevolumes()
     return (np.asarray(U) * vol[None, :]).sum(axis=1, keepdims=True)
+
+
+def enable_boundary_species(sys, ispec, bregions):
+    return sys.enable_boundary_species(ispec, bregions)
 
 
 def enable_species(sys, ispec, regions=None):
