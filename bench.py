@@ -545,7 +545,7 @@ def run_cfg4(args):
     # beside it, per block: the host-buffer path, a probe pass, and the batched solves from zero
     t_sur_host, t_zero, t_zero64, it_zero, sur_err = 0.0, 0.0, 0.0, [], []
     converged = []
-    probes = {_capi.KERNEL_SPMV32: [], _capi.KERNEL_ASSEMBLE: []}
+    probes = {_capi.KERNEL_SPMV32: [], _capi.KERNEL_HALFSTEP32: [], _capi.KERNEL_ASSEMBLE: []}
     for b in range(min(K, nb)):
         f = fams[b]
         infer(b)
@@ -557,7 +557,7 @@ def run_cfg4(args):
         Uh = f.batch.solve([x.reshape(1, -1) for x in gh], ctrl32)
         t_sur_host += time.perf_counter() - t1
         worst = max(worst, max(h.residual_norm for h in f.batch.histories))
-        with ctx.probe(_capi.KERNEL_SPMV32, _capi.KERNEL_ASSEMBLE) as pr:  # kernel launches timed one by one
+        with ctx.probe(_capi.KERNEL_SPMV32, _capi.KERNEL_HALFSTEP32, _capi.KERNEL_ASSEMBLE) as pr:  # timed one by one
             f.batch.solve_device(U_dev[b], G_dev[b], ctrl32)
         for k, v in pr.items():
             probes[k].append(v)
@@ -646,9 +646,17 @@ def run_cfg4(args):
                     avg_ms=sum(x["total_ms"] for x in lst) / max(sum(x["launches"] for x in lst), 1),
                     work=tuple(sum(x["work"][i] for x in lst) for i in range(3)))
     pspmv, pasm = merged(probes[_capi.KERNEL_SPMV32]), merged(probes[_capi.KERNEL_ASSEMBLE])
+    phalf = merged(probes[_capi.KERNEL_HALFSTEP32])
     rl_spmv = roofline_hbm(probe_bytes("spmv", pspmv, sv=4), pspmv, read_traffic("cfg4_bspmv32"),
                            "batched FP32 SpMV")
     rl_spmv["kernel"] = f"batched FP32 SpMV over the active members of {B} x 129^2 (k_bspmv)"
+    # the second operator application with the half step fused: r and v gathered per
+    # column instead of s, s and t written (2 x 4 bytes per row more than an SpMV)
+    rows_half = phalf["work"][0] / max(phalf["launches"], 1)
+    rl_half = roofline_hbm(probe_bytes("spmv", phalf, sv=4) + 2 * 4 * rows_half, phalf,
+                           read_traffic("cfg4_bspmv_s32"), "batched FP32 SpMV fused with the half step")
+    rl_half["kernel"] = (f"batched FP32 SpMV fused with the BiCGStab half step over the active members of "
+                         f"{B} x 129^2 (k_bspmv_s): the kernel with the most time in the timed batches")
     rl_asm = roofline_hbm(probe_bytes("assemble", pasm, transient=False), pasm, read_traffic("cfg4_assemble"),
                           "batched assembly")
     import ctypes as C
@@ -675,7 +683,7 @@ def run_cfg4(args):
                 max_abs_diff_vs_plain=diff, worst_residual=worst,
                 surrogate=dict(train_samples=int(meta["samples"]), epochs=int(meta["epochs"]),
                                final_train_mse=float(meta["final_loss"])),
-                roofline=rl_spmv, roofline_assemble=rl_asm,
+                roofline=rl_half, roofline_bspmv=rl_spmv, roofline_assemble=rl_asm,
                 roofline_mpnn_layer=dict(bound="hbm", achieved=round(layer_gbs, 1), peak=8000.0, unit="GB/s",
                                          frac=round(layer_gbs / 8000.0, 4), traffic=read_traffic("cfg4_mpnn_layer"),
                                          traffic_note="mean over the four layers of one inference; the last "

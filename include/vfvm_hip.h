@@ -316,6 +316,11 @@ int vfvm_eval_jacobian_device(vfvm_sys *sys, const double *du, double time, doub
  * currents of examples/Example161_BipolarDriftDiffusionCurrent.jl). */
 int vfvm_integrate_testfunction(vfvm_sys *sys, const double *U, const double *Uold, double tstep, const double *T,
                                 double *result);
+/* The same at the system parameter lambda (integrate(system, T, U; params), with the
+ * registry's one parameter standing for the params; the form above evaluates at 1, the
+ * reference's embedding parameter of integrate). */
+int vfvm_integrate_testfunction_at(vfvm_sys *sys, const double *U, const double *Uold, double tstep,
+                                   const double *T, double lambda, double *result);
 typedef struct vfvm_impedance vfvm_impedance;
 int vfvm_impedance_create(vfvm_sys *sys, const double *U0, int excited_spec, int excited_bc,
                           vfvm_impedance **out);
@@ -325,6 +330,10 @@ int vfvm_impedance_create_param(vfvm_sys *sys, const double *U0, double lambda0,
 int vfvm_impedance_destroy(vfvm_impedance *imp);
 int vfvm_measurement_derivatives(vfvm_sys *sys, const double *U0, const double *T, int spec,
                                  double *dstdy, double *dtran);
+/* measurement_derivative at the system parameter lambda (the form above: lambda = 0), for
+ * systems whose coefficients depend on lambda */
+int vfvm_measurement_derivatives_at(vfvm_sys *sys, const double *U0, const double *T, int spec, double lambda,
+                                    double *dstdy, double *dtran);
 int vfvm_impedance_sweep(vfvm_impedance *imp, int64_t nomega, const double *omega, const double *dstdy,
                          const double *dtran, double reltol, int32_t maxiters, double *UZ, double *Z,
                          int32_t *iters);
@@ -450,12 +459,17 @@ int vfvm_set_compact_graph(vfvm_sys *sys, int enable);
 #define VFVM_KERNEL_RESIDUAL 1
 #define VFVM_KERNEL_SPMV64 2
 #define VFVM_KERNEL_SPMV32 3
+/* probe classes only: the batched second operator application fused with the BiCGStab
+ * half step (s = r - alpha v gathered per column, t = Ahat s), FP64 and FP32 */
+#define VFVM_KERNEL_HALFSTEP64 4
+#define VFVM_KERNEL_HALFSTEP32 5
 int vfvm_time_kernel(vfvm_sys *sys, int kernel, int reps, double *avg_ms);
 
 /* Benchmark helper (no counterpart in the reference): live timing of kernel classes
  * inside real calls on the context.  After vfvm_probe_arm(kernel), every launch of that
  * class (VFVM_KERNEL_ASSEMBLE: Newton-step Jacobian assembly; _SPMV64 / _SPMV32: Krylov
- * operator applications), single-system or batched, is bracketed by HIP events on the
+ * operator applications; _HALFSTEP64 / _HALFSTEP32: the batched operator applications
+ * fused with the half step), single-system or batched, is bracketed by HIP events on the
  * launching stream, up to max_launches launches.  Several classes may be armed at once.
  * vfvm_probe_read returns one class's summed duration, the number of timed launches and
  * (work, 3 entries, nullable) the work those launches did, then disarms that class:

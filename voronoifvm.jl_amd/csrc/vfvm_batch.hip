@@ -190,12 +190,17 @@ __global__ void __launch_bounds__(BB) k_bspmv_s(Seg S, const int32_t *__restrict
                                                 const int32_t *__restrict__ sptr, const CI *__restrict__ col,
                                                 const VA *__restrict__ val, const V *__restrict__ r,
                                                 const V *__restrict__ v, V *__restrict__ s, V *__restrict__ t,
-                                                KState *st) {
+                                                KState *st, unsigned long long *__restrict__ mwork,
+                                                const int64_t *__restrict__ msize) {
   const int inst = S.ch_inst[blockIdx.x];
   const KState *sti = st + inst;
   if (sti->done) return;
   const double alpha = sti->alpha;
   const int64_t r0 = S.ch_beg[blockIdx.x], r1 = S.ch_end[blockIdx.x];
+  if (mwork && threadIdx.x == 0 && blockIdx.x == S.ch_first[inst]) {  // probe counters, as k_bspmv
+    mwork[2 * inst] += (unsigned long long)msize[2 * inst];
+    mwork[2 * inst + 1] += (unsigned long long)msize[2 * inst + 1];
+  }
   const int32_t sl0 = ch_slice[blockIdx.x];
   double d0 = 0.0, d1 = 0.0, d2 = 0.0;
   for (int64_t l = threadIdx.x; l < r1 - r0; l += BB) {
@@ -434,7 +439,7 @@ struct vfvm_batch {
   int64_t sell_nnz = 0;
   // probe work of the batched SpMV: per member (rows, nonzeros) and per-member counters
   DevBuf<int64_t> msize;
-  DevBuf<unsigned long long> mwork;
+  DevBuf<unsigned long long> mwork, mwork_half;  // k_bspmv's and k_bspmv_s's counters
   bool sell_nt[2] = {false, false};  // non-temporal matrix loads in the SpMV (FP32, FP64)
   // captured Krylov iteration blocks: (value bytes, solution vector, iterations) -> graph
   bool use_graphs = true;
@@ -562,17 +567,20 @@ void bop(vfvm_batch &b, const V *x, V *y, const V *w0) {
 template <class V>
 void bop_s(vfvm_batch &b, const V *r, const V *v, V *s, V *t) {
   const Seg S = b.seg();
+  const int kid = sizeof(V) == 4 ? VFVM_KERNEL_HALFSTEP32 : VFVM_KERNEL_HALFSTEP64;
+  unsigned long long *mwork = b.ctx->probe.counters(kid) ? b.mwork_half.p : nullptr;
+  b.ctx->probe.before(kid, b.ctx->stream);
   auto go = [&](auto ntc, const auto *col) {
     constexpr bool NT = decltype(ntc)::value;
     using CI = std::remove_const_t<std::remove_pointer_t<decltype(col)>>;
     if constexpr (sizeof(V) == 4)
-      hipLaunchKernelGGL((k_bspmv_s<float, V, NT, CI>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                         (const int32_t *)b.ch_slice.p, (const int32_t *)b.sell_ptr.p, col, (const float *)b.A32.p, r,
-                         v, s, t, b.st.p);
+      VFVM_LAUNCH((k_bspmv_s<float, V, NT, CI>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                  (const int32_t *)b.ch_slice.p, (const int32_t *)b.sell_ptr.p, col, (const float *)b.A32.p, r, v, s,
+                  t, b.st.p, mwork, (const int64_t *)b.msize.p);
     else
-      hipLaunchKernelGGL((k_bspmv_s<double, V, NT, CI>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
-                         (const int32_t *)b.ch_slice.p, (const int32_t *)b.sell_ptr.p, col, (const double *)b.A64s.p,
-                         r, v, s, t, b.st.p);
+      VFVM_LAUNCH((k_bspmv_s<double, V, NT, CI>), dim3((unsigned)b.nch), dim3(BB), 0, b.ctx->stream, S,
+                  (const int32_t *)b.ch_slice.p, (const int32_t *)b.sell_ptr.p, col, (const double *)b.A64s.p, r, v,
+                  s, t, b.st.p, mwork, (const int64_t *)b.msize.p);
   };
   const bool nt = b.sell_nt[sizeof(V) == 4 ? 0 : 1];
   if (b.sell_off16.p && b.sell16) {
@@ -586,6 +594,7 @@ void bop_s(vfvm_batch &b, const V *r, const V *v, V *s, V *t) {
     else
       go(std::false_type(), (const int32_t *)b.sell_col.p);
   }
+  b.ctx->probe.after(kid, b.ctx->stream);
   hipLaunchKernelGGL((k_bfinalize<3, STAGE_SOMEGA, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, b.ctx->stream, S,
                      b.st.p, (const int32_t *)nullptr, (double *)nullptr);
 }
@@ -725,6 +734,9 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
     const int kid = sizeof(V) == 4 ? VFVM_KERNEL_SPMV32 : VFVM_KERNEL_SPMV64;
     if (unsigned long long *dw = b.ctx->probe.counters(kid))
       hipLaunchKernelGGL(k_collect_work, dim3(1), dim3(BB), 0, st, b.nsys, b.mwork.p, dw);
+    const int kh = sizeof(V) == 4 ? VFVM_KERNEL_HALFSTEP32 : VFVM_KERNEL_HALFSTEP64;
+    if (unsigned long long *dw = b.ctx->probe.counters(kh))
+      hipLaunchKernelGGL(k_collect_work, dim3(1), dim3(BB), 0, st, b.nsys, b.mwork_half.p, dw);
   }
   int rc = 0;
   for (size_t m = 0; m < h.size(); m++) {
@@ -969,6 +981,7 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
       b->msize.upload(ms, st);
       std::vector<unsigned long long> z((size_t)2 * nsys, 0ull);
       b->mwork.upload(z, st);
+      b->mwork_half.upload(z, st);
     }
     // SELL-64 operator values; padding entries stay 0
     b->A32.alloc((size_t)b->sell_nnz);

@@ -488,6 +488,11 @@ int vfvm_impedance_destroy(vfvm_impedance *z) {
 
 int vfvm_measurement_derivatives(vfvm_sys *s, const double *U0, const double *T, int spec, double *dstdy,
                                  double *dtran) {
+  return vfvm_measurement_derivatives_at(s, U0, T, spec, 0.0, dstdy, dtran);
+}
+
+int vfvm_measurement_derivatives_at(vfvm_sys *s, const double *U0, const double *T, int spec, double lambda,
+                                    double *dstdy, double *dtran) {
   if (!s || !U0 || !T || !dstdy || !dtran) return VFVM_ERR_ARG;
   std::vector<vfvm_bc_t> saved;
   try {
@@ -502,7 +507,7 @@ int vfvm_measurement_derivatives(vfvm_sys *s, const double *U0, const double *T,
     upload_inst(*s);
     s->d_U.upload(U0, (size_t)s->ndof, st);
     launch_fill_i32(1, s->d_flags.p, 0, st);
-    AsmArgs a = asm_args(*s, s->d_U.p, s->d_U.p, s->d_F.p, s->d_J.p, INFINITY, 0.0);
+    AsmArgs a = asm_args(*s, s->d_U.p, s->d_U.p, s->d_F.p, s->d_J.p, INFINITY, lambda);
     run_assembly(*s, a, s->asm_group, st);
     s->bc = saved;
     saved.clear();
@@ -548,15 +553,20 @@ int vfvm_measurement_derivatives(vfvm_sys *s, const double *U0, const double *T,
 
 int vfvm_integrate_testfunction(vfvm_sys *s, const double *U, const double *Uold, double tstep, const double *T,
                                 double *result) {
+  // integrate(system, T, U[, Uold, tstep]) evaluates at time 0 with embedding parameter 1
+  // (src/vfvm_testfunctions.jl:232, Node(system, 0.0, 1.0))
+  return vfvm_integrate_testfunction_at(s, U, Uold, tstep, T, 1.0, result);
+}
+
+int vfvm_integrate_testfunction_at(vfvm_sys *s, const double *U, const double *Uold, double tstep, const double *T,
+                                   double lambda, double *result) {
   if (!s || !U || !T || !result || !(tstep > 0.0)) return VFVM_ERR_ARG;
   try {
     VFVM_HIP_CHECK(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
     s->d_U.upload(U, (size_t)s->ndof, st);
     s->d_Uold.upload(Uold ? Uold : U, (size_t)s->ndof, st);
-    // integrate(system, T, U[, Uold, tstep]) evaluates at time 0 with embedding
-    // parameter 1 (src/vfvm_testfunctions.jl:232, Node(system, 0.0, 1.0))
-    residual_without_bc(*s, s->d_U.p, s->d_Uold.p, tstep, 1.0, s->d_tmp.p);
+    residual_without_bc(*s, s->d_U.p, s->d_Uold.p, tstep, lambda, s->d_tmp.p);
     DevBuf<double> dT, dr;
     dT.upload(T, (size_t)s->nnodes, st);
     dr.alloc((size_t)s->nspec);

@@ -106,6 +106,8 @@ SIGNATURES = {
     "vfvm_parameter_derivative": (C.c_int, [_vp, _dp, C.c_double, C.POINTER(ControlT), _dp]),
     "vfvm_measurement_derivatives": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, _dp]),
     "vfvm_integrate_testfunction": (C.c_int, [_vp, _dp, _dp, C.c_double, _dp, _dp]),
+    "vfvm_integrate_testfunction_at": (C.c_int, [_vp, _dp, _dp, C.c_double, _dp, C.c_double, _dp]),
+    "vfvm_measurement_derivatives_at": (C.c_int, [_vp, _dp, _dp, C.c_int, C.c_double, _dp, _dp]),
     "vfvm_impedance_sweep": (C.c_int, [_vp, C.c_int64, _dp, _vp, _vp, C.c_double, C.c_int32, _vp, _vp,
                                        C.POINTER(C.c_int32)]),
     "vfvm_pattern": (C.c_int, [_vp, _lp, _lp]),
@@ -252,7 +254,7 @@ class Context:
         return _Probe(self, kernels, max_launches)
 
 
-KERNEL_ASSEMBLE, KERNEL_RESIDUAL, KERNEL_SPMV64, KERNEL_SPMV32 = 0, 1, 2, 3
+KERNEL_ASSEMBLE, KERNEL_RESIDUAL, KERNEL_SPMV64, KERNEL_SPMV32, KERNEL_HALFSTEP64, KERNEL_HALFSTEP32 = range(6)
 
 
 class _Probe:

@@ -28,31 +28,34 @@ def testfunction(sys: System, bc0, bc1, reltol_linear=1e-12):
     return np.ascontiguousarray(dofs(T), dtype=np.float64)
 
 
-def integrate_testfunction(sys: System, T, U, Uold=None, tstep=math.inf):
+def integrate_testfunction(sys: System, T, U, Uold=None, tstep=math.inf, params=None):
     """integrate(system, T, U) for a stationary solution (src/vfvm_testfunctions.jl:110-130),
     or integrate(system, T, U, Uold, tstep) for an implicit-Euler step (lines 194-210): per
     species I_flux + I_react - I_src (+ (I_stor - I_oldstor) / tstep) weighted by the test
-    function T; with T from testfunction(sys, bc0, bc1) the flux through the bc1 boundary."""
+    function T; with T from testfunction(sys, bc0, bc1) the flux through the bc1 boundary.
+    params: [lambda], the system parameter (default: the reference's embedding value 1)."""
     u = _dofvec(U)
     uo = None if Uold is None else _dofvec(Uold)
     T = np.ascontiguousarray(T, dtype=np.float64)
     out = np.zeros(sys.num_species)
-    _capi.check(_capi.lib().vfvm_integrate_testfunction(sys.handle(), _capi.dptr(u),
-                                                        None if uo is None else _capi.dptr(uo), float(tstep),
-                                                        _capi.dptr(T), _capi.dptr(out)), sys.ctx.handle)
+    lam = 1.0 if params is None else float(np.atleast_1d(params)[0])
+    _capi.check(_capi.lib().vfvm_integrate_testfunction_at(sys.handle(), _capi.dptr(u),
+                                                           None if uo is None else _capi.dptr(uo), float(tstep),
+                                                           _capi.dptr(T), lam, _capi.dptr(out)), sys.ctx.handle)
     return out
 
 
-def measurement_derivatives(sys: System, U0, T, spec):
+def measurement_derivatives(sys: System, U0, T, spec, params=None):
     """measurement_derivative (src/vfvm_impedance.jl:235-252) of
-    U -> -integrate_stdy(sys, T, U)[spec] and U -> -integrate_tran(sys, T, U)[spec] at U0:
-    two dof vectors (dmeas_stdy, dmeas_tran)."""
+    U -> -integrate_stdy(sys, T, U)[spec] and U -> -integrate_tran(sys, T, U)[spec] at U0
+    (and the system parameter params[0], default 0): two dof vectors (dmeas_stdy, dmeas_tran)."""
     ndof, _, _ = sys.sizes()
     u = np.ascontiguousarray(dofs(U0) if np.ndim(U0) == 2 else np.asarray(U0, dtype=np.float64).reshape(-1))
     T = np.ascontiguousarray(T, dtype=np.float64)
     ds, dt = np.zeros(ndof), np.zeros(ndof)
-    _capi.check(_capi.lib().vfvm_measurement_derivatives(sys.handle(), _capi.dptr(u), _capi.dptr(T), int(spec),
-                                                         _capi.dptr(ds), _capi.dptr(dt)), sys.ctx.handle)
+    lam = 0.0 if params is None else float(np.atleast_1d(params)[0])
+    _capi.check(_capi.lib().vfvm_measurement_derivatives_at(sys.handle(), _capi.dptr(u), _capi.dptr(T), int(spec),
+                                                            lam, _capi.dptr(ds), _capi.dptr(dt)), sys.ctx.handle)
     return ds, dt
 
 
