@@ -91,6 +91,10 @@ typedef struct {
 #define VFVM_PRECON_JACOBI 0       /* point Jacobi (row scaling) */
 #define VFVM_PRECON_BLOCKJACOBI 1  /* nspec x nspec node blocks */
 #define VFVM_PRECON_ILU0 2         /* ILU(0), multicolour level order */
+/* smoothed aggregation AMG, one V(1,1) cycle per application (scalar systems):
+ * KrylovJL_BICGSTAB(precs = SmoothedAggregationPreconBuilder()),
+ * examples/DevEx003_Solvers.jl:149-156 */
+#define VFVM_PRECON_AMG 3
 #define VFVM_PREC_FP64 0
 #define VFVM_PREC_FP32_REFINE 1    /* FP32 Krylov + preconditioner inside FP64 iterative refinement */
 
@@ -369,6 +373,23 @@ int vfvm_parameter_derivative(vfvm_sys *sys, const double *U, double lambda, con
  * vfvm_newton): _solve_linear!, src/vfvm_linsolve.jl:6-61. */
 int vfvm_linsolve(vfvm_sys *sys, const double *b, double *x, const vfvm_control_t *control,
                   vfvm_linstats_t *stats);
+
+/* ---- smoothed aggregation AMG (VFVM_PRECON_AMG) ----
+ * The hierarchy is AlgebraicMultigrid.jl's MultiLevel (levels with A, P, R = P^T), the
+ * preconditioner of examples/DevEx003_Solvers.jl:149-156; the reference reads none of it
+ * back, these calls let tests check it.
+ * Standard aggregation of a CSR pattern (0-based, diagonal ignored), host only: agg[i] in
+ * [0, *naggregates). */
+int vfvm_amg_aggregate(int64_t n, const int32_t *rowptr, const int32_t *colind, int32_t *agg,
+                       int64_t *naggregates);
+/* Levels of the system's current hierarchy (built by a solve or vfvm_linsolve with
+ * VFVM_PRECON_AMG) and the rows of each (rows: up to 16 entries, may be NULL). */
+int vfvm_amg_levels(vfvm_sys *sys, int *nlevels, int64_t *rows);
+/* Level l: sizes[0..3] = rows, nonzeros of A, coarse rows (0 on the coarsest level),
+ * nonzeros of P.  A (0-based CSR; level 0: diag(rowscale) J) and P (rows x coarse rows),
+ * the aggregates and omega = (4/3) / rho; any output pointer may be NULL. */
+int vfvm_amg_level(vfvm_sys *sys, int level, int64_t *sizes, int32_t *arp, int32_t *aci, double *aval,
+                   int32_t *prp, int32_t *pci, double *pval, int32_t *agg, double *omega);
 
 /* Damped Newton: solve_step!, src/vfvm_solver.jl:13-222.  On entry U is ignored and Uold is
  * the initial value / old time step; on return U holds the solution.  Assembly, linear

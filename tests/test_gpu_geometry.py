@@ -192,3 +192,11 @@ def test_update_grid_from_device_memory():
     U = np.random.default_rng(12).uniform(-1, 1, (1, g0.num_nodes))
     assert _same(_asm(a, U), _asm(b, U))
     assert np.array_equal(a.nodevolumes(), b.nodevolumes())
+    # the system's grid holds the new coordinates: a rebuild of the device system keeps them
+    assert np.array_equal(b.grid.coord, g1.coord)
+    b._drop()
+    b.set_compact_graph(False)
+    assert _same(_asm(a, U), _asm(b, U))
+    # arrays of another size or context are refused before any device work
+    with pytest.raises(ValueError):
+        b.update_grid_device(vfvm.DeviceArray(b.ctx, d.n - 1))

@@ -2,6 +2,8 @@
 and the step-control fields it hands to the library."""
 import sys
 
+import pytest
+
 import vfvm
 
 
@@ -64,3 +66,22 @@ def test_benchmark_gpu_legs_import_no_test_code():
                     names = [node.module or ""]
                 for n in names:
                     assert n.split(".")[0] not in ("cases", "oracle", "mpnn_oracle"), (fn.name, n)
+
+
+def test_update_grid_device_refuses_wrong_arrays():
+    """update_grid_device checks the array's size and context before any library call"""
+    import vfvm
+    from vfvm.grids import linspace, simplexgrid
+    g = simplexgrid(linspace(0, 1, 4), linspace(0, 1, 3))
+    ctx = object()
+    s = vfvm.System(g, vfvm.Physics(flux=vfvm.LinearFlux(1.0)), species=[1], context=ctx)
+
+    class Coords:
+        def __init__(self, n, c):
+            self.n, self.ctx, self.ptr = n, c, 1
+
+    with pytest.raises(ValueError):
+        s.update_grid_device(Coords(2 * g.num_nodes - 1, ctx))
+    with pytest.raises(ValueError):
+        s.update_grid_device(Coords(2 * g.num_nodes, object()))
+    assert s._h is None and s.grid is g

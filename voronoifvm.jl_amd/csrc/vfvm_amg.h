@@ -1,0 +1,184 @@
+// Smoothed-aggregation algebraic multigrid preconditioner for scalar systems, one
+// member or a block-diagonal batch of members.
+//
+// The reference's solver tests precondition BiCGStab with smoothed aggregation AMG
+// (KrylovJL_BICGSTAB(precs = SmoothedAggregationPreconBuilder()),
+// examples/DevEx003_Solvers.jl:149-156; the builder comes from AlgebraicMultigrid.jl,
+// not part of the reference's sources).  Its published algorithm (Vanek, Mandel and
+// Brezina 1996, with AlgebraicMultigrid.jl's defaults) is restated here:
+//   * strength: every stored off-diagonal entry (symmetric strength, theta = 0), so the
+//     aggregates depend on the pattern only;
+//   * standard aggregation in three passes (seed full neighbourhoods, attach the rest
+//     to a neighbouring seed aggregate, group what is left);
+//   * tentative prolongator T from the constant candidate, columns normalised per
+//     aggregate (the candidate of the next level is the aggregate norm);
+//   * smoothed prolongator P = (I - omega D^-1 A) T with omega = (4/3) / rho(D^-1 A),
+//     rho bounded by the largest row sum of |D^-1 A| (Gershgorin);
+//   * Galerkin coarse operator P^T A P, until at most 64 rows remain; the coarsest level
+//     is inverted exactly (Gauss-Jordan with partial pivoting, up to 88 rows) or, when
+//     coarsening stalls above that, relaxed by damped Jacobi.
+// One V(1,1) cycle with damped Jacobi smoothing (weight omega, the GPU's parallel
+// stand-in for AlgebraicMultigrid's Gauss-Seidel) is one preconditioner application.
+//
+// Layout for the GPU: the symbolic hierarchy is built on the host once per distinct
+// member pattern and concatenated over the members, level by level (member m's rows at
+// level l are contiguous, in member order), so one launch per level and kernel serves
+// every member.  The numeric setup runs on the device in FP64 from the row-scaled
+// Jacobian: P row by row, P^T A P coarse row by coarse row (no atomics), the coarsest
+// inverses one workgroup per member.  The apply runs in the Krylov precision on SELL-64
+// copies of A, P and P^T; level 0 uses the owner's own SELL-64 operator.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "vfvm_common.h"
+
+namespace vfvm {
+
+struct KState;
+
+// a member's scalar pattern: host CSR, member-local 0-based columns, sorted rows, the
+// diagonal present in every row
+struct AmgMember {
+  int64_t n = 0;
+  const int32_t *rowptr = nullptr, *colind = nullptr;
+};
+
+// The owner's level-0 operator Ahat in SELL-64 order: entry j of row r at rbase[r] + 64 j,
+// row lengths from the CSR rowptr; columns absolute (col32) or 16-bit offsets from the row
+// (col16, used when not null).
+template <class V>
+struct AmgOp {
+  const int32_t *rowptr = nullptr, *rbase = nullptr;
+  const int32_t *col32 = nullptr;
+  const int16_t *col16 = nullptr;
+  const V *val = nullptr;
+};
+
+// Host symbolic hierarchy of one member pattern (member-local indices).
+struct AmgPattern {
+  struct Level {
+    int64_t n = 0;
+    std::vector<int32_t> arp, aci, adp;  // A: CSR and diagonal positions
+    // coarsening levels only
+    int64_t nc = 0;
+    std::vector<int32_t> agg;            // aggregate of each row
+    std::vector<double> tval;            // T's one entry per row
+    std::vector<int32_t> prp, pci;       // P pattern
+    std::vector<int32_t> a2p;            // per A entry (i, k): position of agg[k] in P row i
+    std::vector<int32_t> tpos;           // per row: position of agg[i] in P row i
+    std::vector<int32_t> rrp, rci, r2p;  // P^T: coarse row -> (fine row, P position)
+  };
+  std::vector<Level> lv;  // lv.back() is the coarsest level
+  void build(int64_t n, const int32_t *rowptr, const int32_t *colind);
+};
+
+// Standard aggregation on the pattern graph without its diagonal; returns the number of
+// aggregates and agg[i] in [0, that number).
+int64_t amg_aggregate(int64_t n, const int32_t *rowptr, const int32_t *colind, std::vector<int32_t> &agg);
+
+class Amg {
+ public:
+  static constexpr int MAX_LEVELS = 12;
+  static constexpr int64_t MAX_COARSE = 64;  // coarsening stops at or below this many rows
+  static constexpr int64_t DENSE_MAX = 88;   // coarsest levels up to this size are inverted
+  static constexpr int COARSE_SWEEPS = 4;    // damped Jacobi sweeps on a larger coarsest level
+  enum { MODE_COARSEN = 0, MODE_DENSE = 1, MODE_JACOBI = 2, MODE_ABSENT = -1 };
+
+  // Symbolic setup.  members: the owner's members in order (their level-0 rows are the
+  // owner's rows, concatenated); row_inst0: the owner's row -> member map on the device
+  // (nullptr: one member).  Members with equal patterns share the host work.
+  void build(const std::vector<AmgMember> &members, const int32_t *row_inst0, hipStream_t s);
+  bool built() const { return nlev > 0; }
+  // Numeric setup from the owner's FP64 Jacobian J (global CSR rowptr / colind, diagonal
+  // positions dofdiag) and row scaling: the hierarchy of Ahat = diag(rowscale) J.
+  void setup(const int32_t *rowptr, const int32_t *colind, const int32_t *dofdiag, const double *J,
+             const double *rowscale, hipStream_t s);
+  bool ready() const { return numeric_ready; }
+  // z = M^-1 y, one V(1,1) cycle on every member whose Krylov state is running (st may be
+  // null: all members; skip_half: members whose BiCGStab half step converged are skipped)
+  template <class V>
+  void apply(const AmgOp<V> &A0, const V *y, V *z, const KState *st, int skip_half, hipStream_t s);
+  // allocates apply's work vectors of precision V (before a stream capture of apply)
+  template <class V>
+  void reserve() {
+    for (auto &L : lev) {
+      V *b, *x, *r;
+      work<V>(L, b, x, r);
+    }
+  }
+
+  int levels() const { return nlev; }
+  int64_t level_rows(int l) const { return lev[l].n; }
+  // host copies of member 0's level data: A (l >= 1; FP64 CSR), P (coarsening levels),
+  // omega; false if l is out of range
+  bool export_level(int l, hipStream_t s, std::vector<int32_t> &arp, std::vector<int32_t> &aci,
+                    std::vector<double> &aval, std::vector<int32_t> &prp, std::vector<int32_t> &pci,
+                    std::vector<double> &pval, std::vector<int32_t> &agg, double &omega) const;
+
+ private:
+  struct Lev {
+    int64_t n = 0, nc = 0, nnzA = 0, nnzP = 0, nnzR = 0;
+    bool any_coarsen = false, any_dense = false, any_jacobi = false;
+    std::vector<int64_t> roff;  // per member: first row at this level (-1 absent)
+    std::vector<int32_t> hmode;
+    DevBuf<int32_t> inst, mode;
+    const int32_t *inst_p = nullptr;
+    DevBuf<double> w;
+    // A at levels >= 1 (level 0: the owner's CSR and SELL)
+    DevBuf<int32_t> arp, aci, adp, a_rbase, a_scol;
+    DevBuf<double> aval, a64, dinv64;
+    DevBuf<float> a32, dinv32;
+    // P and P^T of the coarsening members
+    DevBuf<int32_t> prp, pci, a2p, tpos, p_rbase, p_scol;
+    DevBuf<double> tval, pval, p64;
+    DevBuf<float> p32;
+    DevBuf<int32_t> rrp, rci, r2p, r_rbase, r_scol;
+    DevBuf<double> r64;
+    DevBuf<float> r32;
+    // coarsening members: (member, first row, end row) for the omega reduction
+    int64_t ncm = 0;
+    DevBuf<int32_t> cm;
+    DevBuf<int64_t> cb, ce;
+    // exactly inverted members: member, first row, size, inverse offset
+    int64_t ndm = 0, dmax = 0;
+    DevBuf<int32_t> dmem, dsz;
+    DevBuf<int64_t> drow, doff;
+    DevBuf<double> dinvA64;
+    DevBuf<float> dinvA32;
+    // work vectors of the apply (b: right-hand side, x: iterate, r: residual / solution)
+    DevBuf<double> b64, x64, rv64;
+    DevBuf<float> b32, x32, rv32;
+  };
+  template <class V>
+  void work(Lev &L, V *&b, V *&x, V *&r) {
+    if constexpr (sizeof(V) == 4) {
+      L.b32.alloc((size_t)L.n);
+      L.x32.alloc((size_t)L.n);
+      L.rv32.alloc((size_t)L.n);
+      b = L.b32.p;
+      x = L.x32.p;
+      r = L.rv32.p;
+    } else {
+      L.b64.alloc((size_t)L.n);
+      L.x64.alloc((size_t)L.n);
+      L.rv64.alloc((size_t)L.n);
+      b = L.b64.p;
+      x = L.x64.p;
+      r = L.rv64.p;
+    }
+  }
+
+  int64_t nsys = 0;
+  int nlev = 0;
+  bool numeric_ready = false;
+  std::vector<AmgPattern> patterns;
+  std::vector<int32_t> pid;  // member -> pattern
+  std::vector<Lev> lev;
+  DevBuf<int32_t> inst0_own;
+  int64_t nnz0 = 0;
+};
+
+}  // namespace vfvm

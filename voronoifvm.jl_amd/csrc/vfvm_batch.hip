@@ -430,6 +430,11 @@ struct vfvm_batch {
   // Krylov work
   DevBuf<double> k64[8];
   DevBuf<float> k32[8];
+  // smoothed-aggregation AMG over all members (scalar batches) and the preconditioned
+  // BiCGStab's two extra vectors
+  Amg amg;
+  DevBuf<double> kp64[2];
+  DevBuf<float> kp32[2];
   DevBuf<float> A32;
   // SELL-64 operator: slices start at every chunk start; ch_slice = first slice of a chunk
   DevBuf<int32_t> sell_ptr, sell_col, sell_rbase, ch_slice;
@@ -648,18 +653,44 @@ void reset_states(vfvm_batch &b, double rtol, double atol, int maxit) {
   stage_upload(b.st, b.hreset, b.nsys, b.ctx->stream);
 }
 
-// per-member BiCGStab (point Jacobi = identity on the row-scaled operator) on bhat -> x
+// the batch's operator Ahat as the AMG's level-0 operator
+template <class V>
+AmgOp<V> batch_op(vfvm_batch &b) {
+  AmgOp<V> op;
+  op.rowptr = b.rowptr.p;
+  op.rbase = b.sell_rbase.p;
+  op.col32 = b.sell_col.p;
+  op.col16 = (b.sell_off16.p && b.sell16) ? b.sell_off16.p : nullptr;
+  if constexpr (sizeof(V) == 4)
+    op.val = b.A32.p;
+  else
+    op.val = b.A64s.p;
+  return op;
+}
+
+// per-member BiCGStab on bhat -> x: point Jacobi (the identity on the row-scaled
+// operator) or, with amg, right-preconditioned by one AMG V-cycle per application
 template <class V, class VBin>
 int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double rtol, double atol, int maxit,
-              int check_every, std::vector<int64_t> &nlin) {
+              int check_every, std::vector<int64_t> &nlin, bool amg = false) {
   hipStream_t st = b.ctx->stream;
   const Seg S = b.seg();
-  V *r, *r0, *p, *v, *s, *t;
+  V *r, *r0, *p, *v, *s, *t, *ph = nullptr, *sh = nullptr;
   if constexpr (sizeof(V) == 4) {
     r = b.k32[0].p; r0 = b.k32[1].p; p = b.k32[2].p; v = b.k32[3].p; s = b.k32[4].p; t = b.k32[5].p;
+    if (amg) {
+      for (auto &w : b.kp32) w.alloc((size_t)b.ndof);
+      ph = b.kp32[0].p; sh = b.kp32[1].p;
+    }
   } else {
     r = b.k64[0].p; r0 = b.k64[1].p; p = b.k64[2].p; v = b.k64[3].p; s = b.k64[4].p; t = b.k64[5].p;
+    if (amg) {
+      for (auto &w : b.kp64) w.alloc((size_t)b.ndof);
+      ph = b.kp64[0].p; sh = b.kp64[1].p;
+    }
   }
+  const AmgOp<V> op = batch_op<V>(b);
+  if (amg) b.amg.reserve<V>();
   reset_states(b, rtol, atol, maxit);
   hipLaunchKernelGGL((k_binit<V, VBin>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, bh, r, r0, p, v, x, b.st.p, active);
   hipLaunchKernelGGL((k_bfinalize<1, STAGE_INIT, BF_INIT>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S, b.st.p, active,
@@ -670,6 +701,20 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
   // `every` iterations between status reads; kernels of finished members return at entry
   auto body = [&]() {
     for (int q = 0; q < every; q++) {
+      if (amg) {
+        hipLaunchKernelGGL((k_bp<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, r, p, v, b.st.p);
+        b.amg.apply<V>(op, p, ph, b.st.p, 0, st);
+        bop<V, 1, STAGE_ALPHA>(b, ph, v, r0);
+        hipLaunchKernelGGL((k_bs<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, r, v, s, b.st.p);
+        hipLaunchKernelGGL((k_bfinalize<1, STAGE_SNORM, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S,
+                           b.st.p, (const int32_t *)nullptr, (double *)nullptr);
+        b.amg.apply<V>(op, s, sh, b.st.p, 1, st);
+        bop<V, 2, STAGE_OMEGA>(b, sh, t, s);
+        hipLaunchKernelGGL((k_bxr<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, x, ph, sh, s, t, r, r0, b.st.p);
+        hipLaunchKernelGGL((k_bfinalize<2, STAGE_XR, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S,
+                           b.st.p, (const int32_t *)nullptr, (double *)nullptr);
+        continue;
+      }
       hipLaunchKernelGGL((k_bp<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, r, p, v, b.st.p);
       bop<V, 1, STAGE_ALPHA>(b, p, v, r0);
       bop_s<V>(b, r, v, s, t);  // s = r - alpha v, t = Ahat s, half-step test and omega
@@ -683,7 +728,7 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
   // nodes only.  Not while the kernel probe is armed: that brackets single launches.
   hipGraphExec_t ge = nullptr;
   if (b.use_graphs && !b.ctx->probe.any_armed()) {
-    const auto key = std::make_tuple((int)sizeof(V), (const void *)x, every);
+    const auto key = std::make_tuple((int)sizeof(V) + (amg ? 16 : 0), (const void *)x, every);
     auto it = b.graphs.find(key);
     if (it != b.graphs.end()) {
       ge = it->second;
@@ -747,17 +792,32 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
 }
 
 // Solve J dU = F per active member (FP64 BiCGStab, or FP32 BiCGStab inside FP64 refinement)
-void bsolve(vfvm_batch &b, const vfvm_control_t &c, std::vector<int64_t> &nlin) {
+// refactor: rebuild the AMG hierarchy from the current Jacobian (otherwise the one of an
+// earlier Newton step is reused, as an ILU(0) factor would be)
+void bsolve(vfvm_batch &b, const vfvm_control_t &c, std::vector<int64_t> &nlin, bool refactor) {
   hipStream_t st = b.ctx->stream;
   const Seg S = b.seg();
   const bool fp32 = c.precision == VFVM_PREC_FP32_REFINE;
+  const bool amg = c.precon == VFVM_PRECON_AMG;
   launch_scale_rows(b.nspec, b.nnodes, b.rowptr.p, b.colind.p, b.J.p, b.dofdiag.p, b.sell_rbase.p, b.rowscale.p,
                     fp32 ? b.A32.p : nullptr, fp32 ? nullptr : b.A64s.p, nullptr, st);
+  if (amg) {
+    if (!b.amg.built()) {
+      std::vector<AmgMember> ms((size_t)b.nsys);
+      for (int64_t m = 0; m < b.nsys; m++) {
+        ms[m].n = b.members[m]->ndof;
+        ms[m].rowptr = b.members[m]->rowptr.data();
+        ms[m].colind = b.members[m]->colind.data();
+      }
+      b.amg.build(ms, b.node_inst.p, st);
+    }
+    if (refactor || !b.amg.ready()) b.amg.setup(b.rowptr.p, b.colind.p, b.dofdiag.p, b.J.p, b.rowscale.p, st);
+  }
   // bhat = S F
   bres(b, nullptr, b.F.p, b.bhat.p, b.active.p);
   if (!fp32) {
     bbicgstab<double, double>(b, b.bhat.p, b.dU.p, b.active.p, c.reltol_linear, c.abstol_linear, c.maxiters_linear,
-                              c.check_every, nlin);
+                              c.check_every, nlin, amg);
     return;
   }
   // refinement: tolerance per member from ||bhat||
@@ -785,7 +845,7 @@ void bsolve(vfvm_batch &b, const vfvm_control_t &c, std::vector<int64_t> &nlin) 
                        (const double *)b.rres.p, r32);
     int64_t before = 0, after = 0;
     for (auto v : nlin) before += v;
-    bbicgstab<float, float>(b, r32, d32, b.ref_active.p, inner, 0.0, c.maxiters_linear, c.check_every, nlin);
+    bbicgstab<float, float>(b, r32, d32, b.ref_active.p, inner, 0.0, c.maxiters_linear, c.check_every, nlin, amg);
     for (auto v : nlin) after += v;
     hipLaunchKernelGGL((k_badd<float>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, b.dU.p, (const float *)d32,
                        b.ref_active.p);
@@ -1172,8 +1232,10 @@ int batch_newton_impl(vfvm_batch *b, double *const *U, const double *const *Uold
       c = *control;
     else
       vfvm_default_control(&c);
-    if (c.krylov != VFVM_KRYLOV_BICGSTAB || c.precon != VFVM_PRECON_JACOBI)
-      throw Error(VFVM_ERR_ARG, "batched solves use BiCGStab with (row-scaling) Jacobi");
+    if (c.krylov != VFVM_KRYLOV_BICGSTAB || (c.precon != VFVM_PRECON_JACOBI && c.precon != VFVM_PRECON_AMG))
+      throw Error(VFVM_ERR_ARG, "batched solves use BiCGStab with (row-scaling) Jacobi or AMG");
+    if (c.precon == VFVM_PRECON_AMG && b->nspec != 1)
+      throw Error(VFVM_ERR_ARG, "the AMG preconditioner is provided for scalar systems (one species)");
     hipStream_t st = b->ctx->stream;
     const int64_t M = b->nsys;
     const Seg S = b->seg();
@@ -1221,7 +1283,7 @@ int batch_newton_impl(vfvm_batch *b, double *const *U, const double *const *Uold
       b->ctx->probe.before(VFVM_KERNEL_ASSEMBLE, st);
       launch_assemble(a, b->asm_group, st);
       b->ctx->probe.after(VFVM_KERNEL_ASSEMBLE, st, wn, wa, wj);
-      bsolve(*b, c, nlin);
+      bsolve(*b, c, nlin, it == 1 || !c.reuse_precon);
       std::memcpy(b->hdamp, damp.data(), sizeof(double) * M);
       stage_upload(b->damp, (const double *)b->hdamp, M, st);
       hipLaunchKernelGGL(k_bupdate, dim3((unsigned)b->nch), dim3(BB), 0, st, S, b->U.p, (const double *)b->dU.p,

@@ -250,6 +250,8 @@ void setup_linear(vfvm_sys &s, const vfvm_control_t &c) {
   L.rowptr = s.d_rowptr.p;
   L.colind = s.d_colind.p;
   L.dofdiag = s.d_dofdiag.p;
+  L.h_rowptr = s.rowptr.data();
+  L.h_colind = s.colind.data();
   L.sell_ptr = s.d_sell_ptr.p;
   L.sell_col = s.d_sell_col.p;
   L.sell_off16 = s.d_sell_off16.p;
@@ -263,7 +265,9 @@ void setup_linear(vfvm_sys &s, const vfvm_control_t &c) {
 
 void check_control(const vfvm_control_t &c, const vfvm_sys &s) {
   if (c.krylov != VFVM_KRYLOV_BICGSTAB && c.krylov != VFVM_KRYLOV_GMRES) throw Error(VFVM_ERR_ARG, "bad krylov");
-  if (c.precon < 0 || c.precon > 2) throw Error(VFVM_ERR_ARG, "bad precon");
+  if (c.precon < 0 || c.precon > 3) throw Error(VFVM_ERR_ARG, "bad precon");
+  if (c.precon == VFVM_PRECON_AMG && s.nspec != 1)
+    throw Error(VFVM_ERR_ARG, "the AMG preconditioner is provided for scalar systems (one species)");
   if (c.gmres_restart < 1 || c.gmres_restart > 64) throw Error(VFVM_ERR_ARG, "gmres_restart must be in 1..64");
   if (!c.updatecontrol)
     throw Error(VFVM_ERR_ARG, "updatecontrol = false (residual-norm control) is not provided by the device loop");
@@ -1013,6 +1017,55 @@ int vfvm_linsolve(vfvm_sys *s, const double *b, double *x, const vfvm_control_t 
     VFVM_HIP_CHECK(hipStreamSynchronize(st));
     if (rc) s->ctx->err = "linear solver did not reach its tolerance";
     return rc;
+  });
+}
+
+int vfvm_amg_aggregate(int64_t n, const int32_t *rowptr, const int32_t *colind, int32_t *agg, int64_t *na) {
+  if (n < 0 || !rowptr || !colind || !agg || !na) return VFVM_ERR_ARG;
+  return guarded(nullptr, [&]() {
+    std::vector<int32_t> a;
+    *na = amg_aggregate(n, rowptr, colind, a);
+    std::copy(a.begin(), a.end(), agg);
+    return VFVM_OK;
+  });
+}
+
+int vfvm_amg_levels(vfvm_sys *s, int *nlevels, int64_t *rows) {
+  if (!s || !nlevels) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    const Amg &A = s->lin.amg_hierarchy();
+    *nlevels = A.ready() ? A.levels() : 0;
+    if (rows)
+      for (int l = 0; l < *nlevels && l < 16; l++) rows[l] = A.level_rows(l);
+    return VFVM_OK;
+  });
+}
+
+int vfvm_amg_level(vfvm_sys *s, int level, int64_t *sizes, int32_t *arp, int32_t *aci, double *aval, int32_t *prp,
+                   int32_t *pci, double *pval, int32_t *agg, double *omega) {
+  if (!s || !sizes) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    const Amg &A = s->lin.amg_hierarchy();
+    std::vector<int32_t> hrp, hci, hprp, hpci, hagg;
+    std::vector<double> hav, hpv;
+    double w = 0.0;
+    if (!A.ready() || !A.export_level(level, s->ctx->stream, hrp, hci, hav, hprp, hpci, hpv, hagg, w))
+      throw Error(VFVM_ERR_ARG, "no such AMG level");
+    const int64_t n = (int64_t)hrp.size() - 1;
+    const int64_t nc = hagg.empty() ? 0 : (int64_t)(*std::max_element(hagg.begin(), hagg.end())) + 1;
+    sizes[0] = n;
+    sizes[1] = (int64_t)hci.size();
+    sizes[2] = nc;
+    sizes[3] = (int64_t)hpci.size();
+    if (arp) std::copy(hrp.begin(), hrp.end(), arp);
+    if (aci) std::copy(hci.begin(), hci.end(), aci);
+    if (aval) std::copy(hav.begin(), hav.end(), aval);
+    if (prp && nc) std::copy(hprp.begin(), hprp.end(), prp);
+    if (pci) std::copy(hpci.begin(), hpci.end(), pci);
+    if (pval) std::copy(hpv.begin(), hpv.end(), pval);
+    if (agg) std::copy(hagg.begin(), hagg.end(), agg);
+    if (omega) *omega = w;
+    return VFVM_OK;
   });
 }
 

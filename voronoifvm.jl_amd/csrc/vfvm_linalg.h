@@ -6,6 +6,7 @@
 #include <memory>
 #include <type_traits>
 
+#include "vfvm_amg.h"
 #include "vfvm_common.h"
 
 namespace vfvm {
@@ -111,6 +112,8 @@ struct LinSetup {
   int64_t ndof = 0, nnodes = 0, nnz = 0;
   int nspec = 1;
   const int32_t *rowptr = nullptr, *colind = nullptr, *dofdiag = nullptr;
+  // host copy of the CSR pattern (the AMG preconditioner's symbolic setup reads it)
+  const int32_t *h_rowptr = nullptr, *h_colind = nullptr;
   // SELL-64 pattern of the operator (one segment: slice r / 64, lane r % 64) and the
   // position of each row's first entry
   const int32_t *sell_ptr = nullptr, *sell_col = nullptr, *sell_rbase = nullptr;
@@ -132,6 +135,7 @@ class LinearSolver {
   // one plain operator application y = Ahat x in FP64 (J, row scale) or FP32 (A32), for timing
   void apply_plain(bool fp32, const double *J, const void *x, void *y);
   const double *row_scale() const { return rowscale.p; }
+  const Amg &amg_hierarchy() const { return amg; }
 
   template <class V>
   struct Work {
@@ -177,6 +181,7 @@ class LinearSolver {
   Work<double> w64;
   bool ilu_ready = false;
   bool ilu_sell_ready[2] = {false, false};  // SELL-64 copies of the factor (FP32, FP64)
+  Amg amg;  // smoothed-aggregation hierarchy (VFVM_PRECON_AMG)
   DevBuf<double> fallback64;  // FP64 correction when an FP32 refinement step fails
   DevBuf<double> xsave;       // solution before the latest FP32 correction
  public:

@@ -819,6 +819,20 @@ void LinearSolver::prepare(const double *J, const vfvm_control_t &c, bool refact
       break;
   }
   VFVM_HIP_CHECK(hipGetLastError());
+  if (c.precon == VFVM_PRECON_AMG) {
+    if (L.nspec != 1) throw Error(VFVM_ERR_ARG, "the AMG preconditioner is provided for scalar systems");
+    if (!amg.built()) {
+      if (!L.h_rowptr || !L.h_colind) throw Error(VFVM_ERR_ARG, "AMG needs the host pattern");
+      AmgMember m;
+      m.n = L.ndof;
+      m.rowptr = L.h_rowptr;
+      m.colind = L.h_colind;
+      amg.build({m}, nullptr, stream);
+    }
+    // the hierarchy of the first Newton step is reused like an ILU(0) factor
+    // (src/vfvm_solvercontrol.jl:123-131); the level-0 operator is always the current one
+    if (refactor || !amg.ready()) amg.setup(L.rowptr, L.colind, L.dofdiag, J, rowscale.p, stream);
+  }
   if (c.precon == VFVM_PRECON_ILU0 && (refactor || !ilu_ready)) {
     if (!L.ilu) throw Error(VFVM_ERR_ARG, "ILU(0) needs the multicolour structure (build_ilu)");
     const IluStruct &I = *L.ilu;
@@ -891,6 +905,19 @@ const V *LinearSolver::precond(const V *r, V *z, KState *stp, int skip_if_half, 
         hipLaunchKernelGGL((k_ilu_upper_sell<V, V>), dim3(blocks(r0, r1)), dim3(VB), 0, stream, r0, r1, I.sptr, I.scol,
                            I.sdpos, lu, I.perm, y, z, stp, skip_if_half);
     }
+    return z;
+  }
+  if (c.precon == VFVM_PRECON_AMG) {
+    AmgOp<V> op;
+    op.rowptr = L.rowptr;
+    op.rbase = L.sell_rbase;
+    op.col32 = L.sell_col;
+    op.col16 = (L.sell_off16 && sell16) ? L.sell_off16 : nullptr;
+    if constexpr (sizeof(V) == 4)
+      op.val = A32.p;
+    else
+      op.val = A64s.p;
+    amg.apply<V>(op, r, z, stp, skip_if_half, stream);
     return z;
   }
   // point Jacobi on the row-scaled operator is the identity

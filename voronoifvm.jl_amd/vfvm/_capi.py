@@ -114,6 +114,9 @@ SIGNATURES = {
     "vfvm_initialize_dirichlet": (C.c_int, [_vp, _dp, C.c_double, C.c_double]),
     "vfvm_assemble": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),
     "vfvm_linsolve": (C.c_int, [_vp, _dp, _dp, C.POINTER(ControlT), C.POINTER(LinStatsT)]),
+    "vfvm_amg_aggregate": (C.c_int, [C.c_int64, _ip, _ip, _ip, _lp]),
+    "vfvm_amg_levels": (C.c_int, [_vp, C.POINTER(C.c_int), _lp]),
+    "vfvm_amg_level": (C.c_int, [_vp, C.c_int, _lp, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _dp]),
     "vfvm_newton": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.POINTER(ControlT),
                               C.POINTER(HistoryT)]),
     "vfvm_solve_fixed_steps": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int, C.POINTER(ControlT), _dp,

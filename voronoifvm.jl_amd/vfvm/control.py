@@ -5,7 +5,7 @@ from __future__ import annotations
 from dataclasses import dataclass, fields
 
 _KRYLOV = {"bicgstab": 0, "gmres": 1}
-_PRECON = {"jacobi": 0, "blockjacobi": 1, "ilu0": 2}
+_PRECON = {"jacobi": 0, "blockjacobi": 1, "ilu0": 2, "amg": 3}
 _PRECISION = {"fp64": 0, "fp32": 1}
 
 

@@ -346,8 +346,20 @@ class System:
 
     def update_grid_device(self, coord):
         """update_grid! with the new coordinates (dim x nnodes, column-major) in a
-        DeviceArray; the cells and boundary faces stay those of the system's grid."""
+        DeviceArray of the system's context; the cells and boundary faces stay those of
+        the system's grid.  The grid object then holds the new coordinates (read back from
+        the device), so a callable source and a later rebuild of the device system see
+        the updated geometry.  Raises ValueError for an array of another size or context."""
+        g = self.grid
+        n = g.dim * g.num_nodes
+        if getattr(coord, "n", None) != n or getattr(coord, "ptr", None) is None:
+            raise ValueError(f"update_grid_device needs a DeviceArray of dim x nnodes = {n} coordinates")
+        if getattr(coord, "ctx", None) is not self.ctx:
+            raise ValueError("update_grid_device needs coordinates in the system's context")
         _capi.check(_capi.lib().vfvm_update_grid_device(self.handle(), coord.ptr), self.ctx.handle)
+        import dataclasses
+        xy = np.ascontiguousarray(coord.numpy().reshape((g.dim, g.num_nodes), order="F"))
+        self.grid = dataclasses.replace(g, coord=xy)
 
     def update_grid(self, grid: Grid):
         """update_grid!(system; grid) (src/vfvm_system.jl:607-612): `grid` has the cells and
@@ -429,6 +441,22 @@ def evaluate_residual_and_jacobian(sys: System, u, t=0.0, tstep=math.inf, embed=
     cp, rv = sys.pattern()
     A = sp.csc_matrix((nz[:nnz], rv - 1, cp - 1), shape=(ndof, ndof))
     return _from_dofs(sys, F), A
+
+
+def linsolve(sys: System, b, control: Optional[SolverControl] = None, **kwargs):
+    """_solve_linear! (src/vfvm_linsolve.jl:6-61) on the Jacobian of the system's last
+    assembly (evaluate_residual_and_jacobian, or the last Newton step): x with J x = b and
+    the solver statistics (iterations, converged, residual norms).  The preconditioner is
+    built from that Jacobian.  Raises LinearSolverError if the tolerance is not reached."""
+    c = _control(control, kwargs)
+    rhs = _as_dofs(sys, b).copy()
+    x = np.zeros_like(rhs)
+    st = _capi.LinStatsT()
+    rc = _capi.lib().vfvm_linsolve(sys.handle(), _capi.dptr(rhs), _capi.dptr(x), c.to_c(), st)
+    stats = dict(iters=st.iters, refinements=st.refinements, converged=bool(st.converged),
+                 fp64_fallback=bool(st.fp64_fallback), resnorm0=st.resnorm0, resnorm=st.resnorm)
+    _capi.check(rc, sys.ctx.handle)
+    return x, stats
 
 
 def initialize_dirichlet(sys: System, u, t=0.0, embed=0.0):
