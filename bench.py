@@ -8,8 +8,9 @@ rows.  Inputs are resident on the device before the timed region; only the
 solve calls are timed.  Workloads:
 
   cfg4 (default)  1,024 held-out 129x129 nonlinear-Poisson inputs: learned initial
-                  guess + batches of 256 + FP32 Krylov, against the plain FP64 GPU
-                  Newton from zero on the same inputs; a step is one batch of 256
+                  guess + batches of 512 + FP32 BiCGStab with a smoothed-aggregation
+                  AMG preconditioner, against the plain FP64 GPU Newton from zero on
+                  the same inputs; a step is one batch of 512
   cfg0            101^2 nonlinear Poisson (the reference's CPU-runnable case); a step
                   is one solve
   cfg1            129^3 nonlinear diffusion, implicit Euler; a step is one time step
@@ -70,6 +71,8 @@ def probe_bytes(kind, probe, sv=4, nspec=1, transient=True):
     w, n = probe["work"], max(probe["launches"], 1)
     if kind == "spmv":
         return spmv_bytes(w[0] / n, w[1] / n, sv)
+    if kind == "amg_sweep":  # out = x + omega (b - A x): an SpMV's bytes and b once more
+        return spmv_bytes(w[0] / n, w[1] / n, sv) + sv * w[0] / n
     return assembly_bytes(nspec, w[0] / n, w[1] / n, w[2] / n, transient)
 
 
@@ -492,8 +495,12 @@ def run_cfg4(args):
     fam = fams[0]
     for f in fams:
         f.batch.set_graphs(not args.no_graphs)
-    ctrl32 = vfvm.SolverControl(precision="fp32", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=3000,
-                                inner_reltol=INNER_RELTOL, max_refinements=12, check_every=8)
+    # BiCGStab preconditioned by the smoothed-aggregation AMG V-cycle (the reference's
+    # SmoothedAggregationPreconBuilder, examples/DevEx003_Solvers.jl:149-156), FP32 inside
+    # FP64 refinement; the FP32 solve aims at reltol_linear at once (one refinement sweep
+    # per Newton step), and the solver states are read after every iteration
+    ctrl32 = vfvm.SolverControl(precision="fp32", precon="amg", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0,
+                                maxiters_linear=3000, inner_reltol=RELTOL_LINEAR, max_refinements=12, check_every=1)
     ctx = fam.systems[0].ctx
     n = fam.grid.num_nodes
     ndof, off = fam.batch.sizes()
@@ -528,8 +535,8 @@ def run_cfg4(args):
                     nxt.result()
         return hist
 
-    ctrl64b = vfvm.SolverControl(precision="fp64", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=3000,
-                                 check_every=8)
+    ctrl64b = vfvm.SolverControl(precision="fp64", precon="amg", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0,
+                                 maxiters_linear=3000, check_every=1)
     for _ in range(max(args.warmup, 0)):  # warm-up (untimed): every batch, every path
         pipeline(nb)
         for f in fams:
@@ -545,7 +552,7 @@ def run_cfg4(args):
     # beside it, per block: the host-buffer path, a probe pass, and the batched solves from zero
     t_sur_host, t_zero, t_zero64, it_zero, sur_err = 0.0, 0.0, 0.0, [], []
     converged = []
-    probes = {_capi.KERNEL_SPMV32: [], _capi.KERNEL_HALFSTEP32: [], _capi.KERNEL_ASSEMBLE: []}
+    probes = {_capi.KERNEL_SPMV32: [], _capi.KERNEL_AMG32: [], _capi.KERNEL_ASSEMBLE: []}
     for b in range(min(K, nb)):
         f = fams[b]
         infer(b)
@@ -557,7 +564,7 @@ def run_cfg4(args):
         Uh = f.batch.solve([x.reshape(1, -1) for x in gh], ctrl32)
         t_sur_host += time.perf_counter() - t1
         worst = max(worst, max(h.residual_norm for h in f.batch.histories))
-        with ctx.probe(_capi.KERNEL_SPMV32, _capi.KERNEL_HALFSTEP32, _capi.KERNEL_ASSEMBLE) as pr:  # timed one by one
+        with ctx.probe(_capi.KERNEL_SPMV32, _capi.KERNEL_AMG32, _capi.KERNEL_ASSEMBLE) as pr:  # timed one by one
             f.batch.solve_device(U_dev[b], G_dev[b], ctrl32)
         for k, v in pr.items():
             probes[k].append(v)
@@ -646,17 +653,16 @@ def run_cfg4(args):
                     avg_ms=sum(x["total_ms"] for x in lst) / max(sum(x["launches"] for x in lst), 1),
                     work=tuple(sum(x["work"][i] for x in lst) for i in range(3)))
     pspmv, pasm = merged(probes[_capi.KERNEL_SPMV32]), merged(probes[_capi.KERNEL_ASSEMBLE])
-    phalf = merged(probes[_capi.KERNEL_HALFSTEP32])
+    pamg = merged(probes[_capi.KERNEL_AMG32])
     rl_spmv = roofline_hbm(probe_bytes("spmv", pspmv, sv=4), pspmv, read_traffic("cfg4_bspmv32"),
                            "batched FP32 SpMV")
     rl_spmv["kernel"] = f"batched FP32 SpMV over the active members of {B} x 129^2 (k_bspmv)"
-    # the second operator application with the half step fused: r and v gathered per
-    # column instead of s, s and t written (2 x 4 bytes per row more than an SpMV)
-    rows_half = phalf["work"][0] / max(phalf["launches"], 1)
-    rl_half = roofline_hbm(probe_bytes("spmv", phalf, sv=4) + 2 * 4 * rows_half, phalf,
-                           read_traffic("cfg4_bspmv_s32"), "batched FP32 SpMV fused with the half step")
-    rl_half["kernel"] = (f"batched FP32 SpMV fused with the BiCGStab half step over the active members of "
-                         f"{B} x 129^2 (k_bspmv_s): the kernel with the most time in the timed batches")
+    # the AMG V-cycle's finest-level post-smoothing sweep: an SpMV of the row-scaled
+    # operator plus b read once more, over the members still iterating
+    rl_amg = roofline_hbm(probe_bytes("amg_sweep", pamg, sv=4), pamg, read_traffic("cfg4_amg_sweep32"),
+                          "AMG finest-level sweep (FP32)")
+    rl_amg["kernel"] = (f"AMG V-cycle finest-level Jacobi sweep over the active members of {B} x 129^2 "
+                        f"(k_amg_sweep, FP32 SELL-64, shared column indices)")
     rl_asm = roofline_hbm(probe_bytes("assemble", pasm, transient=False), pasm, read_traffic("cfg4_assemble"),
                           "batched assembly")
     import ctypes as C
@@ -683,7 +689,7 @@ def run_cfg4(args):
                 max_abs_diff_vs_plain=diff, worst_residual=worst,
                 surrogate=dict(train_samples=int(meta["samples"]), epochs=int(meta["epochs"]),
                                final_train_mse=float(meta["final_loss"])),
-                roofline=rl_half, roofline_bspmv=rl_spmv, roofline_assemble=rl_asm,
+                roofline=rl_amg, roofline_bspmv=rl_spmv, roofline_assemble=rl_asm,
                 roofline_mpnn_layer=dict(bound="hbm", achieved=round(layer_gbs, 1), peak=8000.0, unit="GB/s",
                                          frac=round(layer_gbs / 8000.0, 4), traffic=read_traffic("cfg4_mpnn_layer"),
                                          traffic_note="mean over the four layers of one inference; the last "
@@ -695,9 +701,10 @@ def run_cfg4(args):
                                      "values U[-1,1]), 1,024 held-out seeds in 2 batches of 512 (step k solves "
                                      "batch k mod 2; the inference of step k+1 runs on a second stream during "
                                      "the Newton solve of step k): MPNN initial guess "
-                                     "(4 layers, width 64, BF16 MFMA) + batched BiCGStab/Jacobi FP32 inside FP64 "
-                                     "Newton, vs plain FP64 GPU Newton from zero one solve at a time; "
-                                     "reltol_linear 1e-4 (the reference's default) in every form",
+                                     "(4 layers, width 64, BF16 MFMA) + batched BiCGStab preconditioned by "
+                                     "smoothed-aggregation AMG, FP32 inside FP64 Newton, vs plain FP64 GPU Newton "
+                                     "(BiCGStab/Jacobi) from zero one solve at a time; reltol_linear 1e-4 (the "
+                                     "reference's default) in every form",
                             solves=nsol, unknowns_per_solve=fam.grid.num_nodes, batch=B,
                             hip_graphs=not args.no_graphs,
                             resident="sources, side values, guesses and solutions in HBM (the *_device "

@@ -484,18 +484,24 @@ int vfvm_set_compact_graph(vfvm_sys *sys, int enable);
  * half step (s = r - alpha v gathered per column, t = Ahat s), FP64 and FP32 */
 #define VFVM_KERNEL_HALFSTEP64 4
 #define VFVM_KERNEL_HALFSTEP32 5
+/* probe classes only: the AMG V-cycle's finest-level post-smoothing sweep
+ * (out = x + omega (b - Ahat x)), FP64 and FP32 */
+#define VFVM_KERNEL_AMG64 6
+#define VFVM_KERNEL_AMG32 7
 int vfvm_time_kernel(vfvm_sys *sys, int kernel, int reps, double *avg_ms);
 
 /* Benchmark helper (no counterpart in the reference): live timing of kernel classes
  * inside real calls on the context.  After vfvm_probe_arm(kernel), every launch of that
  * class (VFVM_KERNEL_ASSEMBLE: Newton-step Jacobian assembly; _SPMV64 / _SPMV32: Krylov
  * operator applications; _HALFSTEP64 / _HALFSTEP32: the batched operator applications
- * fused with the half step), single-system or batched, is bracketed by HIP events on the
+ * fused with the half step; _AMG64 / _AMG32: the AMG cycle's finest-level sweep),
+ * single-system or batched, is bracketed by HIP events on the
  * launching stream, up to max_launches launches.  Several classes may be armed at once.
  * vfvm_probe_read returns one class's summed duration, the number of timed launches and
  * (work, 3 entries, nullable) the work those launches did, then disarms that class:
  * SpMV: rows, nonzeros, 0; assembly: nodes, adjacency entries without the node itself
- * (2 x edges), Jacobian slots.  In batches only the members still active count. */
+ * (2 x edges), Jacobian slots; AMG sweep: rows, nonzeros, 0.  In batches only the members
+ * still active count. */
 int vfvm_probe_arm(vfvm_ctx *ctx, int kernel, int64_t max_launches);
 int vfvm_probe_read(vfvm_ctx *ctx, int kernel, double *total_ms, int64_t *launches, int64_t *work);
 

@@ -160,6 +160,29 @@ def test_block_jacobi_gmres_drift_diffusion_continuation():
         assert err <= SOL_ATOL
 
 
+def test_cfg2_configured_solver_at_65():
+    """The cfg2 solver as configured: GMRES(30) with 3x3 block Jacobi, FP32 inside FP64
+    refinement, Debye length lambda = 1e-2 (lambda^2 = 1e-4), a bias sweep in steps of 0.05 V,
+    on 65 x 65 nodes against the oracle's direct solves.  The FP32 solves that cannot reach
+    the tolerance are finished in FP64 (counted in the history), so the sweep's answer is
+    the oracle's whatever their number."""
+    _print("case cfg2 solver 65")
+    sys, osys = cases.drift_diffusion_2d(65, lambda2=1e-4)
+    U0 = cases.dd_initial(sys.grid)
+    lam = np.linspace(0.0, 0.5, 11)
+    c = O.control(method_linear="direct", dp=0.05, dp_min=0.05, dp_max=0.05, du_opt=10.0)
+    Uo, _, _ = osys.solve_transient(vfvm.dofs(U0), lam, c, transient=False)
+    ctrl = vfvm.SolverControl(krylov="gmres", precon="blockjacobi", precision="fp32", gmres_restart=30,
+                              reltol_linear=1e-10, abstol_linear=0.0, maxiters_linear=6000, Δp=0.05, Δp_min=0.05,
+                              Δp_max=0.05, Δu_opt=10.0)
+    U = vfvm.solve(sys, inival=U0, control=ctrl, embed=lam).u[-1]
+    err = np.abs(vfvm.dofs(U) - Uo).max()
+    _print("cfg2 65: err", err, "fp64 fallbacks of the last step", sys.history.fp64_fallbacks,
+           "krylov", sys.history.nlin)
+    assert err <= SOL_ATOL
+    assert sys.history.residual_norm <= 1e-10
+
+
 def test_ilu0_bicgstab_drift_diffusion_three_species():
     """cfg2 physics (lambda = 1e-2) on 65 x 65 nodes: BiCGStab with multicolour ILU(0)
     of the 3-species dof graph, FP64 and FP32 inside FP64 refinement, against the

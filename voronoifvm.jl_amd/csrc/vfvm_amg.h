@@ -46,15 +46,28 @@ struct AmgMember {
   const int32_t *rowptr = nullptr, *colind = nullptr;
 };
 
-// The owner's level-0 operator Ahat in SELL-64 order: entry j of row r at rbase[r] + 64 j,
-// row lengths from the CSR rowptr; columns absolute (col32) or 16-bit offsets from the row
-// (col16, used when not null).
-template <class V>
-struct AmgOp {
-  const int32_t *rowptr = nullptr, *rbase = nullptr;
-  const int32_t *col32 = nullptr;
-  const int16_t *col16 = nullptr;
-  const V *val = nullptr;
+// One slice of 64 rows of a SELL-64 matrix.  Entry j of lane l of the slice has its value
+// at vbase + 64 j + l and its column at coff + idx[ibase + 64 j + l]; the index array is
+// shared by the members with equal patterns (it stays in the L2), the values are each
+// member's own.  A wave reads its slice's descriptor once.
+struct SellDesc {
+  int64_t vbase;
+  int32_t ibase, coff;
+  int32_t row0, rows, width, inst;
+  int32_t first, pad;  // first: the member's first slice of this matrix
+};
+
+// A level's matrix in that form (A, P or P^T), with the per-row value base the numeric
+// setup scatters CSR values through (-1: row of a member without this matrix).  Values in
+// FP64 for FP64 Krylov solves and in FP16 for FP32 ones: the cycle is a preconditioner,
+// so its matrices need not carry the Krylov precision (vectors and sums keep it).
+struct SellMat {
+  int64_t nsl = 0, nval = 0;
+  DevBuf<SellDesc> desc;
+  DevBuf<int32_t> idx;
+  DevBuf<int64_t> rbase;
+  DevBuf<double> v64;
+  DevBuf<_Float16> v16;
 };
 
 // Host symbolic hierarchy of one member pattern (member-local indices).
@@ -70,6 +83,10 @@ struct AmgPattern {
     std::vector<int32_t> a2p;            // per A entry (i, k): position of agg[k] in P row i
     std::vector<int32_t> tpos;           // per row: position of agg[i] in P row i
     std::vector<int32_t> rrp, rci, r2p;  // P^T: coarse row -> (fine row, P position)
+    // Galerkin recipe (when it fits MAX_TRIPLES): for each entry of the next level's A, in
+    // CSR order, the products P[i,I] A[i,k] P[k,J] that sum to it, as (P position, A
+    // position, P position) triples in the order the sum takes them
+    std::vector<int32_t> gptr, gr, ga, gp;
   };
   std::vector<Level> lv;  // lv.back() is the coarsest level
   void build(int64_t n, const int32_t *rowptr, const int32_t *colind);
@@ -85,6 +102,8 @@ class Amg {
   static constexpr int64_t MAX_COARSE = 64;  // coarsening stops at or below this many rows
   static constexpr int64_t DENSE_MAX = 88;   // coarsest levels up to this size are inverted
   static constexpr int COARSE_SWEEPS = 4;    // damped Jacobi sweeps on a larger coarsest level
+  // patterns whose Galerkin recipe exceeds this many triples form P^T A P by search instead
+  static constexpr int64_t MAX_TRIPLES = int64_t(1) << 24;
   enum { MODE_COARSEN = 0, MODE_DENSE = 1, MODE_JACOBI = 2, MODE_ABSENT = -1 };
 
   // Symbolic setup.  members: the owner's members in order (their level-0 rows are the
@@ -97,10 +116,16 @@ class Amg {
   void setup(const int32_t *rowptr, const int32_t *colind, const int32_t *dofdiag, const double *J,
              const double *rowscale, hipStream_t s);
   bool ready() const { return numeric_ready; }
+  // The level-0 operator of the cycle from the current J (every Newton step; the coarse
+  // levels of the last setup are kept): the FP32 solves' values only (fp32) or both
+  void refresh_fine(const int32_t *rowptr, const double *J, const double *rowscale, bool fp32, hipStream_t s);
   // z = M^-1 y, one V(1,1) cycle on every member whose Krylov state is running (st may be
   // null: all members; skip_half: members whose BiCGStab half step converged are skipped)
+  // probe (nullable): the context's kernel probe; the finest-level post-smoothing sweep is
+  // its class VFVM_KERNEL_AMG32 / _AMG64, its work the rows and nonzeros of the members
+  // it ran on
   template <class V>
-  void apply(const AmgOp<V> &A0, const V *y, V *z, const KState *st, int skip_half, hipStream_t s);
+  void apply(const V *y, V *z, const KState *st, int skip_half, hipStream_t s, KernelProbe *probe = nullptr);
   // allocates apply's work vectors of precision V (before a stream capture of apply)
   template <class V>
   void reserve() {
@@ -123,25 +148,37 @@ class Amg {
     int64_t n = 0, nc = 0, nnzA = 0, nnzP = 0, nnzR = 0;
     bool any_coarsen = false, any_dense = false, any_jacobi = false;
     std::vector<int64_t> roff;  // per member: first row at this level (-1 absent)
+    std::vector<int64_t> aoffm, poffm;  // per member: first A entry, first P entry
     std::vector<int32_t> hmode;
     DevBuf<int32_t> inst, mode;
     const int32_t *inst_p = nullptr;
     DevBuf<double> w;
-    // A at levels >= 1 (level 0: the owner's CSR and SELL)
-    DevBuf<int32_t> arp, aci, adp, a_rbase, a_scol;
-    DevBuf<double> aval, a64, dinv64;
-    DevBuf<float> a32, dinv32;
-    // P and P^T of the coarsening members
-    DevBuf<int32_t> prp, pci, a2p, tpos, p_rbase, p_scol;
-    DevBuf<double> tval, pval, p64;
-    DevBuf<float> p32;
-    DevBuf<int32_t> rrp, rci, r2p, r_rbase, r_scol;
-    DevBuf<double> r64;
-    DevBuf<float> r32;
+    // A: CSR at levels >= 1 (level 0: the owner's CSR), FP64 values for the setup
+    DevBuf<int32_t> arp, aci, adp;
+    DevBuf<double> aval, dinv64;
+    DevBuf<float> dinv32;
+    // P and P^T of the coarsening members (CSR for the setup)
+    DevBuf<int32_t> prp, pci, a2p, tpos;
+    DevBuf<double> tval, pval;
+    DevBuf<int32_t> rrp, rci, r2p;
+    // the matrices of the cycle
+    SellMat A, P, R;
     // coarsening members: (member, first row, end row) for the omega reduction
     int64_t ncm = 0;
     DevBuf<int32_t> cm;
     DevBuf<int64_t> cb, ce;
+    // Galerkin recipe per distinct pattern of the coarsening members: the pattern's
+    // triples and, per member, the offsets of its A and P values and of its coarse A
+    // (on the device the triples of 64 consecutive coarse entries are interleaved: the
+    // t-th triple of entry c at gsb[c / 64] + 64 t + c % 64, gcnt[c] of them)
+    struct Recipe {
+      int64_t nslot = 0, nmem = 0;
+      DevBuf<int32_t> gcnt, gr, ga, gp;
+      DevBuf<int64_t> gsb;
+      DevBuf<int64_t> maoff, mpoff, mcoff;
+    };
+    std::vector<Recipe> recipes;
+    bool by_recipe = false;  // every coarsening member has a recipe
     // exactly inverted members: member, first row, size, inverse offset
     int64_t ndm = 0, dmax = 0;
     DevBuf<int32_t> dmem, dsz;
@@ -177,8 +214,10 @@ class Amg {
   std::vector<AmgPattern> patterns;
   std::vector<int32_t> pid;  // member -> pattern
   std::vector<Lev> lev;
-  DevBuf<int32_t> inst0_own;
   int64_t nnz0 = 0;
+  DevBuf<int64_t> msize0;  // per member: level-0 rows and nonzeros (probe work)
+  // SELL-64 form of one kind of matrix (0 A, 1 P, 2 P^T) of level l over its members
+  void build_sell(int l, int kind, SellMat &M, hipStream_t s);
 };
 
 }  // namespace vfvm

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <unordered_map>
 
 namespace vfvm {
@@ -120,6 +121,32 @@ __global__ void __launch_bounds__(AB) k_amg_rap(int64_t nc, const int32_t *__res
   }
 }
 
+// The same product by the pattern's recipe: one thread per coarse entry of one member
+// (blockIdx.y), no searches, the triples of a wave's 64 entries interleaved so that each
+// step reads them in one sweep; the sum runs in the search kernel's order
+__global__ void __launch_bounds__(AB) k_amg_rap_recipe(int64_t nslot, const int32_t *__restrict__ gcnt,
+                                                       const int64_t *__restrict__ gsb, const int32_t *__restrict__ gr,
+                                                       const int32_t *__restrict__ ga, const int32_t *__restrict__ gp,
+                                                       const int64_t *__restrict__ maoff,
+                                                       const int64_t *__restrict__ mpoff,
+                                                       const int64_t *__restrict__ mcoff, int64_t mbase,
+                                                       const double *__restrict__ aval, const double *__restrict__ pval,
+                                                       double *__restrict__ cval) {
+  const int64_t c = (int64_t)blockIdx.x * AB + threadIdx.x;
+  if (c >= nslot) return;
+  const int64_t k = mbase + blockIdx.y;
+  const double *A = aval + maoff[k];
+  const double *P = pval + mpoff[k];
+  const int64_t t0 = gsb[c >> 6] + (c & 63);
+  const int32_t n = gcnt[c];
+  double acc = 0.0;
+  for (int32_t t = 0; t < n; t++) {
+    const int64_t q = t0 + (int64_t)64 * t;
+    acc += (P[gr[q]] * A[ga[q]]) * P[gp[q]];
+  }
+  cval[mcoff[k] + c] = acc;
+}
+
 __global__ void __launch_bounds__(AB) k_amg_dinv(int64_t n, const int32_t *__restrict__ adp,
                                                  const double *__restrict__ aval, double *__restrict__ d64,
                                                  float *__restrict__ d32) {
@@ -131,19 +158,23 @@ __global__ void __launch_bounds__(AB) k_amg_dinv(int64_t n, const int32_t *__res
   d32[i] = (float)v;
 }
 
-// CSR values (optionally gathered through map) into SELL-64 order, both precisions
+// CSR values (optionally gathered through map, optionally scaled per row) into a SELL
+// matrix's values, in FP64 (d64) and / or FP16 (d16, clamped to its finite range)
 __global__ void __launch_bounds__(AB) k_amg_to_sell(int64_t n, const int32_t *__restrict__ rowptr,
-                                                    const int32_t *__restrict__ rbase, const double *__restrict__ src,
-                                                    const int32_t *__restrict__ map, double *__restrict__ d64,
-                                                    float *__restrict__ d32) {
+                                                    const int64_t *__restrict__ rbase, const double *__restrict__ src,
+                                                    const int32_t *__restrict__ map, const double *__restrict__ scale,
+                                                    double *__restrict__ d64, _Float16 *__restrict__ d16) {
   const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
   if (i >= n) return;
+  const int64_t b = rbase[i];
+  if (b < 0) return;
+  const double sc = scale ? scale[i] : 1.0;
   const int32_t q0 = rowptr[i];
   for (int32_t q = q0; q < rowptr[i + 1]; q++) {
-    const double v = map ? src[map[q]] : src[q];
-    const int64_t o = (int64_t)rbase[i] + (int64_t)64 * (q - q0);
-    d64[o] = v;
-    d32[o] = (float)v;
+    const double v = sc * (map ? src[map[q]] : src[q]);
+    const int64_t o = b + (int64_t)64 * (q - q0);
+    if (d64) d64[o] = v;
+    if (d16) d16[o] = (_Float16)fmin(fmax(v, -65504.0), 65504.0);
   }
 }
 
@@ -221,113 +252,117 @@ __global__ void __launch_bounds__(AB) k_amg_dense_inv(const int32_t *__restrict_
 
 // ---------------------------------------------------------------- V-cycle (precision V)
 
-// pre-smoothing from zero and the residual, on the rows of coarsening members:
-// x = omega D^-1 b, r = b - A x (the neighbours' x recomputed from b)
-template <class V, class CI>
-__global__ void __launch_bounds__(AB) k_amg_down(int64_t n, const int32_t *__restrict__ rowptr,
-                                                 const int32_t *__restrict__ rbase, const CI *__restrict__ col,
-                                                 const V *__restrict__ val, const V *__restrict__ dinv,
-                                                 const double *__restrict__ w, const int32_t *__restrict__ inst,
+// A SELL matrix as the cycle's kernels see it (values of type V).  Blocks of AB threads
+// cover AB / 64 slices, one wave per slice; a lane past the slice's rows, or in a member
+// that is not in `want` mode or whose Krylov state says to skip, does nothing.
+template <class V>
+struct SView {
+  const SellDesc *desc;
+  int64_t nsl;
+  const int32_t *idx;
+  const V *val;
+};
+// value type of the cycle's matrices for Krylov precision V
+template <class V>
+using AmgVal = std::conditional_t<sizeof(V) == 4, _Float16, double>;
+
+struct Lane {
+  SellDesc d;
+  int lane;
+  int64_t row;
+};
+
+template <class V>
+__device__ __forceinline__ bool amg_lane(const SView<V> &M, const int32_t *mode, int want, const KState *st,
+                                         int skip_half, Lane &L) {
+  const int64_t sl = (int64_t)blockIdx.x * (AB / 64) + (threadIdx.x >> 6);
+  if (sl >= M.nsl) return false;
+  L.d = M.desc[sl];
+  L.lane = threadIdx.x & 63;
+  if (L.lane >= L.d.rows) return false;
+  if (mode && mode[L.d.inst] != want) return false;
+  if (amg_skip(st, L.d.inst, skip_half)) return false;
+  L.row = (int64_t)L.d.row0 + L.lane;
+  return true;
+}
+
+// sum_j M[row, j] f(column)
+template <class V, class F>
+__device__ __forceinline__ double amg_row(const SView<V> &M, const Lane &L, F &&f) {
+  double acc = 0.0;
+  const int64_t v0 = L.d.vbase + L.lane;
+  const int64_t i0 = (int64_t)L.d.ibase + L.lane;
+  for (int32_t j = 0; j < L.d.width; j++)
+    acc += (double)M.val[v0 + (int64_t)64 * j] * f((int64_t)L.d.coff + M.idx[i0 + (int64_t)64 * j]);
+  return acc;
+}
+
+// pre-smoothing from zero and the residual on the rows of coarsening members:
+// r = b - A x with x = omega D^-1 b (x itself is not stored: the prolongation recomputes it)
+template <class V>
+__global__ void __launch_bounds__(AB) k_amg_down(SView<AmgVal<V>> A, const V *__restrict__ dinv, const double *__restrict__ w,
                                                  const int32_t *__restrict__ mode, const V *__restrict__ b,
-                                                 V *__restrict__ x, V *__restrict__ r, const KState *st,
-                                                 int skip_half) {
-  const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
-  if (i >= n) return;
-  const int m = amg_member(inst, i);
-  if (mode[m] != Amg::MODE_COARSEN || amg_skip(st, m, skip_half)) return;
-  const double wm = w[m];
-  const int64_t base = rbase[i];
-  const int32_t len = rowptr[i + 1] - rowptr[i];
-  double acc = 0.0;
-  for (int32_t j = 0; j < len; j++) {
-    const int64_t q = base + (int64_t)64 * j;
-    const int64_t k = sell_column(col[q], i);
-    const double dk = dinv ? (double)dinv[k] : 1.0;
-    acc += (double)val[q] * (wm * dk * (double)b[k]);
-  }
-  const double di = dinv ? (double)dinv[i] : 1.0;
-  x[i] = (V)(wm * di * (double)b[i]);
-  r[i] = (V)((double)b[i] - acc);
+                                                 V *__restrict__ r, const KState *st, int skip_half) {
+  Lane L;
+  if (!amg_lane(A, mode, Amg::MODE_COARSEN, st, skip_half, L)) return;
+  const double wm = w[L.d.inst];
+  const double acc =
+      amg_row(A, L, [&](int64_t k) { return wm * (dinv ? (double)dinv[k] : 1.0) * (double)b[k]; });
+  r[L.row] = (V)((double)b[L.row] - acc);
 }
 
-// b_coarse = P^T r (rows of the next level)
+// b_coarse = P^T r (every row of the next level belongs to a coarsening member)
 template <class V>
-__global__ void __launch_bounds__(AB) k_amg_restrict(int64_t nc, const int32_t *__restrict__ rrp,
-                                                     const int32_t *__restrict__ rbase, const int32_t *__restrict__ col,
-                                                     const V *__restrict__ val, const int32_t *__restrict__ inst_c,
-                                                     const V *__restrict__ r, V *__restrict__ bc, const KState *st,
-                                                     int skip_half) {
-  const int64_t I = (int64_t)blockIdx.x * AB + threadIdx.x;
-  if (I >= nc) return;
-  if (amg_skip(st, amg_member(inst_c, I), skip_half)) return;
-  const int64_t base = rbase[I];
-  const int32_t len = rrp[I + 1] - rrp[I];
-  double acc = 0.0;
-  for (int32_t j = 0; j < len; j++) {
-    const int64_t q = base + (int64_t)64 * j;
-    acc += (double)val[q] * (double)r[col[q]];
-  }
-  bc[I] = (V)acc;
+__global__ void __launch_bounds__(AB) k_amg_restrict(SView<AmgVal<V>> R, const V *__restrict__ r, V *__restrict__ bc,
+                                                     const KState *st, int skip_half) {
+  Lane L;
+  if (!amg_lane(R, (const int32_t *)nullptr, 0, st, skip_half, L)) return;
+  bc[L.row] = (V)amg_row(R, L, [&](int64_t k) { return (double)r[k]; });
 }
 
-// x += P x_coarse on the rows of coarsening members
+// x = omega D^-1 b + P x_coarse on the rows of coarsening members
 template <class V>
-__global__ void __launch_bounds__(AB) k_amg_prolong(int64_t n, const int32_t *__restrict__ prp,
-                                                    const int32_t *__restrict__ rbase, const int32_t *__restrict__ col,
-                                                    const V *__restrict__ val, const int32_t *__restrict__ inst,
-                                                    const int32_t *__restrict__ mode, const V *__restrict__ xc,
+__global__ void __launch_bounds__(AB) k_amg_prolong(SView<AmgVal<V>> P, const V *__restrict__ dinv,
+                                                    const double *__restrict__ w, const int32_t *__restrict__ mode,
+                                                    const V *__restrict__ b, const V *__restrict__ xc,
                                                     V *__restrict__ x, const KState *st, int skip_half) {
-  const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
-  if (i >= n) return;
-  const int m = amg_member(inst, i);
-  if (mode[m] != Amg::MODE_COARSEN || amg_skip(st, m, skip_half)) return;
-  const int64_t base = rbase[i];
-  const int32_t len = prp[i + 1] - prp[i];
-  double acc = (double)x[i];
-  for (int32_t j = 0; j < len; j++) {
-    const int64_t q = base + (int64_t)64 * j;
-    acc += (double)val[q] * (double)xc[col[q]];
-  }
-  x[i] = (V)acc;
+  Lane L;
+  if (!amg_lane(P, mode, Amg::MODE_COARSEN, st, skip_half, L)) return;
+  const double di = dinv ? (double)dinv[L.row] : 1.0;
+  const double acc = amg_row(P, L, [&](int64_t k) { return (double)xc[k]; });
+  x[L.row] = (V)(w[L.d.inst] * di * (double)b[L.row] + acc);
 }
 
 // one damped Jacobi sweep out = x + omega D^-1 (b - A x) on the rows of members in `want`
-// mode (post-smoothing of coarsening members, sweeps on a large coarsest level)
-template <class V, class CI>
-__global__ void __launch_bounds__(AB) k_amg_sweep(int64_t n, const int32_t *__restrict__ rowptr,
-                                                  const int32_t *__restrict__ rbase, const CI *__restrict__ col,
-                                                  const V *__restrict__ val, const V *__restrict__ dinv,
-                                                  const double *__restrict__ w, const int32_t *__restrict__ inst,
+// mode (post-smoothing of coarsening members, sweeps on a large coarsest level).  dwork
+// (probe counters, nullable): the first slice of each member it runs on adds the member's
+// rows and nonzeros (msize), one atomic pair per member and launch.
+template <class V>
+__global__ void __launch_bounds__(AB) k_amg_sweep(SView<AmgVal<V>> A, const V *__restrict__ dinv, const double *__restrict__ w,
                                                   const int32_t *__restrict__ mode, int want, const V *__restrict__ b,
                                                   const V *__restrict__ x, V *__restrict__ out, const KState *st,
-                                                  int skip_half) {
-  const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
-  if (i >= n) return;
-  const int m = amg_member(inst, i);
-  if (mode[m] != want || amg_skip(st, m, skip_half)) return;
-  const int64_t base = rbase[i];
-  const int32_t len = rowptr[i + 1] - rowptr[i];
-  double acc = 0.0;
-  for (int32_t j = 0; j < len; j++) {
-    const int64_t q = base + (int64_t)64 * j;
-    acc += (double)val[q] * (double)x[sell_column(col[q], i)];
+                                                  int skip_half, unsigned long long *dwork,
+                                                  const int64_t *__restrict__ msize) {
+  Lane L;
+  if (!amg_lane(A, mode, want, st, skip_half, L)) return;
+  if (dwork && L.lane == 0 && L.d.first) {
+    atomicAdd(dwork, (unsigned long long)msize[2 * L.d.inst]);
+    atomicAdd(dwork + 1, (unsigned long long)msize[2 * L.d.inst + 1]);
   }
-  const double di = dinv ? (double)dinv[i] : 1.0;
-  out[i] = (V)((double)x[i] + w[m] * di * ((double)b[i] - acc));
+  const double acc = amg_row(A, L, [&](int64_t k) { return (double)x[k]; });
+  const double di = dinv ? (double)dinv[L.row] : 1.0;
+  out[L.row] = (V)((double)x[L.row] + w[L.d.inst] * di * ((double)b[L.row] - acc));
 }
 
 // first sweep from zero on a large coarsest level: x = omega D^-1 b
 template <class V>
-__global__ void __launch_bounds__(AB) k_amg_jinit(int64_t n, const V *__restrict__ dinv, const double *__restrict__ w,
-                                                  const int32_t *__restrict__ inst, const int32_t *__restrict__ mode,
-                                                  const V *__restrict__ b, V *__restrict__ x, const KState *st,
-                                                  int skip_half) {
-  const int64_t i = (int64_t)blockIdx.x * AB + threadIdx.x;
-  if (i >= n) return;
-  const int m = amg_member(inst, i);
-  if (mode[m] != Amg::MODE_JACOBI || amg_skip(st, m, skip_half)) return;
-  const double di = dinv ? (double)dinv[i] : 1.0;
-  x[i] = (V)(w[m] * di * (double)b[i]);
+__global__ void __launch_bounds__(AB) k_amg_jinit(SView<AmgVal<V>> A, const V *__restrict__ dinv, const double *__restrict__ w,
+                                                  const int32_t *__restrict__ mode, const V *__restrict__ b,
+                                                  V *__restrict__ x, const KState *st, int skip_half) {
+  Lane L;
+  if (!amg_lane(A, mode, Amg::MODE_JACOBI, st, skip_half, L)) return;
+  const double di = dinv ? (double)dinv[L.row] : 1.0;
+  x[L.row] = (V)(w[L.d.inst] * di * (double)b[L.row]);
 }
 
 // exact coarsest solve out = Ac^-1 b, one block per inverted member
@@ -503,10 +538,120 @@ void AmgPattern::build(int64_t n, const int32_t *rowptr, const int32_t *colind) 
       for (int64_t I = 0; I < na; I++)
         for (int32_t q = C.arp[I]; q < C.arp[I + 1]; q++)
           if (C.aci[q] == I) C.adp[I] = q;
+      // Galerkin recipe: count the products of every coarse entry, then list them
+      int64_t ntrip = 0;
+      for (int64_t I = 0; I < na; I++)
+        for (int32_t q = L.rrp[I]; q < L.rrp[I + 1]; q++) {
+          const int32_t i = L.rci[q];
+          for (int32_t a = L.arp[i]; a < L.arp[i + 1]; a++) ntrip += L.prp[L.aci[a] + 1] - L.prp[L.aci[a]];
+        }
+      if (ntrip <= Amg::MAX_TRIPLES) {
+        const size_t nz = C.aci.size();
+        L.gptr.assign(nz + 1, 0);
+        auto each = [&](auto &&fn) {
+          for (int64_t I = 0; I < na; I++) {
+            const int32_t c0 = C.arp[I], c1 = C.arp[I + 1];
+            for (int32_t q = L.rrp[I]; q < L.rrp[I + 1]; q++) {
+              const int32_t i = L.rci[q];
+              for (int32_t a = L.arp[i]; a < L.arp[i + 1]; a++) {
+                const int32_t k = L.aci[a];
+                for (int32_t b = L.prp[k]; b < L.prp[k + 1]; b++) {
+                  const int32_t slot =
+                      c0 + (int32_t)(std::lower_bound(C.aci.begin() + c0, C.aci.begin() + c1, L.pci[b]) -
+                                     (C.aci.begin() + c0));
+                  fn(slot, L.r2p[q], a, b);
+                }
+              }
+            }
+          }
+        };
+        each([&](int32_t slot, int32_t, int32_t, int32_t) { L.gptr[slot + 1]++; });
+        for (size_t c = 0; c < nz; c++) L.gptr[c + 1] += L.gptr[c];
+        L.gr.assign((size_t)ntrip, 0);
+        L.ga.assign((size_t)ntrip, 0);
+        L.gp.assign((size_t)ntrip, 0);
+        std::vector<int32_t> fill(L.gptr.begin(), L.gptr.end() - 1);
+        each([&](int32_t slot, int32_t r, int32_t a, int32_t b) {
+          const int32_t t = fill[slot]++;
+          L.gr[t] = r;
+          L.ga[t] = a;
+          L.gp[t] = b;
+        });
+      }
       B = std::move(nrm);
     }
     lv.push_back(std::move(C));
   }
+}
+
+void Amg::build_sell(int l, int kind, SellMat &M, hipStream_t s) {
+  const Lev &L = lev[l];
+  struct Local {
+    std::vector<int32_t> sptr, scol, rbase;
+    int32_t ibase = 0;
+  };
+  std::map<int32_t, Local> loc;  // distinct pattern -> its SELL layout (member-local)
+  std::vector<int32_t> idx;
+  std::vector<SellDesc> desc;
+  const int64_t nrows = kind == 2 ? lev[l + 1].n : L.n;
+  std::vector<int64_t> rbase((size_t)nrows, -1);
+  int64_t vtot = 0;
+  for (int64_t m = 0; m < nsys; m++) {
+    const int hm = L.hmode[m];
+    if (hm == MODE_ABSENT || (kind > 0 && hm != MODE_COARSEN)) continue;
+    const auto &PL = patterns[pid[m]].lv[l];
+    const std::vector<int32_t> &rp = kind == 0 ? PL.arp : (kind == 1 ? PL.prp : PL.rrp);
+    const std::vector<int32_t> &ci = kind == 0 ? PL.aci : (kind == 1 ? PL.pci : PL.rci);
+    const int64_t n = (int64_t)rp.size() - 1;
+    auto it = loc.find(pid[m]);
+    if (it == loc.end()) {
+      Local lc;
+      std::vector<int32_t> segsl;
+      build_sell64(n, rp, ci, {0}, {n}, lc.sptr, lc.scol, segsl, lc.rbase);
+      // padding entries: value 0 at the member's column 0 (a valid column of any kind)
+      for (size_t sl = 0; sl + 1 < lc.sptr.size(); sl++) {
+        const int32_t w = (lc.sptr[sl + 1] - lc.sptr[sl]) / 64;
+        for (int ln = 0; ln < 64; ln++) {
+          const int64_t r = (int64_t)sl * 64 + ln;
+          const int32_t len = r < n ? rp[r + 1] - rp[r] : 0;
+          for (int32_t j = len; j < w; j++) lc.scol[(size_t)lc.sptr[sl] + (size_t)j * 64 + ln] = 0;
+        }
+      }
+      if ((int64_t)idx.size() + (int64_t)lc.scol.size() >= (int64_t(1) << 31))
+        throw Error(VFVM_ERR_ARG, "AMG: patterns too large for 32-bit indices");
+      lc.ibase = (int32_t)idx.size();
+      idx.insert(idx.end(), lc.scol.begin(), lc.scol.end());
+      it = loc.emplace(pid[m], std::move(lc)).first;
+    }
+    const Local &lc = it->second;
+    const int64_t row0 = kind == 2 ? lev[l + 1].roff[m] : L.roff[m];
+    const int64_t coff = kind == 1 ? lev[l + 1].roff[m] : L.roff[m];
+    for (size_t sl = 0; sl + 1 < lc.sptr.size(); sl++) {
+      SellDesc d;
+      d.vbase = vtot + lc.sptr[sl];
+      d.ibase = lc.ibase + lc.sptr[sl];
+      d.coff = (int32_t)coff;
+      d.row0 = (int32_t)(row0 + (int64_t)sl * 64);
+      d.rows = (int32_t)std::min<int64_t>(64, n - (int64_t)sl * 64);
+      d.width = (lc.sptr[sl + 1] - lc.sptr[sl]) / 64;
+      d.inst = (int32_t)m;
+      d.first = sl == 0;
+      d.pad = 0;
+      desc.push_back(d);
+    }
+    for (int64_t li = 0; li < n; li++) rbase[row0 + li] = vtot + lc.rbase[li];
+    vtot += lc.sptr.back();
+  }
+  M.nsl = (int64_t)desc.size();
+  M.nval = vtot;
+  if (desc.empty()) return;
+  M.desc.upload(desc, s);
+  M.idx.upload(idx, s);
+  M.rbase.upload(rbase, s);
+  M.v64.alloc((size_t)std::max<int64_t>(vtot, 1));
+  M.v16.alloc((size_t)std::max<int64_t>(vtot, 1));
+  VFVM_HIP_CHECK(hipMemsetAsync(M.v64.p, 0, sizeof(double) * M.v64.n, s));
+  VFVM_HIP_CHECK(hipMemsetAsync(M.v16.p, 0, sizeof(_Float16) * M.v16.n, s));
 }
 
 void Amg::build(const std::vector<AmgMember> &members, const int32_t *row_inst0, hipStream_t s) {
@@ -543,16 +688,20 @@ void Amg::build(const std::vector<AmgMember> &members, const int32_t *row_inst0,
   for (int l = 0; l < nlev; l++) {
     Lev &L = lev[l];
     L.roff.assign((size_t)nsys, -1);
+    L.aoffm.assign((size_t)nsys, -1);
+    L.poffm.assign((size_t)nsys, -1);
     L.hmode.assign((size_t)nsys, MODE_ABSENT);
     for (int64_t m = 0; m < nsys; m++) {
       const AmgPattern &P = patterns[pid[m]];
       if ((int)P.lv.size() <= l) continue;
       const auto &PL = P.lv[l];
       L.roff[m] = L.n;
+      L.aoffm[m] = L.nnzA;
       L.n += PL.n;
       L.nnzA += PL.arp.back();
       if (l + 1 < (int)P.lv.size()) {
         L.hmode[m] = MODE_COARSEN;
+        L.poffm[m] = L.nnzP;
         L.nnzP += PL.prp.back();
         L.any_coarsen = true;
       } else if (PL.n <= DENSE_MAX) {
@@ -567,6 +716,14 @@ void Amg::build(const std::vector<AmgMember> &members, const int32_t *row_inst0,
       throw Error(VFVM_ERR_ARG, "AMG: level too large for 32-bit indices");
   }
   nnz0 = lev[0].nnzA;
+  {
+    std::vector<int64_t> ms((size_t)2 * nsys);
+    for (int64_t m = 0; m < nsys; m++) {
+      ms[2 * m] = members[m].n;
+      ms[2 * m + 1] = members[m].rowptr[members[m].n];
+    }
+    msize0.upload(ms, s);
+  }
   for (int l = 0; l < nlev; l++) {
     Lev &L = lev[l];
     const bool coarsen = L.any_coarsen;
@@ -649,20 +806,13 @@ void Amg::build(const std::vector<AmgMember> &members, const int32_t *row_inst0,
       L.aci.upload(aci, s);
       L.adp.upload(adp, s);
       L.aval.alloc((size_t)L.nnzA);
-      std::vector<int32_t> sptr, scol, segsl, rbase;
-      build_sell64(L.n, arp, aci, {0}, {L.n}, sptr, scol, segsl, rbase);
-      L.a_rbase.upload(rbase, s);
-      L.a_scol.upload(scol, s);
-      L.a64.alloc(scol.size());
-      L.a32.alloc(scol.size());
-      VFVM_HIP_CHECK(hipMemsetAsync(L.a64.p, 0, sizeof(double) * scol.size(), s));
-      VFVM_HIP_CHECK(hipMemsetAsync(L.a32.p, 0, sizeof(float) * scol.size(), s));
       L.dinv64.alloc((size_t)L.n);
       L.dinv32.alloc((size_t)L.n);
     } else {
       L.inst_p = row_inst0;
       L.aval.alloc((size_t)L.nnzA);
     }
+    build_sell(l, 0, L.A, s);
     {
       std::vector<int32_t> hm(L.hmode.begin(), L.hmode.end());
       L.mode.upload(hm, s);
@@ -686,21 +836,58 @@ void Amg::build(const std::vector<AmgMember> &members, const int32_t *row_inst0,
       L.rrp.upload(rrp, s);
       L.rci.upload(rci, s);
       L.r2p.upload(r2p, s);
-      std::vector<int32_t> sptr, scol, segsl, rbase;
-      build_sell64(L.n, prp, pci, {0}, {L.n}, sptr, scol, segsl, rbase);
-      L.p_rbase.upload(rbase, s);
-      L.p_scol.upload(scol, s);
-      L.p64.alloc(std::max<size_t>(scol.size(), 1));
-      L.p32.alloc(std::max<size_t>(scol.size(), 1));
-      VFVM_HIP_CHECK(hipMemsetAsync(L.p64.p, 0, sizeof(double) * L.p64.n, s));
-      VFVM_HIP_CHECK(hipMemsetAsync(L.p32.p, 0, sizeof(float) * L.p32.n, s));
-      build_sell64(L.nc, rrp, rci, {0}, {L.nc}, sptr, scol, segsl, rbase);
-      L.r_rbase.upload(rbase, s);
-      L.r_scol.upload(scol, s);
-      L.r64.alloc(std::max<size_t>(scol.size(), 1));
-      L.r32.alloc(std::max<size_t>(scol.size(), 1));
-      VFVM_HIP_CHECK(hipMemsetAsync(L.r64.p, 0, sizeof(double) * L.r64.n, s));
-      VFVM_HIP_CHECK(hipMemsetAsync(L.r32.p, 0, sizeof(float) * L.r32.n, s));
+      build_sell(l, 1, L.P, s);
+      build_sell(l, 2, L.R, s);
+    }
+    if (coarsen) {  // Galerkin recipes, one per distinct pattern of the coarsening members
+      std::map<int32_t, std::vector<int64_t>> bypat;
+      for (int64_t m = 0; m < nsys; m++)
+        if (L.hmode[m] == MODE_COARSEN) bypat[pid[m]].push_back(m);
+      L.by_recipe = true;
+      for (const auto &kv : bypat) L.by_recipe &= !patterns[kv.first].lv[l].gptr.empty();
+      if (L.by_recipe)
+        for (const auto &kv : bypat) {
+          const auto &PL = patterns[kv.first].lv[l];
+          Lev::Recipe R;
+          R.nslot = (int64_t)PL.gptr.size() - 1;
+          R.nmem = (int64_t)kv.second.size();
+          {  // interleave the triples of each 64 consecutive entries
+            const int64_t ns = R.nslot, nsl = (ns + 63) / 64;
+            std::vector<int32_t> cnt((size_t)ns), gr, ga, gp;
+            std::vector<int64_t> sb((size_t)nsl + 1, 0);
+            for (int64_t c = 0; c < ns; c++) cnt[c] = PL.gptr[c + 1] - PL.gptr[c];
+            for (int64_t sl = 0; sl < nsl; sl++) {
+              int32_t w = 0;
+              for (int64_t c = sl * 64; c < std::min(ns, sl * 64 + 64); c++) w = std::max(w, cnt[c]);
+              sb[sl + 1] = sb[sl] + (int64_t)w * 64;
+            }
+            gr.assign((size_t)sb[nsl], 0);
+            ga.assign((size_t)sb[nsl], 0);
+            gp.assign((size_t)sb[nsl], 0);
+            for (int64_t c = 0; c < ns; c++)
+              for (int32_t t = 0; t < cnt[c]; t++) {
+                const size_t q = (size_t)(sb[c / 64] + (c % 64) + (int64_t)64 * t);
+                gr[q] = PL.gr[PL.gptr[c] + t];
+                ga[q] = PL.ga[PL.gptr[c] + t];
+                gp[q] = PL.gp[PL.gptr[c] + t];
+              }
+            R.gcnt.upload(cnt, s);
+            R.gsb.upload(sb, s);
+            R.gr.upload(gr, s);
+            R.ga.upload(ga, s);
+            R.gp.upload(gp, s);
+          }
+          std::vector<int64_t> ao, po, co;
+          for (int64_t m : kv.second) {
+            ao.push_back(L.aoffm[m]);
+            po.push_back(L.poffm[m]);
+            co.push_back(lev[l + 1].aoffm[m]);
+          }
+          R.maoff.upload(ao, s);
+          R.mpoff.upload(po, s);
+          R.mcoff.upload(co, s);
+          L.recipes.push_back(std::move(R));
+        }
     }
     L.ndm = (int64_t)dmem.size();
     if (L.ndm) {
@@ -720,6 +907,9 @@ void Amg::setup(const int32_t *rowptr, const int32_t *colind, const int32_t *dof
   if (!built()) throw Error(VFVM_ERR_ARG, "AMG: numeric setup before the symbolic one");
   hipLaunchKernelGGL(k_amg_scale0, dim3(agrid(lev[0].n)), dim3(AB), 0, s, lev[0].n, rowptr, rowscale, J,
                      lev[0].aval.p);
+  hipLaunchKernelGGL(k_amg_to_sell, dim3(agrid(lev[0].n)), dim3(AB), 0, s, lev[0].n, rowptr,
+                     (const int64_t *)lev[0].A.rbase.p, (const double *)lev[0].aval.p, (const int32_t *)nullptr,
+                     (const double *)nullptr, lev[0].A.v64.p, lev[0].A.v16.p);
   for (int l = 0; l < nlev; l++) {
     Lev &L = lev[l];
     const int32_t *arp = l ? L.arp.p : rowptr;
@@ -730,19 +920,30 @@ void Amg::setup(const int32_t *rowptr, const int32_t *colind, const int32_t *dof
                          L.aval.p, L.w.p);
     if (l > 0) {
       hipLaunchKernelGGL(k_amg_dinv, dim3(agrid(L.n)), dim3(AB), 0, s, L.n, adp, L.aval.p, L.dinv64.p, L.dinv32.p);
-      hipLaunchKernelGGL(k_amg_to_sell, dim3(agrid(L.n)), dim3(AB), 0, s, L.n, arp, L.a_rbase.p, L.aval.p,
-                         (const int32_t *)nullptr, L.a64.p, L.a32.p);
+      hipLaunchKernelGGL(k_amg_to_sell, dim3(agrid(L.n)), dim3(AB), 0, s, L.n, arp, (const int64_t *)L.A.rbase.p,
+                         (const double *)L.aval.p, (const int32_t *)nullptr, (const double *)nullptr, L.A.v64.p,
+                         L.A.v16.p);
     }
     if (L.any_coarsen) {
       Lev &C = lev[l + 1];
       hipLaunchKernelGGL(k_amg_prol, dim3(agrid(L.n)), dim3(AB), 0, s, L.n, L.inst_p, L.mode.p, L.w.p, arp, aci, adp,
                          L.aval.p, L.prp.p, L.a2p.p, L.tpos.p, L.tval.p, L.pval.p);
-      hipLaunchKernelGGL(k_amg_rap, dim3(agrid(C.n)), dim3(AB), 0, s, C.n, L.rrp.p, L.rci.p, L.r2p.p, arp, aci,
-                         L.aval.p, L.prp.p, L.pci.p, L.pval.p, C.arp.p, C.aci.p, C.aval.p);
-      hipLaunchKernelGGL(k_amg_to_sell, dim3(agrid(L.n)), dim3(AB), 0, s, L.n, L.prp.p, L.p_rbase.p, L.pval.p,
-                         (const int32_t *)nullptr, L.p64.p, L.p32.p);
-      hipLaunchKernelGGL(k_amg_to_sell, dim3(agrid(C.n)), dim3(AB), 0, s, C.n, L.rrp.p, L.r_rbase.p, L.pval.p,
-                         L.r2p.p, L.r64.p, L.r32.p);
+      if (L.by_recipe) {
+        for (const auto &R : L.recipes)
+          for (int64_t m0 = 0; m0 < R.nmem; m0 += 65535)
+            hipLaunchKernelGGL(k_amg_rap_recipe, dim3(agrid(R.nslot), (unsigned)std::min<int64_t>(R.nmem - m0, 65535)),
+                               dim3(AB), 0, s, R.nslot, R.gcnt.p, R.gsb.p, R.gr.p, R.ga.p, R.gp.p, R.maoff.p,
+                               R.mpoff.p, R.mcoff.p, m0, L.aval.p, L.pval.p, C.aval.p);
+      } else {
+        hipLaunchKernelGGL(k_amg_rap, dim3(agrid(C.n)), dim3(AB), 0, s, C.n, L.rrp.p, L.rci.p, L.r2p.p, arp, aci,
+                           L.aval.p, L.prp.p, L.pci.p, L.pval.p, C.arp.p, C.aci.p, C.aval.p);
+      }
+      hipLaunchKernelGGL(k_amg_to_sell, dim3(agrid(L.n)), dim3(AB), 0, s, L.n, (const int32_t *)L.prp.p,
+                         (const int64_t *)L.P.rbase.p, (const double *)L.pval.p, (const int32_t *)nullptr,
+                         (const double *)nullptr, L.P.v64.p, L.P.v16.p);
+      hipLaunchKernelGGL(k_amg_to_sell, dim3(agrid(C.n)), dim3(AB), 0, s, C.n, (const int32_t *)L.rrp.p,
+                         (const int64_t *)L.R.rbase.p, (const double *)L.pval.p, (const int32_t *)L.r2p.p,
+                         (const double *)nullptr, L.R.v64.p, L.R.v16.p);
     }
     if (L.ndm) {
       const size_t shm = sizeof(double) * (size_t)L.dmax * (size_t)L.dmax;
@@ -754,100 +955,87 @@ void Amg::setup(const int32_t *rowptr, const int32_t *colind, const int32_t *dof
   numeric_ready = true;
 }
 
+void Amg::refresh_fine(const int32_t *rowptr, const double *J, const double *rowscale, bool fp32, hipStream_t s) {
+  if (!built()) throw Error(VFVM_ERR_ARG, "AMG: refresh before the symbolic setup");
+  Lev &L = lev[0];
+  hipLaunchKernelGGL(k_amg_to_sell, dim3(agrid(L.n)), dim3(AB), 0, s, L.n, rowptr, (const int64_t *)L.A.rbase.p, J,
+                     (const int32_t *)nullptr, rowscale, fp32 ? (double *)nullptr : L.A.v64.p, L.A.v16.p);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
 template <class V>
-void Amg::apply(const AmgOp<V> &A0, const V *y, V *z, const KState *st, int skip_half, hipStream_t s) {
+void Amg::apply(const V *y, V *z, const KState *st, int skip_half, hipStream_t s, KernelProbe *probe) {
   if (!numeric_ready) throw Error(VFVM_ERR_ARG, "AMG: apply before the numeric setup");
   constexpr bool F32 = sizeof(V) == 4;
   std::vector<V *> B((size_t)nlev), X((size_t)nlev), R((size_t)nlev);
   for (int l = 0; l < nlev; l++) work<V>(lev[l], B[l], X[l], R[l]);
   auto bof = [&](int l) -> const V * { return l ? B[l] : y; };
   auto sol = [&](int l) -> V * { return l ? R[l] : z; };
-  // A of level l as (rowptr, rbase, columns, values, D^-1) handed to fn
-  auto withA = [&](int l, auto &&fn) {
-    Lev &L = lev[l];
-    if (l == 0) {
-      if (A0.col16)
-        fn(A0.rowptr, A0.rbase, A0.col16, A0.val, (const V *)nullptr);
-      else
-        fn(A0.rowptr, A0.rbase, A0.col32, A0.val, (const V *)nullptr);
-    } else {
-      const V *av, *dv;
-      if constexpr (F32) {
-        av = L.a32.p;
-        dv = L.dinv32.p;
-      } else {
-        av = L.a64.p;
-        dv = L.dinv64.p;
-      }
-      fn((const int32_t *)L.arp.p, (const int32_t *)L.a_rbase.p, (const int32_t *)L.a_scol.p, av, dv);
-    }
+  auto view = [&](const SellMat &M) {
+    SView<AmgVal<V>> v;
+    v.desc = M.desc.p;
+    v.nsl = M.nsl;
+    v.idx = M.idx.p;
+    if constexpr (F32)
+      v.val = M.v16.p;
+    else
+      v.val = M.v64.p;
+    return v;
   };
-  auto pvals = [&](Lev &L) -> const V * {
-    if constexpr (F32) return L.p32.p;
-    else return L.p64.p;
-  };
-  auto rvals = [&](Lev &L) -> const V * {
-    if constexpr (F32) return L.r32.p;
-    else return L.r64.p;
+  auto dinv = [&](int l) -> const V * {  // level 0: the row-scaled operator's unit diagonal
+    if (l == 0) return nullptr;
+    if constexpr (F32) return lev[l].dinv32.p;
+    else return lev[l].dinv64.p;
   };
   auto dinvA = [&](Lev &L) -> const V * {
     if constexpr (F32) return L.dinvA32.p;
     else return L.dinvA64.p;
   };
+  auto grid = [](const SellMat &M) { return (unsigned)std::max<int64_t>(1, (M.nsl + AB / 64 - 1) / (AB / 64)); };
   // down: smoothing from zero, residual, restriction; coarsest solves where members end
   for (int l = 0; l < nlev; l++) {
     Lev &L = lev[l];
-    const unsigned g = agrid(L.n);
+    const SView<AmgVal<V>> A = view(L.A);
     if (L.any_coarsen) {
-      withA(l, [&](const int32_t *rp, const int32_t *rb, const auto *col, const V *val, const V *dinv) {
-        using CI = std::remove_const_t<std::remove_pointer_t<decltype(col)>>;
-        hipLaunchKernelGGL((k_amg_down<V, CI>), dim3(g), dim3(AB), 0, s, L.n, rp, rb, col, val, dinv,
-                           (const double *)L.w.p, L.inst_p, (const int32_t *)L.mode.p, bof(l), X[l], R[l], st,
-                           skip_half);
-      });
-      Lev &C = lev[l + 1];
-      hipLaunchKernelGGL((k_amg_restrict<V>), dim3(agrid(C.n)), dim3(AB), 0, s, C.n, (const int32_t *)L.rrp.p,
-                         (const int32_t *)L.r_rbase.p, (const int32_t *)L.r_scol.p, rvals(L), C.inst_p,
-                         (const V *)R[l], B[l + 1], st, skip_half);
+      hipLaunchKernelGGL((k_amg_down<V>), dim3(grid(L.A)), dim3(AB), 0, s, A, dinv(l), (const double *)L.w.p,
+                         (const int32_t *)L.mode.p, bof(l), R[l], st, skip_half);
+      hipLaunchKernelGGL((k_amg_restrict<V>), dim3(grid(L.R)), dim3(AB), 0, s, view(L.R), (const V *)R[l],
+                         B[l + 1], st, skip_half);
     }
     if (L.ndm)
       hipLaunchKernelGGL((k_amg_dense_apply<V>), dim3((unsigned)L.ndm), dim3(AB), 0, s, (const int32_t *)L.dmem.p,
                          (const int32_t *)L.dsz.p, (const int64_t *)L.drow.p, (const int64_t *)L.doff.p, dinvA(L),
                          bof(l), sol(l), st, skip_half);
     if (L.any_jacobi) {
-      withA(l, [&](const int32_t *rp, const int32_t *rb, const auto *col, const V *val, const V *dinv) {
-        using CI = std::remove_const_t<std::remove_pointer_t<decltype(col)>>;
-        hipLaunchKernelGGL((k_amg_jinit<V>), dim3(g), dim3(AB), 0, s, L.n, dinv, (const double *)L.w.p, L.inst_p,
-                           (const int32_t *)L.mode.p, bof(l), X[l], st, skip_half);
-        V *a = X[l], *o = sol(l);
-        for (int k = 1; k < COARSE_SWEEPS; k++) {  // an odd count of sweeps ends in sol(l)
-          hipLaunchKernelGGL((k_amg_sweep<V, CI>), dim3(g), dim3(AB), 0, s, L.n, rp, rb, col, val, dinv,
-                             (const double *)L.w.p, L.inst_p, (const int32_t *)L.mode.p, (int)MODE_JACOBI, bof(l),
-                             (const V *)a, o, st, skip_half);
-          std::swap(a, o);
-        }
-      });
+      hipLaunchKernelGGL((k_amg_jinit<V>), dim3(grid(L.A)), dim3(AB), 0, s, A, dinv(l), (const double *)L.w.p,
+                         (const int32_t *)L.mode.p, bof(l), X[l], st, skip_half);
+      V *a = X[l], *o = sol(l);
+      for (int k = 1; k < COARSE_SWEEPS; k++) {  // an odd count of sweeps ends in sol(l)
+        hipLaunchKernelGGL((k_amg_sweep<V>), dim3(grid(L.A)), dim3(AB), 0, s, A, dinv(l), (const double *)L.w.p,
+                           (const int32_t *)L.mode.p, (int)MODE_JACOBI, bof(l), (const V *)a, o, st, skip_half,
+                           (unsigned long long *)nullptr, (const int64_t *)nullptr);
+        std::swap(a, o);
+      }
     }
   }
-  // up: prolongation and post-smoothing
+  // up: prolongation (with the pre-smoothing recomputed) and post-smoothing
   for (int l = nlev - 2; l >= 0; l--) {
     Lev &L = lev[l];
     if (!L.any_coarsen) continue;
-    const unsigned g = agrid(L.n);
-    hipLaunchKernelGGL((k_amg_prolong<V>), dim3(g), dim3(AB), 0, s, L.n, (const int32_t *)L.prp.p,
-                       (const int32_t *)L.p_rbase.p, (const int32_t *)L.p_scol.p, pvals(L), L.inst_p,
-                       (const int32_t *)L.mode.p, (const V *)sol(l + 1), X[l], st, skip_half);
-    withA(l, [&](const int32_t *rp, const int32_t *rb, const auto *col, const V *val, const V *dinv) {
-      using CI = std::remove_const_t<std::remove_pointer_t<decltype(col)>>;
-      hipLaunchKernelGGL((k_amg_sweep<V, CI>), dim3(g), dim3(AB), 0, s, L.n, rp, rb, col, val, dinv,
-                         (const double *)L.w.p, L.inst_p, (const int32_t *)L.mode.p, (int)MODE_COARSEN, bof(l),
-                         (const V *)X[l], sol(l), st, skip_half);
-    });
+    hipLaunchKernelGGL((k_amg_prolong<V>), dim3(grid(L.P)), dim3(AB), 0, s, view(L.P), dinv(l), (const double *)L.w.p,
+                       (const int32_t *)L.mode.p, bof(l), (const V *)sol(l + 1), X[l], st, skip_half);
+    const int kid = F32 ? VFVM_KERNEL_AMG32 : VFVM_KERNEL_AMG64;
+    const bool probed = l == 0 && probe && probe->counters(kid);
+    if (probed) probe->before(kid, s);
+    VFVM_LAUNCH((k_amg_sweep<V>), dim3(grid(L.A)), dim3(AB), 0, s, view(L.A), dinv(l), (const double *)L.w.p,
+                (const int32_t *)L.mode.p, (int)MODE_COARSEN, bof(l), (const V *)X[l], sol(l), st, skip_half,
+                probed ? probe->counters(kid) : (unsigned long long *)nullptr, (const int64_t *)msize0.p);
+    if (probed) probe->after(kid, s);
   }
 }
 
-template void Amg::apply<float>(const AmgOp<float> &, const float *, float *, const KState *, int, hipStream_t);
-template void Amg::apply<double>(const AmgOp<double> &, const double *, double *, const KState *, int, hipStream_t);
+template void Amg::apply<float>(const float *, float *, const KState *, int, hipStream_t, KernelProbe *);
+template void Amg::apply<double>(const double *, double *, const KState *, int, hipStream_t, KernelProbe *);
 
 bool Amg::export_level(int l, hipStream_t s, std::vector<int32_t> &arp, std::vector<int32_t> &aci,
                        std::vector<double> &aval, std::vector<int32_t> &prp, std::vector<int32_t> &pci,

@@ -470,8 +470,9 @@ struct vfvm_batch {
     VFVM_HIP_CHECK(hipHostMalloc((void **)&hnorm, sizeof(double) * 2 * nsys, hipHostMallocDefault));
   }
   ~vfvm_batch() {
-    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    // every launch of the cached graphs has finished before they are destroyed
     if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
     for (int i = 0; i < 2; i++) {
       if (hst[i]) (void)hipHostFree(hst[i]);
       if (sev[i]) (void)hipEventDestroy(sev[i]);
@@ -653,21 +654,6 @@ void reset_states(vfvm_batch &b, double rtol, double atol, int maxit) {
   stage_upload(b.st, b.hreset, b.nsys, b.ctx->stream);
 }
 
-// the batch's operator Ahat as the AMG's level-0 operator
-template <class V>
-AmgOp<V> batch_op(vfvm_batch &b) {
-  AmgOp<V> op;
-  op.rowptr = b.rowptr.p;
-  op.rbase = b.sell_rbase.p;
-  op.col32 = b.sell_col.p;
-  op.col16 = (b.sell_off16.p && b.sell16) ? b.sell_off16.p : nullptr;
-  if constexpr (sizeof(V) == 4)
-    op.val = b.A32.p;
-  else
-    op.val = b.A64s.p;
-  return op;
-}
-
 // per-member BiCGStab on bhat -> x: point Jacobi (the identity on the row-scaled
 // operator) or, with amg, right-preconditioned by one AMG V-cycle per application
 template <class V, class VBin>
@@ -689,7 +675,6 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
       ph = b.kp64[0].p; sh = b.kp64[1].p;
     }
   }
-  const AmgOp<V> op = batch_op<V>(b);
   if (amg) b.amg.reserve<V>();
   reset_states(b, rtol, atol, maxit);
   hipLaunchKernelGGL((k_binit<V, VBin>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, bh, r, r0, p, v, x, b.st.p, active);
@@ -703,12 +688,12 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
     for (int q = 0; q < every; q++) {
       if (amg) {
         hipLaunchKernelGGL((k_bp<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, r, p, v, b.st.p);
-        b.amg.apply<V>(op, p, ph, b.st.p, 0, st);
+        b.amg.apply<V>(p, ph, b.st.p, 0, st, &b.ctx->probe);
         bop<V, 1, STAGE_ALPHA>(b, ph, v, r0);
         hipLaunchKernelGGL((k_bs<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, r, v, s, b.st.p);
         hipLaunchKernelGGL((k_bfinalize<1, STAGE_SNORM, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S,
                            b.st.p, (const int32_t *)nullptr, (double *)nullptr);
-        b.amg.apply<V>(op, s, sh, b.st.p, 1, st);
+        b.amg.apply<V>(s, sh, b.st.p, 1, st, &b.ctx->probe);
         bop<V, 2, STAGE_OMEGA>(b, sh, t, s);
         hipLaunchKernelGGL((k_bxr<V>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, x, ph, sh, s, t, r, r0, b.st.p);
         hipLaunchKernelGGL((k_bfinalize<2, STAGE_XR, BF_STAGE>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S,
@@ -812,6 +797,7 @@ void bsolve(vfvm_batch &b, const vfvm_control_t &c, std::vector<int64_t> &nlin, 
       b.amg.build(ms, b.node_inst.p, st);
     }
     if (refactor || !b.amg.ready()) b.amg.setup(b.rowptr.p, b.colind.p, b.dofdiag.p, b.J.p, b.rowscale.p, st);
+    b.amg.refresh_fine(b.rowptr.p, b.J.p, b.rowscale.p, fp32, st);
   }
   // bhat = S F
   bres(b, nullptr, b.F.p, b.bhat.p, b.active.p);

@@ -276,7 +276,7 @@ inline std::pair<hipEvent_t, hipEvent_t> &pending_launch_events() {
 }
 
 struct KernelProbe {
-  static constexpr int NK = 6;
+  static constexpr int NK = 8;
   static constexpr int NW = 4;
   struct Slot {
     bool on = false;

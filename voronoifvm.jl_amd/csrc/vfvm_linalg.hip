@@ -832,6 +832,7 @@ void LinearSolver::prepare(const double *J, const vfvm_control_t &c, bool refact
     // the hierarchy of the first Newton step is reused like an ILU(0) factor
     // (src/vfvm_solvercontrol.jl:123-131); the level-0 operator is always the current one
     if (refactor || !amg.ready()) amg.setup(L.rowptr, L.colind, L.dofdiag, J, rowscale.p, stream);
+    amg.refresh_fine(L.rowptr, J, rowscale.p, false, stream);  // both precisions: FP32 may fall back
   }
   if (c.precon == VFVM_PRECON_ILU0 && (refactor || !ilu_ready)) {
     if (!L.ilu) throw Error(VFVM_ERR_ARG, "ILU(0) needs the multicolour structure (build_ilu)");
@@ -908,16 +909,7 @@ const V *LinearSolver::precond(const V *r, V *z, KState *stp, int skip_if_half, 
     return z;
   }
   if (c.precon == VFVM_PRECON_AMG) {
-    AmgOp<V> op;
-    op.rowptr = L.rowptr;
-    op.rbase = L.sell_rbase;
-    op.col32 = L.sell_col;
-    op.col16 = (L.sell_off16 && sell16) ? L.sell_off16 : nullptr;
-    if constexpr (sizeof(V) == 4)
-      op.val = A32.p;
-    else
-      op.val = A64s.p;
-    amg.apply<V>(op, r, z, stp, skip_if_half, stream);
+    amg.apply<V>(r, z, stp, skip_if_half, stream, L.probe);
     return z;
   }
   // point Jacobi on the row-scaled operator is the identity

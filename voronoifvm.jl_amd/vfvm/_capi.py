@@ -257,7 +257,7 @@ class Context:
         return _Probe(self, kernels, max_launches)
 
 
-KERNEL_ASSEMBLE, KERNEL_RESIDUAL, KERNEL_SPMV64, KERNEL_SPMV32, KERNEL_HALFSTEP64, KERNEL_HALFSTEP32 = range(6)
+KERNEL_ASSEMBLE, KERNEL_RESIDUAL, KERNEL_SPMV64, KERNEL_SPMV32, KERNEL_HALFSTEP64, KERNEL_HALFSTEP32, KERNEL_AMG64, KERNEL_AMG32 = range(8)
 
 
 class _Probe:
