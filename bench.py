@@ -71,8 +71,11 @@ def probe_bytes(kind, probe, sv=4, nspec=1, transient=True):
     w, n = probe["work"], max(probe["launches"], 1)
     if kind == "spmv":
         return spmv_bytes(w[0] / n, w[1] / n, sv)
-    if kind == "amg_sweep":  # out = x + omega (b - A x): an SpMV's bytes and b once more
-        return spmv_bytes(w[0] / n, w[1] / n, sv) + sv * w[0] / n
+    if kind == "amg_sweep":
+        # out = x + omega (b - A x) on the AMG's own SELL copy of the operator: FP16 values,
+        # column indices shared by the members of one pattern (read from the L2, not
+        # counted), b and x read and out written in the Krylov precision
+        return 2 * w[1] / n + 3 * sv * w[0] / n
     return assembly_bytes(nspec, w[0] / n, w[1] / n, w[2] / n, transient)
 
 
@@ -592,7 +595,23 @@ def run_cfg4(args):
     log(f"surrogate+batched+FP32: {nsol} solves in {t_sur:.3f}s -> {sur_rate:.1f} solves/s, "
         f"mean newton {np.mean(it_sur):.2f}; batched+FP32 from zero {zero_rate:.1f} solves/s, mean newton "
         f"{np.mean(it_zero):.2f}; surrogate max-norm error mean {np.mean(sur_err):.3e} max {np.max(sur_err):.3e}")
-    # plain FP64 GPU Newton from zero, one system at a time, same inputs
+    plain_rate = cont_rate = None
+    diff = cdiff = float("nan")
+    it_plain, st_cont, it_cont = [float("nan")], [float("nan")], [float("nan")]
+    ncont = 0
+    if not args.batched_only:
+        plain_rate, it_plain, diff = cfg4_plain(args, S, fam, blocks, K, converged)
+        cont_rate, ncont, st_cont, it_cont, cdiff = cfg4_continuation(S, fam, blocks, nsol, converged)
+    # roofline: batched FP32 SpMV (the Krylov hot op) and the Jacobian assembly, timed live
+    # inside the surrogate solves (work counted over the members still active), and the
+    # network's layer GEMM
+    return cfg4_result(args, S, fam, net, meta, probes, B, K, nb, nsol, t_sur, sur_rate, zero_rate, nfill, t_zero64,
+                       t_sur_host, it_sur, it_zero, sur_err, worst, plain_rate, it_plain, diff, cont_rate, ncont,
+                       st_cont, it_cont, cdiff)
+
+
+def cfg4_plain(args, S, fam, blocks, K, converged):
+    """plain FP64 GPU Newton from zero, one system at a time, on every seed the timed steps solved"""
     single = S.make_system(fam.grid, np.zeros(fam.grid.num_nodes), np.zeros(4))
     ctrl64 = vfvm.SolverControl(precision="fp64", reltol_linear=RELTOL_LINEAR, abstol_linear=0.0, maxiters_linear=3000,
                                 check_every=8)
@@ -617,8 +636,12 @@ def run_cfg4(args):
     plain_rate = k / t_plain
     log(f"plain FP64 GPU Newton from zero: {k} solves in {t_plain:.2f}s -> {plain_rate:.1f} solves/s, "
         f"mean newton {np.mean(it_plain):.2f}; max |U_sur - U_plain| = {diff:.2e}")
-    # continuation on a sample: source and side values ramped by the embedding parameter
-    # over [0, 1] (solve(...; embed)), plain FP64 GPU Newton at every accepted step
+    return plain_rate, it_plain, diff
+
+
+def cfg4_continuation(S, fam, blocks, nsol, converged):
+    """continuation on a sample: source and side values ramped by the embedding parameter
+    over [0, 1] (solve(...; embed)), plain FP64 GPU Newton at every accepted step"""
     ncont = min(32, nsol)
     from vfvm.physics import Physics, LinearFlux, SinhReaction, Dirichlet
     cph = Physics(flux=LinearFlux(S.EPS), reaction=SinhReaction(), source=np.zeros((1, fam.grid.num_nodes), order="F"),
@@ -645,9 +668,12 @@ def run_cfg4(args):
     cont_rate = ncont / t_cont
     log(f"continuation (plain FP64 GPU, {ncont} seeds): {cont_rate:.1f} solves/s, mean accepted steps "
         f"{np.mean(st_cont):.1f}, mean newton {np.mean(it_cont):.1f}; max |U_cont - U_sur| = {cdiff:.2e}")
-    # roofline: batched FP32 SpMV (the Krylov hot op) and the Jacobian assembly, timed live
-    # inside the surrogate solves (work counted over the members still active), and the
-    # network's layer GEMM
+    return cont_rate, ncont, st_cont, it_cont, cdiff
+
+
+def cfg4_result(args, S, fam, net, meta, probes, B, K, nb, nsol, t_sur, sur_rate, zero_rate, nfill, t_zero64,
+                t_sur_host, it_sur, it_zero, sur_err, worst, plain_rate, it_plain, diff, cont_rate, ncont, st_cont,
+                it_cont, cdiff):
     def merged(lst):
         return dict(total_ms=sum(x["total_ms"] for x in lst), launches=sum(x["launches"] for x in lst),
                     avg_ms=sum(x["total_ms"] for x in lst) / max(sum(x["launches"] for x in lst), 1),
@@ -662,7 +688,7 @@ def run_cfg4(args):
     rl_amg = roofline_hbm(probe_bytes("amg_sweep", pamg, sv=4), pamg, read_traffic("cfg4_amg_sweep32"),
                           "AMG finest-level sweep (FP32)")
     rl_amg["kernel"] = (f"AMG V-cycle finest-level Jacobi sweep over the active members of {B} x 129^2 "
-                        f"(k_amg_sweep, FP32 SELL-64, shared column indices)")
+                        f"(k_amg_sweep: FP16 SELL-64 values, shared column indices, FP32 vectors)")
     rl_asm = roofline_hbm(probe_bytes("assemble", pasm, transient=False), pasm, read_traffic("cfg4_assemble"),
                           "batched assembly")
     import ctypes as C
@@ -676,7 +702,8 @@ def run_cfg4(args):
     log(f"inference layer (aggregation + 128x64 MFMA GEMM) over {rows} rows: {t_layer * 1e3:.1f} us, "
         f"{layer_gbs:.0f} GB/s, {layer_tflops:.1f} TFLOP/s")
     return dict(value=sur_rate, ms_per_step=1e3 * t_sur / K, steps=K, plain_gpu_newton_solves_per_s=plain_rate,
-                speedup_vs_plain=sur_rate / plain_rate, batched_fp32_from_zero_solves_per_s=zero_rate,
+                speedup_vs_plain=(sur_rate / plain_rate) if plain_rate else None,
+                batched_fp32_from_zero_solves_per_s=zero_rate,
                 batched_fp64_from_zero_solves_per_s=nfill / t_zero64,
                 host_buffer_path_solves_per_s=nfill / t_sur_host,
                 continuation_plain_solves_per_s=cont_rate,
@@ -881,6 +908,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg4", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="cfg4: skip the plain one-at-a-time and continuation legs (profiling runs)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="batched Krylov blocks launched kernel by kernel instead of as HIP graphs "
                          "(for kernel-trace profiling)")

@@ -419,14 +419,21 @@ typedef struct {
   double d, d_min, d_max, d_grow, d_decrease;
   double du_opt, du_max_factor;
   int32_t num_final_steps, force_first_step;
+  /* SolverControl.handle_exceptions (src/vfvm_solvercontrol.jl:205-216): 1 retries a
+   * failing Newton step with a smaller step, and returns the steps done so far, instead of
+   * an error, when the smallest step fails (src/vfvm_solver.jl:391-452); default 0 */
+  int32_t handle_exceptions, pad;
 } vfvm_stepcontrol_t;
 void vfvm_default_stepcontrol(int transient, vfvm_stepcontrol_t *sc);
 
 /* solve_transient! with its step-size control, on the device Newton loop: Δu =
  * ||U - Uold||_inf; a step with Δu > du_max_factor * du_opt is retried with the step
  * times d_decrease down to d_min, where the call fails (VFVM_ERR_CONVERGENCE) unless the
- * first step is forced; a failing Newton solve fails the call (handle_exceptions =
- * false).  Accepted steps grow as min(d_max, d d_grow, d du_opt / Δu, ...) with the
+ * first step is forced; a failing Newton solve fails the call.  With handle_exceptions a
+ * failing Newton solve is retried like a rejected step, and where d_min fails too (or a
+ * first step that is forced fails its Newton solve) the call stops and returns VFVM_OK
+ * with the steps done so far and hist->converged = 0.  Accepted steps grow as
+ * min(d_max, d d_grow, d du_opt / Δu, ...) with the
  * num_final_steps rule.  Outputs: U_final; up to max_out entries (lambda, solution) in
  * out_lambdas / out_U (max_out x ndof; both nullable): entry 0 is the start (the initial
  * value for time stepping, the solution at lambdas[0] for embedding), then one entry per

@@ -44,7 +44,7 @@ class Control(C.Structure):
                 ("dp", C.c_double), ("dp_min", C.c_double), ("dp_max", C.c_double),
                 ("dp_grow", C.c_double), ("dp_decrease", C.c_double), ("du_opt", C.c_double),
                 ("du_max_factor", C.c_double), ("num_final_steps", C.c_int32),
-                ("force_first_step", C.c_int32)]
+                ("force_first_step", C.c_int32), ("handle_exceptions", C.c_int32), ("pad2", C.c_int32)]
 
 
 class BPhysics(C.Structure):

@@ -1687,7 +1687,7 @@ int orc_solve_step(orc_sys *s, double *U, const double *Uold, const orc_control 
   return 0;
 }
 
-/* solve_transient!, src/vfvm_solver.jl:267-537 (handle_exceptions = false) */
+/* solve_transient!, src/vfvm_solver.jl:267-537 */
 int orc_solve_transient(orc_sys *s, const double *inival, int nl, const double *lambdas, int transient,
                         const orc_control *c, double time, double *U_final, int *nsteps, int *newton_total,
                         double *steps_out, int max_steps) {
@@ -1726,9 +1726,12 @@ int orc_solve_transient(orc_sys *s, const double *inival, int nl, const double *
       double du = 0.0;
       /* src/vfvm_solver.jl:359-455: try the step; a step whose update exceeds
          Δu_max_factor Δu_opt is retried with Δλ Δλ_decrease until Δλ ≈ Δλ_min, where
-         the run stops (handle_exceptions = false) unless the first step is forced.
-         A Newton failure stops the run (handle_exceptions = false throws). Every
-         retry restarts Newton from the old solution (solve_step! copies oldsol). */
+         the run stops unless the first step is forced.  A Newton failure stops the run
+         unless handle_exceptions, which retries it like a rejected step; with
+         handle_exceptions, giving up at Δλ_min (or a forced first step whose Newton
+         solve fails) returns the steps done so far.  Every retry restarts Newton from
+         the old solution (solve_step! copies oldsol). */
+      int stop = 0;
       for (;;) {
         l = l0 + dl;
         double t_, ts_, ep_;
@@ -1743,19 +1746,34 @@ int orc_solve_transient(orc_sys *s, const double *inival, int nl, const double *
         }
         rc = orc_solve_step(s, tmp, old, c, t_, ts_, ep_, &h);
         nnewton += h.niter;
-        if (rc) goto done;
-        memcpy(sol, tmp, sizeof(double) * (size_t)ndof);
-        du = 0.0;
-        for (int k = 0; k < ndof; k++) du = fmax(du, fabs(sol[k] - old[k]));
-        if (!(du > c->du_max_factor * c->du_opt)) break;
+        int errored = 0;
+        if (rc) {
+          if (!c->handle_exceptions) goto done;
+          errored = 1;
+          rc = 0;
+        } else {
+          memcpy(sol, tmp, sizeof(double) * (size_t)ndof);
+          du = 0.0;
+          for (int k = 0; k < ndof; k++) du = fmax(du, fabs(sol[k] - old[k]));
+          if (!(du > c->du_max_factor * c->du_opt)) break;
+        }
         /* Δλ ≈ Δλ_min (isapprox, rtol sqrt(eps)) */
         if (fabs(dl - dl_min) <= 1.4901161193847656e-08 * fmax(fabs(dl), fabs(dl_min))) {
-          if (c->force_first_step && istep == 0) break;
+          if (c->force_first_step && istep == 0 && !errored) break;
+          if (c->handle_exceptions) {
+            stop = 1;
+            break;
+          }
           set_err(s, "step rejected at minimal step size (du=%g)", du);
           rc = ORC_ERR_CONVERGENCE;
           goto done;
         }
         dl = fmax(dl_min, dl * (transient ? c->dt_decrease : c->dp_decrease));
+      }
+      if (stop) { /* the last accepted solution (the start, Dirichlet values set, if none) */
+        memcpy(sol, old, sizeof(double) * (size_t)ndof);
+        if (transient && istep == 0) orc_initialize_dirichlet(s, sol, time, lambdas[0]);
+        goto done;
       }
       if (steps_out && istep < max_steps) steps_out[istep] = l;
       istep++;

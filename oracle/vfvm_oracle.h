@@ -83,6 +83,9 @@ typedef struct {
   double dp, dp_min, dp_max, dp_grow, dp_decrease;
   double du_opt, du_max_factor;
   int32_t num_final_steps, force_first_step;
+  /* src/vfvm_solvercontrol.jl:205-216: failing Newton steps are retried with smaller
+     steps; where the smallest fails the run returns the steps done so far */
+  int32_t handle_exceptions, pad2;
 } orc_control;
 
 typedef struct {
@@ -156,7 +159,8 @@ int orc_solve_step(orc_sys *s, double *U, const double *Uold, const orc_control 
 
 /* a4: solve_transient!  transient=1: lambdas are times; 0: embedding.
    Final solution in U_final; number of accepted steps in *nsteps; the accepted
-   times / parameters in steps_out (up to max_steps; may be NULL). */
+   times / parameters in steps_out (up to max_steps; may be NULL).  With
+   handle_exceptions a run that gives up returns 0 with the steps done so far. */
 int orc_solve_transient(orc_sys *s, const double *inival, int nlambdas, const double *lambdas,
                         int transient, const orc_control *c, double time, double *U_final,
                         int *nsteps, int *newton_total, double *steps_out, int max_steps);

@@ -298,6 +298,40 @@ def test_adaptive_time_stepping_matches_oracle():
         assert np.abs(vfvm.dofs(tsol.u[-1]) - Uo).max() <= SOL_ATOL
 
 
+def test_handle_exceptions_matches_oracle():
+    """SolverControl.handle_exceptions (src/vfvm_solver.jl:391-452) on the device driver:
+    with five Newton iterations the large steps of u_t = div((1e-2 + u^2) grad u) fail and
+    are retried with half the step; the accepted times and solutions equal the oracle's.
+    Without it the same run raises; a run whose smallest step fails returns its start
+    with a warning."""
+    _print("case handle_exceptions")
+    X = linspace(0.0, 1.0, 17)
+    g = simplexgrid(X, X)
+    x, y = g.coord
+    u0 = np.exp(-30.0 * ((x - 0.5) ** 2 + (y - 0.5) ** 2))
+    bcs = [vfvm.Dirichlet(species=1, region=r, value=0.0) for r in (1, 2, 3, 4)]
+    sys, osys = cases.pair(g, 1, dict(flux=vfvm.NonlinearDiffusionFlux(1e-2, 1.0), storage=vfvm.IdentityStorage()),
+                           bcs)
+    kw = dict(precon="ilu0", reltol_linear=1e-12, abstol_linear=0.0, maxiters_linear=2000, maxiters=5,
+              Δt=0.5, Δt_min=1e-4, Δt_max=0.5, Δu_opt=10.0)
+    with pytest.raises(vfvm.ConvergenceError):
+        vfvm.solve(sys, inival=u0.reshape(1, -1), control=vfvm.SolverControl(**kw), times=[0.0, 1.0])
+    tsol = vfvm.solve(sys, inival=u0.reshape(1, -1), control=vfvm.SolverControl(handle_exceptions=True, **kw),
+                      times=[0.0, 1.0])
+    oc = O.control(dt=0.5, dt_min=1e-4, dt_max=0.5, du_opt=10.0, maxiters=5, handle_exceptions=1)
+    Uo, n, _, steps = osys.solve_transient(u0, [0.0, 1.0], oc, return_steps=True)
+    _print("handle_exceptions steps", n, "gpu", len(tsol.t) - 1, "first", tsol.t[1])
+    assert len(tsol.t) == n + 1
+    assert np.abs(np.asarray(tsol.t[1:]) - steps).max() <= 1e-12
+    assert np.abs(vfvm.dofs(tsol.u[-1]) - Uo).max() <= SOL_ATOL
+    kw.update(maxiters=3, Δt_min=0.1)
+    with pytest.warns(UserWarning, match="smallest step"):
+        tsol = vfvm.solve(sys, inival=u0.reshape(1, -1), control=vfvm.SolverControl(handle_exceptions=True, **kw),
+                          times=[0.0, 1.0])
+    assert len(tsol.t) == 1 and tsol.t[0] == 0.0
+    assert np.array_equal(vfvm.dofs(tsol.u[0]), u0)  # the start entry: the initial value as given
+
+
 def test_continuation_with_embedded_source_matches_oracle():
     """Continuation (solve(...; embed)) on the nonlinear Poisson problem with source and
     Dirichlet values ramped by the embedding parameter: assembly at lambda = 0.37 to

@@ -15,6 +15,7 @@ def test_solver_control_defaults_follow_reference():
     assert (c.Δt, c.Δt_min, c.Δt_max, c.Δt_grow, c.Δt_decrease) == (0.1, 1e-3, 1.0, 1.2, 0.5)
     assert (c.Δp, c.Δp_min, c.Δp_max, c.Δp_grow, c.Δp_decrease) == (1.0, 1e-3, 1.0, 1.0, 0.5)
     assert (c.Δu_opt, c.Δu_max_factor, c.num_final_steps, c.force_first_step) == (0.1, 1.2, 5, False)
+    assert c.handle_exceptions is False  # src/vfvm_solvercontrol.jl:216
 
 
 def test_step_control_fields():
@@ -24,6 +25,15 @@ def test_step_control_fields():
     assert (t.d, t.d_min, t.d_max, t.d_grow) == (0.1, 1e-3, 1.0, 1.2)
     assert (e.d, e.d_min, e.d_max, e.d_grow) == (0.05, 0.01, 0.2, 1.0)
     assert e.du_opt == 0.1 and e.num_final_steps == 5
+    assert t.handle_exceptions == 0
+    assert vfvm.SolverControl(handle_exceptions=True).to_step_c(True).handle_exceptions == 1
+    # the library's defaults agree with the mirror's
+    from vfvm import _capi
+    for tr in (1, 0):
+        d = _capi.StepControlT()
+        _capi.lib().vfvm_default_stepcontrol(tr, d)
+        m = vfvm.SolverControl().to_step_c(bool(tr))
+        assert [getattr(d, f) for f, _ in d._fields_] == [getattr(m, f) for f, _ in m._fields_]
 
 
 def test_fixed_timesteps():

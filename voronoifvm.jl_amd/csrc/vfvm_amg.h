@@ -66,6 +66,7 @@ struct SellMat {
   DevBuf<SellDesc> desc;
   DevBuf<int32_t> idx;
   DevBuf<int64_t> rbase;
+  DevBuf<int32_t> msl0, mnsl;  // per member: its first slice and slice count (0: none)
   DevBuf<double> v64;
   DevBuf<_Float16> v16;
 };
@@ -102,6 +103,9 @@ class Amg {
   static constexpr int64_t MAX_COARSE = 64;  // coarsening stops at or below this many rows
   static constexpr int64_t DENSE_MAX = 88;   // coarsest levels up to this size are inverted
   static constexpr int COARSE_SWEEPS = 4;    // damped Jacobi sweeps on a larger coarsest level
+  // levels on which every member has at most this many rows are cycled by one workgroup
+  // per member in a single launch (the coarse tail)
+  static constexpr int64_t TAIL_ROWS = 512;
   // patterns whose Galerkin recipe exceeds this many triples form P^T A P by search instead
   static constexpr int64_t MAX_TRIPLES = int64_t(1) << 24;
   enum { MODE_COARSEN = 0, MODE_DENSE = 1, MODE_JACOBI = 2, MODE_ABSENT = -1 };
@@ -133,6 +137,7 @@ class Amg {
       V *b, *x, *r;
       work<V>(L, b, x, r);
     }
+    build_tail(sizeof(V) == 4);
   }
 
   int levels() const { return nlev; }
@@ -179,7 +184,9 @@ class Amg {
     };
     std::vector<Recipe> recipes;
     bool by_recipe = false;  // every coarsening member has a recipe
-    // exactly inverted members: member, first row, size, inverse offset
+    // exactly inverted members: member, first row, size, inverse offset; per member its
+    // position in that list (-1: none)
+    DevBuf<int32_t> didx;
     int64_t ndm = 0, dmax = 0;
     DevBuf<int32_t> dmem, dsz;
     DevBuf<int64_t> drow, doff;
@@ -215,6 +222,12 @@ class Amg {
   std::vector<int32_t> pid;  // member -> pattern
   std::vector<Lev> lev;
   int64_t nnz0 = 0;
+  // coarse tail: first level, its members, and the per-level device tables per precision
+  int lt = 0;
+  int64_t ntail = 0;
+  DevBuf<int32_t> tail_mem;
+  DevBuf<unsigned char> tail_tab[2];  // FP32 and FP64 apply
+  void build_tail(bool f32);
   DevBuf<int64_t> msize0;  // per member: level-0 rows and nonzeros (probe work)
   // SELL-64 form of one kind of matrix (0 A, 1 P, 2 P^T) of level l over its members
   void build_sell(int l, int kind, SellMat &M, hipStream_t s);

@@ -252,9 +252,13 @@ __global__ void __launch_bounds__(AB) k_amg_dense_inv(const int32_t *__restrict_
 
 // ---------------------------------------------------------------- V-cycle (precision V)
 
-// A SELL matrix as the cycle's kernels see it (values of type V).  Blocks of AB threads
-// cover AB / 64 slices, one wave per slice; a lane past the slice's rows, or in a member
-// that is not in `want` mode or whose Krylov state says to skip, does nothing.
+// A SELL matrix as the cycle's kernels see it (values of type V).  Each wave takes SPW
+// consecutive slices, one lane per row of each: the index and value loads of all its rows
+// are issued before their gathers, so a wave keeps SPW rows' loads in flight and a launch
+// needs 1 / SPW of the waves.  The slice index is wave-uniform, so the descriptors, the
+// members' modes and their Krylov states come in as scalar loads.  A row passes when its
+// lane is past the slice's rows, or its member is not in `want` mode or its Krylov state
+// says to skip.
 template <class V>
 struct SView {
   const SellDesc *desc;
@@ -266,58 +270,98 @@ struct SView {
 template <class V>
 using AmgVal = std::conditional_t<sizeof(V) == 4, _Float16, double>;
 
+// one slice per wave and one entry per round: the form that measured fastest (more
+// slices or entries per wave raise the registers and cut occupancy by more than they gain)
+constexpr int SPW = 1;     // slices per wave
+constexpr int UNROLL = 1;  // row entries per round of loads
+
 struct Lane {
   SellDesc d;
-  int lane;
   int64_t row;
+  bool on;
 };
 
-template <class V>
-__device__ __forceinline__ bool amg_lane(const SView<V> &M, const int32_t *mode, int want, const KState *st,
-                                         int skip_half, Lane &L) {
-  const int64_t sl = (int64_t)blockIdx.x * (AB / 64) + (threadIdx.x >> 6);
-  if (sl >= M.nsl) return false;
-  L.d = M.desc[sl];
-  L.lane = threadIdx.x & 63;
-  if (L.lane >= L.d.rows) return false;
-  if (mode && mode[L.d.inst] != want) return false;
-  if (amg_skip(st, L.d.inst, skip_half)) return false;
-  L.row = (int64_t)L.d.row0 + L.lane;
-  return true;
+// the wave's slices and which of its rows take part; false when none does
+template <class VA>
+__device__ __forceinline__ bool amg_lanes(const SView<VA> &M, const int32_t *mode, int want, const KState *st,
+                                          int skip_half, Lane (&L)[SPW]) {
+  const int lane = threadIdx.x & 63;
+  const int32_t sl0 = __builtin_amdgcn_readfirstlane(((int32_t)blockIdx.x * (AB / 64) + (int32_t)(threadIdx.x >> 6)) * SPW);
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < SPW; k++) {
+    L[k].on = false;
+    L[k].row = 0;
+    if (sl0 + k >= M.nsl) continue;
+    L[k].d = M.desc[sl0 + k];
+    const int m = L[k].d.inst;
+    if ((mode && mode[m] != want) || amg_skip(st, m, skip_half)) continue;
+    L[k].on = lane < L[k].d.rows;
+    L[k].row = (int64_t)L[k].d.row0 + lane;
+    any = true;
+  }
+  return any;
 }
 
-// sum_j M[row, j] f(column)
+// acc[k] = sum_j M[row_k, j] f(k, column) for the wave's rows
 template <class V, class F>
-__device__ __forceinline__ double amg_row(const SView<V> &M, const Lane &L, F &&f) {
-  double acc = 0.0;
-  const int64_t v0 = L.d.vbase + L.lane;
-  const int64_t i0 = (int64_t)L.d.ibase + L.lane;
-  for (int32_t j = 0; j < L.d.width; j++)
-    acc += (double)M.val[v0 + (int64_t)64 * j] * f((int64_t)L.d.coff + M.idx[i0 + (int64_t)64 * j]);
-  return acc;
+__device__ __forceinline__ void amg_rows(const SView<V> &M, const Lane (&L)[SPW], F &&f, double (&acc)[SPW]) {
+  const int lane = threadIdx.x & 63;
+  int32_t w = 0;
+#pragma unroll
+  for (int k = 0; k < SPW; k++) {
+    acc[k] = 0.0;
+    if (L[k].on) w = max(w, L[k].d.width);
+  }
+  for (int32_t j0 = 0; j0 < w; j0 += UNROLL) {
+    int32_t c[SPW][UNROLL];
+    V a[SPW][UNROLL];
+#pragma unroll
+    for (int k = 0; k < SPW; k++)
+#pragma unroll
+      for (int u = 0; u < UNROLL; u++) {
+        const bool ok = L[k].on && j0 + u < L[k].d.width;
+        const int64_t off = (int64_t)lane + (int64_t)64 * (j0 + u);
+        c[k][u] = ok ? M.idx[(int64_t)L[k].d.ibase + off] : 0;
+        a[k][u] = ok ? M.val[L[k].d.vbase + off] : (V)0;
+      }
+#pragma unroll
+    for (int k = 0; k < SPW; k++)
+#pragma unroll
+      for (int u = 0; u < UNROLL; u++)
+        if (L[k].on && j0 + u < L[k].d.width) acc[k] += (double)a[k][u] * f(k, (int64_t)L[k].d.coff + c[k][u]);
+  }
 }
 
 // pre-smoothing from zero and the residual on the rows of coarsening members:
 // r = b - A x with x = omega D^-1 b (x itself is not stored: the prolongation recomputes it)
 template <class V>
-__global__ void __launch_bounds__(AB) k_amg_down(SView<AmgVal<V>> A, const V *__restrict__ dinv, const double *__restrict__ w,
-                                                 const int32_t *__restrict__ mode, const V *__restrict__ b,
-                                                 V *__restrict__ r, const KState *st, int skip_half) {
-  Lane L;
-  if (!amg_lane(A, mode, Amg::MODE_COARSEN, st, skip_half, L)) return;
-  const double wm = w[L.d.inst];
-  const double acc =
-      amg_row(A, L, [&](int64_t k) { return wm * (dinv ? (double)dinv[k] : 1.0) * (double)b[k]; });
-  r[L.row] = (V)((double)b[L.row] - acc);
+__global__ void __launch_bounds__(AB) k_amg_down(SView<AmgVal<V>> A, const V *__restrict__ dinv,
+                                                 const double *__restrict__ w, const int32_t *__restrict__ mode,
+                                                 const V *__restrict__ b, V *__restrict__ r, const KState *st,
+                                                 int skip_half) {
+  Lane L[SPW];
+  if (!amg_lanes(A, mode, Amg::MODE_COARSEN, st, skip_half, L)) return;
+  double wm[SPW], acc[SPW];
+#pragma unroll
+  for (int k = 0; k < SPW; k++) wm[k] = L[k].on ? w[L[k].d.inst] : 0.0;
+  amg_rows(A, L, [&](int k, int64_t c) { return wm[k] * (dinv ? (double)dinv[c] : 1.0) * (double)b[c]; }, acc);
+#pragma unroll
+  for (int k = 0; k < SPW; k++)
+    if (L[k].on) r[L[k].row] = (V)((double)b[L[k].row] - acc[k]);
 }
 
 // b_coarse = P^T r (every row of the next level belongs to a coarsening member)
 template <class V>
-__global__ void __launch_bounds__(AB) k_amg_restrict(SView<AmgVal<V>> R, const V *__restrict__ r, V *__restrict__ bc,
-                                                     const KState *st, int skip_half) {
-  Lane L;
-  if (!amg_lane(R, (const int32_t *)nullptr, 0, st, skip_half, L)) return;
-  bc[L.row] = (V)amg_row(R, L, [&](int64_t k) { return (double)r[k]; });
+__global__ void __launch_bounds__(AB) k_amg_restrict(SView<AmgVal<V>> R, const V *__restrict__ r,
+                                                     V *__restrict__ bc, const KState *st, int skip_half) {
+  Lane L[SPW];
+  if (!amg_lanes(R, (const int32_t *)nullptr, 0, st, skip_half, L)) return;
+  double acc[SPW];
+  amg_rows(R, L, [&](int, int64_t c) { return (double)r[c]; }, acc);
+#pragma unroll
+  for (int k = 0; k < SPW; k++)
+    if (L[k].on) bc[L[k].row] = (V)acc[k];
 }
 
 // x = omega D^-1 b + P x_coarse on the rows of coarsening members
@@ -326,11 +370,16 @@ __global__ void __launch_bounds__(AB) k_amg_prolong(SView<AmgVal<V>> P, const V 
                                                     const double *__restrict__ w, const int32_t *__restrict__ mode,
                                                     const V *__restrict__ b, const V *__restrict__ xc,
                                                     V *__restrict__ x, const KState *st, int skip_half) {
-  Lane L;
-  if (!amg_lane(P, mode, Amg::MODE_COARSEN, st, skip_half, L)) return;
-  const double di = dinv ? (double)dinv[L.row] : 1.0;
-  const double acc = amg_row(P, L, [&](int64_t k) { return (double)xc[k]; });
-  x[L.row] = (V)(w[L.d.inst] * di * (double)b[L.row] + acc);
+  Lane L[SPW];
+  if (!amg_lanes(P, mode, Amg::MODE_COARSEN, st, skip_half, L)) return;
+  double acc[SPW];
+  amg_rows(P, L, [&](int, int64_t c) { return (double)xc[c]; }, acc);
+#pragma unroll
+  for (int k = 0; k < SPW; k++)
+    if (L[k].on) {
+      const double di = dinv ? (double)dinv[L[k].row] : 1.0;
+      x[L[k].row] = (V)(w[L[k].d.inst] * di * (double)b[L[k].row] + acc[k]);
+    }
 }
 
 // one damped Jacobi sweep out = x + omega D^-1 (b - A x) on the rows of members in `want`
@@ -338,31 +387,153 @@ __global__ void __launch_bounds__(AB) k_amg_prolong(SView<AmgVal<V>> P, const V 
 // (probe counters, nullable): the first slice of each member it runs on adds the member's
 // rows and nonzeros (msize), one atomic pair per member and launch.
 template <class V>
-__global__ void __launch_bounds__(AB) k_amg_sweep(SView<AmgVal<V>> A, const V *__restrict__ dinv, const double *__restrict__ w,
-                                                  const int32_t *__restrict__ mode, int want, const V *__restrict__ b,
-                                                  const V *__restrict__ x, V *__restrict__ out, const KState *st,
-                                                  int skip_half, unsigned long long *dwork,
-                                                  const int64_t *__restrict__ msize) {
-  Lane L;
-  if (!amg_lane(A, mode, want, st, skip_half, L)) return;
-  if (dwork && L.lane == 0 && L.d.first) {
-    atomicAdd(dwork, (unsigned long long)msize[2 * L.d.inst]);
-    atomicAdd(dwork + 1, (unsigned long long)msize[2 * L.d.inst + 1]);
+__global__ void __launch_bounds__(AB) k_amg_sweep(SView<AmgVal<V>> A, const V *__restrict__ dinv,
+                                                  const double *__restrict__ w, const int32_t *__restrict__ mode,
+                                                  int want, const V *__restrict__ b, const V *__restrict__ x,
+                                                  V *__restrict__ out, const KState *st, int skip_half,
+                                                  unsigned long long *dwork, const int64_t *__restrict__ msize) {
+  Lane L[SPW];
+  if (!amg_lanes(A, mode, want, st, skip_half, L)) return;
+  double acc[SPW];
+  amg_rows(A, L, [&](int, int64_t c) { return (double)x[c]; }, acc);
+#pragma unroll
+  for (int k = 0; k < SPW; k++) {
+    if (!L[k].on) continue;
+    if (dwork && (threadIdx.x & 63) == 0 && L[k].d.first) {
+      atomicAdd(dwork, (unsigned long long)msize[2 * L[k].d.inst]);
+      atomicAdd(dwork + 1, (unsigned long long)msize[2 * L[k].d.inst + 1]);
+    }
+    const double di = dinv ? (double)dinv[L[k].row] : 1.0;
+    out[L[k].row] = (V)((double)x[L[k].row] + w[L[k].d.inst] * di * ((double)b[L[k].row] - acc[k]));
   }
-  const double acc = amg_row(A, L, [&](int64_t k) { return (double)x[k]; });
-  const double di = dinv ? (double)dinv[L.row] : 1.0;
-  out[L.row] = (V)((double)x[L.row] + w[L.d.inst] * di * ((double)b[L.row] - acc));
 }
 
 // first sweep from zero on a large coarsest level: x = omega D^-1 b
 template <class V>
-__global__ void __launch_bounds__(AB) k_amg_jinit(SView<AmgVal<V>> A, const V *__restrict__ dinv, const double *__restrict__ w,
-                                                  const int32_t *__restrict__ mode, const V *__restrict__ b,
-                                                  V *__restrict__ x, const KState *st, int skip_half) {
-  Lane L;
-  if (!amg_lane(A, mode, Amg::MODE_JACOBI, st, skip_half, L)) return;
-  const double di = dinv ? (double)dinv[L.row] : 1.0;
-  x[L.row] = (V)(w[L.d.inst] * di * (double)b[L.row]);
+__global__ void __launch_bounds__(AB) k_amg_jinit(SView<AmgVal<V>> A, const V *__restrict__ dinv,
+                                                  const double *__restrict__ w, const int32_t *__restrict__ mode,
+                                                  const V *__restrict__ b, V *__restrict__ x, const KState *st,
+                                                  int skip_half) {
+  Lane L[SPW];
+  if (!amg_lanes(A, mode, Amg::MODE_JACOBI, st, skip_half, L)) return;
+#pragma unroll
+  for (int k = 0; k < SPW; k++)
+    if (L[k].on) {
+      const double di = dinv ? (double)dinv[L[k].row] : 1.0;
+      x[L[k].row] = (V)(w[L[k].d.inst] * di * (double)b[L[k].row]);
+    }
+}
+
+// ---------------------------------------------------------------- coarse tail
+
+// one level of the tail as its kernel sees it, precision V
+template <class V>
+struct TailLevel {
+  const SellDesc *ad, *pd, *rd;
+  const int32_t *ai, *pi, *ri;
+  const AmgVal<V> *av, *pv, *rv;
+  const int32_t *am0, *amn, *pm0, *pmn, *rm0, *rmn;
+  const V *dinv;
+  const double *w;
+  const int32_t *mode, *didx, *dsz;
+  const int64_t *drow, *doff;
+  const V *dinvA;
+  V *b, *x, *r;
+};
+
+template <class VA, class F>
+__device__ __forceinline__ double tail_row(const SellDesc &d, const int32_t *__restrict__ idx,
+                                           const VA *__restrict__ val, int lane, F &&f) {
+  double acc = 0.0;
+  const int64_t v0 = d.vbase + lane, i0 = (int64_t)d.ibase + lane;
+  for (int32_t j = 0; j < d.width; j++)
+    acc += (double)val[v0 + (int64_t)64 * j] * f((int64_t)d.coff + idx[i0 + (int64_t)64 * j]);
+  return acc;
+}
+
+// the slices s0 .. s0 + ns - 1 of a member's matrix, one wave per slice
+template <class F>
+__device__ __forceinline__ void tail_slices(const SellDesc *__restrict__ D, int32_t s0, int32_t ns, F &&body) {
+  const int lane = threadIdx.x & 63;
+  for (int32_t s = threadIdx.x >> 6; s < ns; s += AB / 64) {
+    const SellDesc d = D[s0 + s];
+    if (lane < d.rows) body(d, lane, (int64_t)d.row0 + lane);
+  }
+}
+
+// The V-cycle below the tail's first level for one member per block: the same steps as
+// the level-by-level kernels (smoothing from zero and residual, restriction; the coarsest
+// solve; prolongation and post-smoothing), each over the member's own slices, with the
+// block's barriers between the steps instead of kernel boundaries.  In: T[0].b; out:
+// T[0].r.
+template <class V>
+__global__ void __launch_bounds__(AB) k_amg_tail(const TailLevel<V> *__restrict__ T, int nt,
+                                                 const int32_t *__restrict__ members, const KState *st,
+                                                 int skip_half) {
+  const int m = members[blockIdx.x];
+  if (amg_skip(st, m, skip_half)) return;
+  int l = 0;
+  for (; l + 1 < nt; l++) {
+    const TailLevel<V> &L = T[l];
+    if (L.mode[m] != Amg::MODE_COARSEN) break;
+    const TailLevel<V> &C = T[l + 1];
+    const double wm = L.w[m];
+    tail_slices(L.ad, L.am0[m], L.amn[m], [&](const SellDesc &d, int lane, int64_t row) {
+      const double acc =
+          tail_row(d, L.ai, L.av, lane, [&](int64_t k) { return wm * (double)L.dinv[k] * (double)L.b[k]; });
+      L.r[row] = (V)((double)L.b[row] - acc);
+    });
+    __syncthreads();
+    tail_slices(L.rd, L.rm0[m], L.rmn[m], [&](const SellDesc &d, int lane, int64_t row) {
+      C.b[row] = (V)tail_row(d, L.ri, L.rv, lane, [&](int64_t k) { return (double)L.r[k]; });
+    });
+    __syncthreads();
+  }
+  {  // the member's coarsest level
+    const TailLevel<V> &L = T[l];
+    if (L.mode[m] == Amg::MODE_DENSE) {
+      const int k = L.didx[m];
+      const int n = L.dsz[k];
+      const int64_t r0 = L.drow[k];
+      const V *M = L.dinvA + L.doff[k];
+      for (int i = threadIdx.x; i < n; i += AB) {
+        double acc = 0.0;
+        for (int c = 0; c < n; c++) acc += (double)M[(int64_t)i * n + c] * (double)L.b[r0 + c];
+        L.r[r0 + i] = (V)acc;
+      }
+    } else {  // damped Jacobi from zero, an odd count of sweeps after the first ends in r
+      const double wm = L.w[m];
+      tail_slices(L.ad, L.am0[m], L.amn[m], [&](const SellDesc &, int, int64_t row) {
+        L.x[row] = (V)(wm * (double)L.dinv[row] * (double)L.b[row]);
+      });
+      V *a = L.x, *o = L.r;
+      for (int sw = 1; sw < Amg::COARSE_SWEEPS; sw++) {
+        __syncthreads();
+        tail_slices(L.ad, L.am0[m], L.amn[m], [&](const SellDesc &d, int lane, int64_t row) {
+          const double acc = tail_row(d, L.ai, L.av, lane, [&](int64_t k) { return (double)a[k]; });
+          o[row] = (V)((double)a[row] + wm * (double)L.dinv[row] * ((double)L.b[row] - acc));
+        });
+        V *t = a;
+        a = o;
+        o = t;
+      }
+    }
+    __syncthreads();
+  }
+  for (l = l - 1; l >= 0; l--) {
+    const TailLevel<V> &L = T[l], &C = T[l + 1];
+    const double wm = L.w[m];
+    tail_slices(L.pd, L.pm0[m], L.pmn[m], [&](const SellDesc &d, int lane, int64_t row) {
+      const double acc = tail_row(d, L.pi, L.pv, lane, [&](int64_t k) { return (double)C.r[k]; });
+      L.x[row] = (V)(wm * (double)L.dinv[row] * (double)L.b[row] + acc);
+    });
+    __syncthreads();
+    tail_slices(L.ad, L.am0[m], L.amn[m], [&](const SellDesc &d, int lane, int64_t row) {
+      const double acc = tail_row(d, L.ai, L.av, lane, [&](int64_t k) { return (double)L.x[k]; });
+      L.r[row] = (V)((double)L.x[row] + wm * (double)L.dinv[row] * ((double)L.b[row] - acc));
+    });
+    __syncthreads();
+  }
 }
 
 // exact coarsest solve out = Ac^-1 b, one block per inverted member
@@ -595,10 +766,12 @@ void Amg::build_sell(int l, int kind, SellMat &M, hipStream_t s) {
   std::vector<SellDesc> desc;
   const int64_t nrows = kind == 2 ? lev[l + 1].n : L.n;
   std::vector<int64_t> rbase((size_t)nrows, -1);
+  std::vector<int32_t> msl0((size_t)nsys, 0), mnsl((size_t)nsys, 0);
   int64_t vtot = 0;
   for (int64_t m = 0; m < nsys; m++) {
     const int hm = L.hmode[m];
     if (hm == MODE_ABSENT || (kind > 0 && hm != MODE_COARSEN)) continue;
+    msl0[m] = (int32_t)desc.size();
     const auto &PL = patterns[pid[m]].lv[l];
     const std::vector<int32_t> &rp = kind == 0 ? PL.arp : (kind == 1 ? PL.prp : PL.rrp);
     const std::vector<int32_t> &ci = kind == 0 ? PL.aci : (kind == 1 ? PL.pci : PL.rci);
@@ -641,6 +814,7 @@ void Amg::build_sell(int l, int kind, SellMat &M, hipStream_t s) {
     }
     for (int64_t li = 0; li < n; li++) rbase[row0 + li] = vtot + lc.rbase[li];
     vtot += lc.sptr.back();
+    mnsl[m] = (int32_t)desc.size() - msl0[m];
   }
   M.nsl = (int64_t)desc.size();
   M.nval = vtot;
@@ -648,6 +822,8 @@ void Amg::build_sell(int l, int kind, SellMat &M, hipStream_t s) {
   M.desc.upload(desc, s);
   M.idx.upload(idx, s);
   M.rbase.upload(rbase, s);
+  M.msl0.upload(msl0, s);
+  M.mnsl.upload(mnsl, s);
   M.v64.alloc((size_t)std::max<int64_t>(vtot, 1));
   M.v16.alloc((size_t)std::max<int64_t>(vtot, 1));
   VFVM_HIP_CHECK(hipMemsetAsync(M.v64.p, 0, sizeof(double) * M.v64.n, s));
@@ -890,6 +1066,11 @@ void Amg::build(const std::vector<AmgMember> &members, const int32_t *row_inst0,
         }
     }
     L.ndm = (int64_t)dmem.size();
+    {
+      std::vector<int32_t> di((size_t)nsys, -1);
+      for (size_t k = 0; k < dmem.size(); k++) di[dmem[k]] = (int32_t)k;
+      L.didx.upload(di, s);
+    }
     if (L.ndm) {
       L.dmem.upload(dmem, s);
       L.dsz.upload(dsz, s);
@@ -899,7 +1080,86 @@ void Amg::build(const std::vector<AmgMember> &members, const int32_t *row_inst0,
       L.dinvA32.alloc((size_t)ioff);
     }
   }
+  // the coarse tail: the first level (from 1) on which every member present is small
+  lt = nlev;
+  for (int l = 1; l < nlev && lt == nlev; l++) {
+    bool small = true;
+    for (int64_t m = 0; m < nsys; m++)
+      if (lev[l].hmode[m] != MODE_ABSENT && patterns[pid[m]].lv[l].n > TAIL_ROWS) small = false;
+    if (small) lt = l;
+  }
+  ntail = 0;
+  tail_tab[0].release();
+  tail_tab[1].release();
+  if (lt < nlev) {
+    std::vector<int32_t> tm;
+    for (int64_t m = 0; m < nsys; m++)
+      if (lev[lt].hmode[m] != MODE_ABSENT) tm.push_back((int32_t)m);
+    ntail = (int64_t)tm.size();
+    tail_mem.upload(tm, s);
+  }
   VFVM_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void Amg::build_tail(bool f32) {
+  if (lt >= nlev || ntail == 0) return;
+  DevBuf<unsigned char> &tab = tail_tab[f32 ? 0 : 1];
+  if (tab.p) return;  // the tables hold only pointers fixed at build time
+  auto fill = [&](auto tag) {
+    using V = decltype(tag);
+    std::vector<TailLevel<V>> T;
+    for (int l = lt; l < nlev; l++) {
+      Lev &L = lev[l];
+      TailLevel<V> t;
+      std::memset(&t, 0, sizeof(t));
+      auto vals = [&](SellMat &M) -> const AmgVal<V> * {
+        if constexpr (sizeof(V) == 4) return M.v16.p;
+        else return M.v64.p;
+      };
+      t.ad = L.A.desc.p;
+      t.ai = L.A.idx.p;
+      t.av = vals(L.A);
+      t.am0 = L.A.msl0.p;
+      t.amn = L.A.mnsl.p;
+      if (L.any_coarsen) {
+        t.pd = L.P.desc.p;
+        t.pi = L.P.idx.p;
+        t.pv = vals(L.P);
+        t.pm0 = L.P.msl0.p;
+        t.pmn = L.P.mnsl.p;
+        t.rd = L.R.desc.p;
+        t.ri = L.R.idx.p;
+        t.rv = vals(L.R);
+        t.rm0 = L.R.msl0.p;
+        t.rmn = L.R.mnsl.p;
+      }
+      t.w = L.w.p;
+      t.mode = L.mode.p;
+      t.didx = L.didx.p;
+      t.dsz = L.dsz.p;
+      t.drow = L.drow.p;
+      t.doff = L.doff.p;
+      V *b, *x, *r;
+      work<V>(L, b, x, r);
+      t.b = b;
+      t.x = x;
+      t.r = r;
+      if constexpr (sizeof(V) == 4) {
+        t.dinv = L.dinv32.p;
+        t.dinvA = L.dinvA32.p;
+      } else {
+        t.dinv = L.dinv64.p;
+        t.dinvA = L.dinvA64.p;
+      }
+      T.push_back(t);
+    }
+    tab.alloc(sizeof(TailLevel<V>) * T.size());
+    VFVM_HIP_CHECK(hipMemcpy(tab.p, T.data(), sizeof(TailLevel<V>) * T.size(), hipMemcpyHostToDevice));
+  };
+  if (f32)
+    fill(float());
+  else
+    fill(double());
 }
 
 void Amg::setup(const int32_t *rowptr, const int32_t *colind, const int32_t *dofdiag, const double *J,
@@ -991,9 +1251,12 @@ void Amg::apply(const V *y, V *z, const KState *st, int skip_half, hipStream_t s
     if constexpr (F32) return L.dinvA32.p;
     else return L.dinvA64.p;
   };
-  auto grid = [](const SellMat &M) { return (unsigned)std::max<int64_t>(1, (M.nsl + AB / 64 - 1) / (AB / 64)); };
+  auto grid = [](const SellMat &M) {
+    const int64_t per_block = (AB / 64) * SPW;
+    return (unsigned)std::max<int64_t>(1, (M.nsl + per_block - 1) / per_block);
+  };
   // down: smoothing from zero, residual, restriction; coarsest solves where members end
-  for (int l = 0; l < nlev; l++) {
+  for (int l = 0; l < std::min(nlev, lt); l++) {
     Lev &L = lev[l];
     const SView<AmgVal<V>> A = view(L.A);
     if (L.any_coarsen) {
@@ -1018,8 +1281,15 @@ void Amg::apply(const V *y, V *z, const KState *st, int skip_half, hipStream_t s
       }
     }
   }
+  // the coarse tail, one launch
+  if (lt < nlev && ntail) {
+    build_tail(F32);
+    hipLaunchKernelGGL((k_amg_tail<V>), dim3((unsigned)ntail), dim3(AB), 0, s,
+                       (const TailLevel<V> *)tail_tab[F32 ? 0 : 1].p, nlev - lt, (const int32_t *)tail_mem.p, st,
+                       skip_half);
+  }
   // up: prolongation (with the pre-smoothing recomputed) and post-smoothing
-  for (int l = nlev - 2; l >= 0; l--) {
+  for (int l = std::min(nlev, lt) - 1; l >= 0; l--) {
     Lev &L = lev[l];
     if (!L.any_coarsen) continue;
     hipLaunchKernelGGL((k_amg_prolong<V>), dim3(grid(L.P)), dim3(AB), 0, s, view(L.P), dinv(l), (const double *)L.w.p,

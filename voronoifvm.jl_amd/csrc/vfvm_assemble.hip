@@ -301,6 +301,28 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
     inst = a.node_inst ? a.node_inst[K] : 0;
     if (a.inst_active && !a.inst_active[inst]) active = false;
   }
+  // the node's own operands are requested before the block's staging barriers, so their
+  // loads are in flight while the adjacency range is staged; Uold only where the storage
+  // term reads it (a transient step with a storage function)
+  const InstParams *P = a.inst + inst;
+  int32_t a0 = 0, a1 = 0, diag = 0, bb0 = 0, bb1 = 0;
+  double uK[NS], uo[NS], srcv[NS], vol = 0.0;
+  if (active) {
+    a0 = a.adj_ptr[K];
+    a1 = a.adj_ptr[K + 1];
+    diag = a.adj_diag[K];
+    vol = a.nodevol[K];
+    bb0 = a.bn_ptr[K];
+    bb1 = a.bn_ptr[K + 1];
+    const bool old = a.tstepinv != 0.0 && P->storage != VFVM_STOR_NONE;
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      uK[i] = a.U[K * NS + i];
+      srcv[i] = a.src ? a.src[K * NS + i] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NS; i++) uo[i] = old ? a.Uold[K * NS + i] : uK[i];
+  }
   const bool sJ = JAC && (j1 - j0) <= STREAM_J && __syncthreads_and(active || !mine);
   if (sadj)
     for (int64_t q = e0 + threadIdx.x; q < e1; q += ASM_BLOCK) {
@@ -315,21 +337,9 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_assemble_stream(AsmArgs a) {
   __syncthreads();
   bool bad = false;
   if (active) {
-    const InstParams *P = a.inst + inst;
     const int flux_kind = P->flux;
     const double *__restrict__ p = P->p;  // parameters are loaded where the physics reads them
-    const int32_t a0 = a.adj_ptr[K], a1 = a.adj_ptr[K + 1];
     const int deg1 = a1 - a0;
-    const int diag = a.adj_diag[K];
-    double uK[NS], uo[NS], srcv[NS];
-#pragma unroll
-    for (int i = 0; i < NS; i++) {
-      uK[i] = a.U[K * NS + i];
-      uo[i] = a.Uold[K * NS + i];
-      srcv[i] = a.src ? a.src[K * NS + i] : 0.0;
-    }
-    const double vol = a.nodevol[K];
-    const int32_t bb0 = a.bn_ptr[K], bb1 = a.bn_ptr[K + 1];
     int64_t rowbase[NS];
 #pragma unroll
     for (int i = 0; i < NS; i++)

@@ -1240,6 +1240,8 @@ void vfvm_default_stepcontrol(int transient, vfvm_stepcontrol_t *sc) {
   sc->du_max_factor = 1.2;
   sc->num_final_steps = 5;
   sc->force_first_step = 0;
+  sc->handle_exceptions = 0;
+  sc->pad = 0;
 }
 
 int vfvm_solve_transient(vfvm_sys *s, const double *inival, int64_t nl, const double *lambdas, int transient,
@@ -1273,6 +1275,7 @@ int vfvm_solve_transient(vfvm_sys *s, const double *inival, int64_t nl, const do
     };
     int64_t istep = 0, nout = 0;
     int rc = VFVM_OK;
+    bool stop = false;  // handle_exceptions: the steps done so far are returned
     vfvm_history_t h{};
     // entry k of the outputs: (lambda, solution); entry 0 is the start (src/vfvm_solver.jl:304-337)
     auto record = [&](double lam, const DevBuf<double> &u) {
@@ -1297,32 +1300,45 @@ int vfvm_solve_transient(vfvm_sys *s, const double *inival, int64_t nl, const do
     }
     double dl = sc.d;
     auto approx = [](double a, double b) { return std::fabs(a - b) <= 1.4901161193847656e-08 * std::max(std::fabs(a), std::fabs(b)); };
-    for (int64_t i = 0; i + 1 < nl && rc == VFVM_OK; i++) {
+    for (int64_t i = 0; i + 1 < nl && rc == VFVM_OK && !stop; i++) {
       dl = std::max(dl, sc.d_min);
       const double lend = lambdas[i + 1];
       double l = lambdas[i];
-      while (l < lend && rc == VFVM_OK) {
+      while (l < lend && rc == VFVM_OK && !stop) {
         const double l0 = l;
         double du = 0.0;
         for (;;) {
           l = l0 + dl;
           rc = transient ? newton_device(*s, l, dl, 0.0, c, &h) : newton_device(*s, 0.0, INFINITY, l, c, &h);
           account(h);
-          if (rc) break;  // handle_exceptions = false: a failing Newton solve ends the run
-          launch_maxdiff(s->ndof, s->d_U.p, s->d_Uold.p, s->d_partials.p, s->d_red.p + 4, st);
-          VFVM_HIP_CHECK(hipMemcpyAsync(s->h_pinned, s->d_red.p + 4, sizeof(double), hipMemcpyDeviceToHost, st));
-          VFVM_HIP_CHECK(hipStreamSynchronize(st));
-          du = s->h_pinned[0];
-          if (!(du > sc.du_max_factor * sc.du_opt)) break;
+          bool errored = false;
+          if (rc) {
+            if (!sc.handle_exceptions) break;  // a failing Newton solve ends the run
+            errored = true;  // retried with a smaller step (src/vfvm_solver.jl:391-404)
+            rc = VFVM_OK;
+          } else {
+            launch_maxdiff(s->ndof, s->d_U.p, s->d_Uold.p, s->d_partials.p, s->d_red.p + 4, st);
+            VFVM_HIP_CHECK(hipMemcpyAsync(s->h_pinned, s->d_red.p + 4, sizeof(double), hipMemcpyDeviceToHost, st));
+            VFVM_HIP_CHECK(hipStreamSynchronize(st));
+            du = s->h_pinned[0];
+            if (!(du > sc.du_max_factor * sc.du_opt)) break;
+          }
           if (approx(dl, sc.d_min)) {
-            if (sc.force_first_step && istep == 0) break;
+            if (sc.force_first_step && istep == 0 && !errored) break;
+            if (sc.handle_exceptions) {  // src/vfvm_solver.jl:413-440: warn, return what is done
+              s->ctx->err = errored ? "the Newton solve failed at the smallest step; returned before the end"
+                                    : "step size minimum reached while du/du_opt = " + std::to_string(du / sc.du_opt) +
+                                          "; returned before the end";
+              stop = true;
+              break;
+            }
             s->ctx->err = "step size minimum reached while du/du_opt = " + std::to_string(du / sc.du_opt);
             rc = VFVM_ERR_CONVERGENCE;
             break;
           }
           dl = std::max(sc.d_min, dl * sc.d_decrease);
         }
-        if (rc) break;
+        if (rc || stop) break;
         record(l, s->d_U);
         istep++;
         launch_copy(s->ndof, s->d_U.p, s->d_Uold.p, st);
@@ -1341,7 +1357,7 @@ int vfvm_solve_transient(vfvm_sys *s, const double *inival, int64_t nl, const do
     if (U_final) s->d_Uold.download(U_final, (size_t)s->ndof, st);
     VFVM_HIP_CHECK(hipStreamSynchronize(st));
     if (nsteps) *nsteps = istep;
-    tot.converged = rc == VFVM_OK;
+    tot.converged = rc == VFVM_OK && !stop;
     tot.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (hist) *hist = tot;
     return rc;

@@ -521,6 +521,7 @@ struct StepControlT
     d::Float64; d_min::Float64; d_max::Float64; d_grow::Float64; d_decrease::Float64
     du_opt::Float64; du_max_factor::Float64
     num_final_steps::Int32; force_first_step::Int32
+    handle_exceptions::Int32; pad::Int32
 end
 
 """
@@ -532,9 +533,9 @@ The Δt (time stepping) or Δp (embedding) fields of a SolverControl
 function step_control(c, transient)
     transient ?
         StepControlT(c.Δt, c.Δt_min, c.Δt_max, c.Δt_grow, c.Δt_decrease, c.Δu_opt, c.Δu_max_factor,
-                     c.num_final_steps, c.force_first_step) :
+                     c.num_final_steps, c.force_first_step, c.handle_exceptions, 0) :
         StepControlT(c.Δp, c.Δp_min, c.Δp_max, c.Δp_grow, c.Δp_decrease, c.Δu_opt, c.Δu_max_factor,
-                     c.num_final_steps, c.force_first_step)
+                     c.num_final_steps, c.force_first_step, c.handle_exceptions, 0)
 end
 
 """

@@ -60,7 +60,8 @@ class HistoryT(C.Structure):
 class StepControlT(C.Structure):
     _fields_ = [("d", C.c_double), ("d_min", C.c_double), ("d_max", C.c_double), ("d_grow", C.c_double),
                 ("d_decrease", C.c_double), ("du_opt", C.c_double), ("du_max_factor", C.c_double),
-                ("num_final_steps", C.c_int32), ("force_first_step", C.c_int32)]
+                ("num_final_steps", C.c_int32), ("force_first_step", C.c_int32),
+                ("handle_exceptions", C.c_int32), ("pad", C.c_int32)]
 
 
 class LinStatsT(C.Structure):

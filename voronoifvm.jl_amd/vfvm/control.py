@@ -48,6 +48,9 @@ class SolverControl:
     Δu_opt: float = 0.1
     Δu_max_factor: float = 1.2
     force_first_step: bool = False
+    # src/vfvm_solvercontrol.jl:205-216: retry failing Newton steps with smaller steps and
+    # return the steps done so far (with a warning) instead of raising
+    handle_exceptions: bool = False
     num_final_steps: int = 5
     verbose: bool = False
 
@@ -89,6 +92,7 @@ class SolverControl:
         sc.du_max_factor = float(self.Δu_max_factor)
         sc.num_final_steps = int(self.num_final_steps)
         sc.force_first_step = int(bool(self.force_first_step))
+        sc.handle_exceptions = int(bool(self.handle_exceptions))
         return sc
 
 

@@ -565,6 +565,9 @@ def solve(sys: System, inival=0.0, control: Optional[SolverControl] = None, time
         sys.history = NewtonSolverHistory(niter=hist.niter, nlin=hist.nlin, time=hist.time,
                                           residual_norm=hist.residual_norm, fp64_fallbacks=hist.fp64_fallbacks)
         _capi.check(rc, sys.ctx.handle)
+        if not hist.converged:  # handle_exceptions: the steps done so far (src/vfvm_solver.jl:413-440)
+            import warnings
+            warnings.warn("solve: " + _capi.lib().vfvm_last_error(sys.ctx.handle).decode(errors="replace"))
         n = min(int(nst.value) + 1, max_out)  # entries: start + accepted steps
         if store_all:
             ts = list(out_t[:n])
