@@ -76,6 +76,15 @@ def probe_bytes(kind, probe, sv=4, nspec=1, transient=True):
         # column indices shared by the members of one pattern (read from the L2, not
         # counted), b and x read and out written in the Krylov precision
         return 2 * w[1] / n + 3 * sv * w[0] / n
+    if kind == "msolve":
+        # one launch = every member's BiCGStab solve; per iteration of a member: two
+        # operator applications (values sv + 2-byte column offset per nonzero; x gathered
+        # once, y written, one dot vector read: 3 sv per row), two AMG V-cycles (the
+        # cycle's matrix entry reads at 2 bytes (FP16) or 8, eight finest-level vector
+        # passes), and the vector updates (p: 4 vectors, s: 3, x and r: 8)
+        rows, nnz, cyc = w[0] / n, w[1] / n, w[2] / n
+        vb = 2 if sv == 4 else 8
+        return 2 * nnz * (sv + 2) + 2 * 3 * sv * rows + 2 * vb * cyc + 2 * 8 * sv * rows + 15 * sv * rows
     return assembly_bytes(nspec, w[0] / n, w[1] / n, w[2] / n, transient)
 
 
@@ -498,6 +507,7 @@ def run_cfg4(args):
     fam = fams[0]
     for f in fams:
         f.batch.set_graphs(not args.no_graphs)
+        f.batch.set_member_solver(args.member_block)
     # BiCGStab preconditioned by the smoothed-aggregation AMG V-cycle (the reference's
     # SmoothedAggregationPreconBuilder, examples/DevEx003_Solvers.jl:149-156), FP32 inside
     # FP64 refinement; the FP32 solve aims at reltol_linear at once (one refinement sweep
@@ -555,7 +565,9 @@ def run_cfg4(args):
     # beside it, per block: the host-buffer path, a probe pass, and the batched solves from zero
     t_sur_host, t_zero, t_zero64, it_zero, sur_err = 0.0, 0.0, 0.0, [], []
     converged = []
-    probes = {_capi.KERNEL_SPMV32: [], _capi.KERNEL_AMG32: [], _capi.KERNEL_ASSEMBLE: []}
+    kinds = ((_capi.KERNEL_MSOLVE32,) if args.member_block else (_capi.KERNEL_SPMV32, _capi.KERNEL_AMG32)) + \
+        (_capi.KERNEL_ASSEMBLE,)
+    probes = {k: [] for k in kinds}
     for b in range(min(K, nb)):
         f = fams[b]
         infer(b)
@@ -567,7 +579,7 @@ def run_cfg4(args):
         Uh = f.batch.solve([x.reshape(1, -1) for x in gh], ctrl32)
         t_sur_host += time.perf_counter() - t1
         worst = max(worst, max(h.residual_norm for h in f.batch.histories))
-        with ctx.probe(_capi.KERNEL_SPMV32, _capi.KERNEL_AMG32, _capi.KERNEL_ASSEMBLE) as pr:  # timed one by one
+        with ctx.probe(*kinds) as pr:  # timed launch by launch
             f.batch.solve_device(U_dev[b], G_dev[b], ctrl32)
         for k, v in pr.items():
             probes[k].append(v)
@@ -678,17 +690,29 @@ def cfg4_result(args, S, fam, net, meta, probes, B, K, nb, nsol, t_sur, sur_rate
         return dict(total_ms=sum(x["total_ms"] for x in lst), launches=sum(x["launches"] for x in lst),
                     avg_ms=sum(x["total_ms"] for x in lst) / max(sum(x["launches"] for x in lst), 1),
                     work=tuple(sum(x["work"][i] for x in lst) for i in range(3)))
-    pspmv, pasm = merged(probes[_capi.KERNEL_SPMV32]), merged(probes[_capi.KERNEL_ASSEMBLE])
-    pamg = merged(probes[_capi.KERNEL_AMG32])
-    rl_spmv = roofline_hbm(probe_bytes("spmv", pspmv, sv=4), pspmv, read_traffic("cfg4_bspmv32"),
-                           "batched FP32 SpMV")
-    rl_spmv["kernel"] = f"batched FP32 SpMV over the active members of {B} x 129^2 (k_bspmv)"
-    # the AMG V-cycle's finest-level post-smoothing sweep: an SpMV of the row-scaled
-    # operator plus b read once more, over the members still iterating
-    rl_amg = roofline_hbm(probe_bytes("amg_sweep", pamg, sv=4), pamg, read_traffic("cfg4_amg_sweep32"),
-                          "AMG finest-level sweep (FP32)")
-    rl_amg["kernel"] = (f"AMG V-cycle finest-level Jacobi sweep over the active members of {B} x 129^2 "
-                        f"(k_amg_sweep: FP16 SELL-64 values, shared column indices, FP32 vectors)")
+    pasm = merged(probes[_capi.KERNEL_ASSEMBLE])
+    extra = {}
+    if _capi.KERNEL_MSOLVE32 in probes:
+        # the whole FP32 Krylov solve of every member, one block per member (k_msolve):
+        # algorithmic bytes from the iterations each member took
+        pm = merged(probes[_capi.KERNEL_MSOLVE32])
+        rl_main = roofline_hbm(probe_bytes("msolve", pm, sv=4), pm, read_traffic("cfg4_msolve32"),
+                               "one-block-per-member BiCGStab + AMG (FP32)")
+        rl_main["kernel"] = (f"k_msolve: each of the {B} members' right-preconditioned BiCGStab (two operator "
+                             f"applications and two AMG V(1,1) cycles per iteration) in one {args.member_block}-"
+                             f"thread block, one launch per linear solve of the batch")
+    else:
+        pspmv, pamg = merged(probes[_capi.KERNEL_SPMV32]), merged(probes[_capi.KERNEL_AMG32])
+        rl_spmv = roofline_hbm(probe_bytes("spmv", pspmv, sv=4), pspmv, read_traffic("cfg4_bspmv32"),
+                               "batched FP32 SpMV")
+        rl_spmv["kernel"] = f"batched FP32 SpMV over the active members of {B} x 129^2 (k_bspmv)"
+        extra["roofline_bspmv"] = rl_spmv
+        # the AMG V-cycle's finest-level post-smoothing sweep: an SpMV of the row-scaled
+        # operator plus b read once more, over the members still iterating
+        rl_main = roofline_hbm(probe_bytes("amg_sweep", pamg, sv=4), pamg, read_traffic("cfg4_amg_sweep32"),
+                               "AMG finest-level sweep (FP32)")
+        rl_main["kernel"] = (f"AMG V-cycle finest-level Jacobi sweep over the active members of {B} x 129^2 "
+                             f"(k_amg_sweep: FP16 SELL-64 values, shared column indices, FP32 vectors)")
     rl_asm = roofline_hbm(probe_bytes("assemble", pasm, transient=False), pasm, read_traffic("cfg4_assemble"),
                           "batched assembly")
     import ctypes as C
@@ -716,7 +740,7 @@ def cfg4_result(args, S, fam, net, meta, probes, B, K, nb, nsol, t_sur, sur_rate
                 max_abs_diff_vs_plain=diff, worst_residual=worst,
                 surrogate=dict(train_samples=int(meta["samples"]), epochs=int(meta["epochs"]),
                                final_train_mse=float(meta["final_loss"])),
-                roofline=rl_amg, roofline_bspmv=rl_spmv, roofline_assemble=rl_asm,
+                roofline=rl_main, **extra, roofline_assemble=rl_asm,
                 roofline_mpnn_layer=dict(bound="hbm", achieved=round(layer_gbs, 1), peak=8000.0, unit="GB/s",
                                          frac=round(layer_gbs / 8000.0, 4), traffic=read_traffic("cfg4_mpnn_layer"),
                                          traffic_note="mean over the four layers of one inference; the last "
@@ -733,7 +757,7 @@ def cfg4_result(args, S, fam, net, meta, probes, B, K, nb, nsol, t_sur, sur_rate
                                      "(BiCGStab/Jacobi) from zero one solve at a time; reltol_linear 1e-4 (the "
                                      "reference's default) in every form",
                             solves=nsol, unknowns_per_solve=fam.grid.num_nodes, batch=B,
-                            hip_graphs=not args.no_graphs,
+                            hip_graphs=not args.no_graphs, member_solver_block=args.member_block,
                             resident="sources, side values, guesses and solutions in HBM (the *_device "
                                      "entry points); host_buffer_path_solves_per_s includes the PCIe "
                                      "transfers"),
@@ -910,6 +934,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="cfg4: skip the plain one-at-a-time and continuation legs (profiling runs)")
+    ap.add_argument("--member-block", type=int, default=0, choices=(0, 256, 512, 1024),
+                    help="cfg4: batched Krylov solves as one block of this many threads per member "
+                         "(0: one launch per BiCGStab step for all members)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="batched Krylov blocks launched kernel by kernel instead of as HIP graphs "
                          "(for kernel-trace profiling)")

@@ -90,11 +90,19 @@ typedef struct {
 #define VFVM_KRYLOV_GMRES 1
 #define VFVM_PRECON_JACOBI 0       /* point Jacobi (row scaling) */
 #define VFVM_PRECON_BLOCKJACOBI 1  /* nspec x nspec node blocks */
-#define VFVM_PRECON_ILU0 2         /* ILU(0), multicolour level order */
+/* ILU(0) of the rows in a multicolour order: rows of one colour are independent, so a
+ * triangular solve takes one launch per colour (a handful); a reordering, hence another
+ * preconditioner than ILUZero's */
+#define VFVM_PRECON_ILU0 2
 /* smoothed aggregation AMG, one V(1,1) cycle per application (scalar systems):
  * KrylovJL_BICGSTAB(precs = SmoothedAggregationPreconBuilder()),
  * examples/DevEx003_Solvers.jl:149-156 */
 #define VFVM_PRECON_AMG 3
+/* ILU(0) in the natural dof order, ILUZero.jl's ilu0 (Project.toml:67; used as
+ * KrylovJL_BICGSTAB(precs = ILUZeroPreconBuilder()), examples/DevEx003_Solvers.jl:90-120):
+ * factorisation and triangular solves level-scheduled over the dependency levels of the
+ * natural order (one launch per level: about 2N levels on N x N grids, 3N on N^3) */
+#define VFVM_PRECON_ILU0_NATURAL 4
 #define VFVM_PREC_FP64 0
 #define VFVM_PREC_FP32_REFINE 1    /* FP32 Krylov + preconditioner inside FP64 iterative refinement */
 
@@ -391,6 +399,14 @@ int vfvm_amg_levels(vfvm_sys *sys, int *nlevels, int64_t *rows);
 int vfvm_amg_level(vfvm_sys *sys, int level, int64_t *sizes, int32_t *arp, int32_t *aci, double *aval,
                    int32_t *prp, int32_t *pci, double *pval, int32_t *agg, double *omega);
 
+/* The ILU(0) factor of the last linear solve with an ILU(0) preconditioner, in the
+ * system's CSR order (lu[k] for the k-th stored entry of the dof pattern): the strictly
+ * lower part holds L (unit diagonal not stored), the rest U, of the Jacobian without the
+ * row scaling the solver applies; lower/upper refer to the factor's ordering (natural for
+ * VFVM_PRECON_ILU0_NATURAL).  Inspection for tests, as ILUZero.ILU0Precon's l/u fields
+ * (VFVM_ERR_ARG if no factor exists). */
+int vfvm_ilu0_factor(vfvm_sys *sys, double *lu);
+
 /* Damped Newton: solve_step!, src/vfvm_solver.jl:13-222.  On entry U is ignored and Uold is
  * the initial value / old time step; on return U holds the solution.  Assembly, linear
  * solves, damping and the norms stay on the device; hist mirrors NewtonSolverHistory. */
@@ -495,6 +511,10 @@ int vfvm_set_compact_graph(vfvm_sys *sys, int enable);
  * (out = x + omega (b - Ahat x)), FP64 and FP32 */
 #define VFVM_KERNEL_AMG64 6
 #define VFVM_KERNEL_AMG32 7
+/* probe classes only: the batched one-block-per-member BiCGStab solve (k_msolve: one
+ * launch per linear solve of the batch), FP64 and FP32 */
+#define VFVM_KERNEL_MSOLVE64 8
+#define VFVM_KERNEL_MSOLVE32 9
 int vfvm_time_kernel(vfvm_sys *sys, int kernel, int reps, double *avg_ms);
 
 /* Benchmark helper (no counterpart in the reference): live timing of kernel classes
@@ -507,8 +527,11 @@ int vfvm_time_kernel(vfvm_sys *sys, int kernel, int reps, double *avg_ms);
  * vfvm_probe_read returns one class's summed duration, the number of timed launches and
  * (work, 3 entries, nullable) the work those launches did, then disarms that class:
  * SpMV: rows, nonzeros, 0; assembly: nodes, adjacency entries without the node itself
- * (2 x edges), Jacobian slots; AMG sweep: rows, nonzeros, 0.  In batches only the members
- * still active count. */
+ * (2 x edges), Jacobian slots; AMG sweep: rows, nonzeros, 0; member solve: the sums over
+ * members of iterations x rows, iterations x operator nonzeros and iterations x the
+ * matrix entries one AMG V-cycle reads (A of every level twice, P and P^T once; 0 without
+ * AMG).  In batches
+ * only the members still active count. */
 int vfvm_probe_arm(vfvm_ctx *ctx, int kernel, int64_t max_launches);
 int vfvm_probe_read(vfvm_ctx *ctx, int kernel, double *total_ms, int64_t *launches, int64_t *work);
 
@@ -532,6 +555,12 @@ int vfvm_batch_destroy(vfvm_batch *b);
 /* Batched Krylov iterations between status reads run as captured HIP graphs (default on);
  * enable = 0 launches the same kernels one by one.  No counterpart in the reference. */
 int vfvm_batch_set_graphs(vfvm_batch *b, int enable);
+/* The batched Krylov solves run as one block per member (block = 256, 512 or 1024
+ * threads): each member's whole BiCGStab (and AMG V-cycles) in one block, one launch per
+ * linear solve, no host read between iterations; block = 0 selects the kernel-per-step
+ * form (one launch per BiCGStab step for all members, iteration blocks as graphs).  Same
+ * iteration and stopping rules either way.  No counterpart in the reference. */
+int vfvm_batch_set_member_solver(vfvm_batch *b, int block);
 /* the batch's SpMV column width, as vfvm_set_sell_index */
 int vfvm_batch_set_sell_index(vfvm_batch *b, int bits);
 /* the batch's assembly graph, as vfvm_set_compact_graph */

@@ -100,6 +100,8 @@ def lib():
         L.orc_solve_transient.argtypes = [C.c_void_p, dp, C.c_int, dp, C.c_int, C.POINTER(Control),
                                           C.c_double, dp, C.POINTER(C.c_int), C.POINTER(C.c_int), dp, C.c_int]
         L.orc_linsolve.argtypes = [C.c_void_p, C.POINTER(Control), dp, dp, C.POINTER(C.c_int)]
+        L.orc_ilu0.argtypes = [C.c_int, np.ctypeslib.ndpointer(np.int64, flags="C"),
+                               np.ctypeslib.ndpointer(np.int32, flags="C"), dp, dp]
         L.orc_default_control.argtypes = [C.POINTER(Control)]
         L.orc_bernoulli.restype = C.c_double
         L.orc_bernoulli.argtypes = [C.c_double]
@@ -309,6 +311,19 @@ class OracleSystem:
         self._check(lib().orc_linsolve(self._h, C.byref(control),
                                        np.ascontiguousarray(b, dtype=np.float64), x, C.byref(it)))
         return x, it.value
+
+
+def ilu0(rowptr, colind, val):
+    """ILUZero's natural-order ilu0 of a CSR matrix with sorted rows: the L (strictly
+    lower, unit diagonal implied) and U values in the stored positions."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+    ci = np.ascontiguousarray(colind, dtype=np.int32)
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    lu = np.zeros(v.size)
+    rc = lib().orc_ilu0(rp.size - 1, rp, ci, v, lu)
+    if rc:
+        raise RuntimeError(f"oracle ilu0: error {rc}")
+    return lu
 
 
 def bernoulli(x: float) -> float:

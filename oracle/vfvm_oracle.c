@@ -1364,6 +1364,18 @@ static void ilu0_apply(const ilu0_t *f, const double *b, double *x) {
   }
 }
 
+/* ILUZero's ilu0 on a CSR matrix with sorted rows: lu = L (strictly lower, unit diagonal
+   not stored) and U in the stored positions; returns ORC_ERR_LINEAR on a missing diagonal */
+int orc_ilu0(int n, const int64_t *ptr, const int32_t *idx, const double *val, double *lu) {
+  csx_t A = {n, (int64_t *)ptr, (int32_t *)idx, (double *)val};
+  ilu0_t f;
+  int rc = ilu0_factor(&A, &f);
+  if (!rc) memcpy(lu, f.lu.val, sizeof(double) * (size_t)ptr[n]);
+  csx_free(&f.lu);
+  free(f.diag);
+  return rc;
+}
+
 /* block-Jacobi (ns x ns node blocks), Doolittle LU per block
    (src/vfvm_functions.jl:98-150 style, no pivoting) */
 typedef struct {

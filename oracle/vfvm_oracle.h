@@ -165,6 +165,10 @@ int orc_solve_transient(orc_sys *s, const double *inival, int nlambdas, const do
                         int transient, const orc_control *c, double time, double *U_final,
                         int *nsteps, int *newton_total, double *steps_out, int max_steps);
 
+/* ILU(0) in the natural order of a CSR matrix (rows sorted): L below the diagonal (unit
+   diagonal implied), U on and above it, in the stored positions */
+int orc_ilu0(int n, const int64_t *ptr, const int32_t *idx, const double *val, double *lu);
+
 /* linear solve on the currently assembled matrix: x = A^{-1} b */
 int orc_linsolve(orc_sys *s, const orc_control *c, const double *b, double *x, int *iters);
 

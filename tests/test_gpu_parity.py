@@ -142,6 +142,63 @@ def test_linear_solvers_agree_with_direct(krylov, precon, precision):
     assert np.abs(x - xd).max() <= 1e-7 * max(1.0, np.abs(xd).max())
 
 
+@pytest.mark.parametrize("case", ["poisson2d", "nldiff3d", "dd2d"])
+def test_natural_ilu0_factor_matches_oracle(case):
+    """precon "ilu0_natural" is ILUZero's ilu0 (natural order, level-scheduled on the
+    device): its L and U equal the oracle's natural-order factor of the same Jacobian, and
+    its BiCGStab solve agrees with the direct one.  The multicolour "ilu0" factors a
+    reordered matrix, so its factor differs."""
+    _print("case natural ilu0", case)
+    import scipy.sparse.linalg as sla
+    rng = np.random.default_rng(11)
+    if case == "poisson2d":
+        sys, _ = cases.nonlinear_poisson_2d(17)
+        U, tstep = rng.uniform(-1, 1, (1, sys.grid.num_nodes)), math.inf
+    elif case == "nldiff3d":
+        sys, _ = cases.nonlinear_diffusion_3d(6)
+        U, tstep = rng.uniform(0, 1, (1, sys.grid.num_nodes)), 0.05
+    else:
+        sys, _ = cases.drift_diffusion_2d(9, lambda2=1e-2)
+        U, tstep = vfvm.unknowns(sys, inival=cases.dd_initial(sys.grid)), math.inf
+    F, A = vfvm.evaluate_residual_and_jacobian(sys, U, tstep=tstep)
+    b = vfvm.dofs(F)
+    rp, ci = sys.pattern_csr()
+    Ar = A.tocsr()
+    Ar.sort_indices()
+    assert np.array_equal(Ar.indptr, rp) and np.array_equal(Ar.indices, ci)
+    ref = O.ilu0(rp, ci, Ar.data)
+    rowmax = np.array([np.abs(ref[rp[i]:rp[i + 1]]).max() for i in range(rp.size - 1)])
+    tol = 1e-10 * np.abs(ref) + 1e-13 * np.repeat(rowmax, np.diff(rp))
+    kw = dict(reltol_linear=1e-11, abstol_linear=0.0, maxiters_linear=5000)
+    x, st = vfvm.linsolve(sys, b, precon="ilu0_natural", **kw)
+    lu = vfvm.ilu0_factor(sys)
+    worst = np.max(np.abs(lu - ref) / tol)
+    xd = sla.spsolve(A.tocsc(), b)
+    _print(case, "natural ilu0: iters", st["iters"], "factor err / tol", worst)
+    assert worst <= 1.0
+    assert np.abs(vfvm.dofs(x) - xd).max() <= 1e-7 * max(1.0, np.abs(xd).max())
+    try:  # the multicolour factor is built whether or not its solve converges (on dd2d it does not)
+        vfvm.linsolve(sys, b, precon="ilu0", reltol_linear=1e-2, abstol_linear=0.0, maxiters_linear=50)
+    except vfvm.LinearSolverError:
+        pass
+    assert np.max(np.abs(vfvm.ilu0_factor(sys) - ref) / tol) > 1e3
+    # back to the natural structure: the factor is rebuilt in that order
+    vfvm.linsolve(sys, b, precon="ilu0_natural", **kw)
+    assert np.max(np.abs(vfvm.ilu0_factor(sys) - ref) / tol) <= 1.0
+
+
+def test_newton_with_natural_ilu0_matches_oracle():
+    _print("case newton natural ilu0")
+    sys, osys = cases.nonlinear_poisson_2d(33)
+    for precision in ("fp64", "fp32"):
+        ctrl = vfvm.SolverControl(precon="ilu0_natural", precision=precision, reltol_linear=1e-10,
+                                  abstol_linear=0.0, maxiters_linear=2000)
+        U = vfvm.solve(sys, inival=0.0, control=ctrl)
+        Uo, _ = osys.solve_step(np.zeros(sys.grid.num_nodes))
+        assert np.abs(vfvm.dofs(U) - Uo).max() <= SOL_ATOL
+        assert sys.history.residual_norm <= 1e-10
+
+
 def test_block_jacobi_gmres_drift_diffusion_continuation():
     """cfg2 at small size: bias continuation with GMRES + 3x3 block-Jacobi vs oracle."""
     _print("case dd continuation")

@@ -322,3 +322,60 @@ def test_dd_two_node_current_matches_ode(psiK, psiL, nK, nL):
         sc = max(abs(j), abs(U[ispec]).max() / h)
         assert abs(F[ispec, 0] - j) <= 1e-9 * sc, (ispec, F[ispec, 0], j)
         assert abs(F[ispec, 1] + j) <= 1e-9 * sc, (ispec, F[ispec, 1], -j)
+
+
+def _five_point(n, rng):
+    """Random diagonally dominant matrix with the 5-point pattern of an n x n grid (CSR)."""
+    N = n * n
+    rows = []
+    for i in range(N):
+        x, y = i % n, i // n
+        cols = [c for c, ok in ((i - n, y > 0), (i - 1, x > 0), (i, True), (i + 1, x < n - 1), (i + n, y < n - 1))
+                if ok]
+        vals = [-rng.random() if c != i else 0.0 for c in cols]
+        vals[cols.index(i)] = 4.5 + rng.random()
+        rows.append((cols, vals))
+    rp = np.cumsum([0] + [len(c) for c, _ in rows])
+    ci = np.concatenate([c for c, _ in rows]).astype(np.int32)
+    v = np.concatenate([v for _, v in rows])
+    return rp, ci, v
+
+
+def _dense_lu(rp, ci, lu):
+    N = rp.size - 1
+    L, U = np.eye(N), np.zeros((N, N))
+    for i in range(N):
+        for k in range(rp[i], rp[i + 1]):
+            (L if ci[k] < i else U)[i, ci[k]] = lu[k]
+    return L, U
+
+
+def test_oracle_ilu0_is_exact_lu_on_tridiagonal():
+    """ILU(0) drops no fill on a tridiagonal matrix, so it is the LU factorisation."""
+    rng = np.random.default_rng(3)
+    n = 12
+    rp = np.array([0] + [min(i + 2, n) - max(i - 1, 0) for i in range(n)]).cumsum()
+    ci = np.concatenate([np.arange(max(i - 1, 0), min(i + 2, n)) for i in range(n)]).astype(np.int32)
+    v = rng.random(ci.size) + 3.0 * (ci == np.repeat(np.arange(n), np.diff(rp)))
+    L, U = _dense_lu(rp, ci, O.ilu0(rp, ci, v))
+    A = np.zeros((n, n))
+    A[np.repeat(np.arange(n), np.diff(rp)), ci] = v
+    assert np.abs(L @ U - A).max() <= 1e-14 * np.abs(A).max()
+
+
+def test_oracle_ilu0_reproduces_the_matrix_on_its_pattern():
+    """The defining property of ILU(0) (Saad, Iterative Methods, sec. 10.3.2): L unit
+    lower and U upper on the pattern of A with (LU)_ij = a_ij for every stored (i, j);
+    such factors are unique.  Off the pattern LU carries the dropped fill."""
+    rng = np.random.default_rng(4)
+    rp, ci, v = _five_point(7, rng)
+    L, U = _dense_lu(rp, ci, O.ilu0(rp, ci, v))
+    N = rp.size - 1
+    A = np.zeros((N, N))
+    mask = np.zeros((N, N), bool)
+    r = np.repeat(np.arange(N), np.diff(rp))
+    A[r, ci] = v
+    mask[r, ci] = True
+    P = L @ U
+    assert np.abs(P[mask] - A[mask]).max() <= 1e-13 * np.abs(A).max()
+    assert np.abs(P[~mask]).max() > 1e-3  # not the exact LU: fill was dropped

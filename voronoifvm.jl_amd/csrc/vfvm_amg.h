@@ -120,6 +120,14 @@ class Amg {
   void setup(const int32_t *rowptr, const int32_t *colind, const int32_t *dofdiag, const double *J,
              const double *rowscale, hipStream_t s);
   bool ready() const { return numeric_ready; }
+  // matrix entries one V-cycle of member m reads: A of every level twice (smoothing and
+  // residual, or the coarsest solve's sweeps counted as two), P and P^T of every
+  // coarsening level once each
+  int64_t cycle_nnz(int64_t m) const {
+    int64_t t = 0;
+    for (const auto &L : patterns[pid[m]].lv) t += 2 * (int64_t)L.aci.size() + 2 * (int64_t)L.pci.size();
+    return t;
+  }
   // The level-0 operator of the cycle from the current J (every Newton step; the coarse
   // levels of the last setup are kept): the FP32 solves' values only (fp32) or both
   void refresh_fine(const int32_t *rowptr, const double *J, const double *rowscale, bool fp32, hipStream_t s);
@@ -138,6 +146,13 @@ class Amg {
       work<V>(L, b, x, r);
     }
     build_tail(sizeof(V) == 4);
+    build_levels(member_tab[sizeof(V) == 4 ? 0 : 1], 0, sizeof(V) == 4);
+  }
+  // device table of MemberLevel<V> views of every level, for a whole V-cycle of one
+  // member inside one block (amg_member_vcycle, vfvm_amg_dev.h); after reserve<V>()
+  template <class V>
+  const void *member_levels() const {
+    return member_tab[sizeof(V) == 4 ? 0 : 1].p;
   }
 
   int levels() const { return nlev; }
@@ -226,8 +241,12 @@ class Amg {
   int lt = 0;
   int64_t ntail = 0;
   DevBuf<int32_t> tail_mem;
-  DevBuf<unsigned char> tail_tab[2];  // FP32 and FP64 apply
+  DevBuf<unsigned char> tail_tab[2];    // FP32 and FP64 apply
+  DevBuf<unsigned char> member_tab[2];  // every level, FP32 and FP64
   void build_tail(bool f32);
+  // MemberLevel views of the levels from `from` on into tab (once: they hold only
+  // pointers fixed at build time)
+  void build_levels(DevBuf<unsigned char> &tab, int from, bool f32);
   DevBuf<int64_t> msize0;  // per member: level-0 rows and nonzeros (probe work)
   // SELL-64 form of one kind of matrix (0 A, 1 P, 2 P^T) of level l over its members
   void build_sell(int l, int kind, SellMat &M, hipStream_t s);

@@ -1,6 +1,7 @@
 // Smoothed-aggregation AMG: host symbolic hierarchy, device numeric setup and V-cycle.
 // The algorithm and its layout on the GPU are described in vfvm_amg.h.
 #include "vfvm_amg.h"
+#include "vfvm_amg_dev.h"
 #include "vfvm_linalg.h"
 
 #include <algorithm>
@@ -13,6 +14,7 @@ namespace vfvm {
 namespace {
 
 constexpr int AB = 256;
+constexpr int TB = 256;  // block of the coarse-tail kernel (one member per block)
 
 unsigned agrid(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + AB - 1) / AB); }
 
@@ -266,10 +268,6 @@ struct SView {
   const int32_t *idx;
   const V *val;
 };
-// value type of the cycle's matrices for Krylov precision V
-template <class V>
-using AmgVal = std::conditional_t<sizeof(V) == 4, _Float16, double>;
-
 // one slice per wave and one entry per round: the form that measured fastest (more
 // slices or entries per wave raise the registers and cut occupancy by more than they gain)
 constexpr int SPW = 1;     // slices per wave
@@ -426,114 +424,16 @@ __global__ void __launch_bounds__(AB) k_amg_jinit(SView<AmgVal<V>> A, const V *_
 
 // ---------------------------------------------------------------- coarse tail
 
-// one level of the tail as its kernel sees it, precision V
-template <class V>
-struct TailLevel {
-  const SellDesc *ad, *pd, *rd;
-  const int32_t *ai, *pi, *ri;
-  const AmgVal<V> *av, *pv, *rv;
-  const int32_t *am0, *amn, *pm0, *pmn, *rm0, *rmn;
-  const V *dinv;
-  const double *w;
-  const int32_t *mode, *didx, *dsz;
-  const int64_t *drow, *doff;
-  const V *dinvA;
-  V *b, *x, *r;
-};
-
-template <class VA, class F>
-__device__ __forceinline__ double tail_row(const SellDesc &d, const int32_t *__restrict__ idx,
-                                           const VA *__restrict__ val, int lane, F &&f) {
-  double acc = 0.0;
-  const int64_t v0 = d.vbase + lane, i0 = (int64_t)d.ibase + lane;
-  for (int32_t j = 0; j < d.width; j++)
-    acc += (double)val[v0 + (int64_t)64 * j] * f((int64_t)d.coff + idx[i0 + (int64_t)64 * j]);
-  return acc;
-}
-
-// the slices s0 .. s0 + ns - 1 of a member's matrix, one wave per slice
-template <class F>
-__device__ __forceinline__ void tail_slices(const SellDesc *__restrict__ D, int32_t s0, int32_t ns, F &&body) {
-  const int lane = threadIdx.x & 63;
-  for (int32_t s = threadIdx.x >> 6; s < ns; s += AB / 64) {
-    const SellDesc d = D[s0 + s];
-    if (lane < d.rows) body(d, lane, (int64_t)d.row0 + lane);
-  }
-}
-
 // The V-cycle below the tail's first level for one member per block: the same steps as
-// the level-by-level kernels (smoothing from zero and residual, restriction; the coarsest
-// solve; prolongation and post-smoothing), each over the member's own slices, with the
-// block's barriers between the steps instead of kernel boundaries.  In: T[0].b; out:
-// T[0].r.
+// the level-by-level kernels, each over the member's own slices, with the block's
+// barriers between the steps instead of kernel boundaries.  In: T[0].b; out: T[0].r.
 template <class V>
-__global__ void __launch_bounds__(AB) k_amg_tail(const TailLevel<V> *__restrict__ T, int nt,
+__global__ void __launch_bounds__(TB) k_amg_tail(const MemberLevel<V> *__restrict__ T, int nt,
                                                  const int32_t *__restrict__ members, const KState *st,
                                                  int skip_half) {
   const int m = members[blockIdx.x];
   if (amg_skip(st, m, skip_half)) return;
-  int l = 0;
-  for (; l + 1 < nt; l++) {
-    const TailLevel<V> &L = T[l];
-    if (L.mode[m] != Amg::MODE_COARSEN) break;
-    const TailLevel<V> &C = T[l + 1];
-    const double wm = L.w[m];
-    tail_slices(L.ad, L.am0[m], L.amn[m], [&](const SellDesc &d, int lane, int64_t row) {
-      const double acc =
-          tail_row(d, L.ai, L.av, lane, [&](int64_t k) { return wm * (double)L.dinv[k] * (double)L.b[k]; });
-      L.r[row] = (V)((double)L.b[row] - acc);
-    });
-    __syncthreads();
-    tail_slices(L.rd, L.rm0[m], L.rmn[m], [&](const SellDesc &d, int lane, int64_t row) {
-      C.b[row] = (V)tail_row(d, L.ri, L.rv, lane, [&](int64_t k) { return (double)L.r[k]; });
-    });
-    __syncthreads();
-  }
-  {  // the member's coarsest level
-    const TailLevel<V> &L = T[l];
-    if (L.mode[m] == Amg::MODE_DENSE) {
-      const int k = L.didx[m];
-      const int n = L.dsz[k];
-      const int64_t r0 = L.drow[k];
-      const V *M = L.dinvA + L.doff[k];
-      for (int i = threadIdx.x; i < n; i += AB) {
-        double acc = 0.0;
-        for (int c = 0; c < n; c++) acc += (double)M[(int64_t)i * n + c] * (double)L.b[r0 + c];
-        L.r[r0 + i] = (V)acc;
-      }
-    } else {  // damped Jacobi from zero, an odd count of sweeps after the first ends in r
-      const double wm = L.w[m];
-      tail_slices(L.ad, L.am0[m], L.amn[m], [&](const SellDesc &, int, int64_t row) {
-        L.x[row] = (V)(wm * (double)L.dinv[row] * (double)L.b[row]);
-      });
-      V *a = L.x, *o = L.r;
-      for (int sw = 1; sw < Amg::COARSE_SWEEPS; sw++) {
-        __syncthreads();
-        tail_slices(L.ad, L.am0[m], L.amn[m], [&](const SellDesc &d, int lane, int64_t row) {
-          const double acc = tail_row(d, L.ai, L.av, lane, [&](int64_t k) { return (double)a[k]; });
-          o[row] = (V)((double)a[row] + wm * (double)L.dinv[row] * ((double)L.b[row] - acc));
-        });
-        V *t = a;
-        a = o;
-        o = t;
-      }
-    }
-    __syncthreads();
-  }
-  for (l = l - 1; l >= 0; l--) {
-    const TailLevel<V> &L = T[l], &C = T[l + 1];
-    const double wm = L.w[m];
-    tail_slices(L.pd, L.pm0[m], L.pmn[m], [&](const SellDesc &d, int lane, int64_t row) {
-      const double acc = tail_row(d, L.pi, L.pv, lane, [&](int64_t k) { return (double)C.r[k]; });
-      L.x[row] = (V)(wm * (double)L.dinv[row] * (double)L.b[row] + acc);
-    });
-    __syncthreads();
-    tail_slices(L.ad, L.am0[m], L.amn[m], [&](const SellDesc &d, int lane, int64_t row) {
-      const double acc = tail_row(d, L.ai, L.av, lane, [&](int64_t k) { return (double)L.x[k]; });
-      L.r[row] = (V)((double)L.x[row] + wm * (double)L.dinv[row] * ((double)L.b[row] - acc));
-    });
-    __syncthreads();
-  }
+  amg_member_vcycle<V, TB>(T, nt, m, T[0].b, T[0].r);
 }
 
 // exact coarsest solve out = Ac^-1 b, one block per inverted member
@@ -1091,6 +991,8 @@ void Amg::build(const std::vector<AmgMember> &members, const int32_t *row_inst0,
   ntail = 0;
   tail_tab[0].release();
   tail_tab[1].release();
+  member_tab[0].release();
+  member_tab[1].release();
   if (lt < nlev) {
     std::vector<int32_t> tm;
     for (int64_t m = 0; m < nsys; m++)
@@ -1103,14 +1005,17 @@ void Amg::build(const std::vector<AmgMember> &members, const int32_t *row_inst0,
 
 void Amg::build_tail(bool f32) {
   if (lt >= nlev || ntail == 0) return;
-  DevBuf<unsigned char> &tab = tail_tab[f32 ? 0 : 1];
-  if (tab.p) return;  // the tables hold only pointers fixed at build time
+  build_levels(tail_tab[f32 ? 0 : 1], lt, f32);
+}
+
+void Amg::build_levels(DevBuf<unsigned char> &tab, int from, bool f32) {
+  if (tab.p || from >= nlev) return;
   auto fill = [&](auto tag) {
     using V = decltype(tag);
-    std::vector<TailLevel<V>> T;
-    for (int l = lt; l < nlev; l++) {
+    std::vector<MemberLevel<V>> T;
+    for (int l = from; l < nlev; l++) {
       Lev &L = lev[l];
-      TailLevel<V> t;
+      MemberLevel<V> t;
       std::memset(&t, 0, sizeof(t));
       auto vals = [&](SellMat &M) -> const AmgVal<V> * {
         if constexpr (sizeof(V) == 4) return M.v16.p;
@@ -1151,10 +1056,11 @@ void Amg::build_tail(bool f32) {
         t.dinv = L.dinv64.p;
         t.dinvA = L.dinvA64.p;
       }
+      if (l == 0) t.dinv = nullptr;  // the row-scaled operator: unit diagonal
       T.push_back(t);
     }
-    tab.alloc(sizeof(TailLevel<V>) * T.size());
-    VFVM_HIP_CHECK(hipMemcpy(tab.p, T.data(), sizeof(TailLevel<V>) * T.size(), hipMemcpyHostToDevice));
+    tab.alloc(sizeof(MemberLevel<V>) * T.size());
+    VFVM_HIP_CHECK(hipMemcpy(tab.p, T.data(), sizeof(MemberLevel<V>) * T.size(), hipMemcpyHostToDevice));
   };
   if (f32)
     fill(float());
@@ -1284,8 +1190,8 @@ void Amg::apply(const V *y, V *z, const KState *st, int skip_half, hipStream_t s
   // the coarse tail, one launch
   if (lt < nlev && ntail) {
     build_tail(F32);
-    hipLaunchKernelGGL((k_amg_tail<V>), dim3((unsigned)ntail), dim3(AB), 0, s,
-                       (const TailLevel<V> *)tail_tab[F32 ? 0 : 1].p, nlev - lt, (const int32_t *)tail_mem.p, st,
+    hipLaunchKernelGGL((k_amg_tail<V>), dim3((unsigned)ntail), dim3(TB), 0, s,
+                       (const MemberLevel<V> *)tail_tab[F32 ? 0 : 1].p, nlev - lt, (const int32_t *)tail_mem.p, st,
                        skip_half);
   }
   // up: prolongation (with the pre-smoothing recomputed) and post-smoothing

@@ -22,6 +22,7 @@
 #include <tuple>
 #include <type_traits>
 
+#include "vfvm_amg_dev.h"
 #include "vfvm_sys.h"
 
 using namespace vfvm;
@@ -370,6 +371,183 @@ __global__ void k_bconvert(int64_t n, const TI *__restrict__ a, TO *__restrict__
   for (int64_t i = (int64_t)blockIdx.x * BB + threadIdx.x; i < n; i += (int64_t)gridDim.x * BB) b[i] = (TO)a[i];
 }
 
+// ------------------------------------------------------- one block per member ----------
+// The whole BiCGStab solve of a member in one block: the member's vectors are row ranges
+// of the batch's Krylov vectors, the operator its slices of the batch's SELL-64 matrix
+// (a member's chunks, and so its slices, are consecutive) and the preconditioner one AMG
+// V-cycle of the member (amg_member_vcycle) or, with T null, point Jacobi (the identity on
+// the row-scaled operator).  The steps, the solver scalars and the stopping rules are
+// those of the kernel-per-step form (k_bp ... k_bxr with run_stage); dots are block
+// reductions.  One launch solves every member: a block ends when its member converges,
+// breaks down or reaches maxit, and the next waiting member takes its place, so no
+// launch, barrier or host read separates the iterations.
+template <class V, class VA, class CI>
+struct MArgs {
+  const int32_t *ch_first, *ch_slice;
+  const int64_t *ch_beg, *ch_end;
+  const int32_t *sptr;
+  const CI *col;
+  const VA *val;
+  const V *b;
+  V *x, *r, *rh, *p, *v, *s, *t, *ph, *sh;
+  KState *st;
+  const int32_t *active;
+  const MemberLevel<V> *T;  // AMG levels (null: point Jacobi)
+  int nt;
+};
+
+// block sums of v[0..N-1], the totals returned to every thread
+template <int BS, int N>
+__device__ __forceinline__ void msum(double (&v)[N], double *sh) {
+  constexpr int NW = BS / 64;
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    double x = v[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[k * NW + w] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    double x = 0.0;
+#pragma unroll
+    for (int j = 0; j < NW; j++) x += sh[k * NW + j];
+    v[k] = x;
+  }
+  __syncthreads();
+}
+
+template <class V, class VA, class CI, int BS>
+__global__ void __launch_bounds__(BS) k_msolve(MArgs<V, VA, CI> a) {
+  const int m = blockIdx.x;
+  KState *sti = a.st + m;
+  __shared__ double red[3 * (BS / 64)];
+  __shared__ KState ks;
+  if (threadIdx.x == 0) ks = *sti;
+  if (a.active && !a.active[m]) {  // as the initial stage of an inactive member
+    if (threadIdx.x == 0) {
+      const double z[1] = {0.0};
+      run_stage<STAGE_INIT>(&ks, z);
+      ks.done = 1;
+      ks.status = 1;
+      *sti = ks;
+    }
+    return;
+  }
+  const int32_t c0 = a.ch_first[m], c1 = a.ch_first[m + 1];
+  const int64_t r0 = a.ch_beg[c0], n = a.ch_end[c1 - 1] - r0;
+  const int32_t sl0 = a.ch_slice[c0];
+  V *__restrict__ x = a.x + r0;
+  V *__restrict__ r = a.r + r0;
+  V *__restrict__ rh = a.rh + r0;
+  V *__restrict__ p = a.p + r0;
+  V *__restrict__ v = a.v + r0;
+  V *__restrict__ s = a.s + r0;
+  V *__restrict__ t = a.t + r0;
+  const bool amg = a.T != nullptr;
+  // row r0 + l of Ahat y (y indexed by global rows)
+  auto apply = [&](const V *__restrict__ y, int64_t l) -> double {
+    const int32_t sl = sl0 + (int32_t)(l >> 6);
+    const int32_t q1 = a.sptr[sl + 1];
+    const int64_t row = r0 + l;
+    double acc = 0.0;
+    for (int32_t q = a.sptr[sl] + (int32_t)(l & 63); q < q1; q += 64)
+      acc += (double)a.val[q] * (double)y[sell_column(a.col[q], row)];
+    return acc;
+  };
+  {
+    double d[1] = {0.0};
+    for (int64_t l = threadIdx.x; l < n; l += BS) {
+      const V bv = a.b[r0 + l];
+      r[l] = bv;
+      rh[l] = bv;
+      p[l] = (V)0;
+      v[l] = (V)0;
+      x[l] = (V)0;
+      d[0] += (double)bv * (double)bv;
+    }
+    msum<BS>(d, red);
+    if (threadIdx.x == 0) run_stage<STAGE_INIT>(&ks, d);
+    __syncthreads();
+  }
+  while (!ks.done) {
+    {
+      const double beta = ks.beta, omega = ks.omega;
+      for (int64_t l = threadIdx.x; l < n; l += BS)
+        p[l] = (V)((double)r[l] + beta * ((double)p[l] - omega * (double)v[l]));
+      __syncthreads();
+    }
+    const V *ph = a.p;  // M^-1 p (global rows)
+    if (amg) {
+      amg_member_vcycle<V, BS>(a.T, a.nt, m, a.p, a.ph);
+      ph = a.ph;
+    }
+    {
+      double d[1] = {0.0};
+      for (int64_t l = threadIdx.x; l < n; l += BS) {
+        const V vv = (V)apply(ph, l);
+        v[l] = vv;
+        d[0] += (double)rh[l] * (double)vv;
+      }
+      msum<BS>(d, red);
+      if (threadIdx.x == 0) run_stage<STAGE_ALPHA>(&ks, d);
+      __syncthreads();
+    }
+    if (ks.done) break;
+    const double alpha = ks.alpha;
+    {
+      double d[1] = {0.0};
+      for (int64_t l = threadIdx.x; l < n; l += BS) {
+        const V sv = (V)((double)r[l] - alpha * (double)v[l]);
+        s[l] = sv;
+        d[0] += (double)sv * (double)sv;
+      }
+      msum<BS>(d, red);  // its barriers also publish s
+      if (threadIdx.x == 0) run_stage<STAGE_SNORM>(&ks, d);
+      __syncthreads();
+    }
+    const bool half = ks.half != 0;
+    const V *sh = a.s;  // M^-1 s
+    if (!half) {
+      if (amg) {
+        amg_member_vcycle<V, BS>(a.T, a.nt, m, a.s, a.sh);
+        sh = a.sh;
+      }
+      double d[2] = {0.0, 0.0};
+      for (int64_t l = threadIdx.x; l < n; l += BS) {
+        const V tv = (V)apply(sh, l);
+        t[l] = tv;
+        d[0] += (double)s[l] * (double)tv;
+        d[1] += (double)tv * (double)tv;
+      }
+      msum<BS>(d, red);
+      if (threadIdx.x == 0) run_stage<STAGE_OMEGA>(&ks, d);
+      __syncthreads();
+    }
+    {
+      const double omega = ks.omega;
+      double d[2] = {0.0, 0.0};
+      for (int64_t l = threadIdx.x; l < n; l += BS) {
+        if (half) {
+          x[l] = (V)((double)x[l] + alpha * (double)ph[r0 + l]);
+        } else {
+          x[l] = (V)((double)x[l] + alpha * (double)ph[r0 + l] + omega * (double)sh[r0 + l]);
+          const V rv = (V)((double)s[l] - omega * (double)t[l]);
+          r[l] = rv;
+          d[0] += (double)rh[l] * (double)rv;
+          d[1] += (double)rv * (double)rv;
+        }
+      }
+      msum<BS>(d, red);
+      if (threadIdx.x == 0) run_stage<STAGE_XR>(&ks, d);
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) *sti = ks;
+}
+
 // adds the per-member probe counters (rows, nonzeros) to the probe's totals and clears them
 __global__ void k_collect_work(int64_t nsys, unsigned long long *__restrict__ mwork, unsigned long long *dwork) {
   __shared__ unsigned long long sh[2][BB];
@@ -449,6 +627,9 @@ struct vfvm_batch {
   // captured Krylov iteration blocks: (value bytes, solution vector, iterations) -> graph
   bool use_graphs = true;
   std::map<std::tuple<int, const void *, int>, hipGraphExec_t> graphs;
+  // block size of the one-block-per-member Krylov solver (k_msolve); 0: the kernel-per-step
+  // form with captured iteration blocks
+  int member_bs = 0;
   // pinned host staging of the per-member solver states, flags and norms (small
   // transfers every few iterations), and two status slots with their events for the
   // pipelined Krylov loop
@@ -654,6 +835,60 @@ void reset_states(vfvm_batch &b, double rtol, double atol, int maxit) {
   stage_upload(b.st, b.hreset, b.nsys, b.ctx->stream);
 }
 
+// one launch of k_msolve for every member of the batch (inactive members end at entry)
+template <class V>
+void msolve(vfvm_batch &b, const V *bh, V *x, V *r, V *r0, V *p, V *v, V *s, V *t, V *ph, V *sh,
+            const int32_t *active, bool amg) {
+  auto go = [&](const auto *col) {
+    using CI = std::remove_const_t<std::remove_pointer_t<decltype(col)>>;
+    MArgs<V, V, CI> a;
+    a.ch_first = b.ch_first.p;
+    a.ch_slice = b.ch_slice.p;
+    a.ch_beg = b.ch_beg.p;
+    a.ch_end = b.ch_end.p;
+    a.sptr = b.sell_ptr.p;
+    a.col = col;
+    if constexpr (sizeof(V) == 4)
+      a.val = b.A32.p;
+    else
+      a.val = b.A64s.p;
+    a.b = bh;
+    a.x = x;
+    a.r = r;
+    a.rh = r0;
+    a.p = p;
+    a.v = v;
+    a.s = s;
+    a.t = t;
+    a.ph = ph;
+    a.sh = sh;
+    a.st = b.st.p;
+    a.active = active;
+    a.T = amg ? (const MemberLevel<V> *)b.amg.member_levels<V>() : nullptr;
+    a.nt = amg ? b.amg.levels() : 0;
+    if (amg && !a.T) throw Error(VFVM_ERR_ARG, "AMG member levels missing");
+    const int kid = sizeof(V) == 4 ? VFVM_KERNEL_MSOLVE32 : VFVM_KERNEL_MSOLVE64;
+    b.ctx->probe.before(kid, b.ctx->stream);
+    switch (b.member_bs) {
+      case 256:
+        VFVM_LAUNCH((k_msolve<V, V, CI, 256>), dim3((unsigned)b.nsys), dim3(256), 0, b.ctx->stream, a);
+        break;
+      case 512:
+        VFVM_LAUNCH((k_msolve<V, V, CI, 512>), dim3((unsigned)b.nsys), dim3(512), 0, b.ctx->stream, a);
+        break;
+      default:
+        VFVM_LAUNCH((k_msolve<V, V, CI, 1024>), dim3((unsigned)b.nsys), dim3(1024), 0, b.ctx->stream, a);
+        break;
+    }
+    b.ctx->probe.after(kid, b.ctx->stream);
+  };
+  if (b.sell_off16.p && b.sell16)
+    go((const int16_t *)b.sell_off16.p);
+  else
+    go((const int32_t *)b.sell_col.p);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
 // per-member BiCGStab on bhat -> x: point Jacobi (the identity on the row-scaled
 // operator) or, with amg, right-preconditioned by one AMG V-cycle per application
 template <class V, class VBin>
@@ -677,6 +912,21 @@ int bbicgstab(vfvm_batch &b, const VBin *bh, V *x, const int32_t *active, double
   }
   if (amg) b.amg.reserve<V>();
   reset_states(b, rtol, atol, maxit);
+  if (b.member_bs) {
+    msolve<V>(b, bh, x, r, r0, p, v, s, t, ph, sh, active, amg);
+    std::vector<KState> h = read_states(b);
+    int rc = 0;
+    int64_t w[3] = {0, 0, 0};
+    for (size_t m = 0; m < h.size(); m++) {
+      nlin[m] += h[m].iters;
+      if (h[m].status != 1) rc = VFVM_ERR_LINEAR;
+      w[0] += (int64_t)h[m].iters * (b.dof_off[m + 1] - b.dof_off[m]);
+      w[1] += (int64_t)h[m].iters * b.members[m]->nnz;
+      if (amg) w[2] += (int64_t)h[m].iters * b.amg.cycle_nnz((int64_t)m);
+    }
+    b.ctx->probe.add_work(sizeof(V) == 4 ? VFVM_KERNEL_MSOLVE32 : VFVM_KERNEL_MSOLVE64, w[0], w[1], w[2]);
+    return rc;
+  }
   hipLaunchKernelGGL((k_binit<V, VBin>), dim3((unsigned)b.nch), dim3(BB), 0, st, S, bh, r, r0, p, v, x, b.st.p, active);
   hipLaunchKernelGGL((k_bfinalize<1, STAGE_INIT, BF_INIT>), dim3((unsigned)b.nsys), dim3(BB), 0, st, S, b.st.p, active,
                      (double *)nullptr);
@@ -1071,6 +1321,12 @@ int vfvm_batch_set_sell_index(vfvm_batch *b, int bits) {
     b->sell16 = bits == 16;
     return VFVM_OK;
   });
+}
+
+int vfvm_batch_set_member_solver(vfvm_batch *b, int block) {
+  if (!b || (block != 0 && block != 256 && block != 512 && block != 1024)) return VFVM_ERR_ARG;
+  b->member_bs = block;
+  return VFVM_OK;
 }
 
 int vfvm_batch_set_compact_graph(vfvm_batch *b, int enable) {

@@ -173,11 +173,47 @@ void init_inactive(vfvm_sys &s, double *U, hipStream_t st) {
 
 namespace {
 
-void build_ilu(vfvm_sys &s) {
-  if (s.ilu_built) return;
-  // multicolour ordering of the dof graph (for one species the node graph)
+bool is_ilu(int precon) { return precon == VFVM_PRECON_ILU0 || precon == VFVM_PRECON_ILU0_NATURAL; }
+
+// the system's ILU structure is missing or in the other ordering
+bool ilu_stale(const vfvm_sys &s, const vfvm_control_t &c) {
+  return is_ilu(c.precon) && (!s.ilu_built || s.ilu_natural != (c.precon == VFVM_PRECON_ILU0_NATURAL));
+}
+
+// Dependency levels of ILU(0) in the natural order: level(i) = 1 + max level(j) over the
+// neighbours j < i of i in the symmetrised pattern.  Rows of one level are independent in
+// both triangular solves and in the row-by-row factorisation; ordering the rows by level
+// (by index inside a level) keeps, for every stored entry, which of its row and column
+// comes first, so the factor is ILUZero's natural-order ilu0 up to rounding.
+int natural_levels(int64_t n, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind,
+                   std::vector<int32_t> &level) {
+  level.assign((size_t)n, 0);
+  int nl = 0;
+  for (int64_t i = 0; i < n; i++) {
+    int32_t li = level[i];
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      const int32_t j = colind[k];
+      if (j < i) li = std::max(li, level[j] + 1);
+    }
+    level[i] = li;
+    // the transposed entries: i precedes its neighbours j > i
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      const int32_t j = colind[k];
+      if (j > i) level[j] = std::max(level[j], li + 1);
+    }
+    nl = std::max(nl, li + 1);
+  }
+  return nl;
+}
+
+void build_ilu(vfvm_sys &s, bool natural) {
+  if (s.ilu_built && s.ilu_natural == natural) return;
+  s.ilu_built = false;
+  // multicolour ordering of the dof graph (for one species the node graph), or the
+  // natural order's dependency levels
   std::vector<int32_t> color;
-  int nc = greedy_colors(s.ndof, s.rowptr, s.colind, color);
+  int nc = natural ? natural_levels(s.ndof, s.rowptr, s.colind, color)
+                   : greedy_colors(s.ndof, s.rowptr, s.colind, color);
   int64_t n = s.ndof;
   std::vector<int32_t> perm((size_t)n), iperm((size_t)n);
   std::vector<int32_t> cptr((size_t)nc + 1, 0);
@@ -238,10 +274,11 @@ void build_ilu(vfvm_sys &s) {
   s.ilu.src = s.d_ilu_src.p;
   s.ilu.srcrow = s.d_ilu_srcrow.p;
   s.ilu_built = true;
+  s.ilu_natural = natural;
 }
 
 void setup_linear(vfvm_sys &s, const vfvm_control_t &c) {
-  if (c.precon == VFVM_PRECON_ILU0) build_ilu(s);
+  if (is_ilu(c.precon)) build_ilu(s, c.precon == VFVM_PRECON_ILU0_NATURAL);
   LinSetup L;
   L.ndof = s.ndof;
   L.nnodes = s.nnodes;
@@ -265,7 +302,7 @@ void setup_linear(vfvm_sys &s, const vfvm_control_t &c) {
 
 void check_control(const vfvm_control_t &c, const vfvm_sys &s) {
   if (c.krylov != VFVM_KRYLOV_BICGSTAB && c.krylov != VFVM_KRYLOV_GMRES) throw Error(VFVM_ERR_ARG, "bad krylov");
-  if (c.precon < 0 || c.precon > 3) throw Error(VFVM_ERR_ARG, "bad precon");
+  if (c.precon < 0 || c.precon > 4) throw Error(VFVM_ERR_ARG, "bad precon");
   if (c.precon == VFVM_PRECON_AMG && s.nspec != 1)
     throw Error(VFVM_ERR_ARG, "the AMG preconditioner is provided for scalar systems (one species)");
   if (c.gmres_restart < 1 || c.gmres_restart > 64) throw Error(VFVM_ERR_ARG, "gmres_restart must be in 1..64");
@@ -1008,7 +1045,7 @@ int vfvm_linsolve(vfvm_sys *s, const double *b, double *x, const vfvm_control_t 
       vfvm_default_control(&c);
     check_control(c, *s);
     s->lin.fp32_off = false;  // each solve or sweep starts by trying FP32 again
-    if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
+    if (ilu_stale(*s, c)) setup_linear(*s, c);
     hipStream_t st = s->ctx->stream;
     s->d_F.upload(b, (size_t)s->ndof, st);
     s->lin.prepare(s->d_J.p, c, true);
@@ -1017,6 +1054,34 @@ int vfvm_linsolve(vfvm_sys *s, const double *b, double *x, const vfvm_control_t 
     VFVM_HIP_CHECK(hipStreamSynchronize(st));
     if (rc) s->ctx->err = "linear solver did not reach its tolerance";
     return rc;
+  });
+}
+
+int vfvm_ilu0_factor(vfvm_sys *s, double *lu) {
+  if (!s || !lu) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    const double *f = s->lin.ilu_factor64();
+    if (!s->ilu_built || !f) throw Error(VFVM_ERR_ARG, "no ILU(0) factor: no solve with an ILU(0) preconditioner yet");
+    hipStream_t st = s->ctx->stream;
+    const int64_t n = s->ndof, nnz = s->nnz;
+    std::vector<double> hf((size_t)nnz), d((size_t)n);
+    std::vector<int32_t> perm((size_t)n), src((size_t)nnz);
+    VFVM_HIP_CHECK(hipMemcpyAsync(hf.data(), f, sizeof(double) * nnz, hipMemcpyDeviceToHost, st));
+    VFVM_HIP_CHECK(hipMemcpyAsync(d.data(), s->lin.rowscale_ptr(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    VFVM_HIP_CHECK(hipMemcpyAsync(perm.data(), s->d_ilu_perm.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    VFVM_HIP_CHECK(hipMemcpyAsync(src.data(), s->d_ilu_src.p, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, st));
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<int32_t> iperm((size_t)n), pos((size_t)nnz);
+    for (int64_t p = 0; p < n; p++) iperm[perm[p]] = (int32_t)p;
+    for (int64_t p = 0; p < nnz; p++) pos[src[p]] = (int32_t)p;
+    // ILU(0) of diag(d) J is (diag(d) L diag(d)^-1, diag(d) U)
+    for (int64_t r = 0; r < n; r++)
+      for (int32_t k = s->rowptr[r]; k < s->rowptr[r + 1]; k++) {
+        const int32_t j = s->colind[k];
+        const double v = hf[pos[k]];
+        lu[k] = iperm[j] < iperm[r] ? v * d[j] / d[r] : v / d[r];
+      }
+    return VFVM_OK;
   });
 }
 
@@ -1123,7 +1188,7 @@ int vfvm_parameter_derivative(vfvm_sys *s, const double *U, double lambda, const
       vfvm_default_control(&c);
     check_control(c, *s);
     s->lin.fp32_off = false;
-    if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
+    if (ilu_stale(*s, c)) setup_linear(*s, c);
     hipStream_t st = s->ctx->stream;
     // J(U) of the stationary problem, then J x = dF/dlambda, du/dlambda = -x
     s->d_U.upload(U, (size_t)s->ndof, st);
@@ -1158,7 +1223,7 @@ int vfvm_newton(vfvm_sys *s, double *U, const double *Uold, double time, double 
       vfvm_default_control(&c);
     check_control(c, *s);
     s->lin.fp32_off = false;  // each solve or sweep starts by trying FP32 again
-    if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
+    if (ilu_stale(*s, c)) setup_linear(*s, c);
     hipStream_t st = s->ctx->stream;
     s->d_Uold.upload(Uold, (size_t)s->ndof, st);
     int rc = newton_device(*s, time, tstep, lambda, c, hist);
@@ -1179,7 +1244,7 @@ int vfvm_solve_fixed_steps(vfvm_sys *s, const double *inival, int64_t nl, const 
       vfvm_default_control(&c);
     check_control(c, *s);
     s->lin.fp32_off = false;  // each solve or sweep starts by trying FP32 again
-    if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
+    if (ilu_stale(*s, c)) setup_linear(*s, c);
     hipStream_t st = s->ctx->stream;
     auto t0 = std::chrono::steady_clock::now();
     s->d_Uold.upload(inival, (size_t)s->ndof, st);
@@ -1260,7 +1325,7 @@ int vfvm_solve_transient(vfvm_sys *s, const double *inival, int64_t nl, const do
     const vfvm_stepcontrol_t sc = *steps;
     if (!(sc.d_min > 0) || !(sc.d_max >= sc.d_min) || !(sc.d_decrease > 0 && sc.d_decrease < 1))
       throw Error(VFVM_ERR_ARG, "bad step control");
-    if (c.precon == VFVM_PRECON_ILU0 && !s->ilu_built) setup_linear(*s, c);
+    if (ilu_stale(*s, c)) setup_linear(*s, c);
     hipStream_t st = s->ctx->stream;
     auto t0 = std::chrono::steady_clock::now();
     s->d_Uold.upload(inival, (size_t)s->ndof, st);

@@ -276,7 +276,7 @@ inline std::pair<hipEvent_t, hipEvent_t> &pending_launch_events() {
 }
 
 struct KernelProbe {
-  static constexpr int NK = 8;
+  static constexpr int NK = 10;
   static constexpr int NW = 4;
   struct Slot {
     bool on = false;
@@ -312,6 +312,14 @@ struct KernelProbe {
     for (int i = 0; i < NK; i++)
       if (k[i].on) return true;
     return false;
+  }
+  // work counted on the host, added to an armed class
+  void add_work(int id, int64_t w0, int64_t w1, int64_t w2) {
+    Slot &q = k[id];
+    if (!q.on) return;
+    q.work[0] += w0;
+    q.work[1] += w1;
+    q.work[2] += w2;
   }
   // device work counters of an armed class (null when not armed)
   unsigned long long *counters(int id) const { return k[id].on ? k[id].dwork : nullptr; }

@@ -136,6 +136,10 @@ class LinearSolver {
   void apply_plain(bool fp32, const double *J, const void *x, void *y);
   const double *row_scale() const { return rowscale.p; }
   const Amg &amg_hierarchy() const { return amg; }
+  // FP64 ILU(0) factor in the permuted CSR order of the ILU structure, of the row-scaled
+  // operator diag(rowscale) J (null before the first factorisation)
+  const double *ilu_factor64() const { return ilu_ready ? ilu64.p : nullptr; }
+  const double *rowscale_ptr() const { return rowscale.p; }
 
   template <class V>
   struct Work {

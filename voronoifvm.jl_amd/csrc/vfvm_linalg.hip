@@ -739,6 +739,12 @@ void LinearSolver::setup(const LinSetup &ls, hipStream_t s) {
   }
   partials.alloc((size_t)4 * MAX_RED_BLOCKS);
   rowscale.alloc((size_t)n);
+  // the ILU structure may have been rebuilt in another ordering: factor copies are
+  // reallocated, so that the SELL padding of the new layout starts at zero
+  ilu_ready = false;
+  ilu_sell_ready[0] = ilu_sell_ready[1] = false;
+  ilu32s.release();
+  ilu64s.release();
   hipLaunchKernelGGL((k_fill<unsigned char>), dim3(grid_for((int64_t)sizeof(KState))), dim3(VB), 0, s,
                      (int64_t)sizeof(KState), reinterpret_cast<unsigned char *>(st.p), (unsigned char)0);
   hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(4 * MAX_RED_BLOCKS)), dim3(VB), 0, s,
@@ -834,8 +840,9 @@ void LinearSolver::prepare(const double *J, const vfvm_control_t &c, bool refact
     if (refactor || !amg.ready()) amg.setup(L.rowptr, L.colind, L.dofdiag, J, rowscale.p, stream);
     amg.refresh_fine(L.rowptr, J, rowscale.p, false, stream);  // both precisions: FP32 may fall back
   }
-  if (c.precon == VFVM_PRECON_ILU0 && (refactor || !ilu_ready)) {
-    if (!L.ilu) throw Error(VFVM_ERR_ARG, "ILU(0) needs the multicolour structure (build_ilu)");
+  const bool ilu = c.precon == VFVM_PRECON_ILU0 || c.precon == VFVM_PRECON_ILU0_NATURAL;
+  if (ilu && (refactor || !ilu_ready)) {
+    if (!L.ilu) throw Error(VFVM_ERR_ARG, "ILU(0) needs its level structure (build_ilu)");
     const IluStruct &I = *L.ilu;
     // the factorisation runs in FP64 for both precisions (pivots of non-M-matrices can
     // shrink); each precision's solve gets a SELL-64 copy of the factor
@@ -851,7 +858,7 @@ void LinearSolver::prepare(const double *J, const vfvm_control_t &c, bool refact
     ilu_ready = true;
     ilu_sell_ready[0] = ilu_sell_ready[1] = false;
   }
-  if (c.precon == VFVM_PRECON_ILU0 && !ilu_sell_ready[fp32 ? 0 : 1]) {
+  if (ilu && !ilu_sell_ready[fp32 ? 0 : 1]) {
     const IluStruct &I = *L.ilu;
     if (fp32) {
       if (ilu32s.n != (size_t)I.snnz) {
@@ -883,7 +890,7 @@ const V *LinearSolver::precond(const V *r, V *z, KState *stp, int skip_if_half, 
     else
       hipLaunchKernelGGL((k_blockjacobi<3, V>), dim3(g), dim3(VB), 0, stream, L.nnodes, binv.p, r, z, stp, skip_if_half);
     return z;
-  } else if (c.precon == VFVM_PRECON_ILU0) {
+  } else if (c.precon == VFVM_PRECON_ILU0 || c.precon == VFVM_PRECON_ILU0_NATURAL) {
     const IluStruct &I = *L.ilu;
     const V *lu = nullptr;
     if constexpr (sizeof(V) == 4) lu = ilu32s.p;

@@ -39,6 +39,7 @@ struct vfvm_sys {
   // multicolour ILU(0) structure
   IluStruct ilu;
   bool ilu_built = false;
+  bool ilu_natural = false;  // the structure's ordering: natural-order levels or colours
   DevBuf<int32_t> d_ilu_rowptr, d_ilu_colind, d_ilu_diag, d_ilu_perm, d_ilu_src, d_ilu_srcrow, d_ilu_sptr,
       d_ilu_scol, d_ilu_sdpos;
   // SELL-64 copy of the Jacobian pattern for the Krylov operator

@@ -12,7 +12,7 @@ from .physics import (BoundaryFlux, BoundaryReaction, BoundaryReactionField, Bou
                       Robin, SinhReaction)
 from .system import (Batch, NewtonSolverHistory, System, TransientSolution, boundary_dirichlet, boundary_neumann,
                      boundary_robin, dofs, enable_boundary_species, enable_species, eval_jacobian, eval_rhs, evaluate_residual_and_jacobian,
-                     initialize_dirichlet, integrate, linsolve, mass_matrix, solve, unknowns)
+                     ilu0_factor, initialize_dirichlet, integrate, linsolve, mass_matrix, solve, unknowns)
 
 from . import amg
 from .impedance import (ImpedanceSystem, assemble_dudp, assemble_dudp_at, impedance, integrate_testfunction, measurement_derivatives,

@@ -117,6 +117,7 @@ SIGNATURES = {
     "vfvm_linsolve": (C.c_int, [_vp, _dp, _dp, C.POINTER(ControlT), C.POINTER(LinStatsT)]),
     "vfvm_amg_aggregate": (C.c_int, [C.c_int64, _ip, _ip, _ip, _lp]),
     "vfvm_amg_levels": (C.c_int, [_vp, C.POINTER(C.c_int), _lp]),
+    "vfvm_ilu0_factor": (C.c_int, [_vp, _dp]),
     "vfvm_amg_level": (C.c_int, [_vp, C.c_int, _lp, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _dp]),
     "vfvm_newton": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.POINTER(ControlT),
                               C.POINTER(HistoryT)]),
@@ -150,6 +151,7 @@ SIGNATURES = {
     "vfvm_batch_create": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(_vp)]),
     "vfvm_batch_destroy": (C.c_int, [_vp]),
     "vfvm_batch_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "vfvm_batch_set_member_solver": (C.c_int, [_vp, C.c_int]),
     "vfvm_batch_set_sell_index": (C.c_int, [_vp, C.c_int]),
     "vfvm_batch_set_compact_graph": (C.c_int, [_vp, C.c_int]),
     "vfvm_batch_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
@@ -258,7 +260,8 @@ class Context:
         return _Probe(self, kernels, max_launches)
 
 
-KERNEL_ASSEMBLE, KERNEL_RESIDUAL, KERNEL_SPMV64, KERNEL_SPMV32, KERNEL_HALFSTEP64, KERNEL_HALFSTEP32, KERNEL_AMG64, KERNEL_AMG32 = range(8)
+(KERNEL_ASSEMBLE, KERNEL_RESIDUAL, KERNEL_SPMV64, KERNEL_SPMV32, KERNEL_HALFSTEP64, KERNEL_HALFSTEP32, KERNEL_AMG64,
+ KERNEL_AMG32, KERNEL_MSOLVE64, KERNEL_MSOLVE32) = range(10)
 
 
 class _Probe:

@@ -5,7 +5,9 @@ from __future__ import annotations
 from dataclasses import dataclass, fields
 
 _KRYLOV = {"bicgstab": 0, "gmres": 1}
-_PRECON = {"jacobi": 0, "blockjacobi": 1, "ilu0": 2, "amg": 3}
+# "ilu0": ILU(0) in a multicolour order (few launches per triangular solve);
+# "ilu0_natural": ILUZero.jl's natural-order ILU(0), level-scheduled
+_PRECON = {"jacobi": 0, "blockjacobi": 1, "ilu0": 2, "amg": 3, "ilu0_natural": 4}
 _PRECISION = {"fp64": 0, "fp32": 1}
 
 

@@ -459,6 +459,17 @@ def linsolve(sys: System, b, control: Optional[SolverControl] = None, **kwargs):
     return x, stats
 
 
+def ilu0_factor(sys: System):
+    """The ILU(0) factor of the last linear solve with precon "ilu0" or "ilu0_natural"
+    (ILUZero.ILU0Precon's l and u), one value per stored entry of the system's CSR dof
+    pattern (System.pattern_csr order): L strictly below the diagonal of the factor's
+    ordering (unit diagonal not stored), U on and above it, of the Jacobian as assembled."""
+    nnz = sys.sizes()[1]
+    lu = np.zeros(max(nnz, 1))
+    _capi.check(_capi.lib().vfvm_ilu0_factor(sys.handle(), _capi.dptr(lu)), sys.ctx.handle)
+    return lu[:nnz]
+
+
 def initialize_dirichlet(sys: System, u, t=0.0, embed=0.0):
     """_initialize_dirichlet!(U, system, data) (src/vfvm_system.jl:947-1003): a copy of u
     with the Dirichlet values of time t and embedding parameter `embed` set on the
@@ -643,6 +654,12 @@ class Batch:
         """Run the batched Krylov iteration blocks as captured HIP graphs (default) or
         launch their kernels one by one."""
         _capi.check(_capi.lib().vfvm_batch_set_graphs(self._h, int(enable)), self.ctx.handle)
+
+    def set_member_solver(self, block: int):
+        """Batched Krylov solves as one block of `block` threads per member (256, 512 or
+        1024: the member's whole BiCGStab with its AMG cycles in one block, one launch per
+        linear solve), or 0: one launch per BiCGStab step for all members."""
+        _capi.check(_capi.lib().vfvm_batch_set_member_solver(self._h, int(block)), self.ctx.handle)
 
     def set_compact_graph(self, enable: bool):
         """The batched assembly's node graph: compact (default where it fits) or plain."""
