@@ -110,8 +110,10 @@ def read_traffic(name):
 
 
 # linear tolerance of every form: SolverControl's default reltol_linear
-# (src/vfvm_solvercontrol.jl:111-116); FP32 refinement rounds reduce the residual by
-# INNER_RELTOL each
+# (src/vfvm_solvercontrol.jl:111-116).  The FP32 forms aim their inner solve at that
+# tolerance at once (one refinement round per Newton step; FP32 reaches 1e-4 without
+# trouble, and every extra round restarts the Krylov space); cfg2's drift-diffusion
+# inner solves reduce the residual by INNER_RELTOL per round
 RELTOL_LINEAR = 1e-4
 INNER_RELTOL = 1e-2
 
@@ -172,7 +174,7 @@ def build_cfg1(n):
 def cfg1_control(precision):
     """BiCGStab + ILU(0), fixed implicit-Euler steps of 1e-2 (fixed_timesteps!)."""
     c = vfvm.SolverControl(krylov="bicgstab", precon="ilu0", precision=precision, reltol_linear=RELTOL_LINEAR,
-                           abstol_linear=0.0, maxiters_linear=1000, inner_reltol=INNER_RELTOL, max_refinements=10)
+                           abstol_linear=0.0, maxiters_linear=1000, inner_reltol=RELTOL_LINEAR, max_refinements=10)
     return vfvm.fixed_timesteps(c, 1e-2)
 
 
@@ -400,7 +402,7 @@ def build_poisson(n, eps, vals, f):
 
 def cfg3_control(precision):
     return vfvm.SolverControl(krylov="bicgstab", precon="jacobi", precision=precision, reltol_linear=RELTOL_LINEAR,
-                              abstol_linear=0.0, maxiters_linear=3000, inner_reltol=INNER_RELTOL, max_refinements=12,
+                              abstol_linear=0.0, maxiters_linear=3000, inner_reltol=RELTOL_LINEAR, max_refinements=12,
                               check_every=8)
 
 
@@ -419,8 +421,12 @@ def run_cfg3(args):
     K, W = args.steps, args.warmup
     res = {}
     probes = {}
-    for label, prec, kernels in (("batched_fp64", "fp64", (_capi.KERNEL_SPMV64,)),
-                                 ("batched_fp32", "fp32", (_capi.KERNEL_SPMV32, _capi.KERNEL_ASSEMBLE))):
+    # the kernel-per-step form (one launch per BiCGStab step for all members, iteration
+    # blocks as graphs) and the one-launch form (each member's whole BiCGStab in one block)
+    for label, prec, kernels, mblock in (("batched_fp64", "fp64", (_capi.KERNEL_SPMV64,), 0),
+                                         ("batched_fp32", "fp32", (_capi.KERNEL_SPMV32, _capi.KERNEL_ASSEMBLE), 0),
+                                         ("batched_fp32_one_launch", "fp32", (_capi.KERNEL_MSOLVE32,), 1024)):
+        batch.set_member_solver(mblock)
         ctrl = cfg3_control(prec)
         for _ in range(W):
             batch.solve(0.0, ctrl)
@@ -438,6 +444,7 @@ def run_cfg3(args):
                           newton=sum(h.niter for h in batch.histories), krylov=sum(h.nlin for h in batch.histories))
         log(f"{label}: {K} x {len(systems)} solves in {el:.3f}s -> {res[label]['solves_per_s']:.1f} solves/s "
             f"(newton {res[label]['newton']}, krylov {res[label]['krylov']}, worst ||F|| {worst:.2e})")
+    batch.set_member_solver(0)
     # plain GPU Newton: the same 256 solves one system at a time, FP64
     ctrl = cfg3_control("fp64")
     vfvm.solve(systems[0], inival=0.0, control=ctrl)
@@ -452,18 +459,24 @@ def run_cfg3(args):
     log(f"sequential plain GPU Newton: {len(systems)} solves in {el:.2f}s -> {plain:.1f} solves/s")
     diff64 = max(float(np.abs(a - b).max()) for a, b in zip(res["batched_fp64"]["U"], Useq))
     diff32 = max(float(np.abs(a - b).max()) for a, b in zip(res["batched_fp32"]["U"], Useq))
+    diff1 = max(float(np.abs(a - b).max()) for a, b in zip(res["batched_fp32_one_launch"]["U"], Useq))
     p32, pasm, p64 = probes[_capi.KERNEL_SPMV32], probes[_capi.KERNEL_ASSEMBLE], probes[_capi.KERNEL_SPMV64]
     rl = roofline_hbm(probe_bytes("spmv", p32, sv=4), p32, read_traffic("cfg3_bspmv32"), "batched FP32 SpMV")
     rl["kernel"] = "batched FP32 SpMV over the active members (k_bspmv)"
     rl_asm = roofline_hbm(probe_bytes("assemble", pasm, transient=False), pasm, read_traffic("cfg3_assemble"),
                           "batched assembly")
     rl64 = roofline_hbm(probe_bytes("spmv", p64, sv=8), p64, read_traffic("cfg3_bspmv64"), "batched FP64 SpMV")
+    pms = probes[_capi.KERNEL_MSOLVE32]
+    rl_ms = roofline_hbm(probe_bytes("msolve", pms, sv=4), pms, read_traffic("cfg3_msolve32"),
+                         "one-launch batched BiCGStab (FP32)")
+    rl_ms["kernel"] = "k_msolve: each member's whole BiCGStab/Jacobi in one 1024-thread block, one launch per solve"
     return dict(value=res["batched_fp32"]["solves_per_s"], ms_per_step=1e3 * res["batched_fp32"]["seconds"] / K,
                 plain_gpu_newton_solves_per_s=plain,
                 speedup_vs_plain=res["batched_fp32"]["solves_per_s"] / plain,
                 batched_fp64_solves_per_s=res["batched_fp64"]["solves_per_s"],
-                max_abs_diff_vs_plain=dict(batched_fp64=diff64, batched_fp32=diff32),
-                roofline=rl, roofline_assemble=rl_asm, roofline_spmv64=rl64,
+                batched_fp32_one_launch_solves_per_s=res["batched_fp32_one_launch"]["solves_per_s"],
+                max_abs_diff_vs_plain=dict(batched_fp64=diff64, batched_fp32=diff32, batched_fp32_one_launch=diff1),
+                roofline=rl, roofline_assemble=rl_asm, roofline_spmv64=rl64, roofline_one_launch=rl_ms,
                 config=dict(workload="cfg3: 256 nonlinear-Poisson instances, N x N grids with N in "
                                      "{33,65,129,257}, seeded eps/boundary values/source, batched "
                                      "BiCGStab+Jacobi, FP32 inside FP64 Newton (plain: 256 sequential FP64 "

@@ -447,14 +447,30 @@ __global__ void __launch_bounds__(BS) k_msolve(MArgs<V, VA, CI> a) {
   V *__restrict__ s = a.s + r0;
   V *__restrict__ t = a.t + r0;
   const bool amg = a.T != nullptr;
-  // row r0 + l of Ahat y (y indexed by global rows)
+  // row r0 + l of Ahat y (y indexed by global rows): the column and value loads of the
+  // first MEMBER_W entries together (positions past the slice's width re-read its last
+  // entry), then their gathers, as member_row
   auto apply = [&](const V *__restrict__ y, int64_t l) -> double {
     const int32_t sl = sl0 + (int32_t)(l >> 6);
-    const int32_t q1 = a.sptr[sl + 1];
+    const int32_t q0 = a.sptr[sl] + (int32_t)(l & 63), w = (a.sptr[sl + 1] - a.sptr[sl]) >> 6;
     const int64_t row = r0 + l;
+    if (w <= 0) return 0.0;
+    CI c[MEMBER_W];
+    VA v[MEMBER_W];
+#pragma unroll
+    for (int j = 0; j < MEMBER_W; j++) {
+      const int32_t jj = j < w ? j : w - 1;
+      c[j] = a.col[q0 + 64 * jj];
+      v[j] = a.val[q0 + 64 * jj];
+    }
     double acc = 0.0;
-    for (int32_t q = a.sptr[sl] + (int32_t)(l & 63); q < q1; q += 64)
-      acc += (double)a.val[q] * (double)y[sell_column(a.col[q], row)];
+#pragma unroll
+    for (int j = 0; j < MEMBER_W; j++) {
+      const double g = (double)y[sell_column(c[j], row)];
+      acc += j < w ? (double)v[j] * g : 0.0;
+    }
+    for (int32_t j = MEMBER_W; j < w; j++)
+      acc += (double)a.val[q0 + 64 * j] * (double)y[sell_column(a.col[q0 + 64 * j], row)];
     return acc;
   };
   {

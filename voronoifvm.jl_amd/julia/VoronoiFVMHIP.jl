@@ -558,6 +558,9 @@ function hip_solve_transient!(ctx, sys, inival, lambdas, control::ControlT, step
                sys, inival, length(lambdas), collect(Float64, lambdas), transient, control, steps, U, max_out, ts, us,
                nsteps, hist)
     check(rc, ctx)
+    if hist.converged == 0  # handle_exceptions: returned before the end (src/vfvm_solver.jl:413-440)
+        @warn unsafe_string(ccall((:vfvm_last_error, libvfvm), Cstring, (Ptr{Cvoid},), ctx))
+    end
     n = min(nsteps[] + 1, max_out)
     return ts[1:n], [us[:, k] for k in 1:n], hist
 end
