@@ -766,7 +766,8 @@ def cfg4_result(args, S, fam, net, meta, probes, B, K, nb, nsol, t_sur, sur_rate
                                      "batch k mod 2; the inference of step k+1 runs on a second stream during "
                                      "the Newton solve of step k): MPNN initial guess "
                                      "(4 layers, width 64, BF16 MFMA) + batched BiCGStab preconditioned by "
-                                     "smoothed-aggregation AMG, FP32 inside FP64 Newton, vs plain FP64 GPU Newton "
+                                     "smoothed-aggregation AMG (each member's solve in one block, one launch per "
+                                     "linear solve of the batch), FP32 inside FP64 Newton, vs plain FP64 GPU Newton "
                                      "(BiCGStab/Jacobi) from zero one solve at a time; reltol_linear 1e-4 (the "
                                      "reference's default) in every form",
                             solves=nsol, unknowns_per_solve=fam.grid.num_nodes, batch=B,
@@ -947,9 +948,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="cfg4: skip the plain one-at-a-time and continuation legs (profiling runs)")
-    ap.add_argument("--member-block", type=int, default=0, choices=(0, 256, 512, 1024),
-                    help="cfg4: batched Krylov solves as one block of this many threads per member "
-                         "(0: one launch per BiCGStab step for all members)")
+    ap.add_argument("--member-block", type=int, default=1024, choices=(0, 256, 512, 1024),
+                    help="cfg4: batched Krylov solves as one block of this many threads per member, one launch "
+                         "per linear solve (0: one launch per BiCGStab step for all members)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="batched Krylov blocks launched kernel by kernel instead of as HIP graphs "
                          "(for kernel-trace profiling)")
