@@ -80,7 +80,7 @@ __device__ __forceinline__ MemberMat<VA> member_mat(const SellDesc *d, const int
 // last entry, so every address is valid and no branch splits the loads), then their
 // gathers, so a row costs about two memory round trips instead of two per entry; wider
 // rows finish in a loop
-constexpr int MEMBER_W = 8;
+constexpr int MEMBER_W = 6;  // measured best of 6, 8 and 12 (rows of width 5 to 9 on the 2D hierarchies)
 
 template <class VA, class F>
 __device__ __forceinline__ double member_row(const SellDesc &d, const MemberMat<VA> &M, int lane, F &&f) {
