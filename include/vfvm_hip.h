@@ -399,6 +399,19 @@ int vfvm_amg_levels(vfvm_sys *sys, int *nlevels, int64_t *rows);
 int vfvm_amg_level(vfvm_sys *sys, int level, int64_t *sizes, int32_t *arp, int32_t *aci, double *aval,
                    int32_t *prp, int32_t *pci, double *pval, int32_t *agg, double *omega);
 
+/* Generic operator (Physics(; generic = f), _eval_and_assemble_generic_operator,
+ * src/vfvm_assembly.jl:648-677; examples/Example405_GenericOperator.jl) in the registry
+ * form of a constant matrix G over the dofs: every assembly adds G U to F and G to the
+ * Jacobian, after the bulk and boundary terms.  COO triplets with 1-based int64 dof
+ * indices (repeated entries add up); each entry must lie in the system's sparsity pattern
+ * (VFVM_ERR_ARG otherwise, the operator unchanged): the edge couplings are in it for the
+ * species a flux couples, so a system whose only coupling is G takes VFVM_FLUX_LINEAR
+ * with zero coefficients.  nnz = 0 removes the operator.  A
+ * nonlinear generic operator has no registry form.  Systems with a generic operator
+ * cannot join a batch. */
+int vfvm_system_set_generic_matrix(vfvm_sys *sys, int64_t nnz, const int64_t *rows, const int64_t *cols,
+                                   const double *vals);
+
 /* The ILU(0) factor of the last linear solve with an ILU(0) preconditioner, in the
  * system's CSR order (lu[k] for the k-th stored entry of the dof pattern): the strictly
  * lower part holds L (unit diagonal not stored), the rest U, of the Jacobian without the

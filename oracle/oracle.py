@@ -83,6 +83,8 @@ def lib():
         L.orc_set_bfields.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_set_rcoef.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_set_param_dependence.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_generic.argtypes = [C.c_void_p, C.c_int, np.ctypeslib.ndpointer(np.int32, flags="C"),
+                                      np.ctypeslib.ndpointer(np.int32, flags="C"), dp]
         L.orc_node_species.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")]
         L.orc_last_error.restype = C.c_char_p
         L.orc_last_error.argtypes = [C.c_void_p]
@@ -178,6 +180,15 @@ class OracleSystem:
         """registry parameter slots p_j(lambda) = p_j + dp[j] lambda (16 values) or None"""
         self._dp = None if dp is None else np.ascontiguousarray(dp, dtype=np.float64)
         lib().orc_set_param_dependence(self._h, None if self._dp is None else self._dp.ctypes.data)
+
+    def set_generic(self, matrix):
+        """generic operator as a constant matrix over the dofs (scipy sparse or dense):
+        F += G U, J += G"""
+        import scipy.sparse as sp
+        A = sp.coo_matrix(matrix)
+        self._gen = (np.ascontiguousarray(A.row, dtype=np.int32), np.ascontiguousarray(A.col, dtype=np.int32),
+                     np.ascontiguousarray(A.data, dtype=np.float64))
+        lib().orc_set_generic(self._h, A.nnz, *self._gen)
 
     def node_species(self):
         m = np.zeros(self.grid.num_nodes, np.uint8)

@@ -560,6 +560,11 @@ struct orc_sys {
   int has_dp;
   orc_physics peff; /* the parameters at the lambda of the current assembly */
   double *bedgefac;       /* boundary-face edge factors: (dim == 2 ? 1 : 3) x nbfaces */
+  /* generic operator as a constant matrix G (COO, 0-based dofs): F += G U, J += G
+     (src/vfvm_assembly.jl:648-677 for a linear generic operator) */
+  int ngen;
+  int32_t *gen_r, *gen_c;
+  double *gen_v;
   /* state */
   lnk_t mat;
   int have_factorization;
@@ -897,8 +902,27 @@ void orc_destroy(orc_sys *s) {
   free(s->bcoef);
   free(s->rcoef);
   free(s->bedgefac);
+  free(s->gen_r);
+  free(s->gen_c);
+  free(s->gen_v);
   lnk_free(&s->mat);
   free(s);
+}
+
+void orc_set_generic(orc_sys *s, int nnz, const int32_t *rows, const int32_t *cols, const double *vals) {
+  free(s->gen_r);
+  free(s->gen_c);
+  free(s->gen_v);
+  s->gen_r = s->gen_c = NULL;
+  s->gen_v = NULL;
+  s->ngen = nnz > 0 ? nnz : 0;
+  if (!s->ngen) return;
+  s->gen_r = (int32_t *)malloc(sizeof(int32_t) * (size_t)nnz);
+  s->gen_c = (int32_t *)malloc(sizeof(int32_t) * (size_t)nnz);
+  s->gen_v = (double *)malloc(sizeof(double) * (size_t)nnz);
+  memcpy(s->gen_r, rows, sizeof(int32_t) * (size_t)nnz);
+  memcpy(s->gen_c, cols, sizeof(int32_t) * (size_t)nnz);
+  memcpy(s->gen_v, vals, sizeof(double) * (size_t)nnz);
 }
 
 void orc_set_source(orc_sys *s, const double *src) {
@@ -1177,6 +1201,16 @@ int orc_assemble(orc_sys *s, const double *U, const double *Uold, double time, d
         if ((rc = asm_bedge(s, K, L, s->bfaceregions[ib], s->bedgefac[nbe * ib + e], U, F))) return rc;
       }
     }
+  }
+  /* _eval_and_assemble_generic_operator, src/vfvm_assembly.jl:634, 648-677: the operator's
+     value F += G U and its Jacobian G added entry by entry (updateindex!) */
+  for (int q = 0; q < s->ngen; q++) {
+    if (isnan(s->gen_v[q])) {
+      set_err(s, "trying to assemble NaN");
+      return ORC_ERR_ASSEMBLY;
+    }
+    F[s->gen_r[q]] += s->gen_v[q] * U[s->gen_c[q]];
+    lnk_add(&s->mat, s->gen_r[q], s->gen_c[q], s->gen_v[q]);
   }
   /* _eval_and_assemble_inactive_species, src/vfvm_system.jl:1010-1026 */
   if (!s->homogeneous)

@@ -132,6 +132,9 @@ void orc_set_rcoef(orc_sys *s, const double *rcoef);
 /* registry parameters depending on the system parameter lambda: the assembly at lambda
    uses p_j + dp[j] lambda (dp: ORC_NPARAM values, NULL = none) */
 void orc_set_param_dependence(orc_sys *s, const double *dp);
+/* generic operator as a constant matrix G (COO, 0-based dof indices): every assembly adds
+   G U to F and G to the matrix, after the bulk and boundary terms */
+void orc_set_generic(orc_sys *s, int nnz, const int32_t *rows, const int32_t *cols, const double *vals);
 /* node_dof activity mask: bit i of mask[K] set if species i is active at node K */
 void orc_node_species(const orc_sys *s, uint8_t *mask);
 /* per-node source, nspec x nnodes (species fastest); NULL = none */

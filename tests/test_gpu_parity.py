@@ -499,3 +499,42 @@ def test_unstructured_sg_grid_matches_oracle(name):
     _print("newton iters gpu", sys.history.niter, "oracle", ho.niter, "err", err, "res", sys.history.residual_norm)
     assert err <= SOL_ATOL
     assert sys.history.residual_norm <= 1e-10
+
+
+def test_generic_operator_matches_oracle():
+    """Physics(generic = ...) as a constant matrix G on the dof pattern: Example405's known
+    answer on the GPU, F and J against the oracle with G added to a nonlinear Poisson
+    system on 17^2 nodes, and an entry outside the pattern refused."""
+    _print("case generic operator")
+    from test_oracle_known_answers import _example405_matrix
+    X = np.linspace(0.0, 1.0, 11)
+    g = simplexgrid(X)
+    # the zero flux keeps the edge couplings in the pattern that G needs
+    ph = vfvm.Physics(flux=vfvm.LinearFlux(0.0), generic=vfvm.GenericMatrix(_example405_matrix(X)))
+    sys = vfvm.System(g, ph, species=[1])
+    sys.boundary_dirichlet(1, 1, 0.0)
+    sys.boundary_dirichlet(1, 2, 1.0)
+    U = vfvm.solve(sys, inival=0.5, control=vfvm.SolverControl(reltol_linear=1e-12, abstol_linear=0.0))
+    assert float(np.sum(U)) == pytest.approx(1.099999999614456, rel=1e-10)
+    # a random G on the pattern of a 2D system
+    import scipy.sparse as sp
+    X = linspace(0.0, 1.0, 17)
+    g2 = simplexgrid(X, X)
+    base, _ = cases.pair(g2, 1, dict(flux=vfvm.LinearFlux(1e-2), reaction=vfvm.SinhReaction()))
+    rp, ci = base.pattern_csr()
+    rng = np.random.default_rng(8)
+    n = g2.num_nodes
+    G = sp.csr_matrix((rng.uniform(-1, 1, ci.size), ci, rp), shape=(n, n))
+    bcs = [vfvm.Dirichlet(species=1, region=1, value=0.0), vfvm.Dirichlet(species=1, region=3, value=1.0)]
+    sys2, osys2 = cases.pair(g2, 1, dict(flux=vfvm.LinearFlux(1e-2), reaction=vfvm.SinhReaction(),
+                                         generic=vfvm.GenericMatrix(G)), bcs)
+    osys2.set_generic(G)
+    Ut = rng.uniform(-1, 1, (1, n))
+    errs = cases.compare_assembly(sys2, osys2, Ut)
+    _print("generic operator assembly errors", errs)
+    assert max(errs.values()) <= 1e-12
+    # outside the pattern: node 0 and the last node are not neighbours
+    far = sp.coo_matrix(([1.0], ([0], [n - 1])), shape=(n, n))
+    bad = vfvm.System(g2, vfvm.Physics(flux=vfvm.LinearFlux(1e-2), generic=vfvm.GenericMatrix(far)), species=[1])
+    with pytest.raises(vfvm.VFVMError):
+        bad.handle()

@@ -131,3 +131,37 @@ def test_bernoulli_goldens():
     mask = np.abs(x) <= 50
     rel = [abs(O.bernoulli_dual(v)[1] - w) / abs(w) for v, w in zip(x[mask], db[mask]) if w != 0]
     assert max(rel) < 1e-12
+
+
+def _example405_matrix(X, D=1.0e-2, v=1.0):
+    """Example405's generic operator as its matrix: on each interval i, du = D (u_i -
+    u_{i+1}) / h_i + v u_i (upwind, v > 0) is added to row i and subtracted from row i + 1
+    (examples/Example405_GenericOperator.jl:21-32); the operator is linear in u."""
+    n = X.size
+    G = np.zeros((n, n))
+    for i in range(n - 1):
+        h = X[i + 1] - X[i]
+        G[i, i] += D / h + v
+        G[i, i + 1] -= D / h
+        G[i + 1, i] -= D / h + v
+        G[i + 1, i + 1] += D / h
+    return G
+
+
+def test_example405_generic_operator():
+    # examples/Example405_GenericOperator.jl:57-62: testval = sum of the solution
+    X = np.linspace(0.0, 1.0, 11)
+    g = simplexgrid(X)
+    s = O.OracleSystem(g, 1, O.physics(), {(1, 0): ("dirichlet_legacy", 0.0), (2, 0): ("dirichlet_legacy", 1.0)})
+    s.set_generic(_example405_matrix(X))
+    U, _ = s.solve_step(np.full(g.num_nodes, 0.5), O.control())
+    assert U.sum() == pytest.approx(1.099999999614456, rel=1e-12)
+    # F = G U at a state (the boundary rows carry the penalty terms on top)
+    rng = np.random.default_rng(5)
+    u = rng.random(g.num_nodes)
+    F = s.assemble(u, u)
+    A = s.matrix_scipy().toarray()
+    G = _example405_matrix(X)
+    inner = slice(1, g.num_nodes - 1)
+    assert np.abs(F[inner] - (G @ u)[inner]).max() <= 1e-14
+    assert np.abs(A[inner] - G[inner]).max() <= 1e-14

@@ -833,6 +833,31 @@ void launch_gather_or_zero(int64_t n, const int32_t *src, const double *x, doubl
   VFVM_HIP_CHECK(hipGetLastError());
 }
 
+// _eval_and_assemble_generic_operator (src/vfvm_assembly.jl:648-677) for a linear
+// generic operator G: one thread per dof row, F[i] += sum_k G[i,k] U[k], J[i,k] += G[i,k]
+__global__ void __launch_bounds__(256) k_generic(int64_t n, const int32_t *__restrict__ rowptr,
+                                                 const int32_t *__restrict__ colind, const double *__restrict__ g,
+                                                 const double *__restrict__ U, double *__restrict__ F,
+                                                 double *__restrict__ J) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int32_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+    const double gv = g[k];
+    acc += gv * U[colind[k]];
+    if (J) J[k] += gv;
+  }
+  F[i] += acc;
+}
+
+void launch_generic(int64_t ndof, const int32_t *rowptr, const int32_t *colind, const double *g, const double *U,
+                    double *F, double *J, hipStream_t s) {
+  if (ndof == 0) return;
+  hipLaunchKernelGGL(k_generic, dim3((unsigned)((ndof + 255) / 256)), dim3(256), 0, s, ndof, rowptr, colind, g, U, F,
+                     J);
+  VFVM_HIP_CHECK(hipGetLastError());
+}
+
 void launch_init_inactive(int64_t nlist, const int32_t *list, const uint8_t *nodemask, int nspec, double *U,
                           hipStream_t s) {
   if (nlist == 0) return;

@@ -1125,9 +1125,9 @@ int vfvm_batch_create(vfvm_ctx *ctx, int64_t nsys, vfvm_sys *const *systems, vfv
     for (int64_t m = 0; m < nsys; m++) {
       const vfvm_sys *s = systems[m];
       if (!s || s->ctx != ctx) throw Error(VFVM_ERR_ARG, "batch members must live on the batch's context");
-      if (s->bnd.hs.on || s->has_rcoef || s->has_dp)
+      if (s->bnd.hs.on || s->has_rcoef || s->has_dp || s->has_gen)
         throw Error(VFVM_ERR_ARG, "batch members cannot have boundary species, inactive dofs, boundary physics, "
-                                  "reaction fields or lambda-dependent coefficients");
+                                  "reaction fields, lambda-dependent coefficients or a generic operator");
       if (s->nspec != s0->nspec || s->phys.flux != s0->phys.flux || s->phys.reaction != s0->phys.reaction ||
           s->phys.storage != s0->phys.storage)
         throw Error(VFVM_ERR_ARG, "batch members must share species count and physics kinds");

@@ -125,7 +125,7 @@ AsmArgs asm_args(vfvm_sys &s, const double *U, const double *Uold, double *F, do
   return a;
 }
 
-void run_assembly(vfvm_sys &s, AsmArgs a, int group, hipStream_t st) {
+void run_assembly_terms(vfvm_sys &s, AsmArgs a, int group, hipStream_t st) {
   if (!s.bnd.hs.on) {
     launch_assemble(a, group, st);
     return;
@@ -159,6 +159,12 @@ void run_assembly(vfvm_sys &s, AsmArgs a, int group, hipStream_t st) {
   b.flags = a.flags;
   b.nspec = s.nspec;
   launch_bnd(b, st);
+}
+
+// bulk and boundary terms, then the generic operator (src/vfvm_assembly.jl:634-635)
+void run_assembly(vfvm_sys &s, AsmArgs a, int group, hipStream_t st) {
+  run_assembly_terms(s, a, group, st);
+  if (s.has_gen) launch_generic(s.ndof, s.d_rowptr.p, s.d_colind.p, s.d_gen.p, a.U, a.F, a.J, st);
 }
 
 // where the bulk kernel writes its Jacobian values (the bulk pattern) when timed alone
@@ -1054,6 +1060,34 @@ int vfvm_linsolve(vfvm_sys *s, const double *b, double *x, const vfvm_control_t 
     VFVM_HIP_CHECK(hipStreamSynchronize(st));
     if (rc) s->ctx->err = "linear solver did not reach its tolerance";
     return rc;
+  });
+}
+
+int vfvm_system_set_generic_matrix(vfvm_sys *s, int64_t nnz, const int64_t *rows, const int64_t *cols,
+                                   const double *vals) {
+  if (!s || nnz < 0 || (nnz > 0 && (!rows || !cols || !vals))) return VFVM_ERR_ARG;
+  return guarded(s->ctx, [&]() {
+    std::vector<double> g((size_t)s->nnz, 0.0);
+    for (int64_t q = 0; q < nnz; q++) {
+      const int64_t i = rows[q] - 1, j = cols[q] - 1;
+      if (i < 0 || i >= s->ndof || j < 0 || j >= s->ndof) throw Error(VFVM_ERR_ARG, "generic matrix index out of range");
+      const int32_t *b = s->colind.data() + s->rowptr[i], *e = s->colind.data() + s->rowptr[i + 1];
+      const int32_t *p = std::lower_bound(b, e, (int32_t)j);
+      if (p == e || *p != (int32_t)j)
+        throw Error(VFVM_ERR_ARG, "generic matrix entry (" + std::to_string(i + 1) + ", " + std::to_string(j + 1) +
+                                      ") outside the system's sparsity pattern");
+      g[(size_t)(p - s->colind.data())] += vals[q];
+    }
+    hipStream_t st = s->ctx->stream;
+    if (nnz == 0) {
+      s->has_gen = false;
+      s->d_gen.release();
+      return VFVM_OK;
+    }
+    s->d_gen.upload(g, st);
+    VFVM_HIP_CHECK(hipStreamSynchronize(st));
+    s->has_gen = true;
+    return VFVM_OK;
   });
 }
 

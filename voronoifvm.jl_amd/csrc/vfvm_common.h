@@ -138,6 +138,10 @@ void launch_bnd(const BndArgs &b, hipStream_t s);
 // y[k] = src[k] >= 0 ? x[src[k]] : 0 for k < n
 void launch_gather_or_zero(int64_t n, const int32_t *src, const double *x, double *y, hipStream_t s);
 // U[K, i] = 0 for the species inactive at the listed nodes (_initialize_inactive_dof!)
+// the generic operator as a constant matrix G on the dof pattern (g: one value per
+// pattern slot, CSR order): F += G U, and J += G when J is given
+void launch_generic(int64_t ndof, const int32_t *rowptr, const int32_t *colind, const double *g, const double *U,
+                    double *F, double *J, hipStream_t s);
 void launch_init_inactive(int64_t nlist, const int32_t *list, const uint8_t *nodemask, int nspec, double *U,
                           hipStream_t s);
 

@@ -40,6 +40,9 @@ struct vfvm_sys {
   IluStruct ilu;
   bool ilu_built = false;
   bool ilu_natural = false;  // the structure's ordering: natural-order levels or colours
+  // generic operator as a constant matrix on the dof pattern (one value per slot)
+  DevBuf<double> d_gen;
+  bool has_gen = false;
   DevBuf<int32_t> d_ilu_rowptr, d_ilu_colind, d_ilu_diag, d_ilu_perm, d_ilu_src, d_ilu_srcrow, d_ilu_sptr,
       d_ilu_scol, d_ilu_sdpos;
   // SELL-64 copy of the Jacobian pattern for the Krylov operator

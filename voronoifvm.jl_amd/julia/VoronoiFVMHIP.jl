@@ -566,6 +566,22 @@ function hip_solve_transient!(ctx, sys, inival, lambdas, control::ControlT, step
 end
 
 """
+    hip_set_generic_matrix!(ctx, sys, G::SparseMatrixCSC)
+
+Registry form of `Physics(; generic = f)` for a linear f (src/vfvm_assembly.jl:648-677,
+examples/Example405_GenericOperator.jl): every assembly adds G*U to F and G to the
+Jacobian. G's entries must lie in the system's pattern (hip_pattern).
+"""
+function hip_set_generic_matrix!(ctx, sys, G::SparseMatrixCSC)
+    I, J, V = findnz(G)
+    rc = ccall((:vfvm_system_set_generic_matrix, libvfvm), Cint,
+               (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+               sys, length(V), Vector{Int64}(I), Vector{Int64}(J), Vector{Float64}(V))
+    check(rc, ctx)
+    return sys
+end
+
+"""
     hip_batch(ctx, systems) -> handle
 
 Packs independent systems (the one-SystemState-per-thread pattern of

@@ -7,7 +7,7 @@ from ._capi import AssemblyError, ConvergenceError, DeviceArray, LinearSolverErr
 from .control import SolverControl, fixed_timesteps
 from .grids import Grid, circular_symmetric, linspace, read_sg, simplexgrid, spherical_symmetric
 from .physics import (BoundaryFlux, BoundaryReaction, BoundaryReactionField, BoundarySource, BoundaryStorage,
-                      Dirichlet, DriftDiffusionFlux, DriftDiffusionReaction, ExchangeReaction, IdentityStorage,
+                      Dirichlet, GenericMatrix, DriftDiffusionFlux, DriftDiffusionReaction, ExchangeReaction, IdentityStorage,
                       LinearFlux, Neumann, NonlinearDiffusionFlux, Physics, PowerFlux, PowerReaction, ReactionField,
                       Robin, SinhReaction)
 from .system import (Batch, NewtonSolverHistory, System, TransientSolution, boundary_dirichlet, boundary_neumann,

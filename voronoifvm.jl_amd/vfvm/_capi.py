@@ -152,6 +152,7 @@ SIGNATURES = {
     "vfvm_batch_destroy": (C.c_int, [_vp]),
     "vfvm_batch_set_graphs": (C.c_int, [_vp, C.c_int]),
     "vfvm_batch_set_member_solver": (C.c_int, [_vp, C.c_int]),
+    "vfvm_system_set_generic_matrix": (C.c_int, [_vp, C.c_int64, _lp, _lp, _dp]),
     "vfvm_batch_set_sell_index": (C.c_int, [_vp, C.c_int]),
     "vfvm_batch_set_compact_graph": (C.c_int, [_vp, C.c_int]),
     "vfvm_batch_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),

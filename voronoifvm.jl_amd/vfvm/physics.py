@@ -197,6 +197,25 @@ class ReactionField:
 
 
 @dataclass
+class GenericMatrix:
+    """Physics(generic = f) in registry form: a constant matrix G over the dofs (the
+    order of dofs(U)); every assembly adds G U to F and G to the Jacobian
+    (_eval_and_assemble_generic_operator, src/vfvm_assembly.jl:648-677), as
+    examples/Example405_GenericOperator.jl's linear upwind operator does.  G: a scipy
+    sparse matrix or a dense array whose entries lie in the system's sparsity pattern
+    (the pattern holds the edge couplings of the species a flux couples: a zero
+    LinearFlux(0.0) provides them where the physics has no flux of its own)."""
+    matrix: object
+
+    def coo(self):
+        """1-based dof rows, columns and values of the stored entries"""
+        import scipy.sparse as sp
+        A = sp.coo_matrix(self.matrix)
+        return (np.ascontiguousarray(A.row.astype(np.int64) + 1), np.ascontiguousarray(A.col.astype(np.int64) + 1),
+                np.ascontiguousarray(A.data.astype(np.float64)))
+
+
+@dataclass
 class Physics:
     """VoronoiFVM.Physics(; flux, reaction, storage, source, bcondition, breaction,
     bstorage, bflux, bsource).
@@ -217,6 +236,7 @@ class Physics:
     bflux: list = field(default_factory=list)
     bsource: list = field(default_factory=list)
     reaction_fields: list = field(default_factory=list)
+    generic: Optional[GenericMatrix] = None  # the generic operator (a constant matrix)
     # registry objects whose coefficients are the derivatives of flux's and reaction's
     # coefficients with respect to the system parameter lambda (the coefficient is
     # c + dc lambda; Example430_ParameterDerivativesStationary.jl:142-160 passes its p

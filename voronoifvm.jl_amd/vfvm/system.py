@@ -287,6 +287,10 @@ class System:
             if any(f is not None for f in fields):
                 _capi.check(L.vfvm_system_set_fields(h, *[_capi.dptr(f) if f is not None else None for f in fields]),
                             self.ctx.handle)
+            if self.physics.generic is not None:
+                r, c, v = self.physics.generic.coo()
+                _capi.check(L.vfvm_system_set_generic_matrix(h, v.size, _capi.lptr(r), _capi.lptr(c), _capi.dptr(v)),
+                            self.ctx.handle)
             self._bc_dirty = False
         elif self._bc_dirty:
             _capi.check(_capi.lib().vfvm_system_set_bc(self._h, self._bc_table()), self.ctx.handle)
