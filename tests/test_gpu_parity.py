@@ -509,8 +509,7 @@ def test_generic_operator_matches_oracle():
     from test_oracle_known_answers import _example405_matrix
     X = np.linspace(0.0, 1.0, 11)
     g = simplexgrid(X)
-    # the zero flux keeps the edge couplings in the pattern that G needs
-    ph = vfvm.Physics(flux=vfvm.LinearFlux(0.0), generic=vfvm.GenericMatrix(_example405_matrix(X)))
+    ph = vfvm.Physics(generic=vfvm.GenericMatrix(_example405_matrix(X)))
     sys = vfvm.System(g, ph, species=[1])
     sys.boundary_dirichlet(1, 1, 0.0)
     sys.boundary_dirichlet(1, 2, 1.0)

@@ -203,8 +203,9 @@ class GenericMatrix:
     (_eval_and_assemble_generic_operator, src/vfvm_assembly.jl:648-677), as
     examples/Example405_GenericOperator.jl's linear upwind operator does.  G: a scipy
     sparse matrix or a dense array whose entries lie in the system's sparsity pattern
-    (the pattern holds the edge couplings of the species a flux couples: a zero
-    LinearFlux(0.0) provides them where the physics has no flux of its own)."""
+    (the pattern holds the grid's edge couplings when the physics has a flux; with a
+    generic operator and no flux the system takes a flux with zero coefficients, so G
+    may couple neighbouring nodes, as Example405's operator does)."""
     matrix: object
 
     def coo(self):
@@ -261,6 +262,10 @@ class Physics:
     def param_block(self, nspec):
         p = np.zeros(16)
         flux = self.flux.fill(p, nspec) if self.flux is not None else FLUX_NONE
+        if self.flux is None and self.generic is not None:
+            # the edge couplings a generic operator acts through stay in the pattern: a
+            # flux with zero coefficients (it adds nothing)
+            flux = LinearFlux(0.0).fill(p, nspec)
         react = self.reaction.fill(p, nspec) if self.reaction is not None else REACT_NONE
         stor = self.storage.fill(p, nspec) if self.storage is not None else STOR_NONE
         if self.embed_source:
