@@ -655,6 +655,16 @@ void refresh_bf16(vfvm_mpnn &m) {
   VFVM_HIP_CHECK(hipGetLastError());
 }
 
+// samples per inference launch sequence: at most MAX_INFER_ROWS rows, the samples split
+// into chunks of equal size (a short last chunk would pay every launch of the smoothing
+// and the layers for a few samples)
+constexpr int64_t MAX_INFER_ROWS = int64_t(1) << 24;
+inline int64_t sample_chunk(int64_t nsamples, int64_t n) {
+  const int64_t most = std::max<int64_t>(1, MAX_INFER_ROWS / n);
+  const int64_t nchunk = (nsamples + most - 1) / most;
+  return std::max<int64_t>(1, (nsamples + nchunk - 1) / nchunk);
+}
+
 constexpr int64_t WG_ROWS = 4096;  // rows per block of the weight-gradient kernels
 constexpr int GATE_ROWS = 256;     // rows per block of the gate kernel
 
@@ -820,7 +830,7 @@ int infer_sides_impl(vfvm_mpnn *m, int64_t nsamples, const double *f, const doub
   return mguard(m->ctx, [&]() {
     if (!m->xy.p) throw Error(VFVM_ERR_ARG, "side features need a 2D system");
     hipStream_t st = m->ctx->stream;
-    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsamples, (int64_t(1) << 23) / m->n));
+    const int64_t chunk = sample_chunk(nsamples, m->n);
     const double inv_delta = 1.0 / std::sqrt(eps / 2.0);
     std::vector<float> u;
     for (int64_t s0 = 0; s0 < nsamples; s0 += chunk) {
@@ -986,7 +996,7 @@ int vfvm_mpnn_infer(vfvm_mpnn *m, int64_t nsamples, const float *features, doubl
   if (!m || !features || !out || nsamples < 1) return VFVM_ERR_ARG;
   return mguard(m->ctx, [&]() {
     hipStream_t st = m->ctx->stream;
-    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsamples, (int64_t(1) << 23) / m->n));
+    const int64_t chunk = sample_chunk(nsamples, m->n);
     std::vector<float> u;
     for (int64_t s0 = 0; s0 < nsamples; s0 += chunk) {
       const int64_t ns = std::min(chunk, nsamples - s0), rows = ns * m->n;
